@@ -1,0 +1,274 @@
+"""ctypes front end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke() and by the
+CPU-baseline leg of bench.py as the checker; the product library never uses it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+BID_COPPER_SURFACE, BID_VACUUM_TOP, BID_COPPER_BOTTOM, BID_SIDES = 2, 8, 7, 4
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _declare(_lib)
+    return _lib
+
+
+class _Mesh(C.Structure):
+    _fields_ = [("n_dofs", C.c_int), ("n_cells", C.c_int), ("x", C.POINTER(C.c_double)),
+                ("vertex_of_dof", C.POINTER(C.c_int)), ("cells", C.POINTER(C.c_int)),
+                ("n_bfaces", C.c_int), ("bface_cell", C.POINTER(C.c_int)),
+                ("bface_face", C.POINTER(C.c_int)), ("bface_id", C.POINTER(C.c_int))]
+
+
+class _Csr(C.Structure):
+    _fields_ = [("n", C.c_int), ("nnz", C.c_int64), ("rowptr", C.POINTER(C.c_int64)),
+                ("col", C.POINTER(C.c_int)), ("val", C.POINTER(C.c_double)),
+                ("diag", C.POINTER(C.c_int64))]
+
+
+class _Lintet(C.Structure):
+    _fields_ = [("n_tets", C.c_int), ("det0", C.POINTER(C.c_double)), ("det", C.POINTER(C.c_double)),
+                ("centroid", C.POINTER(C.c_double)), ("nbr_ptr", C.POINTER(C.c_int)),
+                ("nbr", C.POINTER(C.c_int)), ("marker", C.POINTER(C.c_int))]
+
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_I = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_U8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def _declare(L):
+    L.orc_mesh_import.argtypes = [C.POINTER(_Mesh), C.c_int, _D, C.c_int, _I, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.orc_mesh_free.argtypes = [C.POINTER(_Mesh)]
+    L.orc_csr_pattern.argtypes = [C.POINTER(_Csr), C.POINTER(_Mesh)]
+    L.orc_csr_free.argtypes = [C.POINTER(_Csr)]
+    L.orc_csr_spmv.argtypes = [C.POINTER(_Csr), _D, _D]
+    L.orc_assemble.argtypes = [C.POINTER(_Csr), C.POINTER(_Mesh), C.c_void_p, C.c_void_p]
+    L.orc_assemble_cell_rhs.argtypes = [_D, C.POINTER(_Mesh), _D]
+    L.orc_assemble_face_rhs.argtypes = [_D, C.POINTER(_Mesh), C.c_int, C.c_void_p, C.c_double]
+    L.orc_dirichlet_mask.argtypes = [C.POINTER(_Mesh), C.c_int, _U8]
+    L.orc_apply_dirichlet.argtypes = [C.POINTER(_Csr), _D, _D, _U8, C.c_void_p]
+    L.orc_pcg.argtypes = [C.POINTER(_Csr), _D, _D, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
+                          C.POINTER(C.c_double)]
+    L.orc_pcg.restype = C.c_int
+    L.orc_export_solution_grad.argtypes = [C.POINTER(_Mesh), _D, _D]
+    L.orc_element_matrix.argtypes = [_D, C.c_void_p, C.c_void_p, _D]
+    L.orc_nodal_field.argtypes = [C.c_int, _D, C.c_int, _I, _I, C.c_int, _D, _D]
+    L.orc_lintet_build.argtypes = [C.POINTER(_Lintet), C.c_int, _D, C.c_int, _I, _I, _I, C.c_int]
+    L.orc_lintet_free.argtypes = [C.POINTER(_Lintet)]
+    L.orc_lintet_locate.argtypes = [C.POINTER(_Lintet), _D, C.c_int]
+    L.orc_lintet_locate.restype = C.c_int
+    L.orc_lintet_bcc.argtypes = [C.POINTER(_Lintet), _D, C.c_int, _D]
+    L.orc_locate_chain.argtypes = [C.POINTER(_Lintet), C.c_int, _D, C.c_int, _I]
+    L.orc_linhex_shape_functions.argtypes = [_D, _D, _D]
+    for f in ("orc_resistivity", "orc_sigma"):
+        getattr(L, f).argtypes = [C.c_double]
+        getattr(L, f).restype = C.c_double
+    L.orc_kappa.argtypes = [C.c_double, C.c_double]
+    L.orc_kappa.restype = C.c_double
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class Mesh:
+    """DealSolver::import_mesh restated: compaction, UCD->lexicographic, orientation fix,
+    boundary faces marked by the bounding box of their centres."""
+
+    def __init__(self, vertices, ucd_cells, top, bottom, sides, other):
+        self._m = _Mesh()
+        v = np.ascontiguousarray(vertices, np.float64).reshape(-1)
+        c = np.ascontiguousarray(ucd_cells, np.int32).reshape(-1)
+        rc = lib().orc_mesh_import(C.byref(self._m), len(v) // 3, v, len(c) // 8, c, top, bottom, sides, other)
+        if rc != 0:
+            raise ValueError("mesh import failed")
+        m = self._m
+        self.n_dofs, self.n_cells = m.n_dofs, m.n_cells
+        self.x = _arr(m.x, 3 * m.n_dofs, np.float64).reshape(-1, 3)
+        self.vertex_of_dof = _arr(m.vertex_of_dof, m.n_dofs, np.int32)
+        self.cells = _arr(m.cells, 8 * m.n_cells, np.int32).reshape(-1, 8)
+        self.bface_cell = _arr(m.bface_cell, m.n_bfaces, np.int32)
+        self.bface_face = _arr(m.bface_face, m.n_bfaces, np.int32)
+        self.bface_id = _arr(m.bface_id, m.n_bfaces, np.int32)
+        self._csr = None
+
+    def __del__(self):
+        try:
+            if self._csr is not None:
+                lib().orc_csr_free(C.byref(self._csr))
+            lib().orc_mesh_free(C.byref(self._m))
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------------------
+    def assemble(self, kappa_q=None, gamma_q=None):
+        if self._csr is None:
+            self._csr = _Csr()
+            lib().orc_csr_pattern(C.byref(self._csr), C.byref(self._m))
+        kq = None if kappa_q is None else np.ascontiguousarray(kappa_q, np.float64)
+        gq = None if gamma_q is None else np.ascontiguousarray(gamma_q, np.float64)
+        lib().orc_assemble(C.byref(self._csr), C.byref(self._m),
+                           None if kq is None else kq.ctypes.data, None if gq is None else gq.ctypes.data)
+        return self
+
+    def spmv(self, x):
+        y = np.zeros(self.n_dofs)
+        lib().orc_csr_spmv(C.byref(self._csr), np.ascontiguousarray(x, np.float64), y)
+        return y
+
+    def csr(self):
+        A = self._csr
+        n, nnz = A.n, A.nnz
+        return (_arr(A.rowptr, n + 1, np.int64), _arr(A.col, nnz, np.int32), _arr(A.val, nnz, np.float64))
+
+    def cell_rhs(self, f_q):
+        b = np.zeros(self.n_dofs)
+        lib().orc_assemble_cell_rhs(b, C.byref(self._m), np.ascontiguousarray(f_q, np.float64))
+        return b
+
+    def face_rhs(self, bid, values=None, constant=0.0, b=None):
+        b = np.zeros(self.n_dofs) if b is None else b
+        fv = None if values is None else np.ascontiguousarray(values, np.float64)
+        lib().orc_assemble_face_rhs(b, C.byref(self._m), bid, None if fv is None else fv.ctypes.data, constant)
+        return b
+
+    def dirichlet_mask(self, bid, mask=None):
+        mask = np.zeros(self.n_dofs, np.uint8) if mask is None else mask
+        lib().orc_dirichlet_mask(C.byref(self._m), bid, mask)
+        return mask
+
+    def apply_dirichlet(self, b, x, mask, values=None):
+        vals = None if values is None else np.ascontiguousarray(values, np.float64)
+        lib().orc_apply_dirichlet(C.byref(self._csr), b, x, mask, None if vals is None else vals.ctypes.data)
+
+    def pcg(self, b, x, max_it=10000, tol=1e-9, relative=False, precond="ssor", omega=1.2):
+        kind = {"none": 0, "jacobi": 1, "ssor": 2}[precond]
+        res = C.c_double()
+        it = lib().orc_pcg(C.byref(self._csr), b, x, max_it, tol, int(relative), kind, omega, C.byref(res))
+        return it, res.value
+
+    def export_solution_grad(self, u):
+        g = np.zeros(3 * self.n_dofs)
+        lib().orc_export_solution_grad(C.byref(self._m), np.ascontiguousarray(u, np.float64), g)
+        return g.reshape(-1, 3)
+
+
+def element_matrix(xe_lex, kappa_q=None, gamma_q=None):
+    K = np.zeros(64)
+    kq = None if kappa_q is None else np.ascontiguousarray(kappa_q, np.float64)
+    gq = None if gamma_q is None else np.ascontiguousarray(gamma_q, np.float64)
+    lib().orc_element_matrix(np.ascontiguousarray(xe_lex, np.float64).reshape(-1),
+                             None if kq is None else kq.ctypes.data, None if gq is None else gq.ctypes.data, K)
+    return K.reshape(8, 8)
+
+
+def poisson_laplace(vertices, ucd_cells, applied_field, tol=1e-9, relative=False, precond="ssor",
+                    omega=1.2, max_it=10000, anode="neumann", potential=0.0):
+    """PoissonSolver setup/assemble/solve for field_mode = laplace (src/PoissonSolver.cpp:162-210,
+    src/DealSolver.cpp:443-458): phi = 0 on the copper surface (id 2), Neumann -E0 on the top
+    (id 8).  Returns (mesh, solution per dof, iterations, residual)."""
+    m = Mesh(vertices, ucd_cells, BID_VACUUM_TOP, BID_COPPER_SURFACE, BID_SIDES, BID_COPPER_SURFACE)
+    m.assemble()
+    b = np.zeros(m.n_dofs)
+    x = np.zeros(m.n_dofs)
+    mask = m.dirichlet_mask(BID_COPPER_SURFACE)
+    vals = np.zeros(m.n_dofs)
+    if anode == "neumann":
+        m.face_rhs(BID_VACUUM_TOP, constant=applied_field, b=b)
+    else:
+        m.dirichlet_mask(BID_VACUUM_TOP, mask)
+        top = np.zeros(m.n_dofs, np.uint8)
+        m.dirichlet_mask(BID_VACUUM_TOP, top)
+        vals[top > 0] = potential
+        vals[m.dirichlet_mask(BID_COPPER_SURFACE) > 0] = 0.0
+    m.apply_dirichlet(b, x, mask, vals)
+    it, res = m.pcg(b, x, max_it=max_it, tol=tol, relative=relative, precond=precond, omega=omega)
+    return m, x, it, res, b
+
+
+def nodal_field(nodes, hexes, hex_marker, region_sign, phi_nodes):
+    n = len(nodes)
+    out = np.zeros(3 * n)
+    lib().orc_nodal_field(n, np.ascontiguousarray(nodes, np.float64).reshape(-1), len(hexes),
+                          np.ascontiguousarray(hexes, np.int32).reshape(-1),
+                          np.ascontiguousarray(hex_marker, np.int32), region_sign,
+                          np.ascontiguousarray(phi_nodes, np.float64), out)
+    return out.reshape(-1, 3)
+
+
+class LinTet:
+    def __init__(self, nodes, tets, tet_nbr, tet_marker, search_region=3):
+        self._L = _Lintet()
+        lib().orc_lintet_build(C.byref(self._L), len(nodes), np.ascontiguousarray(nodes, np.float64).reshape(-1),
+                               len(tets), np.ascontiguousarray(tets, np.int32).reshape(-1),
+                               np.ascontiguousarray(tet_nbr, np.int32).reshape(-1),
+                               np.ascontiguousarray(tet_marker, np.int32), search_region)
+        L = self._L
+        nt = L.n_tets
+        self.det0 = _arr(L.det0, nt, np.float64)
+        self.det = _arr(L.det, 16 * nt, np.float64).reshape(-1, 4, 4)
+        self.centroid = _arr(L.centroid, 3 * nt, np.float64).reshape(-1, 3)
+        self.nbr_ptr = _arr(L.nbr_ptr, nt + 1, np.int32)
+        self.nbr = _arr(L.nbr, int(self.nbr_ptr[-1]), np.int32)
+        self.marker = _arr(L.marker, nt, np.int32)
+
+    def __del__(self):
+        try:
+            lib().orc_lintet_free(C.byref(self._L))
+        except Exception:
+            pass
+
+    def locate(self, p, guess):
+        return lib().orc_lintet_locate(C.byref(self._L), np.ascontiguousarray(p, np.float64), guess)
+
+    def bcc(self, p, tet):
+        b = np.zeros(4)
+        lib().orc_lintet_bcc(C.byref(self._L), np.ascontiguousarray(p, np.float64), tet, b)
+        return b
+
+    def locate_chain(self, points, rank=1):
+        pts = np.ascontiguousarray(points, np.float64).reshape(-1)
+        out = np.zeros(len(pts) // 3, np.int32)
+        lib().orc_locate_chain(C.byref(self._L), len(out), pts, rank, out)
+        return out
+
+
+def linhex_shape_functions(xe_ucd, p):
+    sf = np.zeros(8)
+    lib().orc_linhex_shape_functions(np.ascontiguousarray(xe_ucd, np.float64).reshape(-1),
+                                     np.ascontiguousarray(p, np.float64), sf)
+    return sf
+
+
+def resistivity(T):
+    return lib().orc_resistivity(float(T))
+
+
+def sigma(T):
+    return lib().orc_sigma(float(T))
+
+
+def kappa(T, lorentz=2.44e-8):
+    return lib().orc_kappa(float(T), float(lorentz))

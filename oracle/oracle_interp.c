@@ -1,0 +1,350 @@
+/*
+ * CPU oracle, transfer-to-atoms part and coefficient tables.  TEST INFRASTRUCTURE ONLY
+ * (see oracle.h).  Every routine keeps the reference's operation order so that, compiled
+ * without FP contraction, it rounds exactly as the reference does:
+ *   InterpolatorCells::locate_cell       src/InterpolatorCells.cpp:268-307
+ *   LinearTetrahedra::precompute etc.    src/InterpolatorCells.cpp:523-661, 720-742
+ *   LinearHexahedra                      src/InterpolatorCells.cpp:1205-1562
+ *   Interpolator::store_elfield          src/Interpolator.cpp:125-140, 172-190
+ *   PhysicalQuantities                   src/PhysicalQuantities.cpp:31-64, 173-185
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ZERO 1e-15 /* InterpolatorCells::zero, include/InterpolatorCells.h:241 */
+
+typedef struct { double x, y, z; } v3;
+static v3 mk(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 scl(v3 a, double r) { return mk(a.x * r, a.y * r, a.z * r); }
+static v3 dvd(v3 a, double r) { return mk(a.x / r, a.y / r, a.z / r); }
+static v3 ld(const double *xyz, int i) { return mk(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]); }
+
+/* InterpolatorCells::determinant overloads, src/InterpolatorCells.cpp:471-500 */
+static double det2(v3 v1, v3 v2) {
+    return v1.x * (v2.y - v2.z) - v1.y * (v2.x - v2.z) + v1.z * (v2.x - v2.y);
+}
+static double det3(v3 v1, v3 v2, v3 v3_) {
+    return v1.x * (v2.y * v3_.z - v3_.y * v2.z) - v2.x * (v1.y * v3_.z - v3_.y * v1.z)
+         + v3_.x * (v1.y * v2.z - v2.y * v1.z);
+}
+static double det4(v3 v1, v3 v2, v3 v3_, v3 v4) {
+    const double d1 = det3(v2, v3_, v4);
+    const double d2 = det3(v1, v3_, v4);
+    const double d3 = det3(v1, v2, v4);
+    const double d4 = det3(v1, v2, v3_);
+    return d4 - d3 + d2 - d1;
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* linear tetrahedra                                                                      */
+/* ------------------------------------------------------------------------------------- */
+int orc_lintet_build(orc_lintet *L, int n_nodes, const double *xyz, int nt, const int *tets,
+                     const int *tet_nbr, const int *tet_marker, int search_region) {
+    memset(L, 0, sizeof(*L));
+    L->n_tets = nt;
+    L->det0 = malloc(sizeof(double) * (nt ? nt : 1));
+    L->det = malloc(sizeof(double) * 16 * (nt ? nt : 1));
+    L->centroid = malloc(sizeof(double) * 3 * (nt ? nt : 1));
+    L->marker = malloc(sizeof(int) * (nt ? nt : 1));
+    /* node2tets in increasing tet order */
+    int *cnt = calloc(n_nodes + 1, sizeof(int));
+    for (int i = 0; i < 4 * nt; ++i) cnt[tets[i] + 1]++;
+    for (int i = 0; i < n_nodes; ++i) cnt[i + 1] += cnt[i];
+    int *n2t = malloc(sizeof(int) * 4 * (nt ? nt : 1)), *pos = malloc(sizeof(int) * (n_nodes + 1));
+    memcpy(pos, cnt, sizeof(int) * (n_nodes + 1));
+    for (int t = 0; t < nt; ++t)
+        for (int k = 0; k < 4; ++k) n2t[pos[tets[4 * t + k]]++] = t;
+    free(pos);
+    /* neighbours: face neighbours in stored order, then node-sharing tets per node */
+    int64_t cap = 64LL * nt + 16, nn = 0;
+    L->nbr = malloc(sizeof(int) * cap);
+    L->nbr_ptr = malloc(sizeof(int) * (nt + 1));
+    L->nbr_ptr[0] = 0;
+    for (int t = 0; t < nt; ++t) {
+        const int *nb = tet_nbr + 4 * t;
+        for (int k = 0; k < 4; ++k)
+            if (nb[k] >= 0) {
+                if (nn >= cap) L->nbr = realloc(L->nbr, sizeof(int) * (cap *= 2));
+                L->nbr[nn++] = nb[k];
+            }
+        for (int k = 0; k < 4; ++k) {
+            const int node = tets[4 * t + k];
+            for (int e = cnt[node]; e < cnt[node + 1]; ++e) {
+                const int o = n2t[e];
+                if (o != t && o != nb[0] && o != nb[1] && o != nb[2] && o != nb[3]) {
+                    if (nn >= cap) L->nbr = realloc(L->nbr, sizeof(int) * (cap *= 2));
+                    L->nbr[nn++] = o;
+                }
+            }
+        }
+        L->nbr_ptr[t + 1] = (int)nn;
+    }
+    free(n2t);
+    free(cnt);
+    for (int t = 0; t < nt; ++t) {
+        const int *s = tets + 4 * t;
+        v3 v1 = ld(xyz, s[0]), v2 = ld(xyz, s[1]), v3_ = ld(xyz, s[2]), v4 = ld(xyz, s[3]);
+        /* TetgenCells::get_centroid: sum in order, times 1/dim */
+        v3 c = mk(0, 0, 0);
+        c = add(c, v1); c = add(c, v2); c = add(c, v3_); c = add(c, v4);
+        c = scl(c, 1.0 / 4);
+        L->centroid[3 * t] = c.x; L->centroid[3 * t + 1] = c.y; L->centroid[3 * t + 2] = c.z;
+        const double d0 = det4(v1, v2, v3_, v4);
+        L->det0[t] = 1.0 / d0;
+        double *D = L->det + 16 * t;
+        double d1, dd2, d3, dd4;
+        d1 = det2(mk(v2.y, v3_.y, v4.y), mk(v2.z, v3_.z, v4.z));
+        dd2 = det2(mk(v2.x, v3_.x, v4.x), mk(v2.z, v3_.z, v4.z));
+        d3 = det2(mk(v2.x, v3_.x, v4.x), mk(v2.y, v3_.y, v4.y));
+        dd4 = det3(mk(v2.x, v3_.x, v4.x), mk(v2.y, v3_.y, v4.y), mk(v2.z, v3_.z, v4.z));
+        D[0] = d1; D[1] = -dd2; D[2] = d3; D[3] = -dd4;
+        d1 = det2(mk(v1.y, v3_.y, v4.y), mk(v1.z, v3_.z, v4.z));
+        dd2 = det2(mk(v1.x, v3_.x, v4.x), mk(v1.z, v3_.z, v4.z));
+        d3 = det2(mk(v1.x, v3_.x, v4.x), mk(v1.y, v3_.y, v4.y));
+        dd4 = det3(mk(v1.x, v3_.x, v4.x), mk(v1.y, v3_.y, v4.y), mk(v1.z, v3_.z, v4.z));
+        D[4] = -d1; D[5] = dd2; D[6] = -d3; D[7] = dd4;
+        d1 = det2(mk(v1.y, v2.y, v4.y), mk(v1.z, v2.z, v4.z));
+        dd2 = det2(mk(v1.x, v2.x, v4.x), mk(v1.z, v2.z, v4.z));
+        d3 = det2(mk(v1.x, v2.x, v4.x), mk(v1.y, v2.y, v4.y));
+        dd4 = det3(mk(v1.x, v2.x, v4.x), mk(v1.y, v2.y, v4.y), mk(v1.z, v2.z, v4.z));
+        D[8] = d1; D[9] = -dd2; D[10] = d3; D[11] = -dd4;
+        d1 = det2(mk(v1.y, v2.y, v3_.y), mk(v1.z, v2.z, v3_.z));
+        dd2 = det2(mk(v1.x, v2.x, v3_.x), mk(v1.z, v2.z, v3_.z));
+        d3 = det2(mk(v1.x, v2.x, v3_.x), mk(v1.y, v2.y, v3_.y));
+        dd4 = det3(v1, v2, v3_);
+        D[12] = -d1; D[13] = dd2; D[14] = -d3; D[15] = dd4;
+        /* LinearTetrahedra::narrow_search_to (src/InterpolatorCells.cpp:720-742) */
+        if (search_region == 3) L->marker[t] = tet_marker[t] != 3;
+        else if (search_region == 1) L->marker[t] = tet_marker[t] == 3;
+        else L->marker[t] = 0;
+    }
+    return 0;
+}
+
+void orc_lintet_free(orc_lintet *L) {
+    free(L->det0); free(L->det); free(L->centroid); free(L->nbr_ptr); free(L->nbr); free(L->marker);
+    memset(L, 0, sizeof(*L));
+}
+
+static double dot4(const double p[3], const double *d) {
+    /* Vec4(point, 1).dotProduct(det_k) */
+    return p[0] * d[0] + p[1] * d[1] + p[2] * d[2] + 1.0 * d[3];
+}
+
+int orc_lintet_point_in_cell(const orc_lintet *L, const double p[3], int i) {
+    const double *D = L->det + 16 * i;
+    const double d0 = L->det0[i];
+    if (d0 * dot4(p, D) < -ZERO) return 0;
+    if (d0 * dot4(p, D + 4) < -ZERO) return 0;
+    if (d0 * dot4(p, D + 8) < -ZERO) return 0;
+    if (d0 * dot4(p, D + 12) < -ZERO) return 0;
+    return 1;
+}
+
+void orc_lintet_bcc(const orc_lintet *L, const double p[3], int t, double b[4]) {
+    const double *D = L->det + 16 * t;
+    for (int k = 0; k < 4; ++k) b[k] = ZERO + L->det0[t] * dot4(p, D + 4 * k);
+}
+
+int orc_lintet_locate(const orc_lintet *L, const double p[3], int guess) {
+    const int n = L->n_tets;
+    if (guess >= 0 && guess < n) {
+        if (orc_lintet_point_in_cell(L, p, guess)) return guess;
+        for (int e = L->nbr_ptr[guess]; e < L->nbr_ptr[guess + 1]; ++e)
+            if (orc_lintet_point_in_cell(L, p, L->nbr[e])) return L->nbr[e];
+    }
+    double best = 1e100;
+    int best_i = 0;
+    for (int c = 0; c < n; ++c) {
+        if (L->marker[c] == 0 && orc_lintet_point_in_cell(L, p, c)) return c;
+        const double xx = p[0] - L->centroid[3 * c];
+        const double yy = p[1] - L->centroid[3 * c + 1];
+        const double zz = p[2] - L->centroid[3 * c + 2];
+        const double d2 = xx * xx + yy * yy + zz * zz;
+        if (d2 < best) {
+            best = d2;
+            best_i = c;
+        }
+    }
+    return -best_i;
+}
+
+int orc_linhex_locate(const orc_lintet *L, const double p[3], int guess) {
+    int tet = guess / 4;
+    tet = orc_lintet_locate(L, p, tet);
+    const int sign = tet < 0 ? -1 : 1;
+    tet = abs(tet);
+    double b[4];
+    orc_lintet_bcc(L, p, tet, b);
+    if (b[0] >= b[1] && b[0] >= b[2] && b[0] >= b[3]) return sign * (4 * tet + 0);
+    if (b[1] >= b[0] && b[1] >= b[2] && b[1] >= b[3]) return sign * (4 * tet + 1);
+    if (b[2] >= b[0] && b[2] >= b[1] && b[2] >= b[3]) return sign * (4 * tet + 2);
+    if (b[3] >= b[0] && b[3] >= b[1] && b[3] >= b[2]) return sign * (4 * tet + 3);
+    return -1;
+}
+
+void orc_locate_chain(const orc_lintet *L, int np, const double *xyz, int rank, int *cells) {
+    int cell = -1;
+    for (int i = 0; i < np; ++i) {
+        const double *p = xyz + 3 * i;
+        const int g = abs(cell);
+        cell = rank == 3 ? orc_linhex_locate(L, p, g) : orc_lintet_locate(L, p, g);
+        cells[i] = cell;
+    }
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* linear hexahedra                                                                       */
+/* ------------------------------------------------------------------------------------- */
+void orc_linhex_shape_functions(const double xe[8][3], const double p[3], double sf[8]) {
+    v3 x[8];
+    for (int i = 0; i < 8; ++i) x[i] = mk(xe[i][0], xe[i][1], xe[i][2]);
+    /* LinearHexahedra::precompute, src/InterpolatorCells.cpp:1237-1244 */
+    v3 f0s = dvd(add(add(add(add(add(add(add(x[0], x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    v3 f1 = dvd(sub(add(add(sub(sub(add(add(scl(x[0], -1), x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    v3 f2 = dvd(add(add(sub(sub(add(add(sub(scl(x[0], -1), x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    v3 f3 = dvd(add(add(add(add(sub(sub(sub(scl(x[0], -1), x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    v3 f4 = dvd(sub(add(sub(add(sub(add(sub(x[0], x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    v3 f5 = dvd(sub(add(add(sub(add(sub(sub(x[0], x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    v3 f6 = dvd(add(add(sub(sub(sub(sub(add(x[0], x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    v3 f7 = dvd(sub(add(sub(add(add(sub(add(scl(x[0], -1), x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    /* project_to_nat_coords, src/InterpolatorCells.cpp:1272-1317 */
+    v3 f0 = sub(mk(p[0], p[1], p[2]), f0s);
+    double u = 0, v = 0, w = 0;
+    for (int i = 0; i < 20; ++i) {
+        v3 f = sub(sub(sub(sub(sub(sub(sub(f0, scl(f1, u)), scl(f2, v)), scl(f3, w)), scl(f4, u * v)),
+                           scl(f5, u * w)), scl(f6, v * w)), scl(f7, u * v * w));
+        v3 fu = add(add(add(f1, scl(f4, v)), scl(f5, w)), scl(f7, v * w));
+        v3 fv = add(add(add(f2, scl(f4, u)), scl(f6, w)), scl(f7, u * w));
+        v3 fw = add(add(add(f3, scl(f5, u)), scl(f6, v)), scl(f7, u * v));
+        double D = det3(fu, fv, fw);
+        D = 1.0 / D;
+        const double du = det3(f, fv, fw) * D;
+        const double dv = det3(fu, f, fw) * D;
+        const double dw = det3(fu, fv, f) * D;
+        u += du;
+        v += dv;
+        w += dw;
+        if (du * du + dv * dv + dw * dw < ZERO) break;
+    }
+    sf[0] = (1 - u) * (1 - v) * (1 - w) / 8.0;
+    sf[1] = (1 + u) * (1 - v) * (1 - w) / 8.0;
+    sf[2] = (1 + u) * (1 + v) * (1 - w) / 8.0;
+    sf[3] = (1 - u) * (1 + v) * (1 - w) / 8.0;
+    sf[4] = (1 - u) * (1 - v) * (1 + w) / 8.0;
+    sf[5] = (1 + u) * (1 - v) * (1 + w) / 8.0;
+    sf[6] = (1 + u) * (1 + v) * (1 + w) / 8.0;
+    sf[7] = (1 - u) * (1 + v) * (1 + w) / 8.0;
+}
+
+/* shape_fun_grads(hex, node), src/InterpolatorCells.cpp:1319-1332, 1462-1505 */
+static void corner_grads(const double *xyz, const int *shex, int node, v3 sfg[8]) {
+    static const int UVW[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                  {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    static const int NN[8][3] = {{1, 3, 4}, {0, 2, 5}, {3, 1, 6}, {2, 0, 7},
+                                 {5, 7, 0}, {4, 6, 1}, {7, 5, 2}, {6, 4, 3}};
+    const double uvw[3] = {UVW[node][0], UVW[node][1], UVW[node][2]};
+    const int *nnn = NN[node];
+    v3 vec0 = ld(xyz, shex[node]);
+    double J[3][3];
+    for (int i = 0; i < 3; ++i) {
+        v3 r = scl(sub(vec0, ld(xyz, shex[nnn[i]])), uvw[i] * 0.5);
+        J[i][0] = r.x; J[i][1] = r.y; J[i][2] = r.z;
+    }
+    double Jdet = det3(mk(J[0][0], J[0][1], J[0][2]), mk(J[1][0], J[1][1], J[1][2]),
+                       mk(J[2][0], J[2][1], J[2][2]));
+    Jdet = 1.0 / Jdet;
+    double Ji[3][3] = {
+        {J[1][1] * J[2][2] - J[2][1] * J[1][2], J[2][1] * J[0][2] - J[0][1] * J[2][2],
+         J[0][1] * J[1][2] - J[1][1] * J[0][2]},
+        {J[2][0] * J[1][2] - J[1][0] * J[2][2], J[0][0] * J[2][2] - J[2][0] * J[0][2],
+         J[1][0] * J[0][2] - J[0][0] * J[1][2]},
+        {J[1][0] * J[2][1] - J[2][0] * J[1][1], J[2][0] * J[0][1] - J[0][0] * J[2][1],
+         J[0][0] * J[1][1] - J[1][0] * J[0][1]}};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Ji[i][j] *= Jdet;
+    double g[8][3];
+    memset(g, 0, sizeof(g));
+    for (int i = 0; i < 3; ++i) {
+        g[node][i] = 0.5 * (Ji[i][0] * uvw[0] + Ji[i][1] * uvw[1] + Ji[i][2] * uvw[2]);
+        for (int j = 0; j < 3; ++j) g[nnn[j]][i] = -0.5 * Ji[i][j] * uvw[j];
+    }
+    for (int k = 0; k < 8; ++k) sfg[k] = mk(g[k][0], g[k][1], g[k][2]);
+}
+
+void orc_nodal_field(int n_nodes, const double *xyz, int nh, const int *hexes, const int *hex_marker,
+                     int region_sign, const double *phi, double *out) {
+    /* node2cells in hex order (src/Interpolator.cpp:61-76) */
+    int *cnt = calloc(n_nodes + 1, sizeof(int));
+    for (int h = 0; h < nh; ++h)
+        if ((region_sign > 0 && hex_marker[h] > 0) || (region_sign < 0 && hex_marker[h] < 0))
+            for (int k = 0; k < 8; ++k) cnt[hexes[8 * h + k] + 1]++;
+    for (int i = 0; i < n_nodes; ++i) cnt[i + 1] += cnt[i];
+    int *lst = malloc(sizeof(int) * (cnt[n_nodes] ? cnt[n_nodes] : 1));
+    int *pos = malloc(sizeof(int) * (n_nodes + 1));
+    memcpy(pos, cnt, sizeof(int) * (n_nodes + 1));
+    for (int h = 0; h < nh; ++h)
+        if ((region_sign > 0 && hex_marker[h] > 0) || (region_sign < 0 && hex_marker[h] < 0))
+            for (int k = 0; k < 8; ++k) lst[pos[hexes[8 * h + k]]++] = 8 * h + k;
+    for (int n = 0; n < n_nodes; ++n) {
+        v3 mean = mk(0, 0, 0);
+        const int nf = cnt[n + 1] - cnt[n];
+        if (nf > 0) {
+            for (int e = cnt[n]; e < cnt[n + 1]; ++e) {
+                const int h = lst[e] / 8, node = lst[e] % 8;
+                v3 sfg[8];
+                corner_grads(xyz, hexes + 8 * h, node, sfg);
+                v3 vi = mk(0, 0, 0);
+                for (int i = 0; i < 8; ++i) vi = sub(vi, scl(sfg[i], phi[hexes[8 * h + i]]));
+                mean = add(mean, vi);
+            }
+            mean = scl(mean, 1.0 / nf);
+        }
+        out[3 * n] = mean.x; out[3 * n + 1] = mean.y; out[3 * n + 2] = mean.z;
+    }
+    free(pos);
+    free(lst);
+    free(cnt);
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* physical quantities                                                                    */
+/* ------------------------------------------------------------------------------------- */
+/* resistivity of bulk Cu, (T [K], rho [Ohm nm]) pairs, src/PhysicalQuantityData.cpp:47-76 */
+static const double RHO_T[26] = {200, 250, 273.15, 300, 350, 400, 450, 500, 550, 600, 650, 700, 750,
+                                 800, 850, 900, 950, 1000, 1050, 1100, 1150, 1200, 1250, 1300, 1350, 1400};
+static const double RHO_V[26] = {
+    1.04880891725670e1, 1.38746906313026e1, 1.54302576609579e1, 1.72303253767393e1,
+    2.05822342557969e1, 2.39476190263668e1, 2.73376276457821e1, 3.07598544167200e1,
+    3.42197627890272e1, 3.77214997429762e1, 4.12683756079926e1, 4.48631564739703e1,
+    4.85082484331790e1, 5.22058173711257e1, 5.59578691681847e1, 5.97663048918425e1,
+    6.36329597840890e1, 6.75596315079091e1, 7.15481011300350e1, 7.56001491045029e1,
+    7.97175677635730e1, 8.39021713382574e1, 8.81558042149690e1, 9.24803479251429e1,
+    9.68777272230961e1, 1.01349915510292e2};
+
+static double lin_interp(double x) {
+    if (x <= RHO_T[0]) return RHO_V[0];
+    if (x >= RHO_T[25]) return RHO_V[25];
+    int i1 = 0;
+    while (RHO_T[i1] < x) ++i1; /* std::lower_bound on first */
+    const int i2 = i1 - 1;
+    return RHO_V[i2] + (RHO_V[i1] - RHO_V[i2]) * (x - RHO_T[i2]) / (RHO_T[i1] - RHO_T[i2]);
+}
+
+static double clampT(double T) {
+    if (T < RHO_T[0]) T = RHO_T[0];
+    if (T > RHO_T[25]) T = RHO_T[25];
+    return T;
+}
+
+double orc_resistivity(double T) { return 10. * lin_interp(clampT(T)); }
+double orc_sigma(double T) { return 1.0 / orc_resistivity(T); }
+double orc_kappa(double T, double lorentz) {
+    T = clampT(T);
+    return lorentz * T * orc_sigma(T);
+}
