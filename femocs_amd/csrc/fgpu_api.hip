@@ -1,0 +1,850 @@
+// C ABI of libfemocs_gpu and the batched CG drivers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "fgpu_internal.h"
+#include "fgpu_kernels.h"
+
+using namespace fgpu;
+
+namespace {
+
+// error plumbing: every entry point catches and converts to a status code
+struct HipError {
+    hipError_t e;
+    const char *what;
+};
+
+#define HIPCHECK(call)                                        \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) throw HipError{e_, #call};      \
+    } while (0)
+
+template <class F> int guarded(fgpu_ctx *ctx, F &&f) {
+    try {
+        return f();
+    } catch (const HipError &h) {
+        if (ctx) {
+            ctx->err = std::string(h.what) + ": " + hipGetErrorString(h.e);
+        }
+        return h.e == hipErrorOutOfMemory ? FGPU_ENOMEM : FGPU_EHIP;
+    } catch (const std::bad_alloc &) {
+        if (ctx) ctx->err = "host allocation failed";
+        return FGPU_ENOMEM;
+    } catch (const std::exception &ex) {
+        if (ctx) ctx->err = ex.what();
+        return FGPU_EINVAL;
+    }
+}
+
+template <class T> T *dalloc(size_t n) {
+    void *p = nullptr;
+    HIPCHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return static_cast<T *>(p);
+}
+template <class T> void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+template <class T> void h2d(T *dst, const T *src, size_t n, hipStream_t s) {
+    if (n) HIPCHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <class T> void d2h(T *dst, const T *src, size_t n, hipStream_t s) {
+    if (n) HIPCHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+constexpr int kXcd = 8;  // MI355X: workgroups are dealt round-robin over 8 XCDs
+
+int padded_blocks(int n) {
+    const int nb = (n + kBlock - 1) / kBlock;
+    return ((std::max(nb, 1) + kXcd - 1) / kXcd) * kXcd;
+}
+
+// Block table of one system: position i (relative to the system's base, a multiple of 8)
+// runs local block (i % 8) * (nb / 8) + i / 8, so each XCD gets one contiguous slice of the
+// system's cells or dofs and its L2 sees neighbouring data.
+void append_table(std::vector<int2> &tab, int sys, int nb) {
+    const int per = nb / kXcd;
+    for (int i = 0; i < nb; ++i) tab.push_back(make_int2(sys, (i % kXcd) * per + i / kXcd));
+}
+
+}  // namespace
+
+fgpu::Mesh::~Mesh() {
+    dfree(d_cells);
+    dfree(d_X64);
+    dfree(d_X32);
+    dfree(d_gptr);
+    dfree(d_gidx);
+    dfree(d_bf_icell);
+    dfree(d_bf_face);
+    dfree(d_fgptr);
+    dfree(d_fgidx);
+    dfree(d_v2c);
+    dfree(d_v2n);
+}
+
+fgpu::Sys::~Sys() {
+    dfree(b); dfree(x); dfree(r); dfree(z); dfree(p); dfree(q); dfree(Ye); dfree(invd); dfree(diag);
+    dfree(g); dfree(tmp); dfree(kq); dfree(gq); dfree(fslot); dfree(fval); dfree(d_dmask);
+    dfree(r32); dfree(z32); dfree(p32); dfree(q32); dfree(d32); dfree(Ye32); dfree(invd32);
+    dfree(kq32); dfree(gq32); dfree(scal); dfree(part); dfree(cnt);
+}
+
+// ------------------------------------------------------------------------------------------
+// context
+// ------------------------------------------------------------------------------------------
+extern "C" int fgpu_ctx_create(int device, fgpu_ctx **out) {
+    if (!out) return FGPU_EINVAL;
+    *out = nullptr;
+    fgpu_ctx *c = new (std::nothrow) fgpu_ctx();
+    if (!c) return FGPU_ENOMEM;
+    const int rc = guarded(c, [&] {
+        int n = 0;
+        HIPCHECK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) throw std::invalid_argument("no such device");
+        c->device = device;
+        HIPCHECK(hipSetDevice(device));
+        HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        return FGPU_OK;
+    });
+    if (rc != FGPU_OK) {
+        fprintf(stderr, "fgpu_ctx_create: %s\n", c->err.c_str());
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return FGPU_OK;
+}
+
+extern "C" void fgpu_ctx_destroy(fgpu_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+    }
+    if (c->d_tab) (void)hipFree(c->d_tab);
+    if (c->h_flags) (void)hipHostFree(c->h_flags);
+    delete c;
+}
+
+extern "C" const char *fgpu_ctx_error(const fgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int fgpu_ctx_sync(fgpu_ctx *c) {
+    if (!c) return FGPU_EINVAL;
+    return guarded(c, [&] {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return FGPU_OK;
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// mesh
+// ------------------------------------------------------------------------------------------
+static void upload_coords(fgpu_mesh *m, const std::vector<double> &X) {
+    hipStream_t s = m->ctx->stream;
+    h2d(m->d_X64, X.data(), X.size(), s);
+    std::vector<float> X32(X.size());
+    for (size_t i = 0; i < X.size(); ++i) X32[i] = (float)X[i];
+    h2d(m->d_X32, X32.data(), X32.size(), s);
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
+extern "C" int fgpu_mesh_import(fgpu_ctx *ctx, int nv, const double *xyz, int nc, const int *ucd, int top, int bottom,
+                                int sides, int other, fgpu_mesh **out) {
+    if (!ctx || !out) return FGPU_EINVAL;
+    *out = nullptr;
+    fgpu_mesh *m = new (std::nothrow) fgpu_mesh();
+    if (!m) return FGPU_ENOMEM;
+    m->ctx = ctx;
+    const int rc = guarded(ctx, [&] {
+        HIPCHECK(hipSetDevice(ctx->device));
+        const int r = build_topology(*m, nv, xyz, nc, ucd, top, bottom, sides, other, ctx->err);
+        if (r != FGPU_OK) return r;
+        if (m->n_cells == 0 || m->n_dofs == 0) {
+            ctx->err = "mesh import: empty mesh";
+            return FGPU_EINVAL;
+        }
+        std::vector<int> icells, gptr, gidx, fgptr, fgidx, bf_icell;
+        std::vector<double> X;
+        build_internal_order(*m, icells, X, gptr, gidx, fgptr, fgidx, bf_icell);
+        hipStream_t s = ctx->stream;
+        m->d_cells = dalloc<int>(icells.size());
+        h2d(m->d_cells, icells.data(), icells.size(), s);
+        m->d_X64 = dalloc<double>(X.size());
+        m->d_X32 = dalloc<float>(X.size());
+        m->d_gptr = dalloc<int>(gptr.size());
+        h2d(m->d_gptr, gptr.data(), gptr.size(), s);
+        m->d_gidx = dalloc<int>(gidx.size());
+        h2d(m->d_gidx, gidx.data(), gidx.size(), s);
+        const size_t nb = m->bf_cell.size();
+        m->d_bf_icell = dalloc<int>(nb);
+        h2d(m->d_bf_icell, bf_icell.data(), nb, s);
+        m->d_bf_face = dalloc<int>(nb);
+        h2d(m->d_bf_face, m->bf_face.data(), nb, s);
+        m->d_fgptr = dalloc<int>(fgptr.size());
+        h2d(m->d_fgptr, fgptr.data(), fgptr.size(), s);
+        m->d_fgidx = dalloc<int>(fgidx.size());
+        h2d(m->d_fgidx, fgidx.data(), fgidx.size(), s);
+        // gradient export: last cell (API order) touching each dof, in internal numbering
+        std::vector<int> v2c(m->n_dofs), v2n(m->n_dofs);
+        for (int i = 0; i < m->n_dofs; ++i) {
+            const int d = m->dof_of_idof[i];
+            v2c[i] = m->icell_of_cell[m->v2c_cell[d]];
+            v2n[i] = m->v2c_node[d];
+        }
+        m->d_v2c = dalloc<int>(m->n_dofs);
+        h2d(m->d_v2c, v2c.data(), v2c.size(), s);
+        m->d_v2n = dalloc<int>(m->n_dofs);
+        h2d(m->d_v2n, v2n.data(), v2n.size(), s);
+        upload_coords(m, X);
+        return FGPU_OK;
+    });
+    if (rc != FGPU_OK) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return FGPU_OK;
+}
+
+extern "C" void fgpu_mesh_destroy(fgpu_mesh *m) {
+    if (!m) return;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    delete m;
+}
+
+extern "C" int fgpu_mesh_sizes(const fgpu_mesh *m, int *nd, int *nc, int *nb) {
+    if (!m) return FGPU_EINVAL;
+    if (nd) *nd = m->n_dofs;
+    if (nc) *nc = m->n_cells;
+    if (nb) *nb = (int)m->bf_cell.size();
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_mesh_export_dofs(const fgpu_mesh *m, double *xyz, int *vod) {
+    if (!m) return FGPU_EINVAL;
+    if (xyz) std::memcpy(xyz, m->x.data(), m->x.size() * sizeof(double));
+    if (vod) std::memcpy(vod, m->vertex_of_dof.data(), m->vertex_of_dof.size() * sizeof(int));
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_mesh_export_bfaces(const fgpu_mesh *m, int *cell, int *face, int *bid) {
+    if (!m) return FGPU_EINVAL;
+    const size_t nb = m->bf_cell.size();
+    if (cell) std::memcpy(cell, m->bf_cell.data(), nb * sizeof(int));
+    if (face) std::memcpy(face, m->bf_face.data(), nb * sizeof(int));
+    if (bid) std::memcpy(bid, m->bf_id.data(), nb * sizeof(int));
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_mesh_export_face_centroids(const fgpu_mesh *m, int bid, double *xyz, int *n) {
+    static const int kFV[6][4] = {{0, 2, 4, 6}, {1, 3, 5, 7}, {0, 1, 4, 5}, {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
+    if (!m || !n) return FGPU_EINVAL;
+    int k = 0;
+    for (size_t f = 0; f < m->bf_cell.size(); ++f) {
+        if (m->bf_id[f] != bid) continue;
+        if (xyz)
+            for (int a = 0; a < 3; ++a) {
+                double s = 0;
+                for (int j = 0; j < 4; ++j) s += m->x[3 * (size_t)m->cells_lex[8 * (size_t)m->bf_cell[f] + kFV[m->bf_face[f]][j]] + a];
+                xyz[3 * (size_t)k + a] = s / 4.0;
+            }
+        ++k;
+    }
+    *n = k;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_mesh_update_vertices(fgpu_mesh *m, const double *xyz) {
+    if (!m || !xyz) return FGPU_EINVAL;
+    return guarded(m->ctx, [&] {
+        HIPCHECK(hipSetDevice(m->ctx->device));
+        for (int d = 0; d < m->n_dofs; ++d)
+            for (int a = 0; a < 3; ++a) m->x[3 * (size_t)d + a] = xyz[3 * (size_t)m->vertex_of_dof[d] + a];
+        std::vector<double> X(4 * (size_t)m->n_dofs, 0.0);
+        for (int i = 0; i < m->n_dofs; ++i)
+            for (int a = 0; a < 3; ++a) X[4 * (size_t)i + a] = m->x[3 * (size_t)m->dof_of_idof[i] + a];
+        upload_coords(m, X);
+        return FGPU_OK;
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// systems
+// ------------------------------------------------------------------------------------------
+static Desc make_desc64(fgpu_sys *s, bool residual) {
+    Desc d{};
+    fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+    d.n_dofs = m->n_dofs;
+    d.n_cells = m->n_cells;
+    d.nb_cell = s->nb_cell;
+    d.nb_dof = s->nb_dof;
+    d.cells = m->d_cells;
+    d.X = m->d_X64;
+    d.gptr = m->d_gptr;
+    d.gidx = m->d_gidx;
+    d.dmask = s->d_dmask;
+    d.kq = s->have_coef ? s->kq : nullptr;
+    d.gq = s->have_coef ? s->gq : nullptr;
+    d.ua = residual ? s->x : s->z;
+    d.ub = residual ? nullptr : s->p;
+    d.Ye = s->Ye;
+    d.q = s->q;
+    d.p = s->p;
+    d.z = s->z;
+    d.x = s->x;
+    d.r = s->r;
+    d.zw = s->z;
+    d.invdiag = s->invd;
+    d.b = s->b;
+    d.scal = s->scal;
+    d.part = s->part;
+    d.cnt = s->cnt;
+    return d;
+}
+
+static Desc make_desc32(fgpu_sys *s) {
+    Desc d{};
+    fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+    d.n_dofs = m->n_dofs;
+    d.n_cells = m->n_cells;
+    d.nb_cell = s->nb_cell;
+    d.nb_dof = s->nb_dof;
+    d.cells = m->d_cells;
+    d.X = m->d_X32;
+    d.gptr = m->d_gptr;
+    d.gidx = m->d_gidx;
+    d.dmask = s->d_dmask;
+    d.kq = s->have_coef ? s->kq32 : nullptr;
+    d.gq = s->have_coef ? s->gq32 : nullptr;
+    d.ua = s->z32;
+    d.ub = s->p32;
+    d.Ye = s->Ye32;
+    d.q = s->q32;
+    d.p = s->p32;
+    d.z = s->z32;
+    d.x = s->d32;
+    d.r = s->r32;
+    d.zw = s->z32;
+    d.invdiag = s->invd32;
+    d.b = nullptr;
+    d.scal = s->scal + kScal;
+    d.part = s->part + 6 * (size_t)s->nb_dof;
+    d.cnt = s->cnt + 8;
+    return d;
+}
+
+extern "C" int fgpu_sys_create(fgpu_mesh *m, fgpu_sys **out) {
+    if (!m || !out) return FGPU_EINVAL;
+    *out = nullptr;
+    fgpu_sys *s = new (std::nothrow) fgpu_sys();
+    if (!s) return FGPU_ENOMEM;
+    s->mesh = m;
+    s->ctx = m->ctx;
+    const int rc = guarded(m->ctx, [&] {
+        HIPCHECK(hipSetDevice(m->ctx->device));
+        const size_t n = m->n_dofs, nc = m->n_cells;
+        s->nb_cell = padded_blocks(m->n_cells);
+        s->nb_dof = padded_blocks(m->n_dofs);
+        s->b = dalloc<double>(n); s->x = dalloc<double>(n); s->r = dalloc<double>(n); s->z = dalloc<double>(n);
+        s->p = dalloc<double>(n); s->q = dalloc<double>(n); s->Ye = dalloc<double>(8 * nc);
+        s->invd = dalloc<double>(n); s->diag = dalloc<double>(n); s->g = dalloc<double>(n); s->tmp = dalloc<double>(n);
+        s->fslot = dalloc<double>(4 * m->bf_cell.size()); s->fval = dalloc<double>(m->bf_cell.size());
+        s->d_dmask = dalloc<unsigned char>(n);
+        s->r32 = dalloc<float>(n); s->z32 = dalloc<float>(n); s->p32 = dalloc<float>(n); s->q32 = dalloc<float>(n);
+        s->d32 = dalloc<float>(n); s->Ye32 = dalloc<float>(8 * nc); s->invd32 = dalloc<float>(n);
+        s->scal = dalloc<double>(2 * kScal);
+        s->part = dalloc<double>(12 * (size_t)s->nb_dof);
+        s->cnt = dalloc<unsigned int>(16);
+        hipStream_t st = m->ctx->stream;
+        HIPCHECK(hipMemsetAsync(s->cnt, 0, 16 * sizeof(unsigned int), st));
+        HIPCHECK(hipMemsetAsync(s->scal, 0, 2 * kScal * sizeof(double), st));
+        HIPCHECK(hipMemsetAsync(s->b, 0, n * sizeof(double), st));
+        HIPCHECK(hipMemsetAsync(s->x, 0, n * sizeof(double), st));
+        HIPCHECK(hipMemsetAsync(s->g, 0, n * sizeof(double), st));
+        HIPCHECK(hipMemsetAsync(s->d_dmask, 0, n, st));
+        s->dmask.assign(n, 0);
+        s->gval.assign(n, 0.0);
+        HIPCHECK(hipStreamSynchronize(st));
+        return FGPU_OK;
+    });
+    if (rc != FGPU_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return FGPU_OK;
+}
+
+extern "C" void fgpu_sys_destroy(fgpu_sys *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+}
+
+// single-system tables for setup launches: [Desc][elem tab][dof tab] in a scratch buffer
+struct OneSys {
+    Desc *D;
+    int2 *etab, *dtab;
+    int ne, nd;
+};
+
+static void *ensure_tab(fgpu_ctx *c, size_t bytes) {
+    if (bytes > c->tab_bytes) {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (c->d_tab) HIPCHECK(hipFree(c->d_tab));
+        c->d_tab = nullptr;
+        c->tab_bytes = 0;
+        const size_t nb = std::max<size_t>(bytes * 2, 1 << 16);
+        HIPCHECK(hipMalloc(&c->d_tab, nb));
+        c->tab_bytes = nb;
+    }
+    return c->d_tab;
+}
+
+// Uploads descriptors and tables of a batch into the context's table buffer.  Layout:
+// [ndesc_sets x n Desc][elem table][dof table]
+struct BatchTabs {
+    std::vector<const Desc *> sets;  // device pointers of each descriptor set
+    const int2 *etab = nullptr, *dtab = nullptr;
+    int ne = 0, nd = 0;
+};
+
+static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> &sets, const std::vector<fgpu_sys *> &sys) {
+    std::vector<int2> et, dt;
+    for (size_t i = 0; i < sys.size(); ++i) {
+        append_table(et, (int)i, sys[i]->nb_cell);
+        append_table(dt, (int)i, sys[i]->nb_dof);
+    }
+    const size_t n = sys.size();
+    const size_t dbytes = sets.size() * n * sizeof(Desc);
+    const size_t align = 256;
+    const size_t off_e = (dbytes + align - 1) / align * align;
+    const size_t off_d = off_e + (et.size() * sizeof(int2) + align - 1) / align * align;
+    const size_t total = off_d + dt.size() * sizeof(int2);
+    std::vector<char> host(total);
+    for (size_t k = 0; k < sets.size(); ++k) std::memcpy(host.data() + k * n * sizeof(Desc), sets[k].data(), n * sizeof(Desc));
+    std::memcpy(host.data() + off_e, et.data(), et.size() * sizeof(int2));
+    std::memcpy(host.data() + off_d, dt.data(), dt.size() * sizeof(int2));
+    char *dev = static_cast<char *>(ensure_tab(c, total));
+    HIPCHECK(hipMemcpyAsync(dev, host.data(), total, hipMemcpyHostToDevice, c->stream));
+    // the host buffer dies at return: make the copy complete first
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    BatchTabs t;
+    for (size_t k = 0; k < sets.size(); ++k) t.sets.push_back(reinterpret_cast<const Desc *>(dev + k * n * sizeof(Desc)));
+    t.etab = reinterpret_cast<const int2 *>(dev + off_e);
+    t.dtab = reinterpret_cast<const int2 *>(dev + off_d);
+    t.ne = (int)et.size();
+    t.nd = (int)dt.size();
+    return t;
+}
+
+static void ensure_diag(fgpu_sys *s) {
+    if (s->diag_valid) return;
+    fgpu_ctx *c = s->ctx;
+    fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+    Desc d = make_desc64(s, true);
+    d.q = s->diag;
+    BatchTabs t = upload_batch(c, {{d}}, {s});
+    launch_elem_diag<double>(c->stream, t.sets[0], t.etab, t.ne, s->have_coef);
+    launch_gather_plain<double>(c->stream, t.sets[0], t.dtab, t.nd, false);
+    launch_invdiag(c->stream, m->n_dofs, s->d_dmask, s->diag, s->invd, s->invd32);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    s->diag_valid = true;
+}
+
+extern "C" int fgpu_sys_set_coefficients(fgpu_sys *s, const double *kq, const double *gq) {
+    if (!s) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        hipStream_t st = s->ctx->stream;
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        if (!kq && !gq) {
+            s->have_coef = false;
+        } else {
+            const size_t nc = m->n_cells;
+            std::vector<double> k(8 * nc, 1.0), g(8 * nc, 0.0);
+            for (size_t i = 0; i < nc; ++i) {
+                const size_t c = m->cell_of_icell[i];
+                for (int q = 0; q < 8; ++q) {
+                    if (kq) k[8 * i + q] = kq[8 * c + q];
+                    if (gq) g[8 * i + q] = gq[8 * c + q];
+                }
+            }
+            if (!s->kq) {
+                s->kq = dalloc<double>(8 * nc); s->gq = dalloc<double>(8 * nc);
+                s->kq32 = dalloc<float>(8 * nc); s->gq32 = dalloc<float>(8 * nc);
+            }
+            std::vector<float> k32(k.begin(), k.end()), g32(g.begin(), g.end());
+            h2d(s->kq, k.data(), k.size(), st);
+            h2d(s->gq, g.data(), g.size(), st);
+            h2d(s->kq32, k32.data(), k32.size(), st);
+            h2d(s->gq32, g32.data(), g32.size(), st);
+            HIPCHECK(hipStreamSynchronize(st));
+            s->have_coef = true;
+        }
+        s->diag_valid = false;
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_sys_reset(fgpu_sys *s) {
+    if (!s) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        launch_set_const(s->ctx->stream, m->n_dofs, 0.0, s->b);
+        std::fill(s->dmask.begin(), s->dmask.end(), 0);
+        std::fill(s->gval.begin(), s->gval.end(), 0.0);
+        HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_sys_add_neumann(fgpu_sys *s, int bid, const double *fv, double cst) {
+    if (!s) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        hipStream_t st = s->ctx->stream;
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        const size_t nb = m->bf_cell.size();
+        std::vector<double> val(nb, 0.0);
+        std::vector<int> ids(m->bf_id);
+        int k = 0;
+        for (size_t f = 0; f < nb; ++f)
+            if (m->bf_id[f] == bid) val[f] = fv ? fv[k++] : cst;
+        int *d_id = dalloc<int>(nb);
+        h2d(d_id, ids.data(), nb, st);
+        h2d(s->fval, val.data(), nb, st);
+        launch_face_load(st, (int)nb, m->d_bf_icell, m->d_bf_face, d_id, bid, s->fval, m->d_cells, m->d_X64, s->fslot);
+        launch_face_gather(st, m->n_dofs, m->d_fgptr, m->d_fgidx, s->fslot, s->b);
+        HIPCHECK(hipStreamSynchronize(st));
+        dfree(d_id);
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_sys_add_cell_rhs(fgpu_sys *s, const double *fq) {
+    if (!s || !fq) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        hipStream_t st = s->ctx->stream;
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        const size_t nc = m->n_cells;
+        std::vector<double> f(8 * nc);
+        for (size_t i = 0; i < nc; ++i)
+            for (int q = 0; q < 8; ++q) f[8 * i + q] = fq[8 * (size_t)m->cell_of_icell[i] + q];
+        double *d_f = dalloc<double>(8 * nc);
+        h2d(d_f, f.data(), f.size(), st);
+        Desc d = make_desc64(s, true);
+        d.q = s->b;
+        BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
+        launch_elem_rhs(st, t.sets[0], t.etab, t.ne, d_f);
+        launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, true);
+        HIPCHECK(hipStreamSynchronize(st));
+        dfree(d_f);
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_sys_add_dirichlet(fgpu_sys *s, int bid, double value) {
+    static const int kFV[6][4] = {{0, 2, 4, 6}, {1, 3, 5, 7}, {0, 1, 4, 5}, {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
+    if (!s) return FGPU_EINVAL;
+    fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+    for (size_t f = 0; f < m->bf_cell.size(); ++f)
+        if (m->bf_id[f] == bid)
+            for (int j = 0; j < 4; ++j) {
+                const int d = m->cells_lex[8 * (size_t)m->bf_cell[f] + kFV[m->bf_face[f]][j]];
+                s->dmask[d] = 1;
+                s->gval[d] = value;
+            }
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_sys_apply_dirichlet(fgpu_sys *s) {
+    if (!s) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        hipStream_t st = s->ctx->stream;
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        const int n = m->n_dofs;
+        std::vector<unsigned char> mk(n);
+        std::vector<double> gv(n);
+        for (int i = 0; i < n; ++i) {
+            const int d = m->dof_of_idof[i];
+            mk[i] = s->dmask[d];
+            gv[i] = s->dmask[d] ? s->gval[d] : 0.0;
+        }
+        h2d(s->d_dmask, mk.data(), n, st);
+        h2d(s->g, gv.data(), n, st);
+        s->diag_valid = false;
+        ensure_diag(s);
+        // A g with the un-lifted operator
+        Desc d = make_desc64(s, true);
+        d.ua = s->g;
+        d.q = s->tmp;
+        BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
+        launch_elem<double>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA);
+        launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, false);
+        launch_lift(st, n, s->d_dmask, s->g, s->tmp, s->diag, s->b, s->x);
+        HIPCHECK(hipStreamSynchronize(st));
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_sys_fill_solution(fgpu_sys *s, double v) {
+    if (!s) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        launch_set_const(s->ctx->stream, m->n_dofs, v, s->x);
+        HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+        return FGPU_OK;
+    });
+}
+
+static void permute_to_internal(const fgpu_mesh *m, const double *api, std::vector<double> &out) {
+    out.resize(m->n_dofs);
+    for (int i = 0; i < m->n_dofs; ++i) out[i] = api[m->dof_of_idof[i]];
+}
+
+extern "C" int fgpu_sys_set_solution(fgpu_sys *s, const double *x) {
+    if (!s || !x) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        std::vector<double> v;
+        permute_to_internal(m, x, v);
+        h2d(s->x, v.data(), v.size(), s->ctx->stream);
+        HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+        return FGPU_OK;
+    });
+}
+
+static int get_vec(fgpu_sys *s, const double *dev, double *out) {
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        std::vector<double> v(m->n_dofs);
+        d2h(v.data(), dev, v.size(), s->ctx->stream);
+        HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+        for (int i = 0; i < m->n_dofs; ++i) out[m->dof_of_idof[i]] = v[i];
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_sys_get_solution(fgpu_sys *s, double *x) {
+    if (!s || !x) return FGPU_EINVAL;
+    return get_vec(s, s->x, x);
+}
+
+extern "C" int fgpu_sys_get_rhs(fgpu_sys *s, double *b) {
+    if (!s || !b) return FGPU_EINVAL;
+    return get_vec(s, s->b, b);
+}
+
+extern "C" int fgpu_sys_apply(fgpu_sys *s, const double *x, double *y) {
+    if (!s || !x || !y) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        hipStream_t st = s->ctx->stream;
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        std::vector<double> v;
+        permute_to_internal(m, x, v);
+        h2d(s->tmp, v.data(), v.size(), st);
+        Desc d = make_desc64(s, true);
+        d.ua = s->tmp;
+        d.q = s->q;
+        BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
+        launch_elem<double>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA);
+        launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, false);
+        HIPCHECK(hipStreamSynchronize(st));
+        return get_vec(s, s->q, y);
+    });
+}
+
+extern "C" int fgpu_sys_minmax(fgpu_sys *s, double *mn, double *mx) {
+    if (!s) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        launch_minmax(s->ctx->stream, m->n_dofs, s->x, s->tmp);
+        double h[2];
+        d2h(h, s->tmp, 2, s->ctx->stream);
+        HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+        if (mn) *mn = h[0];
+        if (mx) *mx = h[1];
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_sys_solution_grad(fgpu_sys *s, double *grad) {
+    if (!s || !grad) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        hipStream_t st = s->ctx->stream;
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        const int n = m->n_dofs;
+        double *dg = dalloc<double>(3 * (size_t)n);
+        launch_grad_export(st, n, m->d_v2c, m->d_v2n, m->d_cells, m->d_X64, s->x, dg);
+        std::vector<double> h(3 * (size_t)n);
+        d2h(h.data(), dg, h.size(), st);
+        HIPCHECK(hipStreamSynchronize(st));
+        dfree(dg);
+        for (int i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) grad[3 * (size_t)m->dof_of_idof[i] + a] = h[3 * (size_t)i + a];
+        return FGPU_OK;
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// batched solve
+// ------------------------------------------------------------------------------------------
+__global__ void k_collect(const Desc *D, int n, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = D[i].scal[S_ACTIVE];
+}
+
+static double *flags_buffer(fgpu_ctx *c, size_t n) {
+    if (n > c->flags_cap) {
+        if (c->h_flags) HIPCHECK(hipHostFree(c->h_flags));
+        c->h_flags = nullptr;
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->h_flags), 2 * n * sizeof(double), hipHostMallocDefault));
+        c->flags_cap = n;
+    }
+    return c->h_flags;
+}
+
+// true while any system of the set is active
+static bool any_active(fgpu_ctx *c, const Desc *D, int n, double *dflags) {
+    double *h = flags_buffer(c, n);
+    hipLaunchKernelGGL(k_collect, dim3((n + 255) / 256), dim3(256), 0, c->stream, D, n, dflags);
+    HIPCHECK(hipMemcpyAsync(h, dflags, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i)
+        if (h[i] != 0.0) return true;
+    return false;
+}
+
+template <class T>
+static void cg_chunk(hipStream_t st, const Desc *D, const BatchTabs &t, int k, bool coef) {
+    for (int j = 0; j < k; ++j) {
+        launch_elem<T>(st, D, t.etab, t.ne, coef, false, true, S_BETA);
+        launch_gather_cg<T>(st, D, t.dtab, t.nd);
+        launch_update_cg<T>(st, D, t.dtab, t.nd);
+    }
+}
+
+// runs CG chunks until every system of the set is inactive
+template <class T>
+static void run_cg(fgpu_ctx *c, const Desc *D, int n, const BatchTabs &t, bool coef, int k, bool use_graph, int max_chunks,
+                   double *dflags) {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    for (int chunk = 0; chunk < max_chunks; ++chunk) {
+        if (use_graph) {
+            if (!exec) {
+                HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                cg_chunk<T>(c->stream, D, t, k, coef);
+                HIPCHECK(hipStreamEndCapture(c->stream, &graph));
+                HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            }
+            HIPCHECK(hipGraphLaunch(exec, c->stream));
+        } else {
+            cg_chunk<T>(c->stream, D, t, k, coef);
+        }
+        if (!any_active(c, D, n, dflags)) break;
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+}
+
+static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_solve_opts *o, fgpu_solve_stats *stats) {
+    HIPCHECK(hipSetDevice(c->device));
+    std::vector<fgpu_sys *> sys(systems, systems + n);
+    bool coef = false;
+    for (auto *s : sys) {
+        if (!s || s->ctx != c) throw std::invalid_argument("system belongs to another context");
+        coef = coef || s->have_coef;
+    }
+    for (auto *s : sys)
+        if (s->have_coef != coef) throw std::invalid_argument("a batch mixes coefficient and plain systems");
+    const bool mixed = o->precision == FGPU_MIXED;
+    const int k = o->check_every > 0 ? o->check_every : 32;
+    const int max_it = std::max(0, o->max_it);
+    // initial scalars
+    for (auto *s : sys) {
+        ensure_diag(s);
+        double h[2 * kScal] = {0};
+        h[S_ACTIVE] = 1.0;
+        h[S_MAXIT] = max_it;
+        h[S_TOL] = o->tol;
+        h[S_REL] = o->relative ? 1.0 : 0.0;
+        h[kScal + S_ACTIVE] = 0.0;
+        HIPCHECK(hipMemcpyAsync(s->scal, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        launch_enforce_dirichlet(c->stream, m->n_dofs, s->d_dmask, s->g, s->x);
+    }
+    std::vector<Desc> cg64, res64, cg32;
+    for (auto *s : sys) {
+        cg64.push_back(make_desc64(s, false));
+        res64.push_back(make_desc64(s, true));
+        cg32.push_back(make_desc32(s));
+    }
+    BatchTabs t = upload_batch(c, {cg64, res64, cg32}, sys);
+    double *dflags = nullptr;
+    HIPCHECK(hipMalloc(&dflags, std::max(1, n) * sizeof(double)));
+    const Desc *D64 = t.sets[0], *R64 = t.sets[1], *D32 = t.sets[2];
+    // initial residual of the lifted system
+    launch_elem<double>(c->stream, R64, t.etab, t.ne, coef, true, true, S_BETA);
+    launch_gather_res<double>(c->stream, R64, t.dtab, t.nd);
+    const int max_chunks = max_it / k + 2;
+    if (!mixed) {
+        run_cg<double>(c, D64, n, t, coef, k, o->use_graph != 0, max_chunks, dflags);
+    } else {
+        const double eta = o->inner_tol > 0 ? o->inner_tol : 1e-4;
+        for (int outer = 0; outer < 1000; ++outer) {
+            if (!any_active(c, R64, n, dflags)) break;
+            launch_mixed_init(c->stream, D32, R64, t.dtab, t.nd, eta);
+            run_cg<float>(c, D32, n, t, coef, k, o->use_graph != 0, max_chunks, dflags);
+            launch_mixed_acc(c->stream, D32, R64, t.dtab, t.nd);
+            launch_elem<double>(c->stream, R64, t.etab, t.ne, coef, true, true, S_BETA);
+            launch_gather_res<double>(c->stream, R64, t.dtab, t.nd);
+        }
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipFree(dflags));
+    for (int i = 0; i < n; ++i) {
+        double h[kScal];
+        HIPCHECK(hipMemcpy(h, sys[i]->scal, sizeof(h), hipMemcpyDeviceToHost));
+        if (stats) {
+            const int it = (int)h[S_IT];
+            stats[i].iterations = h[S_CONV] != 0.0 ? it : -std::max(it, 1);
+            stats[i].outer = mixed ? (int)h[S_PQ] + 1 : 1;
+            stats[i].residual = std::sqrt(std::max(0.0, h[S_RR]));
+            stats[i].rhs_norm = std::sqrt(std::max(0.0, h[S_BB]));
+        }
+    }
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_solve_opts *o, fgpu_solve_stats *stats) {
+    if (!c || n <= 0 || !systems || !o) return FGPU_EINVAL;
+    return guarded(c, [&] { return solve_batch(c, n, systems, o, stats); });
+}
+
+extern "C" int fgpu_sys_solve(fgpu_sys *s, const fgpu_solve_opts *o, fgpu_solve_stats *stats) {
+    if (!s || !o) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] { return solve_batch(s->ctx, 1, &s, o, stats); });
+}

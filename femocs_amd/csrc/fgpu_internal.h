@@ -1,0 +1,129 @@
+// Internal structures of libfemocs_gpu: host-side mesh topology, device buffers, and the
+// per-system descriptor the batched kernels read.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/femocs_gpu.h"
+
+namespace fgpu {
+
+constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
+constexpr int kScal = 16;     // doubles of per-system scalars
+
+// indices into the per-system scalar block
+enum Scal : int {
+    S_RZ = 0,      // r.z of the current iterate
+    S_PQ = 1,      // p.q
+    S_ALPHA = 2,
+    S_BETA = 3,
+    S_RR = 4,      // |r|^2
+    S_THR2 = 5,    // stop threshold on |r|^2
+    S_IT = 6,      // iterations done
+    S_ACTIVE = 7,  // 1 while iterating
+    S_MAXIT = 8,
+    S_BB = 9,      // |b|^2
+    S_CONV = 10,   // 1 once converged
+    S_TOL = 11,    // tolerance (absolute or relative, see S_REL)
+    S_REL = 12,
+};
+
+// Device descriptor of one system for one launch family.  Pointers are typed by the
+// launch's precision (double or float) where marked T.
+struct Desc {
+    int n_dofs, n_cells, nb_cell, nb_dof;
+    const int *cells;             // 8 per cell, lexicographic, internal dof ids
+    const void *X;                // T4 coordinates per dof
+    const int *gptr, *gidx;       // dof -> element slots (cell*8+k), ascending
+    const unsigned char *dmask;   // 1 on constrained dofs
+    const void *kq, *gq;          // T, 8 per cell (stiffness / mass coefficient) or null
+    const void *ua, *ub;          // T: element input u = ua + beta*ub (ub may be null)
+    void *Ye;                     // T: 8 per cell
+    void *q;                      // T: gathered A u
+    void *p;                      // T: search direction (written by the CG gather)
+    const void *z;                // T: preconditioned residual
+    void *x, *r, *zw;             // T: iterate, residual, preconditioned residual (written)
+    const void *invdiag;          // T: 1/diag, 0 on constrained dofs
+    const double *b;              // FP64 rhs (residual mode)
+    double *scal;                 // kScal doubles
+    double *part;                 // 4*nb_dof partial sums
+    unsigned int *cnt;            // last-block counters
+};
+
+struct Mesh {
+    fgpu_ctx *ctx = nullptr;
+    int n_vertices = 0, n_dofs = 0, n_cells = 0;
+    // host topology in the API numbering
+    std::vector<int> vertex_of_dof;        // n_dofs
+    std::vector<int> dof_of_vertex;        // n_vertices, -1 unused
+    std::vector<int> cells_lex;            // 8*n_cells, API dofs, lexicographic
+    std::vector<double> x;                 // 3*n_dofs (API order)
+    std::vector<int> bf_cell, bf_face, bf_id;
+    // internal numbering: cells sorted along a space-filling curve, dofs by first touch
+    std::vector<int> icell_of_cell, cell_of_icell;
+    std::vector<int> idof_of_dof, dof_of_idof;
+    std::vector<int> v2c_cell, v2c_node;   // last cell (API) touching each dof and its node
+    // device arrays (internal numbering)
+    int *d_cells = nullptr;                // 8*n_cells
+    double *d_X64 = nullptr;               // 4*n_dofs
+    float *d_X32 = nullptr;                // 4*n_dofs
+    int *d_gptr = nullptr, *d_gidx = nullptr;
+    int *d_bf_icell = nullptr, *d_bf_face = nullptr;       // boundary faces (API order)
+    int *d_fgptr = nullptr, *d_fgidx = nullptr;            // dof -> boundary-face slots
+    int *d_v2c = nullptr, *d_v2n = nullptr;                 // gradient export (internal)
+    ~Mesh();
+};
+
+struct Sys {
+    Mesh *mesh = nullptr;
+    fgpu_ctx *ctx = nullptr;
+    // host-side boundary data (API dof order)
+    std::vector<unsigned char> dmask;
+    std::vector<double> gval;
+    bool have_coef = false;
+    bool diag_valid = false;
+    // FP64 arrays
+    double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+    double *Ye = nullptr, *invd = nullptr, *diag = nullptr, *g = nullptr, *tmp = nullptr;
+    double *kq = nullptr, *gq = nullptr, *fslot = nullptr, *fval = nullptr;
+    unsigned char *d_dmask = nullptr;
+    // FP32 arrays (mixed precision inner solve)
+    float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *d32 = nullptr;
+    float *Ye32 = nullptr, *invd32 = nullptr, *kq32 = nullptr, *gq32 = nullptr;
+    // scalars and reduction scratch: [0] FP64 CG / outer, [1] FP32 inner
+    double *scal = nullptr;       // 2*kScal
+    double *part = nullptr;       // 2 * 4*nb_dof
+    unsigned int *cnt = nullptr;  // 2 * 8
+    int nb_cell = 0, nb_dof = 0;
+    ~Sys();
+};
+
+}  // namespace fgpu
+
+struct fgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // batch tables reused between solves (grown on demand)
+    void *d_tab = nullptr;
+    size_t tab_bytes = 0;
+    double *h_flags = nullptr;    // pinned, per system
+    size_t flags_cap = 0;
+};
+
+struct fgpu_mesh : fgpu::Mesh {};
+struct fgpu_sys : fgpu::Sys {};
+
+namespace fgpu {
+// mesh topology built on the host (fgpu_mesh.cpp)
+int build_topology(Mesh &m, int nv, const double *xyz, int nc, const int *ucd, int top, int bottom,
+                   int sides, int other, std::string &err);
+// internal order and gather lists
+void build_internal_order(Mesh &m, std::vector<int> &icells, std::vector<double> &X,
+                          std::vector<int> &gptr, std::vector<int> &gidx, std::vector<int> &fgptr,
+                          std::vector<int> &fgidx, std::vector<int> &bf_icell);
+}  // namespace fgpu

@@ -1,0 +1,28 @@
+// Launch helpers of the gfx950 kernels (fgpu_kernels.hip).
+#pragma once
+#include "fgpu_internal.h"
+
+namespace fgpu {
+template <class T>
+void launch_elem(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef, bool maskin, bool active_only,
+                 int scal_slot);
+template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef);
+void launch_elem_rhs(hipStream_t s, const Desc *D, const int2 *tab, int nblk, const double *fq);
+template <class T> void launch_gather_cg(hipStream_t s, const Desc *D, const int2 *tab, int nblk);
+template <class T> void launch_update_cg(hipStream_t s, const Desc *D, const int2 *tab, int nblk);
+template <class T> void launch_gather_res(hipStream_t s, const Desc *D, const int2 *tab, int nblk);
+template <class T> void launch_gather_plain(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool accum);
+void launch_mixed_init(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk, double inner_tol);
+void launch_mixed_acc(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk);
+void launch_enforce_dirichlet(hipStream_t s, int n, const unsigned char *mask, const double *g, double *x);
+void launch_lift(hipStream_t s, int n, const unsigned char *mask, const double *g, const double *Ag, const double *diag,
+                 double *b, double *x);
+void launch_invdiag(hipStream_t s, int n, const unsigned char *mask, const double *diag, double *inv64, float *inv32);
+void launch_set_const(hipStream_t s, int n, double v, double *x);
+void launch_face_load(hipStream_t s, int nb, const int *bf_icell, const int *bf_face, const int *bf_id, int bid,
+                      const double *fval, const int *cells, const double *X, double *fslot);
+void launch_face_gather(hipStream_t s, int n, const int *fgptr, const int *fgidx, const double *fslot, double *b);
+void launch_minmax(hipStream_t s, int n, const double *x, double *out);
+void launch_grad_export(hipStream_t s, int n, const int *v2c, const int *v2n, const int *cells, const double *X,
+                        const double *u, double *grad);
+}  // namespace fgpu

@@ -1,0 +1,785 @@
+// gfx950 kernels of the field-solve path.
+//
+// Q1 hexahedra, QGauss(2) quadrature, trilinear geometry recomputed in registers.  The
+// operator is applied matrix-free in two deterministic steps per product:
+//   k_elem    one thread per cell: gather 8 node coordinates and values, sum-factorised
+//             reference gradients, per-point metric, transposed sum factorisation ->
+//             8 element outputs written to the slot buffer Ye (cell*8 + k)
+//   k_gather  one thread per dof: add the dof's slots in ascending order, fused with the
+//             CG vector work of that step and a block reduction; the last block of each
+//             system finishes the reduction in a fixed order and updates the scalars.
+// No floating-point atomics anywhere, so a system gives bit-identical results whatever
+// other systems share the launch.  Every kernel walks a block table (system, local block),
+// which is how one launch serves a batch of meshes of any sizes.
+#include <hip/hip_runtime.h>
+
+#include "fgpu_kernels.h"
+
+namespace fgpu {
+
+#define FG_G0 0.21132486540518711775   // (1 - 1/sqrt(3)) / 2: QGauss(2) point on [0,1]
+#define FG_G1 0.78867513459481288225   // (1 + 1/sqrt(3)) / 2
+
+template <class T> struct Vec4T;
+template <> struct Vec4T<double> { using type = double4; };
+template <> struct Vec4T<float> { using type = float4; };
+
+// 1D Q1 basis j at Gauss point a: 1 - g_a for j = 0, g_a for j = 1
+template <class T> __device__ __forceinline__ T Wt(int a, int j) { return a == j ? (T)FG_G1 : (T)FG_G0; }
+
+// values D[i][m] at the two nodes of both directions -> values at the 2x2 Gauss points
+template <class T> __device__ __forceinline__ void tp2(const T D[2][2], T F[2][2]) {
+    T E[2][2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) E[p][m] = Wt<T>(p, 0) * D[0][m] + Wt<T>(p, 1) * D[1][m];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) F[p][s] = Wt<T>(s, 0) * E[p][0] + Wt<T>(s, 1) * E[p][1];
+}
+
+// transpose of tp2
+template <class T> __device__ __forceinline__ void tp2t(const T S[2][2], T R[2][2]) {
+    T E[2][2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) E[p][m] = Wt<T>(0, m) * S[p][0] + Wt<T>(1, m) * S[p][1];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) R[j][m] = Wt<T>(0, j) * E[0][m] + Wt<T>(1, j) * E[1][m];
+}
+
+// reference derivative along direction dir of the trilinear interpolant of f (lexicographic
+// nodes i + 2j + 4k), at the 4 distinct Gauss-point pairs of the two other directions
+template <class T> __device__ __forceinline__ void dref(const T f[8], int dir, T F[2][2]) {
+    T D[2][2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            if (dir == 0) D[s][t] = f[1 + 2 * s + 4 * t] - f[2 * s + 4 * t];
+            else if (dir == 1) D[s][t] = f[s + 2 + 4 * t] - f[s + 4 * t];
+            else D[s][t] = f[s + 2 * t + 4] - f[s + 2 * t];
+        }
+    tp2<T>(D, F);
+}
+
+// scatter of a per-direction sum R back to nodes: +R on the upper node, -R on the lower
+template <class T> __device__ __forceinline__ void dref_t(const T S[2][2], int dir, T y[8]) {
+    T R[2][2];
+    tp2t<T>(S, R);
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            if (dir == 0) { y[1 + 2 * s + 4 * t] += R[s][t]; y[2 * s + 4 * t] -= R[s][t]; }
+            else if (dir == 1) { y[s + 2 + 4 * t] += R[s][t]; y[s + 4 * t] -= R[s][t]; }
+            else { y[s + 2 * t + 4] += R[s][t]; y[s + 2 * t] -= R[s][t]; }
+        }
+}
+
+// y = K_e u for one cell: K_e(i,j) = sum_q JxW (kappa_q grad phi_i . grad phi_j + gamma_q phi_i phi_j)
+template <class T, bool COEF>
+__device__ __forceinline__ void q1_cell(const T X[8][3], const T u[8], const T *kq, const T *gq, T y[8]) {
+    // columns of the Jacobian: dx/dxi at (b,c), dx/deta at (a,c), dx/dzeta at (a,b)
+    T J0[3][2][2], J1[3][2][2], J2[3][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T f[8];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) f[n] = X[n][c];
+        dref<T>(f, 0, J0[c]);
+        dref<T>(f, 1, J1[c]);
+        dref<T>(f, 2, J2[c]);
+    }
+    T du0[2][2], du1[2][2], du2[2][2];
+    dref<T>(u, 0, du0);
+    dref<T>(u, 1, du1);
+    dref<T>(u, 2, du2);
+    T uq[8];
+    if (COEF) {
+        // values at the Gauss points (three 1D interpolation sweeps)
+        T a1[8], a2[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int a = q & 1, r = q & 6;
+            a1[q] = Wt<T>(a, 0) * u[r] + Wt<T>(a, 1) * u[r + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int b = (q >> 1) & 1, r = q & 5;
+            a2[q] = Wt<T>(b, 0) * a1[r] + Wt<T>(b, 1) * a1[r + 2];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = (q >> 2) & 1, r = q & 3;
+            uq[q] = Wt<T>(c, 0) * a2[r] + Wt<T>(c, 1) * a2[r + 4];
+        }
+    }
+    T S0[2][2] = {{0, 0}, {0, 0}}, S1[2][2] = {{0, 0}, {0, 0}}, S2[2][2] = {{0, 0}, {0, 0}};
+    T mq[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int a = q & 1, b = (q >> 1) & 1, c = (q >> 2) & 1;
+        const T c0x = J0[0][b][c], c0y = J0[1][b][c], c0z = J0[2][b][c];
+        const T c1x = J1[0][a][c], c1y = J1[1][a][c], c1z = J1[2][a][c];
+        const T c2x = J2[0][a][b], c2y = J2[1][a][b], c2z = J2[2][a][b];
+        // rows of det * J^{-1}
+        const T a0x = c1y * c2z - c1z * c2y, a0y = c1z * c2x - c1x * c2z, a0z = c1x * c2y - c1y * c2x;
+        const T a1x = c2y * c0z - c2z * c0y, a1y = c2z * c0x - c2x * c0z, a1z = c2x * c0y - c2y * c0x;
+        const T a2x = c0y * c1z - c0z * c1y, a2y = c0z * c1x - c0x * c1z, a2z = c0x * c1y - c0y * c1x;
+        const T det = c0x * a0x + c0y * a0y + c0z * a0z;
+        const T g0 = du0[b][c], g1 = du1[a][c], g2 = du2[a][b];
+        // det * grad u
+        const T vx = g0 * a0x + g1 * a1x + g2 * a2x;
+        const T vy = g0 * a0y + g1 * a1y + g2 * a2y;
+        const T vz = g0 * a0z + g1 * a1z + g2 * a2z;
+        T s = (T)0.125 / det;
+        if (COEF) s *= kq[q];
+        S0[b][c] += s * (a0x * vx + a0y * vy + a0z * vz);
+        S1[a][c] += s * (a1x * vx + a1y * vy + a1z * vz);
+        S2[a][b] += s * (a2x * vx + a2y * vy + a2z * vz);
+        if (COEF) mq[q] = (T)0.125 * det * gq[q] * uq[q];
+    }
+#pragma unroll
+    for (int n = 0; n < 8; ++n) y[n] = 0;
+    dref_t<T>(S0, 0, y);
+    dref_t<T>(S1, 1, y);
+    dref_t<T>(S2, 2, y);
+    if (COEF) {
+        // y_n += sum_q phi_n(q) m_q: transposed interpolation sweeps
+        T b2[8], b1[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int k = (r >> 2) & 1, base = r & 3;
+            b2[r] = Wt<T>(0, k) * mq[base] + Wt<T>(1, k) * mq[base + 4];
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int j = (r >> 1) & 1, base = r & 5;
+            b1[r] = Wt<T>(0, j) * b2[base] + Wt<T>(1, j) * b2[base + 2];
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int i = r & 1, base = r & 6;
+            y[r] += Wt<T>(0, i) * b1[base] + Wt<T>(1, i) * b1[base + 1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// block sum of NV values; result valid in thread 0
+template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV]) {
+    __shared__ double sh[NV][kBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        v[i] = wave_sum(v[i]);
+        if (lane == 0) sh[i][w] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double s = 0;
+#pragma unroll
+            for (int k = 0; k < kBlock / 64; ++k) s += sh[i][k];
+            v[i] = s;
+        }
+    }
+}
+
+// Stores this block's partials and reports whether it is the last block of the system to do
+// so.  The last block then sums all partials in block order (fixed, so deterministic).
+template <int NV>
+__device__ __forceinline__ bool last_block_sum(double (&v)[NV], double *part, unsigned int *cnt, int nb, int lb) {
+    __shared__ bool is_last;
+    block_sum<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) part[i * nb + lb] = v[i];
+        __threadfence();
+        const unsigned prev = atomicAdd(cnt, 1u);
+        is_last = prev == (unsigned)(nb - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        acc[i] = 0;
+        for (int k = threadIdx.x; k < nb; k += kBlock)
+            acc[i] += __hip_atomic_load(part + i * nb + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum<NV>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = acc[i];
+        __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// element kernel
+// ---------------------------------------------------------------------------------------
+// u_n = ua_n + beta * ub_n (ub optional); MASKIN zeroes constrained dofs (lifted columns)
+template <class T, bool COEF, bool MASKIN, bool ACTIVE_ONLY>
+__global__ __launch_bounds__(kBlock) void k_elem(const Desc *__restrict__ D, const int2 *__restrict__ tab, int scal_slot) {
+    const int2 t = tab[blockIdx.x];
+    const Desc &d = D[t.x];
+    if (ACTIVE_ONLY && d.scal[S_ACTIVE] == 0.0) return;
+    const int c = t.y * kBlock + threadIdx.x;
+    if (c >= d.n_cells) return;
+    using V4 = typename Vec4T<T>::type;
+    const int4 *cp = reinterpret_cast<const int4 *>(d.cells) + 2 * (size_t)c;
+    const int4 n0 = cp[0], n1 = cp[1];
+    const int nid[8] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
+    const V4 *X = reinterpret_cast<const V4 *>(d.X);
+    const T *ua = reinterpret_cast<const T *>(d.ua);
+    const T *ub = reinterpret_cast<const T *>(d.ub);
+    const T beta = ub ? (T)d.scal[scal_slot] : (T)0;
+    T Xe[8][3], u[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const V4 p = X[nid[n]];
+        Xe[n][0] = p.x;
+        Xe[n][1] = p.y;
+        Xe[n][2] = p.z;
+        T v = ua[nid[n]];
+        if (ub) v += beta * ub[nid[n]];
+        if (MASKIN && d.dmask[nid[n]]) v = 0;
+        u[n] = v;
+    }
+    T kq[8], gq[8];
+    if (COEF) {
+        const T *k = reinterpret_cast<const T *>(d.kq) + 8 * (size_t)c;
+        const T *g = reinterpret_cast<const T *>(d.gq) + 8 * (size_t)c;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            kq[q] = k[q];
+            gq[q] = g[q];
+        }
+    }
+    T y[8];
+    q1_cell<T, COEF>(Xe, u, kq, gq, y);
+    T *Ye = reinterpret_cast<T *>(d.Ye) + 8 * (size_t)c;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) Ye[n] = y[n];
+}
+
+// element diagonals (unit vectors through the same element kernel)
+template <class T, bool COEF>
+__global__ __launch_bounds__(kBlock) void k_elem_diag(const Desc *__restrict__ D, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const Desc &d = D[t.x];
+    const int c = t.y * kBlock + threadIdx.x;
+    if (c >= d.n_cells) return;
+    using V4 = typename Vec4T<T>::type;
+    const int *cl = d.cells + 8 * (size_t)c;
+    const V4 *X = reinterpret_cast<const V4 *>(d.X);
+    T Xe[8][3];
+    for (int n = 0; n < 8; ++n) {
+        const V4 p = X[cl[n]];
+        Xe[n][0] = p.x;
+        Xe[n][1] = p.y;
+        Xe[n][2] = p.z;
+    }
+    T kq[8], gq[8];
+    if (COEF) {
+        for (int q = 0; q < 8; ++q) {
+            kq[q] = reinterpret_cast<const T *>(d.kq)[8 * (size_t)c + q];
+            gq[q] = reinterpret_cast<const T *>(d.gq)[8 * (size_t)c + q];
+        }
+    }
+    T *Ye = reinterpret_cast<T *>(d.Ye) + 8 * (size_t)c;
+    for (int n = 0; n < 8; ++n) {
+        T u[8], y[8];
+        for (int m = 0; m < 8; ++m) u[m] = m == n ? (T)1 : (T)0;
+        q1_cell<T, COEF>(Xe, u, kq, gq, y);
+        Ye[n] = y[n];
+    }
+}
+
+// cell right-hand side: 8 contributions int phi_n f, f per Gauss point (FP64)
+__global__ __launch_bounds__(kBlock) void k_elem_rhs(const Desc *__restrict__ D, const int2 *__restrict__ tab, const double *__restrict__ fq) {
+    const int2 t = tab[blockIdx.x];
+    const Desc &d = D[t.x];
+    const int c = t.y * kBlock + threadIdx.x;
+    if (c >= d.n_cells) return;
+    const int *cl = d.cells + 8 * (size_t)c;
+    const double4 *X = reinterpret_cast<const double4 *>(d.X);
+    double Xe[8][3];
+    for (int n = 0; n < 8; ++n) {
+        const double4 p = X[cl[n]];
+        Xe[n][0] = p.x;
+        Xe[n][1] = p.y;
+        Xe[n][2] = p.z;
+    }
+    double y[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < 8; ++q) {
+        const int a = q & 1, b = (q >> 1) & 1, cc = (q >> 2) & 1;
+        double J[3][3];
+        for (int co = 0; co < 3; ++co) {
+            double f[8], F[2][2];
+            for (int n = 0; n < 8; ++n) f[n] = Xe[n][co];
+            dref<double>(f, 0, F); J[co][0] = F[b][cc];
+            dref<double>(f, 1, F); J[co][1] = F[a][cc];
+            dref<double>(f, 2, F); J[co][2] = F[a][b];
+        }
+        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        const double jxw = 0.125 * det * fq[8 * (size_t)c + q];
+        for (int n = 0; n < 8; ++n) {
+            const double phi = Wt<double>(a, n & 1) * Wt<double>(b, (n >> 1) & 1) * Wt<double>(cc, (n >> 2) & 1);
+            y[n] += phi * jxw;
+        }
+    }
+    double *Ye = reinterpret_cast<double *>(d.Ye) + 8 * (size_t)c;
+    for (int n = 0; n < 8; ++n) Ye[n] = y[n];
+}
+
+// ---------------------------------------------------------------------------------------
+// per-dof kernels
+// ---------------------------------------------------------------------------------------
+template <class T> __device__ __forceinline__ T gather_slots(const Desc &d, int i) {
+    const T *Ye = reinterpret_cast<const T *>(d.Ye);
+    const int e = d.gptr[i + 1];
+    T s = 0;
+    for (int k = d.gptr[i]; k < e; ++k) s += Ye[d.gidx[k]];
+    return s;
+}
+
+// CG step part 1: q = A p (masked rows), p = z + beta p, p.q -> alpha
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gather_cg(const Desc *__restrict__ D, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const Desc &d = D[t.x];
+    if (d.scal[S_ACTIVE] == 0.0) return;
+    const int i = t.y * kBlock + threadIdx.x;
+    double v[1] = {0};
+    if (i < d.n_dofs) {
+        T q = gather_slots<T>(d, i);
+        if (d.dmask[i]) q = 0;
+        T *p = reinterpret_cast<T *>(d.p);
+        const T pn = reinterpret_cast<const T *>(d.z)[i] + (T)d.scal[S_BETA] * p[i];
+        p[i] = pn;
+        reinterpret_cast<T *>(d.q)[i] = q;
+        v[0] = (double)pn * (double)q;
+    }
+    if (last_block_sum<1>(v, d.part, d.cnt, d.nb_dof, t.y) && threadIdx.x == 0) {
+        d.scal[S_PQ] = v[0];
+        d.scal[S_ALPHA] = v[0] != 0.0 ? d.scal[S_RZ] / v[0] : 0.0;
+    }
+}
+
+// CG step part 2: x += alpha p, r -= alpha q, z = r / diag; r.z, r.r -> beta, convergence
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_update_cg(const Desc *__restrict__ D, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const Desc &d = D[t.x];
+    if (d.scal[S_ACTIVE] == 0.0) return;
+    const int i = t.y * kBlock + threadIdx.x;
+    double v[2] = {0, 0};
+    if (i < d.n_dofs) {
+        const T alpha = (T)d.scal[S_ALPHA];
+        T *x = reinterpret_cast<T *>(d.x);
+        T *r = reinterpret_cast<T *>(d.r);
+        const T pi = reinterpret_cast<const T *>(d.p)[i];
+        const T qi = reinterpret_cast<const T *>(d.q)[i];
+        x[i] += alpha * pi;
+        const T ri = r[i] - alpha * qi;
+        r[i] = ri;
+        const T zi = ri * reinterpret_cast<const T *>(d.invdiag)[i];
+        reinterpret_cast<T *>(d.zw)[i] = zi;
+        v[0] = (double)ri * (double)zi;
+        v[1] = (double)ri * (double)ri;
+    }
+    if (last_block_sum<2>(v, d.part + d.nb_dof, d.cnt + 1, d.nb_dof, t.y) && threadIdx.x == 0) {
+        double *s = d.scal;
+        const double rz_old = s[S_RZ];
+        s[S_BETA] = rz_old != 0.0 ? v[0] / rz_old : 0.0;
+        s[S_RZ] = v[0];
+        s[S_RR] = v[1];
+        s[S_IT] += 1.0;
+        const bool conv = v[1] <= s[S_THR2];
+        s[S_CONV] = conv ? 1.0 : 0.0;
+        s[S_ACTIVE] = (!conv && s[S_IT] < s[S_MAXIT]) ? 1.0 : 0.0;
+    }
+}
+
+// initial residual of the lifted system: r = b - A x (masked rows), z = r / diag, p = 0;
+// r.z, r.r, b.b -> scalars; activates the system unless already converged
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gather_res(const Desc *__restrict__ D, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const Desc &d = D[t.x];
+    if (d.scal[S_ACTIVE] == 0.0) return;
+    const int i = t.y * kBlock + threadIdx.x;
+    double v[3] = {0, 0, 0};
+    if (i < d.n_dofs) {
+        const double bi = d.b[i];
+        T r = d.dmask[i] ? (T)0 : (T)(bi - (double)gather_slots<T>(d, i));
+        const T z = r * reinterpret_cast<const T *>(d.invdiag)[i];
+        reinterpret_cast<T *>(d.r)[i] = r;
+        reinterpret_cast<T *>(d.zw)[i] = z;
+        reinterpret_cast<T *>(d.p)[i] = 0;
+        v[0] = (double)r * (double)z;
+        v[1] = (double)r * (double)r;
+        v[2] = bi * bi;
+    }
+    if (last_block_sum<3>(v, d.part + 2 * d.nb_dof, d.cnt + 2, d.nb_dof, t.y) && threadIdx.x == 0) {
+        double *s = d.scal;
+        s[S_RZ] = v[0];
+        s[S_RR] = v[1];
+        s[S_BB] = v[2];
+        s[S_BETA] = 0.0;
+        const double thr = s[S_REL] != 0.0 ? s[S_TOL] * sqrt(v[2]) : s[S_TOL];
+        s[S_THR2] = thr * thr;
+        const bool conv = v[1] <= s[S_THR2];
+        s[S_CONV] = conv ? 1.0 : 0.0;
+        s[S_ACTIVE] = (!conv && s[S_IT] < s[S_MAXIT]) ? 1.0 : 0.0;
+    }
+}
+
+// plain gather: q = sum of slots (the un-lifted product), optionally accumulated into q
+template <class T, bool ACCUM>
+__global__ __launch_bounds__(kBlock) void k_gather_plain(const Desc *__restrict__ D, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const Desc &d = D[t.x];
+    const int i = t.y * kBlock + threadIdx.x;
+    if (i >= d.n_dofs) return;
+    const T s = gather_slots<T>(d, i);
+    T *q = reinterpret_cast<T *>(d.q);
+    if (ACCUM) q[i] += s;
+    else q[i] = s;
+}
+
+// ---------------------------------------------------------------------------------------
+// mixed precision: FP32 inner solve of A d = r64 inside FP64 defect correction
+// ---------------------------------------------------------------------------------------
+// D32: the FP32 descriptor; r64: FP64 residual from k_gather_res<double> (outer scalars in
+// scal64).  Sets up d = 0, r32 = r64, z32 = r32/diag, p32 = 0 and the inner scalars.
+__global__ __launch_bounds__(kBlock) void k_mixed_init(const Desc *__restrict__ D32, const Desc *__restrict__ D64,
+                                                       const int2 *__restrict__ tab, double inner_tol) {
+    const int2 t = tab[blockIdx.x];
+    const Desc &d = D32[t.x];
+    const Desc &o = D64[t.x];
+    if (o.scal[S_ACTIVE] <= 0.0) {
+        if (t.y == 0 && threadIdx.x == 0) d.scal[S_ACTIVE] = 0.0;
+        return;
+    }
+    const int i = t.y * kBlock + threadIdx.x;
+    double v[2] = {0, 0};
+    if (i < d.n_dofs) {
+        const float r = (float)reinterpret_cast<const double *>(o.r)[i];
+        const float z = r * reinterpret_cast<const float *>(d.invdiag)[i];
+        reinterpret_cast<float *>(d.r)[i] = r;
+        reinterpret_cast<float *>(d.zw)[i] = z;
+        reinterpret_cast<float *>(d.p)[i] = 0.f;
+        reinterpret_cast<float *>(d.x)[i] = 0.f;
+        v[0] = (double)r * (double)z;
+        v[1] = (double)r * (double)r;
+    }
+    if (last_block_sum<2>(v, d.part, d.cnt, d.nb_dof, t.y) && threadIdx.x == 0) {
+        double *s = d.scal;
+        s[S_RZ] = v[0];
+        s[S_RR] = v[1];
+        s[S_BETA] = 0.0;
+        s[S_IT] = 0.0;
+        s[S_THR2] = inner_tol * inner_tol * v[1];
+        s[S_MAXIT] = o.scal[S_MAXIT] - o.scal[S_IT];
+        s[S_CONV] = 0.0;
+        s[S_ACTIVE] = (v[1] > s[S_THR2] && s[S_MAXIT] > 0) ? 1.0 : 0.0;
+    }
+}
+
+// x64 += d32 on systems whose outer iteration is active; counts the inner iterations
+__global__ __launch_bounds__(kBlock) void k_mixed_acc(const Desc *__restrict__ D32, const Desc *__restrict__ D64,
+                                                      const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const Desc &d = D32[t.x];
+    const Desc &o = D64[t.x];
+    if (o.scal[S_ACTIVE] <= 0.0) return;
+    const int i = t.y * kBlock + threadIdx.x;
+    if (i < d.n_dofs)
+        reinterpret_cast<double *>(o.x)[i] += (double)reinterpret_cast<const float *>(d.x)[i];
+    if (t.y == 0 && threadIdx.x == 0) {
+        o.scal[S_IT] += d.scal[S_IT];
+        o.scal[S_PQ] += 1.0;  // outer steps (S_PQ is unused by the outer FP64 residual)
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// setup kernels (FP64)
+// ---------------------------------------------------------------------------------------
+__global__ void k_enforce_dirichlet(int n, const unsigned char *__restrict__ mask, const double *__restrict__ g, double *__restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && mask[i]) x[i] = g[i];
+}
+
+// b_I -= (A g)_I, b_D = diag_D g_D, x_D = g_D
+__global__ void k_lift(int n, const unsigned char *__restrict__ mask, const double *__restrict__ g, const double *__restrict__ Ag,
+                       const double *__restrict__ diag, double *__restrict__ b, double *__restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mask[i]) {
+        b[i] = g[i] * diag[i];
+        x[i] = g[i];
+    } else {
+        b[i] -= Ag[i];
+    }
+}
+
+__global__ void k_invdiag(int n, const unsigned char *__restrict__ mask, const double *__restrict__ diag, double *__restrict__ inv64,
+                          float *__restrict__ inv32) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = mask[i] ? 0.0 : 1.0 / diag[i];
+    inv64[i] = v;
+    inv32[i] = (float)v;
+}
+
+__global__ void k_set_const(int n, double v, double *__restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+// Neumann load on the boundary faces with id bid: 4 contributions per face into fslot
+// (DealSolver::assemble_rhs, QGauss(2) on the face, bilinear face map)
+__global__ void k_face_load(int nb, const int *__restrict__ bf_icell, const int *__restrict__ bf_face, const int *__restrict__ bf_id,
+                            int bid, const double *__restrict__ fval, const int *__restrict__ cells, const double4 *__restrict__ X,
+                            double *__restrict__ fslot) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nb) return;
+    double out[4] = {0, 0, 0, 0};
+    if (bf_id[f] == bid) {
+        const int face = bf_face[f], dd = face / 2, side = face % 2;
+        const int a0 = dd == 0 ? 1 : 0, a1 = dd == 2 ? 1 : 2;
+        const int *cl = cells + 8 * (size_t)bf_icell[f];
+        // face vertices in lexicographic order of (a0, a1)
+        int fv[4];
+        for (int j = 0; j < 4; ++j) {
+            int idx[3];
+            idx[dd] = side;
+            idx[a0] = j & 1;
+            idx[a1] = j >> 1;
+            fv[j] = cl[idx[0] + 2 * idx[1] + 4 * idx[2]];
+        }
+        double P[4][3];
+        for (int j = 0; j < 4; ++j) {
+            const double4 p = X[fv[j]];
+            P[j][0] = p.x;
+            P[j][1] = p.y;
+            P[j][2] = p.z;
+        }
+        const double g = fval[f];
+        for (int q = 0; q < 4; ++q) {
+            const int qa = q & 1, qb = q >> 1;
+            double t0[3], t1[3];
+            for (int c = 0; c < 3; ++c) {
+                t0[c] = Wt<double>(qb, 0) * (P[1][c] - P[0][c]) + Wt<double>(qb, 1) * (P[3][c] - P[2][c]);
+                t1[c] = Wt<double>(qa, 0) * (P[2][c] - P[0][c]) + Wt<double>(qa, 1) * (P[3][c] - P[1][c]);
+            }
+            const double cx = t0[1] * t1[2] - t0[2] * t1[1];
+            const double cy = t0[2] * t1[0] - t0[0] * t1[2];
+            const double cz = t0[0] * t1[1] - t0[1] * t1[0];
+            const double jxw = sqrt(cx * cx + cy * cy + cz * cz) * 0.25;
+            for (int j = 0; j < 4; ++j) out[j] += Wt<double>(qa, j & 1) * Wt<double>(qb, j >> 1) * g * jxw;
+        }
+    }
+    for (int j = 0; j < 4; ++j) fslot[4 * (size_t)f + j] = out[j];
+}
+
+__global__ void k_face_gather(int n, const int *__restrict__ fgptr, const int *__restrict__ fgidx, const double *__restrict__ fslot,
+                              double *__restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0;
+    for (int k = fgptr[i]; k < fgptr[i + 1]; ++k) s += fslot[fgidx[k]];
+    b[i] += s;
+}
+
+// min / max of a vector (single block per call, n moderate)
+__global__ __launch_bounds__(kBlock) void k_minmax(int n, const double *__restrict__ x, double *__restrict__ out) {
+    double mn = 1e100, mx = -1e100;
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+        mn = fmin(mn, x[i]);
+        mx = fmax(mx, x[i]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = fmin(mn, __shfl_xor(mn, o, 64));
+        mx = fmax(mx, __shfl_xor(mx, o, 64));
+    }
+    __shared__ double s[2][kBlock / 64];
+    if ((threadIdx.x & 63) == 0) {
+        s[0][threadIdx.x >> 6] = mn;
+        s[1][threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; ++k) {
+            mn = fmin(mn, s[0][k]);
+            mx = fmax(mx, s[1][k]);
+        }
+        mn = fmin(mn, s[0][0]);
+        mx = fmax(mx, s[1][0]);
+        out[0] = mn;
+        out[1] = mx;
+    }
+}
+
+// -grad u at the Gauss point v2n of cell v2c (DealSolver::export_solution_grad)
+__global__ void k_grad_export(int n, const int *__restrict__ v2c, const int *__restrict__ v2n, const int *__restrict__ cells,
+                              const double4 *__restrict__ X, const double *__restrict__ u, double *__restrict__ grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = v2c[i], q = v2n[i];
+    const int a = q & 1, b = (q >> 1) & 1, cc = (q >> 2) & 1;
+    const int *cl = cells + 8 * (size_t)c;
+    double J[3][3], uv[8], du[3];
+    for (int co = 0; co < 3; ++co) {
+        double f[8], F[2][2];
+        for (int n2 = 0; n2 < 8; ++n2) {
+            const double4 p = X[cl[n2]];
+            f[n2] = co == 0 ? p.x : (co == 1 ? p.y : p.z);
+        }
+        dref<double>(f, 0, F); J[co][0] = F[b][cc];
+        dref<double>(f, 1, F); J[co][1] = F[a][cc];
+        dref<double>(f, 2, F); J[co][2] = F[a][b];
+    }
+    for (int n2 = 0; n2 < 8; ++n2) uv[n2] = u[cl[n2]];
+    {
+        double F[2][2];
+        dref<double>(uv, 0, F); du[0] = F[b][cc];
+        dref<double>(uv, 1, F); du[1] = F[a][cc];
+        dref<double>(uv, 2, F); du[2] = F[a][b];
+    }
+    // grad u = J^{-T} du: solve J^T g = du
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    double inv[3][3];
+    inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+    inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+    inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+    inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+    inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+    inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+    inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+    inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+    inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    for (int k = 0; k < 3; ++k) {
+        // dxi_j/dx_k = inv[j][k]
+        const double g = inv[0][k] * du[0] + inv[1][k] * du[1] + inv[2][k] * du[2];
+        grad[3 * (size_t)i + k] = -g;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// explicit instantiations through launch helpers
+// ---------------------------------------------------------------------------------------
+template <class T>
+void launch_elem(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef, bool maskin, bool active_only, int scal_slot) {
+    if (nblk <= 0) return;
+    const dim3 g(nblk), b(kBlock);
+#define FG_ELEM(C, M, A) hipLaunchKernelGGL((k_elem<T, C, M, A>), g, b, 0, s, D, tab, scal_slot)
+    if (coef) {
+        if (maskin) { if (active_only) FG_ELEM(true, true, true); else FG_ELEM(true, true, false); }
+        else { if (active_only) FG_ELEM(true, false, true); else FG_ELEM(true, false, false); }
+    } else {
+        if (maskin) { if (active_only) FG_ELEM(false, true, true); else FG_ELEM(false, true, false); }
+        else { if (active_only) FG_ELEM(false, false, true); else FG_ELEM(false, false, false); }
+    }
+#undef FG_ELEM
+}
+template void launch_elem<double>(hipStream_t, const Desc *, const int2 *, int, bool, bool, bool, int);
+template void launch_elem<float>(hipStream_t, const Desc *, const int2 *, int, bool, bool, bool, int);
+
+template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef) {
+    if (nblk <= 0) return;
+    if (coef) hipLaunchKernelGGL((k_elem_diag<T, true>), dim3(nblk), dim3(kBlock), 0, s, D, tab);
+    else hipLaunchKernelGGL((k_elem_diag<T, false>), dim3(nblk), dim3(kBlock), 0, s, D, tab);
+}
+template void launch_elem_diag<double>(hipStream_t, const Desc *, const int2 *, int, bool);
+
+void launch_elem_rhs(hipStream_t s, const Desc *D, const int2 *tab, int nblk, const double *fq) {
+    if (nblk > 0) hipLaunchKernelGGL(k_elem_rhs, dim3(nblk), dim3(kBlock), 0, s, D, tab, fq);
+}
+
+template <class T> void launch_gather_cg(hipStream_t s, const Desc *D, const int2 *tab, int nblk) {
+    if (nblk > 0) hipLaunchKernelGGL(k_gather_cg<T>, dim3(nblk), dim3(kBlock), 0, s, D, tab);
+}
+template <class T> void launch_update_cg(hipStream_t s, const Desc *D, const int2 *tab, int nblk) {
+    if (nblk > 0) hipLaunchKernelGGL(k_update_cg<T>, dim3(nblk), dim3(kBlock), 0, s, D, tab);
+}
+template <class T> void launch_gather_res(hipStream_t s, const Desc *D, const int2 *tab, int nblk) {
+    if (nblk > 0) hipLaunchKernelGGL(k_gather_res<T>, dim3(nblk), dim3(kBlock), 0, s, D, tab);
+}
+template <class T> void launch_gather_plain(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool accum) {
+    if (nblk <= 0) return;
+    if (accum) hipLaunchKernelGGL((k_gather_plain<T, true>), dim3(nblk), dim3(kBlock), 0, s, D, tab);
+    else hipLaunchKernelGGL((k_gather_plain<T, false>), dim3(nblk), dim3(kBlock), 0, s, D, tab);
+}
+template void launch_gather_cg<double>(hipStream_t, const Desc *, const int2 *, int);
+template void launch_gather_cg<float>(hipStream_t, const Desc *, const int2 *, int);
+template void launch_update_cg<double>(hipStream_t, const Desc *, const int2 *, int);
+template void launch_update_cg<float>(hipStream_t, const Desc *, const int2 *, int);
+template void launch_gather_res<double>(hipStream_t, const Desc *, const int2 *, int);
+template void launch_gather_plain<double>(hipStream_t, const Desc *, const int2 *, int, bool);
+
+void launch_mixed_init(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk, double inner_tol) {
+    if (nblk > 0) hipLaunchKernelGGL(k_mixed_init, dim3(nblk), dim3(kBlock), 0, s, D32, D64, tab, inner_tol);
+}
+void launch_mixed_acc(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk) {
+    if (nblk > 0) hipLaunchKernelGGL(k_mixed_acc, dim3(nblk), dim3(kBlock), 0, s, D32, D64, tab);
+}
+
+static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
+
+void launch_enforce_dirichlet(hipStream_t s, int n, const unsigned char *mask, const double *g, double *x) {
+    if (n > 0) hipLaunchKernelGGL(k_enforce_dirichlet, dim3(grid_for(n)), dim3(kBlock), 0, s, n, mask, g, x);
+}
+void launch_lift(hipStream_t s, int n, const unsigned char *mask, const double *g, const double *Ag, const double *diag,
+                 double *b, double *x) {
+    if (n > 0) hipLaunchKernelGGL(k_lift, dim3(grid_for(n)), dim3(kBlock), 0, s, n, mask, g, Ag, diag, b, x);
+}
+void launch_invdiag(hipStream_t s, int n, const unsigned char *mask, const double *diag, double *inv64, float *inv32) {
+    if (n > 0) hipLaunchKernelGGL(k_invdiag, dim3(grid_for(n)), dim3(kBlock), 0, s, n, mask, diag, inv64, inv32);
+}
+void launch_set_const(hipStream_t s, int n, double v, double *x) {
+    if (n > 0) hipLaunchKernelGGL(k_set_const, dim3(grid_for(n)), dim3(kBlock), 0, s, n, v, x);
+}
+void launch_face_load(hipStream_t s, int nb, const int *bf_icell, const int *bf_face, const int *bf_id, int bid,
+                      const double *fval, const int *cells, const double *X, double *fslot) {
+    if (nb > 0)
+        hipLaunchKernelGGL(k_face_load, dim3(grid_for(nb)), dim3(kBlock), 0, s, nb, bf_icell, bf_face, bf_id, bid, fval, cells,
+                           reinterpret_cast<const double4 *>(X), fslot);
+}
+void launch_face_gather(hipStream_t s, int n, const int *fgptr, const int *fgidx, const double *fslot, double *b) {
+    if (n > 0) hipLaunchKernelGGL(k_face_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, n, fgptr, fgidx, fslot, b);
+}
+void launch_minmax(hipStream_t s, int n, const double *x, double *out) {
+    hipLaunchKernelGGL(k_minmax, dim3(1), dim3(kBlock), 0, s, n, x, out);
+}
+void launch_grad_export(hipStream_t s, int n, const int *v2c, const int *v2n, const int *cells, const double *X, const double *u,
+                        double *grad) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_grad_export, dim3(grid_for(n)), dim3(kBlock), 0, s, n, v2c, v2n, cells,
+                           reinterpret_cast<const double4 *>(X), u, grad);
+}
+
+}  // namespace fgpu

@@ -1,0 +1,257 @@
+"""Python front end of libfemocs_gpu.so (the C ABI in include/femocs_gpu.h).
+
+Thin ctypes wrappers: every call goes to the HIP library; there is no CPU fallback.  If the
+library is missing or no GPU is visible, the constructors raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfemocs_gpu.so")
+
+FGPU_FP64, FGPU_MIXED = 0, 1
+FGPU_PC_NONE, FGPU_PC_JACOBI = 0, 1
+BID_COPPER_SURFACE, BID_VACUUM_TOP, BID_COPPER_BOTTOM, BID_SIDES = 2, 8, 7, 4
+
+_lib = None
+
+
+class FgpuError(RuntimeError):
+    pass
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("max_it", C.c_int), ("tol", C.c_double), ("relative", C.c_int), ("precond", C.c_int),
+                ("precision", C.c_int), ("inner_tol", C.c_double), ("check_every", C.c_int),
+                ("use_graph", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("outer", C.c_int), ("residual", C.c_double), ("rhs_norm", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_I = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_P = C.c_void_p
+
+# (name, restype, argtypes): the exported C ABI
+API = [
+    ("fgpu_ctx_create", C.c_int, [C.c_int, C.POINTER(_P)]),
+    ("fgpu_ctx_destroy", None, [_P]),
+    ("fgpu_ctx_error", C.c_char_p, [_P]),
+    ("fgpu_ctx_sync", C.c_int, [_P]),
+    ("fgpu_mesh_import", C.c_int, [_P, C.c_int, _D, C.c_int, _I, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
+    ("fgpu_mesh_destroy", None, [_P]),
+    ("fgpu_mesh_sizes", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("fgpu_mesh_export_dofs", C.c_int, [_P, _P, _P]),
+    ("fgpu_mesh_export_bfaces", C.c_int, [_P, _P, _P, _P]),
+    ("fgpu_mesh_export_face_centroids", C.c_int, [_P, C.c_int, _P, C.POINTER(C.c_int)]),
+    ("fgpu_mesh_update_vertices", C.c_int, [_P, _D]),
+    ("fgpu_sys_create", C.c_int, [_P, C.POINTER(_P)]),
+    ("fgpu_sys_destroy", None, [_P]),
+    ("fgpu_sys_set_coefficients", C.c_int, [_P, _P, _P]),
+    ("fgpu_sys_reset", C.c_int, [_P]),
+    ("fgpu_sys_add_neumann", C.c_int, [_P, C.c_int, _P, C.c_double]),
+    ("fgpu_sys_add_cell_rhs", C.c_int, [_P, _D]),
+    ("fgpu_sys_add_dirichlet", C.c_int, [_P, C.c_int, C.c_double]),
+    ("fgpu_sys_apply_dirichlet", C.c_int, [_P]),
+    ("fgpu_sys_fill_solution", C.c_int, [_P, C.c_double]),
+    ("fgpu_sys_set_solution", C.c_int, [_P, _D]),
+    ("fgpu_sys_get_solution", C.c_int, [_P, _D]),
+    ("fgpu_sys_get_rhs", C.c_int, [_P, _D]),
+    ("fgpu_sys_apply", C.c_int, [_P, _D, _D]),
+    ("fgpu_sys_solve", C.c_int, [_P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    ("fgpu_solve_batch", C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    ("fgpu_sys_minmax", C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("fgpu_sys_solution_grad", C.c_int, [_P, _D]),
+]
+
+
+def load(path: str = LIB_PATH):
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise FgpuError(f"{path} is missing: build it with `python -m femocs_amd.build`")
+        lib = C.CDLL(path)
+        for name, res, args in API:
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = _P()
+        rc = self.lib.fgpu_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise FgpuError(f"fgpu_ctx_create failed ({rc}): no usable GPU")
+        self.h = h
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.fgpu_ctx_error(self.h)
+            raise FgpuError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def sync(self):
+        self.check(self.lib.fgpu_ctx_sync(self.h), "sync")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fgpu_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve_batch(self, systems, opts: SolveOpts):
+        n = len(systems)
+        arr = (_P * n)(*[s.h for s in systems])
+        stats = (SolveStats * n)()
+        self.check(self.lib.fgpu_solve_batch(self.h, n, arr, C.byref(opts), stats), "fgpu_solve_batch")
+        return [stats[i] for i in range(n)]
+
+
+class Mesh:
+    """Device mesh: DealSolver::import_mesh + mark_boundary."""
+
+    def __init__(self, ctx: Context, vertices, ucd_cells, top, bottom, sides, other):
+        self.ctx = ctx
+        v = np.ascontiguousarray(vertices, np.float64).reshape(-1)
+        c = np.ascontiguousarray(ucd_cells, np.int32).reshape(-1)
+        h = _P()
+        ctx.check(ctx.lib.fgpu_mesh_import(ctx.h, len(v) // 3, v, len(c) // 8, c, top, bottom, sides, other,
+                                           C.byref(h)), "fgpu_mesh_import")
+        self.h = h
+        nd, nc, nb = C.c_int(), C.c_int(), C.c_int()
+        ctx.lib.fgpu_mesh_sizes(h, C.byref(nd), C.byref(nc), C.byref(nb))
+        self.n_dofs, self.n_cells, self.n_bfaces = nd.value, nc.value, nb.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.fgpu_mesh_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def dofs(self):
+        x = np.zeros(3 * self.n_dofs)
+        v = np.zeros(self.n_dofs, np.int32)
+        self.ctx.lib.fgpu_mesh_export_dofs(self.h, x.ctypes.data, v.ctypes.data)
+        return x.reshape(-1, 3), v
+
+    def bfaces(self):
+        c, f, b = (np.zeros(self.n_bfaces, np.int32) for _ in range(3))
+        self.ctx.lib.fgpu_mesh_export_bfaces(self.h, c.ctypes.data, f.ctypes.data, b.ctypes.data)
+        return c, f, b
+
+    def face_centroids(self, bid):
+        n = C.c_int()
+        self.ctx.lib.fgpu_mesh_export_face_centroids(self.h, bid, None, C.byref(n))
+        out = np.zeros(3 * n.value)
+        self.ctx.lib.fgpu_mesh_export_face_centroids(self.h, bid, out.ctypes.data, C.byref(n))
+        return out.reshape(-1, 3)
+
+    def update_vertices(self, vertices):
+        v = np.ascontiguousarray(vertices, np.float64).reshape(-1)
+        self.ctx.check(self.ctx.lib.fgpu_mesh_update_vertices(self.h, v), "fgpu_mesh_update_vertices")
+
+
+class System:
+    def __init__(self, mesh: Mesh):
+        self.mesh = mesh
+        self.ctx = mesh.ctx
+        h = _P()
+        self.ctx.check(self.ctx.lib.fgpu_sys_create(mesh.h, C.byref(h)), "fgpu_sys_create")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.fgpu_sys_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _c(self, rc, what):
+        self.ctx.check(rc, what)
+
+    def set_coefficients(self, kappa_q=None, gamma_q=None):
+        k = None if kappa_q is None else np.ascontiguousarray(kappa_q, np.float64)
+        g = None if gamma_q is None else np.ascontiguousarray(gamma_q, np.float64)
+        self._c(self.ctx.lib.fgpu_sys_set_coefficients(self.h, _ptr(k), _ptr(g)), "set_coefficients")
+
+    def reset(self):
+        self._c(self.ctx.lib.fgpu_sys_reset(self.h), "reset")
+
+    def add_neumann(self, bid, values=None, constant=0.0):
+        v = None if values is None else np.ascontiguousarray(values, np.float64)
+        self._c(self.ctx.lib.fgpu_sys_add_neumann(self.h, bid, _ptr(v), float(constant)), "add_neumann")
+
+    def add_cell_rhs(self, f_q):
+        self._c(self.ctx.lib.fgpu_sys_add_cell_rhs(self.h, np.ascontiguousarray(f_q, np.float64)), "add_cell_rhs")
+
+    def add_dirichlet(self, bid, value):
+        self._c(self.ctx.lib.fgpu_sys_add_dirichlet(self.h, bid, float(value)), "add_dirichlet")
+
+    def apply_dirichlet(self):
+        self._c(self.ctx.lib.fgpu_sys_apply_dirichlet(self.h), "apply_dirichlet")
+
+    def fill_solution(self, v):
+        self._c(self.ctx.lib.fgpu_sys_fill_solution(self.h, float(v)), "fill_solution")
+
+    def set_solution(self, x):
+        self._c(self.ctx.lib.fgpu_sys_set_solution(self.h, np.ascontiguousarray(x, np.float64)), "set_solution")
+
+    def solution(self):
+        x = np.zeros(self.mesh.n_dofs)
+        self._c(self.ctx.lib.fgpu_sys_get_solution(self.h, x), "get_solution")
+        return x
+
+    def rhs(self):
+        b = np.zeros(self.mesh.n_dofs)
+        self._c(self.ctx.lib.fgpu_sys_get_rhs(self.h, b), "get_rhs")
+        return b
+
+    def apply(self, x):
+        y = np.zeros(self.mesh.n_dofs)
+        self._c(self.ctx.lib.fgpu_sys_apply(self.h, np.ascontiguousarray(x, np.float64), y), "apply")
+        return y
+
+    def solve(self, opts: SolveOpts) -> SolveStats:
+        st = SolveStats()
+        self._c(self.ctx.lib.fgpu_sys_solve(self.h, C.byref(opts), C.byref(st)), "solve")
+        return st
+
+    def minmax(self):
+        a, b = C.c_double(), C.c_double()
+        self._c(self.ctx.lib.fgpu_sys_minmax(self.h, C.byref(a), C.byref(b)), "minmax")
+        return a.value, b.value
+
+    def solution_grad(self):
+        g = np.zeros(3 * self.mesh.n_dofs)
+        self._c(self.ctx.lib.fgpu_sys_solution_grad(self.h, g), "solution_grad")
+        return g.reshape(-1, 3)
+
+
+def opts(max_it=10000, tol=1e-9, relative=True, precond=FGPU_PC_JACOBI, precision=FGPU_FP64, inner_tol=1e-4,
+         check_every=32, use_graph=True) -> SolveOpts:
+    return SolveOpts(max_it, tol, int(relative), precond, precision, inner_tol, check_every, int(use_graph))

@@ -1,0 +1,154 @@
+"""Host-side mirror of the reference's solver classes, running on libfemocs_gpu.
+
+``PoissonSolver`` keeps the method contract of the reference's PoissonSolver<3>
+(include/PoissonSolver.h:24-105, src/PoissonSolver.cpp): import_mesh, setup, assemble, solve,
+export_solution, check_limits/stat, size, get_n_cells.  The matrix never exists: the
+operator is applied matrix-free on the GPU.  Solver differences, stated where they arise:
+Jacobi instead of SSOR (SSOR's sweeps are sequential) and an optional FP32-inner mixed
+precision mode; the converged solution is the same discrete solution.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import fgpu
+from .fgpu import BID_COPPER_BOTTOM, BID_COPPER_SURFACE, BID_SIDES, BID_VACUUM_TOP
+
+
+@dataclass
+class FieldConfig:
+    """Config::Field with the reference defaults (src/Config.cpp:63-72)."""
+    E0: float = 0.0
+    ssor_param: float = 1.2
+    cg_tolerance: float = 1e-9
+    n_cg: int = 10000
+    V0: float = 0.0
+    anode_BC: str = "neumann"
+    mode: str = "laplace"
+    V_min: float = -1.0
+    V_max: float = 1e4
+
+
+_default_ctx = None
+
+
+def default_context() -> fgpu.Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = fgpu.Context(0)
+    return _default_ctx
+
+
+class Stat:
+    def __init__(self):
+        self.sol_min = 0.0
+        self.sol_max = 0.0
+
+    def __repr__(self):
+        return f"min={self.sol_min}, max={self.sol_max}"
+
+
+class DealSolver:
+    """Shared part of the reference's DealSolver<3> (include/DealSolver.h:33-242)."""
+
+    #: boundary ids passed to mark_boundary(top, bottom, sides, other)
+    MARKS = (BID_VACUUM_TOP, BID_COPPER_SURFACE, BID_SIDES, BID_COPPER_SURFACE)
+
+    def __init__(self, ctx: fgpu.Context | None = None):
+        self.ctx = ctx or default_context()
+        self.mesh = None
+        self.system = None
+        self.stat = Stat()
+        self.dirichlet_bc_value = 0.0
+        self.precision = fgpu.FGPU_FP64
+        self.relative = False
+        self.inner_tol = 1e-4
+        self.use_graph = True
+        self.last_stats = None
+
+    def import_mesh(self, vertices, cells) -> bool:
+        """DealSolver::import_mesh (src/DealSolver.cpp:192-209): False on a bad mesh."""
+        try:
+            self.mesh = fgpu.Mesh(self.ctx, vertices, cells, *self.MARKS)
+        except fgpu.FgpuError:
+            self.mesh = None
+            return False
+        self.system = fgpu.System(self.mesh)
+        return True
+
+    def size(self) -> int:
+        return self.mesh.n_dofs
+
+    def get_n_cells(self) -> int:
+        return self.mesh.n_cells
+
+    def setup_system(self):
+        """DealSolver::setup_system (src/DealSolver.cpp:369-387): solution = dirichlet value."""
+        self.system.reset()
+        self.system.fill_solution(self.dirichlet_bc_value)
+
+    def export_solution(self) -> np.ndarray:
+        """Solution per mesh vertex (DealSolver::export_solution, src/DealSolver.cpp:270-278)."""
+        return self.system.solution()
+
+    def import_solution(self, sol):
+        self.system.set_solution(sol)
+
+    def export_solution_grad(self) -> np.ndarray:
+        return self.system.solution_grad()
+
+    def check_limits(self, low_limit: float, high_limit: float) -> bool:
+        """DealSolver::check_limits (src/DealSolver.cpp:158-167): True if out of limits."""
+        self.stat.sol_min, self.stat.sol_max = self.system.minmax()
+        return self.stat.sol_min < low_limit or self.stat.sol_max > high_limit
+
+    def solve_cg(self, n_steps: int, tolerance: float, precond: int = fgpu.FGPU_PC_JACOBI) -> int:
+        """DealSolver::solve_cg (src/DealSolver.cpp:443-458): iterations, negative on failure."""
+        o = fgpu.opts(max_it=n_steps, tol=tolerance, relative=self.relative, precond=precond,
+                      precision=self.precision, inner_tol=self.inner_tol, use_graph=self.use_graph)
+        st = self.system.solve(o)
+        self.last_stats = st
+        return st.iterations
+
+
+class PoissonSolver(DealSolver):
+    """PoissonSolver<3> (src/PoissonSolver.cpp)."""
+
+    MARKS = (BID_VACUUM_TOP, BID_COPPER_SURFACE, BID_SIDES, BID_COPPER_SURFACE)
+
+    def __init__(self, particles=None, conf: FieldConfig | None = None, interpolator=None,
+                 ctx: fgpu.Context | None = None):
+        super().__init__(ctx)
+        self.particles = particles
+        self.conf = conf or FieldConfig()
+        self.interpolator = interpolator
+        self.applied_field = 0.0
+        self.applied_potential = 0.0
+
+    def setup(self, field: float, potential: float):
+        """PoissonSolver::setup (src/PoissonSolver.cpp:162-167)."""
+        self.setup_system()
+        self.applied_field = field
+        self.applied_potential = potential
+
+    def assemble(self, full_run: bool = True):
+        """PoissonSolver::assemble (src/PoissonSolver.cpp:170-210) for laplace mode: the
+        matrix is implicit, so full_run only matters for the cached diagonal."""
+        if self.conf.anode_BC not in ("neumann", "dirichlet"):
+            raise ValueError("Unimplemented anode BC: " + self.conf.anode_BC)
+        s = self.system
+        s.reset()
+        s.add_dirichlet(BID_COPPER_SURFACE, self.dirichlet_bc_value)
+        if self.conf.anode_BC == "neumann":
+            s.add_neumann(BID_VACUUM_TOP, None, self.applied_field)
+        else:
+            s.add_dirichlet(BID_VACUUM_TOP, self.applied_potential)
+        if self.conf.mode != "laplace":
+            raise NotImplementedError("space charge (PIC) assembly is outside this path")
+        s.apply_dirichlet()
+
+    def solve(self) -> int:
+        """PoissonSolver::solve (include/PoissonSolver.h:54)."""
+        return self.solve_cg(self.conf.n_cg, self.conf.cg_tolerance)

@@ -1,0 +1,137 @@
+/*
+ * femocs_gpu: MI355X (gfx950) field-solve path of FEMOCS behind a C ABI.
+ *
+ * Plain C types only; no torch, no C++ in the signatures.  Every entry point returns a status
+ * (FGPU_OK or a negative code) and never throws; fgpu_ctx_error() gives the message of the
+ * last failure on a context.  Host arrays are caller-owned and copied; device memory is owned
+ * by the handle that allocated it.  Handles are independent: one context per thread or
+ * per client is re-entrant, calls on one context are serialised on its stream.
+ *
+ * The reference path these entry points replace is the deal.II-based DealSolver family and
+ * the interpolators; each declaration cites the reference function it stands in for.
+ */
+#ifndef FEMOCS_GPU_H
+#define FEMOCS_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FGPU_OK 0
+#define FGPU_EINVAL (-1)
+#define FGPU_EHIP (-2)
+#define FGPU_ENOMEM (-3)
+#define FGPU_ESTATE (-4)
+
+/* boundary ids, include/Globals.h:60-68 */
+#define FGPU_BID_COPPER_SURFACE 2
+#define FGPU_BID_VACUUM_TOP 8
+#define FGPU_BID_COPPER_BOTTOM 7
+#define FGPU_BID_SIDES 4
+
+/* precision of a system's iterations */
+#define FGPU_FP64 0   /* FP64 CG (the reference's arithmetic)                            */
+#define FGPU_MIXED 1  /* FP32 operator and CG inside FP64 defect correction               */
+
+/* preconditioners */
+#define FGPU_PC_NONE 0
+#define FGPU_PC_JACOBI 1
+
+typedef struct fgpu_ctx fgpu_ctx;
+typedef struct fgpu_mesh fgpu_mesh;
+typedef struct fgpu_sys fgpu_sys;
+
+typedef struct {
+    int max_it;        /* max CG iterations (inner + outer for FGPU_MIXED)                    */
+    double tol;        /* stop when |r| <= tol (absolute) or |r|/|b| <= tol (relative)         */
+    int relative;      /* 0: absolute, as SolverControl in src/DealSolver.cpp:444; 1: relative */
+    int precond;       /* FGPU_PC_*                                                           */
+    int precision;     /* FGPU_FP64 or FGPU_MIXED                                             */
+    double inner_tol;  /* FGPU_MIXED: relative reduction asked of each FP32 inner solve        */
+    int check_every;   /* iterations between host convergence checks (0: default)             */
+    int use_graph;     /* capture the iterations between checks in a hipGraph                 */
+} fgpu_solve_opts;
+
+typedef struct {
+    int iterations;       /* CG iterations (negative: not converged), as solve_cg's return   */
+    int outer;            /* FP64 residual evaluations (defect-correction steps)             */
+    double residual;      /* final |b - Ax| (FP64 true residual for FGPU_MIXED)              */
+    double rhs_norm;      /* |b| of the lifted system                                        */
+} fgpu_solve_stats;
+
+/* ---- context ------------------------------------------------------------------------ */
+int fgpu_ctx_create(int device, fgpu_ctx **out);
+void fgpu_ctx_destroy(fgpu_ctx *ctx);
+const char *fgpu_ctx_error(const fgpu_ctx *ctx);
+int fgpu_ctx_sync(fgpu_ctx *ctx);
+
+/* ---- mesh ---------------------------------------------------------------------------
+ * DealSolver::import_mesh(vector<Point<3>>, vector<CellData<3>>) (src/DealSolver.cpp:192-209)
+ * followed by mark_boundary(top, bottom, sides, other) (src/DealSolver.cpp:461-518):
+ * unused vertices are dropped (order kept), UCD cells are converted to lexicographic order,
+ * negatively oriented cells are inverted, boundary faces are marked by the bounding box of
+ * their centres with eps = 1e-6.  xyz: 3*n_vertices, ucd_cells: 8*n_cells. */
+int fgpu_mesh_import(fgpu_ctx *ctx, int n_vertices, const double *xyz, int n_cells,
+                     const int *ucd_cells, int top, int bottom, int sides, int other,
+                     fgpu_mesh **out);
+void fgpu_mesh_destroy(fgpu_mesh *mesh);
+/* DealSolver::size / get_n_cells; n_bfaces: boundary faces */
+int fgpu_mesh_sizes(const fgpu_mesh *mesh, int *n_dofs, int *n_cells, int *n_bfaces);
+/* DealSolver::export_vertices/export_dofs (src/DealSolver.cpp:248-267): dof coordinates
+ * (3*n_dofs) and the input vertex index of every dof; either pointer may be NULL */
+int fgpu_mesh_export_dofs(const fgpu_mesh *mesh, double *xyz, int *vertex_of_dof);
+/* boundary faces in active-cell x face order: cell index, deal.II face 0..5, boundary id */
+int fgpu_mesh_export_bfaces(const fgpu_mesh *mesh, int *cell, int *face, int *bid);
+/* DealSolver::export_surface_centroids (src/DealSolver.cpp:229-245) for a boundary id;
+ * returns the count in *n (xyz may be NULL to query it) */
+int fgpu_mesh_export_face_centroids(const fgpu_mesh *mesh, int bid, double *xyz, int *n);
+/* moved vertices, same topology (3*n_vertices of the import numbering) */
+int fgpu_mesh_update_vertices(fgpu_mesh *mesh, const double *xyz);
+
+/* ---- linear system on a mesh ----------------------------------------------------------
+ * One fgpu_sys = the system_matrix/system_rhs/solution triple of a DealSolver, held on the
+ * GPU matrix-free. */
+int fgpu_sys_create(fgpu_mesh *mesh, fgpu_sys **out);
+void fgpu_sys_destroy(fgpu_sys *sys);
+/* Cell coefficients per quadrature point (8 per cell, lexicographic QGauss(2) points, in the
+ * mesh's cell order): stiffness kappa and mass gamma.  NULL: kappa = 1, gamma = 0
+ * (PoissonSolver::assemble_local_cell src/PoissonSolver.cpp:239-263;
+ *  HeatSolver::assemble_local_cell src/CurrentHeatSolver.cpp:347-393). */
+int fgpu_sys_set_coefficients(fgpu_sys *sys, const double *kappa_q, const double *gamma_q);
+/* system_rhs = 0 and boundary_values.clear() (src/PoissonSolver.cpp:176, DealSolver.h:151) */
+int fgpu_sys_reset(fgpu_sys *sys);
+/* DealSolver::assemble_rhs(bid) (src/DealSolver.cpp:390-430): b_i += int phi_i g dS over the
+ * faces with id bid; g = face_values[k] for the k-th such face in cell x face order, or the
+ * constant when face_values is NULL */
+int fgpu_sys_add_neumann(fgpu_sys *sys, int bid, const double *face_values, double constant);
+/* b_i += int phi_i f over cells, f per quadrature point (8 per cell) */
+int fgpu_sys_add_cell_rhs(fgpu_sys *sys, const double *f_q);
+/* DealSolver::append_dirichlet(bid, value) (src/DealSolver.cpp:433-435) */
+int fgpu_sys_add_dirichlet(fgpu_sys *sys, int bid, double value);
+/* DealSolver::apply_dirichlet (src/DealSolver.cpp:438-440): lifting with column elimination;
+ * sets solution = value on the constrained dofs */
+int fgpu_sys_apply_dirichlet(fgpu_sys *sys);
+/* solution = value everywhere (DealSolver::setup_system, src/DealSolver.cpp:383-384) */
+int fgpu_sys_fill_solution(fgpu_sys *sys, double value);
+/* DealSolver::import_solution / export_solution (src/DealSolver.cpp:270-315), dof order */
+int fgpu_sys_set_solution(fgpu_sys *sys, const double *x);
+int fgpu_sys_get_solution(fgpu_sys *sys, double *x);
+/* lifted right-hand side (dof order), for checks */
+int fgpu_sys_get_rhs(fgpu_sys *sys, double *b);
+/* y = A x with the un-lifted operator (the matrix before apply_dirichlet), FP64 */
+int fgpu_sys_apply(fgpu_sys *sys, const double *x, double *y);
+/* DealSolver::solve_cg (src/DealSolver.cpp:443-458) */
+int fgpu_sys_solve(fgpu_sys *sys, const fgpu_solve_opts *opts, fgpu_solve_stats *stats);
+/* many systems in one set of launches; stats: one per system */
+int fgpu_solve_batch(fgpu_ctx *ctx, int n, fgpu_sys *const *systems, const fgpu_solve_opts *opts,
+                     fgpu_solve_stats *stats);
+/* DealSolver::check_limits / stat (src/DealSolver.cpp:158-167) */
+int fgpu_sys_minmax(fgpu_sys *sys, double *min, double *max);
+/* DealSolver::export_solution_grad (src/DealSolver.cpp:281-301): -grad u at the Gauss point
+ * vertex2node of the last cell touching each dof; 3*n_dofs */
+int fgpu_sys_solution_grad(fgpu_sys *sys, double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

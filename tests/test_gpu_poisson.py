@@ -1,0 +1,74 @@
+"""GPU parity of the Poisson path: matrix-free apply, right-hand side, lifting and the CG
+solution against the CPU oracle on the same seeded meshes."""
+import numpy as np
+import pytest
+
+from femocs_amd import fgpu
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+MARKS = (8, 2, 4, 2)
+
+
+def _pair(ctx, mesh):
+    om = O.Mesh(mesh.nodes, mesh.vacuum_cells(), *MARKS)
+    gm = fgpu.Mesh(ctx, mesh.nodes, mesh.vacuum_cells(), *MARKS)
+    return om, gm
+
+
+def test_mesh_import_matches_oracle(ctx, small_mesh):
+    om, gm = _pair(ctx, small_mesh)
+    assert gm.n_dofs == om.n_dofs and gm.n_cells == om.n_cells
+    x, vod = gm.dofs()
+    assert np.array_equal(vod, om.vertex_of_dof)
+    assert np.array_equal(x, om.x)
+    c, f, b = gm.bfaces()
+    assert np.array_equal(c, om.bface_cell) and np.array_equal(f, om.bface_face)
+    assert np.array_equal(b, om.bface_id)
+
+
+def test_apply_matches_csr(ctx, small_mesh):
+    om, gm = _pair(ctx, small_mesh)
+    om.assemble()
+    s = fgpu.System(gm)
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        u = rng.standard_normal(om.n_dofs)
+        y_ref = om.spmv(u)
+        y = s.apply(u)
+        assert np.linalg.norm(y - y_ref) <= 1e-13 * np.linalg.norm(y_ref)
+
+
+def test_neumann_and_lifting_match_oracle(ctx, small_mesh):
+    om, gm = _pair(ctx, small_mesh)
+    om.assemble()
+    b = om.face_rhs(8, constant=1.0)
+    x = np.zeros(om.n_dofs)
+    mask = om.dirichlet_mask(2)
+    vals = np.where(mask > 0, 0.5, 0.0)
+    om.apply_dirichlet(b, x, mask, vals)
+    s = fgpu.System(gm)
+    s.reset()
+    s.add_dirichlet(2, 0.5)
+    s.add_neumann(8, None, 1.0)
+    s.apply_dirichlet()
+    bg = s.rhs()
+    assert np.linalg.norm(bg - b) <= 1e-13 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("precision", [fgpu.FGPU_FP64, fgpu.FGPU_MIXED])
+def test_poisson_solution_matches_oracle(ctx, small_mesh, precision):
+    om, xo, it, res, _ = O.poisson_laplace(small_mesh.nodes, small_mesh.vacuum_cells(), -1.0,
+                                           tol=1e-13, relative=True, precond="ssor")
+    assert it > 0
+    gm = fgpu.Mesh(ctx, small_mesh.nodes, small_mesh.vacuum_cells(), *MARKS)
+    s = fgpu.System(gm)
+    s.reset()
+    s.add_dirichlet(2, 0.0)
+    s.add_neumann(8, None, -1.0)
+    s.apply_dirichlet()
+    st = s.solve(fgpu.opts(tol=1e-12, relative=True, precision=precision))
+    assert st.iterations > 0
+    x = s.solution()
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    assert err <= 1e-10, err
