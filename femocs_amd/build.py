@@ -17,9 +17,9 @@ LIB = os.path.join(HERE, "libfemocs_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["fgpu_mesh.cpp", "fgpu_kernels.hip", "fgpu_api.hip", "fgpu_interp.hip"]
-HEADERS = ["fgpu_internal.h", "fgpu_kernels.h", "fgpu_interp.h"]
+HEADERS = ["fgpu_internal.h", "fgpu_kernels.h", "fgpu_interp.h", "fgpu_cg.inc"]
 
-# -packed-fp32-ops: no v_pk_*_f32 (hardware notes)
+# -packed-fp32-ops: the compiler emits no packed-FP32 (v_pk_*_f32) instructions
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
          "-I" + os.path.join(ROOT, "include")]

@@ -82,7 +82,7 @@ fgpu::Mesh::~Mesh() {
     dfree(d_X64);
     dfree(d_X32);
     dfree(d_gptr);
-    dfree(d_gidx);
+    dfree(d_spos);
     dfree(d_bf_icell);
     dfree(d_bf_face);
     dfree(d_fgptr);
@@ -95,7 +95,7 @@ fgpu::Sys::~Sys() {
     dfree(b); dfree(x); dfree(r); dfree(z); dfree(p); dfree(q); dfree(Ye); dfree(invd); dfree(diag);
     dfree(g); dfree(tmp); dfree(kq); dfree(gq); dfree(fslot); dfree(fval); dfree(d_dmask);
     dfree(r32); dfree(z32); dfree(p32); dfree(q32); dfree(d32); dfree(Ye32); dfree(invd32);
-    dfree(kq32); dfree(gq32); dfree(scal); dfree(part); dfree(cnt);
+    dfree(kq32); dfree(gq32); dfree(scal); dfree(part);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -124,8 +124,7 @@ extern "C" int fgpu_ctx_create(int device, fgpu_ctx **out) {
     return FGPU_OK;
 }
 
-extern "C" void fgpu_ctx_destroy(fgpu_ctx *c) {
-    if (!c) return;
+static void ctx_free(fgpu_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) {
         (void)hipStreamSynchronize(c->stream);
@@ -134,6 +133,21 @@ extern "C" void fgpu_ctx_destroy(fgpu_ctx *c) {
     if (c->d_tab) (void)hipFree(c->d_tab);
     if (c->h_flags) (void)hipHostFree(c->h_flags);
     delete c;
+}
+
+// A context outlives its meshes and systems: destroying it while they are alive only marks
+// it, and the last child to go frees it.  Same for a mesh and its systems.
+extern "C" void fgpu_ctx_destroy(fgpu_ctx *c) {
+    if (!c) return;
+    if (c->refs > 0) {
+        c->closing = true;
+        return;
+    }
+    ctx_free(c);
+}
+
+static void ctx_release(fgpu_ctx *c) {
+    if (--c->refs == 0 && c->closing) ctx_free(c);
 }
 
 extern "C" const char *fgpu_ctx_error(const fgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
@@ -173,9 +187,9 @@ extern "C" int fgpu_mesh_import(fgpu_ctx *ctx, int nv, const double *xyz, int nc
             ctx->err = "mesh import: empty mesh";
             return FGPU_EINVAL;
         }
-        std::vector<int> icells, gptr, gidx, fgptr, fgidx, bf_icell;
+        std::vector<int> icells, gptr, spos, fgptr, fgidx, bf_icell;
         std::vector<double> X;
-        build_internal_order(*m, icells, X, gptr, gidx, fgptr, fgidx, bf_icell);
+        build_internal_order(*m, icells, X, gptr, spos, fgptr, fgidx, bf_icell);
         hipStream_t s = ctx->stream;
         m->d_cells = dalloc<int>(icells.size());
         h2d(m->d_cells, icells.data(), icells.size(), s);
@@ -183,8 +197,8 @@ extern "C" int fgpu_mesh_import(fgpu_ctx *ctx, int nv, const double *xyz, int nc
         m->d_X32 = dalloc<float>(X.size());
         m->d_gptr = dalloc<int>(gptr.size());
         h2d(m->d_gptr, gptr.data(), gptr.size(), s);
-        m->d_gidx = dalloc<int>(gidx.size());
-        h2d(m->d_gidx, gidx.data(), gidx.size(), s);
+        m->d_spos = dalloc<int>(spos.size());
+        h2d(m->d_spos, spos.data(), spos.size(), s);
         const size_t nb = m->bf_cell.size();
         m->d_bf_icell = dalloc<int>(nb);
         h2d(m->d_bf_icell, bf_icell.data(), nb, s);
@@ -212,15 +226,26 @@ extern "C" int fgpu_mesh_import(fgpu_ctx *ctx, int nv, const double *xyz, int nc
         delete m;
         return rc;
     }
+    ctx->refs++;
     *out = m;
     return FGPU_OK;
 }
 
+static void mesh_free(fgpu_mesh *m) {
+    fgpu_ctx *c = m->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    delete m;
+    ctx_release(c);
+}
+
 extern "C" void fgpu_mesh_destroy(fgpu_mesh *m) {
     if (!m) return;
-    (void)hipSetDevice(m->ctx->device);
-    (void)hipStreamSynchronize(m->ctx->stream);
-    delete m;
+    if (m->refs > 0) {
+        m->closing = true;
+        return;
+    }
+    mesh_free(m);
 }
 
 extern "C" int fgpu_mesh_sizes(const fgpu_mesh *m, int *nd, int *nc, int *nb) {
@@ -292,7 +317,7 @@ static Desc make_desc64(fgpu_sys *s, bool residual) {
     d.cells = m->d_cells;
     d.X = m->d_X64;
     d.gptr = m->d_gptr;
-    d.gidx = m->d_gidx;
+    d.spos = m->d_spos;
     d.dmask = s->d_dmask;
     d.kq = s->have_coef ? s->kq : nullptr;
     d.gq = s->have_coef ? s->gq : nullptr;
@@ -309,7 +334,6 @@ static Desc make_desc64(fgpu_sys *s, bool residual) {
     d.b = s->b;
     d.scal = s->scal;
     d.part = s->part;
-    d.cnt = s->cnt;
     return d;
 }
 
@@ -323,7 +347,7 @@ static Desc make_desc32(fgpu_sys *s) {
     d.cells = m->d_cells;
     d.X = m->d_X32;
     d.gptr = m->d_gptr;
-    d.gidx = m->d_gidx;
+    d.spos = m->d_spos;
     d.dmask = s->d_dmask;
     d.kq = s->have_coef ? s->kq32 : nullptr;
     d.gq = s->have_coef ? s->gq32 : nullptr;
@@ -339,8 +363,7 @@ static Desc make_desc32(fgpu_sys *s) {
     d.invdiag = s->invd32;
     d.b = nullptr;
     d.scal = s->scal + kScal;
-    d.part = s->part + 6 * (size_t)s->nb_dof;
-    d.cnt = s->cnt + 8;
+    d.part = s->part + 4 * (size_t)s->nb_dof;
     return d;
 }
 
@@ -364,10 +387,8 @@ extern "C" int fgpu_sys_create(fgpu_mesh *m, fgpu_sys **out) {
         s->r32 = dalloc<float>(n); s->z32 = dalloc<float>(n); s->p32 = dalloc<float>(n); s->q32 = dalloc<float>(n);
         s->d32 = dalloc<float>(n); s->Ye32 = dalloc<float>(8 * nc); s->invd32 = dalloc<float>(n);
         s->scal = dalloc<double>(2 * kScal);
-        s->part = dalloc<double>(12 * (size_t)s->nb_dof);
-        s->cnt = dalloc<unsigned int>(16);
+        s->part = dalloc<double>(8 * (size_t)s->nb_dof);
         hipStream_t st = m->ctx->stream;
-        HIPCHECK(hipMemsetAsync(s->cnt, 0, 16 * sizeof(unsigned int), st));
         HIPCHECK(hipMemsetAsync(s->scal, 0, 2 * kScal * sizeof(double), st));
         HIPCHECK(hipMemsetAsync(s->b, 0, n * sizeof(double), st));
         HIPCHECK(hipMemsetAsync(s->x, 0, n * sizeof(double), st));
@@ -382,15 +403,21 @@ extern "C" int fgpu_sys_create(fgpu_mesh *m, fgpu_sys **out) {
         delete s;
         return rc;
     }
+    m->refs++;
+    m->ctx->refs++;
     *out = s;
     return FGPU_OK;
 }
 
 extern "C" void fgpu_sys_destroy(fgpu_sys *s) {
     if (!s) return;
-    (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->ctx->stream);
+    fgpu_ctx *c = s->ctx;
+    fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     delete s;
+    if (--m->refs == 0 && m->closing) mesh_free(m);
+    ctx_release(c);
 }
 
 // single-system tables for setup launches: [Desc][elem tab][dof tab] in a scratch buffer
@@ -588,8 +615,8 @@ extern "C" int fgpu_sys_apply_dirichlet(fgpu_sys *s) {
         }
         h2d(s->d_dmask, mk.data(), n, st);
         h2d(s->g, gv.data(), n, st);
-        s->diag_valid = false;
         ensure_diag(s);
+        launch_invdiag(st, n, s->d_dmask, s->diag, s->invd, s->invd32);
         // A g with the un-lifted operator
         Desc d = make_desc64(s, true);
         d.ua = s->g;
@@ -711,63 +738,70 @@ extern "C" int fgpu_sys_solution_grad(fgpu_sys *s, double *grad) {
 // ------------------------------------------------------------------------------------------
 // batched solve
 // ------------------------------------------------------------------------------------------
-__global__ void k_collect(const Desc *D, int n, double *out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = D[i].scal[S_ACTIVE];
-}
-
 static double *flags_buffer(fgpu_ctx *c, size_t n) {
     if (n > c->flags_cap) {
         if (c->h_flags) HIPCHECK(hipHostFree(c->h_flags));
         c->h_flags = nullptr;
-        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->h_flags), 2 * n * sizeof(double), hipHostMallocDefault));
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&c->h_flags), n * sizeof(double), hipHostMallocDefault));
         c->flags_cap = n;
     }
     return c->h_flags;
 }
 
-// true while any system of the set is active
-static bool any_active(fgpu_ctx *c, const Desc *D, int n, double *dflags) {
+// true while some system of the set has scalar `slot` equal to `want`
+static bool any_flag(fgpu_ctx *c, const Desc *D, int n, int slot, double want, double *dflags) {
     double *h = flags_buffer(c, n);
-    hipLaunchKernelGGL(k_collect, dim3((n + 255) / 256), dim3(256), 0, c->stream, D, n, dflags);
+    launch_collect(c->stream, D, n, slot, dflags);
     HIPCHECK(hipMemcpyAsync(h, dflags, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     for (int i = 0; i < n; ++i)
-        if (h[i] != 0.0) return true;
+        if (h[i] == want) return true;
     return false;
 }
 
+// one CG iteration of every system of the set
 template <class T>
-static void cg_chunk(hipStream_t st, const Desc *D, const BatchTabs &t, int k, bool coef) {
-    for (int j = 0; j < k; ++j) {
-        launch_elem<T>(st, D, t.etab, t.ne, coef, false, true, S_BETA);
-        launch_gather_cg<T>(st, D, t.dtab, t.nd);
-        launch_update_cg<T>(st, D, t.dtab, t.nd);
-    }
+static void cg_iter(hipStream_t st, const Desc *D, const BatchTabs &t, int n, bool coef, int reliable) {
+    launch_elem<T>(st, D, t.etab, t.ne, coef, false, true, S_BETA);
+    launch_gather_cg<T>(st, D, t.dtab, t.nd);
+    launch_scal_alpha(st, D, n);
+    launch_update_cg<T>(st, D, t.dtab, t.nd);
+    launch_scal_beta(st, D, n, reliable);
 }
 
-// runs CG chunks until every system of the set is inactive
-template <class T>
-static void run_cg(fgpu_ctx *c, const Desc *D, int n, const BatchTabs &t, bool coef, int k, bool use_graph, int max_chunks,
-                   double *dflags) {
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    for (int chunk = 0; chunk < max_chunks; ++chunk) {
-        if (use_graph) {
-            if (!exec) {
-                HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                cg_chunk<T>(c->stream, D, t, k, coef);
-                HIPCHECK(hipStreamEndCapture(c->stream, &graph));
-                HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            }
-            HIPCHECK(hipGraphLaunch(exec, c->stream));
+// FP64 residual of the lifted system (R64: residual descriptors, u = x with the constrained
+// dofs zeroed) and the resulting state
+static void fp64_residual(hipStream_t st, const Desc *R64, const Desc *D32, const BatchTabs &t, int n, bool coef, int mixed) {
+    launch_elem<double>(st, R64, t.etab, t.ne, coef, true, true, S_BETA);
+    launch_gather_res<double>(st, R64, t.dtab, t.nd);
+    launch_scal_res(st, R64, D32, n, mixed);
+}
+
+// reliable update of the systems whose FP32 CG waits: fold the FP32 correction into x64,
+// recompute the FP64 residual, resume the FP32 CG from it
+static void reliable_update(hipStream_t st, const Desc *D32, const Desc *R64, const BatchTabs &t, int n, bool coef) {
+    launch_mixed_acc(st, D32, R64, t.dtab, t.nd);
+    fp64_residual(st, R64, D32, t, n, coef, 1);
+    launch_mixed_restart(st, D32, R64, t.dtab, t.nd, 0);
+    launch_scal_restart(st, D32, R64, n, 0);
+}
+
+struct ChunkArgs {
+    bool mixed, coef;
+    int k, u, n;
+    const Desc *D64, *R64, *D32;
+    const BatchTabs *t;
+};
+
+static void chunk(hipStream_t st, const ChunkArgs &a) {
+    for (int j = 0; j < a.k; ++j) {
+        if (!a.mixed) {
+            cg_iter<double>(st, a.D64, *a.t, a.n, a.coef, 0);
         } else {
-            cg_chunk<T>(c->stream, D, t, k, coef);
+            cg_iter<float>(st, a.D32, *a.t, a.n, a.coef, 1);
+            if ((j + 1) % a.u == 0) reliable_update(st, a.D32, a.R64, *a.t, a.n, a.coef);
         }
-        if (!any_active(c, D, n, dflags)) break;
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
 }
 
 static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_solve_opts *o, fgpu_solve_stats *stats) {
@@ -781,18 +815,24 @@ static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_
     for (auto *s : sys)
         if (s->have_coef != coef) throw std::invalid_argument("a batch mixes coefficient and plain systems");
     const bool mixed = o->precision == FGPU_MIXED;
-    const int k = o->check_every > 0 ? o->check_every : 32;
+    const int k = std::max(2, o->check_every > 0 ? o->check_every : 32);
     const int max_it = std::max(0, o->max_it);
-    // initial scalars
-    for (auto *s : sys) {
+    const double delta = o->inner_tol > 0 && o->inner_tol < 1 ? o->inner_tol : 0.1;
+    // initial scalars: [FP64 | FP32] per system
+    std::vector<double> h((size_t)2 * kScal * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double *s64 = &h[(size_t)2 * kScal * i], *s32 = s64 + kScal;
+        s64[S_ACTIVE] = 1.0;
+        s64[S_MAXIT] = max_it;
+        s64[S_TOL] = o->tol;
+        s64[S_REL] = o->relative ? 1.0 : 0.0;
+        s32[S_MAXIT] = max_it;
+        s32[S_DELTA] = delta * delta;
+    }
+    for (int i = 0; i < n; ++i) {
+        fgpu_sys *s = sys[i];
         ensure_diag(s);
-        double h[2 * kScal] = {0};
-        h[S_ACTIVE] = 1.0;
-        h[S_MAXIT] = max_it;
-        h[S_TOL] = o->tol;
-        h[S_REL] = o->relative ? 1.0 : 0.0;
-        h[kScal + S_ACTIVE] = 0.0;
-        HIPCHECK(hipMemcpyAsync(s->scal, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+        h2d(s->scal, &h[(size_t)2 * kScal * i], 2 * kScal, c->stream);
         fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
         launch_enforce_dirichlet(c->stream, m->n_dofs, s->d_dmask, s->g, s->x);
     }
@@ -802,38 +842,52 @@ static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_
         res64.push_back(make_desc64(s, true));
         cg32.push_back(make_desc32(s));
     }
-    BatchTabs t = upload_batch(c, {cg64, res64, cg32}, sys);
+    BatchTabs t = upload_batch(c, {cg64, res64, cg32}, sys);  // synchronises: h is consumed
     double *dflags = nullptr;
     HIPCHECK(hipMalloc(&dflags, std::max(1, n) * sizeof(double)));
     const Desc *D64 = t.sets[0], *R64 = t.sets[1], *D32 = t.sets[2];
-    // initial residual of the lifted system
-    launch_elem<double>(c->stream, R64, t.etab, t.ne, coef, true, true, S_BETA);
-    launch_gather_res<double>(c->stream, R64, t.dtab, t.nd);
-    const int max_chunks = max_it / k + 2;
-    if (!mixed) {
-        run_cg<double>(c, D64, n, t, coef, k, o->use_graph != 0, max_chunks, dflags);
-    } else {
-        const double eta = o->inner_tol > 0 ? o->inner_tol : 1e-4;
-        for (int outer = 0; outer < 1000; ++outer) {
-            if (!any_active(c, R64, n, dflags)) break;
-            launch_mixed_init(c->stream, D32, R64, t.dtab, t.nd, eta);
-            run_cg<float>(c, D32, n, t, coef, k, o->use_graph != 0, max_chunks, dflags);
-            launch_mixed_acc(c->stream, D32, R64, t.dtab, t.nd);
-            launch_elem<double>(c->stream, R64, t.etab, t.ne, coef, true, true, S_BETA);
-            launch_gather_res<double>(c->stream, R64, t.dtab, t.nd);
-        }
+    fp64_residual(c->stream, R64, D32, t, n, coef, mixed ? 1 : 0);
+    if (mixed) {
+        launch_mixed_restart(c->stream, D32, R64, t.dtab, t.nd, 1);
+        launch_scal_restart(c->stream, D32, R64, n, 1);
     }
+    ChunkArgs a{mixed, coef, k, 8, n, D64, R64, D32, &t};
+    // a system waiting for its reliable update idles at most u iterations per update
+    const int max_chunks = 2 * (max_it / k) + 16;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    const int slot = mixed ? S_DONE : S_ACTIVE;
+    const double want = mixed ? 0.0 : 1.0;
+    const Desc *Dflag = mixed ? R64 : D64;
+    bool running = any_flag(c, Dflag, n, slot, want, dflags);
+    for (int ch = 0; running && ch < max_chunks; ++ch) {
+        if (o->use_graph) {
+            if (!exec) {
+                HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                chunk(c->stream, a);
+                HIPCHECK(hipStreamEndCapture(c->stream, &graph));
+                HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            }
+            HIPCHECK(hipGraphLaunch(exec, c->stream));
+        } else {
+            chunk(c->stream, a);
+        }
+        running = any_flag(c, Dflag, n, slot, want, dflags);
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     HIPCHECK(hipStreamSynchronize(c->stream));
     HIPCHECK(hipFree(dflags));
     for (int i = 0; i < n; ++i) {
-        double h[kScal];
-        HIPCHECK(hipMemcpy(h, sys[i]->scal, sizeof(h), hipMemcpyDeviceToHost));
+        double hs[2 * kScal];
+        HIPCHECK(hipMemcpy(hs, sys[i]->scal, sizeof(hs), hipMemcpyDeviceToHost));
+        const double *s64 = hs, *s32 = hs + kScal;
         if (stats) {
-            const int it = (int)h[S_IT];
-            stats[i].iterations = h[S_CONV] != 0.0 ? it : -std::max(it, 1);
-            stats[i].outer = mixed ? (int)h[S_PQ] + 1 : 1;
-            stats[i].residual = std::sqrt(std::max(0.0, h[S_RR]));
-            stats[i].rhs_norm = std::sqrt(std::max(0.0, h[S_BB]));
+            const int it = (int)(mixed ? s32[S_IT] : s64[S_IT]);
+            stats[i].iterations = s64[S_CONV] != 0.0 ? it : -std::max(it, 1);
+            stats[i].outer = mixed ? (int)s32[S_NUPD] + 1 : 1;
+            stats[i].residual = std::sqrt(std::max(0.0, s64[S_RR]));
+            stats[i].rhs_norm = std::sqrt(std::max(0.0, s64[S_BB]));
         }
     }
     return FGPU_OK;
@@ -847,4 +901,43 @@ extern "C" int fgpu_solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, co
 extern "C" int fgpu_sys_solve(fgpu_sys *s, const fgpu_solve_opts *o, fgpu_solve_stats *stats) {
     if (!s || !o) return FGPU_EINVAL;
     return guarded(s->ctx, [&] { return solve_batch(s->ctx, 1, &s, o, stats); });
+}
+
+// ------------------------------------------------------------------------------------------
+// kernel timing (benchmark helper): average duration of `reps` back-to-back launches of one
+// kernel of the solve, measured with HIP events on the context stream
+// ------------------------------------------------------------------------------------------
+extern "C" int fgpu_sys_time_kernel(fgpu_sys *s, int which, int precision, int reps, double *avg_ms) {
+    if (!s || !avg_ms || reps <= 0) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_ctx *c = s->ctx;
+        HIPCHECK(hipSetDevice(c->device));
+        ensure_diag(s);
+        Desc d = precision == FGPU_MIXED ? make_desc32(s) : make_desc64(s, true);
+        // inputs: the current z / x vectors; outputs: the slot buffer and q
+        BatchTabs t = upload_batch(c, {{d}}, {s});
+        auto one = [&] {
+            if (which == 0) {
+                if (precision == FGPU_MIXED) launch_elem<float>(c->stream, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA);
+                else launch_elem<double>(c->stream, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA);
+            } else {
+                if (precision == FGPU_MIXED) throw std::invalid_argument("gather timing is FP64 only");
+                launch_gather_plain<double>(c->stream, t.sets[0], t.dtab, t.nd, false);
+            }
+        };
+        one();
+        hipEvent_t e0, e1;
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        HIPCHECK(hipEventRecord(e0, c->stream));
+        for (int i = 0; i < reps; ++i) one();
+        HIPCHECK(hipEventRecord(e1, c->stream));
+        HIPCHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_ms = ms / reps;
+        return FGPU_OK;
+    });
 }

@@ -13,23 +13,29 @@
 namespace fgpu {
 
 constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
-constexpr int kScal = 16;     // doubles of per-system scalars
+constexpr int kScal = 24;     // doubles of per-system scalars
 
 // indices into the per-system scalar block
 enum Scal : int {
     S_RZ = 0,      // r.z of the current iterate
-    S_PQ = 1,      // p.q
-    S_ALPHA = 2,
-    S_BETA = 3,
-    S_RR = 4,      // |r|^2
-    S_THR2 = 5,    // stop threshold on |r|^2
-    S_IT = 6,      // iterations done
-    S_ACTIVE = 7,  // 1 while iterating
-    S_MAXIT = 8,
-    S_BB = 9,      // |b|^2
-    S_CONV = 10,   // 1 once converged
-    S_TOL = 11,    // tolerance (absolute or relative, see S_REL)
-    S_REL = 12,
+    S_RZOLD = 1,   // r.z of the previous iterate (beta denominator)
+    S_PQ = 2,      // p.q
+    S_ALPHA = 3,
+    S_BETA = 4,
+    S_RR = 5,      // |r|^2
+    S_THR2 = 6,    // stop threshold on |r|^2
+    S_IT = 7,      // iterations done
+    S_ACTIVE = 8,  // CG: 1 while iterating; FP64 check of the mixed solver: residual requested
+    S_MAXIT = 9,
+    S_BB = 10,     // |b|^2
+    S_CONV = 11,   // 1 once converged
+    S_TOL = 12,    // tolerance (absolute, or relative to |b| when S_REL != 0)
+    S_REL = 13,
+    S_RRREF = 14,  // |r|^2 at the last reliable update (mixed precision)
+    S_PEND = 15,   // FP32 CG waits for a reliable update
+    S_NUPD = 16,   // reliable updates done
+    S_DELTA = 17,  // reliable update when |r|^2 < S_DELTA * S_RRREF
+    S_DONE = 18,   // solve finished (converged or out of iterations)
 };
 
 // Device descriptor of one system for one launch family.  Pointers are typed by the
@@ -38,11 +44,12 @@ struct Desc {
     int n_dofs, n_cells, nb_cell, nb_dof;
     const int *cells;             // 8 per cell, lexicographic, internal dof ids
     const void *X;                // T4 coordinates per dof
-    const int *gptr, *gidx;       // dof -> element slots (cell*8+k), ascending
+    const int *gptr;              // dof i owns slot-buffer positions [gptr[i], gptr[i+1])
+    const int *spos;              // 8 per cell: slot-buffer position of each cell output
     const unsigned char *dmask;   // 1 on constrained dofs
     const void *kq, *gq;          // T, 8 per cell (stiffness / mass coefficient) or null
     const void *ua, *ub;          // T: element input u = ua + beta*ub (ub may be null)
-    void *Ye;                     // T: 8 per cell
+    void *Ye;                     // T: element outputs, dof-major (see gptr/spos)
     void *q;                      // T: gathered A u
     void *p;                      // T: search direction (written by the CG gather)
     const void *z;                // T: preconditioned residual
@@ -50,12 +57,13 @@ struct Desc {
     const void *invdiag;          // T: 1/diag, 0 on constrained dofs
     const double *b;              // FP64 rhs (residual mode)
     double *scal;                 // kScal doubles
-    double *part;                 // 4*nb_dof partial sums
-    unsigned int *cnt;            // last-block counters
+    double *part;                 // 4 regions of nb_dof per-block partial sums
 };
 
 struct Mesh {
     fgpu_ctx *ctx = nullptr;
+    int refs = 0;              // live systems on this mesh
+    bool closing = false;      // destroy requested while systems were alive
     int n_vertices = 0, n_dofs = 0, n_cells = 0;
     // host topology in the API numbering
     std::vector<int> vertex_of_dof;        // n_dofs
@@ -71,7 +79,7 @@ struct Mesh {
     int *d_cells = nullptr;                // 8*n_cells
     double *d_X64 = nullptr;               // 4*n_dofs
     float *d_X32 = nullptr;                // 4*n_dofs
-    int *d_gptr = nullptr, *d_gidx = nullptr;
+    int *d_gptr = nullptr, *d_spos = nullptr;
     int *d_bf_icell = nullptr, *d_bf_face = nullptr;       // boundary faces (API order)
     int *d_fgptr = nullptr, *d_fgidx = nullptr;            // dof -> boundary-face slots
     int *d_v2c = nullptr, *d_v2n = nullptr;                 // gradient export (internal)
@@ -97,7 +105,6 @@ struct Sys {
     // scalars and reduction scratch: [0] FP64 CG / outer, [1] FP32 inner
     double *scal = nullptr;       // 2*kScal
     double *part = nullptr;       // 2 * 4*nb_dof
-    unsigned int *cnt = nullptr;  // 2 * 8
     int nb_cell = 0, nb_dof = 0;
     ~Sys();
 };
@@ -113,6 +120,8 @@ struct fgpu_ctx {
     size_t tab_bytes = 0;
     double *h_flags = nullptr;    // pinned, per system
     size_t flags_cap = 0;
+    int refs = 0;                 // live meshes and systems
+    bool closing = false;         // destroy requested while children were alive
 };
 
 struct fgpu_mesh : fgpu::Mesh {};
@@ -124,6 +133,6 @@ int build_topology(Mesh &m, int nv, const double *xyz, int nc, const int *ucd, i
                    int sides, int other, std::string &err);
 // internal order and gather lists
 void build_internal_order(Mesh &m, std::vector<int> &icells, std::vector<double> &X,
-                          std::vector<int> &gptr, std::vector<int> &gidx, std::vector<int> &fgptr,
+                          std::vector<int> &gptr, std::vector<int> &spos, std::vector<int> &fgptr,
                           std::vector<int> &fgidx, std::vector<int> &bf_icell);
 }  // namespace fgpu

@@ -12,8 +12,13 @@ template <class T> void launch_gather_cg(hipStream_t s, const Desc *D, const int
 template <class T> void launch_update_cg(hipStream_t s, const Desc *D, const int2 *tab, int nblk);
 template <class T> void launch_gather_res(hipStream_t s, const Desc *D, const int2 *tab, int nblk);
 template <class T> void launch_gather_plain(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool accum);
-void launch_mixed_init(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk, double inner_tol);
+void launch_scal_alpha(hipStream_t s, const Desc *D, int n);
+void launch_scal_beta(hipStream_t s, const Desc *D, int n, int reliable);
+void launch_scal_res(hipStream_t s, const Desc *D, const Desc *D32, int n, int mixed);
+void launch_scal_restart(hipStream_t s, const Desc *D32, const Desc *D64, int n, int first);
+void launch_mixed_restart(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk, int first);
 void launch_mixed_acc(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk);
+void launch_collect(hipStream_t s, const Desc *D, int n, int slot, double *out);
 void launch_enforce_dirichlet(hipStream_t s, int n, const unsigned char *mask, const double *g, double *x);
 void launch_lift(hipStream_t s, int n, const unsigned char *mask, const double *g, const double *Ag, const double *diag,
                  double *b, double *x);

@@ -201,36 +201,28 @@ template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     }
 }
 
-// Stores this block's partials and reports whether it is the last block of the system to do
-// so.  The last block then sums all partials in block order (fixed, so deterministic).
-template <int NV>
-__device__ __forceinline__ bool last_block_sum(double (&v)[NV], double *part, unsigned int *cnt, int nb, int lb) {
-    __shared__ bool is_last;
-    block_sum<NV>(v);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) part[i * nb + lb] = v[i];
-        __threadfence();
-        const unsigned prev = atomicAdd(cnt, 1u);
-        is_last = prev == (unsigned)(nb - 1);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        acc[i] = 0;
-        for (int k = threadIdx.x; k < nb; k += kBlock)
-            acc[i] += __hip_atomic_load(part + i * nb + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<NV>(acc);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) v[i] = acc[i];
-        __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return true;
+// the dof's element outputs sit contiguously in the slot buffer, in ascending cell order
+template <class T> __device__ __forceinline__ T gather_slots(const Desc &d, int i) {
+    const T *Ye = reinterpret_cast<const T *>(d.Ye);
+    const int e = d.gptr[i + 1];
+    T s = 0;
+    for (int k = d.gptr[i]; k < e; ++k) s += Ye[k];
+    return s;
+}
+
+// stores the 8 outputs of cell c at their dof-major positions
+template <class T> __device__ __forceinline__ void store_slots(const Desc &d, int c, const T y[8]) {
+    const int4 *sp = reinterpret_cast<const int4 *>(d.spos) + 2 * (size_t)c;
+    const int4 s0 = sp[0], s1 = sp[1];
+    T *Ye = reinterpret_cast<T *>(d.Ye);
+    Ye[s0.x] = y[0];
+    Ye[s0.y] = y[1];
+    Ye[s0.z] = y[2];
+    Ye[s0.w] = y[3];
+    Ye[s1.x] = y[4];
+    Ye[s1.y] = y[5];
+    Ye[s1.z] = y[6];
+    Ye[s1.w] = y[7];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -276,9 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_elem(const Desc *__restrict__ D, con
     }
     T y[8];
     q1_cell<T, COEF>(Xe, u, kq, gq, y);
-    T *Ye = reinterpret_cast<T *>(d.Ye) + 8 * (size_t)c;
-#pragma unroll
-    for (int n = 0; n < 8; ++n) Ye[n] = y[n];
+    store_slots<T>(d, c, y);
 }
 
 // element diagonals (unit vectors through the same element kernel)
@@ -305,13 +295,14 @@ __global__ __launch_bounds__(kBlock) void k_elem_diag(const Desc *__restrict__ D
             gq[q] = reinterpret_cast<const T *>(d.gq)[8 * (size_t)c + q];
         }
     }
-    T *Ye = reinterpret_cast<T *>(d.Ye) + 8 * (size_t)c;
+    T dg[8];
     for (int n = 0; n < 8; ++n) {
         T u[8], y[8];
         for (int m = 0; m < 8; ++m) u[m] = m == n ? (T)1 : (T)0;
         q1_cell<T, COEF>(Xe, u, kq, gq, y);
-        Ye[n] = y[n];
+        dg[n] = y[n];
     }
+    store_slots<T>(d, c, dg);
 }
 
 // cell right-hand side: 8 contributions int phi_n f, f per Gauss point (FP64)
@@ -348,180 +339,10 @@ __global__ __launch_bounds__(kBlock) void k_elem_rhs(const Desc *__restrict__ D,
             y[n] += phi * jxw;
         }
     }
-    double *Ye = reinterpret_cast<double *>(d.Ye) + 8 * (size_t)c;
-    for (int n = 0; n < 8; ++n) Ye[n] = y[n];
+    store_slots<double>(d, c, y);
 }
 
-// ---------------------------------------------------------------------------------------
-// per-dof kernels
-// ---------------------------------------------------------------------------------------
-template <class T> __device__ __forceinline__ T gather_slots(const Desc &d, int i) {
-    const T *Ye = reinterpret_cast<const T *>(d.Ye);
-    const int e = d.gptr[i + 1];
-    T s = 0;
-    for (int k = d.gptr[i]; k < e; ++k) s += Ye[d.gidx[k]];
-    return s;
-}
-
-// CG step part 1: q = A p (masked rows), p = z + beta p, p.q -> alpha
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_gather_cg(const Desc *__restrict__ D, const int2 *__restrict__ tab) {
-    const int2 t = tab[blockIdx.x];
-    const Desc &d = D[t.x];
-    if (d.scal[S_ACTIVE] == 0.0) return;
-    const int i = t.y * kBlock + threadIdx.x;
-    double v[1] = {0};
-    if (i < d.n_dofs) {
-        T q = gather_slots<T>(d, i);
-        if (d.dmask[i]) q = 0;
-        T *p = reinterpret_cast<T *>(d.p);
-        const T pn = reinterpret_cast<const T *>(d.z)[i] + (T)d.scal[S_BETA] * p[i];
-        p[i] = pn;
-        reinterpret_cast<T *>(d.q)[i] = q;
-        v[0] = (double)pn * (double)q;
-    }
-    if (last_block_sum<1>(v, d.part, d.cnt, d.nb_dof, t.y) && threadIdx.x == 0) {
-        d.scal[S_PQ] = v[0];
-        d.scal[S_ALPHA] = v[0] != 0.0 ? d.scal[S_RZ] / v[0] : 0.0;
-    }
-}
-
-// CG step part 2: x += alpha p, r -= alpha q, z = r / diag; r.z, r.r -> beta, convergence
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_update_cg(const Desc *__restrict__ D, const int2 *__restrict__ tab) {
-    const int2 t = tab[blockIdx.x];
-    const Desc &d = D[t.x];
-    if (d.scal[S_ACTIVE] == 0.0) return;
-    const int i = t.y * kBlock + threadIdx.x;
-    double v[2] = {0, 0};
-    if (i < d.n_dofs) {
-        const T alpha = (T)d.scal[S_ALPHA];
-        T *x = reinterpret_cast<T *>(d.x);
-        T *r = reinterpret_cast<T *>(d.r);
-        const T pi = reinterpret_cast<const T *>(d.p)[i];
-        const T qi = reinterpret_cast<const T *>(d.q)[i];
-        x[i] += alpha * pi;
-        const T ri = r[i] - alpha * qi;
-        r[i] = ri;
-        const T zi = ri * reinterpret_cast<const T *>(d.invdiag)[i];
-        reinterpret_cast<T *>(d.zw)[i] = zi;
-        v[0] = (double)ri * (double)zi;
-        v[1] = (double)ri * (double)ri;
-    }
-    if (last_block_sum<2>(v, d.part + d.nb_dof, d.cnt + 1, d.nb_dof, t.y) && threadIdx.x == 0) {
-        double *s = d.scal;
-        const double rz_old = s[S_RZ];
-        s[S_BETA] = rz_old != 0.0 ? v[0] / rz_old : 0.0;
-        s[S_RZ] = v[0];
-        s[S_RR] = v[1];
-        s[S_IT] += 1.0;
-        const bool conv = v[1] <= s[S_THR2];
-        s[S_CONV] = conv ? 1.0 : 0.0;
-        s[S_ACTIVE] = (!conv && s[S_IT] < s[S_MAXIT]) ? 1.0 : 0.0;
-    }
-}
-
-// initial residual of the lifted system: r = b - A x (masked rows), z = r / diag, p = 0;
-// r.z, r.r, b.b -> scalars; activates the system unless already converged
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_gather_res(const Desc *__restrict__ D, const int2 *__restrict__ tab) {
-    const int2 t = tab[blockIdx.x];
-    const Desc &d = D[t.x];
-    if (d.scal[S_ACTIVE] == 0.0) return;
-    const int i = t.y * kBlock + threadIdx.x;
-    double v[3] = {0, 0, 0};
-    if (i < d.n_dofs) {
-        const double bi = d.b[i];
-        T r = d.dmask[i] ? (T)0 : (T)(bi - (double)gather_slots<T>(d, i));
-        const T z = r * reinterpret_cast<const T *>(d.invdiag)[i];
-        reinterpret_cast<T *>(d.r)[i] = r;
-        reinterpret_cast<T *>(d.zw)[i] = z;
-        reinterpret_cast<T *>(d.p)[i] = 0;
-        v[0] = (double)r * (double)z;
-        v[1] = (double)r * (double)r;
-        v[2] = bi * bi;
-    }
-    if (last_block_sum<3>(v, d.part + 2 * d.nb_dof, d.cnt + 2, d.nb_dof, t.y) && threadIdx.x == 0) {
-        double *s = d.scal;
-        s[S_RZ] = v[0];
-        s[S_RR] = v[1];
-        s[S_BB] = v[2];
-        s[S_BETA] = 0.0;
-        const double thr = s[S_REL] != 0.0 ? s[S_TOL] * sqrt(v[2]) : s[S_TOL];
-        s[S_THR2] = thr * thr;
-        const bool conv = v[1] <= s[S_THR2];
-        s[S_CONV] = conv ? 1.0 : 0.0;
-        s[S_ACTIVE] = (!conv && s[S_IT] < s[S_MAXIT]) ? 1.0 : 0.0;
-    }
-}
-
-// plain gather: q = sum of slots (the un-lifted product), optionally accumulated into q
-template <class T, bool ACCUM>
-__global__ __launch_bounds__(kBlock) void k_gather_plain(const Desc *__restrict__ D, const int2 *__restrict__ tab) {
-    const int2 t = tab[blockIdx.x];
-    const Desc &d = D[t.x];
-    const int i = t.y * kBlock + threadIdx.x;
-    if (i >= d.n_dofs) return;
-    const T s = gather_slots<T>(d, i);
-    T *q = reinterpret_cast<T *>(d.q);
-    if (ACCUM) q[i] += s;
-    else q[i] = s;
-}
-
-// ---------------------------------------------------------------------------------------
-// mixed precision: FP32 inner solve of A d = r64 inside FP64 defect correction
-// ---------------------------------------------------------------------------------------
-// D32: the FP32 descriptor; r64: FP64 residual from k_gather_res<double> (outer scalars in
-// scal64).  Sets up d = 0, r32 = r64, z32 = r32/diag, p32 = 0 and the inner scalars.
-__global__ __launch_bounds__(kBlock) void k_mixed_init(const Desc *__restrict__ D32, const Desc *__restrict__ D64,
-                                                       const int2 *__restrict__ tab, double inner_tol) {
-    const int2 t = tab[blockIdx.x];
-    const Desc &d = D32[t.x];
-    const Desc &o = D64[t.x];
-    if (o.scal[S_ACTIVE] <= 0.0) {
-        if (t.y == 0 && threadIdx.x == 0) d.scal[S_ACTIVE] = 0.0;
-        return;
-    }
-    const int i = t.y * kBlock + threadIdx.x;
-    double v[2] = {0, 0};
-    if (i < d.n_dofs) {
-        const float r = (float)reinterpret_cast<const double *>(o.r)[i];
-        const float z = r * reinterpret_cast<const float *>(d.invdiag)[i];
-        reinterpret_cast<float *>(d.r)[i] = r;
-        reinterpret_cast<float *>(d.zw)[i] = z;
-        reinterpret_cast<float *>(d.p)[i] = 0.f;
-        reinterpret_cast<float *>(d.x)[i] = 0.f;
-        v[0] = (double)r * (double)z;
-        v[1] = (double)r * (double)r;
-    }
-    if (last_block_sum<2>(v, d.part, d.cnt, d.nb_dof, t.y) && threadIdx.x == 0) {
-        double *s = d.scal;
-        s[S_RZ] = v[0];
-        s[S_RR] = v[1];
-        s[S_BETA] = 0.0;
-        s[S_IT] = 0.0;
-        s[S_THR2] = inner_tol * inner_tol * v[1];
-        s[S_MAXIT] = o.scal[S_MAXIT] - o.scal[S_IT];
-        s[S_CONV] = 0.0;
-        s[S_ACTIVE] = (v[1] > s[S_THR2] && s[S_MAXIT] > 0) ? 1.0 : 0.0;
-    }
-}
-
-// x64 += d32 on systems whose outer iteration is active; counts the inner iterations
-__global__ __launch_bounds__(kBlock) void k_mixed_acc(const Desc *__restrict__ D32, const Desc *__restrict__ D64,
-                                                      const int2 *__restrict__ tab) {
-    const int2 t = tab[blockIdx.x];
-    const Desc &d = D32[t.x];
-    const Desc &o = D64[t.x];
-    if (o.scal[S_ACTIVE] <= 0.0) return;
-    const int i = t.y * kBlock + threadIdx.x;
-    if (i < d.n_dofs)
-        reinterpret_cast<double *>(o.x)[i] += (double)reinterpret_cast<const float *>(d.x)[i];
-    if (t.y == 0 && threadIdx.x == 0) {
-        o.scal[S_IT] += d.scal[S_IT];
-        o.scal[S_PQ] += 1.0;  // outer steps (S_PQ is unused by the outer FP64 residual)
-    }
-}
+#include "fgpu_cg.inc"
 
 // ---------------------------------------------------------------------------------------
 // setup kernels (FP64)
@@ -718,34 +539,6 @@ template void launch_elem_diag<double>(hipStream_t, const Desc *, const int2 *, 
 
 void launch_elem_rhs(hipStream_t s, const Desc *D, const int2 *tab, int nblk, const double *fq) {
     if (nblk > 0) hipLaunchKernelGGL(k_elem_rhs, dim3(nblk), dim3(kBlock), 0, s, D, tab, fq);
-}
-
-template <class T> void launch_gather_cg(hipStream_t s, const Desc *D, const int2 *tab, int nblk) {
-    if (nblk > 0) hipLaunchKernelGGL(k_gather_cg<T>, dim3(nblk), dim3(kBlock), 0, s, D, tab);
-}
-template <class T> void launch_update_cg(hipStream_t s, const Desc *D, const int2 *tab, int nblk) {
-    if (nblk > 0) hipLaunchKernelGGL(k_update_cg<T>, dim3(nblk), dim3(kBlock), 0, s, D, tab);
-}
-template <class T> void launch_gather_res(hipStream_t s, const Desc *D, const int2 *tab, int nblk) {
-    if (nblk > 0) hipLaunchKernelGGL(k_gather_res<T>, dim3(nblk), dim3(kBlock), 0, s, D, tab);
-}
-template <class T> void launch_gather_plain(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool accum) {
-    if (nblk <= 0) return;
-    if (accum) hipLaunchKernelGGL((k_gather_plain<T, true>), dim3(nblk), dim3(kBlock), 0, s, D, tab);
-    else hipLaunchKernelGGL((k_gather_plain<T, false>), dim3(nblk), dim3(kBlock), 0, s, D, tab);
-}
-template void launch_gather_cg<double>(hipStream_t, const Desc *, const int2 *, int);
-template void launch_gather_cg<float>(hipStream_t, const Desc *, const int2 *, int);
-template void launch_update_cg<double>(hipStream_t, const Desc *, const int2 *, int);
-template void launch_update_cg<float>(hipStream_t, const Desc *, const int2 *, int);
-template void launch_gather_res<double>(hipStream_t, const Desc *, const int2 *, int);
-template void launch_gather_plain<double>(hipStream_t, const Desc *, const int2 *, int, bool);
-
-void launch_mixed_init(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk, double inner_tol) {
-    if (nblk > 0) hipLaunchKernelGGL(k_mixed_init, dim3(nblk), dim3(kBlock), 0, s, D32, D64, tab, inner_tol);
-}
-void launch_mixed_acc(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk) {
-    if (nblk > 0) hipLaunchKernelGGL(k_mixed_acc, dim3(nblk), dim3(kBlock), 0, s, D32, D64, tab);
 }
 
 static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }

@@ -156,7 +156,7 @@ static uint64_t spread3(uint64_t v) {
 }
 
 void build_internal_order(Mesh &m, std::vector<int> &icells, std::vector<double> &X, std::vector<int> &gptr,
-                          std::vector<int> &gidx, std::vector<int> &fgptr, std::vector<int> &fgidx,
+                          std::vector<int> &spos, std::vector<int> &fgptr, std::vector<int> &fgidx,
                           std::vector<int> &bf_icell) {
     const int nc = m.n_cells, nd = m.n_dofs;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -206,14 +206,16 @@ void build_internal_order(Mesh &m, std::vector<int> &icells, std::vector<double>
     X.assign(4 * (size_t)nd, 0.0);
     for (int i = 0; i < nd; ++i)
         for (int a = 0; a < 3; ++a) X[4 * (size_t)i + a] = m.x[3 * (size_t)m.dof_of_idof[i] + a];
-    // dof -> element slots, ascending slot order (deterministic gather)
+    // element outputs are stored dof-major: dof i owns positions [gptr[i], gptr[i+1]) of the
+    // slot buffer, its cells in ascending (cell, node) order (deterministic sum); spos[8c+k]
+    // is where cell c writes its output for node k
     gptr.assign(nd + 1, 0);
     for (size_t s = 0; s < icells.size(); ++s) gptr[icells[s] + 1]++;
     for (int i = 0; i < nd; ++i) gptr[i + 1] += gptr[i];
-    gidx.resize(icells.size());
+    spos.resize(icells.size());
     {
         std::vector<int> pos(gptr.begin(), gptr.end() - 1);
-        for (size_t s = 0; s < icells.size(); ++s) gidx[pos[icells[s]]++] = (int)s;
+        for (size_t s = 0; s < icells.size(); ++s) spos[s] = pos[icells[s]]++;
     }
     // dof -> boundary-face slots (4 per face, API face order)
     static const int kFV[6][4] = {{0, 2, 4, 6}, {1, 3, 5, 7}, {0, 1, 4, 5}, {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};

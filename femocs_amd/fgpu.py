@@ -71,6 +71,7 @@ API = [
     ("fgpu_solve_batch", C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     ("fgpu_sys_minmax", C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("fgpu_sys_solution_grad", C.c_int, [_P, _D]),
+    ("fgpu_sys_time_kernel", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
 ]
 
 
@@ -245,6 +246,11 @@ class System:
         a, b = C.c_double(), C.c_double()
         self._c(self.ctx.lib.fgpu_sys_minmax(self.h, C.byref(a), C.byref(b)), "minmax")
         return a.value, b.value
+
+    def time_kernel(self, which=0, precision=FGPU_FP64, reps=50):
+        ms = C.c_double()
+        self._c(self.ctx.lib.fgpu_sys_time_kernel(self.h, which, precision, reps, C.byref(ms)), "time_kernel")
+        return ms.value
 
     def solution_grad(self):
         g = np.zeros(3 * self.mesh.n_dofs)

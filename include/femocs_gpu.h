@@ -131,6 +131,12 @@ int fgpu_sys_minmax(fgpu_sys *sys, double *min, double *max);
  * vertex2node of the last cell touching each dof; 3*n_dofs */
 int fgpu_sys_solution_grad(fgpu_sys *sys, double *grad);
 
+/* Benchmark helper, no counterpart in the reference: average duration in ms of `reps`
+ * back-to-back launches of one kernel of the CG step on this system's data (which = 0: the
+ * element apply in the given precision, 1: the FP64 dof gather), timed with HIP events on
+ * the context stream. */
+int fgpu_sys_time_kernel(fgpu_sys *sys, int which, int precision, int reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
