@@ -62,8 +62,8 @@ template <class T> void d2h(T *dst, const T *src, size_t n, hipStream_t s) {
 
 constexpr int kXcd = 8;  // MI355X: workgroups are dealt round-robin over 8 XCDs
 
-int padded_blocks(int n) {
-    const int nb = (n + kBlock - 1) / kBlock;
+int padded_blocks(int n, int per_block) {
+    const int nb = (n + per_block - 1) / per_block;
     return ((std::max(nb, 1) + kXcd - 1) / kXcd) * kXcd;
 }
 
@@ -82,7 +82,13 @@ fgpu::Mesh::~Mesh() {
     dfree(d_X64);
     dfree(d_X32);
     dfree(d_gptr);
-    dfree(d_spos);
+    dfree(d_pcell0);
+    dfree(d_pnode0);
+    dfree(d_pnodes);
+    dfree(d_pslot);
+    dfree(d_plidx);
+    dfree(d_plptr);
+    dfree(d_plent);
     dfree(d_bf_icell);
     dfree(d_bf_face);
     dfree(d_fgptr);
@@ -172,6 +178,34 @@ static void upload_coords(fgpu_mesh *m, const std::vector<double> &X) {
     HIPCHECK(hipStreamSynchronize(s));
 }
 
+// (re)uploads the patch structures and the dof-major slot layout
+static void upload_patches(fgpu_mesh *m, const std::vector<int> &gptr) {
+    hipStream_t s = m->ctx->stream;
+    HIPCHECK(hipStreamSynchronize(s));
+    dfree(m->d_gptr); dfree(m->d_pcell0); dfree(m->d_pnode0); dfree(m->d_pnodes); dfree(m->d_pslot);
+    dfree(m->d_plidx); dfree(m->d_plptr); dfree(m->d_plent);
+    const Patches &P = m->patches;
+    m->n_slots = P.node0.back();
+    m->d_gptr = dalloc<int>(gptr.size());
+    h2d(m->d_gptr, gptr.data(), gptr.size(), s);
+    m->d_pcell0 = dalloc<int>(P.cell0.size());
+    h2d(m->d_pcell0, P.cell0.data(), P.cell0.size(), s);
+    m->d_pnode0 = dalloc<int>(P.node0.size());
+    h2d(m->d_pnode0, P.node0.data(), P.node0.size(), s);
+    m->d_pnodes = dalloc<int>(P.nodes.size());
+    h2d(m->d_pnodes, P.nodes.data(), P.nodes.size(), s);
+    m->d_pslot = dalloc<int>(P.slot.size());
+    h2d(m->d_pslot, P.slot.data(), P.slot.size(), s);
+    m->d_plidx = dalloc<uint16_t>(P.lidx.size());
+    h2d(m->d_plidx, P.lidx.data(), P.lidx.size(), s);
+    m->d_plptr = dalloc<uint16_t>(P.lptr.size());
+    h2d(m->d_plptr, P.lptr.data(), P.lptr.size(), s);
+    m->d_plent = dalloc<uint16_t>(P.lent.size());
+    h2d(m->d_plent, P.lent.data(), P.lent.size(), s);
+    HIPCHECK(hipStreamSynchronize(s));
+    m->patch_gen++;
+}
+
 extern "C" int fgpu_mesh_import(fgpu_ctx *ctx, int nv, const double *xyz, int nc, const int *ucd, int top, int bottom,
                                 int sides, int other, fgpu_mesh **out) {
     if (!ctx || !out) return FGPU_EINVAL;
@@ -193,12 +227,11 @@ extern "C" int fgpu_mesh_import(fgpu_ctx *ctx, int nv, const double *xyz, int nc
         hipStream_t s = ctx->stream;
         m->d_cells = dalloc<int>(icells.size());
         h2d(m->d_cells, icells.data(), icells.size(), s);
+        HIPCHECK(hipStreamSynchronize(s));
         m->d_X64 = dalloc<double>(X.size());
         m->d_X32 = dalloc<float>(X.size());
-        m->d_gptr = dalloc<int>(gptr.size());
-        h2d(m->d_gptr, gptr.data(), gptr.size(), s);
-        m->d_spos = dalloc<int>(spos.size());
-        h2d(m->d_spos, spos.data(), spos.size(), s);
+        upload_patches(m, gptr);
+        m->icells = std::move(icells);
         const size_t nb = m->bf_cell.size();
         m->d_bf_icell = dalloc<int>(nb);
         h2d(m->d_bf_icell, bf_icell.data(), nb, s);
@@ -304,10 +337,49 @@ extern "C" int fgpu_mesh_update_vertices(fgpu_mesh *m, const double *xyz) {
     });
 }
 
+extern "C" int fgpu_mesh_set_patch_cells(fgpu_mesh *m, int cells) {
+    if (!m || cells < 8 || cells > kPatchCells) return FGPU_EINVAL;
+    return guarded(m->ctx, [&] {
+        HIPCHECK(hipSetDevice(m->ctx->device));
+        if (cells == m->patches.cells) return FGPU_OK;
+        std::vector<int> gptr;
+        build_patches(m->n_cells, m->n_dofs, m->icells, cells, m->patches);
+        build_slots(m->n_dofs, m->patches, gptr);
+        upload_patches(m, gptr);
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_mesh_patch_info(const fgpu_mesh *m, int *cells, int *n_patches, int *node_cap, int *n_slots) {
+    if (!m) return FGPU_EINVAL;
+    if (cells) *cells = m->patches.cells;
+    if (n_patches) *n_patches = m->patches.n;
+    if (node_cap) *node_cap = m->patches.node_cap;
+    if (n_slots) *n_slots = m->n_slots;
+    return FGPU_OK;
+}
+
 // ------------------------------------------------------------------------------------------
 // systems
 // ------------------------------------------------------------------------------------------
+// slot buffers and element block counts follow the mesh's current patches and the system's
+// patches-per-workgroup setting
+static void ensure_layout(fgpu_sys *s) {
+    fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+    s->nb_cell = padded_blocks(m->patches.n, s->ppb);
+    if (s->patch_gen == m->patch_gen) return;
+    HIPCHECK(hipStreamSynchronize(m->ctx->stream));
+    dfree(s->Ye);
+    dfree(s->Ye32);
+    s->Ye = dalloc<double>(m->n_slots);
+    s->Ye32 = dalloc<float>(m->n_slots);
+    s->patch_gen = m->patch_gen;
+    s->diag_valid = false;
+}
+static void ensure_layout(fgpu_sys *s);
+
 static Desc make_desc64(fgpu_sys *s, bool residual) {
+    ensure_layout(s);
     Desc d{};
     fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
     d.n_dofs = m->n_dofs;
@@ -317,7 +389,17 @@ static Desc make_desc64(fgpu_sys *s, bool residual) {
     d.cells = m->d_cells;
     d.X = m->d_X64;
     d.gptr = m->d_gptr;
-    d.spos = m->d_spos;
+    d.n_patch = m->patches.n;
+    d.pcells = m->patches.cells;
+    d.node_cap = m->patches.node_cap;
+    d.ppb = s->ppb;
+    d.pcell0 = m->d_pcell0;
+    d.pnode0 = m->d_pnode0;
+    d.pnodes = m->d_pnodes;
+    d.pslot = m->d_pslot;
+    d.plidx = m->d_plidx;
+    d.plptr = m->d_plptr;
+    d.plent = m->d_plent;
     d.dmask = s->d_dmask;
     d.kq = s->have_coef ? s->kq : nullptr;
     d.gq = s->have_coef ? s->gq : nullptr;
@@ -338,6 +420,7 @@ static Desc make_desc64(fgpu_sys *s, bool residual) {
 }
 
 static Desc make_desc32(fgpu_sys *s) {
+    ensure_layout(s);
     Desc d{};
     fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
     d.n_dofs = m->n_dofs;
@@ -347,7 +430,17 @@ static Desc make_desc32(fgpu_sys *s) {
     d.cells = m->d_cells;
     d.X = m->d_X32;
     d.gptr = m->d_gptr;
-    d.spos = m->d_spos;
+    d.n_patch = m->patches.n;
+    d.pcells = m->patches.cells;
+    d.node_cap = m->patches.node_cap;
+    d.ppb = s->ppb;
+    d.pcell0 = m->d_pcell0;
+    d.pnode0 = m->d_pnode0;
+    d.pnodes = m->d_pnodes;
+    d.pslot = m->d_pslot;
+    d.plidx = m->d_plidx;
+    d.plptr = m->d_plptr;
+    d.plent = m->d_plent;
     d.dmask = s->d_dmask;
     d.kq = s->have_coef ? s->kq32 : nullptr;
     d.gq = s->have_coef ? s->gq32 : nullptr;
@@ -376,16 +469,17 @@ extern "C" int fgpu_sys_create(fgpu_mesh *m, fgpu_sys **out) {
     s->ctx = m->ctx;
     const int rc = guarded(m->ctx, [&] {
         HIPCHECK(hipSetDevice(m->ctx->device));
-        const size_t n = m->n_dofs, nc = m->n_cells;
-        s->nb_cell = padded_blocks(m->n_cells);
-        s->nb_dof = padded_blocks(m->n_dofs);
+        const size_t n = m->n_dofs;
+        s->nb_cell = padded_blocks(m->patches.n, s->ppb);
+        s->nb_dof = padded_blocks(m->n_dofs, kDofBlock);
         s->b = dalloc<double>(n); s->x = dalloc<double>(n); s->r = dalloc<double>(n); s->z = dalloc<double>(n);
-        s->p = dalloc<double>(n); s->q = dalloc<double>(n); s->Ye = dalloc<double>(8 * nc);
+        s->p = dalloc<double>(n); s->q = dalloc<double>(n);
         s->invd = dalloc<double>(n); s->diag = dalloc<double>(n); s->g = dalloc<double>(n); s->tmp = dalloc<double>(n);
         s->fslot = dalloc<double>(4 * m->bf_cell.size()); s->fval = dalloc<double>(m->bf_cell.size());
         s->d_dmask = dalloc<unsigned char>(n);
         s->r32 = dalloc<float>(n); s->z32 = dalloc<float>(n); s->p32 = dalloc<float>(n); s->q32 = dalloc<float>(n);
-        s->d32 = dalloc<float>(n); s->Ye32 = dalloc<float>(8 * nc); s->invd32 = dalloc<float>(n);
+        s->d32 = dalloc<float>(n); s->invd32 = dalloc<float>(n);
+        ensure_layout(s);
         s->scal = dalloc<double>(2 * kScal);
         s->part = dalloc<double>(8 * (size_t)s->nb_dof);
         hipStream_t st = m->ctx->stream;
@@ -406,6 +500,12 @@ extern "C" int fgpu_sys_create(fgpu_mesh *m, fgpu_sys **out) {
     m->refs++;
     m->ctx->refs++;
     *out = s;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_sys_set_patches_per_block(fgpu_sys *s, int ppb) {
+    if (!s || ppb < 1 || ppb > 64) return FGPU_EINVAL;
+    s->ppb = ppb;
     return FGPU_OK;
 }
 
@@ -446,7 +546,12 @@ struct BatchTabs {
     std::vector<const Desc *> sets;  // device pointers of each descriptor set
     const int2 *etab = nullptr, *dtab = nullptr;
     int ne = 0, nd = 0;
+    size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the element kernel, largest patch of the batch
 };
+
+static size_t patch_lds(int node_cap, int cells, size_t tsize) {
+    return (4 * (size_t)node_cap + 8 * (size_t)cells) * tsize + 16 * (size_t)cells;
+}
 
 static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> &sets, const std::vector<fgpu_sys *> &sys) {
     std::vector<int2> et, dt;
@@ -474,6 +579,11 @@ static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> 
     t.dtab = reinterpret_cast<const int2 *>(dev + off_d);
     t.ne = (int)et.size();
     t.nd = (int)dt.size();
+    for (auto *x : sys) {
+        const fgpu_mesh *m = static_cast<const fgpu_mesh *>(x->mesh);
+        t.lds64 = std::max(t.lds64, patch_lds(m->patches.node_cap, m->patches.cells, sizeof(double)));
+        t.lds32 = std::max(t.lds32, patch_lds(m->patches.node_cap, m->patches.cells, sizeof(float)));
+    }
     return t;
 }
 
@@ -484,7 +594,7 @@ static void ensure_diag(fgpu_sys *s) {
     Desc d = make_desc64(s, true);
     d.q = s->diag;
     BatchTabs t = upload_batch(c, {{d}}, {s});
-    launch_elem_diag<double>(c->stream, t.sets[0], t.etab, t.ne, s->have_coef);
+    launch_elem_diag<double>(c->stream, t.sets[0], t.etab, t.ne, s->have_coef, t.lds64);
     launch_gather_plain<double>(c->stream, t.sets[0], t.dtab, t.nd, false);
     launch_invdiag(c->stream, m->n_dofs, s->d_dmask, s->diag, s->invd, s->invd32);
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -577,7 +687,7 @@ extern "C" int fgpu_sys_add_cell_rhs(fgpu_sys *s, const double *fq) {
         Desc d = make_desc64(s, true);
         d.q = s->b;
         BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
-        launch_elem_rhs(st, t.sets[0], t.etab, t.ne, d_f);
+        launch_elem_rhs(st, t.sets[0], t.etab, t.ne, d_f, t.lds64);
         launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, true);
         HIPCHECK(hipStreamSynchronize(st));
         dfree(d_f);
@@ -622,7 +732,7 @@ extern "C" int fgpu_sys_apply_dirichlet(fgpu_sys *s) {
         d.ua = s->g;
         d.q = s->tmp;
         BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
-        launch_elem<double>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA);
+        launch_elem<double>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds64);
         launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, false);
         launch_lift(st, n, s->d_dmask, s->g, s->tmp, s->diag, s->b, s->x);
         HIPCHECK(hipStreamSynchronize(st));
@@ -694,7 +804,7 @@ extern "C" int fgpu_sys_apply(fgpu_sys *s, const double *x, double *y) {
         d.ua = s->tmp;
         d.q = s->q;
         BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
-        launch_elem<double>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA);
+        launch_elem<double>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds64);
         launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, false);
         HIPCHECK(hipStreamSynchronize(st));
         return get_vec(s, s->q, y);
@@ -762,7 +872,7 @@ static bool any_flag(fgpu_ctx *c, const Desc *D, int n, int slot, double want, d
 // one CG iteration of every system of the set
 template <class T>
 static void cg_iter(hipStream_t st, const Desc *D, const BatchTabs &t, int n, bool coef, int reliable) {
-    launch_elem<T>(st, D, t.etab, t.ne, coef, false, true, S_BETA);
+    launch_elem<T>(st, D, t.etab, t.ne, coef, false, true, S_BETA, sizeof(T) == 8 ? t.lds64 : t.lds32);
     launch_gather_cg<T>(st, D, t.dtab, t.nd);
     launch_scal_alpha(st, D, n);
     launch_update_cg<T>(st, D, t.dtab, t.nd);
@@ -772,7 +882,7 @@ static void cg_iter(hipStream_t st, const Desc *D, const BatchTabs &t, int n, bo
 // FP64 residual of the lifted system (R64: residual descriptors, u = x with the constrained
 // dofs zeroed) and the resulting state
 static void fp64_residual(hipStream_t st, const Desc *R64, const Desc *D32, const BatchTabs &t, int n, bool coef, int mixed) {
-    launch_elem<double>(st, R64, t.etab, t.ne, coef, true, true, S_BETA);
+    launch_elem<double>(st, R64, t.etab, t.ne, coef, true, true, S_BETA, t.lds64);
     launch_gather_res<double>(st, R64, t.dtab, t.nd);
     launch_scal_res(st, R64, D32, n, mixed);
 }
@@ -918,11 +1028,11 @@ extern "C" int fgpu_sys_time_kernel(fgpu_sys *s, int which, int precision, int r
         BatchTabs t = upload_batch(c, {{d}}, {s});
         auto one = [&] {
             if (which == 0) {
-                if (precision == FGPU_MIXED) launch_elem<float>(c->stream, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA);
-                else launch_elem<double>(c->stream, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA);
+                if (precision == FGPU_MIXED) launch_elem<float>(c->stream, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds32);
+                else launch_elem<double>(c->stream, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds64);
             } else {
-                if (precision == FGPU_MIXED) throw std::invalid_argument("gather timing is FP64 only");
-                launch_gather_plain<double>(c->stream, t.sets[0], t.dtab, t.nd, false);
+                if (precision == FGPU_MIXED) launch_gather_plain<float>(c->stream, t.sets[0], t.dtab, t.nd, false);
+                else launch_gather_plain<double>(c->stream, t.sets[0], t.dtab, t.nd, false);
             }
         };
         one();

@@ -13,6 +13,27 @@
 namespace fgpu {
 
 constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
+constexpr int kDofIter = 4;   // dofs per thread in the per-dof kernels
+constexpr int kDofBlock = kBlock * kDofIter;
+constexpr int kPatchCells = 256;  // largest patch: cells per element workgroup (one per thread)
+constexpr int kPatchNodes = 640;  // largest number of distinct nodes a patch may touch
+constexpr int kPatchPerBlock = 4; // default patches processed in turn by one element workgroup
+
+// Cells are processed in patches of consecutive (space-filling-curve ordered) cells.  A patch
+// stages the coordinates and values of its distinct nodes in LDS, applies its cells, sums the
+// contributions per node in LDS in a fixed order and writes one output per (node, patch).
+struct Patches {
+    int n = 0;
+    int cells = kPatchCells;      // cells per patch (the last of a run may have fewer)
+    int node_cap = 0;             // largest node count of a patch, rounded up to 8
+    std::vector<int> cell0;       // n+1: patch p covers internal cells [cell0[p], cell0[p+1])
+    std::vector<int> node0;       // n+1: its nodes are nodes[node0[p] .. node0[p+1])
+    std::vector<int> nodes;       // internal dofs, first-appearance order within each patch
+    std::vector<int> slot;        // slot-buffer position of every patch node
+    std::vector<uint16_t> lidx;   // 8 per cell: patch-local index of each cell vertex
+    std::vector<uint16_t> lptr;   // n_p + 1 per patch (at node0[p] + p): CSR over local nodes
+    std::vector<uint16_t> lent;   // 8 per cell (at 8 * cell0[p]): entries cl*8+k, ascending
+};
 constexpr int kScal = 24;     // doubles of per-system scalars
 
 // indices into the per-system scalar block
@@ -45,7 +66,10 @@ struct Desc {
     const int *cells;             // 8 per cell, lexicographic, internal dof ids
     const void *X;                // T4 coordinates per dof
     const int *gptr;              // dof i owns slot-buffer positions [gptr[i], gptr[i+1])
-    const int *spos;              // 8 per cell: slot-buffer position of each cell output
+    int n_patch;                  // patches (see Patches)
+    int pcells, node_cap, ppb;    // patch size, LDS node capacity, patches per workgroup
+    const int *pcell0, *pnode0, *pnodes, *pslot;
+    const uint16_t *plidx, *plptr, *plent;
     const unsigned char *dmask;   // 1 on constrained dofs
     const void *kq, *gq;          // T, 8 per cell (stiffness / mass coefficient) or null
     const void *ua, *ub;          // T: element input u = ua + beta*ub (ub may be null)
@@ -75,11 +99,17 @@ struct Mesh {
     std::vector<int> icell_of_cell, cell_of_icell;
     std::vector<int> idof_of_dof, dof_of_idof;
     std::vector<int> v2c_cell, v2c_node;   // last cell (API) touching each dof and its node
+    Patches patches;
+    int n_slots = 0;                       // slot-buffer entries (sum of patch node counts)
+    int patch_gen = 0;                     // bumped whenever the patches are rebuilt
+    std::vector<int> icells;               // 8*n_cells, internal order and dofs (host copy)
     // device arrays (internal numbering)
     int *d_cells = nullptr;                // 8*n_cells
     double *d_X64 = nullptr;               // 4*n_dofs
     float *d_X32 = nullptr;                // 4*n_dofs
-    int *d_gptr = nullptr, *d_spos = nullptr;
+    int *d_gptr = nullptr;
+    int *d_pcell0 = nullptr, *d_pnode0 = nullptr, *d_pnodes = nullptr, *d_pslot = nullptr;
+    uint16_t *d_plidx = nullptr, *d_plptr = nullptr, *d_plent = nullptr;
     int *d_bf_icell = nullptr, *d_bf_face = nullptr;       // boundary faces (API order)
     int *d_fgptr = nullptr, *d_fgidx = nullptr;            // dof -> boundary-face slots
     int *d_v2c = nullptr, *d_v2n = nullptr;                 // gradient export (internal)
@@ -106,6 +136,8 @@ struct Sys {
     double *scal = nullptr;       // 2*kScal
     double *part = nullptr;       // 2 * 4*nb_dof
     int nb_cell = 0, nb_dof = 0;
+    int ppb = kPatchPerBlock;     // patches per element workgroup (tunable)
+    int patch_gen = -1;           // mesh patch generation the slot buffers are sized for
     ~Sys();
 };
 
@@ -135,4 +167,7 @@ int build_topology(Mesh &m, int nv, const double *xyz, int nc, const int *ucd, i
 void build_internal_order(Mesh &m, std::vector<int> &icells, std::vector<double> &X,
                           std::vector<int> &gptr, std::vector<int> &spos, std::vector<int> &fgptr,
                           std::vector<int> &fgidx, std::vector<int> &bf_icell);
+void build_patches(int nc, int nd, const std::vector<int> &icells, int cells_per_patch, Patches &P);
+// dof-major slot layout of the patch outputs: gptr (n_dofs+1) and Patches::slot
+void build_slots(int nd, Patches &P, std::vector<int> &gptr);
 }  // namespace fgpu

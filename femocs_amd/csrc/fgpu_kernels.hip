@@ -210,136 +210,142 @@ template <class T> __device__ __forceinline__ T gather_slots(const Desc &d, int 
     return s;
 }
 
-// stores the 8 outputs of cell c at their dof-major positions
-template <class T> __device__ __forceinline__ void store_slots(const Desc &d, int c, const T y[8]) {
-    const int4 *sp = reinterpret_cast<const int4 *>(d.spos) + 2 * (size_t)c;
-    const int4 s0 = sp[0], s1 = sp[1];
-    T *Ye = reinterpret_cast<T *>(d.Ye);
-    Ye[s0.x] = y[0];
-    Ye[s0.y] = y[1];
-    Ye[s0.z] = y[2];
-    Ye[s0.w] = y[3];
-    Ye[s1.x] = y[4];
-    Ye[s1.y] = y[5];
-    Ye[s1.z] = y[6];
-    Ye[s1.w] = y[7];
-}
+// ---------------------------------------------------------------------------------------
+// element kernel: one workgroup per patch of up to kPatchCells consecutive cells
+// ---------------------------------------------------------------------------------------
+// The patch's distinct nodes (at most kPatchNodes) are staged in LDS: coordinates and the
+// input value u = ua + beta * ub (MASKIN: zero on constrained dofs, the lifted columns).
+// Each thread applies one cell reading its 8 nodes from LDS and leaves its 8 outputs in LDS;
+// each node's contributions are then summed in a fixed order and written once to the
+// node's slot for this patch.
+//   MODE 0: y = K_e u          MODE 1: y_n = K_e(n,n)          MODE 2: y_n = int phi_n f
+enum { EM_APPLY = 0, EM_DIAG = 1, EM_RHS = 2 };
 
-// ---------------------------------------------------------------------------------------
-// element kernel
-// ---------------------------------------------------------------------------------------
-// u_n = ua_n + beta * ub_n (ub optional); MASKIN zeroes constrained dofs (lifted columns)
-template <class T, bool COEF, bool MASKIN, bool ACTIVE_ONLY>
-__global__ __launch_bounds__(kBlock) void k_elem(const Desc *__restrict__ D, const int2 *__restrict__ tab, int scal_slot) {
+extern __shared__ __attribute__((aligned(16))) unsigned char fg_smem[];
+
+template <class T, bool COEF, bool MASKIN, bool ACTIVE_ONLY, int MODE>
+__global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const Desc *__restrict__ D, const int2 *__restrict__ tab, int scal_slot,
+                                                     const double *__restrict__ fq) {
+    constexpr int kRounds = (kPatchNodes + kBlock - 1) / kBlock;
     const int2 t = tab[blockIdx.x];
     const Desc &d = D[t.x];
     if (ACTIVE_ONLY && d.scal[S_ACTIVE] == 0.0) return;
-    const int c = t.y * kBlock + threadIdx.x;
-    if (c >= d.n_cells) return;
     using V4 = typename Vec4T<T>::type;
-    const int4 *cp = reinterpret_cast<const int4 *>(d.cells) + 2 * (size_t)c;
-    const int4 n0 = cp[0], n1 = cp[1];
-    const int nid[8] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
+    // dynamic LDS: node coordinates and inputs (node_cap each), cell outputs, entry lists
+    T *sx = reinterpret_cast<T *>(fg_smem);
+    T *sy = sx + d.node_cap, *sz = sy + d.node_cap, *su = sz + d.node_cap, *sout = su + d.node_cap;
+    uint4 *slent = reinterpret_cast<uint4 *>(sout + 8 * d.pcells);
     const V4 *X = reinterpret_cast<const V4 *>(d.X);
     const T *ua = reinterpret_cast<const T *>(d.ua);
     const T *ub = reinterpret_cast<const T *>(d.ub);
-    const T beta = ub ? (T)d.scal[scal_slot] : (T)0;
-    T Xe[8][3], u[8];
+    const T beta = (MODE == EM_APPLY && ub) ? (T)d.scal[scal_slot] : (T)0;
+    T *Ye = reinterpret_cast<T *>(d.Ye);
+    const int tid = threadIdx.x;
+    const int p_end = min(d.n_patch, (t.y + 1) * d.ppb);
+    for (int p = t.y * d.ppb; p < p_end; ++p) {
+        const int c0 = d.pcell0[p], nc = d.pcell0[p + 1] - c0;
+        const int n0 = d.pnode0[p], np = d.pnode0[p + 1] - n0;
+        // phase 1: every global load of the patch is issued here, independent of each other
+        uint4 packed = make_uint4(0, 0, 0, 0);
+        if (tid < nc) {
+            packed = reinterpret_cast<const uint4 *>(d.plidx)[c0 + tid];
+            slent[tid] = reinterpret_cast<const uint4 *>(d.plent)[c0 + tid];
+        }
+        int eb[kRounds], ee[kRounds], os[kRounds];
 #pragma unroll
-    for (int n = 0; n < 8; ++n) {
-        const V4 p = X[nid[n]];
-        Xe[n][0] = p.x;
-        Xe[n][1] = p.y;
-        Xe[n][2] = p.z;
-        T v = ua[nid[n]];
-        if (ub) v += beta * ub[nid[n]];
-        if (MASKIN && d.dmask[nid[n]]) v = 0;
-        u[n] = v;
-    }
-    T kq[8], gq[8];
-    if (COEF) {
-        const T *k = reinterpret_cast<const T *>(d.kq) + 8 * (size_t)c;
-        const T *g = reinterpret_cast<const T *>(d.gq) + 8 * (size_t)c;
+        for (int r = 0; r < kRounds; ++r) {
+            const int j = tid + r * kBlock;
+            eb[r] = ee[r] = os[r] = 0;
+            if (j < np) {
+                eb[r] = d.plptr[n0 + p + j];
+                ee[r] = d.plptr[n0 + p + j + 1];
+                os[r] = d.pslot[n0 + j];
+                const int node = d.pnodes[n0 + j];
+                const V4 x = X[node];
+                sx[j] = x.x;
+                sy[j] = x.y;
+                sz[j] = x.z;
+                if (MODE == EM_APPLY) {
+                    T v = ua[node];
+                    if (ub) v += beta * ub[node];
+                    if (MASKIN && d.dmask[node]) v = 0;
+                    su[j] = v;
+                }
+            }
+        }
+        __syncthreads();
+        // phase 2: one cell per thread, from LDS
+        if (tid < nc) {
+            const int c = c0 + tid;
+            const int lid[8] = {(int)(packed.x & 0xffff), (int)(packed.x >> 16), (int)(packed.y & 0xffff), (int)(packed.y >> 16),
+                                (int)(packed.z & 0xffff), (int)(packed.z >> 16), (int)(packed.w & 0xffff), (int)(packed.w >> 16)};
+            T Xe[8][3];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            kq[q] = k[q];
-            gq[q] = g[q];
+            for (int n = 0; n < 8; ++n) {
+                Xe[n][0] = sx[lid[n]];
+                Xe[n][1] = sy[lid[n]];
+                Xe[n][2] = sz[lid[n]];
+            }
+            T kq[8], gq[8];
+            if (COEF) {
+                const T *k = reinterpret_cast<const T *>(d.kq) + 8 * (size_t)c;
+                const T *g = reinterpret_cast<const T *>(d.gq) + 8 * (size_t)c;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    kq[q] = k[q];
+                    gq[q] = g[q];
+                }
+            }
+            T y[8];
+            if (MODE == EM_APPLY) {
+                T u[8];
+#pragma unroll
+                for (int n = 0; n < 8; ++n) u[n] = su[lid[n]];
+                q1_cell<T, COEF>(Xe, u, kq, gq, y);
+            } else if (MODE == EM_DIAG) {
+                for (int n = 0; n < 8; ++n) {
+                    T u[8], yy[8];
+                    for (int m = 0; m < 8; ++m) u[m] = m == n ? (T)1 : (T)0;
+                    q1_cell<T, COEF>(Xe, u, kq, gq, yy);
+                    y[n] = yy[n];
+                }
+            } else {
+                for (int n = 0; n < 8; ++n) y[n] = 0;
+                for (int q = 0; q < 8; ++q) {
+                    const int a = q & 1, b = (q >> 1) & 1, cc = (q >> 2) & 1;
+                    T J[3][3];
+                    for (int co = 0; co < 3; ++co) {
+                        T f[8], F[2][2];
+                        for (int n = 0; n < 8; ++n) f[n] = Xe[n][co];
+                        dref<T>(f, 0, F); J[co][0] = F[b][cc];
+                        dref<T>(f, 1, F); J[co][1] = F[a][cc];
+                        dref<T>(f, 2, F); J[co][2] = F[a][b];
+                    }
+                    const T det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                                  J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                  J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                    const T jxw = (T)0.125 * det * (T)fq[8 * (size_t)c + q];
+                    for (int n = 0; n < 8; ++n)
+                        y[n] += Wt<T>(a, n & 1) * Wt<T>(b, (n >> 1) & 1) * Wt<T>(cc, (n >> 2) & 1) * jxw;
+                }
+            }
+#pragma unroll
+            for (int n = 0; n < 8; ++n) sout[8 * tid + n] = y[n];
         }
-    }
-    T y[8];
-    q1_cell<T, COEF>(Xe, u, kq, gq, y);
-    store_slots<T>(d, c, y);
-}
-
-// element diagonals (unit vectors through the same element kernel)
-template <class T, bool COEF>
-__global__ __launch_bounds__(kBlock) void k_elem_diag(const Desc *__restrict__ D, const int2 *__restrict__ tab) {
-    const int2 t = tab[blockIdx.x];
-    const Desc &d = D[t.x];
-    const int c = t.y * kBlock + threadIdx.x;
-    if (c >= d.n_cells) return;
-    using V4 = typename Vec4T<T>::type;
-    const int *cl = d.cells + 8 * (size_t)c;
-    const V4 *X = reinterpret_cast<const V4 *>(d.X);
-    T Xe[8][3];
-    for (int n = 0; n < 8; ++n) {
-        const V4 p = X[cl[n]];
-        Xe[n][0] = p.x;
-        Xe[n][1] = p.y;
-        Xe[n][2] = p.z;
-    }
-    T kq[8], gq[8];
-    if (COEF) {
-        for (int q = 0; q < 8; ++q) {
-            kq[q] = reinterpret_cast<const T *>(d.kq)[8 * (size_t)c + q];
-            gq[q] = reinterpret_cast<const T *>(d.gq)[8 * (size_t)c + q];
+        __syncthreads();
+        // phase 3: per patch node, its cell contributions in ascending order
+        const uint16_t *le = reinterpret_cast<const uint16_t *>(slent);
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            const int j = tid + r * kBlock;
+            if (j < np) {
+                T s = 0;
+                for (int e = eb[r]; e < ee[r]; ++e) s += sout[le[e]];
+                Ye[os[r]] = s;
+            }
         }
+        __syncthreads();
     }
-    T dg[8];
-    for (int n = 0; n < 8; ++n) {
-        T u[8], y[8];
-        for (int m = 0; m < 8; ++m) u[m] = m == n ? (T)1 : (T)0;
-        q1_cell<T, COEF>(Xe, u, kq, gq, y);
-        dg[n] = y[n];
-    }
-    store_slots<T>(d, c, dg);
-}
-
-// cell right-hand side: 8 contributions int phi_n f, f per Gauss point (FP64)
-__global__ __launch_bounds__(kBlock) void k_elem_rhs(const Desc *__restrict__ D, const int2 *__restrict__ tab, const double *__restrict__ fq) {
-    const int2 t = tab[blockIdx.x];
-    const Desc &d = D[t.x];
-    const int c = t.y * kBlock + threadIdx.x;
-    if (c >= d.n_cells) return;
-    const int *cl = d.cells + 8 * (size_t)c;
-    const double4 *X = reinterpret_cast<const double4 *>(d.X);
-    double Xe[8][3];
-    for (int n = 0; n < 8; ++n) {
-        const double4 p = X[cl[n]];
-        Xe[n][0] = p.x;
-        Xe[n][1] = p.y;
-        Xe[n][2] = p.z;
-    }
-    double y[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int q = 0; q < 8; ++q) {
-        const int a = q & 1, b = (q >> 1) & 1, cc = (q >> 2) & 1;
-        double J[3][3];
-        for (int co = 0; co < 3; ++co) {
-            double f[8], F[2][2];
-            for (int n = 0; n < 8; ++n) f[n] = Xe[n][co];
-            dref<double>(f, 0, F); J[co][0] = F[b][cc];
-            dref<double>(f, 1, F); J[co][1] = F[a][cc];
-            dref<double>(f, 2, F); J[co][2] = F[a][b];
-        }
-        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-        const double jxw = 0.125 * det * fq[8 * (size_t)c + q];
-        for (int n = 0; n < 8; ++n) {
-            const double phi = Wt<double>(a, n & 1) * Wt<double>(b, (n >> 1) & 1) * Wt<double>(cc, (n >> 2) & 1);
-            y[n] += phi * jxw;
-        }
-    }
-    store_slots<double>(d, c, y);
 }
 
 #include "fgpu_cg.inc"
@@ -514,10 +520,11 @@ __global__ void k_grad_export(int n, const int *__restrict__ v2c, const int *__r
 // explicit instantiations through launch helpers
 // ---------------------------------------------------------------------------------------
 template <class T>
-void launch_elem(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef, bool maskin, bool active_only, int scal_slot) {
+void launch_elem(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef, bool maskin, bool active_only, int scal_slot,
+                 size_t lds) {
     if (nblk <= 0) return;
     const dim3 g(nblk), b(kBlock);
-#define FG_ELEM(C, M, A) hipLaunchKernelGGL((k_elem<T, C, M, A>), g, b, 0, s, D, tab, scal_slot)
+#define FG_ELEM(C, M, A) hipLaunchKernelGGL((k_patch<T, C, M, A, EM_APPLY>), g, b, lds, s, D, tab, scal_slot, nullptr)
     if (coef) {
         if (maskin) { if (active_only) FG_ELEM(true, true, true); else FG_ELEM(true, true, false); }
         else { if (active_only) FG_ELEM(true, false, true); else FG_ELEM(true, false, false); }
@@ -527,18 +534,18 @@ void launch_elem(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool c
     }
 #undef FG_ELEM
 }
-template void launch_elem<double>(hipStream_t, const Desc *, const int2 *, int, bool, bool, bool, int);
-template void launch_elem<float>(hipStream_t, const Desc *, const int2 *, int, bool, bool, bool, int);
+template void launch_elem<double>(hipStream_t, const Desc *, const int2 *, int, bool, bool, bool, int, size_t);
+template void launch_elem<float>(hipStream_t, const Desc *, const int2 *, int, bool, bool, bool, int, size_t);
 
-template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef) {
+template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef, size_t lds) {
     if (nblk <= 0) return;
-    if (coef) hipLaunchKernelGGL((k_elem_diag<T, true>), dim3(nblk), dim3(kBlock), 0, s, D, tab);
-    else hipLaunchKernelGGL((k_elem_diag<T, false>), dim3(nblk), dim3(kBlock), 0, s, D, tab);
+    if (coef) hipLaunchKernelGGL((k_patch<T, true, false, false, EM_DIAG>), dim3(nblk), dim3(kBlock), lds, s, D, tab, 0, nullptr);
+    else hipLaunchKernelGGL((k_patch<T, false, false, false, EM_DIAG>), dim3(nblk), dim3(kBlock), lds, s, D, tab, 0, nullptr);
 }
-template void launch_elem_diag<double>(hipStream_t, const Desc *, const int2 *, int, bool);
+template void launch_elem_diag<double>(hipStream_t, const Desc *, const int2 *, int, bool, size_t);
 
-void launch_elem_rhs(hipStream_t s, const Desc *D, const int2 *tab, int nblk, const double *fq) {
-    if (nblk > 0) hipLaunchKernelGGL(k_elem_rhs, dim3(nblk), dim3(kBlock), 0, s, D, tab, fq);
+void launch_elem_rhs(hipStream_t s, const Desc *D, const int2 *tab, int nblk, const double *fq, size_t lds) {
+    if (nblk > 0) hipLaunchKernelGGL((k_patch<double, false, false, false, EM_RHS>), dim3(nblk), dim3(kBlock), lds, s, D, tab, 0, fq);
 }
 
 static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }

@@ -155,6 +155,65 @@ static uint64_t spread3(uint64_t v) {
     return v;
 }
 
+void build_patches(int nc, int nd, const std::vector<int> &icells, int cells_per_patch, Patches &P) {
+    P = Patches();
+    cells_per_patch = std::max(8, std::min(kPatchCells, cells_per_patch));
+    P.cells = cells_per_patch;
+    const int cap = std::min(kPatchNodes, 3 * cells_per_patch + 64);
+    P.cell0.push_back(0);
+    P.node0.push_back(0);
+    std::vector<int> stamp(nd, -1), local(nd, 0);
+    int c = 0;
+    while (c < nc) {
+        const int p = P.n, c_start = c;
+        int np = 0;
+        while (c < nc && c - c_start < cells_per_patch) {
+            int fresh = 0;
+            for (int k = 0; k < 8; ++k) fresh += stamp[icells[8 * (size_t)c + k]] != p;
+            if (np + fresh > cap) break;
+            for (int k = 0; k < 8; ++k) {
+                const int d = icells[8 * (size_t)c + k];
+                if (stamp[d] != p) {
+                    stamp[d] = p;
+                    local[d] = np++;
+                    P.nodes.push_back(d);
+                }
+                P.lidx.push_back((uint16_t)local[d]);
+            }
+            ++c;
+        }
+        P.cell0.push_back(c);
+        P.node0.push_back((int)P.nodes.size());
+        P.node_cap = std::max(P.node_cap, np);
+        P.n++;
+    }
+    P.node_cap = (P.node_cap + 7) / 8 * 8;
+    // per patch: local node -> its (cell, vertex) entries, ascending
+    P.lptr.assign(P.nodes.size() + P.n, 0);
+    P.lent.assign(8 * (size_t)nc, 0);
+    for (int p = 0; p < P.n; ++p) {
+        const int e0 = 8 * P.cell0[p], ne = 8 * (P.cell0[p + 1] - P.cell0[p]);
+        const int np = P.node0[p + 1] - P.node0[p];
+        uint16_t *ptr = &P.lptr[(size_t)P.node0[p] + p];
+        std::vector<int> cnt(np + 1, 0);
+        for (int e = 0; e < ne; ++e) cnt[P.lidx[e0 + e] + 1]++;
+        for (int j = 0; j < np; ++j) cnt[j + 1] += cnt[j];
+        for (int j = 0; j <= np; ++j) ptr[j] = (uint16_t)cnt[j];
+        for (int e = 0; e < ne; ++e) P.lent[e0 + cnt[P.lidx[e0 + e]]++] = (uint16_t)e;
+    }
+}
+
+// patch outputs are stored dof-major: dof i owns positions [gptr[i], gptr[i+1]) of the slot
+// buffer, one per patch touching it, patches ascending (deterministic sum)
+void build_slots(int nd, Patches &P, std::vector<int> &gptr) {
+    gptr.assign(nd + 1, 0);
+    for (int d : P.nodes) gptr[d + 1]++;
+    for (int i = 0; i < nd; ++i) gptr[i + 1] += gptr[i];
+    P.slot.resize(P.nodes.size());
+    std::vector<int> pos(gptr.begin(), gptr.end() - 1);
+    for (size_t j = 0; j < P.nodes.size(); ++j) P.slot[j] = pos[P.nodes[j]]++;
+}
+
 void build_internal_order(Mesh &m, std::vector<int> &icells, std::vector<double> &X, std::vector<int> &gptr,
                           std::vector<int> &spos, std::vector<int> &fgptr, std::vector<int> &fgidx,
                           std::vector<int> &bf_icell) {
@@ -206,17 +265,9 @@ void build_internal_order(Mesh &m, std::vector<int> &icells, std::vector<double>
     X.assign(4 * (size_t)nd, 0.0);
     for (int i = 0; i < nd; ++i)
         for (int a = 0; a < 3; ++a) X[4 * (size_t)i + a] = m.x[3 * (size_t)m.dof_of_idof[i] + a];
-    // element outputs are stored dof-major: dof i owns positions [gptr[i], gptr[i+1]) of the
-    // slot buffer, its cells in ascending (cell, node) order (deterministic sum); spos[8c+k]
-    // is where cell c writes its output for node k
-    gptr.assign(nd + 1, 0);
-    for (size_t s = 0; s < icells.size(); ++s) gptr[icells[s] + 1]++;
-    for (int i = 0; i < nd; ++i) gptr[i + 1] += gptr[i];
-    spos.resize(icells.size());
-    {
-        std::vector<int> pos(gptr.begin(), gptr.end() - 1);
-        for (size_t s = 0; s < icells.size(); ++s) spos[s] = pos[icells[s]]++;
-    }
+    build_patches(nc, nd, icells, kPatchCells, m.patches);
+    build_slots(nd, m.patches, gptr);
+    spos.clear();
     // dof -> boundary-face slots (4 per face, API face order)
     static const int kFV[6][4] = {{0, 2, 4, 6}, {1, 3, 5, 7}, {0, 1, 4, 5}, {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
     const size_t nb = m.bf_cell.size();

@@ -72,6 +72,10 @@ API = [
     ("fgpu_sys_minmax", C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("fgpu_sys_solution_grad", C.c_int, [_P, _D]),
     ("fgpu_sys_time_kernel", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    ("fgpu_mesh_set_patch_cells", C.c_int, [_P, C.c_int]),
+    ("fgpu_mesh_patch_info", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]),
+    ("fgpu_sys_set_patches_per_block", C.c_int, [_P, C.c_int]),
 ]
 
 
@@ -171,6 +175,14 @@ class Mesh:
         self.ctx.lib.fgpu_mesh_export_face_centroids(self.h, bid, out.ctypes.data, C.byref(n))
         return out.reshape(-1, 3)
 
+    def set_patch_cells(self, cells: int):
+        self.ctx.check(self.ctx.lib.fgpu_mesh_set_patch_cells(self.h, int(cells)), "fgpu_mesh_set_patch_cells")
+
+    def patch_info(self):
+        v = [C.c_int() for _ in range(4)]
+        self.ctx.lib.fgpu_mesh_patch_info(self.h, *[C.byref(x) for x in v])
+        return dict(zip(("cells", "n_patches", "node_cap", "n_slots"), (x.value for x in v)))
+
     def update_vertices(self, vertices):
         v = np.ascontiguousarray(vertices, np.float64).reshape(-1)
         self.ctx.check(self.ctx.lib.fgpu_mesh_update_vertices(self.h, v), "fgpu_mesh_update_vertices")
@@ -246,6 +258,9 @@ class System:
         a, b = C.c_double(), C.c_double()
         self._c(self.ctx.lib.fgpu_sys_minmax(self.h, C.byref(a), C.byref(b)), "minmax")
         return a.value, b.value
+
+    def set_patches_per_block(self, ppb: int):
+        self._c(self.ctx.lib.fgpu_sys_set_patches_per_block(self.h, int(ppb)), "set_patches_per_block")
 
     def time_kernel(self, which=0, precision=FGPU_FP64, reps=50):
         ms = C.c_double()

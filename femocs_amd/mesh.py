@@ -379,7 +379,7 @@ def config2_geometry(**kw) -> TipGeometry:
 
 def small_geometry(**kw) -> TipGeometry:
     """A small tip for tests that run the CPU oracle in seconds."""
-    kw.setdefault("h_min", 6.0)
-    kw.setdefault("growth", 0.5)
-    return TipGeometry(radius=12.0, height=30.0, slab_width=90.0, slab_thickness=12.0,
-                       top_factor=4.0, **kw)
+    for k, v in (("h_min", 6.0), ("growth", 0.5), ("radius", 12.0), ("height", 30.0),
+                 ("slab_width", 90.0), ("slab_thickness", 12.0), ("top_factor", 4.0)):
+        kw.setdefault(k, v)
+    return TipGeometry(**kw)

@@ -88,6 +88,14 @@ int fgpu_mesh_export_face_centroids(const fgpu_mesh *mesh, int bid, double *xyz,
 /* moved vertices, same topology (3*n_vertices of the import numbering) */
 int fgpu_mesh_update_vertices(fgpu_mesh *mesh, const double *xyz);
 
+/* Tuning, no counterpart in the reference: the element kernel works on patches of
+ * consecutive cells whose nodes are staged in LDS; cells per patch (8..256) and patches per
+ * workgroup (1..64) trade LDS reuse against parallelism.  Changing the patch size rebuilds
+ * the mesh's patch tables; systems on the mesh follow at their next call. */
+int fgpu_mesh_set_patch_cells(fgpu_mesh *mesh, int cells);
+int fgpu_mesh_patch_info(const fgpu_mesh *mesh, int *cells, int *n_patches, int *node_cap, int *n_slots);
+int fgpu_sys_set_patches_per_block(fgpu_sys *sys, int patches_per_block);
+
 /* ---- linear system on a mesh ----------------------------------------------------------
  * One fgpu_sys = the system_matrix/system_rhs/solution triple of a DealSolver, held on the
  * GPU matrix-free. */
@@ -133,8 +141,8 @@ int fgpu_sys_solution_grad(fgpu_sys *sys, double *grad);
 
 /* Benchmark helper, no counterpart in the reference: average duration in ms of `reps`
  * back-to-back launches of one kernel of the CG step on this system's data (which = 0: the
- * element apply in the given precision, 1: the FP64 dof gather), timed with HIP events on
- * the context stream. */
+ * element apply, 1: the dof gather, in the given precision), timed with HIP events on the
+ * context stream. */
 int fgpu_sys_time_kernel(fgpu_sys *sys, int which, int precision, int reps, double *avg_ms);
 
 #ifdef __cplusplus

@@ -72,3 +72,49 @@ def test_poisson_solution_matches_oracle(ctx, small_mesh, precision):
     x = s.solution()
     err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
     assert err <= 1e-10, err
+
+
+def _poisson_system(ctx, mesh, field=-1.0):
+    gm = fgpu.Mesh(ctx, mesh.nodes, mesh.vacuum_cells(), *MARKS)
+    s = fgpu.System(gm)
+    s.reset()
+    s.add_dirichlet(2, 0.0)
+    s.add_neumann(8, None, field)
+    s.apply_dirichlet()
+    s.fill_solution(0.0)
+    return s
+
+
+@pytest.fixture(scope="module")
+def three_meshes(small_mesh):
+    from femocs_amd import mesh as M
+    m2 = M.make_tip_mesh(M.small_geometry(radius=10.0, height=24.0, seed=7))
+    m3 = M.make_tip_mesh(M.small_geometry(radius=14.0, height=40.0, h_min=7.0, seed=99))
+    return [small_mesh, m2, m3]
+
+
+@pytest.mark.parametrize("precision", [fgpu.FGPU_FP64, fgpu.FGPU_MIXED])
+def test_batch_results_equal_alone_and_in_mix(ctx, three_meshes, precision):
+    """A system's solution is bit-identical whether it is solved alone or batched with others
+    of different sizes (no floating-point atomics, per-system fixed reduction order)."""
+    o = fgpu.opts(tol=1e-10, relative=True, precision=precision)
+    alone = []
+    for i, m in enumerate(three_meshes):
+        s = _poisson_system(ctx, m, field=-1.0 - 0.1 * i)
+        st = s.solve(o)
+        assert st.iterations > 0
+        alone.append((s.solution(), st.iterations))
+    systems = [_poisson_system(ctx, m, field=-1.0 - 0.1 * i) for i, m in enumerate(three_meshes)]
+    stats = ctx.solve_batch(systems, o)
+    for (xa, ita), s, st in zip(alone, systems, stats):
+        assert st.iterations == ita
+        assert np.array_equal(s.solution(), xa)
+
+
+def test_context_closed_before_children(small_mesh):
+    c = fgpu.Context(0)
+    s = _poisson_system(c, small_mesh)
+    c.close()                      # deferred: the system still holds the context
+    st = s.solve(fgpu.opts(tol=1e-8, relative=True))
+    assert st.iterations > 0
+    del s                          # the last child frees the context
