@@ -97,7 +97,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--small", action="store_true", help="config-1 mesh instead (quick check)")
+    ap.add_argument("--small", action="store_true", help="nanotip r=3nm h=15nm mesh instead (quick check)")
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--no-graph", action="store_true", help="launch kernels directly (for profilers)")
     args = ap.parse_args()
@@ -111,7 +111,7 @@ def main():
     from femocs_amd import fgpu
     from femocs_amd import mesh as M
 
-    g = M.config1_geometry() if args.small else M.config2_geometry()
+    g = M.tip_r3nm_geometry() if args.small else M.tip_r10nm_geometry()
     t = time.perf_counter()
     mesh = M.make_tip_mesh(g, with_atoms=False)
     log(f"mesh: {mesh.n_tets} tets, {mesh.n_hexes} hexes ({time.perf_counter() - t:.1f} s)")
@@ -122,28 +122,35 @@ def main():
     setup_ms = (time.perf_counter() - t) * 1e3
     log(f"device mesh: {gm.n_dofs} dofs, {gm.n_cells} cells, import {setup_ms:.0f} ms")
 
+    from femocs_amd import autotune
+    tuned = {}
+    best, _ = autotune.autotune(sys_, fgpu.FGPU_FP64)
+    tuned["fp64"] = {"cells_per_patch": best[0], "patches_per_block": best[1]}
     fp64, phi64 = time_mode(sys_, fgpu.FGPU_FP64, args.steps, args.warmup, args.tol)
-    log("fp64", fp64)
+    log("fp64", fp64, tuned["fp64"])
+    best, _ = autotune.autotune(sys_, fgpu.FGPU_MIXED)
+    tuned["mixed"] = {"cells_per_patch": best[0], "patches_per_block": best[1]}
     mixed, phimx = time_mode(sys_, fgpu.FGPU_MIXED, args.steps, args.warmup, args.tol)
-    log("mixed", mixed)
+    log("mixed", mixed, tuned["mixed"])
     err_mixed_vs_fp64 = float(np.linalg.norm(phimx - phi64) / np.linalg.norm(phi64))
 
-    # dominant kernel: the element apply (FP32 in the mixed solver's inner iterations)
+    # dominant kernel: the element apply, FP32 in the mixed solver's iterations (the system
+    # is left tuned for mixed precision).  Algorithmic bytes per launch = compulsory traffic
+    # of a matrix-free Q1 apply: connectivity 32 B/cell (8 x int32) + coordinates 3*vb B/dof
+    # + the two input vectors (z, p) 2*vb B/dof + one output per dof vb B/dof.
     H, N = gm.n_cells, gm.n_dofs
     kern = {}
-    for prec, name, vb, cb in ((fgpu.FGPU_MIXED, "k_elem<float>", 4, 12), (fgpu.FGPU_FP64, "k_elem<double>", 8, 24)):
+    for prec, name, vb in ((fgpu.FGPU_MIXED, "k_patch<float>", 4), (fgpu.FGPU_FP64, "k_patch<double>", 8)):
         ms = sys_.time_kernel(0, prec, reps=50)
-        # algorithmic bytes per launch: connectivity 32 B/cell + coordinates cb B/dof +
-        # two input vectors (z, p) 2*vb B/dof + 8 outputs 8*vb B/cell
-        byts = 32 * H + cb * N + 2 * vb * N + 8 * vb * H
+        byts = 32 * H + 3 * vb * N + 2 * vb * N + vb * N
         kern[name] = {"ms": ms, "bytes": byts, "GBs": byts / (ms * 1e-3) / 1e9}
     log("kernels", kern)
-    k = kern["k_elem<float>"]
+    k = kern["k_patch<float>"]
     roofline = {"bound": "hbm", "achieved": round(k["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(k["GBs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_elem<float>",
+                "frac": round(k["GBs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_patch<float>",
                 "avg_us": round(k["ms"] * 1e3, 2), "algorithmic_bytes": k["bytes"],
-                "fp64_kernel": {"avg_us": round(kern["k_elem<double>"]["ms"] * 1e3, 2),
-                                "GBs": round(kern["k_elem<double>"]["GBs"], 1)}}
+                "fp64_kernel": {"avg_us": round(kern["k_patch<double>"]["ms"] * 1e3, 2),
+                                "GBs": round(kern["k_patch<double>"]["GBs"], 1)}}
 
     cpu = None
     err_vs_cpu = None
@@ -166,8 +173,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 inner / f64 outer",
         "data": "synthetic",
-        "config": {"workload": ("config-1 nanotip r=3nm h=15nm" if args.small else
-                                "config-2 nanotip r=10nm h=60nm on 120x120 nm slab") +
+        "config": {"workload": ("Cu nanotip r=3nm h=15nm on 30x30x3 nm slab" if args.small else
+                                "Cu nanotip r=10nm h=60nm on 120x120 nm slab") +
                                ", Q1 tet-split hexes, 1 Poisson solve per step",
                    "n_dofs": N, "n_cells": H, "tol_rel": args.tol, "precond": "jacobi",
                    "parallelism": "single GPU"},
@@ -178,6 +185,7 @@ def main():
         "err_mixed_vs_fp64": err_mixed_vs_fp64,
         "err_fp64_vs_cpu_oracle": err_vs_cpu,
         "mesh_import_ms": round(setup_ms, 1),
+        "autotuned": tuned,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

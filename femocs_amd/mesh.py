@@ -367,12 +367,12 @@ def make_tip_mesh(g: TipGeometry | None = None, with_atoms: bool = True) -> TipM
                    atoms=atoms if with_atoms else None)
 
 
-def config1_geometry(**kw) -> TipGeometry:
+def tip_r3nm_geometry(**kw) -> TipGeometry:
     """Nanotip r = 3 nm, h = 15 nm on a 30 x 30 x 3 nm slab, top at 10 x apex height."""
     return TipGeometry(radius=30.0, height=150.0, slab_width=300.0, slab_thickness=30.0, **kw)
 
 
-def config2_geometry(**kw) -> TipGeometry:
+def tip_r10nm_geometry(**kw) -> TipGeometry:
     """Nanotip r = 10 nm, h = 60 nm on a 120 x 120 nm slab, top at 10 x apex height."""
     return TipGeometry(radius=100.0, height=600.0, slab_width=1200.0, slab_thickness=60.0, **kw)
 

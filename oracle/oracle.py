@@ -18,8 +18,8 @@ _lib = None
 BID_COPPER_SURFACE, BID_VACUUM_TOP, BID_COPPER_BOTTOM, BID_SIDES = 2, 8, 7, 4
 
 
-def build():
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+def build(force: bool = False):
+    subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE], check=True)
 
 
 def lib():

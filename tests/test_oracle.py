@@ -1,0 +1,332 @@
+"""Pins the CPU oracle with checks that do not restate its own formulas: textbook element
+matrices, invariants, closed-form solutions, convergence order, brute force on tiny inputs
+and the reference's own data tables and mesh file."""
+import os
+
+import numpy as np
+import pytest
+
+from femocs_amd import mesh as M
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+UNIT_LEX = np.array([[i & 1, (i >> 1) & 1, (i >> 2) & 1] for i in range(8)], dtype=np.float64)
+
+
+def box_tets(n=(3, 3, 3), L=(1.0, 1.0, 1.0), jitter=0.0, seed=0):
+    """Structured box split into 6 Kuhn tets per cube (jitter moves interior nodes)."""
+    nx, ny, nz = n
+    xs = [np.linspace(0, L[d], n[d] + 1) for d in range(3)]
+    P = np.stack(np.meshgrid(*xs, indexing="ij"), -1).reshape(-1, 3)
+    idx = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    tets = []
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                for p in perms:
+                    c = [0, 0, 0]
+                    path = [idx(i, j, k)]
+                    for d in p:
+                        c[d] += 1
+                        path.append(idx(i + c[0], j + c[1], k + c[2]))
+                    tets.append(path)
+    tets = np.array(tets)
+    rng = np.random.default_rng(seed)
+    inner = np.all((P > 1e-9) & (P < np.array(L) - 1e-9), axis=1)
+    P[inner] += rng.uniform(-jitter, jitter, (inner.sum(), 3))
+    vol = M._tet_volumes(P, tets)
+    tets[vol < 0] = tets[vol < 0][:, [0, 2, 1, 3]]
+    return P, tets
+
+
+def box_hexes(n=(3, 3, 4), L=(1.0, 1.0, 2.0), jitter=0.05, seed=0):
+    P, T = box_tets(n, L, jitter, seed)
+    return M.tet_to_hex(P, T)
+
+
+# ------------------------------------------------------------------------------------------
+# element matrices
+# ------------------------------------------------------------------------------------------
+def test_unit_cube_stiffness_is_textbook():
+    """Q1 Laplace on the unit cube: 1/3 on the diagonal, 0 for edge neighbours, -1/12 for
+    face- and body-diagonal pairs (exact with 2-point Gauss)."""
+    K = O.element_matrix(UNIT_LEX)
+    for i in range(8):
+        for j in range(8):
+            d = bin(i ^ j).count("1")
+            expect = {0: 1 / 3, 1: 0.0, 2: -1 / 12, 3: -1 / 12}[d]
+            assert abs(K[i, j] - expect) < 1e-15
+
+
+def test_unit_cube_mass_matrix_is_textbook():
+    """Q1 mass on the unit cube: (1/27) * 2^(3-d) / 8 ... i.e. 1/27, 1/54, 1/108, 1/216 by
+    the number d of differing coordinates (exact with 2-point Gauss)."""
+    K = O.element_matrix(UNIT_LEX, kappa_q=np.zeros(8), gamma_q=np.ones(8))
+    for i in range(8):
+        for j in range(8):
+            d = bin(i ^ j).count("1")
+            assert abs(K[i, j] - {0: 1 / 27, 1: 1 / 54, 2: 1 / 108, 3: 1 / 216}[d]) < 1e-16
+
+
+def test_element_matrix_scales_and_is_invariant():
+    rng = np.random.default_rng(3)
+    base = UNIT_LEX + rng.uniform(-0.15, 0.15, (8, 3))
+    K = O.element_matrix(base)
+    assert np.allclose(K, K.T, atol=1e-15)
+    assert np.abs(K.sum(axis=1)).max() < 1e-14          # constants are in the kernel
+    assert np.linalg.eigvalsh(K)[1] > 0                  # only one zero eigenvalue
+    # stiffness scales linearly with size, is invariant under rotation and translation
+    assert np.allclose(O.element_matrix(2.0 * base), 2.0 * K, rtol=1e-13, atol=1e-14)
+    th = 0.7
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1.0]])
+    assert np.allclose(O.element_matrix(base @ R.T + 5.0), K, rtol=1e-12, atol=1e-13)
+
+
+def test_stiffness_quadratic_form_matches_exact_energy_for_linear_field():
+    """For u = a.x the energy u^T K u equals |a|^2 * volume on any trilinear hex."""
+    rng = np.random.default_rng(4)
+    xe = UNIT_LEX * np.array([1.0, 2.0, 0.5]) + rng.uniform(-0.1, 0.1, (8, 3))
+    a = np.array([0.3, -1.2, 0.7])
+    u = xe @ a
+    K = O.element_matrix(xe)
+    M1 = O.element_matrix(xe, kappa_q=np.zeros(8), gamma_q=np.ones(8))
+    vol = M1.sum()
+    assert abs(u @ K @ u - (a @ a) * vol) < 1e-12 * (a @ a) * vol
+
+
+# ------------------------------------------------------------------------------------------
+# mesh import and boundary marking
+# ------------------------------------------------------------------------------------------
+def test_import_compacts_vertices_and_marks_box_faces():
+    nodes, _, hexes = box_hexes()
+    extra = np.vstack([nodes, [[9.0, 9.0, 9.0]]])        # an unused vertex is dropped
+    m = O.Mesh(extra, hexes, 8, 2, 4, 2)
+    assert m.n_dofs == len(nodes) and np.array_equal(m.vertex_of_dof, np.arange(len(nodes)))
+    # every boundary face lies on the box; ids by position
+    FV = [[0, 2, 4, 6], [1, 3, 5, 7], [0, 1, 4, 5], [2, 3, 6, 7], [0, 1, 2, 3], [4, 5, 6, 7]]
+    for c, f, b in zip(m.bface_cell, m.bface_face, m.bface_id):
+        ctr = m.x[m.cells[c][FV[f]]].mean(0)
+        if np.isclose(ctr[0], 0) or np.isclose(ctr[0], 1) or np.isclose(ctr[1], 0) or np.isclose(ctr[1], 1):
+            assert b == 4
+        elif np.isclose(ctr[2], 2.0):
+            assert b == 8
+        else:
+            assert np.isclose(ctr[2], 0.0) and b == 2
+    # lexicographic cells are positively oriented
+    for c in m.cells[:50]:
+        x = m.x[c]
+        assert np.linalg.det(np.array([x[1] - x[0], x[2] - x[0], x[4] - x[0]])) > 0
+
+
+def test_neumann_load_sums_to_flux():
+    nodes, _, hexes = box_hexes()
+    m = O.Mesh(nodes, hexes, 8, 2, 4, 2)
+    b = m.face_rhs(8, constant=2.5)
+    assert abs(b.sum() - 2.5 * 1.0) < 1e-13                 # top face area 1
+    top = m.x[:, 2] > 2.0 - 1e-12
+    assert np.all(b[~top] == 0)
+
+
+# ------------------------------------------------------------------------------------------
+# solutions
+# ------------------------------------------------------------------------------------------
+def test_patch_test_linear_potential_exact():
+    """phi = 0 at the bottom, dphi/dn = -1 on the top: the exact solution -z is linear and
+    reproduced to round-off on distorted trilinear hexes."""
+    nodes, _, hexes = box_hexes(jitter=0.07)
+    om, x, it, res, _ = O.poisson_laplace(nodes, hexes, -1.0, tol=1e-13, relative=True)
+    assert it > 0
+    assert np.abs(x + om.x[:, 2]).max() < 1e-12
+
+
+def _dirichlet_solve(nodes, hexes, g):
+    m = O.Mesh(nodes, hexes, 2, 2, 2, 2).assemble()
+    b = np.zeros(m.n_dofs)
+    x = np.zeros(m.n_dofs)
+    mask = m.dirichlet_mask(2)
+    m.apply_dirichlet(b, x, mask, g(m.x))
+    it, _ = m.pcg(b, x, tol=1e-14, relative=False, precond="ssor", max_it=5000)
+    assert it >= 0
+    return m, x
+
+
+def test_harmonic_solution_converges_at_second_order():
+    """Dirichlet data of the harmonic phi = x^2 + y^2 - 2 z^2 on the whole boundary: the
+    nodal error falls by ~4x when the mesh is refined by 2."""
+    g = lambda x: x[:, 0] ** 2 + x[:, 1] ** 2 - 2 * x[:, 2] ** 2
+    errs = []
+    for n in (3, 6):
+        nodes, _, hexes = box_hexes(n=(n, n, n), L=(1.0, 1.0, 1.0), jitter=0.0)
+        m, x = _dirichlet_solve(nodes, hexes, g)
+        errs.append(np.abs(x - g(m.x)).max())
+    assert 3.0 < errs[0] / errs[1] < 6.0
+
+
+def test_lifted_system_equals_dense_constrained_solve():
+    """deal.II-style lifting (row and column elimination) gives the same solution as solving
+    the reduced system for the free dofs directly (numpy dense solve)."""
+    nodes, _, hexes = box_hexes(n=(2, 2, 2), L=(1, 1, 1), jitter=0.05)
+    m = O.Mesh(nodes, hexes, 8, 2, 4, 2).assemble()
+    rp, ci, vv = m.csr()
+    A = np.zeros((m.n_dofs, m.n_dofs))
+    for i in range(m.n_dofs):
+        A[i, ci[rp[i]:rp[i + 1]]] = vv[rp[i]:rp[i + 1]]
+    b0 = m.face_rhs(8, constant=-0.7)
+    mask = m.dirichlet_mask(2)
+    g = np.where(mask > 0, 0.3 * m.x[:, 0], 0.0)
+    free = mask == 0
+    xd = np.where(mask > 0, g, 0.0)
+    xf = np.linalg.solve(A[np.ix_(free, free)], b0[free] - A[np.ix_(free, ~free)] @ xd[~free])
+    b = b0.copy()
+    x = np.zeros(m.n_dofs)
+    m.apply_dirichlet(b, x, mask, g)
+    it, _ = m.pcg(b, x, tol=1e-15, relative=True, precond="ssor")
+    assert np.allclose(x[free], xf, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(x[~free], g[~free])
+
+
+@pytest.mark.parametrize("precond", ["none", "jacobi", "ssor"])
+def test_pcg_matches_dense_solve(precond):
+    nodes, _, hexes = box_hexes(n=(2, 2, 3), L=(1, 1, 1.5), jitter=0.05)
+    om, x, it, res, b = O.poisson_laplace(nodes, hexes, -1.0, tol=1e-14, relative=True, precond=precond)
+    rp, ci, vv = om.csr()
+    A = np.zeros((om.n_dofs, om.n_dofs))
+    for i in range(om.n_dofs):
+        A[i, ci[rp[i]:rp[i + 1]]] = vv[rp[i]:rp[i + 1]]
+    xd = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xd) <= 1e-11 * np.linalg.norm(xd)
+
+
+def test_absolute_tolerance_and_failure_sign():
+    """SolverControl semantics: absolute |r| <= tol, negative iteration count on failure
+    (src/DealSolver.cpp:443-458)."""
+    nodes, _, hexes = box_hexes()
+    om, x, it, res, b = O.poisson_laplace(nodes, hexes, -1.0, tol=1e-9, relative=False, max_it=3)
+    assert it == -3 and res > 1e-9
+    om, x, it, res, b = O.poisson_laplace(nodes, hexes, -1.0, tol=1e-9, relative=False)
+    assert it > 0 and res <= 1e-9
+
+
+def test_gradient_export_exact_for_linear_field():
+    nodes, _, hexes = box_hexes(jitter=0.06)
+    m = O.Mesh(nodes, hexes, 8, 2, 4, 2)
+    a = np.array([0.4, -0.2, 1.1])
+    g = m.export_solution_grad(m.x @ a)
+    assert np.abs(g + a).max() < 1e-12
+
+
+# ------------------------------------------------------------------------------------------
+# interpolation
+# ------------------------------------------------------------------------------------------
+def test_nodal_field_exact_for_linear_potential():
+    """Interpolator::store_elfield averages exact hex-corner gradients: -grad of a linear
+    potential at every node of the region."""
+    m = M.make_tip_mesh(M.small_geometry(), with_atoms=False)
+    a = np.array([0.2, 0.5, -1.0])
+    E = O.nodal_field(m.nodes, m.hexes, m.hex_marker, 1, m.nodes @ a)
+    used = np.zeros(len(m.nodes), bool)
+    used[m.hexes[m.hex_marker > 0]] = True
+    assert np.abs(E[used] + a).max() < 1e-10
+    assert np.all(E[~used] == 0)
+
+
+def test_linhex_shape_functions_invert_trilinear_map():
+    rng = np.random.default_rng(5)
+    ucd = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                    [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=np.float64)
+    xe = 2.0 * ucd + rng.uniform(-0.2, 0.2, (8, 3))
+    for _ in range(20):
+        uvw = rng.uniform(-0.9, 0.9, 3)
+        N = np.prod(1 + ucd * uvw, axis=1) / 8
+        p = N @ xe
+        sf = O.linhex_shape_functions(xe, p)
+        assert np.abs(sf - N).max() < 1e-12
+    for k in range(8):
+        sf = O.linhex_shape_functions(xe, xe[k])
+        assert abs(sf[k] - 1) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def lintet_small():
+    m = M.make_tip_mesh(M.small_geometry(), with_atoms=True)
+    return m, O.LinTet(m.nodes, m.tets, m.tet_nbr, m.tet_marker, 3)
+
+
+def test_barycentric_coordinates(lintet_small):
+    m, L = lintet_small
+    rng = np.random.default_rng(6)
+    for t in rng.integers(0, m.n_tets, 50):
+        v = m.nodes[m.tets[t]]
+        for k in range(4):
+            b = L.bcc(v[k], t)
+            e = np.eye(4)[k] + 1e-15
+            assert np.abs(b - e).max() < 1e-9
+        w = rng.dirichlet(np.ones(4))
+        b = L.bcc(w @ v, t)
+        assert abs(b.sum() - (1 + 4e-15)) < 1e-10 and np.abs(b - w - 1e-15).max() < 1e-9
+
+
+def _brute_containing(L, p):
+    return [c for c in range(L.det0.size) if np.all(L.det0[c] * (L.det[c] @ np.append(p, 1.0)) >= -1e-15)]
+
+
+def test_locate_cell_semantics_against_brute_force(lintet_small):
+    """locate_cell: a containing searchable cell when the guess chain misses, else minus the
+    cell with the nearest centroid (src/InterpolatorCells.cpp:268-307)."""
+    m, L = lintet_small
+    rng = np.random.default_rng(7)
+    lo, hi = m.nodes.min(0), m.nodes.max(0)
+    pts = rng.uniform(lo, hi, (60, 3))
+    cells = L.locate_chain(pts, rank=1)
+    for p, c in zip(pts, cells):
+        inside = _brute_containing(L, p)
+        if c > 0 or (c == 0 and 0 in inside):
+            assert c in inside
+        else:
+            searchable = [k for k in inside if L.marker[k] == 0]
+            assert not searchable
+            d2 = ((L.centroid - p) ** 2).sum(1)
+            assert -c == int(np.argmin(d2))
+
+
+def test_linhex_locate_picks_largest_barycentric(lintet_small):
+    m, L = lintet_small
+    rng = np.random.default_rng(8)
+    vac = np.nonzero(m.tet_marker == 3)[0]
+    for t in rng.choice(vac, 30):
+        w = rng.dirichlet(np.ones(4))
+        p = w @ m.nodes[m.tets[t]]
+        h = L.locate_chain(p[None], rank=3)[0]
+        assert h // 4 == t and h % 4 == int(np.argmax(w))
+
+
+# ------------------------------------------------------------------------------------------
+# physical quantities: the reference's own resistivity table
+# ------------------------------------------------------------------------------------------
+def test_resistivity_table_and_laws():
+    # table nodes (T, rho [Ohm nm]) of bulk Cu, src/PhysicalQuantityData.cpp:47-76
+    assert O.resistivity(300.0) == 10.0 * 1.72303253767393e1
+    assert O.resistivity(1400.0) == 10.0 * 1.01349915510292e2
+    assert O.resistivity(100.0) == O.resistivity(200.0)          # clamped below
+    assert O.resistivity(5000.0) == O.resistivity(1400.0)        # clamped above
+    mid = 0.5 * (1.72303253767393e1 + 2.05822342557969e1) * 10.0
+    assert abs(O.resistivity(325.0) - mid) < 1e-12 * mid          # linear between nodes
+    for T in (250.0, 600.0, 1234.5):
+        assert O.sigma(T) == 1.0 / O.resistivity(T)
+        assert abs(O.kappa(T) - 2.44e-8 * T * O.sigma(T)) <= 1e-18
+
+
+# ------------------------------------------------------------------------------------------
+# the reference's own mesh
+# ------------------------------------------------------------------------------------------
+def test_reference_hemicone_mesh_solves():
+    from femocs_amd import gmsh
+    m = gmsh.tip_mesh_from_msh(os.path.join(HERE, "golden", "hemicone.msh"))
+    assert m.n_tets == 7338 and m.n_hexes == 4 * 7338 and len(m.nodes) == 32885
+    om, x, it, res, _ = O.poisson_laplace(m.nodes, m.vacuum_cells(), -1.0, tol=1e-12, relative=True)
+    assert it > 0
+    # grounded copper, field -1 V/A on the top at z = 500: 0 >= phi >= -500
+    assert x.max() <= 1e-12 and x.min() > -500.0
+    top = om.x[:, 2] > 500 - 1e-9
+    assert np.all(x[top] < -490.0)
