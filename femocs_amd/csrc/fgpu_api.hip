@@ -333,6 +333,7 @@ extern "C" int fgpu_mesh_update_vertices(fgpu_mesh *m, const double *xyz) {
         for (int i = 0; i < m->n_dofs; ++i)
             for (int a = 0; a < 3; ++a) X[4 * (size_t)i + a] = m->x[3 * (size_t)m->dof_of_idof[i] + a];
         upload_coords(m, X);
+        m->coord_gen++;
         return FGPU_OK;
     });
 }
@@ -348,6 +349,24 @@ extern "C" int fgpu_mesh_set_patch_cells(fgpu_mesh *m, int cells) {
         upload_patches(m, gptr);
         return FGPU_OK;
     });
+}
+
+extern "C" int fgpu_mesh_device_bytes(const fgpu_mesh *m, size_t *bytes) {
+    if (!m || !bytes) return FGPU_EINVAL;
+    const size_t nd = m->n_dofs, nc = m->n_cells, ns = m->n_slots, nb = m->bf_cell.size();
+    const Patches &P = m->patches;
+    *bytes = nd * (32 + 16 + 4 + 4 + 4) + nc * 32 + (P.cell0.size() + P.node0.size()) * 4 + P.nodes.size() * 8 +
+             (P.lidx.size() + P.lptr.size() + P.lent.size()) * 2 + nb * 8 + (nd + 1 + 4 * nb) * 4 + ns * 0;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_sys_device_bytes(const fgpu_sys *s, size_t *bytes) {
+    if (!s || !bytes) return FGPU_EINVAL;
+    const fgpu_mesh *m = static_cast<const fgpu_mesh *>(s->mesh);
+    const size_t nd = m->n_dofs, ns = m->n_slots, nb = m->bf_cell.size(), nc = m->n_cells;
+    *bytes = nd * (11 * 8 + 1 + 6 * 4) + ns * (8 + 4) + nb * 5 * 8 + 8 * (size_t)s->nb_dof * 8 + 2 * kScal * 8 +
+             (s->kq ? nc * 8 * (8 + 8 + 4 + 4) : 0);
+    return FGPU_OK;
 }
 
 extern "C" int fgpu_mesh_patch_info(const fgpu_mesh *m, int *cells, int *n_patches, int *node_cap, int *n_slots) {
@@ -553,7 +572,8 @@ static size_t patch_lds(int node_cap, int cells, size_t tsize) {
     return (4 * (size_t)node_cap + 8 * (size_t)cells) * tsize + 16 * (size_t)cells;
 }
 
-static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> &sets, const std::vector<fgpu_sys *> &sys) {
+static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> &sets, const std::vector<fgpu_sys *> &sys,
+                              void **own_buf = nullptr, size_t *own_cap = nullptr) {
     std::vector<int2> et, dt;
     for (size_t i = 0; i < sys.size(); ++i) {
         append_table(et, (int)i, sys[i]->nb_cell);
@@ -569,7 +589,20 @@ static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> 
     for (size_t k = 0; k < sets.size(); ++k) std::memcpy(host.data() + k * n * sizeof(Desc), sets[k].data(), n * sizeof(Desc));
     std::memcpy(host.data() + off_e, et.data(), et.size() * sizeof(int2));
     std::memcpy(host.data() + off_d, dt.data(), dt.size() * sizeof(int2));
-    char *dev = static_cast<char *>(ensure_tab(c, total));
+    char *dev;
+    if (own_buf) {
+        if (total > *own_cap) {
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            if (*own_buf) HIPCHECK(hipFree(*own_buf));
+            *own_buf = nullptr;
+            *own_cap = 0;
+            HIPCHECK(hipMalloc(own_buf, 2 * total));
+            *own_cap = 2 * total;
+        }
+        dev = static_cast<char *>(*own_buf);
+    } else {
+        dev = static_cast<char *>(ensure_tab(c, total));
+    }
     HIPCHECK(hipMemcpyAsync(dev, host.data(), total, hipMemcpyHostToDevice, c->stream));
     // the host buffer dies at return: make the copy complete first
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -588,7 +621,7 @@ static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> 
 }
 
 static void ensure_diag(fgpu_sys *s) {
-    if (s->diag_valid) return;
+    if (s->diag_valid && s->diag_gen == static_cast<fgpu_mesh *>(s->mesh)->coord_gen) return;
     fgpu_ctx *c = s->ctx;
     fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
     Desc d = make_desc64(s, true);
@@ -599,6 +632,7 @@ static void ensure_diag(fgpu_sys *s) {
     launch_invdiag(c->stream, m->n_dofs, s->d_dmask, s->diag, s->invd, s->invd32);
     HIPCHECK(hipStreamSynchronize(c->stream));
     s->diag_valid = true;
+    s->diag_gen = m->coord_gen;
 }
 
 extern "C" int fgpu_sys_set_coefficients(fgpu_sys *s, const double *kq, const double *gq) {
@@ -914,9 +948,7 @@ static void chunk(hipStream_t st, const ChunkArgs &a) {
     }
 }
 
-static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_solve_opts *o, fgpu_solve_stats *stats) {
-    HIPCHECK(hipSetDevice(c->device));
-    std::vector<fgpu_sys *> sys(systems, systems + n);
+static bool batch_coef(const std::vector<fgpu_sys *> &sys, fgpu_ctx *c) {
     bool coef = false;
     for (auto *s : sys) {
         if (!s || s->ctx != c) throw std::invalid_argument("system belongs to another context");
@@ -924,11 +956,26 @@ static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_
     }
     for (auto *s : sys)
         if (s->have_coef != coef) throw std::invalid_argument("a batch mixes coefficient and plain systems");
+    return coef;
+}
+
+static void make_sets(const std::vector<fgpu_sys *> &sys, std::vector<std::vector<Desc>> &sets) {
+    sets.assign(3, {});
+    for (auto *s : sys) {
+        sets[0].push_back(make_desc64(s, false));
+        sets[1].push_back(make_desc64(s, true));
+        sets[2].push_back(make_desc32(s));
+    }
+}
+
+// Starts the solves of a set of systems: scalars, x = g on constrained dofs, FP64 residual of
+// the lifted system and (mixed) the FP32 CG state.  t: the set's uploaded tables.
+static void start_solves(fgpu_ctx *c, const std::vector<fgpu_sys *> &sys, const fgpu_solve_opts *o, const BatchTabs &t,
+                         bool coef) {
+    const int n = (int)sys.size();
     const bool mixed = o->precision == FGPU_MIXED;
-    const int k = std::max(2, o->check_every > 0 ? o->check_every : 32);
     const int max_it = std::max(0, o->max_it);
     const double delta = o->inner_tol > 0 && o->inner_tol < 1 ? o->inner_tol : 0.1;
-    // initial scalars: [FP64 | FP32] per system
     std::vector<double> h((size_t)2 * kScal * n, 0.0);
     for (int i = 0; i < n; ++i) {
         double *s64 = &h[(size_t)2 * kScal * i], *s32 = s64 + kScal;
@@ -941,35 +988,50 @@ static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_
     }
     for (int i = 0; i < n; ++i) {
         fgpu_sys *s = sys[i];
-        ensure_diag(s);
         h2d(s->scal, &h[(size_t)2 * kScal * i], 2 * kScal, c->stream);
         fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
         launch_enforce_dirichlet(c->stream, m->n_dofs, s->d_dmask, s->g, s->x);
     }
-    std::vector<Desc> cg64, res64, cg32;
-    for (auto *s : sys) {
-        cg64.push_back(make_desc64(s, false));
-        res64.push_back(make_desc64(s, true));
-        cg32.push_back(make_desc32(s));
-    }
-    BatchTabs t = upload_batch(c, {cg64, res64, cg32}, sys);  // synchronises: h is consumed
-    double *dflags = nullptr;
-    HIPCHECK(hipMalloc(&dflags, std::max(1, n) * sizeof(double)));
-    const Desc *D64 = t.sets[0], *R64 = t.sets[1], *D32 = t.sets[2];
+    const Desc *R64 = t.sets[1], *D32 = t.sets[2];
     fp64_residual(c->stream, R64, D32, t, n, coef, mixed ? 1 : 0);
     if (mixed) {
         launch_mixed_restart(c->stream, D32, R64, t.dtab, t.nd, 1);
         launch_scal_restart(c->stream, D32, R64, n, 1);
     }
-    ChunkArgs a{mixed, coef, k, 8, n, D64, R64, D32, &t};
+    HIPCHECK(hipStreamSynchronize(c->stream));  // h is consumed
+}
+
+static void read_stats(fgpu_sys *s, bool mixed, fgpu_solve_stats *st) {
+    double hs[2 * kScal];
+    HIPCHECK(hipMemcpy(hs, s->scal, sizeof(hs), hipMemcpyDeviceToHost));
+    const double *s64 = hs, *s32 = hs + kScal;
+    const int it = (int)(mixed ? s32[S_IT] : s64[S_IT]);
+    st->iterations = s64[S_CONV] != 0.0 ? it : -std::max(it, 1);
+    st->outer = mixed ? (int)s32[S_NUPD] + 1 : 1;
+    st->residual = std::sqrt(std::max(0.0, s64[S_RR]));
+    st->rhs_norm = std::sqrt(std::max(0.0, s64[S_BB]));
+}
+
+static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_solve_opts *o, fgpu_solve_stats *stats) {
+    HIPCHECK(hipSetDevice(c->device));
+    std::vector<fgpu_sys *> sys(systems, systems + n);
+    const bool coef = batch_coef(sys, c);
+    const bool mixed = o->precision == FGPU_MIXED;
+    const int k = std::max(2, o->check_every > 0 ? o->check_every : 32);
+    const int max_it = std::max(0, o->max_it);
+    for (auto *s : sys) ensure_diag(s);
+    std::vector<std::vector<Desc>> sets;
+    make_sets(sys, sets);
+    BatchTabs t = upload_batch(c, sets, sys);
+    start_solves(c, sys, o, t, coef);
+    double *dflags = nullptr;
+    HIPCHECK(hipMalloc(&dflags, std::max(1, n) * sizeof(double)));
+    ChunkArgs a{mixed, coef, k, 8, n, t.sets[0], t.sets[1], t.sets[2], &t};
     // a system waiting for its reliable update idles at most u iterations per update
     const int max_chunks = 2 * (max_it / k) + 16;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    const int slot = mixed ? S_DONE : S_ACTIVE;
-    const double want = mixed ? 0.0 : 1.0;
-    const Desc *Dflag = mixed ? R64 : D64;
-    bool running = any_flag(c, Dflag, n, slot, want, dflags);
+    bool running = any_flag(c, t.sets[1], n, S_DONE, 0.0, dflags);
     for (int ch = 0; running && ch < max_chunks; ++ch) {
         if (o->use_graph) {
             if (!exec) {
@@ -982,25 +1044,163 @@ static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_
         } else {
             chunk(c->stream, a);
         }
-        running = any_flag(c, Dflag, n, slot, want, dflags);
+        running = any_flag(c, t.sets[1], n, S_DONE, 0.0, dflags);
     }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     HIPCHECK(hipStreamSynchronize(c->stream));
     HIPCHECK(hipFree(dflags));
-    for (int i = 0; i < n; ++i) {
-        double hs[2 * kScal];
-        HIPCHECK(hipMemcpy(hs, sys[i]->scal, sizeof(hs), hipMemcpyDeviceToHost));
-        const double *s64 = hs, *s32 = hs + kScal;
-        if (stats) {
-            const int it = (int)(mixed ? s32[S_IT] : s64[S_IT]);
-            stats[i].iterations = s64[S_CONV] != 0.0 ? it : -std::max(it, 1);
-            stats[i].outer = mixed ? (int)s32[S_NUPD] + 1 : 1;
-            stats[i].residual = std::sqrt(std::max(0.0, s64[S_RR]));
-            stats[i].rhs_norm = std::sqrt(std::max(0.0, s64[S_BB]));
-        }
-    }
+    if (stats)
+        for (int i = 0; i < n; ++i) read_stats(sys[i], mixed, &stats[i]);
     return FGPU_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// continuous batching: systems join and leave a running batch between steps
+// ------------------------------------------------------------------------------------------
+struct fgpu_batch {
+    fgpu_ctx *ctx = nullptr;
+    fgpu_solve_opts o{};
+    std::vector<fgpu_sys *> sys;
+    int coef = -1;                // -1: no member yet
+    bool dirty = true;            // membership changed since the tables were uploaded
+    void *dtab = nullptr;
+    size_t cap = 0;
+    BatchTabs t;
+    double *dflags = nullptr;
+    int flags_cap = 0;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_k = 0;
+};
+
+static void batch_drop_graph(fgpu_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+extern "C" int fgpu_batch_create(fgpu_ctx *c, const fgpu_solve_opts *o, fgpu_batch **out) {
+    if (!c || !o || !out) return FGPU_EINVAL;
+    fgpu_batch *b = new (std::nothrow) fgpu_batch();
+    if (!b) return FGPU_ENOMEM;
+    b->ctx = c;
+    b->o = *o;
+    c->refs++;
+    *out = b;
+    return FGPU_OK;
+}
+
+extern "C" void fgpu_batch_destroy(fgpu_batch *b) {
+    if (!b) return;
+    fgpu_ctx *c = b->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    batch_drop_graph(b);
+    if (b->dtab) (void)hipFree(b->dtab);
+    if (b->dflags) (void)hipFree(b->dflags);
+    delete b;
+    ctx_release(c);
+}
+
+extern "C" int fgpu_batch_size(const fgpu_batch *b) { return b ? (int)b->sys.size() : 0; }
+
+extern "C" int fgpu_batch_add(fgpu_batch *b, fgpu_sys *s) {
+    if (!b || !s) return FGPU_EINVAL;
+    fgpu_ctx *c = b->ctx;
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        for (auto *x : b->sys)
+            if (x == s) throw std::invalid_argument("system already in the batch");
+        const bool coef = batch_coef({s}, c);
+        if (b->coef >= 0 && (int)coef != b->coef) throw std::invalid_argument("a batch mixes coefficient and plain systems");
+        ensure_diag(s);
+        std::vector<std::vector<Desc>> sets;
+        make_sets({s}, sets);
+        BatchTabs t = upload_batch(c, sets, {s});
+        start_solves(c, {s}, &b->o, t, coef);
+        b->coef = coef;
+        b->sys.push_back(s);
+        b->dirty = true;
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_batch_step(fgpu_batch *b, int iterations, int *n_running) {
+    if (!b || iterations <= 0) return FGPU_EINVAL;
+    fgpu_ctx *c = b->ctx;
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        const int n = (int)b->sys.size();
+        if (n_running) *n_running = 0;
+        if (n == 0) return FGPU_OK;
+        const bool mixed = b->o.precision == FGPU_MIXED;
+        const int k = mixed ? (iterations + 7) / 8 * 8 : iterations;
+        if (b->dirty) {
+            batch_drop_graph(b);
+            std::vector<std::vector<Desc>> sets;
+            make_sets(b->sys, sets);
+            b->t = upload_batch(c, sets, b->sys, &b->dtab, &b->cap);
+            if (n > b->flags_cap) {
+                if (b->dflags) HIPCHECK(hipFree(b->dflags));
+                b->dflags = nullptr;
+                HIPCHECK(hipMalloc(&b->dflags, 2 * n * sizeof(double)));
+                b->flags_cap = 2 * n;
+            }
+            b->dirty = false;
+        }
+        ChunkArgs a{mixed, b->coef > 0, k, 8, n, b->t.sets[0], b->t.sets[1], b->t.sets[2], &b->t};
+        if (b->o.use_graph) {
+            if (!b->exec || b->graph_k != k) {
+                batch_drop_graph(b);
+                HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                chunk(c->stream, a);
+                HIPCHECK(hipStreamEndCapture(c->stream, &b->graph));
+                HIPCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+                b->graph_k = k;
+            }
+            HIPCHECK(hipGraphLaunch(b->exec, c->stream));
+        } else {
+            chunk(c->stream, a);
+        }
+        double *h = flags_buffer(c, n);
+        launch_collect(c->stream, b->t.sets[1], n, S_DONE, b->dflags);
+        HIPCHECK(hipMemcpyAsync(h, b->dflags, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        int running = 0;
+        for (int i = 0; i < n; ++i) running += h[i] == 0.0;
+        if (n_running) *n_running = running;
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_batch_retire(fgpu_batch *b, int max, fgpu_sys **out, fgpu_solve_stats *stats, int *n_out) {
+    if (!b || !n_out || (max > 0 && !out)) return FGPU_EINVAL;
+    fgpu_ctx *c = b->ctx;
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        *n_out = 0;
+        std::vector<fgpu_sys *> keep;
+        const bool mixed = b->o.precision == FGPU_MIXED;
+        for (auto *s : b->sys) {
+            double done = 0;
+            HIPCHECK(hipMemcpy(&done, s->scal + S_DONE, sizeof(double), hipMemcpyDeviceToHost));
+            if (done != 0.0 && *n_out < max) {
+                out[*n_out] = s;
+                if (stats) read_stats(s, mixed, &stats[*n_out]);
+                ++*n_out;
+            } else {
+                keep.push_back(s);
+            }
+        }
+        if (keep.size() != b->sys.size()) {
+            b->sys.swap(keep);
+            b->dirty = true;
+            if (b->sys.empty()) b->coef = -1;
+        }
+        return FGPU_OK;
+    });
 }
 
 extern "C" int fgpu_solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_solve_opts *o, fgpu_solve_stats *stats) {

@@ -102,6 +102,7 @@ struct Mesh {
     Patches patches;
     int n_slots = 0;                       // slot-buffer entries (sum of patch node counts)
     int patch_gen = 0;                     // bumped whenever the patches are rebuilt
+    int coord_gen = 0;                     // bumped whenever the vertices move
     std::vector<int> icells;               // 8*n_cells, internal order and dofs (host copy)
     // device arrays (internal numbering)
     int *d_cells = nullptr;                // 8*n_cells
@@ -138,6 +139,7 @@ struct Sys {
     int nb_cell = 0, nb_dof = 0;
     int ppb = kPatchPerBlock;     // patches per element workgroup (tunable)
     int patch_gen = -1;           // mesh patch generation the slot buffers are sized for
+    int diag_gen = -1;            // mesh coordinate generation the diagonal was computed for
     ~Sys();
 };
 

@@ -76,6 +76,14 @@ API = [
     ("fgpu_mesh_patch_info", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),
     ("fgpu_sys_set_patches_per_block", C.c_int, [_P, C.c_int]),
+    ("fgpu_mesh_device_bytes", C.c_int, [_P, C.POINTER(C.c_size_t)]),
+    ("fgpu_sys_device_bytes", C.c_int, [_P, C.POINTER(C.c_size_t)]),
+    ("fgpu_batch_create", C.c_int, [_P, C.POINTER(SolveOpts), C.POINTER(_P)]),
+    ("fgpu_batch_destroy", None, [_P]),
+    ("fgpu_batch_size", C.c_int, [_P]),
+    ("fgpu_batch_add", C.c_int, [_P, _P]),
+    ("fgpu_batch_step", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    ("fgpu_batch_retire", C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(SolveStats), C.POINTER(C.c_int)]),
 ]
 
 
@@ -175,6 +183,11 @@ class Mesh:
         self.ctx.lib.fgpu_mesh_export_face_centroids(self.h, bid, out.ctypes.data, C.byref(n))
         return out.reshape(-1, 3)
 
+    def device_bytes(self) -> int:
+        b = C.c_size_t()
+        self.ctx.lib.fgpu_mesh_device_bytes(self.h, C.byref(b))
+        return b.value
+
     def set_patch_cells(self, cells: int):
         self.ctx.check(self.ctx.lib.fgpu_mesh_set_patch_cells(self.h, int(cells)), "fgpu_mesh_set_patch_cells")
 
@@ -259,6 +272,11 @@ class System:
         self._c(self.ctx.lib.fgpu_sys_minmax(self.h, C.byref(a), C.byref(b)), "minmax")
         return a.value, b.value
 
+    def device_bytes(self) -> int:
+        b = C.c_size_t()
+        self.ctx.lib.fgpu_sys_device_bytes(self.h, C.byref(b))
+        return b.value
+
     def set_patches_per_block(self, ppb: int):
         self._c(self.ctx.lib.fgpu_sys_set_patches_per_block(self.h, int(ppb)), "set_patches_per_block")
 
@@ -271,6 +289,50 @@ class System:
         g = np.zeros(3 * self.mesh.n_dofs)
         self._c(self.ctx.lib.fgpu_sys_solution_grad(self.h, g), "solution_grad")
         return g.reshape(-1, 3)
+
+
+class Batch:
+    """Continuous batching: systems join a running batch (their solve starts at once) and
+    leave when finished; every step runs `iterations` CG iterations of all running members."""
+
+    def __init__(self, ctx: Context, o: SolveOpts):
+        self.ctx = ctx
+        h = _P()
+        ctx.check(ctx.lib.fgpu_batch_create(ctx.h, C.byref(o), C.byref(h)), "fgpu_batch_create")
+        self.h = h
+        self.members = {}
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.fgpu_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def __len__(self):
+        return self.ctx.lib.fgpu_batch_size(self.h)
+
+    def add(self, system: "System"):
+        self.ctx.check(self.ctx.lib.fgpu_batch_add(self.h, system.h), "fgpu_batch_add")
+        self.members[system.h.value] = system
+
+    def step(self, iterations: int = 32) -> int:
+        n = C.c_int()
+        self.ctx.check(self.ctx.lib.fgpu_batch_step(self.h, int(iterations), C.byref(n)), "fgpu_batch_step")
+        return n.value
+
+    def retire(self):
+        """Removes the finished members; returns [(System, SolveStats)]."""
+        cap = max(1, len(self.members))
+        out = (_P * cap)()
+        st = (SolveStats * cap)()
+        n = C.c_int()
+        self.ctx.check(self.ctx.lib.fgpu_batch_retire(self.h, cap, out, st, C.byref(n)), "fgpu_batch_retire")
+        res = []
+        for i in range(n.value):
+            res.append((self.members.pop(out[i]), st[i]))
+        return res
 
 
 def opts(max_it=10000, tol=1e-9, relative=True, precond=FGPU_PC_JACOBI, precision=FGPU_FP64, inner_tol=1e-4,

@@ -87,6 +87,10 @@ int fgpu_mesh_export_bfaces(const fgpu_mesh *mesh, int *cell, int *face, int *bi
 int fgpu_mesh_export_face_centroids(const fgpu_mesh *mesh, int bid, double *xyz, int *n);
 /* moved vertices, same topology (3*n_vertices of the import numbering) */
 int fgpu_mesh_update_vertices(fgpu_mesh *mesh, const double *xyz);
+/* device memory held by a mesh / a system (bytes), for admission by memory budget; no
+ * counterpart in the reference */
+int fgpu_mesh_device_bytes(const fgpu_mesh *mesh, size_t *bytes);
+int fgpu_sys_device_bytes(const fgpu_sys *sys, size_t *bytes);
 
 /* Tuning, no counterpart in the reference: the element kernel works on patches of
  * consecutive cells whose nodes are staged in LDS; cells per patch (8..256) and patches per
@@ -133,6 +137,19 @@ int fgpu_sys_solve(fgpu_sys *sys, const fgpu_solve_opts *opts, fgpu_solve_stats 
 /* many systems in one set of launches; stats: one per system */
 int fgpu_solve_batch(fgpu_ctx *ctx, int n, fgpu_sys *const *systems, const fgpu_solve_opts *opts,
                      fgpu_solve_stats *stats);
+/* Continuous batching, no counterpart in the reference (it solves one system at a time):
+ * callers that serve many independent solves keep one batch running and let systems join
+ * (fgpu_batch_add: the solve of that system starts at once) and leave (fgpu_batch_retire: the
+ * finished ones, with their stats) between steps (fgpu_batch_step: `iterations` CG iterations of
+ * every running member; *n_running: members not finished).  A system's result is the same
+ * as when solved alone.  Each member keeps the options given at creation. */
+typedef struct fgpu_batch fgpu_batch;
+int fgpu_batch_create(fgpu_ctx *ctx, const fgpu_solve_opts *opts, fgpu_batch **out);
+void fgpu_batch_destroy(fgpu_batch *batch);
+int fgpu_batch_size(const fgpu_batch *batch);
+int fgpu_batch_add(fgpu_batch *batch, fgpu_sys *sys);
+int fgpu_batch_step(fgpu_batch *batch, int iterations, int *n_running);
+int fgpu_batch_retire(fgpu_batch *batch, int max, fgpu_sys **out, fgpu_solve_stats *stats, int *n_out);
 /* DealSolver::check_limits / stat (src/DealSolver.cpp:158-167) */
 int fgpu_sys_minmax(fgpu_sys *sys, double *min, double *max);
 /* DealSolver::export_solution_grad (src/DealSolver.cpp:281-301): -grad u at the Gauss point

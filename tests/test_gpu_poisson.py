@@ -118,3 +118,59 @@ def test_context_closed_before_children(small_mesh):
     st = s.solve(fgpu.opts(tol=1e-8, relative=True))
     assert st.iterations > 0
     del s                          # the last child frees the context
+
+
+@pytest.mark.parametrize("precision", [fgpu.FGPU_FP64, fgpu.FGPU_MIXED])
+def test_continuous_batching_joins_and_leaves(ctx, three_meshes, precision):
+    """Systems that join a running batch at different steps finish with the result and the
+    iteration count they have alone."""
+    o = fgpu.opts(tol=1e-10, relative=True, precision=precision, use_graph=False)
+    alone = []
+    for i, m in enumerate(three_meshes):
+        s = _poisson_system(ctx, m, field=-1.0 - 0.1 * i)
+        st = s.solve(o)
+        alone.append((s.solution(), st.iterations))
+    b = fgpu.Batch(ctx, o)
+    systems = [_poisson_system(ctx, m, field=-1.0 - 0.1 * i) for i, m in enumerate(three_meshes)]
+    done = {}
+    b.add(systems[0])
+    for step in range(400):
+        if step == 3:
+            b.add(systems[1])
+        if step == 7:
+            b.add(systems[2])
+        b.step(16)
+        for s, st in b.retire():
+            done[systems.index(s)] = (s.solution(), st.iterations)
+        if len(done) == 3:
+            break
+    assert len(done) == 3 and len(b) == 0
+    for i in range(3):
+        assert done[i][1] == alone[i][1]
+        assert np.array_equal(done[i][0], alone[i][0])
+
+
+def test_moved_vertices_match_fresh_import(ctx, small_mesh):
+    """After a vertex update (same topology) the operator, the Jacobi diagonal and the solution
+    follow the new coordinates: the result matches the oracle on the moved mesh."""
+    s = _poisson_system(ctx, small_mesh)
+    o = fgpu.opts(tol=1e-12, relative=True)
+    assert s.solve(o).iterations > 0
+    X = small_mesh.nodes.copy()
+    lo, hi = X.min(axis=0), X.max(axis=0)
+    # the box faces stay planes, so the boundary ids read at import still hold
+    X[:, 0] += 0.3 * np.sin(np.pi * (X[:, 0] - lo[0]) / (hi[0] - lo[0])) * np.cos(X[:, 2] / 17.0)
+    X[:, 2] *= 1.02
+    s.mesh.update_vertices(X)
+    s.reset()
+    s.add_dirichlet(2, 0.0)
+    s.add_neumann(8, None, -1.0)
+    s.apply_dirichlet()
+    s.fill_solution(0.0)
+    st = s.solve(o)
+    assert st.iterations > 0
+    _, xo, it, _, _ = O.poisson_laplace(X, small_mesh.vacuum_cells(), -1.0, tol=1e-13,
+                                        relative=True, precond="ssor")
+    assert it > 0
+    x = s.solution()
+    assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
