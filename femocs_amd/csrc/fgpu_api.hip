@@ -27,6 +27,10 @@ struct HipError {
         if (e_ != hipSuccess) throw HipError{e_, #call};      \
     } while (0)
 
+struct MeshError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
 template <class F> int guarded(fgpu_ctx *ctx, F &&f) {
     try {
         return f();
@@ -38,6 +42,9 @@ template <class F> int guarded(fgpu_ctx *ctx, F &&f) {
     } catch (const std::bad_alloc &) {
         if (ctx) ctx->err = "host allocation failed";
         return FGPU_ENOMEM;
+    } catch (const MeshError &ex) {
+        if (ctx) ctx->err = ex.what();
+        return FGPU_EMESH;
     } catch (const std::exception &ex) {
         if (ctx) ctx->err = ex.what();
         return FGPU_EINVAL;
@@ -327,11 +334,33 @@ extern "C" int fgpu_mesh_update_vertices(fgpu_mesh *m, const double *xyz) {
     if (!m || !xyz) return FGPU_EINVAL;
     return guarded(m->ctx, [&] {
         HIPCHECK(hipSetDevice(m->ctx->device));
+        std::vector<double> X(4 * (size_t)m->n_dofs, 0.0);
+        for (int i = 0; i < m->n_dofs; ++i) {
+            const size_t v = m->vertex_of_dof[m->dof_of_idof[i]];
+            for (int a = 0; a < 3; ++a) X[4 * (size_t)i + a] = xyz[3 * v + a];
+        }
+        h2d(m->d_X64, X.data(), X.size(), m->ctx->stream);
+        // the import fixed each cell's orientation by the sign of its centre Jacobian; a move
+        // that flips one would change the operator the import semantics define
+        const int nb = center_det_blocks(m->n_cells);
+        double *dmin = dalloc<double>(nb);
+        launch_center_det_min(m->ctx->stream, m->n_cells, m->d_cells, m->d_X64, dmin);
+        std::vector<double> bmin(nb);
+        d2h(bmin.data(), dmin, nb, m->ctx->stream);
+        HIPCHECK(hipStreamSynchronize(m->ctx->stream));
+        dfree(dmin);
+        double mn = 1e300;
+        for (double v : bmin) mn = std::min(mn, v);
+        if (!(mn >= 0.0)) {
+            std::vector<double> Xold(4 * (size_t)m->n_dofs, 0.0);
+            for (int i = 0; i < m->n_dofs; ++i)
+                for (int a = 0; a < 3; ++a) Xold[4 * (size_t)i + a] = m->x[3 * (size_t)m->dof_of_idof[i] + a];
+            upload_coords(m, Xold);
+            throw MeshError("moved vertices turn a cell inside out (centre Jacobian " + std::to_string(mn) +
+                            "); coordinates left unchanged, the mesh must be rebuilt");
+        }
         for (int d = 0; d < m->n_dofs; ++d)
             for (int a = 0; a < 3; ++a) m->x[3 * (size_t)d + a] = xyz[3 * (size_t)m->vertex_of_dof[d] + a];
-        std::vector<double> X(4 * (size_t)m->n_dofs, 0.0);
-        for (int i = 0; i < m->n_dofs; ++i)
-            for (int a = 0; a < 3; ++a) X[4 * (size_t)i + a] = m->x[3 * (size_t)m->dof_of_idof[i] + a];
         upload_coords(m, X);
         m->coord_gen++;
         return FGPU_OK;

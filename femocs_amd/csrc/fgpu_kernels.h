@@ -28,6 +28,8 @@ void launch_face_load(hipStream_t s, int nb, const int *bf_icell, const int *bf_
                       const double *fval, const int *cells, const double *X, double *fslot);
 void launch_face_gather(hipStream_t s, int n, const int *fgptr, const int *fgidx, const double *fslot, double *b);
 void launch_minmax(hipStream_t s, int n, const double *x, double *out);
+int center_det_blocks(int n);
+void launch_center_det_min(hipStream_t s, int n, const int *cells, const double *X, double *out);
 void launch_grad_export(hipStream_t s, int n, const int *v2c, const int *v2n, const int *cells, const double *X,
                         const double *u, double *grad);
 }  // namespace fgpu

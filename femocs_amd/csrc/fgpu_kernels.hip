@@ -470,6 +470,43 @@ __global__ __launch_bounds__(kBlock) void k_minmax(int n, const double *__restri
     }
 }
 
+// per block: the smallest Jacobian determinant at the cell centres of its cells, in the
+// orientation fixed at import (same formula as the import's orientation test)
+__global__ __launch_bounds__(kBlock) void k_center_det_min(int n, const int *__restrict__ cells, const double4 *__restrict__ X,
+                                                           double *__restrict__ out) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double v = 1e300;
+    if (c < n) {
+        double p[8][3];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double4 q = X[cells[8 * (size_t)c + k]];
+            p[k][0] = q.x;
+            p[k][1] = q.y;
+            p[k][2] = q.z;
+        }
+        double J[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            J[a][0] = 0.25 * ((p[1][a] - p[0][a]) + (p[3][a] - p[2][a]) + (p[5][a] - p[4][a]) + (p[7][a] - p[6][a]));
+            J[a][1] = 0.25 * ((p[2][a] - p[0][a]) + (p[3][a] - p[1][a]) + (p[6][a] - p[4][a]) + (p[7][a] - p[5][a]));
+            J[a][2] = 0.25 * ((p[4][a] - p[0][a]) + (p[5][a] - p[1][a]) + (p[6][a] - p[2][a]) + (p[7][a] - p[3][a]));
+        }
+        v = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    __shared__ double s[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; ++k) v = fmin(v, s[k]);
+        v = fmin(v, s[0]);
+        out[blockIdx.x] = v;
+    }
+}
+
 // -grad u at the Gauss point v2n of cell v2c (DealSolver::export_solution_grad)
 __global__ void k_grad_export(int n, const int *__restrict__ v2c, const int *__restrict__ v2n, const int *__restrict__ cells,
                               const double4 *__restrict__ X, const double *__restrict__ u, double *__restrict__ grad) {
@@ -574,6 +611,12 @@ void launch_face_gather(hipStream_t s, int n, const int *fgptr, const int *fgidx
 }
 void launch_minmax(hipStream_t s, int n, const double *x, double *out) {
     hipLaunchKernelGGL(k_minmax, dim3(1), dim3(kBlock), 0, s, n, x, out);
+}
+int center_det_blocks(int n) { return grid_for(n); }
+void launch_center_det_min(hipStream_t s, int n, const int *cells, const double *X, double *out) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_center_det_min, dim3(grid_for(n)), dim3(kBlock), 0, s, n, cells,
+                           reinterpret_cast<const double4 *>(X), out);
 }
 void launch_grad_export(hipStream_t s, int n, const int *v2c, const int *v2n, const int *cells, const double *X, const double *u,
                         double *grad) {

@@ -20,8 +20,17 @@ BID_COPPER_SURFACE, BID_VACUUM_TOP, BID_COPPER_BOTTOM, BID_SIDES = 2, 8, 7, 4
 _lib = None
 
 
+FGPU_EMESH = -5
+
+
 class FgpuError(RuntimeError):
-    pass
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
+
+
+class MeshInvertedError(FgpuError):
+    """A vertex update would turn a cell inside out; the mesh is left as it was."""
 
 
 class SolveOpts(C.Structure):
@@ -118,7 +127,8 @@ class Context:
     def check(self, rc, what):
         if rc != 0:
             msg = self.lib.fgpu_ctx_error(self.h)
-            raise FgpuError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+            cls = MeshInvertedError if rc == FGPU_EMESH else FgpuError
+            raise cls(f"{what} failed ({rc}): {msg.decode() if msg else ''}", rc)
 
     def sync(self):
         self.check(self.lib.fgpu_ctx_sync(self.h), "sync")

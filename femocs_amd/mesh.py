@@ -268,25 +268,51 @@ class TipMesh:
         return len(self.hexes)
 
 
+_PAIRS = np.array([[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3]])
+
+
+def _tet_entities(nv: int, tets: np.ndarray):
+    """Unique edges and faces of a tet mesh in sorted-key order: (tet_edges [T,6], edge vertices
+    [E,2], tet_faces [T,4] (face opposite local vertex k), face vertices [F,3])."""
+    T = len(tets)
+    e = np.sort(tets[:, _PAIRS], axis=2).reshape(-1, 2).astype(np.int64)
+    ekey = e[:, 0] * nv + e[:, 1]
+    uek, einv = np.unique(ekey, return_inverse=True)
+    opp = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]])
+    f = np.sort(tets[:, opp], axis=2).reshape(-1, 3).astype(np.int64)
+    fkey = (f[:, 0] * nv + f[:, 1]) * nv + f[:, 2]
+    ufk, finv = np.unique(fkey, return_inverse=True)
+    ev = np.stack([uek // nv, uek % nv], 1)
+    fv = np.stack([ufk // (nv * nv), (ufk // nv) % nv, ufk % nv], 1)
+    return einv.reshape(T, 6), ev, finv.reshape(T, 4), fv
+
+
+def hex_node_weights(n_tetnodes: int, tets: np.ndarray):
+    """Sparse map from tet-vertex positions to all hex nodes (vertices, edge midpoints, face and
+    tet centroids, in the order tet_to_hex emits them): moving tet vertices by D moves the hex
+    nodes by W @ D, as Tethex set_new_vertices places them (src/Tethex.cpp:965-1102)."""
+    from scipy.sparse import csr_matrix
+
+    _, ev, _, fv = _tet_entities(n_tetnodes, tets)
+    T, E, F = len(tets), len(ev), len(fv)
+    rows = np.concatenate([np.arange(n_tetnodes), np.repeat(n_tetnodes + np.arange(E), 2),
+                           np.repeat(n_tetnodes + E + np.arange(F), 3),
+                           np.repeat(n_tetnodes + E + F + np.arange(T), 4)])
+    cols = np.concatenate([np.arange(n_tetnodes), ev.ravel(), fv.ravel(), np.asarray(tets).ravel()])
+    vals = np.concatenate([np.ones(n_tetnodes), np.full(2 * E, 0.5), np.full(3 * F, 1.0 / 3.0),
+                           np.full(4 * T, 0.25)])
+    n = n_tetnodes + E + F + T
+    return csr_matrix((vals, (rows, cols)), shape=(n, n_tetnodes))
+
+
 def tet_to_hex(nodes: np.ndarray, tets: np.ndarray):
     """Split every tet into four hexes around its vertices (Tethex convert_3D,
     src/Tethex.cpp:965-1102).  Returns (all_nodes, node_kind, hexes_ucd)."""
     T = len(tets)
     nv = len(nodes)
-    pairs = np.array([[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3]])
-    e = np.sort(tets[:, pairs], axis=2).reshape(-1, 2).astype(np.int64)
-    ekey = e[:, 0] * nv + e[:, 1]
-    uek, einv = np.unique(ekey, return_inverse=True)
-    tet_edges = einv.reshape(T, 6)                                   # local edge -> global edge
-    E = len(uek)
-    opp = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]])
-    f = np.sort(tets[:, opp], axis=2).reshape(-1, 3).astype(np.int64)
-    fkey = (f[:, 0] * nv + f[:, 1]) * nv + f[:, 2]
-    ufk, finv = np.unique(fkey, return_inverse=True)
-    tet_faces = finv.reshape(T, 4)                                   # face opposite local k
-    F = len(ufk)
-    ev = np.stack([uek // nv, uek % nv], 1)
-    fv = np.stack([ufk // (nv * nv), (ufk // nv) % nv, ufk % nv], 1)
+    pairs = _PAIRS
+    tet_edges, ev, tet_faces, fv = _tet_entities(nv, tets)
+    E, F = len(ev), len(fv)
     # new nodes: sum of vertices divided by the count, as Tethex set_new_vertices
     emid = (nodes[ev[:, 0]] + nodes[ev[:, 1]]) / 2.0
     fcen = (nodes[fv[:, 0]] + nodes[fv[:, 1]] + nodes[fv[:, 2]]) / 3.0

@@ -1,0 +1,318 @@
+"""Continuous-batching field-solve server for concurrent molecular-dynamics clients.
+
+The reference solves one Laplace problem per MD timestep, inline, with the previous solution as
+the initial guess only inside a PIC run (src/ProjectRunaway.cpp:422-431, 455-498; the solution
+vector survives between `PoissonSolver::assemble` calls, src/DealSolver.cpp:369-387 resets it
+only in `setup_system`).  Here many MD clients share one GPU:
+
+* every client keeps its tip mesh and its last solution resident in device memory, so a new
+  timestep only uploads moved vertex coordinates and starts CG from the previous potential;
+* a request is admitted into the running batch (`fgpu.Batch`) as soon as its client is idle
+  and the device-memory budget holds the client's mesh and system; idle clients are evicted
+  least-recently-used first when the budget is short;
+* the batch runs a fixed number of CG iterations per step, finished solves leave and new ones
+  join between steps, so short solves are not held back by long ones.
+
+`Server(..., batched=False, warm=False)` is the plain form: the same requests solved one at a
+time, each from the Dirichlet value, in arrival order.
+"""
+from __future__ import annotations
+
+import heapq
+import time
+from collections import OrderedDict, deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import fgpu
+from . import mesh as M
+
+# Poisson boundary ids for the vacuum mesh: top 8 (Neumann field), bottom/sides/other
+# as src/PoissonSolver.cpp:52-55
+POISSON_MARKS = (8, 2, 4, 2)
+BYTES_PER_DOF_ESTIMATE = 480          # mesh + system device bytes per dof before the first import
+
+
+@dataclass
+class Request:
+    client: int
+    seq: int
+    t_submit: float
+    t_admit: float = -1.0
+    t_done: float = -1.0
+    iterations: int = 0
+    residual: float = 0.0
+    x_init: np.ndarray | None = None        # kept only when the server records inputs
+    positions: np.ndarray | None = None
+    x_final: np.ndarray | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.t_submit
+
+
+def vertex_altitudes(nodes: np.ndarray, tets: np.ndarray) -> np.ndarray:
+    """Per tet vertex: the smallest altitude (distance to the opposite face) over its tets."""
+    P = nodes[tets]
+    v6 = np.abs(np.einsum("ij,ij->i", P[:, 1] - P[:, 0], np.cross(P[:, 2] - P[:, 0], P[:, 3] - P[:, 0])))
+    h = np.full(len(nodes), np.inf)
+    for k in range(4):
+        a, b, c = [P[:, j] for j in range(4) if j != k]
+        area2 = np.linalg.norm(np.cross(b - a, c - a), axis=1)          # twice the face area
+        np.minimum.at(h, tets[:, k], v6 / np.maximum(area2, 1e-300))
+    return h
+
+
+class Client:
+    """An MD client: a tip whose surface atoms vibrate about their lattice sites with Gaussian
+    displacements of `sigma` (Angstrom) per timestep; the rest of the mesh follows through the
+    tet-to-hex averaging of node positions.
+
+    A resident mesh keeps its topology, so a move may not invert a tet: each surface node's
+    displacement is capped at `cap` times its smallest tet altitude (the first-order volume
+    change of a tet is then below 4 * cap of its volume).  Without the cap, Gaussian noise of
+    0.1 A turns the thinnest Delaunay slivers inside out."""
+
+    def __init__(self, cid: int, tip: M.TipMesh, field: float = -1.0, sigma: float = 0.1, seed: int = 0,
+                 cap: float = 0.15):
+        self.cid = cid
+        self.tip = tip
+        self.field = field
+        self.sigma = sigma
+        self.rng = np.random.default_rng(seed)
+        self.cells = np.ascontiguousarray(tip.vacuum_cells(), np.int32)
+        self.X0 = np.ascontiguousarray(tip.nodes, np.float64)
+        self.n_dofs = int(np.unique(self.cells).size)
+        W = M.hex_node_weights(tip.n_tetnodes, tip.tets)
+        ns = tip.n_surface_nodes
+        self.W_surf = W[:, :ns].tocsr()
+        self.limit = cap * vertex_altitudes(tip.nodes[: tip.n_tetnodes], tip.tets)[:ns]
+        self.seq = 0
+
+    def displacements(self) -> np.ndarray:
+        d = self.rng.normal(0.0, self.sigma, size=(self.tip.n_surface_nodes, 3))
+        norm = np.linalg.norm(d, axis=1)
+        scale = np.minimum(1.0, self.limit / np.maximum(norm, 1e-300))
+        return d * scale[:, None]
+
+    def next_positions(self) -> np.ndarray:
+        return self.X0 + self.W_surf @ self.displacements()
+
+
+@dataclass
+class _Resident:
+    mesh: fgpu.Mesh
+    sys: fgpu.System
+    bytes: int
+    mesh_positions: np.ndarray
+    fresh: bool = True                       # solution still holds the initial fill
+
+
+@dataclass
+class ServeStats:
+    requests: list = field(default_factory=list)
+    evictions: int = 0
+    rejected_moves: int = 0
+    t_start: float = 0.0
+    t_end: float = 0.0
+
+    def summary(self) -> dict:
+        """Throughput counts the solves finished inside [t_start, t_end]; latency percentiles
+        cover every finished request, including those drained after the window."""
+        done = [r for r in self.requests if r.t_done >= 0]
+        lat = np.array([r.latency for r in done]) * 1e3
+        span = max(self.t_end - self.t_start, 1e-12)
+        its = np.array([r.iterations for r in done]) if done else np.zeros(1)
+        in_window = sum(1 for r in done if r.t_done <= self.t_end)
+        return {
+            "solves": in_window,
+            "solves_per_s": in_window / span,
+            "p50_ms": float(np.percentile(lat, 50)) if done else None,
+            "p99_ms": float(np.percentile(lat, 99)) if done else None,
+            "mean_iterations": float(its.mean()),
+            "evictions": self.evictions,
+            "rejected_moves": self.rejected_moves,
+            "seconds": span,
+        }
+
+
+class Server:
+    def __init__(self, ctx: fgpu.Context, clients: list[Client], opts: fgpu.SolveOpts,
+                 budget_bytes: int = 64 << 30, batched: bool = True, warm: bool = True,
+                 step_iterations: int = 32, max_batch: int = 64, record_inputs: bool = False):
+        self.ctx = ctx
+        self.clients = clients
+        self.opts = opts
+        self.budget = int(budget_bytes)
+        self.batched = batched
+        self.warm = warm
+        self.step_iterations = step_iterations
+        self.max_batch = max_batch
+        self.record_inputs = record_inputs
+        self.resident: OrderedDict[int, _Resident] = OrderedDict()   # LRU order
+        self.known_bytes: dict[int, int] = {}
+        self.pending: deque[Request] = deque()
+        self.busy: dict[int, Request] = {}                            # client -> in-flight request
+        self.by_sys: dict[int, Request] = {}
+        self.stats = ServeStats()
+        self.batch = fgpu.Batch(ctx, opts) if batched else None
+
+    # ---- memory-budget admission
+    def used_bytes(self) -> int:
+        return sum(r.bytes for r in self.resident.values())
+
+    def _make_resident(self, c: Client) -> _Resident | None:
+        r = self.resident.get(c.cid)
+        if r is not None:
+            self.resident.move_to_end(c.cid)
+            return r
+        need = self.known_bytes.get(c.cid, BYTES_PER_DOF_ESTIMATE * c.n_dofs)
+        if need > self.budget:
+            raise ValueError(f"client {c.cid} needs {need} bytes, more than the whole budget")
+        for cid in list(self.resident):
+            if self.used_bytes() + need <= self.budget:
+                break
+            if cid not in self.busy:                  # idle clients leave, oldest first
+                del self.resident[cid]
+                self.stats.evictions += 1
+        if self.used_bytes() + need > self.budget:
+            return None
+        m = fgpu.Mesh(self.ctx, c.X0, c.cells, *POISSON_MARKS)
+        s = fgpu.System(m)
+        s.fill_solution(0.0)
+        b = m.device_bytes() + s.device_bytes()
+        self.known_bytes[c.cid] = b
+        r = _Resident(m, s, b, c.X0)
+        self.resident[c.cid] = r
+        return r
+
+    def _assemble(self, c: Client, r: _Resident, req: Request):
+        X = c.next_positions()
+        try:
+            r.mesh.update_vertices(X)
+        except fgpu.MeshInvertedError:
+            # the step would need a new mesh; this request is solved on the last valid one
+            self.stats.rejected_moves += 1
+            X = r.mesh_positions
+        r.mesh_positions = X
+        s = r.sys
+        s.reset()
+        s.add_dirichlet(2, 0.0)
+        s.add_neumann(8, None, c.field)
+        s.apply_dirichlet()
+        if not self.warm or r.fresh:
+            s.fill_solution(0.0)
+        r.fresh = False
+        if self.record_inputs:
+            req.positions = X
+            req.x_init = s.solution()
+
+    # ---- request flow
+    def submit(self, req: Request):
+        self.stats.requests.append(req)
+        self.pending.append(req)
+
+    def _admit(self, now: float) -> int:
+        admitted = 0
+        waiting = deque()
+        while self.pending:
+            req = self.pending.popleft()
+            if req.client in self.busy or len(self.busy) >= self.max_batch:
+                waiting.append(req)
+                continue
+            c = self.clients[req.client]
+            r = self._make_resident(c)
+            if r is None:
+                waiting.append(req)
+                continue
+            self._assemble(c, r, req)
+            req.t_admit = now
+            self.busy[req.client] = req
+            self.by_sys[r.sys.h.value] = req
+            self.batch.add(r.sys)
+            admitted += 1
+        self.pending = waiting
+        return admitted
+
+    def _finish(self, req: Request, st: fgpu.SolveStats, now: float):
+        req.t_done = now
+        req.iterations = st.iterations
+        req.residual = st.residual
+        del self.busy[req.client]
+        if self.record_inputs:
+            req.x_final = self.resident[req.client].sys.solution()
+
+    def step(self, clock=time.perf_counter) -> list[Request]:
+        """Admits what fits, advances the GPU work by one step and returns finished requests."""
+        done = []
+        if not self.batched:
+            # plain form: the oldest request is solved to the end, alone
+            if not self.pending:
+                return done
+            req = self.pending.popleft()
+            c = self.clients[req.client]
+            r = self._make_resident(c)
+            if r is None:
+                raise RuntimeError("budget too small for a single client")
+            self._assemble(c, r, req)
+            req.t_admit = clock()
+            self.busy[req.client] = req
+            st = r.sys.solve(self.opts)
+            self._finish(req, st, clock())
+            return [req]
+        self._admit(clock())
+        if len(self.batch) == 0:
+            return done
+        self.batch.step(self.step_iterations)
+        now = clock()
+        for s, st in self.batch.retire():
+            req = self.by_sys.pop(s.h.value)
+            self._finish(req, st, now)
+            done.append(req)
+        return done
+
+    def idle(self) -> bool:
+        return not self.pending and not self.busy
+
+    def solution(self, cid: int) -> np.ndarray:
+        return self.resident[cid].sys.solution()
+
+
+def run_clients(server: Server, rate: float, duration: float, seed: int = 0, max_requests: int | None = None,
+                clock=time.perf_counter) -> ServeStats:
+    """Drives the server with closed-loop MD clients: each client submits its next timestep's
+    solve an exponential think time (total rate `rate` over all clients) after its previous
+    result came back.  Runs `duration` seconds (or until `max_requests` were submitted) and
+    then drains what is in flight; throughput counts the solves finished inside the window."""
+    rng = np.random.default_rng(seed)
+    n = len(server.clients)
+    mean_think = n / rate
+    t0 = clock()
+    server.stats.t_start = t0
+    events = [(t0 + rng.exponential(mean_think), cid) for cid in range(n)]
+    heapq.heapify(events)
+    submitted = 0
+    while True:
+        now = clock()
+        in_window = now - t0 < duration and (max_requests is None or submitted < max_requests)
+        while in_window and events and events[0][0] <= now:
+            _, cid = heapq.heappop(events)
+            c = server.clients[cid]
+            server.submit(Request(cid, c.seq, now))
+            c.seq += 1
+            submitted += 1
+            if max_requests is not None and submitted >= max_requests:
+                break
+        finished = server.step(clock)
+        now = clock()
+        for req in finished:
+            heapq.heappush(events, (now + rng.exponential(mean_think), req.client))
+        if not in_window and server.idle():
+            break
+        if not finished and server.idle() and events:
+            wait = events[0][0] - clock()
+            if wait > 0:
+                time.sleep(min(wait, 0.01))
+    server.stats.t_end = min(clock(), t0 + duration) if max_requests is None else clock()
+    return server.stats

@@ -22,6 +22,7 @@ extern "C" {
 #define FGPU_EHIP (-2)
 #define FGPU_ENOMEM (-3)
 #define FGPU_ESTATE (-4)
+#define FGPU_EMESH (-5) /* moved vertices would invert a cell: rebuild the mesh instead */
 
 /* boundary ids, include/Globals.h:60-68 */
 #define FGPU_BID_COPPER_SURFACE 2
@@ -85,7 +86,10 @@ int fgpu_mesh_export_bfaces(const fgpu_mesh *mesh, int *cell, int *face, int *bi
 /* DealSolver::export_surface_centroids (src/DealSolver.cpp:229-245) for a boundary id;
  * returns the count in *n (xyz may be NULL to query it) */
 int fgpu_mesh_export_face_centroids(const fgpu_mesh *mesh, int bid, double *xyz, int *n);
-/* moved vertices, same topology (3*n_vertices of the import numbering) */
+/* moved vertices, same topology (3*n_vertices of the import numbering).  The reference
+ * rebuilds the mesh every step (src/ProjectRunaway.cpp:214-228); this keeps it resident.
+ * FGPU_EMESH, coordinates unchanged, when a cell's centre Jacobian changes sign: the
+ * orientation chosen at import would no longer be the one a fresh import picks. */
 int fgpu_mesh_update_vertices(fgpu_mesh *mesh, const double *xyz);
 /* device memory held by a mesh / a system (bytes), for admission by memory budget; no
  * counterpart in the reference */

@@ -174,3 +174,25 @@ def test_moved_vertices_match_fresh_import(ctx, small_mesh):
     assert it > 0
     x = s.solution()
     assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
+
+
+def test_inverting_move_is_rejected(ctx, small_mesh):
+    """A vertex update that turns cells inside out (here: a mirror image) is refused with
+    FGPU_EMESH and leaves the mesh as it was."""
+    s = _poisson_system(ctx, small_mesh)
+    o = fgpu.opts(tol=1e-10, relative=True)
+    s.solve(o)
+    x0 = s.solution()
+    X = small_mesh.nodes.copy()
+    X[:, 0] = -X[:, 0]
+    with pytest.raises(fgpu.MeshInvertedError):
+        s.mesh.update_vertices(X)
+    s2 = _poisson_system(ctx, small_mesh)
+    s.reset()
+    s.add_dirichlet(2, 0.0)
+    s.add_neumann(8, None, -1.0)
+    s.apply_dirichlet()
+    s.fill_solution(0.0)
+    s.solve(o)
+    assert np.array_equal(s.solution(), x0)
+    del s2

@@ -1,0 +1,80 @@
+"""Serving: each request's result is the same in the continuous batch as when its system is
+solved alone from the same initial guess, with and without evictions by the memory budget."""
+import numpy as np
+import pytest
+
+from femocs_amd import fgpu, serving
+from femocs_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def clients():
+    geos = [M.small_geometry(), M.small_geometry(radius=10.0, height=24.0, seed=7),
+            M.small_geometry(radius=14.0, height=40.0, h_min=7.0, seed=99)]
+    return [serving.Client(i, M.make_tip_mesh(g), field=-1.0 - 0.1 * i, seed=i) for i, g in enumerate(geos)]
+
+
+def _alone(ctx, c, req, o):
+    m = fgpu.Mesh(ctx, c.X0, c.cells, *serving.POISSON_MARKS)
+    m.update_vertices(req.positions)
+    s = fgpu.System(m)
+    s.reset()
+    s.add_dirichlet(2, 0.0)
+    s.add_neumann(8, None, c.field)
+    s.apply_dirichlet()
+    s.set_solution(req.x_init)
+    st = s.solve(o)
+    return s.solution(), st.iterations
+
+
+def _check(ctx, srv, o):
+    done = [r for r in srv.stats.requests if r.t_done >= 0]
+    assert done and all(r.iterations > 0 for r in done)
+    for r in done:
+        x, it = _alone(ctx, srv.clients[r.client], r, o)
+        assert it == r.iterations, (r.client, r.seq)
+        assert np.array_equal(x, r.x_final), (r.client, r.seq)
+
+
+@pytest.mark.parametrize("precision", [fgpu.FGPU_FP64, fgpu.FGPU_MIXED])
+def test_serving_results_equal_alone(ctx, clients, precision):
+    o = fgpu.opts(tol=1e-9, relative=True, precision=precision, inner_tol=0.1, use_graph=False)
+    for c in clients:
+        c.seq = 0
+    srv = serving.Server(ctx, clients, o, step_iterations=16, record_inputs=True)
+    serving.run_clients(srv, rate=400.0, duration=60.0, max_requests=9, seed=1)
+    assert srv.idle()
+    _check(ctx, srv, o)
+    # warm starts: later requests of a client start from its previous solution
+    firsts = {}
+    for r in srv.stats.requests:
+        if r.client in firsts:
+            assert np.any(r.x_init != 0.0)
+        firsts.setdefault(r.client, r)
+
+
+def test_serving_evicts_within_budget(ctx, clients):
+    o = fgpu.opts(tol=1e-9, relative=True, precision=fgpu.FGPU_MIXED, inner_tol=0.1, use_graph=False)
+    sizes = []
+    for c in clients:
+        m = fgpu.Mesh(ctx, c.X0, c.cells, *serving.POISSON_MARKS)
+        sizes.append(m.device_bytes() + fgpu.System(m).device_bytes())
+    budget = sum(sorted(sizes)[-2:])              # never all three at once
+    srv = serving.Server(ctx, clients, o, budget_bytes=budget, step_iterations=16, record_inputs=True)
+    serving.run_clients(srv, rate=400.0, duration=60.0, max_requests=12, seed=2)
+    assert srv.idle()
+    assert srv.stats.evictions > 0
+    assert srv.used_bytes() <= budget
+    _check(ctx, srv, o)
+
+
+def test_plain_server_solves_cold(ctx, clients):
+    o = fgpu.opts(tol=1e-9, relative=True, precision=fgpu.FGPU_FP64)
+    srv = serving.Server(ctx, clients, o, batched=False, warm=False, record_inputs=True)
+    serving.run_clients(srv, rate=400.0, duration=60.0, max_requests=5, seed=3)
+    assert srv.idle()
+    for r in srv.stats.requests:
+        assert not np.any(r.x_init)
+    _check(ctx, srv, o)
