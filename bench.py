@@ -1,10 +1,17 @@
-"""Benchmark of the FEMOCS field-solve hot path on one MI355X.
+"""Benchmark of the FEMOCS field-solve hot path on one MI355X: Poisson solves served to
+concurrent molecular-dynamics clients.
 
-One step = one Poisson solve of the nanotip problem (Neumann face load + FP64 Dirichlet
-lifting + CG from phi = 0 to |b - Ax| / |b| <= 1e-9 + export of the nodal potential), on the
-mesh of a nanotip r = 10 nm, h = 60 nm on a 120 x 120 nm slab (tet-split Q1 hexahedra).
-The headline value is the mixed-precision solver (FP32 operator and CG inside FP64 defect
-correction); the plain FP64 CG is measured on the same mesh beside it.
+Workload: 16 MD clients, each with its own Cu nanotip (radius 3-12 nm, aspect ratio 4-7,
+112k-1.6M vacuum Q1 hexes from the tet split, ~10M dofs in all), whose surface atoms vibrate
+(0.1 A Gaussian noise per step).  A client submits the Laplace solve of its next timestep
+(Dirichlet 0 on the copper surface, Neumann field on the top; |b - Ax| / |b| <= 1e-9) an
+exponential think time after its previous result came back, 200 requests/s offered in all.
+One step = one served solve; `value` is solves/s over the K timed requests.
+
+* headline: mixed precision (FP32 operator and CG inside FP64 defect correction), continuous
+  batching, meshes and last solutions resident on the GPU (warm starts);
+* `plain_fp64`: the same request stream through the plain form: FP64 CG, one solve at a time,
+  from phi = 0, in arrival order.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--small]
 """
@@ -23,143 +30,174 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Poisson solves/s + p50/p99 latency at 1e-9 residual: mixed vs FP64 CG, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+N_CLIENTS = 16
+RATE = 200.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-USE_GRAPH = True
+def client_geometry(i: int, n: int = N_CLIENTS, small: bool = False):
+    """Tip i of the client set: radii spread over 3-12 nm, aspect ratios 4-7 interleaved."""
+    from femocs_amd import mesh as M
+    if small:
+        return M.small_geometry(radius=10.0 + 2.0 * i, height=24.0 + 4.0 * i, seed=100 + i)
+    r = 3.0 + 9.0 * i / (n - 1)
+    a = 4.0 + 3.0 * ((5 * i) % n) / (n - 1)
+    return M.TipGeometry(radius=10 * r, height=10 * r * a, slab_width=120 * r,
+                         slab_thickness=30.0 if r < 6 else 60.0, seed=1000 + i)
 
 
-def one_solve(solver, sys_, precision, tol):
+def _make_client(args):
+    i, n, small = args
+    from femocs_amd import mesh as M
+    from femocs_amd import serving
+    tip = M.make_tip_mesh(client_geometry(i, n, small), with_atoms=False)
+    return serving.Client(i, tip, field=-1.0, sigma=0.1, seed=i)
+
+
+def make_clients(n: int, small: bool):
+    from multiprocessing import get_context
+    workers = max(1, min(n, 16, os.cpu_count() or 1))
+    with get_context("fork").Pool(workers) as pool:
+        return pool.map(_make_client, [(i, n, small) for i in range(n)])
+
+
+def serve(ctx, clients, precision, batched, warm, steps, warmup, tuned, seed):
+    from femocs_amd import fgpu, serving
+    o = fgpu.opts(tol=1e-9, relative=True, precision=precision, inner_tol=0.1, use_graph=False)
+    cells, ppb = tuned
+
+    def configure(mesh, system):
+        mesh.set_patch_cells(cells)
+        system.set_patches_per_block(ppb)
+
+    for c in clients:
+        c.seq = 0
+        c.rng = np.random.default_rng(1000 + c.cid)
+    srv = serving.Server(ctx, clients, o, batched=batched, warm=warm, configure=configure)
+    serving.run_clients(srv, rate=RATE, duration=1e9, max_requests=warmup, seed=seed)
+    srv.stats = serving.ServeStats()
+    ctx.sync()
+    serving.run_clients(srv, rate=RATE, duration=1e9, max_requests=steps, seed=seed + 1)
+    ctx.sync()
+    summ = srv.stats.summary()
+    bad = [r for r in srv.stats.requests if r.iterations <= 0]
+    if bad:
+        raise RuntimeError(f"{len(bad)} solves did not converge")
+    return srv, summ
+
+
+def fp64_errors(srv):
+    """Error of each client's last mixed result against an FP64 solve of the same system."""
     from femocs_amd import fgpu
-    sys_.reset()
-    sys_.add_dirichlet(fgpu.BID_COPPER_SURFACE, 0.0)
-    sys_.add_neumann(fgpu.BID_VACUUM_TOP, None, 1.0)   # applied_field = -E0, E0 = -1 V/A
-    sys_.apply_dirichlet()
-    sys_.fill_solution(0.0)
-    st = sys_.solve(fgpu.opts(tol=tol, relative=True, precision=precision, inner_tol=0.1, use_graph=USE_GRAPH))
-    phi = sys_.solution()
-    return st, phi
-
-
-def time_mode(sys_, precision, steps, warmup, tol):
-    for _ in range(warmup):
-        one_solve(None, sys_, precision, tol)
-    lat = []
-    its = []
-    sys_.ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        t = time.perf_counter()
-        st, phi = one_solve(None, sys_, precision, tol)
-        lat.append(time.perf_counter() - t)
-        its.append(st.iterations)
+    o = fgpu.opts(tol=1e-12, relative=True, precision=fgpu.FGPU_FP64)
+    errs = []
+    for cid, r in srv.resident.items():
+        xm = r.sys.solution()
+        r.sys.fill_solution(0.0)
+        st = r.sys.solve(o)
         if st.iterations <= 0:
-            raise RuntimeError(f"solve did not converge: {st.as_dict()}")
-    sys_.ctx.sync()
-    total = time.perf_counter() - t0
-    lat = np.array(lat) * 1e3
-    return {
-        "solves_per_s": steps / total,
-        "ms_per_solve": total * 1e3 / steps,
-        "p50_ms": float(np.percentile(lat, 50)),
-        "p99_ms": float(np.percentile(lat, 99)),
-        "iterations": int(np.median(its)),
-        "outer": int(st.outer),
-        "rel_residual": st.residual / st.rhs_norm,
-    }, phi
+            raise RuntimeError("FP64 reference solve did not converge")
+        x64 = r.sys.solution()
+        errs.append(float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)))
+    return errs
 
 
-def cpu_baseline(mesh, budget_s=60.0):
+def cpu_baseline(client):
     """CPU oracle (restatement of the reference algorithm: assembled CSR, QGauss(2), deal.II
-    lifting, SSOR(1.2)-PCG), one full solve of the same problem on one host core."""
+    lifting, SSOR(1.2)-PCG), one cold solve of one client's request on one host core."""
     from oracle import oracle as O
+    X = client.next_positions()
     t0 = time.perf_counter()
-    om = O.Mesh(mesh.nodes, mesh.vacuum_cells(), 8, 2, 4, 2)
+    om = O.Mesh(X, client.cells, 8, 2, 4, 2)
     om.assemble()
-    b = om.face_rhs(8, constant=1.0)
+    b = om.face_rhs(8, constant=-client.field)
     x = np.zeros(om.n_dofs)
-    mask = om.dirichlet_mask(2)
-    om.apply_dirichlet(b, x, mask, None)
+    om.apply_dirichlet(b, x, om.dirichlet_mask(2), None)
     t1 = time.perf_counter()
-    it, res = om.pcg(b, x, tol=1e-9, relative=True, precond="ssor", omega=1.2, max_it=10000)
+    it, _ = om.pcg(b, x, tol=1e-9, relative=True, precond="ssor", omega=1.2, max_it=10000)
     t2 = time.perf_counter()
-    return {"value": 1.0 / (t2 - t0), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"one full solve of the same mesh ({om.n_dofs} dofs): import+assembly+lifting "
-                      f"{t1 - t0:.2f} s, SSOR(1.2)-PCG {it} iterations {t2 - t1:.2f} s, 1 thread"}, x
+    if it <= 0:
+        raise RuntimeError("CPU oracle did not converge")
+    return {"value": round(1.0 / (t2 - t0), 5), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"one cold solve of the median client's request ({om.n_dofs} dofs, "
+                      f"{len(client.cells)} hexes): import+assembly+lifting {t1 - t0:.2f} s, "
+                      f"SSOR(1.2)-PCG {it} iterations {t2 - t1:.2f} s, 1 thread"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=160, help="timed requests")
+    ap.add_argument("--warmup", type=int, default=16, help="untimed requests (meshes become resident)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--small", action="store_true", help="nanotip r=3nm h=15nm mesh instead (quick check)")
-    ap.add_argument("--tol", type=float, default=1e-9)
-    ap.add_argument("--no-graph", action="store_true", help="launch kernels directly (for profilers)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
+    ap.add_argument("--small", action="store_true", help="4 small tips instead (quick check)")
     args = ap.parse_args()
-    global USE_GRAPH
-    USE_GRAPH = not args.no_graph
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: the solver has no multi-GPU path in this build "
+        print(f"bench.py measures one GPU: the field solver has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
 
-    from femocs_amd import fgpu
-    from femocs_amd import mesh as M
+    from femocs_amd import autotune, fgpu
 
-    g = M.tip_r3nm_geometry() if args.small else M.tip_r10nm_geometry()
+    n = 4 if args.small else N_CLIENTS
     t = time.perf_counter()
-    mesh = M.make_tip_mesh(g, with_atoms=False)
-    log(f"mesh: {mesh.n_tets} tets, {mesh.n_hexes} hexes ({time.perf_counter() - t:.1f} s)")
+    clients = make_clients(n, args.small)
+    hexes = [len(c.cells) for c in clients]
+    dofs = [c.n_dofs for c in clients]
+    log(f"clients: {n} tips, hexes {min(hexes)}..{max(hexes)}, {sum(dofs)} dofs ({time.perf_counter() - t:.1f} s)")
+
     ctx = fgpu.Context(0)
-    t = time.perf_counter()
-    gm = fgpu.Mesh(ctx, mesh.nodes, mesh.vacuum_cells(), 8, 2, 4, 2)
-    sys_ = fgpu.System(gm)
-    setup_ms = (time.perf_counter() - t) * 1e3
-    log(f"device mesh: {gm.n_dofs} dofs, {gm.n_cells} cells, import {setup_ms:.0f} ms")
-
-    from femocs_amd import autotune
+    big = int(np.argmax(hexes))
+    gm = fgpu.Mesh(ctx, clients[big].X0, clients[big].cells, 8, 2, 4, 2)
+    gs = fgpu.System(gm)
     tuned = {}
-    best, _ = autotune.autotune(sys_, fgpu.FGPU_FP64)
-    tuned["fp64"] = {"cells_per_patch": best[0], "patches_per_block": best[1]}
-    fp64, phi64 = time_mode(sys_, fgpu.FGPU_FP64, args.steps, args.warmup, args.tol)
-    log("fp64", fp64, tuned["fp64"])
-    best, _ = autotune.autotune(sys_, fgpu.FGPU_MIXED)
-    tuned["mixed"] = {"cells_per_patch": best[0], "patches_per_block": best[1]}
-    mixed, phimx = time_mode(sys_, fgpu.FGPU_MIXED, args.steps, args.warmup, args.tol)
-    log("mixed", mixed, tuned["mixed"])
-    err_mixed_vs_fp64 = float(np.linalg.norm(phimx - phi64) / np.linalg.norm(phi64))
+    for name, prec in (("mixed", fgpu.FGPU_MIXED), ("fp64", fgpu.FGPU_FP64)):
+        best, _ = autotune.autotune(gs, prec)
+        tuned[name] = best
+    del gs, gm
+    log("autotuned (cells per patch, patches per workgroup):", tuned)
 
-    # dominant kernel: the element apply, FP32 in the mixed solver's iterations (the system
-    # is left tuned for mixed precision).  Algorithmic bytes per launch = compulsory traffic
-    # of a matrix-free Q1 apply: connectivity 32 B/cell (8 x int32) + coordinates 3*vb B/dof
-    # + the two input vectors (z, p) 2*vb B/dof + one output per dof vb B/dof.
-    H, N = gm.n_cells, gm.n_dofs
-    kern = {}
-    for prec, name, vb in ((fgpu.FGPU_MIXED, "k_patch<float>", 4), (fgpu.FGPU_FP64, "k_patch<double>", 8)):
-        ms = sys_.time_kernel(0, prec, reps=50)
-        byts = 32 * H + 3 * vb * N + 2 * vb * N + vb * N
-        kern[name] = {"ms": ms, "bytes": byts, "GBs": byts / (ms * 1e-3) / 1e9}
-    log("kernels", kern)
-    k = kern["k_patch<float>"]
-    roofline = {"bound": "hbm", "achieved": round(k["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(k["GBs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_patch<float>",
-                "avg_us": round(k["ms"] * 1e3, 2), "algorithmic_bytes": k["bytes"],
-                "fp64_kernel": {"avg_us": round(kern["k_patch<double>"]["ms"] * 1e3, 2),
-                                "GBs": round(kern["k_patch<double>"]["GBs"], 1)}}
+    plain = None
+    if not args.no_plain:
+        _, plain = serve(ctx, clients, fgpu.FGPU_FP64, False, False, args.steps, args.warmup, tuned["fp64"], 1)
+        log("plain fp64", plain)
+    srv, mixed = serve(ctx, clients, fgpu.FGPU_MIXED, True, True, args.steps, args.warmup, tuned["mixed"], 1)
+    log("mixed batched", mixed)
+
+    # dominant kernel: the FP32 element apply over the whole batch (all clients resident).
+    # Algorithmic bytes per launch = compulsory traffic of a matrix-free Q1 apply: cell
+    # connectivity 32 B/hex (8 x int32), coordinates 3 x 4 B/dof, the two input vectors
+    # (z, p) 2 x 4 B/dof and one output 4 B/dof.
+    b = fgpu.Batch(ctx, srv.opts)
+    for r in srv.resident.values():
+        b.add(r.sys)
+    ms = b.time_kernel(0, reps=30)
+    H = sum(r.mesh.n_cells for r in srv.resident.values())
+    N = sum(r.mesh.n_dofs for r in srv.resident.values())
+    byts = 32 * H + 12 * N + 8 * N + 4 * N
+    gbs = byts / (ms * 1e-3) / 1e9
+    gather_ms = b.time_kernel(1, reps=30)
+    del b
+    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_patch<float>",
+                "avg_us": round(ms * 1e3, 2), "algorithmic_bytes": byts, "cells": H, "dofs": N,
+                "gather_avg_us": round(gather_ms * 1e3, 2)}
+    log("roofline", roofline)
+
+    errs = fp64_errors(srv)
+    log("max rel error mixed vs fp64", max(errs))
 
     cpu = None
-    err_vs_cpu = None
     if not args.no_cpu:
-        cpu, xcpu = cpu_baseline(mesh)
-        err_vs_cpu = float(np.linalg.norm(phi64 - xcpu) / np.linalg.norm(xcpu))
-        cpu["value"] = round(cpu["value"], 5)
+        order = np.argsort(hexes)
+        cpu = cpu_baseline(clients[int(order[len(order) // 2])])
         log("cpu", cpu)
 
+    ms_per = 1e3 / mixed["solves_per_s"]
     out = {
         "metric": METRIC,
         "value": round(mixed["solves_per_s"], 3),
@@ -167,25 +205,22 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(mixed["ms_per_solve"], 3),
+        "ms_per_step": round(ms_per, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 inner / f64 outer",
         "data": "synthetic",
-        "config": {"workload": ("Cu nanotip r=3nm h=15nm on 30x30x3 nm slab" if args.small else
-                                "Cu nanotip r=10nm h=60nm on 120x120 nm slab") +
-                               ", Q1 tet-split hexes, 1 Poisson solve per step",
-                   "n_dofs": N, "n_cells": H, "tol_rel": args.tol, "precond": "jacobi",
-                   "parallelism": "single GPU"},
-        "latency_ms": {"p50": round(mixed["p50_ms"], 3), "p99": round(mixed["p99_ms"], 3)},
-        "mixed": mixed,
-        "fp64": fp64,
-        "speedup_mixed_vs_fp64": round(mixed["solves_per_s"] / fp64["solves_per_s"], 3),
-        "err_mixed_vs_fp64": err_mixed_vs_fp64,
-        "err_fp64_vs_cpu_oracle": err_vs_cpu,
-        "mesh_import_ms": round(setup_ms, 1),
-        "autotuned": tuned,
+        "config": {"workload": f"{n} MD clients, Cu nanotips r=3-12 nm, {min(hexes)}-{max(hexes)} Q1 hexes each, "
+                               f"Poisson solve per timestep, closed-loop arrivals {RATE:.0f} req/s offered",
+                   "clients": n, "total_dofs": int(sum(dofs)), "total_hexes": int(sum(hexes)),
+                   "tol_rel": 1e-9, "precond": "jacobi", "parallelism": "single GPU"},
+        "latency_ms": {"p50": round(mixed["p50_ms"], 2), "p99": round(mixed["p99_ms"], 2)},
+        "mixed_batched": mixed,
+        "plain_fp64": plain,
+        "speedup_vs_plain_fp64": round(mixed["solves_per_s"] / plain["solves_per_s"], 3) if plain else None,
+        "err_mixed_vs_fp64_max": max(errs),
+        "autotuned": {k: {"cells_per_patch": v[0], "patches_per_block": v[1]} for k, v in tuned.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -98,13 +98,23 @@ fgpu::Mesh::~Mesh() {
     dfree(d_plent);
     dfree(d_bf_icell);
     dfree(d_bf_face);
+    dfree(d_bf_id);
+    dfree(motion.idof); dfree(motion.ptr); dfree(motion.ctrl); dfree(motion.w);
+    dfree(motion.base); dfree(motion.disp); dfree(motion.prev);
     dfree(d_fgptr);
     dfree(d_fgidx);
     dfree(d_v2c);
     dfree(d_v2n);
 }
 
+void fgpu::Sys::clear_terms() {
+    for (auto &t : rhs_terms) dfree(t.dval);
+    rhs_terms.clear();
+    lifted = false;
+}
+
 fgpu::Sys::~Sys() {
+    clear_terms();
     dfree(b); dfree(x); dfree(r); dfree(z); dfree(p); dfree(q); dfree(Ye); dfree(invd); dfree(diag);
     dfree(g); dfree(tmp); dfree(kq); dfree(gq); dfree(fslot); dfree(fval); dfree(d_dmask);
     dfree(r32); dfree(z32); dfree(p32); dfree(q32); dfree(d32); dfree(Ye32); dfree(invd32);
@@ -244,6 +254,8 @@ extern "C" int fgpu_mesh_import(fgpu_ctx *ctx, int nv, const double *xyz, int nc
         h2d(m->d_bf_icell, bf_icell.data(), nb, s);
         m->d_bf_face = dalloc<int>(nb);
         h2d(m->d_bf_face, m->bf_face.data(), nb, s);
+        m->d_bf_id = dalloc<int>(nb);
+        h2d(m->d_bf_id, m->bf_id.data(), nb, s);
         m->d_fgptr = dalloc<int>(fgptr.size());
         h2d(m->d_fgptr, fgptr.data(), fgptr.size(), s);
         m->d_fgidx = dalloc<int>(fgidx.size());
@@ -296,9 +308,29 @@ extern "C" int fgpu_mesh_sizes(const fgpu_mesh *m, int *nd, int *nc, int *nb) {
     return FGPU_OK;
 }
 
+// the host coordinate copy after device-side motion (fgpu_mesh_displace)
+static void refresh_host_x(const fgpu_mesh *cm) {
+    if (!cm->x_stale) return;
+    fgpu_mesh *m = const_cast<fgpu_mesh *>(cm);
+    std::vector<double> X(4 * (size_t)m->n_dofs);
+    d2h(X.data(), m->d_X64, X.size(), m->ctx->stream);
+    HIPCHECK(hipStreamSynchronize(m->ctx->stream));
+    for (int i = 0; i < m->n_dofs; ++i)
+        for (int a = 0; a < 3; ++a) m->x[3 * (size_t)m->dof_of_idof[i] + a] = X[4 * (size_t)i + a];
+    m->x_stale = false;
+}
+
 extern "C" int fgpu_mesh_export_dofs(const fgpu_mesh *m, double *xyz, int *vod) {
     if (!m) return FGPU_EINVAL;
-    if (xyz) std::memcpy(xyz, m->x.data(), m->x.size() * sizeof(double));
+    if (xyz) {
+        const int rc = guarded(m->ctx, [&] {
+            HIPCHECK(hipSetDevice(m->ctx->device));
+            refresh_host_x(m);
+            return FGPU_OK;
+        });
+        if (rc != FGPU_OK) return rc;
+        std::memcpy(xyz, m->x.data(), m->x.size() * sizeof(double));
+    }
     if (vod) std::memcpy(vod, m->vertex_of_dof.data(), m->vertex_of_dof.size() * sizeof(int));
     return FGPU_OK;
 }
@@ -315,6 +347,14 @@ extern "C" int fgpu_mesh_export_bfaces(const fgpu_mesh *m, int *cell, int *face,
 extern "C" int fgpu_mesh_export_face_centroids(const fgpu_mesh *m, int bid, double *xyz, int *n) {
     static const int kFV[6][4] = {{0, 2, 4, 6}, {1, 3, 5, 7}, {0, 1, 4, 5}, {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
     if (!m || !n) return FGPU_EINVAL;
+    if (xyz) {
+        const int rc = guarded(m->ctx, [&] {
+            HIPCHECK(hipSetDevice(m->ctx->device));
+            refresh_host_x(m);
+            return FGPU_OK;
+        });
+        if (rc != FGPU_OK) return rc;
+    }
     int k = 0;
     for (size_t f = 0; f < m->bf_cell.size(); ++f) {
         if (m->bf_id[f] != bid) continue;
@@ -330,10 +370,13 @@ extern "C" int fgpu_mesh_export_face_centroids(const fgpu_mesh *m, int bid, doub
     return FGPU_OK;
 }
 
+static double min_center_det(fgpu_mesh *m);
+
 extern "C" int fgpu_mesh_update_vertices(fgpu_mesh *m, const double *xyz) {
     if (!m || !xyz) return FGPU_EINVAL;
     return guarded(m->ctx, [&] {
         HIPCHECK(hipSetDevice(m->ctx->device));
+        refresh_host_x(m);
         std::vector<double> X(4 * (size_t)m->n_dofs, 0.0);
         for (int i = 0; i < m->n_dofs; ++i) {
             const size_t v = m->vertex_of_dof[m->dof_of_idof[i]];
@@ -342,15 +385,7 @@ extern "C" int fgpu_mesh_update_vertices(fgpu_mesh *m, const double *xyz) {
         h2d(m->d_X64, X.data(), X.size(), m->ctx->stream);
         // the import fixed each cell's orientation by the sign of its centre Jacobian; a move
         // that flips one would change the operator the import semantics define
-        const int nb = center_det_blocks(m->n_cells);
-        double *dmin = dalloc<double>(nb);
-        launch_center_det_min(m->ctx->stream, m->n_cells, m->d_cells, m->d_X64, dmin);
-        std::vector<double> bmin(nb);
-        d2h(bmin.data(), dmin, nb, m->ctx->stream);
-        HIPCHECK(hipStreamSynchronize(m->ctx->stream));
-        dfree(dmin);
-        double mn = 1e300;
-        for (double v : bmin) mn = std::min(mn, v);
+        const double mn = min_center_det(m);
         if (!(mn >= 0.0)) {
             std::vector<double> Xold(4 * (size_t)m->n_dofs, 0.0);
             for (int i = 0; i < m->n_dofs; ++i)
@@ -362,6 +397,141 @@ extern "C" int fgpu_mesh_update_vertices(fgpu_mesh *m, const double *xyz) {
         for (int d = 0; d < m->n_dofs; ++d)
             for (int a = 0; a < 3; ++a) m->x[3 * (size_t)d + a] = xyz[3 * (size_t)m->vertex_of_dof[d] + a];
         upload_coords(m, X);
+        m->coord_gen++;
+        return FGPU_OK;
+    });
+}
+
+// smallest centre Jacobian over the cells at the current device coordinates
+static double min_center_det(fgpu_mesh *m) {
+    const int nb = center_det_blocks(m->n_cells);
+    double *dmin = dalloc<double>(nb);
+    launch_center_det_min(m->ctx->stream, m->n_cells, m->d_cells, m->d_X64, dmin);
+    std::vector<double> bmin(nb);
+    d2h(bmin.data(), dmin, nb, m->ctx->stream);
+    HIPCHECK(hipStreamSynchronize(m->ctx->stream));
+    dfree(dmin);
+    double mn = 1e300;
+    for (double v : bmin) mn = std::min(mn, v);
+    return mn;
+}
+
+extern "C" int fgpu_mesh_move_vertices(fgpu_mesh *m, int n, const int *ids, const double *xyz) {
+    if (!m || n < 0 || (n > 0 && (!ids || !xyz))) return FGPU_EINVAL;
+    return guarded(m->ctx, [&] {
+        HIPCHECK(hipSetDevice(m->ctx->device));
+        hipStream_t st = m->ctx->stream;
+        refresh_host_x(m);
+        std::vector<int> idx;
+        std::vector<double> nx, ox;
+        idx.reserve(n);
+        for (int k = 0; k < n; ++k) {
+            if (ids[k] < 0 || ids[k] >= m->n_vertices) throw std::invalid_argument("vertex id out of range");
+            const int d = m->dof_of_vertex[ids[k]];
+            if (d < 0) continue;                          // vertex not used by any cell
+            idx.push_back(m->idof_of_dof[d]);
+            for (int a = 0; a < 3; ++a) {
+                nx.push_back(xyz[3 * (size_t)k + a]);
+                ox.push_back(m->x[3 * (size_t)d + a]);
+            }
+        }
+        const int cnt = (int)idx.size();
+        if (cnt == 0) return FGPU_OK;
+        int *d_idx = dalloc<int>(cnt);
+        double *d_xyz = dalloc<double>(3 * (size_t)cnt);
+        h2d(d_idx, idx.data(), cnt, st);
+        h2d(d_xyz, nx.data(), nx.size(), st);
+        launch_scatter_coords(st, cnt, d_idx, d_xyz, m->d_X64, m->d_X32);
+        const double mn = min_center_det(m);
+        if (!(mn >= 0.0)) {
+            h2d(d_xyz, ox.data(), ox.size(), st);
+            launch_scatter_coords(st, cnt, d_idx, d_xyz, m->d_X64, m->d_X32);
+            HIPCHECK(hipStreamSynchronize(st));
+            dfree(d_idx);
+            dfree(d_xyz);
+            throw MeshError("moved vertices turn a cell inside out (centre Jacobian " + std::to_string(mn) +
+                            "); coordinates left unchanged, the mesh must be rebuilt");
+        }
+        dfree(d_idx);
+        dfree(d_xyz);
+        for (int k = 0, j = 0; k < n; ++k) {
+            const int d = m->dof_of_vertex[ids[k]];
+            if (d < 0) continue;
+            for (int a = 0; a < 3; ++a) m->x[3 * (size_t)d + a] = nx[3 * (size_t)j + a];
+            ++j;
+        }
+        m->coord_gen++;
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_mesh_set_motion(fgpu_mesh *m, int rows, const int *vertex_ids, const int *ptr, const int *ctrl,
+                                    const double *w, int n_ctrl) {
+    if (!m || rows < 0 || n_ctrl < 0 || (rows > 0 && (!vertex_ids || !ptr || !ctrl || !w))) return FGPU_EINVAL;
+    return guarded(m->ctx, [&] {
+        HIPCHECK(hipSetDevice(m->ctx->device));
+        hipStream_t st = m->ctx->stream;
+        refresh_host_x(m);
+        // rows of unused vertices are dropped
+        std::vector<int> idof, p{0}, cc;
+        std::vector<double> ww, base;
+        for (int r = 0; r < rows; ++r) {
+            if (vertex_ids[r] < 0 || vertex_ids[r] >= m->n_vertices) throw std::invalid_argument("vertex id out of range");
+            if (ptr[r + 1] < ptr[r]) throw std::invalid_argument("motion rows must be ordered");
+            const int d = m->dof_of_vertex[vertex_ids[r]];
+            if (d < 0) continue;
+            idof.push_back(m->idof_of_dof[d]);
+            for (int a = 0; a < 3; ++a) base.push_back(m->x[3 * (size_t)d + a]);
+            for (int k = ptr[r]; k < ptr[r + 1]; ++k) {
+                if (ctrl[k] < 0 || ctrl[k] >= n_ctrl) throw std::invalid_argument("control index out of range");
+                cc.push_back(ctrl[k]);
+                ww.push_back(w[k]);
+            }
+            p.push_back((int)cc.size());
+        }
+        Mesh::Motion &M = m->motion;
+        dfree(M.idof); dfree(M.ptr); dfree(M.ctrl); dfree(M.w); dfree(M.base); dfree(M.disp); dfree(M.prev);
+        M = Mesh::Motion{};
+        M.rows = (int)idof.size();
+        M.n_ctrl = n_ctrl;
+        M.idof = dalloc<int>(idof.size());
+        M.ptr = dalloc<int>(p.size());
+        M.ctrl = dalloc<int>(cc.size());
+        M.w = dalloc<double>(ww.size());
+        M.base = dalloc<double>(base.size());
+        M.disp = dalloc<double>(3 * (size_t)n_ctrl);
+        M.prev = dalloc<double>(3 * (size_t)n_ctrl);
+        h2d(M.idof, idof.data(), idof.size(), st);
+        h2d(M.ptr, p.data(), p.size(), st);
+        h2d(M.ctrl, cc.data(), cc.size(), st);
+        h2d(M.w, ww.data(), ww.size(), st);
+        h2d(M.base, base.data(), base.size(), st);
+        HIPCHECK(hipStreamSynchronize(st));
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_mesh_displace(fgpu_mesh *m, const double *disp) {
+    if (!m || !disp) return FGPU_EINVAL;
+    return guarded(m->ctx, [&] {
+        HIPCHECK(hipSetDevice(m->ctx->device));
+        hipStream_t st = m->ctx->stream;
+        Mesh::Motion &M = m->motion;
+        if (!M.idof) throw std::invalid_argument("no motion map set (fgpu_mesh_set_motion)");
+        h2d(M.disp, disp, 3 * (size_t)M.n_ctrl, st);
+        launch_motion(st, M.rows, M.idof, M.ptr, M.ctrl, M.w, M.base, M.disp, m->d_X64, m->d_X32);
+        const double mn = min_center_det(m);
+        if (!(mn >= 0.0)) {
+            // back to the last accepted displacement (the base positions before the first)
+            if (!M.have_prev) launch_set_const(st, 3 * M.n_ctrl, 0.0, M.prev);
+            launch_motion(st, M.rows, M.idof, M.ptr, M.ctrl, M.w, M.base, M.prev, m->d_X64, m->d_X32);
+            HIPCHECK(hipStreamSynchronize(st));
+            throw MeshError("displacement turns a cell inside out (centre Jacobian " + std::to_string(mn) +
+                            "); coordinates left unchanged, the mesh must be rebuilt");
+        }
+        std::swap(M.disp, M.prev);
+        M.have_prev = true;
+        m->x_stale = true;
         m->coord_gen++;
         return FGPU_OK;
     });
@@ -708,6 +878,7 @@ extern "C" int fgpu_sys_reset(fgpu_sys *s) {
         std::fill(s->dmask.begin(), s->dmask.end(), 0);
         std::fill(s->gval.begin(), s->gval.end(), 0.0);
         HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+        s->clear_terms();
         return FGPU_OK;
     });
 }
@@ -720,17 +891,15 @@ extern "C" int fgpu_sys_add_neumann(fgpu_sys *s, int bid, const double *fv, doub
         HIPCHECK(hipSetDevice(s->ctx->device));
         const size_t nb = m->bf_cell.size();
         std::vector<double> val(nb, 0.0);
-        std::vector<int> ids(m->bf_id);
         int k = 0;
         for (size_t f = 0; f < nb; ++f)
             if (m->bf_id[f] == bid) val[f] = fv ? fv[k++] : cst;
-        int *d_id = dalloc<int>(nb);
-        h2d(d_id, ids.data(), nb, st);
-        h2d(s->fval, val.data(), nb, st);
-        launch_face_load(st, (int)nb, m->d_bf_icell, m->d_bf_face, d_id, bid, s->fval, m->d_cells, m->d_X64, s->fslot);
+        double *dval = dalloc<double>(nb);
+        s->rhs_terms.push_back({0, bid, dval});
+        h2d(dval, val.data(), nb, st);
+        launch_face_load(st, (int)nb, m->d_bf_icell, m->d_bf_face, m->d_bf_id, bid, dval, m->d_cells, m->d_X64, s->fslot);
         launch_face_gather(st, m->n_dofs, m->d_fgptr, m->d_fgidx, s->fslot, s->b);
         HIPCHECK(hipStreamSynchronize(st));
-        dfree(d_id);
         return FGPU_OK;
     });
 }
@@ -746,6 +915,7 @@ extern "C" int fgpu_sys_add_cell_rhs(fgpu_sys *s, const double *fq) {
         for (size_t i = 0; i < nc; ++i)
             for (int q = 0; q < 8; ++q) f[8 * i + q] = fq[8 * (size_t)m->cell_of_icell[i] + q];
         double *d_f = dalloc<double>(8 * nc);
+        s->rhs_terms.push_back({1, 0, d_f});
         h2d(d_f, f.data(), f.size(), st);
         Desc d = make_desc64(s, true);
         d.q = s->b;
@@ -753,7 +923,6 @@ extern "C" int fgpu_sys_add_cell_rhs(fgpu_sys *s, const double *fq) {
         launch_elem_rhs(st, t.sets[0], t.etab, t.ne, d_f, t.lds64);
         launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, true);
         HIPCHECK(hipStreamSynchronize(st));
-        dfree(d_f);
         return FGPU_OK;
     });
 }
@@ -772,6 +941,55 @@ extern "C" int fgpu_sys_add_dirichlet(fgpu_sys *s, int bid, double value) {
     return FGPU_OK;
 }
 
+// Jacobi diagonal and the Dirichlet lifting from the device mask and values: A g with the
+// un-lifted operator, then b_i -= (A g)_i on free rows, b_i = A_ii g_i and x_i = g_i on
+// constrained ones
+static void lift(fgpu_sys *s) {
+    fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+    hipStream_t st = s->ctx->stream;
+    const int n = m->n_dofs;
+    ensure_diag(s);
+    launch_invdiag(st, n, s->d_dmask, s->diag, s->invd, s->invd32);
+    Desc d = make_desc64(s, true);
+    d.ua = s->g;
+    d.q = s->tmp;
+    BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
+    launch_elem<double>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds64);
+    launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, false);
+    launch_lift(st, n, s->d_dmask, s->g, s->tmp, s->diag, s->b, s->x);
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
+// Rebuilds the right-hand side, the diagonal and the lifting for the current vertex positions
+// from the terms recorded since the last reset (same kernels, same order, so the result is
+// the one a reset and the same calls would give)
+extern "C" int fgpu_sys_reassemble(fgpu_sys *s) {
+    if (!s) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        hipStream_t st = s->ctx->stream;
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        launch_set_const(st, m->n_dofs, 0.0, s->b);
+        const size_t nb = m->bf_cell.size();
+        for (const auto &term : s->rhs_terms) {
+            if (term.kind == 0) {
+                launch_face_load(st, (int)nb, m->d_bf_icell, m->d_bf_face, m->d_bf_id, term.bid, term.dval, m->d_cells,
+                                 m->d_X64, s->fslot);
+                launch_face_gather(st, m->n_dofs, m->d_fgptr, m->d_fgidx, s->fslot, s->b);
+            } else {
+                Desc d = make_desc64(s, true);
+                d.q = s->b;
+                BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
+                launch_elem_rhs(st, t.sets[0], t.etab, t.ne, term.dval, t.lds64);
+                launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, true);
+            }
+        }
+        if (s->lifted) lift(s);
+        HIPCHECK(hipStreamSynchronize(st));
+        return FGPU_OK;
+    });
+}
+
 extern "C" int fgpu_sys_apply_dirichlet(fgpu_sys *s) {
     if (!s) return FGPU_EINVAL;
     return guarded(s->ctx, [&] {
@@ -788,17 +1006,8 @@ extern "C" int fgpu_sys_apply_dirichlet(fgpu_sys *s) {
         }
         h2d(s->d_dmask, mk.data(), n, st);
         h2d(s->g, gv.data(), n, st);
-        ensure_diag(s);
-        launch_invdiag(st, n, s->d_dmask, s->diag, s->invd, s->invd32);
-        // A g with the un-lifted operator
-        Desc d = make_desc64(s, true);
-        d.ua = s->g;
-        d.q = s->tmp;
-        BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
-        launch_elem<double>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds64);
-        launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, false);
-        launch_lift(st, n, s->d_dmask, s->g, s->tmp, s->diag, s->b, s->x);
-        HIPCHECK(hipStreamSynchronize(st));
+        lift(s);
+        s->lifted = true;
         return FGPU_OK;
     });
 }
@@ -1087,6 +1296,8 @@ static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_
 // ------------------------------------------------------------------------------------------
 // continuous batching: systems join and leave a running batch between steps
 // ------------------------------------------------------------------------------------------
+static double time_launches(fgpu_ctx *c, const Desc *D, const BatchTabs &t, bool coef, int which, int precision, int reps);
+
 struct fgpu_batch {
     fgpu_ctx *ctx = nullptr;
     fgpu_solve_opts o{};
@@ -1204,6 +1415,21 @@ extern "C" int fgpu_batch_step(fgpu_batch *b, int iterations, int *n_running) {
     });
 }
 
+extern "C" int fgpu_batch_time_kernel(fgpu_batch *b, int which, int reps, double *avg_ms) {
+    if (!b || !avg_ms || reps <= 0) return FGPU_EINVAL;
+    fgpu_ctx *c = b->ctx;
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        if (b->sys.empty()) throw std::invalid_argument("empty batch");
+        std::vector<std::vector<Desc>> sets;
+        make_sets(b->sys, sets);
+        BatchTabs t = upload_batch(c, sets, b->sys);
+        const bool mixed = b->o.precision == FGPU_MIXED;
+        *avg_ms = time_launches(c, mixed ? t.sets[2] : t.sets[1], t, b->coef > 0, which, b->o.precision, reps);
+        return FGPU_OK;
+    });
+}
+
 extern "C" int fgpu_batch_retire(fgpu_batch *b, int max, fgpu_sys **out, fgpu_solve_stats *stats, int *n_out) {
     if (!b || !n_out || (max > 0 && !out)) return FGPU_EINVAL;
     fgpu_ctx *c = b->ctx;
@@ -1246,6 +1472,33 @@ extern "C" int fgpu_sys_solve(fgpu_sys *s, const fgpu_solve_opts *o, fgpu_solve_
 // kernel timing (benchmark helper): average duration of `reps` back-to-back launches of one
 // kernel of the solve, measured with HIP events on the context stream
 // ------------------------------------------------------------------------------------------
+// average duration of `reps` back-to-back launches of the element apply (which 0) or of the
+// dof gather (which 1) over descriptor set D, HIP events on the context's stream
+static double time_launches(fgpu_ctx *c, const Desc *D, const BatchTabs &t, bool coef, int which, int precision, int reps) {
+    auto one = [&] {
+        if (which == 0) {
+            if (precision == FGPU_MIXED) launch_elem<float>(c->stream, D, t.etab, t.ne, coef, false, false, S_BETA, t.lds32);
+            else launch_elem<double>(c->stream, D, t.etab, t.ne, coef, false, false, S_BETA, t.lds64);
+        } else {
+            if (precision == FGPU_MIXED) launch_gather_plain<float>(c->stream, D, t.dtab, t.nd, false);
+            else launch_gather_plain<double>(c->stream, D, t.dtab, t.nd, false);
+        }
+    };
+    one();
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; ++i) one();
+    HIPCHECK(hipEventRecord(e1, c->stream));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / reps;
+}
+
 extern "C" int fgpu_sys_time_kernel(fgpu_sys *s, int which, int precision, int reps, double *avg_ms) {
     if (!s || !avg_ms || reps <= 0) return FGPU_EINVAL;
     return guarded(s->ctx, [&] {
@@ -1255,28 +1508,7 @@ extern "C" int fgpu_sys_time_kernel(fgpu_sys *s, int which, int precision, int r
         Desc d = precision == FGPU_MIXED ? make_desc32(s) : make_desc64(s, true);
         // inputs: the current z / x vectors; outputs: the slot buffer and q
         BatchTabs t = upload_batch(c, {{d}}, {s});
-        auto one = [&] {
-            if (which == 0) {
-                if (precision == FGPU_MIXED) launch_elem<float>(c->stream, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds32);
-                else launch_elem<double>(c->stream, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds64);
-            } else {
-                if (precision == FGPU_MIXED) launch_gather_plain<float>(c->stream, t.sets[0], t.dtab, t.nd, false);
-                else launch_gather_plain<double>(c->stream, t.sets[0], t.dtab, t.nd, false);
-            }
-        };
-        one();
-        hipEvent_t e0, e1;
-        HIPCHECK(hipEventCreate(&e0));
-        HIPCHECK(hipEventCreate(&e1));
-        HIPCHECK(hipEventRecord(e0, c->stream));
-        for (int i = 0; i < reps; ++i) one();
-        HIPCHECK(hipEventRecord(e1, c->stream));
-        HIPCHECK(hipEventSynchronize(e1));
-        float ms = 0;
-        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        *avg_ms = ms / reps;
+        *avg_ms = time_launches(c, t.sets[0], t, s->have_coef, which, precision, reps);
         return FGPU_OK;
     });
 }

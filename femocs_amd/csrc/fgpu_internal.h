@@ -103,6 +103,15 @@ struct Mesh {
     int n_slots = 0;                       // slot-buffer entries (sum of patch node counts)
     int patch_gen = 0;                     // bumped whenever the patches are rebuilt
     int coord_gen = 0;                     // bumped whenever the vertices move
+    bool x_stale = false;                  // device coordinates moved, host copy `x` not yet
+    // linear mesh motion (fgpu_mesh_set_motion): row r (dof idof[r]) sits at base[r] +
+    // sum_k w[k] * disp[ctrl[k]] for k in ptr[r]..ptr[r+1]
+    struct Motion {
+        int rows = 0, n_ctrl = 0;
+        int *idof = nullptr, *ptr = nullptr, *ctrl = nullptr;
+        double *w = nullptr, *base = nullptr, *disp = nullptr, *prev = nullptr;
+        bool have_prev = false;
+    } motion;
     std::vector<int> icells;               // 8*n_cells, internal order and dofs (host copy)
     // device arrays (internal numbering)
     int *d_cells = nullptr;                // 8*n_cells
@@ -112,6 +121,7 @@ struct Mesh {
     int *d_pcell0 = nullptr, *d_pnode0 = nullptr, *d_pnodes = nullptr, *d_pslot = nullptr;
     uint16_t *d_plidx = nullptr, *d_plptr = nullptr, *d_plent = nullptr;
     int *d_bf_icell = nullptr, *d_bf_face = nullptr;       // boundary faces (API order)
+    int *d_bf_id = nullptr;                                  // boundary id per face
     int *d_fgptr = nullptr, *d_fgidx = nullptr;            // dof -> boundary-face slots
     int *d_v2c = nullptr, *d_v2n = nullptr;                 // gradient export (internal)
     ~Mesh();
@@ -140,6 +150,16 @@ struct Sys {
     int ppb = kPatchPerBlock;     // patches per element workgroup (tunable)
     int patch_gen = -1;           // mesh patch generation the slot buffers are sized for
     int diag_gen = -1;            // mesh coordinate generation the diagonal was computed for
+    // right-hand-side terms since the last reset, replayed by fgpu_sys_reassemble after the
+    // vertices moved: Neumann face values (one per boundary face) or cell values (8 per cell)
+    struct RhsTerm {
+        int kind;                 // 0: Neumann on boundary id `bid`, 1: cell source
+        int bid;
+        double *dval;
+    };
+    std::vector<RhsTerm> rhs_terms;
+    bool lifted = false;          // fgpu_sys_apply_dirichlet ran since the last reset
+    void clear_terms();
     ~Sys();
 };
 

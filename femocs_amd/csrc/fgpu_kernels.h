@@ -29,6 +29,9 @@ void launch_face_load(hipStream_t s, int nb, const int *bf_icell, const int *bf_
 void launch_face_gather(hipStream_t s, int n, const int *fgptr, const int *fgidx, const double *fslot, double *b);
 void launch_minmax(hipStream_t s, int n, const double *x, double *out);
 int center_det_blocks(int n);
+void launch_motion(hipStream_t s, int rows, const int *idof, const int *ptr, const int *ctrl, const double *w,
+                   const double *base, const double *disp, double *X64, float *X32);
+void launch_scatter_coords(hipStream_t s, int n, const int *idx, const double *xyz, double *X64, float *X32);
 void launch_center_det_min(hipStream_t s, int n, const int *cells, const double *X, double *out);
 void launch_grad_export(hipStream_t s, int n, const int *v2c, const int *v2n, const int *cells, const double *X,
                         const double *u, double *grad);

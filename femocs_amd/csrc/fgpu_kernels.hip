@@ -470,6 +470,39 @@ __global__ __launch_bounds__(kBlock) void k_minmax(int n, const double *__restri
     }
 }
 
+// new positions of a subset of dofs (internal numbering) in both coordinate copies
+__global__ void k_scatter_coords(int n, const int *__restrict__ idx, const double *__restrict__ xyz, double4 *__restrict__ X64,
+                                 float4 *__restrict__ X32) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = idx[k];
+    const double x = xyz[3 * (size_t)k], y = xyz[3 * (size_t)k + 1], z = xyz[3 * (size_t)k + 2];
+    X64[i] = make_double4(x, y, z, 0.0);
+    X32[i] = make_float4((float)x, (float)y, (float)z, 0.f);
+}
+
+// linear mesh motion: row r of the map moves dof idof[r] to base[r] + sum_k w[k] disp[ctrl[k]]
+__global__ void k_motion(int rows, const int *__restrict__ idof, const int *__restrict__ ptr, const int *__restrict__ ctrl,
+                         const double *__restrict__ w, const double *__restrict__ base, const double *__restrict__ disp,
+                         double4 *__restrict__ X64, float4 *__restrict__ X32) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    double x = 0, y = 0, z = 0;
+    for (int k = ptr[r]; k < ptr[r + 1]; ++k) {
+        const double wk = w[k];
+        const double *d = disp + 3 * (size_t)ctrl[k];
+        x += wk * d[0];
+        y += wk * d[1];
+        z += wk * d[2];
+    }
+    x += base[3 * (size_t)r];
+    y += base[3 * (size_t)r + 1];
+    z += base[3 * (size_t)r + 2];
+    const int i = idof[r];
+    X64[i] = make_double4(x, y, z, 0.0);
+    X32[i] = make_float4((float)x, (float)y, (float)z, 0.f);
+}
+
 // per block: the smallest Jacobian determinant at the cell centres of its cells, in the
 // orientation fixed at import (same formula as the import's orientation test)
 __global__ __launch_bounds__(kBlock) void k_center_det_min(int n, const int *__restrict__ cells, const double4 *__restrict__ X,
@@ -613,6 +646,17 @@ void launch_minmax(hipStream_t s, int n, const double *x, double *out) {
     hipLaunchKernelGGL(k_minmax, dim3(1), dim3(kBlock), 0, s, n, x, out);
 }
 int center_det_blocks(int n) { return grid_for(n); }
+void launch_motion(hipStream_t s, int rows, const int *idof, const int *ptr, const int *ctrl, const double *w,
+                   const double *base, const double *disp, double *X64, float *X32) {
+    if (rows > 0)
+        hipLaunchKernelGGL(k_motion, dim3(grid_for(rows)), dim3(kBlock), 0, s, rows, idof, ptr, ctrl, w, base, disp,
+                           reinterpret_cast<double4 *>(X64), reinterpret_cast<float4 *>(X32));
+}
+void launch_scatter_coords(hipStream_t s, int n, const int *idx, const double *xyz, double *X64, float *X32) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_scatter_coords, dim3(grid_for(n)), dim3(kBlock), 0, s, n, idx, xyz,
+                           reinterpret_cast<double4 *>(X64), reinterpret_cast<float4 *>(X32));
+}
 void launch_center_det_min(hipStream_t s, int n, const int *cells, const double *X, double *out) {
     if (n > 0)
         hipLaunchKernelGGL(k_center_det_min, dim3(grid_for(n)), dim3(kBlock), 0, s, n, cells,

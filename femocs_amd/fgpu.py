@@ -78,6 +78,7 @@ API = [
     ("fgpu_sys_apply", C.c_int, [_P, _D, _D]),
     ("fgpu_sys_solve", C.c_int, [_P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     ("fgpu_solve_batch", C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    ("fgpu_batch_time_kernel", C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     ("fgpu_sys_minmax", C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("fgpu_sys_solution_grad", C.c_int, [_P, _D]),
     ("fgpu_sys_time_kernel", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
@@ -85,6 +86,10 @@ API = [
     ("fgpu_mesh_patch_info", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),
     ("fgpu_sys_set_patches_per_block", C.c_int, [_P, C.c_int]),
+    ("fgpu_mesh_move_vertices", C.c_int, [_P, C.c_int, _I, _D]),
+    ("fgpu_mesh_set_motion", C.c_int, [_P, C.c_int, _I, _I, _I, _D, C.c_int]),
+    ("fgpu_mesh_displace", C.c_int, [_P, _D]),
+    ("fgpu_sys_reassemble", C.c_int, [_P]),
     ("fgpu_mesh_device_bytes", C.c_int, [_P, C.POINTER(C.c_size_t)]),
     ("fgpu_sys_device_bytes", C.c_int, [_P, C.POINTER(C.c_size_t)]),
     ("fgpu_batch_create", C.c_int, [_P, C.POINTER(SolveOpts), C.POINTER(_P)]),
@@ -198,6 +203,32 @@ class Mesh:
         self.ctx.lib.fgpu_mesh_device_bytes(self.h, C.byref(b))
         return b.value
 
+    def set_motion(self, ids, W):
+        """Rows ids follow the control points through the sparse weights W (len(ids) x n_ctrl)."""
+        from scipy.sparse import csr_matrix
+        W = csr_matrix(W)
+        ids = np.ascontiguousarray(ids, np.int32)
+        if W.shape[0] != ids.size:
+            raise ValueError("one weight row per vertex id")
+        self.n_ctrl = W.shape[1]
+        self.ctx.check(self.ctx.lib.fgpu_mesh_set_motion(
+            self.h, int(ids.size), ids, np.ascontiguousarray(W.indptr, np.int32),
+            np.ascontiguousarray(W.indices, np.int32), np.ascontiguousarray(W.data, np.float64), int(W.shape[1])),
+            "fgpu_mesh_set_motion")
+
+    def displace(self, disp):
+        d = np.ascontiguousarray(disp, np.float64).reshape(-1)
+        if d.size != 3 * self.n_ctrl:
+            raise ValueError("3 displacement components per control point")
+        self.ctx.check(self.ctx.lib.fgpu_mesh_displace(self.h, d), "fgpu_mesh_displace")
+
+    def move_vertices(self, ids, xyz):
+        ids = np.ascontiguousarray(ids, np.int32)
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        if xyz.size != 3 * ids.size:
+            raise ValueError("xyz must hold 3 coordinates per vertex id")
+        self.ctx.check(self.ctx.lib.fgpu_mesh_move_vertices(self.h, int(ids.size), ids, xyz), "fgpu_mesh_move_vertices")
+
     def set_patch_cells(self, cells: int):
         self.ctx.check(self.ctx.lib.fgpu_mesh_set_patch_cells(self.h, int(cells)), "fgpu_mesh_set_patch_cells")
 
@@ -250,6 +281,9 @@ class System:
 
     def apply_dirichlet(self):
         self._c(self.ctx.lib.fgpu_sys_apply_dirichlet(self.h), "apply_dirichlet")
+
+    def reassemble(self):
+        self._c(self.ctx.lib.fgpu_sys_reassemble(self.h), "reassemble")
 
     def fill_solution(self, v):
         self._c(self.ctx.lib.fgpu_sys_fill_solution(self.h, float(v)), "fill_solution")
@@ -331,6 +365,12 @@ class Batch:
         n = C.c_int()
         self.ctx.check(self.ctx.lib.fgpu_batch_step(self.h, int(iterations), C.byref(n)), "fgpu_batch_step")
         return n.value
+
+    def time_kernel(self, which=0, reps=50) -> float:
+        ms = C.c_double()
+        self.ctx.check(self.ctx.lib.fgpu_batch_time_kernel(self.h, int(which), int(reps), C.byref(ms)),
+                       "fgpu_batch_time_kernel")
+        return ms.value
 
     def retire(self):
         """Removes the finished members; returns [(System, SolveStats)]."""

@@ -201,10 +201,12 @@ def generator_points(g: TipGeometry, atoms: np.ndarray, rng: np.random.Generator
     surf = surf[np.all((surf > lo + 0.25 * g.h_min) & (surf < hi - 0.25 * g.h_min), axis=1)]
     # interior points must not crowd the box faces either
     inner = interior[np.all((interior > lo + 0.3 * g.h_min) & (interior < hi - 0.3 * g.h_min), axis=1)]
-    pts = np.concatenate([surf, inner, bp, corners])
-    # exact duplicates (box edges are touched from two faces) are merged
-    pts = np.unique(np.round(pts, 9), axis=0)
-    return pts, len(surf)
+    pts = np.round(np.concatenate([surf, inner, bp, corners]), 9)
+    # exact duplicates (box edges are touched from two faces) are merged; the first copy is
+    # kept in place, so the surface generators stay in front
+    _, first = np.unique(pts, axis=0, return_index=True)
+    first = np.sort(first)
+    return pts[first], int(np.sum(first < len(surf)))
 
 
 # ----------------------------------------------------------------------------------------

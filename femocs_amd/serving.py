@@ -87,6 +87,10 @@ class Client:
         W = M.hex_node_weights(tip.n_tetnodes, tip.tets)
         ns = tip.n_surface_nodes
         self.W_surf = W[:, :ns].tocsr()
+        # the nodes a surface move reaches (the surface vertices and the edge, face and tet
+        # centres around them): only these are sent to the server
+        self.moved_ids = np.flatnonzero(np.diff(self.W_surf.indptr) > 0).astype(np.int32)
+        self.W_moved = self.W_surf[self.moved_ids]
         self.limit = cap * vertex_altitudes(tip.nodes[: tip.n_tetnodes], tip.tets)[:ns]
         self.seq = 0
 
@@ -99,13 +103,21 @@ class Client:
     def next_positions(self) -> np.ndarray:
         return self.X0 + self.W_surf @ self.displacements()
 
+    def next_move(self):
+        """(vertex ids, their new positions) of the next timestep."""
+        return self.moved_ids, self.X0[self.moved_ids] + self.W_moved @ self.displacements()
+
+    def positions_after(self, move) -> np.ndarray:
+        X = self.X0.copy()
+        X[move[0]] = move[1]
+        return X
+
 
 @dataclass
 class _Resident:
     mesh: fgpu.Mesh
     sys: fgpu.System
     bytes: int
-    mesh_positions: np.ndarray
     fresh: bool = True                       # solution still holds the initial fill
 
 
@@ -114,6 +126,8 @@ class ServeStats:
     requests: list = field(default_factory=list)
     evictions: int = 0
     rejected_moves: int = 0
+    prep_seconds: float = 0.0                # host time assembling requests (GPU idle meanwhile)
+    step_seconds: float = 0.0                # time in batch steps and retirement
     t_start: float = 0.0
     t_end: float = 0.0
 
@@ -133,6 +147,8 @@ class ServeStats:
             "mean_iterations": float(its.mean()),
             "evictions": self.evictions,
             "rejected_moves": self.rejected_moves,
+            "prep_seconds": round(self.prep_seconds, 4),
+            "step_seconds": round(self.step_seconds, 4),
             "seconds": span,
         }
 
@@ -140,7 +156,7 @@ class ServeStats:
 class Server:
     def __init__(self, ctx: fgpu.Context, clients: list[Client], opts: fgpu.SolveOpts,
                  budget_bytes: int = 64 << 30, batched: bool = True, warm: bool = True,
-                 step_iterations: int = 32, max_batch: int = 64, record_inputs: bool = False):
+                 step_iterations: int = 32, max_batch: int = 64, record_inputs: bool = False, configure=None):
         self.ctx = ctx
         self.clients = clients
         self.opts = opts
@@ -150,6 +166,7 @@ class Server:
         self.step_iterations = step_iterations
         self.max_batch = max_batch
         self.record_inputs = record_inputs
+        self.configure = configure                    # called as configure(mesh, system) on import
         self.resident: OrderedDict[int, _Resident] = OrderedDict()   # LRU order
         self.known_bytes: dict[int, int] = {}
         self.pending: deque[Request] = deque()
@@ -179,33 +196,47 @@ class Server:
         if self.used_bytes() + need > self.budget:
             return None
         m = fgpu.Mesh(self.ctx, c.X0, c.cells, *POISSON_MARKS)
+        m.set_motion(c.moved_ids, c.W_moved)
         s = fgpu.System(m)
+        if self.configure is not None:
+            self.configure(m, s)
         s.fill_solution(0.0)
         b = m.device_bytes() + s.device_bytes()
         self.known_bytes[c.cid] = b
-        r = _Resident(m, s, b, c.X0)
+        r = _Resident(m, s, b)
         self.resident[c.cid] = r
         return r
 
     def _assemble(self, c: Client, r: _Resident, req: Request):
-        X = c.next_positions()
+        t0 = time.perf_counter()
         try:
-            r.mesh.update_vertices(X)
+            self._assemble_timed(c, r, req)
+        finally:
+            self.stats.prep_seconds += time.perf_counter() - t0
+
+    def _assemble_timed(self, c: Client, r: _Resident, req: Request):
+        try:
+            r.mesh.displace(c.displacements())
         except fgpu.MeshInvertedError:
             # the step would need a new mesh; this request is solved on the last valid one
             self.stats.rejected_moves += 1
-            X = r.mesh_positions
-        r.mesh_positions = X
         s = r.sys
-        s.reset()
-        s.add_dirichlet(2, 0.0)
-        s.add_neumann(8, None, c.field)
-        s.apply_dirichlet()
-        if not self.warm or r.fresh:
+        if r.fresh:
+            s.reset()
+            s.add_dirichlet(2, 0.0)
+            s.add_neumann(8, None, c.field)
+            s.apply_dirichlet()
             s.fill_solution(0.0)
-        r.fresh = False
+            r.fresh = False
+        else:
+            # same boundary terms, new geometry: rebuilt on the device
+            s.reassemble()
+            if not self.warm:
+                s.fill_solution(0.0)
         if self.record_inputs:
-            req.positions = X
+            xyz, vod = r.mesh.dofs()
+            req.positions = c.X0.copy()
+            req.positions[vod] = xyz
             req.x_init = s.solution()
 
     # ---- request flow
@@ -258,18 +289,22 @@ class Server:
             self._assemble(c, r, req)
             req.t_admit = clock()
             self.busy[req.client] = req
+            t0 = time.perf_counter()
             st = r.sys.solve(self.opts)
             self._finish(req, st, clock())
+            self.stats.step_seconds += time.perf_counter() - t0
             return [req]
         self._admit(clock())
         if len(self.batch) == 0:
             return done
+        t0 = time.perf_counter()
         self.batch.step(self.step_iterations)
         now = clock()
         for s, st in self.batch.retire():
             req = self.by_sys.pop(s.h.value)
             self._finish(req, st, now)
             done.append(req)
+        self.stats.step_seconds += time.perf_counter() - t0
         return done
 
     def idle(self) -> bool:

@@ -91,6 +91,18 @@ int fgpu_mesh_export_face_centroids(const fgpu_mesh *mesh, int bid, double *xyz,
  * FGPU_EMESH, coordinates unchanged, when a cell's centre Jacobian changes sign: the
  * orientation chosen at import would no longer be the one a fresh import picks. */
 int fgpu_mesh_update_vertices(fgpu_mesh *mesh, const double *xyz);
+/* the same for a subset: n vertices (import numbering) get new positions xyz (3*n); for
+ * callers that move few vertices per step (surface atoms), no counterpart in the reference */
+int fgpu_mesh_move_vertices(fgpu_mesh *mesh, int n, const int *vertex_ids, const double *xyz);
+/* linear mesh motion computed on the device, no counterpart in the reference: `rows` vertices
+ * (import numbering) follow n_ctrl control points, vertex_ids[r] sitting at its position at
+ * this call plus sum_k w[k] * disp[ctrl[k]] for k in ptr[r]..ptr[r+1]-1 (e.g. the Tethex
+ * edge, face and tet centres following the surface atoms they average).  fgpu_mesh_displace
+ * sets the displacements (3*n_ctrl, relative to the positions at fgpu_mesh_set_motion);
+ * FGPU_EMESH as for fgpu_mesh_update_vertices. */
+int fgpu_mesh_set_motion(fgpu_mesh *mesh, int rows, const int *vertex_ids, const int *ptr, const int *ctrl,
+                         const double *w, int n_ctrl);
+int fgpu_mesh_displace(fgpu_mesh *mesh, const double *disp);
 /* device memory held by a mesh / a system (bytes), for admission by memory budget; no
  * counterpart in the reference */
 int fgpu_mesh_device_bytes(const fgpu_mesh *mesh, size_t *bytes);
@@ -127,6 +139,11 @@ int fgpu_sys_add_dirichlet(fgpu_sys *sys, int bid, double value);
 /* DealSolver::apply_dirichlet (src/DealSolver.cpp:438-440): lifting with column elimination;
  * sets solution = value on the constrained dofs */
 int fgpu_sys_apply_dirichlet(fgpu_sys *sys);
+/* after the vertices moved: rebuilds the right-hand side terms added since the last reset
+ * (Neumann loads, cell sources), the diagonal and the Dirichlet lifting on the device, as the
+ * same calls would after a reset; no counterpart in the reference (it reassembles from scratch,
+ * src/PoissonSolver.cpp:170-210) */
+int fgpu_sys_reassemble(fgpu_sys *sys);
 /* solution = value everywhere (DealSolver::setup_system, src/DealSolver.cpp:383-384) */
 int fgpu_sys_fill_solution(fgpu_sys *sys, double value);
 /* DealSolver::import_solution / export_solution (src/DealSolver.cpp:270-315), dof order */
@@ -154,6 +171,10 @@ int fgpu_batch_size(const fgpu_batch *batch);
 int fgpu_batch_add(fgpu_batch *batch, fgpu_sys *sys);
 int fgpu_batch_step(fgpu_batch *batch, int iterations, int *n_running);
 int fgpu_batch_retire(fgpu_batch *batch, int max, fgpu_sys **out, fgpu_solve_stats *stats, int *n_out);
+/* average duration (ms) of one launch of the element apply (which 0) or of the dof gather
+ * (which 1) over all members, in the batch's precision; for measurement, no counterpart in
+ * the reference.  Overwrites the members' slot buffers. */
+int fgpu_batch_time_kernel(fgpu_batch *batch, int which, int reps, double *avg_ms);
 /* DealSolver::check_limits / stat (src/DealSolver.cpp:158-167) */
 int fgpu_sys_minmax(fgpu_sys *sys, double *min, double *max);
 /* DealSolver::export_solution_grad (src/DealSolver.cpp:281-301): -grad u at the Gauss point

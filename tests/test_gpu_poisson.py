@@ -196,3 +196,64 @@ def test_inverting_move_is_rejected(ctx, small_mesh):
     s.solve(o)
     assert np.array_equal(s.solution(), x0)
     del s2
+
+
+def test_move_and_reassemble_equal_fresh_assembly(ctx, small_mesh):
+    """Moving a vertex subset and rebuilding the system on the device gives the right-hand
+    side, lifting and solution of a fresh import of the moved mesh assembled from scratch."""
+    from femocs_amd import serving
+    c = serving.Client(0, small_mesh, sigma=0.1, seed=5)
+    s = _poisson_system(ctx, small_mesh)
+    ids, xyz = c.next_move()
+    s.mesh.move_vertices(ids, xyz)
+    s.reassemble()
+    s.fill_solution(0.0)
+    gm = fgpu.Mesh(ctx, small_mesh.nodes, small_mesh.vacuum_cells(), *MARKS)
+    gm.update_vertices(c.positions_after((ids, xyz)))
+    f = fgpu.System(gm)
+    f.reset()
+    f.add_dirichlet(2, 0.0)
+    f.add_neumann(8, None, -1.0)
+    f.apply_dirichlet()
+    f.fill_solution(0.0)
+    assert np.array_equal(s.rhs(), f.rhs())
+    o = fgpu.opts(tol=1e-10, relative=True, precision=fgpu.FGPU_MIXED)
+    st1, st2 = s.solve(o), f.solve(o)
+    assert st1.iterations == st2.iterations > 0
+    assert np.array_equal(s.solution(), f.solution())
+
+
+def test_device_motion_map(ctx, small_mesh):
+    """Control-point displacements applied through the motion map on the device place the
+    vertices where the host-side averaging puts them, and the system built on them equals a
+    fresh import of those positions; an inverting displacement is refused and reverted."""
+    from femocs_amd import serving
+    c = serving.Client(0, small_mesh, sigma=0.1, seed=9)
+    s = _poisson_system(ctx, small_mesh)
+    s.mesh.set_motion(c.moved_ids, c.W_moved)
+    d = c.displacements()
+    s.mesh.displace(d)
+    s.reassemble()
+    s.fill_solution(0.0)
+    xyz, vod = s.mesh.dofs()
+    X = small_mesh.nodes.copy()
+    X[vod] = xyz
+    Xh = c.positions_after((c.moved_ids, small_mesh.nodes[c.moved_ids] + c.W_moved @ d))
+    # vertices of the other (bulk) hexes are not part of this mesh and stay put
+    assert np.abs(X[vod] - Xh[vod]).max() <= 1e-12 * np.abs(Xh).max()
+    gm = fgpu.Mesh(ctx, small_mesh.nodes, small_mesh.vacuum_cells(), *MARKS)
+    gm.update_vertices(X)
+    f = fgpu.System(gm)
+    f.reset()
+    f.add_dirichlet(2, 0.0)
+    f.add_neumann(8, None, -1.0)
+    f.apply_dirichlet()
+    f.fill_solution(0.0)
+    assert np.array_equal(s.rhs(), f.rhs())
+    o = fgpu.opts(tol=1e-10, relative=True, precision=fgpu.FGPU_MIXED)
+    assert s.solve(o).iterations == f.solve(o).iterations > 0
+    assert np.array_equal(s.solution(), f.solution())
+    with pytest.raises(fgpu.MeshInvertedError):
+        s.mesh.displace(d * 1e4)
+    xyz2, _ = s.mesh.dofs()
+    assert np.array_equal(xyz2, xyz)

@@ -5,6 +5,7 @@ import pytest
 
 from femocs_amd import fgpu, serving
 from femocs_amd import mesh as M
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +48,13 @@ def test_serving_results_equal_alone(ctx, clients, precision):
     serving.run_clients(srv, rate=400.0, duration=60.0, max_requests=9, seed=1)
     assert srv.idle()
     _check(ctx, srv, o)
+    # against the oracle on the moved mesh (error bound for a 1e-9 relative residual)
+    for r in [q for q in srv.stats.requests if q.seq == 1][:2]:
+        c = clients[r.client]
+        _, xo, it, _, _ = O.poisson_laplace(r.positions, c.cells, c.field, tol=1e-13, relative=True,
+                                            precond="ssor")
+        assert it > 0
+        assert np.linalg.norm(r.x_final - xo) <= 1e-8 * np.linalg.norm(xo), (r.client, r.seq)
     # warm starts: later requests of a client start from its previous solution
     firsts = {}
     for r in srv.stats.requests:
