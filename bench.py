@@ -92,7 +92,7 @@ def serve(ctx, clients, precision, batched, warm, steps, warmup, tuned, seed):
 def fp64_errors(srv):
     """Error of each client's last mixed result against an FP64 solve of the same system."""
     from femocs_amd import fgpu
-    o = fgpu.opts(tol=1e-12, relative=True, precision=fgpu.FGPU_FP64)
+    o = fgpu.opts(tol=1e-12, relative=True, precision=fgpu.FGPU_FP64, use_graph=False)
     errs = []
     for cid, r in srv.resident.items():
         xm = r.sys.solution()
