@@ -16,7 +16,9 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfemocs_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["fgpu_mesh.cpp", "fgpu_kernels.hip", "fgpu_api.hip", "fgpu_interp.hip"]
+SOURCES = ["fgpu_mesh.cpp", "fgpu_kernels.hip", "fgpu_api.hip", "fgpu_interp_host.cpp", "fgpu_interp.hip"]
+# the transfer to atoms reproduces the reference's rounding: no floating-point contraction
+EXTRA = {"fgpu_interp_host.cpp": ["-ffp-contract=off"], "fgpu_interp.hip": ["-ffp-contract=off"]}
 HEADERS = ["fgpu_internal.h", "fgpu_kernels.h", "fgpu_interp.h", "fgpu_cg.inc"]
 
 # -packed-fp32-ops: the compiler emits no packed-FP32 (v_pk_*_f32) instructions
@@ -43,9 +45,10 @@ def build(force: bool = False, verbose: bool = False) -> str:
         obj = os.path.join(OBJ, src + ".o")
         objs.append(obj)
         if force or _stale(obj, [path] + hdrs):
-            cmd = [HIPCC] + FLAGS + ["-c", path, "-o", obj]
+            extra = EXTRA.get(src, [])
+            cmd = [HIPCC] + FLAGS + extra + ["-c", path, "-o", obj]
             if src.endswith(".cpp"):
-                cmd = [HIPCC, "-x", "hip"] + FLAGS + ["-c", path, "-o", obj]
+                cmd = [HIPCC, "-x", "hip"] + FLAGS + extra + ["-c", path, "-o", obj]
             if verbose:
                 print(" ".join(cmd), flush=True)
             subprocess.run(cmd, check=True)

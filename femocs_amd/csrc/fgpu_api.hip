@@ -173,6 +173,9 @@ static void ctx_release(fgpu_ctx *c) {
     if (--c->refs == 0 && c->closing) ctx_free(c);
 }
 
+// for the other translation units holding a context reference
+void fgpu_ctx_release_ref(fgpu_ctx *c) { ctx_release(c); }
+
 extern "C" const char *fgpu_ctx_error(const fgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
 extern "C" int fgpu_ctx_sync(fgpu_ctx *c) {

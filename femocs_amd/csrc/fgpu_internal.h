@@ -181,6 +181,9 @@ struct fgpu_ctx {
 struct fgpu_mesh : fgpu::Mesh {};
 struct fgpu_sys : fgpu::Sys {};
 
+// drops a reference on the context (frees it when closed and unreferenced)
+void fgpu_ctx_release_ref(fgpu_ctx *c);
+
 namespace fgpu {
 // mesh topology built on the host (fgpu_mesh.cpp)
 int build_topology(Mesh &m, int nv, const double *xyz, int nc, const int *ucd, int top, int bottom,

@@ -1,0 +1,707 @@
+// Transfer of the field solution to atoms on the GPU: nodal solution (Interpolator), point
+// location and interpolation (InterpolatorCells, SolutionReader).  Compiled without
+// floating-point contraction: cell indices and interpolated values are bit-identical to the
+// reference's sequential code on the same inputs.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "femocs_gpu.h"
+#include "fgpu_internal.h"
+#include "fgpu_interp.h"
+
+using namespace fgpu;
+
+struct fgpu_interp {
+    fgpu_ctx *ctx = nullptr;
+    int n_nodes = 0, n_tetnodes = 0, n_tets = 0, n_hexes = 0, region = 3;
+    double empty = 0, decay = 0;
+    LinTetHost L;
+    Bvh B;
+    // device tables
+    double *xyz = nullptr, *det0 = nullptr, *det = nullptr, *cen = nullptr;
+    int *tets = nullptr, *c10 = nullptr, *hexes = nullptr, *marker = nullptr;
+    int *child = nullptr, *prim = nullptr;
+    float *box = nullptr, *cbox = nullptr, *leafbox = nullptr;
+    int *n2c_ptr = nullptr, *n2c = nullptr, *vor_ptr = nullptr, *vor = nullptr;
+    double *sol = nullptr;                  // 5 per node: vector xyz, scalar1, scalar2
+    int *node_idof = nullptr;               // node -> internal dof of `map_mesh`, -1 outside
+    const void *map_mesh = nullptr;
+    // point buffers (grown on demand)
+    double *pts = nullptr, *out = nullptr;
+    int4 *cand = nullptr;
+    int *tet = nullptr, *cells = nullptr;
+    size_t cap = 0;
+};
+
+namespace {
+
+struct HipErr : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+#define IPCHECK(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) throw HipErr(std::string(#call) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+template <class T> T *upload(const T *h, size_t n, hipStream_t s) {
+    void *p = nullptr;
+    IPCHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    if (n) IPCHECK(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return static_cast<T *>(p);
+}
+template <class T> void release(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <class F> int guard(fgpu_interp *ip, F &&f) {
+    try {
+        return f();
+    } catch (const HipErr &e) {
+        if (ip && ip->ctx) ip->ctx->err = e.what();
+        return FGPU_EHIP;
+    } catch (const std::bad_alloc &) {
+        if (ip && ip->ctx) ip->ctx->err = "host allocation failed";
+        return FGPU_ENOMEM;
+    } catch (const std::exception &e) {
+        if (ip && ip->ctx) ip->ctx->err = e.what();
+        return FGPU_EINVAL;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// device helpers: same operation order as the reference (no contraction in this file)
+// ------------------------------------------------------------------------------------------
+struct D3 {
+    double x, y, z;
+};
+__device__ inline D3 mk(double x, double y, double z) { return {x, y, z}; }
+__device__ inline D3 add(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline D3 sub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline D3 scl(D3 a, double r) { return {a.x * r, a.y * r, a.z * r}; }
+__device__ inline D3 dvd(D3 a, double r) { return {a.x / r, a.y / r, a.z / r}; }
+__device__ inline D3 ldx(const double *xyz, int i) { return {xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2]}; }
+__device__ inline double det3(D3 a, D3 b, D3 c) {
+    return a.x * (b.y * c.z - c.y * b.z) - b.x * (a.y * c.z - c.y * a.z) + c.x * (a.y * b.z - b.y * a.z);
+}
+__device__ inline double dot4(const double *p, const double *d) { return p[0] * d[0] + p[1] * d[1] + p[2] * d[2] + 1.0 * d[3]; }
+
+// LinearTetrahedra::point_in_cell (src/InterpolatorCells.cpp:631-647)
+__device__ inline bool pic(const double *det0, const double *det, const double *p, int t) {
+    const double *D = det + 16 * (size_t)t;
+    const double d0 = det0[t];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (d0 * dot4(p, D + 4 * k) < -kZero) return false;
+    return true;
+}
+// LinearTetrahedra::shape_functions (src/InterpolatorCells.cpp:649-661): bcc + 1e-15
+__device__ inline void bcc(const double *det0, const double *det, const double *p, int t, double b[4]) {
+    const double *D = det + 16 * (size_t)t;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) b[k] = kZero + det0[t] * dot4(p, D + 4 * k);
+}
+
+__device__ inline bool in_box(const float *b, const double *p) {
+    return p[0] >= (double)b[0] && p[1] >= (double)b[1] && p[2] >= (double)b[2] && p[0] <= (double)b[3] &&
+           p[1] <= (double)b[4] && p[2] <= (double)b[5];
+}
+// lower bound of the squared distance from p to any point of the box, rounded like the
+// centroid distances it is compared with
+__device__ inline double box_d2(const float *b, const double *p) {
+    double g[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double lo = (double)b[a], hi = (double)b[3 + a];
+        g[a] = p[a] < lo ? lo - p[a] : (p[a] > hi ? p[a] - hi : 0.0);
+    }
+    return g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+}
+
+// ------------------------------------------------------------------------------------------
+// kernels
+// ------------------------------------------------------------------------------------------
+// Per point: the lowest searchable tet containing it (x, -1 if none), the number of tets of
+// any marker containing it (y), the tet of the nearest centroid, lowest index among equal
+// distances (z, only when x < 0) and a traversal-overflow flag (w).
+__global__ __launch_bounds__(256) void k_candidates(int n, const double *__restrict__ pts, int n_leaves,
+                                                    const int *__restrict__ child, const float *__restrict__ box,
+                                                    const float *__restrict__ cbox, const float *__restrict__ leafbox,
+                                                    const int *__restrict__ prim, const double *__restrict__ det0,
+                                                    const double *__restrict__ det, const int *__restrict__ marker,
+                                                    const double *__restrict__ cen, int4 *__restrict__ cand) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double p[3] = {pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]};
+    int t0 = INT_MAX, cnt = 0, overflow = 0;
+    int stack[kBvhStack];
+    int sp = 0;
+    auto leaf_hit = [&](int leaf) {
+        if (!in_box(leafbox + 6 * (size_t)leaf, p)) return;
+        const int t = prim[leaf];
+        if (pic(det0, det, p, t)) {
+            ++cnt;
+            if (marker[t] == 0 && t < t0) t0 = t;
+        }
+    };
+    if (n_leaves == 1) {
+        leaf_hit(0);
+    } else {
+        stack[sp++] = 0;
+        while (sp > 0) {
+            const int v = stack[--sp];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int c = child[2 * (size_t)v + k];
+                if (c < 0) {
+                    leaf_hit(~c);
+                } else if (in_box(box + 6 * (size_t)c, p)) {
+                    if (sp < kBvhStack) stack[sp++] = c;
+                    else overflow = 1;
+                }
+            }
+        }
+    }
+    int amin = 0;
+    if (t0 == INT_MAX) {
+        // nearest centroid: strict < over increasing indices in the reference's scan, i.e. the
+        // lowest index among the minima
+        double best = 1e100;
+        auto consider = [&](int leaf) {
+            const int t = prim[leaf];
+            const double xx = p[0] - cen[3 * (size_t)t];
+            const double yy = p[1] - cen[3 * (size_t)t + 1];
+            const double zz = p[2] - cen[3 * (size_t)t + 2];
+            const double d2 = xx * xx + yy * yy + zz * zz;
+            if (d2 < best || (d2 == best && t < amin)) {
+                best = d2;
+                amin = t;
+            }
+        };
+        if (n_leaves == 1) {
+            consider(0);
+        } else {
+            sp = 0;
+            stack[sp++] = 0;
+            while (sp > 0) {
+                const int v = stack[--sp];
+                if (box_d2(cbox + 6 * (size_t)v, p) > best) continue;
+                const int c0 = child[2 * (size_t)v], c1 = child[2 * (size_t)v + 1];
+                if (c0 < 0) consider(~c0);
+                if (c1 < 0) consider(~c1);
+                const double d0 = c0 >= 0 ? box_d2(cbox + 6 * (size_t)c0, p) : 1e308;
+                const double d1 = c1 >= 0 ? box_d2(cbox + 6 * (size_t)c1, p) : 1e308;
+                // nearer child on top of the stack
+                const int far = d0 <= d1 ? c1 : c0, near = d0 <= d1 ? c0 : c1;
+                const double dfar = d0 <= d1 ? d1 : d0, dnear = d0 <= d1 ? d0 : d1;
+                if (far >= 0 && dfar <= best) {
+                    if (sp < kBvhStack) stack[sp++] = far;
+                    else overflow = 1;
+                }
+                if (near >= 0 && dnear <= best) {
+                    if (sp < kBvhStack) stack[sp++] = near;
+                    else overflow = 1;
+                }
+            }
+        }
+    }
+    cand[i] = make_int4(t0 == INT_MAX ? -1 : t0, cnt, amin, overflow);
+}
+
+// LinearHexahedra::shape_functions via project_to_nat_coords (src/InterpolatorCells.cpp:
+// 1237-1244, 1272-1317, 1334-1353), vertices in femocs order
+__device__ void linhex_sf(const D3 x[8], const double *pp, double sf[8]) {
+    const D3 f0s = dvd(add(add(add(add(add(add(add(x[0], x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    const D3 f1 = dvd(sub(add(add(sub(sub(add(add(scl(x[0], -1), x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    const D3 f2 = dvd(add(add(sub(sub(add(add(sub(scl(x[0], -1), x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    const D3 f3 = dvd(add(add(add(add(sub(sub(sub(scl(x[0], -1), x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    const D3 f4 = dvd(sub(add(sub(add(sub(add(sub(x[0], x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    const D3 f5 = dvd(sub(add(add(sub(add(sub(sub(x[0], x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    const D3 f6 = dvd(add(add(sub(sub(sub(sub(add(x[0], x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    const D3 f7 = dvd(sub(add(sub(add(add(sub(add(scl(x[0], -1), x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
+    const D3 f0 = sub(mk(pp[0], pp[1], pp[2]), f0s);
+    double u = 0, v = 0, w = 0;
+    for (int i = 0; i < 20; ++i) {
+        const D3 f = sub(sub(sub(sub(sub(sub(sub(f0, scl(f1, u)), scl(f2, v)), scl(f3, w)), scl(f4, u * v)), scl(f5, u * w)),
+                             scl(f6, v * w)),
+                         scl(f7, u * v * w));
+        const D3 fu = add(add(add(f1, scl(f4, v)), scl(f5, w)), scl(f7, v * w));
+        const D3 fv = add(add(add(f2, scl(f4, u)), scl(f6, w)), scl(f7, u * w));
+        const D3 fw = add(add(add(f3, scl(f5, u)), scl(f6, v)), scl(f7, u * v));
+        double D = det3(fu, fv, fw);
+        D = 1.0 / D;
+        const double du = det3(f, fv, fw) * D;
+        const double dv = det3(fu, f, fw) * D;
+        const double dw = det3(fu, fv, f) * D;
+        u += du;
+        v += dv;
+        w += dw;
+        if (du * du + dv * dv + dw * dw < kZero) break;
+    }
+    sf[0] = (1 - u) * (1 - v) * (1 - w) / 8.0;
+    sf[1] = (1 + u) * (1 - v) * (1 - w) / 8.0;
+    sf[2] = (1 + u) * (1 + v) * (1 - w) / 8.0;
+    sf[3] = (1 - u) * (1 + v) * (1 - w) / 8.0;
+    sf[4] = (1 - u) * (1 - v) * (1 + w) / 8.0;
+    sf[5] = (1 + u) * (1 - v) * (1 + w) / 8.0;
+    sf[6] = (1 + u) * (1 + v) * (1 + w) / 8.0;
+    sf[7] = (1 - u) * (1 + v) * (1 + w) / 8.0;
+}
+
+// InterpolatorCells::interp_solution (src/InterpolatorCells.cpp:331-355) for ranks 1-3.
+// from_tet != 0 (rank 3): cell holds the signed tet of the locate step and the hex is chosen
+// by the largest barycentric coordinate (LinearHexahedra::locate_cell, :1536-1562).
+__global__ __launch_bounds__(256) void k_interp(int n, const double *__restrict__ pts, int rank, int from_tet,
+                                                const int *__restrict__ cell_in, const double *__restrict__ det0,
+                                                const double *__restrict__ det, const int *__restrict__ tets,
+                                                const int *__restrict__ c10, const int *__restrict__ hexes,
+                                                const double *__restrict__ xyz, const double *__restrict__ sol,
+                                                int *__restrict__ cell_out, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double p[3] = {pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]};
+    int cell = cell_in[i];
+    double w[10];
+    int nodes[10];
+    int nn;
+    if (rank == 3) {
+        if (from_tet) {
+            const int sign = cell < 0 ? -1 : 1;
+            const int t = cell < 0 ? -cell : cell;
+            double b[4];
+            bcc(det0, det, p, t, b);
+            int k = -1;
+            if (b[0] >= b[1] && b[0] >= b[2] && b[0] >= b[3]) k = 0;
+            else if (b[1] >= b[0] && b[1] >= b[2] && b[1] >= b[3]) k = 1;
+            else if (b[2] >= b[0] && b[2] >= b[1] && b[2] >= b[3]) k = 2;
+            else if (b[3] >= b[0] && b[3] >= b[1] && b[3] >= b[2]) k = 3;
+            cell = k < 0 ? -1 : sign * (4 * t + k);
+        }
+        const int h = cell < 0 ? -cell : cell;
+        D3 x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            nodes[k] = hexes[8 * (size_t)h + k];
+            x[k] = ldx(xyz, nodes[k]);
+        }
+        linhex_sf(x, p, w);
+        nn = 8;
+    } else {
+        const int t = cell < 0 ? -cell : cell;
+        double b[4];
+        bcc(det0, det, p, t, b);
+        if (rank == 1) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                w[k] = b[k];
+                nodes[k] = tets[4 * (size_t)t + k];
+            }
+            nn = 4;
+        } else {
+            const double b1 = b[0], b2 = b[1], b3 = b[2], b4 = b[3];
+            w[0] = b1 * (2 * b1 - 1);
+            w[1] = b2 * (2 * b2 - 1);
+            w[2] = b3 * (2 * b3 - 1);
+            w[3] = b4 * (2 * b4 - 1);
+            w[4] = 4 * b1 * b2;
+            w[5] = 4 * b2 * b3;
+            w[6] = 4 * b3 * b1;
+            w[7] = 4 * b1 * b4;
+            w[8] = 4 * b2 * b4;
+            w[9] = 4 * b3 * b4;
+#pragma unroll
+            for (int k = 0; k < 10; ++k) nodes[k] = c10[10 * (size_t)t + k];
+            nn = 10;
+        }
+    }
+    double vx = 0, vy = 0, vz = 0, s1 = 0, s2 = 0;
+    for (int k = 0; k < nn; ++k) {
+        const double *s = sol + 5 * (size_t)nodes[k];
+        vx += s[0] * w[k];
+        vy += s[1] * w[k];
+        vz += s[2] * w[k];
+        s1 += s[3] * w[k];
+        s2 += s[4] * w[k];
+    }
+    double *o = out + 5 * (size_t)i;
+    o[0] = vx;
+    o[1] = vy;
+    o[2] = vz;
+    o[3] = s1;
+    o[4] = s2;
+    cell_out[i] = cell;
+}
+
+// Interpolator::extract_solution(PoissonSolver&) with store_elfield (src/Interpolator.cpp:79-140,
+// 172-190; LinearHexahedra::shape_fun_grads(hex, node), src/InterpolatorCells.cpp:1462-1505)
+__global__ __launch_bounds__(256) void k_extract(int n_nodes, const int *__restrict__ node_idof,
+                                                 const double *__restrict__ u, const int *__restrict__ n2c_ptr,
+                                                 const int *__restrict__ n2c, const int *__restrict__ hexes,
+                                                 const double *__restrict__ xyz, double empty, double *__restrict__ sol) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    double *s = sol + 5 * (size_t)n;
+    const int id = node_idof[n];
+    if (id < 0) {
+        for (int k = 0; k < 5; ++k) s[k] = empty;
+        return;
+    }
+    constexpr int UVW[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                               {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    constexpr int NN[8][3] = {{1, 3, 4}, {0, 2, 5}, {3, 1, 6}, {2, 0, 7}, {5, 7, 0}, {4, 6, 1}, {7, 5, 2}, {6, 4, 3}};
+    D3 mean = mk(0, 0, 0);
+    const int e0 = n2c_ptr[n], e1 = n2c_ptr[n + 1];
+    for (int e = e0; e < e1; ++e) {
+        const int h = n2c[e] >> 3, node = n2c[e] & 7;
+        const int *sh = hexes + 8 * (size_t)h;
+        const double uvw[3] = {(double)UVW[node][0], (double)UVW[node][1], (double)UVW[node][2]};
+        const D3 v0 = ldx(xyz, sh[node]);
+        double J[3][3];
+        for (int a = 0; a < 3; ++a) {
+            const D3 r = scl(sub(v0, ldx(xyz, sh[NN[node][a]])), uvw[a] * 0.5);
+            J[a][0] = r.x;
+            J[a][1] = r.y;
+            J[a][2] = r.z;
+        }
+        double Jdet = det3(mk(J[0][0], J[0][1], J[0][2]), mk(J[1][0], J[1][1], J[1][2]), mk(J[2][0], J[2][1], J[2][2]));
+        Jdet = 1.0 / Jdet;
+        double Ji[3][3] = {{J[1][1] * J[2][2] - J[2][1] * J[1][2], J[2][1] * J[0][2] - J[0][1] * J[2][2],
+                            J[0][1] * J[1][2] - J[1][1] * J[0][2]},
+                           {J[2][0] * J[1][2] - J[1][0] * J[2][2], J[0][0] * J[2][2] - J[2][0] * J[0][2],
+                            J[1][0] * J[0][2] - J[0][0] * J[1][2]},
+                           {J[1][0] * J[2][1] - J[2][0] * J[1][1], J[2][0] * J[0][1] - J[0][0] * J[2][1],
+                            J[0][0] * J[1][1] - J[1][0] * J[0][1]}};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) Ji[a][b] *= Jdet;
+        double g[8][3];
+        for (int k = 0; k < 8; ++k) g[k][0] = g[k][1] = g[k][2] = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            g[node][a] = 0.5 * (Ji[a][0] * uvw[0] + Ji[a][1] * uvw[1] + Ji[a][2] * uvw[2]);
+            for (int b = 0; b < 3; ++b) g[NN[node][b]][a] = -0.5 * Ji[a][b] * uvw[b];
+        }
+        D3 vi = mk(0, 0, 0);
+        for (int k = 0; k < 8; ++k) vi = sub(vi, scl(mk(g[k][0], g[k][1], g[k][2]), u[node_idof[sh[k]]]));
+        mean = add(mean, vi);
+    }
+    if (e1 > e0) mean = scl(mean, 1.0 / (e1 - e0));
+    s[0] = mean.x;
+    s[1] = mean.y;
+    s[2] = mean.z;
+    s[3] = 0.0;
+    s[4] = u[id];
+}
+
+// Interpolator::average_nodal_fields (src/Interpolator.cpp:142-170)
+__global__ __launch_bounds__(256) void k_smooth(int n_tetnodes, const int *__restrict__ ptr, const int *__restrict__ lst,
+                                                const double *__restrict__ xyz, double decay, double *__restrict__ sol) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_tetnodes) return;
+    const int e0 = ptr[i], e1 = ptr[i + 1];
+    if (e0 == e1) return;
+    const D3 xi = ldx(xyz, i);
+    D3 vec = mk(0, 0, 0);
+    double w_sum = 0;
+    for (int e = e0; e < e1; ++e) {
+        const int nb = lst[e];
+        const D3 xn = ldx(xyz, nb);
+        const double xx = xi.x - xn.x, yy = xi.y - xn.y, zz = xi.z - xn.z;
+        const double w = exp(decay * sqrt(xx * xx + yy * yy + zz * zz));
+        w_sum += w;
+        vec = add(vec, scl(mk(sol[5 * (size_t)nb], sol[5 * (size_t)nb + 1], sol[5 * (size_t)nb + 2]), w));
+    }
+    if (w_sum > 0) {
+        vec = scl(vec, 1.0 / w_sum);
+        sol[5 * (size_t)i] = vec.x;
+        sol[5 * (size_t)i + 1] = vec.y;
+        sol[5 * (size_t)i + 2] = vec.z;
+    }
+}
+
+inline int grid(int n) { return (n + 255) / 256; }
+
+void ensure_points(fgpu_interp *ip, size_t n) {
+    if (n <= ip->cap) return;
+    IPCHECK(hipStreamSynchronize(ip->ctx->stream));
+    release(ip->pts);
+    release(ip->out);
+    release(ip->cand);
+    release(ip->tet);
+    release(ip->cells);
+    const size_t c = std::max<size_t>(n, 1024);
+    IPCHECK(hipMalloc(&ip->pts, 3 * c * sizeof(double)));
+    IPCHECK(hipMalloc(&ip->out, 5 * c * sizeof(double)));
+    IPCHECK(hipMalloc(&ip->cand, c * sizeof(int4)));
+    IPCHECK(hipMalloc(&ip->tet, c * sizeof(int)));
+    IPCHECK(hipMalloc(&ip->cells, c * sizeof(int)));
+    ip->cap = c;
+}
+
+void free_interp(fgpu_interp *ip) {
+    for (double **p : {&ip->xyz, &ip->det0, &ip->det, &ip->cen, &ip->sol, &ip->pts, &ip->out}) release(*p);
+    for (int **p : {&ip->tets, &ip->c10, &ip->hexes, &ip->marker, &ip->child, &ip->prim, &ip->n2c_ptr, &ip->n2c,
+                    &ip->vor_ptr, &ip->vor, &ip->node_idof, &ip->tet, &ip->cells})
+        release(*p);
+    for (float **p : {&ip->box, &ip->cbox, &ip->leafbox}) release(*p);
+    release(ip->cand);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------
+extern "C" int fgpu_interp_create(fgpu_ctx *c, int n_nodes, const double *xyz, const int *node_kind, int n_tets,
+                                  const int *tets, const int *tet_nbr, const int *tet_marker, int n_hexes,
+                                  const int *hexes, const int *hex_marker, int search_region, double empty_value,
+                                  fgpu_interp **out) {
+    if (!c || !out || n_nodes <= 0 || n_tets <= 0 || !xyz || !node_kind || !tets || !tet_nbr || !tet_marker ||
+        (n_hexes > 0 && (!hexes || !hex_marker)) || (n_hexes != 0 && n_hexes != 4 * n_tets))
+        return FGPU_EINVAL;
+    *out = nullptr;
+    fgpu_interp *ip = new (std::nothrow) fgpu_interp();
+    if (!ip) return FGPU_ENOMEM;
+    ip->ctx = c;
+    const int rc = guard(ip, [&] {
+        IPCHECK(hipSetDevice(c->device));
+        hipStream_t s = c->stream;
+        for (size_t i = 0; i < 4 * (size_t)n_tets; ++i)
+            if (tets[i] < 0 || tets[i] >= n_nodes) throw std::invalid_argument("tet node out of range");
+        for (size_t i = 0; i < 8 * (size_t)n_hexes; ++i)
+            if (hexes[i] < 0 || hexes[i] >= n_nodes) throw std::invalid_argument("hex node out of range");
+        ip->n_nodes = n_nodes;
+        ip->n_tets = n_tets;
+        ip->n_hexes = n_hexes;
+        ip->region = search_region;
+        ip->empty = empty_value;
+        int tn = 0;
+        for (size_t i = 0; i < 4 * (size_t)n_tets; ++i) tn = std::max(tn, tets[i] + 1);
+        ip->n_tetnodes = tn;
+        build_lintet(ip->L, n_nodes, xyz, n_tets, tets, tet_nbr, tet_marker, search_region);
+        // LinearTetrahedra::precompute: decay factor of the smoothing, -1 / longest tet edge
+        double emax = -1e308;
+        for (int t = 0; t < n_tets; ++t) {
+            const double *v[4];
+            for (int k = 0; k < 4; ++k) v[k] = xyz + 3 * (size_t)tets[4 * (size_t)t + k];
+            const int P[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+            for (auto &pq : P) {
+                const double xx = v[pq[0]][0] - v[pq[1]][0], yy = v[pq[0]][1] - v[pq[1]][1], zz = v[pq[0]][2] - v[pq[1]][2];
+                emax = std::max(emax, xx * xx + yy * yy + zz * zz);
+            }
+        }
+        ip->decay = -1.0 / std::sqrt(emax);
+        // bounding boxes of the tets for the search tree
+        std::vector<double> lo(3 * (size_t)n_tets), hi(3 * (size_t)n_tets);
+        for (int t = 0; t < n_tets; ++t)
+            for (int a = 0; a < 3; ++a) {
+                double l = 1e300, h = -1e300;
+                for (int k = 0; k < 4; ++k) {
+                    const double x = xyz[3 * (size_t)tets[4 * (size_t)t + k] + a];
+                    l = std::min(l, x);
+                    h = std::max(h, x);
+                }
+                lo[3 * (size_t)t + a] = l;
+                hi[3 * (size_t)t + a] = h;
+            }
+        build_bvh(n_tets, lo, hi, ip->L.centroid, ip->B);
+        std::vector<int> c10;
+        if (n_hexes > 0) build_quadtet(n_tets, tets, hexes, node_kind, c10);
+        else c10.assign(10 * (size_t)n_tets, 0);
+        // node -> (hex, corner) of the region's hexes in hex order (Interpolator::initialize)
+        std::vector<int> nptr(n_nodes + 1, 0), n2c;
+        std::vector<int> vptr(tn + 1, 0), vor;
+        if (n_hexes > 0) {
+            auto in_region = [&](int h) { return search_region == 3 ? hex_marker[h] > 0 : hex_marker[h] < 0; };
+            for (int h = 0; h < n_hexes; ++h)
+                if (in_region(h))
+                    for (int k = 0; k < 8; ++k) nptr[hexes[8 * (size_t)h + k] + 1]++;
+            for (int i = 0; i < n_nodes; ++i) nptr[i + 1] += nptr[i];
+            n2c.resize(nptr[n_nodes]);
+            std::vector<int> pos(nptr.begin(), nptr.end());
+            for (int h = 0; h < n_hexes; ++h)
+                if (in_region(h))
+                    for (int k = 0; k < 8; ++k) n2c[pos[hexes[8 * (size_t)h + k]]++] = 8 * h + k;
+            // TetgenMesh::calc_pseudo_3D_vorocells (src/TetgenMesh.cpp:819-837)
+            const int mult = search_region == 3 ? 1 : -1;
+            for (int pass = 0; pass < 2; ++pass) {
+                std::vector<int> vpos(vptr.begin(), vptr.end());
+                for (int h = 0; h < n_hexes; ++h) {
+                    int t = mult * hex_marker[h];
+                    if (t <= 0) continue;
+                    t -= 1;
+                    if (t >= tn) continue;
+                    for (int k = 0; k < 8; ++k) {
+                        const int node = hexes[8 * (size_t)h + k];
+                        if (node != t && node_kind[node] >= 2) {
+                            if (pass == 0) vptr[t + 1]++;
+                            else vor[vpos[t]++] = node;
+                        }
+                    }
+                }
+                if (pass == 0) {
+                    for (int i = 0; i < tn; ++i) vptr[i + 1] += vptr[i];
+                    vor.resize(vptr[tn]);
+                }
+            }
+        }
+        ip->xyz = upload(xyz, 3 * (size_t)n_nodes, s);
+        ip->det0 = upload(ip->L.det0.data(), ip->L.det0.size(), s);
+        ip->det = upload(ip->L.det.data(), ip->L.det.size(), s);
+        ip->cen = upload(ip->L.centroid.data(), ip->L.centroid.size(), s);
+        ip->marker = upload(ip->L.marker.data(), ip->L.marker.size(), s);
+        ip->tets = upload(tets, 4 * (size_t)n_tets, s);
+        ip->c10 = upload(c10.data(), c10.size(), s);
+        ip->hexes = upload(hexes, 8 * (size_t)n_hexes, s);
+        ip->child = upload(ip->B.child.data(), ip->B.child.size(), s);
+        ip->box = upload(ip->B.box.data(), ip->B.box.size(), s);
+        ip->cbox = upload(ip->B.cbox.data(), ip->B.cbox.size(), s);
+        ip->leafbox = upload(ip->B.leafbox.data(), ip->B.leafbox.size(), s);
+        ip->prim = upload(ip->B.prim.data(), ip->B.prim.size(), s);
+        ip->n2c_ptr = upload(nptr.data(), nptr.size(), s);
+        ip->n2c = upload(n2c.data(), n2c.size(), s);
+        ip->vor_ptr = upload(vptr.data(), vptr.size(), s);
+        ip->vor = upload(vor.data(), vor.size(), s);
+        std::vector<double> sol(5 * (size_t)n_nodes, empty_value);
+        ip->sol = upload(sol.data(), sol.size(), s);
+        IPCHECK(hipStreamSynchronize(s));
+        return FGPU_OK;
+    });
+    if (rc != FGPU_OK) {
+        free_interp(ip);
+        delete ip;
+        return rc;
+    }
+    c->refs++;
+    *out = ip;
+    return FGPU_OK;
+}
+
+extern "C" void fgpu_interp_destroy(fgpu_interp *ip) {
+    if (!ip) return;
+    fgpu_ctx *c = ip->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_interp(ip);
+    delete ip;
+    fgpu_ctx_release_ref(c);
+}
+
+extern "C" int fgpu_interp_extract(fgpu_interp *ip, fgpu_sys *s, int smoothen) {
+    if (!ip || !s) return FGPU_EINVAL;
+    return guard(ip, [&] {
+        IPCHECK(hipSetDevice(ip->ctx->device));
+        const fgpu_mesh *m = static_cast<const fgpu_mesh *>(s->mesh);
+        if (s->ctx != ip->ctx) throw std::invalid_argument("system of another context");
+        if (m->n_vertices != ip->n_nodes)
+            throw std::invalid_argument("the system's mesh was not imported from these nodes");
+        hipStream_t st = ip->ctx->stream;
+        if (ip->map_mesh != (const void *)m) {
+            std::vector<int> map(ip->n_nodes, -1);
+            for (int v = 0; v < ip->n_nodes; ++v) {
+                const int d = m->dof_of_vertex[v];
+                if (d >= 0) map[v] = m->idof_of_dof[d];
+            }
+            // every corner of the region's hexes must be a dof of the system
+            std::vector<int> nptr(ip->n_nodes + 1);
+            IPCHECK(hipMemcpy(nptr.data(), ip->n2c_ptr, nptr.size() * sizeof(int), hipMemcpyDeviceToHost));
+            std::vector<int> n2c(nptr.back());
+            if (!n2c.empty()) IPCHECK(hipMemcpy(n2c.data(), ip->n2c, n2c.size() * sizeof(int), hipMemcpyDeviceToHost));
+            std::vector<int> hx(8 * (size_t)ip->n_hexes);
+            if (!hx.empty()) IPCHECK(hipMemcpy(hx.data(), ip->hexes, hx.size() * sizeof(int), hipMemcpyDeviceToHost));
+            for (int e : n2c)
+                for (int k = 0; k < 8; ++k)
+                    if (map[hx[8 * (size_t)(e >> 3) + k]] < 0)
+                        throw std::invalid_argument("a hex of the search region has a node outside the system's mesh");
+            release(ip->node_idof);
+            ip->node_idof = upload(map.data(), map.size(), st);
+            ip->map_mesh = m;
+        }
+        hipLaunchKernelGGL(k_extract, dim3(grid(ip->n_nodes)), dim3(256), 0, st, ip->n_nodes, ip->node_idof, s->x,
+                           ip->n2c_ptr, ip->n2c, ip->hexes, ip->xyz, ip->empty, ip->sol);
+        if (smoothen && ip->n_tetnodes > 0)
+            hipLaunchKernelGGL(k_smooth, dim3(grid(ip->n_tetnodes)), dim3(256), 0, st, ip->n_tetnodes, ip->vor_ptr,
+                               ip->vor, ip->xyz, ip->decay, ip->sol);
+        IPCHECK(hipGetLastError());
+        IPCHECK(hipStreamSynchronize(st));
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_interp_get_nodal(fgpu_interp *ip, double *sol5) {
+    if (!ip || !sol5) return FGPU_EINVAL;
+    return guard(ip, [&] {
+        IPCHECK(hipSetDevice(ip->ctx->device));
+        IPCHECK(hipMemcpyAsync(sol5, ip->sol, 5 * (size_t)ip->n_nodes * sizeof(double), hipMemcpyDeviceToHost,
+                               ip->ctx->stream));
+        IPCHECK(hipStreamSynchronize(ip->ctx->stream));
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_interp_set_nodal(fgpu_interp *ip, const double *sol5) {
+    if (!ip || !sol5) return FGPU_EINVAL;
+    return guard(ip, [&] {
+        IPCHECK(hipSetDevice(ip->ctx->device));
+        IPCHECK(hipMemcpyAsync(ip->sol, sol5, 5 * (size_t)ip->n_nodes * sizeof(double), hipMemcpyHostToDevice,
+                               ip->ctx->stream));
+        IPCHECK(hipStreamSynchronize(ip->ctx->stream));
+        return FGPU_OK;
+    });
+}
+
+static int interp_run(fgpu_interp *ip, int n, const double *xyz, int rank, int *cells, double *out, bool locate) {
+    if (!ip || n < 0 || (n > 0 && (!xyz || !cells || !out)) || rank < 1 || rank > 3) return FGPU_EINVAL;
+    if (rank >= 2 && ip->n_hexes == 0) return FGPU_EINVAL;
+    if (n == 0) return FGPU_OK;
+    return guard(ip, [&] {
+        IPCHECK(hipSetDevice(ip->ctx->device));
+        hipStream_t st = ip->ctx->stream;
+        ensure_points(ip, n);
+        IPCHECK(hipMemcpyAsync(ip->pts, xyz, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+        if (locate) {
+            hipLaunchKernelGGL(k_candidates, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, ip->B.n_leaves, ip->child,
+                               ip->box, ip->cbox, ip->leafbox, ip->prim, ip->det0, ip->det, ip->marker, ip->cen, ip->cand);
+            IPCHECK(hipGetLastError());
+            std::vector<int4> cand(n);
+            IPCHECK(hipMemcpyAsync(cand.data(), ip->cand, n * sizeof(int4), hipMemcpyDeviceToHost, st));
+            IPCHECK(hipStreamSynchronize(st));
+            for (int i = 0; i < n; ++i)
+                if (cand[i].w) throw std::runtime_error("point search stack overflow");
+            std::vector<int> tet(n);
+            resolve_chain(ip->L, n, xyz, cand.data(), rank, tet.data());
+            IPCHECK(hipMemcpyAsync(ip->tet, tet.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_interp, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, rank, rank == 3 ? 1 : 0, ip->tet,
+                               ip->det0, ip->det, ip->tets, ip->c10, ip->hexes, ip->xyz, ip->sol, ip->cells, ip->out);
+            IPCHECK(hipGetLastError());
+            IPCHECK(hipMemcpyAsync(cells, ip->cells, n * sizeof(int), hipMemcpyDeviceToHost, st));
+            IPCHECK(hipMemcpyAsync(out, ip->out, 5 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+            IPCHECK(hipStreamSynchronize(st));
+        } else {
+            const int lim = rank == 3 ? ip->n_hexes : ip->n_tets;
+            for (int i = 0; i < n; ++i)
+                if (cells[i] <= -lim || cells[i] >= lim) throw std::invalid_argument("cell index out of range");
+            IPCHECK(hipMemcpyAsync(ip->tet, cells, n * sizeof(int), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_interp, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, rank, 0, ip->tet, ip->det0, ip->det,
+                               ip->tets, ip->c10, ip->hexes, ip->xyz, ip->sol, ip->cells, ip->out);
+            IPCHECK(hipGetLastError());
+            IPCHECK(hipMemcpyAsync(out, ip->out, 5 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+            IPCHECK(hipStreamSynchronize(st));
+        }
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_interp_points(fgpu_interp *ip, int n, const double *xyz, int rank, int *cells, double *out) {
+    return interp_run(ip, n, xyz, rank, cells, out, true);
+}
+
+extern "C" int fgpu_interp_cells(fgpu_interp *ip, int n, const double *xyz, int rank, const int *cells, double *out) {
+    return interp_run(ip, n, xyz, rank, const_cast<int *>(cells), out, false);
+}

@@ -1,0 +1,325 @@
+// Host-side tables of the transfer to atoms.  Compiled with -ffp-contract=off: the tet tables
+// must round exactly as the reference computes them, since the located cells depend on the
+// signs of these numbers to the last bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+#include <numeric>
+#include <stdexcept>
+
+#include "fgpu_interp.h"
+
+namespace fgpu {
+
+namespace {
+
+struct V3 {
+    double x, y, z;
+};
+inline V3 ld(const double *xyz, int i) { return {xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2]}; }
+
+// InterpolatorCells::determinant overloads, src/InterpolatorCells.cpp:471-500
+inline double det2(V3 a, V3 b) { return a.x * (b.y - b.z) - a.y * (b.x - b.z) + a.z * (b.x - b.y); }
+inline double det3(V3 a, V3 b, V3 c) {
+    return a.x * (b.y * c.z - c.y * b.z) - b.x * (a.y * c.z - c.y * a.z) + c.x * (a.y * b.z - b.y * a.z);
+}
+inline double det4(V3 a, V3 b, V3 c, V3 d) {
+    const double d1 = det3(b, c, d), d2 = det3(a, c, d), d3 = det3(a, b, d), d4 = det3(a, b, c);
+    return d4 - d3 + d2 - d1;
+}
+
+// cofactor row of the three vertices p, q, r (the other one left out); the reference takes
+// the 3x3 determinant of the last row from the untransposed vertices
+inline void cof(V3 p, V3 q, V3 r, double sign, bool last, double *D) {
+    const double d1 = det2({p.y, q.y, r.y}, {p.z, q.z, r.z});
+    const double d2 = det2({p.x, q.x, r.x}, {p.z, q.z, r.z});
+    const double d3 = det2({p.x, q.x, r.x}, {p.y, q.y, r.y});
+    const double d4 = last ? det3(p, q, r) : det3({p.x, q.x, r.x}, {p.y, q.y, r.y}, {p.z, q.z, r.z});
+    D[0] = sign * d1;
+    D[1] = -sign * d2;
+    D[2] = sign * d3;
+    D[3] = -sign * d4;
+}
+
+inline float down(double v) {
+    float f = (float)v;
+    if ((double)f > v) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+inline float up(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+}  // namespace
+
+void build_lintet(LinTetHost &L, int n_nodes, const double *xyz, int nt, const int *tets, const int *tet_nbr,
+                  const int *tet_marker, int search_region) {
+    L.n = nt;
+    L.det0.resize(nt);
+    L.det.resize(16 * (size_t)nt);
+    L.centroid.resize(3 * (size_t)nt);
+    L.marker.resize(nt);
+    // node -> tets in increasing tet order
+    std::vector<int> cnt(n_nodes + 1, 0), n2t(4 * (size_t)nt);
+    for (size_t i = 0; i < 4 * (size_t)nt; ++i) cnt[tets[i] + 1]++;
+    for (int i = 0; i < n_nodes; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int> pos(cnt.begin(), cnt.end());
+    for (int t = 0; t < nt; ++t)
+        for (int k = 0; k < 4; ++k) n2t[pos[tets[4 * t + k]]++] = t;
+    L.nbr_ptr.assign(nt + 1, 0);
+    L.nbr.clear();
+    L.nbr.reserve(40 * (size_t)nt);
+    for (int t = 0; t < nt; ++t) {
+        const int *nb = tet_nbr + 4 * (size_t)t;
+        for (int k = 0; k < 4; ++k)
+            if (nb[k] >= 0) L.nbr.push_back(nb[k]);
+        for (int k = 0; k < 4; ++k) {
+            const int node = tets[4 * t + k];
+            for (int e = cnt[node]; e < cnt[node + 1]; ++e) {
+                const int o = n2t[e];
+                if (o != t && o != nb[0] && o != nb[1] && o != nb[2] && o != nb[3]) L.nbr.push_back(o);
+            }
+        }
+        L.nbr_ptr[t + 1] = (int)L.nbr.size();
+    }
+    for (int t = 0; t < nt; ++t) {
+        const int *s = tets + 4 * (size_t)t;
+        const V3 v1 = ld(xyz, s[0]), v2 = ld(xyz, s[1]), v3 = ld(xyz, s[2]), v4 = ld(xyz, s[3]);
+        // TetgenCells::get_centroid: running sum, then times 1/4
+        V3 c{0, 0, 0};
+        c = {c.x + v1.x, c.y + v1.y, c.z + v1.z};
+        c = {c.x + v2.x, c.y + v2.y, c.z + v2.z};
+        c = {c.x + v3.x, c.y + v3.y, c.z + v3.z};
+        c = {c.x + v4.x, c.y + v4.y, c.z + v4.z};
+        const double q = 1.0 / 4;
+        L.centroid[3 * (size_t)t] = c.x * q;
+        L.centroid[3 * (size_t)t + 1] = c.y * q;
+        L.centroid[3 * (size_t)t + 2] = c.z * q;
+        L.det0[t] = 1.0 / det4(v1, v2, v3, v4);
+        double *D = &L.det[16 * (size_t)t];
+        cof(v2, v3, v4, 1.0, false, D);
+        cof(v1, v3, v4, -1.0, false, D + 4);
+        cof(v1, v2, v4, 1.0, false, D + 8);
+        cof(v1, v2, v3, -1.0, true, D + 12);
+        if (search_region == 3) L.marker[t] = tet_marker[t] != 3;
+        else if (search_region == 1) L.marker[t] = tet_marker[t] == 3;
+        else L.marker[t] = 0;
+    }
+}
+
+bool LinTetHost::point_in_cell(const double p[3], int t) const {
+    const double *D = &det[16 * (size_t)t];
+    const double d0 = det0[t];
+    for (int k = 0; k < 4; ++k) {
+        const double *d = D + 4 * k;
+        if (d0 * (p[0] * d[0] + p[1] * d[1] + p[2] * d[2] + 1.0 * d[3]) < -kZero) return false;
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------
+// linear BVH
+// ------------------------------------------------------------------------------------------
+namespace {
+
+inline uint64_t spread21(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+struct Radix {
+    const std::vector<uint64_t> &code;
+    int n;
+    int delta(int i, int j) const {
+        if (j < 0 || j >= n) return -1;
+        if (code[i] == code[j]) return 64 + __builtin_clz((unsigned)(i ^ j));
+        return __builtin_clzll(code[i] ^ code[j]);
+    }
+};
+
+}  // namespace
+
+void build_bvh(int n, const std::vector<double> &lo, const std::vector<double> &hi, const std::vector<double> &cen, Bvh &B) {
+    B = Bvh{};
+    B.n_leaves = n;
+    if (n <= 0) return;
+    double cl[3] = {1e300, 1e300, 1e300}, ch[3] = {-1e300, -1e300, -1e300}, ext = 0;
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            cl[a] = std::min(cl[a], cen[3 * (size_t)i + a]);
+            ch[a] = std::max(ch[a], cen[3 * (size_t)i + a]);
+            ext = std::max(ext, hi[3 * (size_t)i + a] - lo[3 * (size_t)i + a]);
+        }
+    std::vector<uint64_t> key(n);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) {
+        uint64_t m = 0;
+        for (int a = 0; a < 3; ++a) {
+            const double s = ch[a] > cl[a] ? (cen[3 * (size_t)i + a] - cl[a]) / (ch[a] - cl[a]) : 0.0;
+            const uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, s * 2097151.0));
+            m |= spread21(q) << (2 - a);
+        }
+        key[i] = m;
+    }
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return key[a] != key[b] ? key[a] < key[b] : a < b; });
+    std::vector<uint64_t> code(n);
+    B.prim.resize(n);
+    B.leafbox.resize(6 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const int t = order[i];
+        code[i] = key[t];
+        B.prim[i] = t;
+        for (int a = 0; a < 3; ++a) {
+            // widen by a margin far above the 1e-15 tolerance of the containment predicate
+            const double m = 1e-9 * (std::fabs(lo[3 * (size_t)t + a]) + std::fabs(hi[3 * (size_t)t + a]) + ext);
+            B.leafbox[6 * (size_t)i + a] = down(lo[3 * (size_t)t + a] - m);
+            B.leafbox[6 * (size_t)i + 3 + a] = up(hi[3 * (size_t)t + a] + m);
+        }
+    }
+    if (n == 1) return;
+    const int ni = n - 1;
+    B.child.assign(2 * (size_t)ni, 0);
+    std::vector<int> parent_of_internal(ni, -1);
+    Radix R{code, n};
+    for (int i = 0; i < ni; ++i) {
+        const int d = (R.delta(i, i + 1) - R.delta(i, i - 1)) >= 0 ? 1 : -1;
+        const int dmin = R.delta(i, i - d);
+        int lmax = 2;
+        while (R.delta(i, i + lmax * d) > dmin) lmax *= 2;
+        int l = 0;
+        for (int t = lmax / 2; t >= 1; t /= 2)
+            if (R.delta(i, i + (l + t) * d) > dmin) l += t;
+        const int j = i + l * d;
+        const int dnode = R.delta(i, j);
+        int s = 0;
+        for (int div = 2;; div *= 2) {
+            const int t = (l + div - 1) / div;
+            if (R.delta(i, i + (s + t) * d) > dnode) s += t;
+            if (t <= 1) break;
+        }
+        const int gamma = i + s * d + std::min(d, 0);
+        const int left = std::min(i, j) == gamma ? ~gamma : gamma;
+        const int right = std::max(i, j) == gamma + 1 ? ~(gamma + 1) : gamma + 1;
+        B.child[2 * (size_t)i] = left;
+        B.child[2 * (size_t)i + 1] = right;
+        if (left >= 0) parent_of_internal[left] = i;
+        if (right >= 0) parent_of_internal[right] = i;
+    }
+    // boxes bottom-up: post-order over the tree from the root (node 0)
+    B.box.assign(6 * (size_t)ni, 0.f);
+    B.cbox.assign(6 * (size_t)ni, 0.f);
+    std::vector<int> stack{0}, post;
+    post.reserve(ni);
+    while (!stack.empty()) {
+        const int v = stack.back();
+        stack.pop_back();
+        post.push_back(v);
+        for (int k = 0; k < 2; ++k)
+            if (B.child[2 * (size_t)v + k] >= 0) stack.push_back(B.child[2 * (size_t)v + k]);
+    }
+    if ((int)post.size() != ni) throw std::runtime_error("bvh: tree does not reach every internal node");
+    for (auto it = post.rbegin(); it != post.rend(); ++it) {
+        const int v = *it;
+        float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        float cb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (int k = 0; k < 2; ++k) {
+            const int c = B.child[2 * (size_t)v + k];
+            const float *cbx, *ccb;
+            float lc[6];
+            if (c >= 0) {
+                cbx = &B.box[6 * (size_t)c];
+                ccb = &B.cbox[6 * (size_t)c];
+            } else {
+                const int leaf = ~c, t = B.prim[leaf];
+                cbx = &B.leafbox[6 * (size_t)leaf];
+                for (int a = 0; a < 3; ++a) {
+                    lc[a] = down(cen[3 * (size_t)t + a]);
+                    lc[3 + a] = up(cen[3 * (size_t)t + a]);
+                }
+                ccb = lc;
+            }
+            for (int a = 0; a < 3; ++a) {
+                b[a] = std::min(b[a], cbx[a]);
+                b[3 + a] = std::max(b[3 + a], cbx[3 + a]);
+                cb[a] = std::min(cb[a], ccb[a]);
+                cb[3 + a] = std::max(cb[3 + a], ccb[3 + a]);
+            }
+        }
+        std::copy(b, b + 6, &B.box[6 * (size_t)v]);
+        std::copy(cb, cb + 6, &B.cbox[6 * (size_t)v]);
+    }
+}
+
+void build_quadtet(int nt, const int *tets, const int *hexes, const int *node_kind, std::vector<int> &c10) {
+    auto common = [](const int *a, int na, const int *b, int nb) {
+        for (int i = 0; i < na; ++i)
+            for (int j = 0; j < nb; ++j)
+                if (a[i] == b[j]) return a[i];
+        return -1;
+    };
+    c10.assign(10 * (size_t)nt, -1);
+    for (int t = 0; t < nt; ++t) {
+        int e[4][8], ne[4] = {0, 0, 0, 0};
+        for (int i = 0; i < 4; ++i)
+            for (int k = 0; k < 8; ++k) {
+                const int node = hexes[8 * (4 * (size_t)t + i) + k];
+                if (node_kind[node] == 2) e[i][ne[i]++] = node;
+            }
+        int *c = &c10[10 * (size_t)t];
+        for (int k = 0; k < 4; ++k) c[k] = tets[4 * (size_t)t + k];
+        c[4] = common(e[0], ne[0], e[1], ne[1]);
+        c[5] = common(e[1], ne[1], e[2], ne[2]);
+        c[6] = common(e[2], ne[2], e[0], ne[0]);
+        c[7] = common(e[0], ne[0], e[3], ne[3]);
+        c[8] = common(e[1], ne[1], e[3], ne[3]);
+        c[9] = common(e[2], ne[2], e[3], ne[3]);
+    }
+}
+
+// cand[i] = {lowest searchable tet containing p_i or -1, number of tets containing p_i,
+//            nearest centroid (lowest index among equals) when no searchable tet contains p_i,
+//            unused}
+void resolve_chain(const LinTetHost &L, int n, const double *pts, const int4 *cand, int rank, int *tet) {
+    int prev = -1;     // previous result in the rank's numbering (hex for rank 3)
+    for (int i = 0; i < n; ++i) {
+        const double *p = pts + 3 * (size_t)i;
+        const int4 c = cand[i];
+        const int g = rank == 3 ? std::abs(prev) / 4 : std::abs(prev);
+        int r;
+        if (c.y == 0) {
+            r = -c.z;
+        } else if (c.y == 1 && c.x >= 0) {
+            r = c.x;
+        } else {
+            r = INT_MIN;
+            if (g >= 0 && g < L.n) {
+                if (L.point_in_cell(p, g)) {
+                    r = g;
+                } else {
+                    for (int e = L.nbr_ptr[g]; e < L.nbr_ptr[g + 1]; ++e)
+                        if (L.point_in_cell(p, L.nbr[e])) {
+                            r = L.nbr[e];
+                            break;
+                        }
+                }
+            }
+            if (r == INT_MIN) r = c.x >= 0 ? c.x : -c.z;
+        }
+        tet[i] = r;
+        // rank 3: |hex| / 4 == |tet| for the next guess (hex = +-(4 |tet| + k), k < 4)
+        prev = rank == 3 ? (r < 0 ? -(4 * -r) : 4 * r) : r;
+    }
+}
+
+}  // namespace fgpu

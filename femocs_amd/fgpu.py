@@ -79,6 +79,14 @@ API = [
     ("fgpu_sys_solve", C.c_int, [_P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     ("fgpu_solve_batch", C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     ("fgpu_batch_time_kernel", C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    ("fgpu_interp_create", C.c_int, [_P, C.c_int, _D, _I, C.c_int, _I, _I, _I, C.c_int, _I, _I, C.c_int,
+                                     C.c_double, C.POINTER(_P)]),
+    ("fgpu_interp_destroy", None, [_P]),
+    ("fgpu_interp_extract", C.c_int, [_P, _P, C.c_int]),
+    ("fgpu_interp_get_nodal", C.c_int, [_P, _D]),
+    ("fgpu_interp_set_nodal", C.c_int, [_P, _D]),
+    ("fgpu_interp_points", C.c_int, [_P, C.c_int, _D, C.c_int, _I, _D]),
+    ("fgpu_interp_cells", C.c_int, [_P, C.c_int, _D, C.c_int, _I, _D]),
     ("fgpu_sys_minmax", C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("fgpu_sys_solution_grad", C.c_int, [_P, _D]),
     ("fgpu_sys_time_kernel", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),

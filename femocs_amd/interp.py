@@ -1,0 +1,102 @@
+"""Field and temperature transfer to atoms on the GPU, mirroring the reference's Interpolator
+(src/Interpolator.cpp) and SolutionReader (src/SolutionReader.cpp) for dim 3.
+
+    ip = Interpolator(ctx, tip, search_region=TYPES_VACUUM)
+    ip.extract_solution(poisson_system)            # nodal (E, rho, phi), smoothed
+    reader = SolutionReader(ip, rank=1)
+    sol = reader.calc_interpolation(atom_xyz)       # (n, 5): Ex Ey Ez rho phi
+
+The C library does the work; this module only holds the handles and the atom-to-cell map.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import fgpu
+
+TYPES_BULK = 1
+TYPES_VACUUM = 3
+
+
+class Interpolator:
+    """Interpolator::initialize (src/Interpolator.cpp:28-77) on a tet mesh and its tet-split
+    hexes (hex 4t+k around vertex k of tet t)."""
+
+    def __init__(self, ctx: fgpu.Context, tip, search_region: int = TYPES_VACUUM, empty_value: float = 0.0):
+        self.ctx = ctx
+        self.n_nodes = len(tip.nodes)
+        self.search_region = search_region
+        h = fgpu._P()
+        ctx.check(ctx.lib.fgpu_interp_create(
+            ctx.h, self.n_nodes, np.ascontiguousarray(tip.nodes, np.float64).reshape(-1),
+            np.ascontiguousarray(tip.node_kind, np.int32), len(tip.tets),
+            np.ascontiguousarray(tip.tets, np.int32).reshape(-1),
+            np.ascontiguousarray(tip.tet_nbr, np.int32).reshape(-1),
+            np.ascontiguousarray(tip.tet_marker, np.int32), len(tip.hexes),
+            np.ascontiguousarray(tip.hexes, np.int32).reshape(-1),
+            np.ascontiguousarray(tip.hex_marker, np.int32), int(search_region), float(empty_value), C.byref(h)),
+            "fgpu_interp_create")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.fgpu_interp_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def extract_solution(self, system: fgpu.System, smoothen: bool = True):
+        """Interpolator::extract_solution(PoissonSolver&, smoothen) (src/Interpolator.cpp:172-190)."""
+        self.ctx.check(self.ctx.lib.fgpu_interp_extract(self.h, system.h, int(bool(smoothen))), "fgpu_interp_extract")
+
+    def nodal(self) -> np.ndarray:
+        out = np.zeros(5 * self.n_nodes)
+        self.ctx.check(self.ctx.lib.fgpu_interp_get_nodal(self.h, out), "fgpu_interp_get_nodal")
+        return out.reshape(-1, 5)
+
+    def set_nodal(self, sol5):
+        s = np.ascontiguousarray(sol5, np.float64).reshape(-1)
+        if s.size != 5 * self.n_nodes:
+            raise ValueError("5 values per node")
+        self.ctx.check(self.ctx.lib.fgpu_interp_set_nodal(self.h, s), "fgpu_interp_set_nodal")
+
+    def points(self, xyz, rank: int):
+        pts = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        n = pts.size // 3
+        cells = np.zeros(n, np.int32)
+        out = np.zeros(5 * n)
+        self.ctx.check(self.ctx.lib.fgpu_interp_points(self.h, n, pts, int(rank), cells, out), "fgpu_interp_points")
+        return cells, out.reshape(-1, 5)
+
+    def in_cells(self, xyz, rank: int, cells):
+        pts = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        n = pts.size // 3
+        c = np.ascontiguousarray(cells, np.int32)
+        out = np.zeros(5 * n)
+        self.ctx.check(self.ctx.lib.fgpu_interp_cells(self.h, n, pts, int(rank), c, out), "fgpu_interp_cells")
+        return out.reshape(-1, 5)
+
+
+class SolutionReader:
+    """SolutionReader for dim 3 without atom sorting (src/SolutionReader.cpp:43-190): the first
+    call maps the atoms to cells (markers), later calls interpolate in the mapped cells."""
+
+    def __init__(self, interpolator: Interpolator, rank: int = 1):
+        if rank not in (1, 2, 3):
+            raise ValueError("rank must be 1, 2 or 3")
+        self.interpolator = interpolator
+        self.rank = rank
+        self.markers = None
+
+    def calc_full_interpolation(self, xyz) -> np.ndarray:
+        self.markers, sol = self.interpolator.points(xyz, self.rank)
+        return sol
+
+    def calc_interpolation(self, xyz) -> np.ndarray:
+        n = np.asarray(xyz).reshape(-1, 3).shape[0]
+        if self.markers is None or len(self.markers) != n:
+            return self.calc_full_interpolation(xyz)
+        return self.interpolator.in_cells(xyz, self.rank, self.markers)

@@ -175,6 +175,41 @@ int fgpu_batch_retire(fgpu_batch *batch, int max, fgpu_sys **out, fgpu_solve_sta
  * (which 1) over all members, in the batch's precision; for measurement, no counterpart in
  * the reference.  Overwrites the members' slot buffers. */
 int fgpu_batch_time_kernel(fgpu_batch *batch, int which, int reps, double *avg_ms);
+/* ---- transfer to atoms (Interpolator, InterpolatorCells, SolutionReader) ---------------
+ * An interpolator holds the tet mesh, its Tethex hexes (4 per tet, hex 4t+k around vertex k,
+ * femocs/UCD node order) and the nodal solution, 5 doubles per node (Solution: vector x y z,
+ * scalar1, scalar2). */
+#define FGPU_TYPE_BULK 1
+#define FGPU_TYPE_VACUUM 3
+typedef struct fgpu_interp fgpu_interp;
+/* Interpolator::initialize (src/Interpolator.cpp:28-77) with LinearTetrahedra /
+ * QuadraticTetrahedra / LinearHexahedra::precompute (src/InterpolatorCells.cpp:523-629,
+ * 761-781, 1151-1174, 1205-1267): node_kind 1 tet vertex, 2 edge, 3 face, 4 tet centre;
+ * tet_nbr: 4 face neighbours per tet (-1 none); search_region FGPU_TYPE_VACUUM or
+ * FGPU_TYPE_BULK; every node starts at empty_value.  Cells are found with a bounding-volume
+ * tree on the device instead of the reference's linear scan, with the same results. */
+int fgpu_interp_create(fgpu_ctx *ctx, int n_nodes, const double *xyz, const int *node_kind, int n_tets,
+                       const int *tets, const int *tet_nbr, const int *tet_marker, int n_hexes, const int *hexes,
+                       const int *hex_marker, int search_region, double empty_value, fgpu_interp **out);
+void fgpu_interp_destroy(fgpu_interp *interp);
+/* Interpolator::extract_solution(PoissonSolver&, smoothen) (src/Interpolator.cpp:142-190): from
+ * a solved system whose mesh was imported with these nodes as vertices: Solution{nodal field
+ * (store_elfield), charge density 0 (field_mode laplace), potential}; nodes outside the
+ * region get empty_value; smoothen: average_nodal_fields over the pseudo-Voronoi cells */
+int fgpu_interp_extract(fgpu_interp *interp, fgpu_sys *sys, int smoothen);
+/* the nodal solution (5 * n_nodes), as Interpolator::store_solution / nodes hold it */
+int fgpu_interp_get_nodal(fgpu_interp *interp, double *sol5);
+int fgpu_interp_set_nodal(fgpu_interp *interp, const double *sol5);
+/* SolutionReader::calc_full_interpolation, dim 3 (src/SolutionReader.cpp:43-165): every point
+ * located with the guess chained from the previous one (starting at -1), then interpolated;
+ * rank 1 linear tets, 2 quadratic tets, 3 linear hexes.  cells: tet (ranks 1, 2) or hex
+ * (rank 3) per point, negative when outside the region (cell of the nearest centroid);
+ * out: 5 per point */
+int fgpu_interp_points(fgpu_interp *interp, int n, const double *xyz, int rank, int *cells, double *out);
+/* SolutionReader::calc_interpolation with the atoms already mapped (cells from a previous
+ * fgpu_interp_points call) */
+int fgpu_interp_cells(fgpu_interp *interp, int n, const double *xyz, int rank, const int *cells, double *out);
+
 /* DealSolver::check_limits / stat (src/DealSolver.cpp:158-167) */
 int fgpu_sys_minmax(fgpu_sys *sys, double *min, double *max);
 /* DealSolver::export_solution_grad (src/DealSolver.cpp:281-301): -grad u at the Gauss point

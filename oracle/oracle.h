@@ -118,6 +118,43 @@ int  orc_linhex_locate(const orc_lintet *L, const double p[3], int guess);
  * (src/InterpolatorCells.cpp:1237-1244, 1272-1317, 1334-1353) */
 void orc_linhex_shape_functions(const double xe[8][3], const double p[3], double sf[8]);
 
+/* QuadraticTetrahedra::calc_cell (src/InterpolatorCells.cpp:1151-1174): per tet its four
+ * vertices and the six edge-centre nodes n5..n10 found as the first common EDGECENTROID node
+ * (node_kind 2) of the hexes 4t+i, 4t+j (pairs 01, 12, 20, 03, 13, 23).  cells10: 10 per tet */
+void orc_quadtet_cells(int n_tets, const int *tets, const int *hexes, const int *node_kind, int *cells10);
+
+/* Interpolator::extract_solution(PoissonSolver&) without smoothing (src/Interpolator.cpp:79-140,
+ * 172-190): Solution{E, charge density, phi} per node, 5 doubles (vector x y z, scalar1,
+ * scalar2).  in_region[n] != 0 for nodes of the solver mesh (femocs2deal >= 0): E = nodal
+ * field (zero if no hex of the region touches the node), scalar1 = rho[n], scalar2 = phi[n];
+ * other nodes get empty in all five. */
+void orc_extract_nodal(int n_nodes, const double *xyz, int n_hexes, const int *hexes, const int *hex_marker,
+                       int region_sign, const unsigned char *in_region, const double *phi, const double *rho,
+                       double empty, double *sol5);
+
+/* Interpolator::average_nodal_fields + TetgenMesh::calc_pseudo_3D_vorocells
+ * (src/Interpolator.cpp:142-170, src/TetgenMesh.cpp:819-837): for each tet node i < n_tetnodes,
+ * E_i <- sum_w E_nbr / sum w, w = exp(decay * |x_i - x_nbr|), over the nodes of the region's
+ * hexes tied to i (|hex marker| - 1 == i) with node_kind >= 2, repeats kept.  decay =
+ * -1 / (longest tet edge) (LinearTetrahedra::precompute, src/InterpolatorCells.cpp:534) */
+double orc_decay_factor(const double *xyz, int n_tets, const int *tets);
+void orc_smooth_nodal(int n_nodes, int n_tetnodes, const double *xyz, const int *node_kind, int n_hexes,
+                      const int *hexes, const int *hex_marker, int vacuum, double decay, double *sol5);
+
+/* SolutionReader::calc_full_interpolation for dim 3, unsorted atoms (src/SolutionReader.cpp:43-165,
+ * InterpolatorCells::locate_interpolate / interp_solution src/InterpolatorCells.cpp:331-355,
+ * 441-445): chained locate from guess -1, then sum_i w_i Solution_i with rank 1 the linear tet
+ * weights, rank 2 the quadratic tet weights (src/InterpolatorCells.cpp:795-812) over cells10,
+ * rank 3 the trilinear hex weights over hexes.  cells: tet (ranks 1, 2) or hex (rank 3) per
+ * point, negative outside; out: 5 per point. */
+void orc_interp_points(const orc_lintet *L, const double *xyz, const int *tets, const int *cells10,
+                       const int *hexes, const double *sol5, int n_points, const double *pts, int rank,
+                       int *cells, double *out);
+/* SolutionReader::calc_interpolation with mapped atoms: interpolation in |cells[i]| */
+void orc_interp_cells(const orc_lintet *L, const double *xyz, const int *tets, const int *cells10,
+                      const int *hexes, const double *sol5, int n_points, const double *pts, int rank,
+                      const int *cells, double *out);
+
 /* physical quantities: resistivity table, sigma, kappa (src/PhysicalQuantities.cpp:31-64,
  * 173-185; table src/PhysicalQuantityData.cpp:47-76) */
 double orc_resistivity(double T);

@@ -80,6 +80,13 @@ def _declare(L):
     L.orc_lintet_bcc.argtypes = [C.POINTER(_Lintet), _D, C.c_int, _D]
     L.orc_locate_chain.argtypes = [C.POINTER(_Lintet), C.c_int, _D, C.c_int, _I]
     L.orc_linhex_shape_functions.argtypes = [_D, _D, _D]
+    L.orc_quadtet_cells.argtypes = [C.c_int, _I, _I, _I, _I]
+    L.orc_extract_nodal.argtypes = [C.c_int, _D, C.c_int, _I, _I, C.c_int, _U8, _D, C.c_void_p, C.c_double, _D]
+    L.orc_decay_factor.argtypes = [_D, C.c_int, _I]
+    L.orc_decay_factor.restype = C.c_double
+    L.orc_smooth_nodal.argtypes = [C.c_int, C.c_int, _D, _I, C.c_int, _I, _I, C.c_int, C.c_double, _D]
+    L.orc_interp_points.argtypes = [C.POINTER(_Lintet), _D, _I, _I, _I, _D, C.c_int, _D, C.c_int, _I, _D]
+    L.orc_interp_cells.argtypes = [C.POINTER(_Lintet), _D, _I, _I, _I, _D, C.c_int, _D, C.c_int, _I, _D]
     for f in ("orc_resistivity", "orc_sigma"):
         getattr(L, f).argtypes = [C.c_double]
         getattr(L, f).restype = C.c_double
@@ -253,6 +260,64 @@ class LinTet:
         out = np.zeros(len(pts) // 3, np.int32)
         lib().orc_locate_chain(C.byref(self._L), len(out), pts, rank, out)
         return out
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32).reshape(-1)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64).reshape(-1)
+
+
+def quadtet_cells(tets, hexes, node_kind):
+    out = np.zeros((len(tets), 10), np.int32)
+    lib().orc_quadtet_cells(len(tets), _i32(tets), _i32(hexes), _i32(node_kind), out.reshape(-1))
+    return out
+
+
+def extract_nodal(nodes, hexes, hex_marker, region_sign, in_region, phi, rho=None, empty=0.0):
+    """Nodal Solution rows (Ex, Ey, Ez, scalar1, scalar2) of Interpolator::extract_solution."""
+    n = len(nodes)
+    out = np.zeros(5 * n)
+    r = None if rho is None else _f64(rho)
+    lib().orc_extract_nodal(n, _f64(nodes), len(hexes), _i32(hexes), _i32(hex_marker), region_sign,
+                            np.ascontiguousarray(in_region, np.uint8), _f64(phi),
+                            None if r is None else r.ctypes.data, float(empty), out)
+    return out.reshape(-1, 5)
+
+
+def decay_factor(nodes, tets):
+    return lib().orc_decay_factor(_f64(nodes), len(tets), _i32(tets))
+
+
+def smooth_nodal(sol5, nodes, n_tetnodes, node_kind, hexes, hex_marker, vacuum=True, decay=None, tets=None):
+    sol = np.ascontiguousarray(sol5, np.float64).copy()
+    if decay is None:
+        decay = decay_factor(nodes, tets)
+    lib().orc_smooth_nodal(len(nodes), n_tetnodes, _f64(nodes), _i32(node_kind), len(hexes), _i32(hexes),
+                           _i32(hex_marker), int(bool(vacuum)), float(decay), sol.reshape(-1))
+    return sol
+
+
+def interp_points(L: "LinTet", nodes, tets, cells10, hexes, sol5, points, rank):
+    """SolutionReader::calc_full_interpolation (dim 3): (cells, Solution rows)."""
+    pts = _f64(points)
+    n = len(pts) // 3
+    cells = np.zeros(n, np.int32)
+    out = np.zeros(5 * n)
+    lib().orc_interp_points(C.byref(L._L), _f64(nodes), _i32(tets), _i32(cells10), _i32(hexes), _f64(sol5), n, pts,
+                            rank, cells, out)
+    return cells, out.reshape(-1, 5)
+
+
+def interp_cells(L: "LinTet", nodes, tets, cells10, hexes, sol5, points, rank, cells):
+    pts = _f64(points)
+    n = len(pts) // 3
+    out = np.zeros(5 * n)
+    lib().orc_interp_cells(C.byref(L._L), _f64(nodes), _i32(tets), _i32(cells10), _i32(hexes), _f64(sol5), n, pts,
+                           rank, _i32(cells), out)
+    return out.reshape(-1, 5)
 
 
 def linhex_shape_functions(xe_ucd, p):

@@ -313,6 +313,190 @@ void orc_nodal_field(int n_nodes, const double *xyz, int nh, const int *hexes, c
 }
 
 /* ------------------------------------------------------------------------------------- */
+/* quadratic tetrahedra, nodal solution, smoothing, per-point interpolation               */
+/* ------------------------------------------------------------------------------------- */
+static int common_entry(const int *a, int na, const int *b, int nb) {
+    for (int i = 0; i < na; ++i)
+        for (int j = 0; j < nb; ++j)
+            if (a[i] == b[j]) return a[i];
+    return -1;
+}
+
+void orc_quadtet_cells(int nt, const int *tets, const int *hexes, const int *node_kind, int *cells10) {
+    for (int t = 0; t < nt; ++t) {
+        int e[4][8], ne[4] = {0, 0, 0, 0};
+        for (int i = 0; i < 4; ++i)
+            for (int k = 0; k < 8; ++k) {
+                const int node = hexes[8 * (4 * t + i) + k];
+                if (node_kind[node] == 2) e[i][ne[i]++] = node;
+            }
+        int *c = cells10 + 10 * t;
+        for (int k = 0; k < 4; ++k) c[k] = tets[4 * t + k];
+        c[4] = common_entry(e[0], ne[0], e[1], ne[1]);
+        c[5] = common_entry(e[1], ne[1], e[2], ne[2]);
+        c[6] = common_entry(e[2], ne[2], e[0], ne[0]);
+        c[7] = common_entry(e[0], ne[0], e[3], ne[3]);
+        c[8] = common_entry(e[1], ne[1], e[3], ne[3]);
+        c[9] = common_entry(e[2], ne[2], e[3], ne[3]);
+    }
+}
+
+void orc_extract_nodal(int n_nodes, const double *xyz, int nh, const int *hexes, const int *hex_marker,
+                       int region_sign, const unsigned char *in_region, const double *phi, const double *rho,
+                       double empty, double *sol5) {
+    double *E = malloc(sizeof(double) * 3 * (n_nodes ? n_nodes : 1));
+    orc_nodal_field(n_nodes, xyz, nh, hexes, hex_marker, region_sign, phi, E);
+    for (int n = 0; n < n_nodes; ++n) {
+        double *s = sol5 + 5 * n;
+        if (in_region[n]) {
+            s[0] = E[3 * n]; s[1] = E[3 * n + 1]; s[2] = E[3 * n + 2];
+            s[3] = rho ? rho[n] : 0.0;
+            s[4] = phi[n];
+        } else {
+            for (int k = 0; k < 5; ++k) s[k] = empty;
+        }
+    }
+    free(E);
+}
+
+static double dist2(const double *a, const double *b) {
+    const double xx = a[0] - b[0], yy = a[1] - b[1], zz = a[2] - b[2];
+    return xx * xx + yy * yy + zz * zz;
+}
+
+double orc_decay_factor(const double *xyz, int nt, const int *tets) {
+    double emax = -1e308;
+    for (int t = 0; t < nt; ++t) {
+        const double *n1 = xyz + 3 * tets[4 * t], *n2 = xyz + 3 * tets[4 * t + 1];
+        const double *n3 = xyz + 3 * tets[4 * t + 2], *n4 = xyz + 3 * tets[4 * t + 3];
+        const double e[6] = {dist2(n1, n2), dist2(n1, n3), dist2(n1, n4), dist2(n2, n3), dist2(n2, n4), dist2(n3, n4)};
+        for (int k = 0; k < 6; ++k) emax = e[k] > emax ? e[k] : emax;
+    }
+    return -1.0 / sqrt(emax);
+}
+
+void orc_smooth_nodal(int n_nodes, int n_tetnodes, const double *xyz, const int *node_kind, int nh,
+                      const int *hexes, const int *hex_marker, int vacuum, double decay, double *sol5) {
+    (void)n_nodes;
+    const int mult = vacuum ? 1 : -1;
+    /* neighbour lists in hex order */
+    int *cnt = calloc(n_tetnodes + 1, sizeof(int));
+    for (int h = 0; h < nh; ++h) {
+        int tn = mult * hex_marker[h];
+        if (tn <= 0) continue;
+        tn -= 1;
+        for (int k = 0; k < 8; ++k) {
+            const int node = hexes[8 * h + k];
+            if (node != tn && node_kind[node] >= 2) cnt[tn + 1]++;
+        }
+    }
+    for (int i = 0; i < n_tetnodes; ++i) cnt[i + 1] += cnt[i];
+    int *lst = malloc(sizeof(int) * (cnt[n_tetnodes] ? cnt[n_tetnodes] : 1));
+    int *pos = malloc(sizeof(int) * (n_tetnodes + 1));
+    memcpy(pos, cnt, sizeof(int) * (n_tetnodes + 1));
+    for (int h = 0; h < nh; ++h) {
+        int tn = mult * hex_marker[h];
+        if (tn <= 0) continue;
+        tn -= 1;
+        for (int k = 0; k < 8; ++k) {
+            const int node = hexes[8 * h + k];
+            if (node != tn && node_kind[node] >= 2) lst[pos[tn]++] = node;
+        }
+    }
+    for (int i = 0; i < n_tetnodes; ++i) {
+        if (cnt[i + 1] == cnt[i]) continue;
+        v3 vec = mk(0, 0, 0);
+        double w_sum = 0;
+        for (int e = cnt[i]; e < cnt[i + 1]; ++e) {
+            const int nb = lst[e];
+            const double w = exp(decay * sqrt(dist2(xyz + 3 * i, xyz + 3 * nb)));
+            w_sum += w;
+            vec = add(vec, scl(mk(sol5[5 * nb], sol5[5 * nb + 1], sol5[5 * nb + 2]), w));
+        }
+        if (w_sum > 0) {
+            vec = scl(vec, 1.0 / w_sum);
+            sol5[5 * i] = vec.x; sol5[5 * i + 1] = vec.y; sol5[5 * i + 2] = vec.z;
+        }
+    }
+    free(pos);
+    free(lst);
+    free(cnt);
+}
+
+/* weights of point p in cell c (already |c|) and their nodes */
+static int cell_weights(const orc_lintet *L, const double *xyz, const int *tets, const int *cells10,
+                        const int *hexes, const double p[3], int rank, int c, double w[10], int nodes[10]) {
+    if (rank == 3) {
+        double xe[8][3];
+        for (int k = 0; k < 8; ++k) {
+            nodes[k] = hexes[8 * c + k];
+            for (int a = 0; a < 3; ++a) xe[k][a] = xyz[3 * nodes[k] + a];
+        }
+        orc_linhex_shape_functions(xe, p, w);
+        return 8;
+    }
+    double b[4];
+    orc_lintet_bcc(L, p, c, b);
+    if (rank == 1) {
+        for (int k = 0; k < 4; ++k) {
+            w[k] = b[k];
+            nodes[k] = tets[4 * c + k];
+        }
+        return 4;
+    }
+    const double b1 = b[0], b2 = b[1], b3 = b[2], b4 = b[3];
+    w[0] = b1 * (2 * b1 - 1);
+    w[1] = b2 * (2 * b2 - 1);
+    w[2] = b3 * (2 * b3 - 1);
+    w[3] = b4 * (2 * b4 - 1);
+    w[4] = 4 * b1 * b2;
+    w[5] = 4 * b2 * b3;
+    w[6] = 4 * b3 * b1;
+    w[7] = 4 * b1 * b4;
+    w[8] = 4 * b2 * b4;
+    w[9] = 4 * b3 * b4;
+    for (int k = 0; k < 10; ++k) nodes[k] = cells10[10 * c + k];
+    return 10;
+}
+
+static void interp_one(const orc_lintet *L, const double *xyz, const int *tets, const int *cells10,
+                       const int *hexes, const double *sol5, const double p[3], int rank, int cell, double *out) {
+    double w[10];
+    int nodes[10];
+    const int n = cell_weights(L, xyz, tets, cells10, hexes, p, rank, abs(cell), w, nodes);
+    double vx = 0, vy = 0, vz = 0, s1 = 0, s2 = 0;
+    for (int i = 0; i < n; ++i) {
+        const double *s = sol5 + 5 * nodes[i];
+        vx += s[0] * w[i];
+        vy += s[1] * w[i];
+        vz += s[2] * w[i];
+        s1 += s[3] * w[i];
+        s2 += s[4] * w[i];
+    }
+    out[0] = vx; out[1] = vy; out[2] = vz; out[3] = s1; out[4] = s2;
+}
+
+void orc_interp_points(const orc_lintet *L, const double *xyz, const int *tets, const int *cells10,
+                       const int *hexes, const double *sol5, int np, const double *pts, int rank,
+                       int *cells, double *out) {
+    int cell = -1;
+    for (int i = 0; i < np; ++i) {
+        const double *p = pts + 3 * i;
+        const int g = abs(cell);
+        cell = rank == 3 ? orc_linhex_locate(L, p, g) : orc_lintet_locate(L, p, g);
+        cells[i] = cell;
+        interp_one(L, xyz, tets, cells10, hexes, sol5, p, rank, cell, out + 5 * i);
+    }
+}
+
+void orc_interp_cells(const orc_lintet *L, const double *xyz, const int *tets, const int *cells10,
+                      const int *hexes, const double *sol5, int np, const double *pts, int rank,
+                      const int *cells, double *out) {
+    for (int i = 0; i < np; ++i)
+        interp_one(L, xyz, tets, cells10, hexes, sol5, pts + 3 * i, rank, cells[i], out + 5 * i);
+}
+
+/* ------------------------------------------------------------------------------------- */
 /* physical quantities                                                                    */
 /* ------------------------------------------------------------------------------------- */
 /* resistivity of bulk Cu, (T [K], rho [Ohm nm]) pairs, src/PhysicalQuantityData.cpp:47-76 */

@@ -1,0 +1,113 @@
+"""GPU transfer to atoms against the oracle: nodal solution, located cells (bit-exact, with the
+chained guess) and interpolated values (bit-exact for every rank) on seeded point sets that mix
+interior points, points on mesh nodes, points in the other region and points outside."""
+import numpy as np
+import pytest
+
+from femocs_amd import fgpu, interp
+from femocs_amd import mesh as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+MARKS = (8, 2, 4, 2)
+
+
+@pytest.fixture(scope="module")
+def tip():
+    return M.make_tip_mesh(M.small_geometry())
+
+
+@pytest.fixture(scope="module")
+def solved(ctx, tip):
+    gm = fgpu.Mesh(ctx, tip.nodes, tip.vacuum_cells(), *MARKS)
+    s = fgpu.System(gm)
+    s.reset()
+    s.add_dirichlet(2, 0.0)
+    s.add_neumann(8, None, -1.0)
+    s.apply_dirichlet()
+    s.fill_solution(0.0)
+    assert s.solve(fgpu.opts(tol=1e-12, relative=True)).iterations > 0
+    xyz, vod = gm.dofs()
+    phi = np.zeros(len(tip.nodes))
+    phi[vod] = s.solution()
+    used = np.zeros(len(tip.nodes), np.uint8)
+    used[vod] = 1
+    return gm, s, phi, used
+
+
+def _points(tip, n, seed):
+    rng = np.random.default_rng(seed)
+    x = tip.nodes
+    vac = np.nonzero(tip.tet_marker == 3)[0]
+    blk = np.nonzero(tip.tet_marker != 3)[0]
+    b = rng.dirichlet(np.ones(4), n)
+    inside = np.einsum("nk,nkd->nd", b, x[tip.tets[rng.choice(vac, n)]])
+    bulk = np.einsum("nk,nkd->nd", b[: n // 4], x[tip.tets[rng.choice(blk, n // 4)]])
+    on_nodes = x[rng.choice(tip.n_surface_nodes, n // 4)]            # surface atoms are tet vertices
+    lo, hi = x.min(0), x.max(0)
+    outside = rng.uniform(lo - 20, hi + 20, (n // 8, 3))
+    pts = np.concatenate([inside, on_nodes, bulk, outside])
+    return pts[rng.permutation(len(pts))] if seed % 2 else pts
+
+
+@pytest.mark.parametrize("smoothen", [False, True])
+def test_nodal_solution_matches_oracle(ctx, tip, solved, smoothen):
+    _, s, phi, used = solved
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM, empty_value=-3.5)
+    ip.extract_solution(s, smoothen=smoothen)
+    ref = O.extract_nodal(tip.nodes, tip.hexes, tip.hex_marker, 1, used, phi, empty=-3.5)
+    if smoothen:
+        ref = O.smooth_nodal(ref, tip.nodes, tip.n_tetnodes, tip.node_kind, tip.hexes, tip.hex_marker, True,
+                             tets=tip.tets)
+        # exp() of the device library and of the host C library may differ in the last bit
+        got = ip.nodal()
+        assert np.array_equal(got[:, 3:], ref[:, 3:])
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref[:, :3]).max()
+    else:
+        assert np.array_equal(ip.nodal(), ref)
+
+
+@pytest.mark.parametrize("rank", [1, 2, 3])
+def test_located_cells_and_values_bit_exact(ctx, tip, solved, rank):
+    _, s, phi, used = solved
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+    ip.extract_solution(s, smoothen=False)
+    sol = ip.nodal()
+    L = O.LinTet(tip.nodes, tip.tets, tip.tet_nbr, tip.tet_marker, 3)
+    q10 = O.quadtet_cells(tip.tets, tip.hexes, tip.node_kind)
+    for seed in (1, 2):
+        pts = _points(tip, 400, seed + 10 * rank)
+        cells_o, out_o = O.interp_points(L, tip.nodes, tip.tets, q10, tip.hexes, sol, pts, rank)
+        cells_g, out_g = ip.points(pts, rank)
+        assert np.array_equal(cells_g, cells_o)
+        assert np.array_equal(out_g, out_o)
+        # mapped atoms: interpolation in the stored cells
+        assert np.array_equal(ip.in_cells(pts, rank, cells_g), out_o)
+        assert (cells_o < 0).any() and (cells_o > 0).any()
+
+
+def test_bulk_search_region_and_reader(ctx, tip):
+    """The bulk interpolator searches the bulk tets; the reader maps once and reuses it."""
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_BULK)
+    rng = np.random.default_rng(3)
+    sol = rng.normal(size=(len(tip.nodes), 5))
+    ip.set_nodal(sol)
+    L = O.LinTet(tip.nodes, tip.tets, tip.tet_nbr, tip.tet_marker, 1)
+    q10 = O.quadtet_cells(tip.tets, tip.hexes, tip.node_kind)
+    pts = _points(tip, 300, 5)
+    r = interp.SolutionReader(ip, rank=1)
+    got = r.calc_interpolation(pts)
+    cells_o, out_o = O.interp_points(L, tip.nodes, tip.tets, q10, tip.hexes, sol, pts, 1)
+    assert np.array_equal(r.markers, cells_o)
+    assert np.array_equal(got, out_o)
+    assert np.array_equal(r.calc_interpolation(pts), out_o)
+
+
+def test_empty_and_invalid_calls(ctx, tip):
+    ip = interp.Interpolator(ctx, tip)
+    cells, out = ip.points(np.zeros((0, 3)), 1)
+    assert cells.size == 0 and out.size == 0
+    with pytest.raises(fgpu.FgpuError):
+        ip.points(np.zeros((2, 3)), 4)
+    with pytest.raises(fgpu.FgpuError):
+        ip.in_cells(np.zeros((1, 3)), 1, np.array([len(tip.tets) + 5]))
