@@ -1,6 +1,8 @@
 // C ABI of libfemocs_gpu and the batched CG drivers.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -248,6 +250,8 @@ extern "C" int fgpu_mesh_import(fgpu_ctx *ctx, int nv, const double *xyz, int nc
         m->d_cells = dalloc<int>(icells.size());
         h2d(m->d_cells, icells.data(), icells.size(), s);
         HIPCHECK(hipStreamSynchronize(s));
+        static std::atomic<unsigned long long> next_uid{1};
+        m->uid = next_uid++;
         m->d_X64 = dalloc<double>(X.size());
         m->d_X32 = dalloc<float>(X.size());
         upload_patches(m, gptr);
@@ -868,6 +872,33 @@ extern "C" int fgpu_sys_set_coefficients(fgpu_sys *s, const double *kq, const do
             s->have_coef = true;
         }
         s->diag_valid = false;
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_sys_heat_terms(fgpu_sys *h, const fgpu_sys *cur, double gamma, double lorentz) {
+    if (!h || !cur || !(gamma >= 0)) return FGPU_EINVAL;
+    return guarded(h->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(h->mesh);
+        if (cur->mesh != h->mesh) throw std::invalid_argument("heat and current systems must share the mesh");
+        hipStream_t st = h->ctx->stream;
+        HIPCHECK(hipSetDevice(h->ctx->device));
+        const size_t nc = m->n_cells;
+        if (!h->kq) {
+            h->kq = dalloc<double>(8 * nc); h->gq = dalloc<double>(8 * nc);
+            h->kq32 = dalloc<float>(8 * nc); h->gq32 = dalloc<float>(8 * nc);
+        }
+        double *f = dalloc<double>(8 * nc);
+        h->rhs_terms.push_back({1, 0, f});
+        launch_heat_terms(st, (int)nc, m->d_cells, m->d_X64, h->x, cur->x, gamma, lorentz, h->kq, h->gq, h->kq32, h->gq32, f);
+        h->have_coef = true;
+        h->diag_valid = false;
+        Desc d = make_desc64(h, true);
+        d.q = h->b;
+        BatchTabs t = upload_batch(h->ctx, {{d}}, {h});
+        launch_elem_rhs(st, t.sets[0], t.etab, t.ne, f, t.lds64);
+        launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, true);
+        HIPCHECK(hipStreamSynchronize(st));
         return FGPU_OK;
     });
 }

@@ -103,6 +103,7 @@ struct Mesh {
     int n_slots = 0;                       // slot-buffer entries (sum of patch node counts)
     int patch_gen = 0;                     // bumped whenever the patches are rebuilt
     int coord_gen = 0;                     // bumped whenever the vertices move
+    unsigned long long uid = 0;            // unique per imported mesh (caches keyed on it)
     bool x_stale = false;                  // device coordinates moved, host copy `x` not yet
     // linear mesh motion (fgpu_mesh_set_motion): row r (dof idof[r]) sits at base[r] +
     // sum_k w[k] * disp[ctrl[k]] for k in ptr[r]..ptr[r+1]

@@ -15,6 +15,7 @@
 #include "femocs_gpu.h"
 #include "fgpu_internal.h"
 #include "fgpu_interp.h"
+#include "fgpu_kernels.h"
 
 using namespace fgpu;
 
@@ -32,7 +33,7 @@ struct fgpu_interp {
     int *n2c_ptr = nullptr, *n2c = nullptr, *vor_ptr = nullptr, *vor = nullptr;
     double *sol = nullptr;                  // 5 per node: vector xyz, scalar1, scalar2
     int *node_idof = nullptr;               // node -> internal dof of `map_mesh`, -1 outside
-    const void *map_mesh = nullptr;
+    unsigned long long map_mesh = 0;        // uid of that mesh
     // point buffers (grown on demand)
     double *pts = nullptr, *out = nullptr;
     int4 *cand = nullptr;
@@ -425,6 +426,27 @@ __global__ __launch_bounds__(256) void k_smooth(int n_tetnodes, const int *__res
     }
 }
 
+// Interpolator::extract_solution(CurrentHeatSolver&) (src/Interpolator.cpp:207-216):
+// Solution{-grad phi_c (export_solution_grad), phi_c, T}; nodes outside: {0, 0, empty}
+__global__ __launch_bounds__(256) void k_extract_heat(int n_nodes, const int *__restrict__ node_idof,
+                                                      const double *__restrict__ grad, const double *__restrict__ phic,
+                                                      const double *__restrict__ T, double empty, double *__restrict__ sol) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    double *s = sol + 5 * (size_t)n;
+    const int id = node_idof[n];
+    if (id < 0) {
+        s[0] = s[1] = s[2] = s[3] = 0.0;
+        s[4] = empty;
+        return;
+    }
+    s[0] = grad[3 * (size_t)id];
+    s[1] = grad[3 * (size_t)id + 1];
+    s[2] = grad[3 * (size_t)id + 2];
+    s[3] = phic[id];
+    s[4] = T[id];
+}
+
 inline int grid(int n) { return (n + 255) / 256; }
 
 void ensure_points(fgpu_interp *ip, size_t n) {
@@ -593,6 +615,8 @@ extern "C" void fgpu_interp_destroy(fgpu_interp *ip) {
     fgpu_ctx_release_ref(c);
 }
 
+static void node_map(fgpu_interp *ip, const fgpu_mesh *m);
+
 extern "C" int fgpu_interp_extract(fgpu_interp *ip, fgpu_sys *s, int smoothen) {
     if (!ip || !s) return FGPU_EINVAL;
     return guard(ip, [&] {
@@ -602,7 +626,24 @@ extern "C" int fgpu_interp_extract(fgpu_interp *ip, fgpu_sys *s, int smoothen) {
         if (m->n_vertices != ip->n_nodes)
             throw std::invalid_argument("the system's mesh was not imported from these nodes");
         hipStream_t st = ip->ctx->stream;
-        if (ip->map_mesh != (const void *)m) {
+        node_map(ip, m);
+        hipLaunchKernelGGL(k_extract, dim3(grid(ip->n_nodes)), dim3(256), 0, st, ip->n_nodes, ip->node_idof, s->x,
+                           ip->n2c_ptr, ip->n2c, ip->hexes, ip->xyz, ip->empty, ip->sol);
+        if (smoothen && ip->n_tetnodes > 0)
+            hipLaunchKernelGGL(k_smooth, dim3(grid(ip->n_tetnodes)), dim3(256), 0, st, ip->n_tetnodes, ip->vor_ptr,
+                               ip->vor, ip->xyz, ip->decay, ip->sol);
+        IPCHECK(hipGetLastError());
+        IPCHECK(hipStreamSynchronize(st));
+        return FGPU_OK;
+    });
+}
+
+// node -> internal dof of mesh m (cached per mesh); every corner of the region's hexes must
+// be a dof of m
+static void node_map(fgpu_interp *ip, const fgpu_mesh *m) {
+    {
+        hipStream_t st = ip->ctx->stream;
+        if (ip->map_mesh != m->uid) {
             std::vector<int> map(ip->n_nodes, -1);
             for (int v = 0; v < ip->n_nodes; ++v) {
                 const int d = m->dof_of_vertex[v];
@@ -621,15 +662,32 @@ extern "C" int fgpu_interp_extract(fgpu_interp *ip, fgpu_sys *s, int smoothen) {
                         throw std::invalid_argument("a hex of the search region has a node outside the system's mesh");
             release(ip->node_idof);
             ip->node_idof = upload(map.data(), map.size(), st);
-            ip->map_mesh = m;
+            IPCHECK(hipStreamSynchronize(st));
+            ip->map_mesh = m->uid;
         }
-        hipLaunchKernelGGL(k_extract, dim3(grid(ip->n_nodes)), dim3(256), 0, st, ip->n_nodes, ip->node_idof, s->x,
-                           ip->n2c_ptr, ip->n2c, ip->hexes, ip->xyz, ip->empty, ip->sol);
-        if (smoothen && ip->n_tetnodes > 0)
-            hipLaunchKernelGGL(k_smooth, dim3(grid(ip->n_tetnodes)), dim3(256), 0, st, ip->n_tetnodes, ip->vor_ptr,
-                               ip->vor, ip->xyz, ip->decay, ip->sol);
-        IPCHECK(hipGetLastError());
-        IPCHECK(hipStreamSynchronize(st));
+    }
+}
+
+extern "C" int fgpu_interp_extract_heat(fgpu_interp *ip, fgpu_sys *cur, fgpu_sys *heat) {
+    if (!ip || !cur || !heat) return FGPU_EINVAL;
+    return guard(ip, [&] {
+        IPCHECK(hipSetDevice(ip->ctx->device));
+        const fgpu_mesh *m = static_cast<const fgpu_mesh *>(cur->mesh);
+        if (cur->ctx != ip->ctx || heat->ctx != ip->ctx) throw std::invalid_argument("system of another context");
+        if (heat->mesh != cur->mesh) throw std::invalid_argument("current and heat systems must share the mesh");
+        if (m->n_vertices != ip->n_nodes) throw std::invalid_argument("the systems' mesh was not imported from these nodes");
+        hipStream_t st = ip->ctx->stream;
+        node_map(ip, m);
+        double *grad = nullptr;
+        IPCHECK(hipMalloc(&grad, 3 * (size_t)std::max(1, m->n_dofs) * sizeof(double)));
+        launch_grad_export(st, m->n_dofs, m->d_v2c, m->d_v2n, m->d_cells, m->d_X64, cur->x, grad);
+        hipLaunchKernelGGL(k_extract_heat, dim3(grid(ip->n_nodes)), dim3(256), 0, st, ip->n_nodes, ip->node_idof, grad,
+                           cur->x, heat->x, ip->empty, ip->sol);
+        const hipError_t e = hipGetLastError();
+        const hipError_t e2 = hipStreamSynchronize(st);
+        (void)hipFree(grad);
+        IPCHECK(e);
+        IPCHECK(e2);
         return FGPU_OK;
     });
 }

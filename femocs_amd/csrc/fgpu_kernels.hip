@@ -540,6 +540,98 @@ __global__ __launch_bounds__(kBlock) void k_center_det_min(int n, const int *__r
     }
 }
 
+// resistivity of bulk Cu, (T [K], rho [Ohm nm]) pairs of the reference's table
+// (src/PhysicalQuantityData.cpp:47-76); rho(T) = 10 * linear interpolation [Ohm A]
+__constant__ double kRhoT[26] = {200, 250, 273.15, 300, 350, 400, 450, 500, 550, 600, 650, 700, 750,
+                                 800, 850, 900, 950, 1000, 1050, 1100, 1150, 1200, 1250, 1300, 1350, 1400};
+__constant__ double kRhoV[26] = {
+    1.04880891725670e1, 1.38746906313026e1, 1.54302576609579e1, 1.72303253767393e1, 2.05822342557969e1,
+    2.39476190263668e1, 2.73376276457821e1, 3.07598544167200e1, 3.42197627890272e1, 3.77214997429762e1,
+    4.12683756079926e1, 4.48631564739703e1, 4.85082484331790e1, 5.22058173711257e1, 5.59578691681847e1,
+    5.97663048918425e1, 6.36329597840890e1, 6.75596315079091e1, 7.15481011300350e1, 7.56001491045029e1,
+    7.97175677635730e1, 8.39021713382574e1, 8.81558042149690e1, 9.24803479251429e1, 9.68777272230961e1,
+    1.01349915510292e2};
+
+// PhysicalQuantities::sigma / kappa (src/PhysicalQuantities.cpp:31-64, 173-185): T clamped to
+// the table, rho = 10 * lin_interp, sigma = 1 / rho, kappa = L T sigma
+__device__ inline double pq_sigma(double T) {
+    T = T < kRhoT[0] ? kRhoT[0] : (T > kRhoT[25] ? kRhoT[25] : T);
+    double r;
+    if (T <= kRhoT[0]) r = kRhoV[0];
+    else if (T >= kRhoT[25]) r = kRhoV[25];
+    else {
+        int i1 = 0;
+        while (kRhoT[i1] < T) ++i1;
+        const int i2 = i1 - 1;
+        r = kRhoV[i2] + (kRhoV[i1] - kRhoV[i2]) * (T - kRhoT[i2]) / (kRhoT[i1] - kRhoT[i2]);
+    }
+    return 1.0 / (10. * r);
+}
+
+// HeatSolver::assemble_local_cell (src/CurrentHeatSolver.cpp:347-393) per Gauss point q of
+// cell c (internal order): T_old and grad phi_c interpolated at q; stiffness coefficient
+// kappa(T_old), mass coefficient gamma, source gamma T_old + sigma(T_old) |grad phi_c|^2
+__global__ void k_heat_terms(int nc, const int *__restrict__ cells, const double4 *__restrict__ X,
+                             const double *__restrict__ T, const double *__restrict__ phic, double gamma, double lorentz,
+                             double *__restrict__ kq, double *__restrict__ gq, float *__restrict__ kq32,
+                             float *__restrict__ gq32, double *__restrict__ f) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int *cl = cells + 8 * (size_t)c;
+    double xs[3][8], t[8], u[8];
+    for (int n = 0; n < 8; ++n) {
+        const double4 p = X[cl[n]];
+        xs[0][n] = p.x;
+        xs[1][n] = p.y;
+        xs[2][n] = p.z;
+        t[n] = T[cl[n]];
+        u[n] = phic[cl[n]];
+    }
+    double J0[3][2][2], J1[3][2][2], J2[3][2][2], d0[2][2], d1[2][2], d2[2][2];
+    for (int co = 0; co < 3; ++co) {
+        dref<double>(xs[co], 0, J0[co]);
+        dref<double>(xs[co], 1, J1[co]);
+        dref<double>(xs[co], 2, J2[co]);
+    }
+    dref<double>(u, 0, d0);
+    dref<double>(u, 1, d1);
+    dref<double>(u, 2, d2);
+    for (int q = 0; q < 8; ++q) {
+        const int a = q & 1, b = (q >> 1) & 1, cc = (q >> 2) & 1;
+        // T at the Gauss point: trilinear weights
+        double Tq = 0;
+        for (int n = 0; n < 8; ++n)
+            Tq += Wt<double>(a, n & 1) * Wt<double>(b, (n >> 1) & 1) * Wt<double>(cc, (n >> 2) & 1) * t[n];
+        const double c0x = J0[0][b][cc], c0y = J0[1][b][cc], c0z = J0[2][b][cc];
+        const double c1x = J1[0][a][cc], c1y = J1[1][a][cc], c1z = J1[2][a][cc];
+        const double c2x = J2[0][a][b], c2y = J2[1][a][b], c2z = J2[2][a][b];
+        const double a0x = c1y * c2z - c1z * c2y, a0y = c1z * c2x - c1x * c2z, a0z = c1x * c2y - c1y * c2x;
+        const double a1x = c2y * c0z - c2z * c0y, a1y = c2z * c0x - c2x * c0z, a1z = c2x * c0y - c2y * c0x;
+        const double a2x = c0y * c1z - c0z * c1y, a2y = c0z * c1x - c0x * c1z, a2z = c0x * c1y - c0y * c1x;
+        const double det = c0x * a0x + c0y * a0y + c0z * a0z;
+        const double g0 = d0[b][cc], g1 = d1[a][cc], g2 = d2[a][b];
+        const double vx = (g0 * a0x + g1 * a1x + g2 * a2x) / det;
+        const double vy = (g0 * a0y + g1 * a1y + g2 * a2y) / det;
+        const double vz = (g0 * a0z + g1 * a1z + g2 * a2z) / det;
+        const double sg = pq_sigma(Tq);
+        const double Tc = Tq < kRhoT[0] ? kRhoT[0] : (Tq > kRhoT[25] ? kRhoT[25] : Tq);
+        const double kap = lorentz * Tc * pq_sigma(Tc);
+        const size_t o = 8 * (size_t)c + q;
+        kq[o] = kap;
+        gq[o] = gamma;
+        kq32[o] = (float)kap;
+        gq32[o] = (float)gamma;
+        f[o] = gamma * Tq + sg * (vx * vx + vy * vy + vz * vz);
+    }
+}
+
+void launch_heat_terms(hipStream_t s, int nc, const int *cells, const double *X, const double *T, const double *phic,
+                       double gamma, double lorentz, double *kq, double *gq, float *kq32, float *gq32, double *f) {
+    if (nc > 0)
+        hipLaunchKernelGGL(k_heat_terms, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nc, cells,
+                           reinterpret_cast<const double4 *>(X), T, phic, gamma, lorentz, kq, gq, kq32, gq32, f);
+}
+
 // -grad u at the Gauss point v2n of cell v2c (DealSolver::export_solution_grad)
 __global__ void k_grad_export(int n, const int *__restrict__ v2c, const int *__restrict__ v2n, const int *__restrict__ cells,
                               const double4 *__restrict__ X, const double *__restrict__ u, double *__restrict__ grad) {

@@ -83,10 +83,12 @@ API = [
                                      C.c_double, C.POINTER(_P)]),
     ("fgpu_interp_destroy", None, [_P]),
     ("fgpu_interp_extract", C.c_int, [_P, _P, C.c_int]),
+    ("fgpu_interp_extract_heat", C.c_int, [_P, _P, _P]),
     ("fgpu_interp_get_nodal", C.c_int, [_P, _D]),
     ("fgpu_interp_set_nodal", C.c_int, [_P, _D]),
     ("fgpu_interp_points", C.c_int, [_P, C.c_int, _D, C.c_int, _I, _D]),
     ("fgpu_interp_cells", C.c_int, [_P, C.c_int, _D, C.c_int, _I, _D]),
+    ("fgpu_sys_heat_terms", C.c_int, [_P, _P, C.c_double, C.c_double]),
     ("fgpu_sys_minmax", C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("fgpu_sys_solution_grad", C.c_int, [_P, _D]),
     ("fgpu_sys_time_kernel", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
@@ -289,6 +291,9 @@ class System:
 
     def apply_dirichlet(self):
         self._c(self.ctx.lib.fgpu_sys_apply_dirichlet(self.h), "apply_dirichlet")
+
+    def heat_terms(self, current: "System", gamma: float, lorentz: float = 2.44e-8):
+        self._c(self.ctx.lib.fgpu_sys_heat_terms(self.h, current.h, float(gamma), float(lorentz)), "heat_terms")
 
     def reassemble(self):
         self._c(self.ctx.lib.fgpu_sys_reassemble(self.h), "reassemble")

@@ -52,6 +52,10 @@ class Interpolator:
         """Interpolator::extract_solution(PoissonSolver&, smoothen) (src/Interpolator.cpp:172-190)."""
         self.ctx.check(self.ctx.lib.fgpu_interp_extract(self.h, system.h, int(bool(smoothen))), "fgpu_interp_extract")
 
+    def extract_heat(self, current: fgpu.System, heat: fgpu.System):
+        """Interpolator::extract_solution(CurrentHeatSolver&) (src/Interpolator.cpp:207-216)."""
+        self.ctx.check(self.ctx.lib.fgpu_interp_extract_heat(self.h, current.h, heat.h), "fgpu_interp_extract_heat")
+
     def nodal(self) -> np.ndarray:
         out = np.zeros(5 * self.n_nodes)
         self.ctx.check(self.ctx.lib.fgpu_interp_get_nodal(self.h, out), "fgpu_interp_get_nodal")

@@ -152,3 +152,103 @@ class PoissonSolver(DealSolver):
     def solve(self) -> int:
         """PoissonSolver::solve (include/PoissonSolver.h:54)."""
         return self.solve_cg(self.conf.n_cg, self.conf.cg_tolerance)
+
+
+@dataclass
+class HeatingConfig:
+    """Config::Heating with the reference defaults (src/Config.cpp:73-84)."""
+    t_ambient: float = 300.0
+    lorentz: float = 2.44e-8
+    n_cg: int = 2000
+    cg_tolerance: float = 1e-9
+    ssor_param: float = 1.2
+    delta_time: float = 10.0          # fs
+
+
+CU_RHO_CP = 3.4496e-24                # J/(K A^3), include/CurrentHeatSolver.h:110
+
+
+class _BulkSolver(DealSolver):
+    """A solver on the bulk mesh of a CurrentHeatSolver, sharing its mesh."""
+
+    def __init__(self, owner: "CurrentHeatSolver"):
+        super().__init__(owner.ctx)
+        self.owner = owner
+        self.conf = owner.conf
+
+    def solve(self) -> int:
+        """EmissionSolver::solve (include/CurrentHeatSolver.h:35)."""
+        return self.solve_cg(self.conf.n_cg, self.conf.cg_tolerance)
+
+
+class CurrentSolver(_BulkSolver):
+    """CurrentSolver<3> (src/CurrentHeatSolver.cpp:400-484): sigma = 1, the per-face current
+    densities as Neumann flux on the copper surface, phi = 0 on the copper bottom."""
+
+    def __init__(self, owner):
+        super().__init__(owner)
+        self.bc_values = None            # one current density per copper-surface face
+
+    def assemble(self):
+        s = self.system
+        s.reset()
+        s.add_neumann(BID_COPPER_SURFACE, self.bc_values, 0.0)
+        s.add_dirichlet(BID_COPPER_BOTTOM, self.dirichlet_bc_value)
+        s.apply_dirichlet()
+
+
+class HeatSolver(_BulkSolver):
+    """HeatSolver<3> (src/CurrentHeatSolver.cpp:106-152, 347-393): implicit Euler with kappa and
+    sigma at the previous temperature, Joule heat from the current solution, Nottingham heat
+    per copper-surface face, T = T_ambient on the copper bottom."""
+
+    def __init__(self, owner):
+        super().__init__(owner)
+        self.bc_values = None            # Nottingham heat per copper-surface face
+
+    def assemble(self, delta_time: float):
+        """delta_time in seconds (the caller converts fs, src/ProjectRunaway.cpp:549)."""
+        if not delta_time > 0:
+            raise ValueError(f"Invalid delta time: {delta_time}")
+        s = self.system
+        s.reset()
+        s.heat_terms(self.owner.current.system, CU_RHO_CP / delta_time, self.conf.lorentz)
+        if self.bc_values is not None:
+            s.add_neumann(BID_COPPER_SURFACE, self.bc_values, 0.0)
+        s.add_dirichlet(BID_COPPER_BOTTOM, self.dirichlet_bc_value)
+        s.apply_dirichlet()
+
+
+class CurrentHeatSolver:
+    """CurrentHeatSolver<3> (src/CurrentHeatSolver.cpp:487-538): current and heat solvers on one
+    bulk mesh, marked (copper surface, copper bottom, sides, copper surface)."""
+
+    MARKS = (BID_COPPER_SURFACE, BID_COPPER_BOTTOM, BID_SIDES, BID_COPPER_SURFACE)
+
+    def __init__(self, conf: HeatingConfig | None = None, ctx: fgpu.Context | None = None):
+        self.ctx = ctx or default_context()
+        self.conf = conf or HeatingConfig()
+        self.mesh = None
+        self.current = CurrentSolver(self)
+        self.heat = HeatSolver(self)
+
+    def import_mesh(self, vertices, cells) -> bool:
+        try:
+            self.mesh = fgpu.Mesh(self.ctx, vertices, cells, *self.MARKS)
+        except fgpu.FgpuError:
+            self.mesh = None
+            return False
+        for sub in (self.current, self.heat):
+            sub.mesh = self.mesh
+            sub.system = fgpu.System(self.mesh)
+        return True
+
+    def setup(self, temperature: float):
+        """CurrentHeatSolver::setup (src/CurrentHeatSolver.cpp:510-514)."""
+        self.heat.dirichlet_bc_value = temperature
+        self.current.setup_system()
+        self.heat.setup_system()
+
+    def export_temp_rho(self):
+        """(temperature per dof, current density -grad phi_c per dof) (src/CurrentHeatSolver.cpp:525-532)."""
+        return self.heat.export_solution(), self.current.export_solution_grad()

@@ -197,6 +197,11 @@ void fgpu_interp_destroy(fgpu_interp *interp);
  * (store_elfield), charge density 0 (field_mode laplace), potential}; nodes outside the
  * region get empty_value; smoothen: average_nodal_fields over the pseudo-Voronoi cells */
 int fgpu_interp_extract(fgpu_interp *interp, fgpu_sys *sys, int smoothen);
+/* Interpolator::extract_solution(CurrentHeatSolver&) (src/Interpolator.cpp:207-216): with
+ * the current and heat systems on one mesh imported from these nodes: Solution{current
+ * density as DealSolver::export_solution_grad gives it (src/DealSolver.cpp:281-301), current
+ * potential, temperature}; nodes outside the region get {0, 0, empty_value} */
+int fgpu_interp_extract_heat(fgpu_interp *interp, fgpu_sys *current, fgpu_sys *heat);
 /* the nodal solution (5 * n_nodes), as Interpolator::store_solution / nodes hold it */
 int fgpu_interp_get_nodal(fgpu_interp *interp, double *sol5);
 int fgpu_interp_set_nodal(fgpu_interp *interp, const double *sol5);
@@ -210,6 +215,13 @@ int fgpu_interp_points(fgpu_interp *interp, int n, const double *xyz, int rank, 
  * fgpu_interp_points call) */
 int fgpu_interp_cells(fgpu_interp *interp, int n, const double *xyz, int rank, const int *cells, double *out);
 
+/* HeatSolver::assemble cell terms, implicit Euler (src/CurrentHeatSolver.cpp:116-152, 347-393;
+ * PhysicalQuantities::sigma/kappa src/PhysicalQuantities.cpp:31-64, 173-185): with T_old the
+ * heat system's current solution and phi_c the solution of `current` on the same mesh, sets the
+ * coefficients kappa(T_old) (stiffness) and gamma (mass) at every Gauss point and adds
+ * int phi_i (gamma T_old + sigma(T_old) |grad phi_c|^2) to the right-hand side.  Call after
+ * fgpu_sys_reset; gamma = 3.4496e-24 / dt [J/(K A^3 s)], lorentz [W Ohm/K^2] (2.44e-8). */
+int fgpu_sys_heat_terms(fgpu_sys *heat, const fgpu_sys *current, double gamma, double lorentz);
 /* DealSolver::check_limits / stat (src/DealSolver.cpp:158-167) */
 int fgpu_sys_minmax(fgpu_sys *sys, double *min, double *max);
 /* DealSolver::export_solution_grad (src/DealSolver.cpp:281-301): -grad u at the Gauss point

@@ -82,6 +82,10 @@ int  orc_pcg(const orc_csr *A, const double *b, double *x, int max_it, double to
 /* -grad u at every vertex, taken at the Gauss point "vertex2node" of the last cell touching it
  * (DealSolver::export_solution_grad, src/DealSolver.cpp:281-301). */
 void orc_export_solution_grad(const orc_mesh *m, const double *u, double *grad);
+/* FEValues::get_function_values / get_function_gradients at the QGauss(2) points of every
+ * cell (as the heat and current assemblies read them, src/CurrentHeatSolver.cpp:355-356,
+ * 462): val 8 per cell, grad 24 per cell (point-major, x y z); either may be NULL */
+void orc_qp_fields(const orc_mesh *m, const double *u, double *val, double *grad);
 
 /* ---- interpolation (src/Interpolator.cpp, src/InterpolatorCells.cpp) ------------------ */
 /* Nodal field: mean over the region's hexes of the exact -grad phi at the hex corner

@@ -80,6 +80,7 @@ def _declare(L):
     L.orc_lintet_bcc.argtypes = [C.POINTER(_Lintet), _D, C.c_int, _D]
     L.orc_locate_chain.argtypes = [C.POINTER(_Lintet), C.c_int, _D, C.c_int, _I]
     L.orc_linhex_shape_functions.argtypes = [_D, _D, _D]
+    L.orc_qp_fields.argtypes = [C.POINTER(_Mesh), _D, C.c_void_p, C.c_void_p]
     L.orc_quadtet_cells.argtypes = [C.c_int, _I, _I, _I, _I]
     L.orc_extract_nodal.argtypes = [C.c_int, _D, C.c_int, _I, _I, C.c_int, _U8, _D, C.c_void_p, C.c_double, _D]
     L.orc_decay_factor.argtypes = [_D, C.c_int, _I]
@@ -176,6 +177,13 @@ class Mesh:
         it = lib().orc_pcg(C.byref(self._csr), b, x, max_it, tol, int(relative), kind, omega, C.byref(res))
         return it, res.value
 
+    def qp_fields(self, u):
+        """(values 8 per cell, gradients 8x3 per cell) of a dof vector at the Gauss points."""
+        val = np.zeros(8 * self.n_cells)
+        grad = np.zeros(24 * self.n_cells)
+        lib().orc_qp_fields(C.byref(self._m), np.ascontiguousarray(u, np.float64), val.ctypes.data, grad.ctypes.data)
+        return val.reshape(-1, 8), grad.reshape(-1, 8, 3)
+
     def export_solution_grad(self, u):
         g = np.zeros(3 * self.n_dofs)
         lib().orc_export_solution_grad(C.byref(self._m), np.ascontiguousarray(u, np.float64), g)
@@ -213,6 +221,44 @@ def poisson_laplace(vertices, ucd_cells, applied_field, tol=1e-9, relative=False
     m.apply_dirichlet(b, x, mask, vals)
     it, res = m.pcg(b, x, max_it=max_it, tol=tol, relative=relative, precond=precond, omega=omega)
     return m, x, it, res, b
+
+
+CU_RHO_CP = 3.4496e-24      # volumetric heat capacity of copper [J/(K A^3)], include/CurrentHeatSolver.h:110
+LORENTZ = 2.44e-8           # Lorenz number [W Ohm/K^2], src/Config.cpp:75
+BID_COPPER_BOTTOM = 7
+
+
+def current_solve(vertices, bulk_cells, face_j, tol=1e-12, precond="ssor", max_it=10000):
+    """CurrentSolver::assemble + solve (src/CurrentHeatSolver.cpp:421-484): sigma = 1, Neumann
+    flux face_j on the copper surface (one value per id-2 face), phi = 0 on the bottom (id 7)."""
+    m = Mesh(vertices, bulk_cells, BID_COPPER_SURFACE, BID_COPPER_BOTTOM, BID_SIDES, BID_COPPER_SURFACE)
+    m.assemble()
+    b = m.face_rhs(BID_COPPER_SURFACE, values=face_j)
+    x = np.zeros(m.n_dofs)
+    m.apply_dirichlet(b, x, m.dirichlet_mask(BID_COPPER_BOTTOM), None)
+    it, res = m.pcg(b, x, max_it=max_it, tol=tol, relative=True, precond=precond)
+    return m, x, it
+
+
+def heat_step(m: "Mesh", T_old, phi_c, dt, face_nottingham, T_amb, tol=1e-12, lorentz=LORENTZ, precond="ssor"):
+    """HeatSolver::assemble (implicit Euler, src/CurrentHeatSolver.cpp:116-152, 347-393) and solve
+    on the bulk mesh m: A = gamma M + K_kappa(T_old), b = int phi_i (gamma T_old + sigma(T_old)
+    |grad phi_c|^2) + Nottingham flux on id 2, T = T_amb on id 7; T_old is the initial guess."""
+    gamma = CU_RHO_CP / dt
+    Tq, _ = m.qp_fields(T_old)
+    _, gq = m.qp_fields(phi_c)
+    kq = np.vectorize(lambda t: kappa(t, lorentz))(Tq)
+    sq = np.vectorize(sigma)(Tq)
+    m.assemble(kappa_q=kq.reshape(-1), gamma_q=np.full(Tq.size, gamma))
+    f = gamma * Tq + sq * (gq ** 2).sum(-1)
+    b = m.cell_rhs(f.reshape(-1))
+    m.face_rhs(BID_COPPER_SURFACE, values=face_nottingham, b=b)
+    x = np.ascontiguousarray(T_old, np.float64).copy()
+    mask = m.dirichlet_mask(BID_COPPER_BOTTOM)
+    vals = np.where(mask > 0, T_amb, 0.0)
+    m.apply_dirichlet(b, x, mask, vals)
+    it, res = m.pcg(b, x, tol=tol, relative=True, precond=precond)
+    return x, it
 
 
 def nodal_field(nodes, hexes, hex_marker, region_sign, phi_nodes):

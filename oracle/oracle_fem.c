@@ -517,6 +517,26 @@ int orc_pcg(const orc_csr *A, const double *b, double *x, int max_it, double tol
 /* ------------------------------------------------------------------------------------- */
 /* gradient export                                                                         */
 /* ------------------------------------------------------------------------------------- */
+void orc_qp_fields(const orc_mesh *m, const double *u, double *val, double *grad) {
+    for (int c = 0; c < m->n_cells; ++c) {
+        double xe[8][3];
+        cell_coords(m, c, xe);
+        for (int q = 0; q < 8; ++q) {
+            double phi[8], g[8][3];
+            qpoint(xe, q, phi, g);
+            double v = 0, gr[3] = {0, 0, 0};
+            for (int k = 0; k < 8; ++k) {
+                const double uk = u[m->cells[8 * c + k]];
+                v += uk * phi[k];
+                for (int a = 0; a < 3; ++a) gr[a] += uk * g[k][a];
+            }
+            if (val) val[8 * (size_t)c + q] = v;
+            if (grad)
+                for (int a = 0; a < 3; ++a) grad[24 * (size_t)c + 3 * q + a] = gr[a];
+        }
+    }
+}
+
 void orc_export_solution_grad(const orc_mesh *m, const double *u, double *grad) {
     const int n = m->n_dofs;
     int *v2c = malloc(sizeof(int) * (n > 0 ? n : 1)), *v2n = malloc(sizeof(int) * (n > 0 ? n : 1));

@@ -330,3 +330,18 @@ def test_reference_hemicone_mesh_solves():
     assert x.max() <= 1e-12 and x.min() > -500.0
     top = om.x[:, 2] > 500 - 1e-9
     assert np.all(x[top] < -490.0)
+
+
+def test_qp_fields_of_linear_function():
+    """Values and gradients at the Gauss points: exact for a linear function, points at
+    (1 +- 1/sqrt 3) / 2 of a unit cube, x fastest."""
+    g = (1 - 1 / np.sqrt(3)) / 2
+    x = np.array([[i, j, k] for k in (0, 1) for j in (0, 1) for i in (0, 1)], float)
+    ucd = np.array([[0, 1, 5, 4, 2, 3, 7, 6]], np.int32)       # UCD order of the lexicographic cube
+    m = O.Mesh(x, ucd, 8, 2, 4, 2)
+    a = np.array([0.3, -0.7, 1.1])
+    v, gr = m.qp_fields(m.x @ a + 2.0)
+    t = lambda bit: 1 - g if bit else g
+    pts = np.array([[t(q & 1), t((q >> 1) & 1), t((q >> 2) & 1)] for q in range(8)])
+    assert np.allclose(gr[0], a, atol=1e-14)
+    assert np.allclose(v[0], pts @ a + 2.0, atol=1e-14)
