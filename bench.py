@@ -170,21 +170,25 @@ def main():
 
     # dominant kernel: the FP32 element apply over the whole batch (all clients resident).
     # Algorithmic bytes per launch = compulsory traffic of a matrix-free Q1 apply: cell
-    # connectivity 32 B/hex (8 x int32), coordinates 3 x 4 B/dof, the two input vectors
-    # (z, p) 2 x 4 B/dof and one output 4 B/dof.
+    # connectivity 32 B/hex (8 x int32), the geometry (tet-split form: the 6-float metric,
+    # 24 B/hex; otherwise coordinates 3 x 4 B/dof), the two input vectors (z, p) 2 x 4 B/dof
+    # and one output 4 B/dof.
     b = fgpu.Batch(ctx, srv.opts)
+    affine = True
     for r in srv.resident.values():
+        affine = r.mesh.use_affine(True) and affine
         b.add(r.sys)
     ms = b.time_kernel(0, reps=30)
     H = sum(r.mesh.n_cells for r in srv.resident.values())
     N = sum(r.mesh.n_dofs for r in srv.resident.values())
-    byts = 32 * H + 12 * N + 8 * N + 4 * N
+    byts = 32 * H + (24 * H if affine else 12 * N) + 8 * N + 4 * N
     gbs = byts / (ms * 1e-3) / 1e9
     gather_ms = b.time_kernel(1, reps=30)
     del b
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_patch<float>",
                 "avg_us": round(ms * 1e3, 2), "algorithmic_bytes": byts, "cells": H, "dofs": N,
+                "tet_split_form": affine,
                 "gather_avg_us": round(gather_ms * 1e3, 2)}
     log("roofline", roofline)
 

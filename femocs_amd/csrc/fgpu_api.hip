@@ -101,6 +101,7 @@ fgpu::Mesh::~Mesh() {
     dfree(d_bf_icell);
     dfree(d_bf_face);
     dfree(d_bf_id);
+    dfree(d_metric);
     dfree(motion.idof); dfree(motion.ptr); dfree(motion.ctrl); dfree(motion.w);
     dfree(motion.base); dfree(motion.disp); dfree(motion.prev);
     dfree(d_fgptr);
@@ -544,6 +545,19 @@ extern "C" int fgpu_mesh_displace(fgpu_mesh *m, const double *disp) {
     });
 }
 
+static void ensure_metric(fgpu_mesh *m);
+
+extern "C" int fgpu_mesh_use_affine(fgpu_mesh *m, int allow, int *active) {
+    if (!m) return FGPU_EINVAL;
+    return guarded(m->ctx, [&] {
+        HIPCHECK(hipSetDevice(m->ctx->device));
+        m->affine_allowed = allow != 0;
+        ensure_metric(m);
+        if (active) *active = m->affine && m->affine_allowed;
+        return FGPU_OK;
+    });
+}
+
 extern "C" int fgpu_mesh_set_patch_cells(fgpu_mesh *m, int cells) {
     if (!m || cells < 8 || cells > kPatchCells) return FGPU_EINVAL;
     return guarded(m->ctx, [&] {
@@ -644,10 +658,36 @@ static Desc make_desc64(fgpu_sys *s, bool residual) {
     return d;
 }
 
+// the per-cell metric of the tet-split form for the current coordinates, and whether every
+// cell has that form (nodes within 1e-9 of it, relative to the cell size)
+static void ensure_metric(fgpu_mesh *m) {
+    if (m->metric_gen == m->coord_gen) return;
+    hipStream_t st = m->ctx->stream;
+    if (!m->ctx->affine_table) {
+        HIPCHECK(upload_affine_table(st));
+        m->ctx->affine_table = true;
+    }
+    if (!m->d_metric) m->d_metric = dalloc<float>(6 * (size_t)std::max(1, m->n_cells));
+    const int nb = metric_blocks(m->n_cells);
+    double *dev = dalloc<double>(nb);
+    launch_metric(st, m->n_cells, m->d_cells, m->d_X64, m->d_metric, dev);
+    std::vector<double> h(nb);
+    d2h(h.data(), dev, nb, st);
+    HIPCHECK(hipStreamSynchronize(st));
+    dfree(dev);
+    double mx = 0;
+    for (double v : h) mx = std::max(mx, v);
+    m->affine = m->n_cells > 0 && mx <= 1e-9;
+    m->metric_gen = m->coord_gen;
+}
+
 static Desc make_desc32(fgpu_sys *s) {
     ensure_layout(s);
     Desc d{};
     fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+    ensure_metric(m);
+    d.metric = (m->affine && m->affine_allowed && !s->have_coef) ? m->d_metric : nullptr;
+    s->desc32_affine = d.metric != nullptr;   // sizes the LDS of the launches using it
     d.n_dofs = m->n_dofs;
     d.n_cells = m->n_cells;
     d.nb_cell = s->nb_cell;
@@ -774,8 +814,10 @@ struct BatchTabs {
     size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the element kernel, largest patch of the batch
 };
 
-static size_t patch_lds(int node_cap, int cells, size_t tsize) {
-    return (4 * (size_t)node_cap + 8 * (size_t)cells) * tsize + 16 * (size_t)cells;
+// coordinates and inputs per node (inputs only for the tet-split FP32 form), outputs per
+// cell, entry lists
+static size_t patch_lds(int node_cap, int cells, size_t tsize, bool affine = false) {
+    return ((affine ? 1 : 4) * (size_t)node_cap + 8 * (size_t)cells) * tsize + 16 * (size_t)cells;
 }
 
 static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> &sets, const std::vector<fgpu_sys *> &sys,
@@ -821,7 +863,9 @@ static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> 
     for (auto *x : sys) {
         const fgpu_mesh *m = static_cast<const fgpu_mesh *>(x->mesh);
         t.lds64 = std::max(t.lds64, patch_lds(m->patches.node_cap, m->patches.cells, sizeof(double)));
-        t.lds32 = std::max(t.lds32, patch_lds(m->patches.node_cap, m->patches.cells, sizeof(float)));
+        // the tet-split FP32 form needs no coordinates: as decided by the system's FP32
+        // descriptor, which every caller builds before uploading it
+        t.lds32 = std::max(t.lds32, patch_lds(m->patches.node_cap, m->patches.cells, sizeof(float), x->desc32_affine));
     }
     return t;
 }
@@ -1095,6 +1139,31 @@ extern "C" int fgpu_sys_get_solution(fgpu_sys *s, double *x) {
 extern "C" int fgpu_sys_get_rhs(fgpu_sys *s, double *b) {
     if (!s || !b) return FGPU_EINVAL;
     return get_vec(s, s->b, b);
+}
+
+extern "C" int fgpu_sys_apply32(fgpu_sys *s, const double *x, double *y) {
+    if (!s || !x || !y) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        hipStream_t st = s->ctx->stream;
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        std::vector<double> v;
+        permute_to_internal(m, x, v);
+        std::vector<float> v32(v.begin(), v.end());
+        h2d(s->p32, v32.data(), v32.size(), st);
+        Desc d = make_desc32(s);
+        d.ua = s->p32;
+        d.ub = nullptr;
+        d.q = s->q32;
+        BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
+        launch_elem<float>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds32);
+        launch_gather_plain<float>(st, t.sets[0], t.dtab, t.nd, false);
+        std::vector<float> o(m->n_dofs);
+        d2h(o.data(), s->q32, o.size(), st);
+        HIPCHECK(hipStreamSynchronize(st));
+        for (int i = 0; i < m->n_dofs; ++i) y[m->dof_of_idof[i]] = o[i];
+        return FGPU_OK;
+    });
 }
 
 extern "C" int fgpu_sys_apply(fgpu_sys *s, const double *x, double *y) {

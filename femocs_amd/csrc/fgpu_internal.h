@@ -82,6 +82,7 @@ struct Desc {
     const double *b;              // FP64 rhs (residual mode)
     double *scal;                 // kScal doubles
     double *part;                 // 4 regions of nb_dof per-block partial sums
+    const float *metric;          // FP32 apply of tet-split meshes: 6 per cell, or null
 };
 
 struct Mesh {
@@ -104,6 +105,13 @@ struct Mesh {
     int patch_gen = 0;                     // bumped whenever the patches are rebuilt
     int coord_gen = 0;                     // bumped whenever the vertices move
     unsigned long long uid = 0;            // unique per imported mesh (caches keyed on it)
+    // every cell an affine image of the tet-split reference sub-hex: the FP32 operator reads
+    // a per-cell metric instead of the node coordinates (valid for coordinate generation
+    // metric_gen)
+    float *d_metric = nullptr;             // 6 per cell (internal order)
+    int metric_gen = -1;
+    bool affine = false;
+    bool affine_allowed = true;
     bool x_stale = false;                  // device coordinates moved, host copy `x` not yet
     // linear mesh motion (fgpu_mesh_set_motion): row r (dof idof[r]) sits at base[r] +
     // sum_k w[k] * disp[ctrl[k]] for k in ptr[r]..ptr[r+1]
@@ -160,6 +168,7 @@ struct Sys {
     };
     std::vector<RhsTerm> rhs_terms;
     bool lifted = false;          // fgpu_sys_apply_dirichlet ran since the last reset
+    bool desc32_affine = false;   // the last FP32 descriptor built uses the tet-split form
     void clear_terms();
     ~Sys();
 };
@@ -177,6 +186,7 @@ struct fgpu_ctx {
     size_t flags_cap = 0;
     int refs = 0;                 // live meshes and systems
     bool closing = false;         // destroy requested while children were alive
+    bool affine_table = false;    // reference sub-hex constants uploaded to this device
 };
 
 struct fgpu_mesh : fgpu::Mesh {};

@@ -29,6 +29,9 @@ void launch_face_load(hipStream_t s, int nb, const int *bf_icell, const int *bf_
 void launch_face_gather(hipStream_t s, int n, const int *fgptr, const int *fgidx, const double *fslot, double *b);
 void launch_minmax(hipStream_t s, int n, const double *x, double *out);
 int center_det_blocks(int n);
+hipError_t upload_affine_table(hipStream_t s);
+int metric_blocks(int nc);
+void launch_metric(hipStream_t s, int nc, const int *cells, const double *X, float *metric, double *devmax);
 void launch_heat_terms(hipStream_t s, int nc, const int *cells, const double *X, const double *T, const double *phic,
                        double gamma, double lorentz, double *kq, double *gq, float *kq32, float *gq32, double *f);
 void launch_motion(hipStream_t s, int rows, const int *idof, const int *ptr, const int *ctrl, const double *w,

@@ -2,16 +2,19 @@
 //
 // Q1 hexahedra, QGauss(2) quadrature, trilinear geometry recomputed in registers.  The
 // operator is applied matrix-free in two deterministic steps per product:
-//   k_elem    one thread per cell: gather 8 node coordinates and values, sum-factorised
-//             reference gradients, per-point metric, transposed sum factorisation ->
-//             8 element outputs written to the slot buffer Ye (cell*8 + k)
+//   k_patch   per patch of consecutive cells: node data staged in LDS, one thread per cell
+//             (sum-factorised reference gradients, per-point metric, transposed sum
+//             factorisation), per-node sums in LDS in a fixed order, one store per
+//             (node, patch) into the dof-major slot buffer
 //   k_gather  one thread per dof: add the dof's slots in ascending order, fused with the
-//             CG vector work of that step and a block reduction; the last block of each
-//             system finishes the reduction in a fixed order and updates the scalars.
+//             CG vector work of that step and a block reduction into per-block partials,
+//             which a one-block-per-system scalar kernel sums in block order
 // No floating-point atomics anywhere, so a system gives bit-identical results whatever
 // other systems share the launch.  Every kernel walks a block table (system, local block),
 // which is how one launch serves a batch of meshes of any sizes.
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 #include "fgpu_kernels.h"
 
@@ -223,6 +226,151 @@ enum { EM_APPLY = 0, EM_DIAG = 1, EM_RHS = 2 };
 
 extern __shared__ __attribute__((aligned(16))) unsigned char fg_smem[];
 
+// ---------------------------------------------------------------------------------------
+// tet-split meshes: every hex of the Tethex split of a tet is the image of one reference
+// sub-hex under the affine map of its own frame u1 = 2(x1-x0), u2 = 2(x4-x0), u3 = 2(x2-x0)
+// (lexicographic nodes: vertex, three edge centres, three face centres, tet centre).  With
+// J = A J_ref, w det(J) J^-1 J^-T = P_q^T Mh P_q, P_q = sqrt(|c_q|) J_ref,q^-T, c_q = w det J_ref,q
+// (negative for every q) and Mh = -det(A) (A^T A)^-1: six numbers per cell replace the
+// coordinates, and the per-point metric costs 27 multiply-adds.
+// ---------------------------------------------------------------------------------------
+// lambda coordinates of the reference sub-hex nodes in the frame (u1, u2, u3)
+static const double kSubHex[8][3] = {{0, 0, 0}, {0.5, 0, 0}, {0, 0, 0.5}, {1.0 / 3, 0, 1.0 / 3},
+                                     {0, 0.5, 0}, {1.0 / 3, 1.0 / 3, 0}, {0, 1.0 / 3, 1.0 / 3}, {0.25, 0.25, 0.25}};
+__constant__ float kAffP[8][9];
+__constant__ double kSubHexD[8][3] = {{0, 0, 0}, {0.5, 0, 0}, {0, 0, 0.5}, {1.0 / 3, 0, 1.0 / 3},
+                                      {0, 0.5, 0}, {1.0 / 3, 1.0 / 3, 0}, {0, 1.0 / 3, 1.0 / 3}, {0.25, 0.25, 0.25}};
+
+hipError_t upload_affine_table(hipStream_t s) {
+    const double g[2] = {FG_G0, FG_G1};
+    float P[8][9];
+    for (int q = 0; q < 8; ++q) {
+        const double xi[3] = {g[q & 1], g[(q >> 1) & 1], g[(q >> 2) & 1]};
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int n = 0; n < 8; ++n) {
+            double N[3], dN[3];
+            for (int d = 0; d < 3; ++d) {
+                const int bit = (n >> d) & 1;
+                N[d] = bit ? xi[d] : 1 - xi[d];
+                dN[d] = bit ? 1 : -1;
+            }
+            const double gr[3] = {dN[0] * N[1] * N[2], N[0] * dN[1] * N[2], N[0] * N[1] * dN[2]};
+            for (int i = 0; i < 3; ++i)
+                for (int d = 0; d < 3; ++d) J[i][d] += kSubHex[n][i] * gr[d];
+        }
+        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        // J^-T = cofactor(J) / det
+        double C[3][3];
+        C[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        C[0][1] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        C[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        C[1][0] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+        C[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+        C[1][2] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+        C[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+        C[2][1] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+        C[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        const double sc = std::sqrt(std::fabs(0.125 * det)) / det;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) P[q][3 * i + j] = (float)(C[i][j] * sc);
+    }
+    (void)s;
+    return hipMemcpyToSymbol(HIP_SYMBOL(kAffP), P, sizeof(P), 0, hipMemcpyHostToDevice);
+}
+
+// y = K_e u for a tet-split hex with metric M (m00 m01 m02 m11 m12 m22)
+__device__ __forceinline__ void q1_cell_affine(const float M[6], const float u[8], float y[8]) {
+    float du0[2][2], du1[2][2], du2[2][2];
+    dref<float>(u, 0, du0);
+    dref<float>(u, 1, du1);
+    dref<float>(u, 2, du2);
+    float S0[2][2] = {{0, 0}, {0, 0}}, S1[2][2] = {{0, 0}, {0, 0}}, S2[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int a = q & 1, b = (q >> 1) & 1, c = (q >> 2) & 1;
+        const float g0 = du0[b][c], g1 = du1[a][c], g2 = du2[a][b];
+        const float *P = kAffP[q];
+        const float s0 = P[0] * g0 + P[1] * g1 + P[2] * g2;
+        const float s1 = P[3] * g0 + P[4] * g1 + P[5] * g2;
+        const float s2 = P[6] * g0 + P[7] * g1 + P[8] * g2;
+        const float t0 = M[0] * s0 + M[1] * s1 + M[2] * s2;
+        const float t1 = M[1] * s0 + M[3] * s1 + M[4] * s2;
+        const float t2 = M[2] * s0 + M[4] * s1 + M[5] * s2;
+        S0[b][c] += P[0] * t0 + P[3] * t1 + P[6] * t2;
+        S1[a][c] += P[1] * t0 + P[4] * t1 + P[7] * t2;
+        S2[a][b] += P[2] * t0 + P[5] * t1 + P[8] * t2;
+    }
+#pragma unroll
+    for (int n = 0; n < 8; ++n) y[n] = 0;
+    dref_t<float>(S0, 0, y);
+    dref_t<float>(S1, 1, y);
+    dref_t<float>(S2, 2, y);
+}
+
+// per cell: the metric of the tet-split form and, per block, the largest deviation of the
+// nodes from that form relative to the frame size
+__global__ __launch_bounds__(kBlock) void k_metric(int nc, const int *__restrict__ cells, const double4 *__restrict__ X,
+                                                   float *__restrict__ metric, double *__restrict__ devmax) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double dev = 0;
+    if (c < nc) {
+        double x[8][3];
+        for (int n = 0; n < 8; ++n) {
+            const double4 p = X[cells[8 * (size_t)c + n]];
+            x[n][0] = p.x;
+            x[n][1] = p.y;
+            x[n][2] = p.z;
+        }
+        double u[3][3];
+        for (int a = 0; a < 3; ++a) {
+            u[0][a] = 2 * (x[1][a] - x[0][a]);
+            u[1][a] = 2 * (x[4][a] - x[0][a]);
+            u[2][a] = 2 * (x[2][a] - x[0][a]);
+        }
+        const double size = sqrt(u[0][0] * u[0][0] + u[0][1] * u[0][1] + u[0][2] * u[0][2]) +
+                            sqrt(u[1][0] * u[1][0] + u[1][1] * u[1][1] + u[1][2] * u[1][2]) +
+                            sqrt(u[2][0] * u[2][0] + u[2][1] * u[2][1] + u[2][2] * u[2][2]);
+        for (int n = 0; n < 8; ++n)
+            for (int a = 0; a < 3; ++a) {
+                const double pr = x[0][a] + kSubHexD[n][0] * u[0][a] + kSubHexD[n][1] * u[1][a] + kSubHexD[n][2] * u[2][a];
+                dev = fmax(dev, fabs(pr - x[n][a]) / size);
+            }
+        // rows of adj(A), A = [u1 u2 u3] as columns
+        double r[3][3];
+        for (int i = 0; i < 3; ++i) {
+            const double *p = u[(i + 1) % 3], *q = u[(i + 2) % 3];
+            r[i][0] = p[1] * q[2] - p[2] * q[1];
+            r[i][1] = p[2] * q[0] - p[0] * q[2];
+            r[i][2] = p[0] * q[1] - p[1] * q[0];
+        }
+        const double det = u[0][0] * r[0][0] + u[0][1] * r[0][1] + u[0][2] * r[0][2];
+        const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+        for (int k = 0; k < 6; ++k) {
+            const double *a = r[ij[k][0]], *b = r[ij[k][1]];
+            metric[6 * (size_t)c + k] = (float)(-(a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / det);
+        }
+        if (!(det < 0)) dev = 1e300;     // the reference orientation needs det(A) < 0
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dev = fmax(dev, __shfl_xor(dev, o, 64));
+    __shared__ double s[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = dev;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; ++k) dev = fmax(dev, s[k]);
+        dev = fmax(dev, s[0]);
+        devmax[blockIdx.x] = dev;
+    }
+}
+
+int metric_blocks(int nc) { return (nc + kBlock - 1) / kBlock; }
+void launch_metric(hipStream_t s, int nc, const int *cells, const double *X, float *metric, double *devmax) {
+    if (nc > 0)
+        hipLaunchKernelGGL(k_metric, dim3(metric_blocks(nc)), dim3(kBlock), 0, s, nc, cells,
+                           reinterpret_cast<const double4 *>(X), metric, devmax);
+}
+
 template <class T, bool COEF, bool MASKIN, bool ACTIVE_ONLY, int MODE>
 __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const Desc *__restrict__ D, const int2 *__restrict__ tab, int scal_slot,
                                                      const double *__restrict__ fq) {
@@ -231,9 +379,13 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
     const Desc &d = D[t.x];
     if (ACTIVE_ONLY && d.scal[S_ACTIVE] == 0.0) return;
     using V4 = typename Vec4T<T>::type;
-    // dynamic LDS: node coordinates and inputs (node_cap each), cell outputs, entry lists
+    // FP32 apply of a tet-split mesh: per-cell metric, no coordinates (uniform per workgroup)
+    const bool affine = sizeof(T) == 4 && !COEF && MODE == EM_APPLY && d.metric != nullptr;
+    // dynamic LDS: node coordinates (not for the tet-split form) and inputs (node_cap each),
+    // cell outputs, entry lists
     T *sx = reinterpret_cast<T *>(fg_smem);
-    T *sy = sx + d.node_cap, *sz = sy + d.node_cap, *su = sz + d.node_cap, *sout = su + d.node_cap;
+    T *sy = sx + d.node_cap, *sz = sy + d.node_cap;
+    T *su = affine ? sx : sz + d.node_cap, *sout = su + d.node_cap;
     uint4 *slent = reinterpret_cast<uint4 *>(sout + 8 * d.pcells);
     const V4 *X = reinterpret_cast<const V4 *>(d.X);
     const T *ua = reinterpret_cast<const T *>(d.ua);
@@ -261,10 +413,12 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
                 ee[r] = d.plptr[n0 + p + j + 1];
                 os[r] = d.pslot[n0 + j];
                 const int node = d.pnodes[n0 + j];
-                const V4 x = X[node];
-                sx[j] = x.x;
-                sy[j] = x.y;
-                sz[j] = x.z;
+                if (!affine) {
+                    const V4 x = X[node];
+                    sx[j] = x.x;
+                    sy[j] = x.y;
+                    sz[j] = x.z;
+                }
                 if (MODE == EM_APPLY) {
                     T v = ua[node];
                     if (ub) v += beta * ub[node];
@@ -275,7 +429,24 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
         }
         __syncthreads();
         // phase 2: one cell per thread, from LDS
-        if (tid < nc) {
+        if constexpr (sizeof(T) == 4 && !COEF && MODE == EM_APPLY) {
+            if (affine && tid < nc) {
+                const int c = c0 + tid;
+                const int lid[8] = {(int)(packed.x & 0xffff), (int)(packed.x >> 16), (int)(packed.y & 0xffff),
+                                    (int)(packed.y >> 16), (int)(packed.z & 0xffff), (int)(packed.z >> 16),
+                                    (int)(packed.w & 0xffff), (int)(packed.w >> 16)};
+                const float2 *mp = reinterpret_cast<const float2 *>(d.metric + 6 * (size_t)c);
+                const float2 m0 = mp[0], m1 = mp[1], m2 = mp[2];
+                const float M[6] = {m0.x, m0.y, m1.x, m1.y, m2.x, m2.y};
+                float u[8], y[8];
+#pragma unroll
+                for (int n = 0; n < 8; ++n) u[n] = su[lid[n]];
+                q1_cell_affine(M, u, y);
+#pragma unroll
+                for (int n = 0; n < 8; ++n) sout[8 * tid + n] = y[n];
+            }
+        }
+        if (tid < nc && !affine) {
             const int c = c0 + tid;
             const int lid[8] = {(int)(packed.x & 0xffff), (int)(packed.x >> 16), (int)(packed.y & 0xffff), (int)(packed.y >> 16),
                                 (int)(packed.z & 0xffff), (int)(packed.z >> 16), (int)(packed.w & 0xffff), (int)(packed.w >> 16)};

@@ -182,6 +182,22 @@ void build_patches(int nc, int nd, const std::vector<int> &icells, int cells_per
             }
             ++c;
         }
+        // the patch's nodes by decreasing number of cell entries (stable): the threads of a
+        // wave then sum lists of similar length in the per-node reduction.  Only the thread
+        // assignment changes; each node still adds its entries in ascending order.
+        {
+            const size_t nb = P.node0.back();
+            std::vector<int> deg(np, 0), ord(np), newid(np);
+            for (size_t e = 8 * (size_t)c_start; e < 8 * (size_t)c; ++e) deg[P.lidx[e]]++;
+            for (int j = 0; j < np; ++j) ord[j] = j;
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return deg[a] > deg[b]; });
+            std::vector<int> old(P.nodes.begin() + nb, P.nodes.end());
+            for (int j = 0; j < np; ++j) {
+                newid[ord[j]] = j;
+                P.nodes[nb + j] = old[ord[j]];
+            }
+            for (size_t e = 8 * (size_t)c_start; e < 8 * (size_t)c; ++e) P.lidx[e] = (uint16_t)newid[P.lidx[e]];
+        }
         P.cell0.push_back(c);
         P.node0.push_back((int)P.nodes.size());
         P.node_cap = std::max(P.node_cap, np);

@@ -89,6 +89,8 @@ API = [
     ("fgpu_interp_points", C.c_int, [_P, C.c_int, _D, C.c_int, _I, _D]),
     ("fgpu_interp_cells", C.c_int, [_P, C.c_int, _D, C.c_int, _I, _D]),
     ("fgpu_sys_heat_terms", C.c_int, [_P, _P, C.c_double, C.c_double]),
+    ("fgpu_sys_apply32", C.c_int, [_P, _D, _D]),
+    ("fgpu_mesh_use_affine", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
     ("fgpu_sys_minmax", C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("fgpu_sys_solution_grad", C.c_int, [_P, _D]),
     ("fgpu_sys_time_kernel", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
@@ -232,6 +234,12 @@ class Mesh:
             raise ValueError("3 displacement components per control point")
         self.ctx.check(self.ctx.lib.fgpu_mesh_displace(self.h, d), "fgpu_mesh_displace")
 
+    def use_affine(self, allow: bool = True) -> bool:
+        """Tet-split FP32 operator on or off; returns whether it is in use."""
+        a = C.c_int()
+        self.ctx.check(self.ctx.lib.fgpu_mesh_use_affine(self.h, int(bool(allow)), C.byref(a)), "fgpu_mesh_use_affine")
+        return bool(a.value)
+
     def move_vertices(self, ids, xyz):
         ids = np.ascontiguousarray(ids, np.int32)
         xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
@@ -317,6 +325,11 @@ class System:
     def apply(self, x):
         y = np.zeros(self.mesh.n_dofs)
         self._c(self.ctx.lib.fgpu_sys_apply(self.h, np.ascontiguousarray(x, np.float64), y), "apply")
+        return y
+
+    def apply32(self, x):
+        y = np.zeros(self.mesh.n_dofs)
+        self._c(self.ctx.lib.fgpu_sys_apply32(self.h, np.ascontiguousarray(x, np.float64), y), "apply32")
         return y
 
     def solve(self, opts: SolveOpts) -> SolveStats:

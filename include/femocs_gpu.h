@@ -222,6 +222,14 @@ int fgpu_interp_cells(fgpu_interp *interp, int n, const double *xyz, int rank, c
  * int phi_i (gamma T_old + sigma(T_old) |grad phi_c|^2) to the right-hand side.  Call after
  * fgpu_sys_reset; gamma = 3.4496e-24 / dt [J/(K A^3 s)], lorentz [W Ohm/K^2] (2.44e-8). */
 int fgpu_sys_heat_terms(fgpu_sys *heat, const fgpu_sys *current, double gamma, double lorentz);
+/* y = A x with the FP32 operator of the mixed solver (inputs rounded to FP32, result widened);
+ * for checks, no counterpart in the reference */
+int fgpu_sys_apply32(fgpu_sys *sys, const double *x, double *y);
+/* FP32 operator of tet-split meshes (every hex the Tethex sub-hex of a tet, as mesh import
+ * orients them): a 6-number metric per cell instead of the node coordinates.  Detected per
+ * coordinate generation; allow = 0 forces the general trilinear kernel.  *active: whether the
+ * mesh uses it now.  No counterpart in the reference. */
+int fgpu_mesh_use_affine(fgpu_mesh *mesh, int allow, int *active);
 /* DealSolver::check_limits / stat (src/DealSolver.cpp:158-167) */
 int fgpu_sys_minmax(fgpu_sys *sys, double *min, double *max);
 /* DealSolver::export_solution_grad (src/DealSolver.cpp:281-301): -grad u at the Gauss point

@@ -257,3 +257,57 @@ def test_device_motion_map(ctx, small_mesh):
         s.mesh.displace(d * 1e4)
     xyz2, _ = s.mesh.dofs()
     assert np.array_equal(xyz2, xyz)
+
+
+def test_tet_split_fp32_operator(ctx, small_mesh):
+    """The FP32 operator of tet-split meshes (per-cell metric) is the FP64 operator to FP32
+    precision, before and after a device-side move; mixed solves reach the same solution."""
+    from femocs_amd import serving
+    s = _poisson_system(ctx, small_mesh)
+    gm = s.mesh
+    assert gm.use_affine(True)
+    x = np.random.default_rng(4).normal(size=gm.n_dofs)
+    y64 = s.apply(x)
+    ya = s.apply32(x)
+    assert not gm.use_affine(False)
+    yg = s.apply32(x)
+    ea = np.linalg.norm(ya - y64) / np.linalg.norm(y64)
+    eg = np.linalg.norm(yg - y64) / np.linalg.norm(y64)
+    # FP32 rounding of element sums that cancel: both kernels alike
+    assert ea <= 5e-5 and eg <= 5e-5 and ea <= 3 * eg, (ea, eg)
+    assert gm.use_affine(True)
+    c = serving.Client(0, small_mesh, sigma=0.1, seed=3)
+    gm.set_motion(c.moved_ids, c.W_moved)
+    gm.displace(c.displacements())
+    s.reassemble()
+    assert gm.use_affine(True)
+    assert np.linalg.norm(s.apply32(x) - s.apply(x)) <= 5e-5 * np.linalg.norm(s.apply(x))
+    o = fgpu.opts(tol=1e-10, relative=True, precision=fgpu.FGPU_MIXED)
+    s.fill_solution(0.0)
+    its_a = s.solve(o).iterations
+    xa = s.solution()
+    gm.use_affine(False)
+    s.fill_solution(0.0)
+    its_g = s.solve(o).iterations
+    xg = s.solution()
+    assert its_a > 0 and its_g > 0 and abs(its_a - its_g) <= 0.1 * its_g
+    assert np.linalg.norm(xa - xg) <= 1e-9 * np.linalg.norm(xg)
+
+
+def test_general_hex_mesh_keeps_trilinear_fp32_operator(ctx):
+    """A mesh that is not a tet split (a distorted box of hexes) keeps the general kernel."""
+    n = 6
+    g = np.stack(np.meshgrid(*[np.arange(n + 1.0)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    g += 0.15 * np.sin(g[:, [1, 2, 0]])
+    idx = lambda i, j, k: (i * (n + 1) + j) * (n + 1) + k
+    cells = []
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                lex = [idx(i + a, j + b, k + c) for c in (0, 1) for b in (0, 1) for a in (0, 1)]
+                cells.append([lex[u] for u in (0, 1, 5, 4, 2, 3, 7, 6)])
+    gm = fgpu.Mesh(ctx, g, np.array(cells, np.int32), *MARKS)
+    assert not gm.use_affine(True)
+    s = fgpu.System(gm)
+    x = np.random.default_rng(2).normal(size=gm.n_dofs)
+    assert np.linalg.norm(s.apply32(x) - s.apply(x)) <= 5e-5 * np.linalg.norm(s.apply(x))
