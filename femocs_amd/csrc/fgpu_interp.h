@@ -24,6 +24,21 @@ struct LinTetHost {
 void build_lintet(LinTetHost &L, int n_nodes, const double *xyz, int nt, const int *tets, const int *tet_nbr,
                   const int *tet_marker, int search_region);
 
+// Linear-triangle tables of the surface faces as LinearTriangles::precompute builds them
+// (src/InterpolatorCells.cpp:1585-1637, norms and centroids of TetgenFaces::calc_appendices,
+// src/TetgenCells.cpp:387-409): per face 16 doubles {vert0, edge1/det, edge2, pvec/det, norm,
+// |edge2|}, centroid, neighbour lists (for each face vertex, the other faces touching it in
+// increasing order, repeats kept) and the two tets of the face.
+struct LinTriHost {
+    int n = 0;
+    std::vector<double> tab, centroid;
+    std::vector<int> nbr_ptr, nbr, tri2tet;
+    bool point_in_cell(const double p[3], int f) const;
+    double fast_distance(const double p[3], int f) const;
+};
+
+void build_lintri(LinTriHost &T, int n_nodes, const double *xyz, int nf, const int *tris, const int *tri2tet);
+
 // Linear BVH over the tets (Karras 2012 radix tree on 63-bit Morton codes of the centroids).
 // Child c >= 0: internal node c; c < 0: leaf ~c.  Boxes are float, rounded outwards and
 // widened by a margin, so a point the exact predicate accepts is never pruned.
@@ -45,5 +60,14 @@ void build_quadtet(int nt, const int *tets, const int *hexes, const int *node_ki
 // Chain of SolutionReader::calc_full_interpolation resolved on the host from the per-point
 // candidates found on the device (see fgpu_interp.hip): tet per point (negative outside).
 void resolve_chain(const LinTetHost &L, int n, const double *pts, const int4 *cand, int rank, int *tet);
+
+// The same chain over the surface faces (InterpolatorCells<3>::locate_cell, every face searchable).
+void resolve_tri_chain(const LinTriHost &T, int n, const double *pts, const int4 *cand, int *tri);
+
+// LinearTetrahedra::locate_cell(p, guess) with the point's device candidates for the full scan.
+int locate_tet(const LinTetHost &L, const double p[3], int guess, int4 cand);
+
+// The tet LinearTriangles::interp_solution interpolates in (src/InterpolatorCells.cpp:1662-1686).
+int surface_tet(const LinTriHost &T, const LinTetHost &L, const double p[3], int tri, int4 tet_cand);
 
 }  // namespace fgpu

@@ -36,9 +36,16 @@ struct fgpu_interp {
     unsigned long long map_mesh = 0;        // uid of that mesh
     // point buffers (grown on demand)
     double *pts = nullptr, *out = nullptr;
-    int4 *cand = nullptr;
+    int4 *cand = nullptr, *cand2 = nullptr;
     int *tet = nullptr, *cells = nullptr;
     size_t cap = 0;
+    // surface faces (LinearTriangles), set by fgpu_interp_set_triangles
+    std::vector<double> hxyz;
+    LinTriHost T;
+    Bvh TB;
+    double *tri_tab = nullptr, *tri_cen = nullptr;
+    int *tri_child = nullptr, *tri_prim = nullptr;
+    float *tri_box = nullptr, *tri_cbox = nullptr, *tri_leafbox = nullptr;
 };
 
 namespace {
@@ -130,14 +137,40 @@ __device__ inline double box_d2(const float *b, const double *p) {
 // ------------------------------------------------------------------------------------------
 // kernels
 // ------------------------------------------------------------------------------------------
-// Per point: the lowest searchable tet containing it (x, -1 if none), the number of tets of
-// any marker containing it (y), the tet of the nearest centroid, lowest index among equal
+// Cell families the point search runs over: linear tets (searchable by the marker of the
+// search region) and the surface faces (all searchable).
+struct TetCells {
+    const double *det0, *det;
+    const int *marker;
+    __device__ bool contains(const double *p, int t) const { return pic(det0, det, p, t); }
+    __device__ bool searchable(int t) const { return marker[t] == 0; }
+};
+
+// LinearTriangles::point_in_cell (src/InterpolatorCells.cpp:1639-1650) on the 16-double rows
+struct TriCells {
+    const double *tab;
+    __device__ bool contains(const double *p, int f) const {
+        const double *t = tab + 16 * (size_t)f;
+        const D3 tv = sub(mk(p[0], p[1], p[2]), mk(t[0], t[1], t[2]));
+        const double u = tv.x * t[9] + tv.y * t[10] + tv.z * t[11];
+        if (u < -kZero || u > 1 + kZero) return false;
+        const D3 e1 = mk(t[3], t[4], t[5]);
+        const D3 q = mk(tv.y * e1.z - tv.z * e1.y, tv.z * e1.x - tv.x * e1.z, tv.x * e1.y - tv.y * e1.x);
+        const double v = q.x * t[12] + q.y * t[13] + q.z * t[14];
+        if (v < -kZero || u + v > 1 + kZero) return false;
+        return fabs(q.x * t[6] + q.y * t[7] + q.z * t[8]) < t[15];
+    }
+    __device__ bool searchable(int) const { return true; }
+};
+
+// Per point: the lowest searchable cell containing it (x, -1 if none), the number of cells of
+// any marker containing it (y), the cell of the nearest centroid, lowest index among equal
 // distances (z, only when x < 0) and a traversal-overflow flag (w).
+template <class Cells>
 __global__ __launch_bounds__(256) void k_candidates(int n, const double *__restrict__ pts, int n_leaves,
                                                     const int *__restrict__ child, const float *__restrict__ box,
                                                     const float *__restrict__ cbox, const float *__restrict__ leafbox,
-                                                    const int *__restrict__ prim, const double *__restrict__ det0,
-                                                    const double *__restrict__ det, const int *__restrict__ marker,
+                                                    const int *__restrict__ prim, Cells cells,
                                                     const double *__restrict__ cen, int4 *__restrict__ cand) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -148,9 +181,9 @@ __global__ __launch_bounds__(256) void k_candidates(int n, const double *__restr
     auto leaf_hit = [&](int leaf) {
         if (!in_box(leafbox + 6 * (size_t)leaf, p)) return;
         const int t = prim[leaf];
-        if (pic(det0, det, p, t)) {
+        if (cells.contains(p, t)) {
             ++cnt;
-            if (marker[t] == 0 && t < t0) t0 = t;
+            if (cells.searchable(t) && t < t0) t0 = t;
         }
     };
     if (n_leaves == 1) {
@@ -455,15 +488,25 @@ void ensure_points(fgpu_interp *ip, size_t n) {
     release(ip->pts);
     release(ip->out);
     release(ip->cand);
+    release(ip->cand2);
     release(ip->tet);
     release(ip->cells);
     const size_t c = std::max<size_t>(n, 1024);
+    IPCHECK(hipMalloc(&ip->cand2, c * sizeof(int4)));
     IPCHECK(hipMalloc(&ip->pts, 3 * c * sizeof(double)));
     IPCHECK(hipMalloc(&ip->out, 5 * c * sizeof(double)));
     IPCHECK(hipMalloc(&ip->cand, c * sizeof(int4)));
     IPCHECK(hipMalloc(&ip->tet, c * sizeof(int)));
     IPCHECK(hipMalloc(&ip->cells, c * sizeof(int)));
     ip->cap = c;
+}
+
+void free_triangles(fgpu_interp *ip) {
+    for (double **p : {&ip->tri_tab, &ip->tri_cen}) release(*p);
+    for (int **p : {&ip->tri_child, &ip->tri_prim}) release(*p);
+    for (float **p : {&ip->tri_box, &ip->tri_cbox, &ip->tri_leafbox}) release(*p);
+    ip->T = LinTriHost{};
+    ip->TB = Bvh{};
 }
 
 void free_interp(fgpu_interp *ip) {
@@ -473,6 +516,14 @@ void free_interp(fgpu_interp *ip) {
         release(*p);
     for (float **p : {&ip->box, &ip->cbox, &ip->leafbox}) release(*p);
     release(ip->cand);
+    release(ip->cand2);
+    free_triangles(ip);
+}
+
+void tet_candidates(fgpu_interp *ip, int n, int4 *cand) {
+    hipLaunchKernelGGL(k_candidates<TetCells>, dim3(grid(n)), dim3(256), 0, ip->ctx->stream, n, ip->pts,
+                       ip->B.n_leaves, ip->child, ip->box, ip->cbox, ip->leafbox, ip->prim,
+                       TetCells{ip->det0, ip->det, ip->marker}, ip->cen, cand);
 }
 
 }  // namespace
@@ -507,6 +558,7 @@ extern "C" int fgpu_interp_create(fgpu_ctx *c, int n_nodes, const double *xyz, c
         for (size_t i = 0; i < 4 * (size_t)n_tets; ++i) tn = std::max(tn, tets[i] + 1);
         ip->n_tetnodes = tn;
         build_lintet(ip->L, n_nodes, xyz, n_tets, tets, tet_nbr, tet_marker, search_region);
+        ip->hxyz.assign(xyz, xyz + 3 * (size_t)n_nodes);
         // LinearTetrahedra::precompute: decay factor of the smoothing, -1 / longest tet edge
         double emax = -1e308;
         for (int t = 0; t < n_tets; ++t) {
@@ -724,8 +776,7 @@ static int interp_run(fgpu_interp *ip, int n, const double *xyz, int rank, int *
         ensure_points(ip, n);
         IPCHECK(hipMemcpyAsync(ip->pts, xyz, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
         if (locate) {
-            hipLaunchKernelGGL(k_candidates, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, ip->B.n_leaves, ip->child,
-                               ip->box, ip->cbox, ip->leafbox, ip->prim, ip->det0, ip->det, ip->marker, ip->cen, ip->cand);
+            tet_candidates(ip, n, ip->cand);
             IPCHECK(hipGetLastError());
             std::vector<int4> cand(n);
             IPCHECK(hipMemcpyAsync(cand.data(), ip->cand, n * sizeof(int4), hipMemcpyDeviceToHost, st));
@@ -762,4 +813,101 @@ extern "C" int fgpu_interp_points(fgpu_interp *ip, int n, const double *xyz, int
 
 extern "C" int fgpu_interp_cells(fgpu_interp *ip, int n, const double *xyz, int rank, const int *cells, double *out) {
     return interp_run(ip, n, xyz, rank, const_cast<int *>(cells), out, false);
+}
+
+// ------------------------------------------------------------------------------------------
+// surface faces (dim 2)
+// ------------------------------------------------------------------------------------------
+extern "C" int fgpu_interp_set_triangles(fgpu_interp *ip, int n_tris, const int *tris, const int *tri2tet) {
+    if (!ip || n_tris < 0 || (n_tris > 0 && (!tris || !tri2tet))) return FGPU_EINVAL;
+    return guard(ip, [&] {
+        for (size_t i = 0; i < 3 * (size_t)n_tris; ++i)
+            if (tris[i] < 0 || tris[i] >= ip->n_nodes) throw std::invalid_argument("triangle node out of range");
+        for (size_t i = 0; i < 2 * (size_t)n_tris; ++i)
+            if (tri2tet[i] < 0 || tri2tet[i] >= ip->n_tets)
+                throw std::invalid_argument("every triangle needs two tets of the mesh");
+        IPCHECK(hipSetDevice(ip->ctx->device));
+        hipStream_t s = ip->ctx->stream;
+        IPCHECK(hipStreamSynchronize(s));
+        free_triangles(ip);
+        if (n_tris == 0) return FGPU_OK;
+        build_lintri(ip->T, ip->n_nodes, ip->hxyz.data(), n_tris, tris, tri2tet);
+        // search boxes: the face widened by |edge2| on every side contains the prism of
+        // points within that distance of the face plane that point_in_cell accepts
+        std::vector<double> lo(3 * (size_t)n_tris), hi(3 * (size_t)n_tris);
+        for (int f = 0; f < n_tris; ++f) {
+            const double m = ip->T.tab[16 * (size_t)f + 15];
+            for (int a = 0; a < 3; ++a) {
+                double l = 1e300, h = -1e300;
+                for (int k = 0; k < 3; ++k) {
+                    const double x = ip->hxyz[3 * (size_t)tris[3 * (size_t)f + k] + a];
+                    l = std::min(l, x);
+                    h = std::max(h, x);
+                }
+                lo[3 * (size_t)f + a] = l - m;
+                hi[3 * (size_t)f + a] = h + m;
+            }
+        }
+        build_bvh(n_tris, lo, hi, ip->T.centroid, ip->TB);
+        ip->tri_tab = upload(ip->T.tab.data(), ip->T.tab.size(), s);
+        ip->tri_cen = upload(ip->T.centroid.data(), ip->T.centroid.size(), s);
+        ip->tri_child = upload(ip->TB.child.data(), ip->TB.child.size(), s);
+        ip->tri_prim = upload(ip->TB.prim.data(), ip->TB.prim.size(), s);
+        ip->tri_box = upload(ip->TB.box.data(), ip->TB.box.size(), s);
+        ip->tri_cbox = upload(ip->TB.cbox.data(), ip->TB.cbox.size(), s);
+        ip->tri_leafbox = upload(ip->TB.leafbox.data(), ip->TB.leafbox.size(), s);
+        IPCHECK(hipStreamSynchronize(s));
+        return FGPU_OK;
+    });
+}
+
+// SolutionReader::calc_full_interpolation / calc_interpolation for dim 2, rank 1
+// (src/SolutionReader.cpp:43-65, 91-101, 136-163): faces located with the chained guess,
+// values interpolated in the tet LinearTriangles::interp_solution picks
+static int surface_run(fgpu_interp *ip, int n, const double *xyz, int *cells, double *out, bool locate) {
+    if (!ip || n < 0 || (n > 0 && (!xyz || !cells || !out))) return FGPU_EINVAL;
+    if (n == 0) return FGPU_OK;
+    if (ip->T.n == 0) {
+        if (ip->ctx) ip->ctx->err = "no surface triangles set";
+        return FGPU_EINVAL;
+    }
+    return guard(ip, [&] {
+        if (!locate)
+            for (int i = 0; i < n; ++i)
+                if (cells[i] <= -ip->T.n || cells[i] >= ip->T.n) throw std::invalid_argument("cell index out of range");
+        IPCHECK(hipSetDevice(ip->ctx->device));
+        hipStream_t st = ip->ctx->stream;
+        ensure_points(ip, n);
+        IPCHECK(hipMemcpyAsync(ip->pts, xyz, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+        tet_candidates(ip, n, ip->cand);
+        if (locate)
+            hipLaunchKernelGGL(k_candidates<TriCells>, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, ip->TB.n_leaves,
+                               ip->tri_child, ip->tri_box, ip->tri_cbox, ip->tri_leafbox, ip->tri_prim,
+                               TriCells{ip->tri_tab}, ip->tri_cen, ip->cand2);
+        IPCHECK(hipGetLastError());
+        std::vector<int4> tc(n), fc(locate ? n : 0);
+        IPCHECK(hipMemcpyAsync(tc.data(), ip->cand, n * sizeof(int4), hipMemcpyDeviceToHost, st));
+        if (locate) IPCHECK(hipMemcpyAsync(fc.data(), ip->cand2, n * sizeof(int4), hipMemcpyDeviceToHost, st));
+        IPCHECK(hipStreamSynchronize(st));
+        for (int i = 0; i < n; ++i)
+            if (tc[i].w || (locate && fc[i].w)) throw std::runtime_error("point search stack overflow");
+        if (locate) resolve_tri_chain(ip->T, n, xyz, fc.data(), cells);
+        std::vector<int> tet(n);
+        for (int i = 0; i < n; ++i) tet[i] = surface_tet(ip->T, ip->L, xyz + 3 * (size_t)i, std::abs(cells[i]), tc[i]);
+        IPCHECK(hipMemcpyAsync(ip->tet, tet.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_interp, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, 1, 0, ip->tet, ip->det0, ip->det,
+                           ip->tets, ip->c10, ip->hexes, ip->xyz, ip->sol, ip->cells, ip->out);
+        IPCHECK(hipGetLastError());
+        IPCHECK(hipMemcpyAsync(out, ip->out, 5 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        IPCHECK(hipStreamSynchronize(st));
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_interp_surface_points(fgpu_interp *ip, int n, const double *xyz, int *cells, double *out) {
+    return surface_run(ip, n, xyz, cells, out, true);
+}
+
+extern "C" int fgpu_interp_surface_cells(fgpu_interp *ip, int n, const double *xyz, const int *cells, double *out) {
+    return surface_run(ip, n, xyz, const_cast<int *>(cells), out, false);
 }

@@ -123,6 +123,82 @@ bool LinTetHost::point_in_cell(const double p[3], int t) const {
 }
 
 // ------------------------------------------------------------------------------------------
+// linear triangles
+// ------------------------------------------------------------------------------------------
+namespace {
+inline V3 vsub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 vscl(V3 a, double r) { return {a.x * r, a.y * r, a.z * r}; }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline void put(double *d, V3 a) {
+    d[0] = a.x;
+    d[1] = a.y;
+    d[2] = a.z;
+}
+inline V3 at(const double *d) { return {d[0], d[1], d[2]}; }
+}  // namespace
+
+void build_lintri(LinTriHost &T, int n_nodes, const double *xyz, int nf, const int *tris, const int *tri2tet) {
+    T = LinTriHost{};
+    T.n = nf;
+    T.tab.assign(16 * (size_t)nf, 0.0);
+    T.centroid.assign(3 * (size_t)nf, 0.0);
+    T.tri2tet.assign(tri2tet, tri2tet + 2 * (size_t)nf);
+    std::vector<int> ptr(n_nodes + 1, 0), n2f(3 * (size_t)nf);
+    for (size_t i = 0; i < 3 * (size_t)nf; ++i) ptr[tris[i] + 1]++;
+    for (int i = 0; i < n_nodes; ++i) ptr[i + 1] += ptr[i];
+    {
+        std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+        for (int f = 0; f < nf; ++f)
+            for (int k = 0; k < 3; ++k) n2f[pos[tris[3 * (size_t)f + k]]++] = f;
+    }
+    T.nbr_ptr.assign(nf + 1, 0);
+    for (int f = 0; f < nf; ++f) {
+        for (int k = 0; k < 3; ++k) {
+            const int node = tris[3 * (size_t)f + k];
+            for (int e = ptr[node]; e < ptr[node + 1]; ++e)
+                if (n2f[e] != f) T.nbr.push_back(n2f[e]);
+        }
+        T.nbr_ptr[f + 1] = (int)T.nbr.size();
+    }
+    for (int f = 0; f < nf; ++f) {
+        const V3 a = ld(xyz, tris[3 * (size_t)f]), b = ld(xyz, tris[3 * (size_t)f + 1]), c = ld(xyz, tris[3 * (size_t)f + 2]);
+        const V3 e1 = vsub(b, a), e2 = vsub(c, a);
+        V3 nrm = cross(e1, e2);
+        const double len = std::sqrt(nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z);
+        if (len > 0) nrm = vscl(nrm, 1 / len);
+        const V3 pv = cross(nrm, e2);
+        const double idet = 1.0 / dot(e1, pv);
+        double *t = &T.tab[16 * (size_t)f];
+        put(t, a);
+        put(t + 3, vscl(e1, idet));
+        put(t + 6, e2);
+        put(t + 9, vscl(pv, idet));
+        put(t + 12, nrm);
+        t[15] = std::sqrt(e2.x * e2.x + e2.y * e2.y + e2.z * e2.z);
+        const V3 s = {a.x + b.x + c.x, a.y + b.y + c.y, a.z + b.z + c.z};
+        put(&T.centroid[3 * (size_t)f], {s.x / 3.0, s.y / 3.0, s.z / 3.0});
+    }
+}
+
+bool LinTriHost::point_in_cell(const double p[3], int f) const {
+    const double *t = &tab[16 * (size_t)f];
+    const V3 tv = vsub({p[0], p[1], p[2]}, at(t));
+    const double u = dot(tv, at(t + 9));
+    if (u < -kZero || u > 1 + kZero) return false;
+    const V3 q = cross(tv, at(t + 3));
+    const double v = dot(q, at(t + 12));
+    if (v < -kZero || u + v > 1 + kZero) return false;
+    return std::fabs(dot(q, at(t + 6))) < t[15];
+}
+
+double LinTriHost::fast_distance(const double p[3], int f) const {
+    const double *t = &tab[16 * (size_t)f];
+    const V3 tv = vsub({p[0], p[1], p[2]}, at(t));
+    return dot(at(t + 6), cross(tv, at(t + 3)));
+}
+
+// ------------------------------------------------------------------------------------------
 // linear BVH
 // ------------------------------------------------------------------------------------------
 namespace {
@@ -320,6 +396,53 @@ void resolve_chain(const LinTetHost &L, int n, const double *pts, const int4 *ca
         // rank 3: |hex| / 4 == |tet| for the next guess (hex = +-(4 |tet| + k), k < 4)
         prev = rank == 3 ? (r < 0 ? -(4 * -r) : 4 * r) : r;
     }
+}
+
+void resolve_tri_chain(const LinTriHost &T, int n, const double *pts, const int4 *cand, int *tri) {
+    int prev = -1;
+    for (int i = 0; i < n; ++i) {
+        const double *p = pts + 3 * (size_t)i;
+        const int4 c = cand[i];
+        const int g = std::abs(prev);
+        int r;
+        if (c.y == 0) {
+            r = -c.z;
+        } else if (c.y == 1) {
+            r = c.x;
+        } else {
+            r = INT_MIN;
+            if (g < T.n) {
+                if (T.point_in_cell(p, g)) {
+                    r = g;
+                } else {
+                    for (int e = T.nbr_ptr[g]; e < T.nbr_ptr[g + 1]; ++e)
+                        if (T.point_in_cell(p, T.nbr[e])) {
+                            r = T.nbr[e];
+                            break;
+                        }
+                }
+            }
+            if (r == INT_MIN) r = c.x;
+        }
+        tri[i] = r;
+        prev = r;
+    }
+}
+
+int locate_tet(const LinTetHost &L, const double p[3], int g, int4 c) {
+    if (g >= 0 && g < L.n) {
+        if (L.point_in_cell(p, g)) return g;
+        for (int e = L.nbr_ptr[g]; e < L.nbr_ptr[g + 1]; ++e)
+            if (L.point_in_cell(p, L.nbr[e])) return L.nbr[e];
+    }
+    return c.x >= 0 ? c.x : -c.z;
+}
+
+int surface_tet(const LinTriHost &T, const LinTetHost &L, const double p[3], int tri, int4 tet_cand) {
+    const int t0 = T.tri2tet[2 * (size_t)tri], t1 = T.tri2tet[2 * (size_t)tri + 1];
+    if (std::fabs(T.fast_distance(p, tri)) <= 100.0 * kZero) return t0;
+    if (L.point_in_cell(p, t0)) return t0;
+    return locate_tet(L, p, t1, tet_cand);
 }
 
 }  // namespace fgpu

@@ -1,5 +1,5 @@
 """Field and temperature transfer to atoms on the GPU, mirroring the reference's Interpolator
-(src/Interpolator.cpp) and SolutionReader (src/SolutionReader.cpp) for dim 3.
+(src/Interpolator.cpp) and SolutionReader (src/SolutionReader.cpp) for dim 3 and, on the surface faces, dim 2 rank 1.
 
     ip = Interpolator(ctx, tip, search_region=TYPES_VACUUM)
     ip.extract_solution(poisson_system)            # nodal (E, rho, phi), smoothed
@@ -84,23 +84,56 @@ class Interpolator:
         return out.reshape(-1, 5)
 
 
+    def set_triangles(self, tris, tri2tet):
+        """LinearTriangles on the surface faces (3 nodes and 2 tets per face)."""
+        t = np.ascontiguousarray(tris, np.int32).reshape(-1)
+        t2 = np.ascontiguousarray(tri2tet, np.int32).reshape(-1)
+        if 2 * t.size != 3 * t2.size:
+            raise ValueError("3 nodes and 2 tets per face")
+        self.ctx.check(self.ctx.lib.fgpu_interp_set_triangles(self.h, t.size // 3, t, t2), "fgpu_interp_set_triangles")
+
+    def surface_points(self, xyz):
+        pts = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        n = pts.size // 3
+        cells = np.zeros(n, np.int32)
+        out = np.zeros(5 * n)
+        self.ctx.check(self.ctx.lib.fgpu_interp_surface_points(self.h, n, pts, cells, out), "fgpu_interp_surface_points")
+        return cells, out.reshape(-1, 5)
+
+    def surface_cells(self, xyz, cells):
+        pts = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        n = pts.size // 3
+        c = np.ascontiguousarray(cells, np.int32)
+        out = np.zeros(5 * n)
+        self.ctx.check(self.ctx.lib.fgpu_interp_surface_cells(self.h, n, pts, c, out), "fgpu_interp_surface_cells")
+        return out.reshape(-1, 5)
+
+
 class SolutionReader:
     """SolutionReader for dim 3 without atom sorting (src/SolutionReader.cpp:43-190): the first
     call maps the atoms to cells (markers), later calls interpolate in the mapped cells."""
 
-    def __init__(self, interpolator: Interpolator, rank: int = 1):
+    def __init__(self, interpolator: Interpolator, rank: int = 1, dim: int = 3):
         if rank not in (1, 2, 3):
             raise ValueError("rank must be 1, 2 or 3")
+        if dim not in (2, 3) or (dim == 2 and rank != 1):
+            raise ValueError("dim 3 with ranks 1-3, or dim 2 with rank 1")
         self.interpolator = interpolator
         self.rank = rank
+        self.dim = dim
         self.markers = None
 
     def calc_full_interpolation(self, xyz) -> np.ndarray:
-        self.markers, sol = self.interpolator.points(xyz, self.rank)
+        if self.dim == 2:
+            self.markers, sol = self.interpolator.surface_points(xyz)
+        else:
+            self.markers, sol = self.interpolator.points(xyz, self.rank)
         return sol
 
     def calc_interpolation(self, xyz) -> np.ndarray:
         n = np.asarray(xyz).reshape(-1, 3).shape[0]
         if self.markers is None or len(self.markers) != n:
             return self.calc_full_interpolation(xyz)
+        if self.dim == 2:
+            return self.interpolator.surface_cells(xyz, self.markers)
         return self.interpolator.in_cells(xyz, self.rank, self.markers)

@@ -261,6 +261,19 @@ class TipMesh:
         """UCD cells of the bulk hexes (marker < 0), src/TetgenCells.cpp:688-701."""
         return self.hexes[self.hex_marker < 0]
 
+    def surface_triangles(self):
+        """Faces between a vacuum and a bulk tet (the surface triangles the mesher exports):
+        (triangles (n, 3) as the vacuum tet orders the face, tri2tet (n, 2) = (vacuum tet,
+        bulk tet)), in vacuum-tet order."""
+        tets, nbr, mk = self.tets, self.tet_nbr, self.tet_marker
+        opp = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]])
+        t, k = np.nonzero((mk[:, None] == TYPES_VACUUM) & (nbr >= 0))
+        other = nbr[t, k]
+        sel = mk[other] != TYPES_VACUUM
+        t, k, other = t[sel], k[sel], other[sel]
+        tris = tets[t[:, None], opp[k]]
+        return tris.astype(np.int32), np.stack([t, other], 1).astype(np.int32)
+
     @property
     def n_tets(self) -> int:
         return len(self.tets)

@@ -214,6 +214,19 @@ int fgpu_interp_points(fgpu_interp *interp, int n, const double *xyz, int rank, 
 /* SolutionReader::calc_interpolation with the atoms already mapped (cells from a previous
  * fgpu_interp_points call) */
 int fgpu_interp_cells(fgpu_interp *interp, int n, const double *xyz, int rank, const int *cells, double *out);
+/* Surface faces for dim 2 interpolation: LinearTriangles::precompute on the faces between the
+ * vacuum and the bulk (src/InterpolatorCells.cpp:1585-1637; faces and their two tets as
+ * TetgenFaces holds them, src/TetgenCells.cpp:387-409).  tris: 3 nodes per face; tri2tet: the
+ * face's 2 tets in TetgenFaces::to_tets order (the first is tried first), both tets of the
+ * mesh.  n_tris = 0 drops the faces. */
+int fgpu_interp_set_triangles(fgpu_interp *interp, int n_tris, const int *tris, const int *tri2tet);
+/* SolutionReader::calc_full_interpolation, dim 2, rank 1 (src/SolutionReader.cpp:43-65,
+ * 136-163; LinearTriangles::interp_solution, src/InterpolatorCells.cpp:1662-1686): each point
+ * located among the faces with the chained guess (negative: nearest centroid), then
+ * interpolated in the face's tet. */
+int fgpu_interp_surface_points(fgpu_interp *interp, int n, const double *xyz, int *cells, double *out);
+/* SolutionReader::calc_interpolation, dim 2, rank 1, atoms already mapped to faces */
+int fgpu_interp_surface_cells(fgpu_interp *interp, int n, const double *xyz, const int *cells, double *out);
 
 /* HeatSolver::assemble cell terms, implicit Euler (src/CurrentHeatSolver.cpp:116-152, 347-393;
  * PhysicalQuantities::sigma/kappa src/PhysicalQuantities.cpp:31-64, 173-185): with T_old the

@@ -159,6 +159,27 @@ void orc_interp_cells(const orc_lintet *L, const double *xyz, const int *tets, c
                       const int *hexes, const double *sol5, int n_points, const double *pts, int rank,
                       const int *cells, double *out);
 
+/* LinearTriangles (src/InterpolatorCells.cpp:1585-1747): surface triangles of the tet mesh,
+ * tri2tet the two tets of each triangle (TetgenFaces::to_tets) */
+typedef struct {
+    int n;
+    double *v0, *e1, *e2, *pv, *nrm, *maxd, *cen;   /* 3 per triangle (maxd 1) */
+    int *nbr_ptr, *nbr, *tri2tet;
+} orc_lintri;
+int  orc_lintri_build(orc_lintri *T, int n_nodes, const double *xyz, int n_tris, const int *tris, const int *tri2tet);
+void orc_lintri_free(orc_lintri *T);
+int  orc_lintri_point_in_cell(const orc_lintri *T, const double p[3], int tri);
+void orc_lintri_shape_functions(const orc_lintri *T, const double p[3], int tri, double w[3]);
+double orc_lintri_fast_distance(const orc_lintri *T, const double p[3], int tri);
+int  orc_lintri_locate(const orc_lintri *T, const double p[3], int guess);
+/* SolutionReader::calc_full_interpolation, dim 2 rank 1: chained triangle locate, then
+ * LinearTriangles::interp_solution (the point is interpolated in a tet: tets[0] when on the
+ * triangle's plane or inside tets[0], else LinearTetrahedra::locate_cell from tets[1]) */
+void orc_interp_surface(const orc_lintri *T, const orc_lintet *L, const int *tets, const double *sol5, int n_points,
+                        const double *pts, int *cells, double *out);
+void orc_interp_surface_cells(const orc_lintri *T, const orc_lintet *L, const int *tets, const double *sol5,
+                              int n_points, const double *pts, const int *cells, double *out);
+
 /* physical quantities: resistivity table, sigma, kappa (src/PhysicalQuantities.cpp:31-64,
  * 173-185; table src/PhysicalQuantityData.cpp:47-76) */
 double orc_resistivity(double T);

@@ -56,6 +56,11 @@ _I = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _U8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
+class _Lintri(C.Structure):
+    _fields_ = [("n", C.c_int)] + [(k, C.POINTER(C.c_double)) for k in ("v0", "e1", "e2", "pv", "nrm", "maxd", "cen")] + \
+               [(k, C.POINTER(C.c_int)) for k in ("nbr_ptr", "nbr", "tri2tet")]
+
+
 def _declare(L):
     L.orc_mesh_import.argtypes = [C.POINTER(_Mesh), C.c_int, _D, C.c_int, _I, C.c_int, C.c_int, C.c_int, C.c_int]
     L.orc_mesh_free.argtypes = [C.POINTER(_Mesh)]
@@ -88,6 +93,15 @@ def _declare(L):
     L.orc_smooth_nodal.argtypes = [C.c_int, C.c_int, _D, _I, C.c_int, _I, _I, C.c_int, C.c_double, _D]
     L.orc_interp_points.argtypes = [C.POINTER(_Lintet), _D, _I, _I, _I, _D, C.c_int, _D, C.c_int, _I, _D]
     L.orc_interp_cells.argtypes = [C.POINTER(_Lintet), _D, _I, _I, _I, _D, C.c_int, _D, C.c_int, _I, _D]
+    L.orc_lintri_build.argtypes = [C.POINTER(_Lintri), C.c_int, _D, C.c_int, _I, _I]
+    L.orc_lintri_free.argtypes = [C.POINTER(_Lintri)]
+    L.orc_lintri_point_in_cell.argtypes = [C.POINTER(_Lintri), _D, C.c_int]
+    L.orc_lintri_locate.argtypes = [C.POINTER(_Lintri), _D, C.c_int]
+    L.orc_lintri_shape_functions.argtypes = [C.POINTER(_Lintri), _D, C.c_int, _D]
+    L.orc_lintri_fast_distance.argtypes = [C.POINTER(_Lintri), _D, C.c_int]
+    L.orc_lintri_fast_distance.restype = C.c_double
+    L.orc_interp_surface.argtypes = [C.POINTER(_Lintri), C.POINTER(_Lintet), _I, _D, C.c_int, _D, _I, _D]
+    L.orc_interp_surface_cells.argtypes = [C.POINTER(_Lintri), C.POINTER(_Lintet), _I, _D, C.c_int, _D, _I, _D]
     for f in ("orc_resistivity", "orc_sigma"):
         getattr(L, f).argtypes = [C.c_double]
         getattr(L, f).restype = C.c_double
@@ -364,6 +378,52 @@ def interp_cells(L: "LinTet", nodes, tets, cells10, hexes, sol5, points, rank, c
     lib().orc_interp_cells(C.byref(L._L), _f64(nodes), _i32(tets), _i32(cells10), _i32(hexes), _f64(sol5), n, pts,
                            rank, _i32(cells), out)
     return out.reshape(-1, 5)
+
+
+class LinTri:
+    """LinearTriangles on the surface triangles (3 nodes each) with their two tets."""
+
+    def __init__(self, nodes, tris, tri2tet):
+        self._T = _Lintri()
+        lib().orc_lintri_build(C.byref(self._T), len(nodes), _f64(nodes), len(tris), _i32(tris), _i32(tri2tet))
+        T = self._T
+        self.n = T.n
+        self.centroid = _arr(T.cen, 3 * T.n, np.float64).reshape(-1, 3)
+        self.nbr_ptr = _arr(T.nbr_ptr, T.n + 1, np.int32)
+        self.nbr = _arr(T.nbr, int(self.nbr_ptr[-1]), np.int32)
+
+    def __del__(self):
+        try:
+            lib().orc_lintri_free(C.byref(self._T))
+        except Exception:
+            pass
+
+    def point_in_cell(self, p, tri):
+        return bool(lib().orc_lintri_point_in_cell(C.byref(self._T), _f64(p), tri))
+
+    def locate(self, p, guess):
+        return lib().orc_lintri_locate(C.byref(self._T), _f64(p), guess)
+
+    def shape_functions(self, p, tri):
+        w = np.zeros(3)
+        lib().orc_lintri_shape_functions(C.byref(self._T), _f64(p), tri, w)
+        return w
+
+    def fast_distance(self, p, tri):
+        return lib().orc_lintri_fast_distance(C.byref(self._T), _f64(p), tri)
+
+    def interp(self, L: "LinTet", tets, sol5, points, cells=None):
+        """dim 2 rank 1: (cells, Solution rows); with cells given, the mapped-atom path."""
+        pts = _f64(points)
+        n = len(pts) // 3
+        out = np.zeros(5 * n)
+        if cells is None:
+            cells = np.zeros(n, np.int32)
+            lib().orc_interp_surface(C.byref(self._T), C.byref(L._L), _i32(tets), _f64(sol5), n, pts, cells, out)
+        else:
+            cells = _i32(cells)
+            lib().orc_interp_surface_cells(C.byref(self._T), C.byref(L._L), _i32(tets), _f64(sol5), n, pts, cells, out)
+        return cells, out.reshape(-1, 5)
 
 
 def linhex_shape_functions(xe_ucd, p):

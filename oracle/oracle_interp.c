@@ -497,6 +497,153 @@ void orc_interp_cells(const orc_lintet *L, const double *xyz, const int *tets, c
 }
 
 /* ------------------------------------------------------------------------------------- */
+/* linear triangles (surface atoms, dim 2)                                               */
+/* ------------------------------------------------------------------------------------- */
+static v3 cross(v3 a, v3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static double dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static void st3(double *d, int i, v3 a) { d[3 * i] = a.x; d[3 * i + 1] = a.y; d[3 * i + 2] = a.z; }
+
+int orc_lintri_build(orc_lintri *T, int n_nodes, const double *xyz, int nt, const int *tris, const int *tri2tet) {
+    memset(T, 0, sizeof(*T));
+    T->n = nt;
+    const size_t m = nt ? nt : 1;
+    T->v0 = malloc(sizeof(double) * 3 * m); T->e1 = malloc(sizeof(double) * 3 * m);
+    T->e2 = malloc(sizeof(double) * 3 * m); T->pv = malloc(sizeof(double) * 3 * m);
+    T->nrm = malloc(sizeof(double) * 3 * m); T->cen = malloc(sizeof(double) * 3 * m);
+    T->maxd = malloc(sizeof(double) * m); T->tri2tet = malloc(sizeof(int) * 2 * m);
+    memcpy(T->tri2tet, tri2tet, sizeof(int) * 2 * nt);
+    /* node -> triangles, increasing */
+    int *cnt = calloc(n_nodes + 1, sizeof(int));
+    for (int i = 0; i < 3 * nt; ++i) cnt[tris[i] + 1]++;
+    for (int i = 0; i < n_nodes; ++i) cnt[i + 1] += cnt[i];
+    int *n2t = malloc(sizeof(int) * 3 * m), *pos = malloc(sizeof(int) * (n_nodes + 1));
+    memcpy(pos, cnt, sizeof(int) * (n_nodes + 1));
+    for (int t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k) n2t[pos[tris[3 * t + k]]++] = t;
+    free(pos);
+    /* neighbours: for each vertex, the other triangles touching it (repeats kept) */
+    T->nbr_ptr = malloc(sizeof(int) * (nt + 1));
+    int64_t total = 0;
+    for (int t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k) total += cnt[tris[3 * t + k] + 1] - cnt[tris[3 * t + k]] - 1;
+    T->nbr = malloc(sizeof(int) * (total ? total : 1));
+    int64_t nn = 0;
+    T->nbr_ptr[0] = 0;
+    for (int t = 0; t < nt; ++t) {
+        for (int k = 0; k < 3; ++k) {
+            const int node = tris[3 * t + k];
+            for (int e = cnt[node]; e < cnt[node + 1]; ++e)
+                if (n2t[e] != t) T->nbr[nn++] = n2t[e];
+        }
+        T->nbr_ptr[t + 1] = (int)nn;
+    }
+    free(n2t);
+    free(cnt);
+    for (int t = 0; t < nt; ++t) {
+        v3 a = ld(xyz, tris[3 * t]), b = ld(xyz, tris[3 * t + 1]), c = ld(xyz, tris[3 * t + 2]);
+        v3 e1 = sub(b, a), e2 = sub(c, a);
+        /* TetgenFaces::calc_appendices: normalized e1 x e2; centroid (v0 + v1 + v2) / 3 */
+        v3 n = cross(e1, e2);
+        const double nl = sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+        if (nl > 0) {
+            const double f = 1 / nl;
+            n = mk(n.x * f, n.y * f, n.z * f);
+        }
+        v3 cen = dvd(add(add(a, b), c), 3.0);
+        /* LinearTriangles::precompute */
+        v3 pv = cross(n, e2);
+        const double idet = 1.0 / dot(e1, pv);
+        st3(T->v0, t, a);
+        st3(T->e1, t, scl(e1, idet));
+        st3(T->e2, t, e2);
+        st3(T->pv, t, scl(pv, idet));
+        st3(T->nrm, t, n);
+        T->maxd[t] = sqrt(e2.x * e2.x + e2.y * e2.y + e2.z * e2.z);
+        st3(T->cen, t, cen);
+    }
+    return 0;
+}
+
+void orc_lintri_free(orc_lintri *T) {
+    free(T->v0); free(T->e1); free(T->e2); free(T->pv); free(T->nrm); free(T->maxd); free(T->cen);
+    free(T->nbr_ptr); free(T->nbr); free(T->tri2tet);
+    memset(T, 0, sizeof(*T));
+}
+
+int orc_lintri_point_in_cell(const orc_lintri *T, const double p[3], int f) {
+    v3 tv = sub(mk(p[0], p[1], p[2]), ld(T->v0, f));
+    const double u = dot(tv, ld(T->pv, f));
+    if (u < -ZERO || u > 1 + ZERO) return 0;
+    v3 q = cross(tv, ld(T->e1, f));
+    const double v = dot(q, ld(T->nrm, f));
+    if (v < -ZERO || u + v > 1 + ZERO) return 0;
+    return fabs(dot(q, ld(T->e2, f))) < T->maxd[f];
+}
+
+void orc_lintri_shape_functions(const orc_lintri *T, const double p[3], int f, double w[3]) {
+    v3 tv = sub(mk(p[0], p[1], p[2]), ld(T->v0, f));
+    v3 q = cross(tv, ld(T->e1, f));
+    const double v = dot(tv, ld(T->pv, f));
+    const double ww = dot(q, ld(T->nrm, f));
+    const double u = 1.0 - v - ww;
+    w[0] = ZERO + u; w[1] = ZERO + v; w[2] = ZERO + ww;
+}
+
+double orc_lintri_fast_distance(const orc_lintri *T, const double p[3], int f) {
+    v3 tv = sub(mk(p[0], p[1], p[2]), ld(T->v0, f));
+    v3 q = cross(tv, ld(T->e1, f));
+    return dot(ld(T->e2, f), q);
+}
+
+int orc_lintri_locate(const orc_lintri *T, const double p[3], int guess) {
+    const int n = T->n;
+    if (guess >= 0 && guess < n) {
+        if (orc_lintri_point_in_cell(T, p, guess)) return guess;
+        for (int e = T->nbr_ptr[guess]; e < T->nbr_ptr[guess + 1]; ++e)
+            if (orc_lintri_point_in_cell(T, p, T->nbr[e])) return T->nbr[e];
+    }
+    double best = 1e100;
+    int best_i = 0;
+    for (int c = 0; c < n; ++c) {
+        if (orc_lintri_point_in_cell(T, p, c)) return c;
+        const double xx = p[0] - T->cen[3 * c], yy = p[1] - T->cen[3 * c + 1], zz = p[2] - T->cen[3 * c + 2];
+        const double d2 = xx * xx + yy * yy + zz * zz;
+        if (d2 < best) {
+            best = d2;
+            best_i = c;
+        }
+    }
+    return -best_i;
+}
+
+/* LinearTriangles::interp_solution: the tet the point is interpolated in */
+static int surface_tet(const orc_lintri *T, const orc_lintet *L, const double p[3], int tri) {
+    const int t0 = T->tri2tet[2 * tri], t1 = T->tri2tet[2 * tri + 1];
+    if (fabs(orc_lintri_fast_distance(T, p, tri)) <= 100.0 * ZERO) return t0;
+    if (orc_lintet_point_in_cell(L, p, t0)) return t0;
+    return orc_lintet_locate(L, p, t1);
+}
+
+void orc_interp_surface(const orc_lintri *T, const orc_lintet *L, const int *tets, const double *sol5, int np,
+                        const double *pts, int *cells, double *out) {
+    int cell = -1;
+    for (int i = 0; i < np; ++i) {
+        const double *p = pts + 3 * i;
+        cell = orc_lintri_locate(T, p, abs(cell));
+        cells[i] = cell;
+        interp_one(L, NULL, tets, NULL, NULL, sol5, p, 1, surface_tet(T, L, p, abs(cell)), out + 5 * i);
+    }
+}
+
+void orc_interp_surface_cells(const orc_lintri *T, const orc_lintet *L, const int *tets, const double *sol5, int np,
+                              const double *pts, const int *cells, double *out) {
+    for (int i = 0; i < np; ++i) {
+        const double *p = pts + 3 * i;
+        interp_one(L, NULL, tets, NULL, NULL, sol5, p, 1, surface_tet(T, L, p, abs(cells[i])), out + 5 * i);
+    }
+}
+
+/* ------------------------------------------------------------------------------------- */
 /* physical quantities                                                                    */
 /* ------------------------------------------------------------------------------------- */
 /* resistivity of bulk Cu, (T [K], rho [Ohm nm]) pairs, src/PhysicalQuantityData.cpp:47-76 */

@@ -111,3 +111,58 @@ def test_empty_and_invalid_calls(ctx, tip):
         ip.points(np.zeros((2, 3)), 4)
     with pytest.raises(fgpu.FgpuError):
         ip.in_cells(np.zeros((1, 3)), 1, np.array([len(tip.tets) + 5]))
+
+
+def _surface_points(tip, tris, n, seed):
+    """Atoms on, just above and below the surface faces, on surface nodes and far away."""
+    rng = np.random.default_rng(seed)
+    x = tip.nodes
+    f = rng.choice(len(tris), n)
+    on = np.einsum("nk,nkd->nd", rng.dirichlet(np.ones(3), n), x[tris[f]])
+    near = on[: n // 2] + rng.normal(0, 0.3, (n // 2, 3))
+    nodes = x[rng.choice(tip.n_surface_nodes, n // 4)]
+    lo, hi = x.min(0), x.max(0)
+    far = rng.uniform(lo - 10, hi + 10, (n // 8, 3))
+    pts = np.concatenate([on, near, nodes, far])
+    return pts[rng.permutation(len(pts))] if seed % 2 else pts
+
+
+def test_surface_faces_bit_exact(ctx, tip, solved):
+    """dim 2: located faces and values against the oracle's LinearTriangles, bit for bit."""
+    _, s, _, _ = solved
+    tris, t2t = tip.surface_triangles()
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+    ip.extract_solution(s, smoothen=True)
+    ip.set_triangles(tris, t2t)
+    sol = ip.nodal()
+    L = O.LinTet(tip.nodes, tip.tets, tip.tet_nbr, tip.tet_marker, 3)
+    T = O.LinTri(tip.nodes, tris, t2t)
+    for seed in (1, 2):
+        pts = _surface_points(tip, tris, 400, seed)
+        cells_o, out_o = T.interp(L, tip.tets, sol, pts)
+        cells_g, out_g = ip.surface_points(pts)
+        assert np.array_equal(cells_g, cells_o)
+        assert np.array_equal(out_g, out_o)
+        assert np.array_equal(ip.surface_cells(pts, cells_g), out_o)
+        assert (cells_o < 0).any() and (cells_o > 0).any()
+    r = interp.SolutionReader(ip, rank=1, dim=2)
+    pts = _surface_points(tip, tris, 200, 7)
+    got = r.calc_interpolation(pts)
+    cells_o, out_o = T.interp(L, tip.tets, sol, pts)
+    assert np.array_equal(r.markers, cells_o) and np.array_equal(got, out_o)
+    assert np.array_equal(r.calc_interpolation(pts), out_o)
+
+
+def test_surface_faces_invalid_calls(ctx, tip):
+    ip = interp.Interpolator(ctx, tip)
+    with pytest.raises(fgpu.FgpuError):
+        ip.surface_points(np.zeros((2, 3)))          # no faces set
+    tris, t2t = tip.surface_triangles()
+    bad = t2t.copy()
+    bad[0, 1] = -1
+    with pytest.raises(fgpu.FgpuError):
+        ip.set_triangles(tris, bad)
+    ip.set_triangles(tris, t2t)
+    assert ip.surface_points(np.zeros((0, 3)))[0].size == 0
+    with pytest.raises(fgpu.FgpuError):
+        ip.surface_cells(np.zeros((1, 3)), np.array([len(tris)]))

@@ -112,3 +112,71 @@ def test_smoothing_keeps_constants_and_hull(tip):
         lo, hi = sol[nb, :3].min(0), sol[nb, :3].max(0)
         assert np.all(out[i, :3] >= lo - 1e-12) and np.all(out[i, :3] <= hi + 1e-12)
     assert np.array_equal(out[:, 3:], sol[:, 3:])
+
+
+@pytest.fixture(scope="module")
+def surface(tip):
+    m, L, q10 = tip
+    tris, t2t = m.surface_triangles()
+    return tris, t2t, O.LinTri(m.nodes, tris, t2t)
+
+
+def test_surface_triangles_separate_the_regions(tip, surface):
+    m, _, _ = tip
+    tris, t2t, T = surface
+    assert len(tris) > 100
+    assert np.all(m.tet_marker[t2t[:, 0]] == M.TYPES_VACUUM) and np.all(m.tet_marker[t2t[:, 1]] != M.TYPES_VACUUM)
+    for k in range(3):      # every triangle node is a vertex of both tets
+        assert np.all((m.tets[t2t[:, 0]] == tris[:, k, None]).any(1))
+        assert np.all((m.tets[t2t[:, 1]] == tris[:, k, None]).any(1))
+
+
+def test_triangle_weights_and_distance(tip, surface):
+    """On the triangle: barycentric weights (+1e-15 each) that reproduce the point, zero
+    distance, containment; off the plane: the distance along the normal times |e2|^2 ... sign."""
+    m, _, _ = tip
+    tris, _, T = surface
+    rng = np.random.default_rng(3)
+    for f in rng.choice(len(tris), 50, replace=False):
+        b = rng.dirichlet(np.ones(3))
+        x = m.nodes[tris[f]]
+        p = b @ x
+        w = T.shape_functions(p, f)
+        assert abs(w.sum() - 1.0) < 1e-14 and np.allclose(w - 1e-15, b, atol=1e-10)
+        assert abs(T.fast_distance(p, f)) < 1e-9 * np.abs(x).max() ** 2
+        assert T.point_in_cell(p, f)
+        n = np.cross(x[1] - x[0], x[2] - x[0])
+        far = p + 3.0 * np.linalg.norm(x[2] - x[0]) * n / np.linalg.norm(n)
+        assert not T.point_in_cell(far, f)
+
+
+def test_triangle_locate_against_brute_force(tip, surface):
+    m, _, _ = tip
+    tris, _, T = surface
+    rng = np.random.default_rng(4)
+    f = rng.choice(len(tris), 60)
+    pts = np.einsum("nk,nkd->nd", rng.dirichlet(np.ones(3), 60), m.nodes[tris[f]])
+    pts = np.concatenate([pts, pts[:20] + rng.normal(0, 5.0, (20, 3))])
+    for p in pts:
+        c = T.locate(p, -1)
+        inside = [t for t in range(T.n) if T.point_in_cell(p, t)]
+        if inside:
+            assert c == inside[0]
+        else:
+            d = ((T.centroid - p) ** 2).sum(1)
+            assert c == -int(np.argmin(d))
+
+
+def test_surface_interpolation_of_linear_fields(tip, surface):
+    """Atoms on the surface get the linear field exactly (rank 1 in the tet chosen)."""
+    m, L, q10 = tip
+    tris, t2t, T = surface
+    rng = np.random.default_rng(6)
+    f = rng.choice(len(tris), 200)
+    pts = np.einsum("nk,nkd->nd", rng.dirichlet(np.ones(3), 200), m.nodes[tris[f]])
+    pts = pts + rng.normal(0, 0.05, pts.shape)
+    sol = _solution(m.nodes, lambda x: x @ A + 0.25)
+    cells, out = T.interp(L, m.tets, sol, pts)
+    ref = _solution(pts, lambda x: x @ A + 0.25)
+    assert np.abs(out - ref).max() <= 1e-10 * np.abs(ref).max()
+    assert np.array_equal(T.interp(L, m.tets, sol, pts, cells)[1], out)
