@@ -44,8 +44,14 @@ struct fgpu_interp {
     LinTriHost T;
     Bvh TB;
     double *tri_tab = nullptr, *tri_cen = nullptr;
-    int *tri_child = nullptr, *tri_prim = nullptr;
+    int *tri_child = nullptr, *tri_prim = nullptr, *tri_nodes = nullptr;
     float *tri_box = nullptr, *tri_cbox = nullptr, *tri_leafbox = nullptr;
+    // quadrangles: nodes, 6-node faces, centroids and their tree
+    Bvh QB;
+    double *quad_cen = nullptr;
+    int *quad_nodes = nullptr, *tri6 = nullptr, *quad_child = nullptr, *quad_prim = nullptr;
+    float *quad_cbox = nullptr;
+    std::vector<int> hkind, htets;
 };
 
 namespace {
@@ -374,6 +380,76 @@ __global__ __launch_bounds__(256) void k_interp(int n, const double *__restrict_
     cell_out[i] = cell;
 }
 
+// Per point: the lowest cell whose centroid lies within squared distance 1e-15 (x, -1 if none)
+// and a traversal-overflow flag (w); InterpolatorCells::is_centroid (src/InterpolatorCells.cpp:
+// 330-333) on every centroid the tree does not rule out.
+__global__ __launch_bounds__(256) void k_centroid_match(int n, const double *__restrict__ pts, int n_leaves,
+                                                        const int *__restrict__ child, const float *__restrict__ cbox,
+                                                        const int *__restrict__ prim, const double *__restrict__ cen,
+                                                        int4 *__restrict__ cand) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double p[3] = {pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]};
+    int best = INT_MAX, overflow = 0;
+    auto leaf = [&](int l) {
+        const int c = prim[l];
+        const double xx = p[0] - cen[3 * (size_t)c];
+        const double yy = p[1] - cen[3 * (size_t)c + 1];
+        const double zz = p[2] - cen[3 * (size_t)c + 2];
+        if (xx * xx + yy * yy + zz * zz < kZero && c < best) best = c;
+    };
+    if (n_leaves == 1) {
+        leaf(0);
+    } else {
+        int stack[kBvhStack];
+        int sp = 0;
+        stack[sp++] = 0;
+        while (sp > 0) {
+            const int v = stack[--sp];
+            // box_d2 never exceeds the squared distance to a centroid inside the box
+            if (box_d2(cbox + 6 * (size_t)v, p) >= kZero) continue;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int c = child[2 * (size_t)v + k];
+                if (c < 0) leaf(~c);
+                else if (sp < kBvhStack) stack[sp++] = c;
+                else overflow = 1;
+            }
+        }
+    }
+    cand[i] = make_int4(best == INT_MAX ? -1 : best, 0, 0, overflow);
+}
+
+// InterpolatorCells::interp_centroid (src/InterpolatorCells.cpp:336-356): the cell's nodes
+// weighted alike; cell < 0 (no centroid matched) gives Solution(-1)
+__global__ __launch_bounds__(256) void k_interp_nodes(int n, const int *__restrict__ cell, const int *__restrict__ nodes,
+                                                      int nn, double w, const double *__restrict__ sol,
+                                                      double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double *o = out + 5 * (size_t)i;
+    const int c = cell[i];
+    if (c < 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) o[k] = -1.0;
+        return;
+    }
+    double vx = 0, vy = 0, vz = 0, s1 = 0, s2 = 0;
+    for (int k = 0; k < nn; ++k) {
+        const double *s = sol + 5 * (size_t)nodes[nn * (size_t)c + k];
+        vx += s[0] * w;
+        vy += s[1] * w;
+        vz += s[2] * w;
+        s1 += s[3] * w;
+        s2 += s[4] * w;
+    }
+    o[0] = vx;
+    o[1] = vy;
+    o[2] = vz;
+    o[3] = s1;
+    o[4] = s2;
+}
+
 // Interpolator::extract_solution(PoissonSolver&) with store_elfield (src/Interpolator.cpp:79-140,
 // 172-190; LinearHexahedra::shape_fun_grads(hex, node), src/InterpolatorCells.cpp:1462-1505)
 __global__ __launch_bounds__(256) void k_extract(int n_nodes, const int *__restrict__ node_idof,
@@ -502,11 +578,14 @@ void ensure_points(fgpu_interp *ip, size_t n) {
 }
 
 void free_triangles(fgpu_interp *ip) {
-    for (double **p : {&ip->tri_tab, &ip->tri_cen}) release(*p);
-    for (int **p : {&ip->tri_child, &ip->tri_prim}) release(*p);
-    for (float **p : {&ip->tri_box, &ip->tri_cbox, &ip->tri_leafbox}) release(*p);
+    for (double **p : {&ip->tri_tab, &ip->tri_cen, &ip->quad_cen}) release(*p);
+    for (int **p : {&ip->tri_child, &ip->tri_prim, &ip->tri_nodes, &ip->quad_nodes, &ip->tri6, &ip->quad_child,
+                    &ip->quad_prim})
+        release(*p);
+    for (float **p : {&ip->tri_box, &ip->tri_cbox, &ip->tri_leafbox, &ip->quad_cbox}) release(*p);
     ip->T = LinTriHost{};
     ip->TB = Bvh{};
+    ip->QB = Bvh{};
 }
 
 void free_interp(fgpu_interp *ip) {
@@ -559,6 +638,8 @@ extern "C" int fgpu_interp_create(fgpu_ctx *c, int n_nodes, const double *xyz, c
         ip->n_tetnodes = tn;
         build_lintet(ip->L, n_nodes, xyz, n_tets, tets, tet_nbr, tet_marker, search_region);
         ip->hxyz.assign(xyz, xyz + 3 * (size_t)n_nodes);
+        ip->hkind.assign(node_kind, node_kind + n_nodes);
+        ip->htets.assign(tets, tets + 4 * (size_t)n_tets);
         // LinearTetrahedra::precompute: decay factor of the smoothing, -1 / longest tet edge
         double emax = -1e308;
         for (int t = 0; t < n_tets; ++t) {
@@ -818,7 +899,8 @@ extern "C" int fgpu_interp_cells(fgpu_interp *ip, int n, const double *xyz, int 
 // ------------------------------------------------------------------------------------------
 // surface faces (dim 2)
 // ------------------------------------------------------------------------------------------
-extern "C" int fgpu_interp_set_triangles(fgpu_interp *ip, int n_tris, const int *tris, const int *tri2tet) {
+extern "C" int fgpu_interp_set_triangles(fgpu_interp *ip, int n_tris, const int *tris, const int *tri2tet,
+                                         const int *quads) {
     if (!ip || n_tris < 0 || (n_tris > 0 && (!tris || !tri2tet))) return FGPU_EINVAL;
     return guard(ip, [&] {
         for (size_t i = 0; i < 3 * (size_t)n_tris; ++i)
@@ -826,17 +908,22 @@ extern "C" int fgpu_interp_set_triangles(fgpu_interp *ip, int n_tris, const int 
         for (size_t i = 0; i < 2 * (size_t)n_tris; ++i)
             if (tri2tet[i] < 0 || tri2tet[i] >= ip->n_tets)
                 throw std::invalid_argument("every triangle needs two tets of the mesh");
+        if (quads)
+            for (size_t i = 0; i < 12 * (size_t)n_tris; ++i)
+                if (quads[i] < 0 || quads[i] >= ip->n_nodes) throw std::invalid_argument("quad node out of range");
         IPCHECK(hipSetDevice(ip->ctx->device));
         hipStream_t s = ip->ctx->stream;
         IPCHECK(hipStreamSynchronize(s));
         free_triangles(ip);
         if (n_tris == 0) return FGPU_OK;
-        build_lintri(ip->T, ip->n_nodes, ip->hxyz.data(), n_tris, tris, tri2tet);
+        LinTriHost T;
+        build_lintri(T, ip->n_nodes, ip->hxyz.data(), n_tris, tris, tri2tet);
+        if (quads) set_quads(T, ip->hxyz.data(), ip->hkind.data(), ip->htets.data(), quads);
         // search boxes: the face widened by |edge2| on every side contains the prism of
         // points within that distance of the face plane that point_in_cell accepts
         std::vector<double> lo(3 * (size_t)n_tris), hi(3 * (size_t)n_tris);
         for (int f = 0; f < n_tris; ++f) {
-            const double m = ip->T.tab[16 * (size_t)f + 15];
+            const double m = T.tab[16 * (size_t)f + 15];
             for (int a = 0; a < 3; ++a) {
                 double l = 1e300, h = -1e300;
                 for (int k = 0; k < 3; ++k) {
@@ -848,55 +935,102 @@ extern "C" int fgpu_interp_set_triangles(fgpu_interp *ip, int n_tris, const int 
                 hi[3 * (size_t)f + a] = h + m;
             }
         }
+        ip->T = std::move(T);
         build_bvh(n_tris, lo, hi, ip->T.centroid, ip->TB);
         ip->tri_tab = upload(ip->T.tab.data(), ip->T.tab.size(), s);
         ip->tri_cen = upload(ip->T.centroid.data(), ip->T.centroid.size(), s);
+        ip->tri_nodes = upload(ip->T.tris.data(), ip->T.tris.size(), s);
         ip->tri_child = upload(ip->TB.child.data(), ip->TB.child.size(), s);
         ip->tri_prim = upload(ip->TB.prim.data(), ip->TB.prim.size(), s);
         ip->tri_box = upload(ip->TB.box.data(), ip->TB.box.size(), s);
         ip->tri_cbox = upload(ip->TB.cbox.data(), ip->TB.cbox.size(), s);
         ip->tri_leafbox = upload(ip->TB.leafbox.data(), ip->TB.leafbox.size(), s);
+        if (quads) {
+            build_bvh(3 * n_tris, ip->T.qcen, ip->T.qcen, ip->T.qcen, ip->QB);
+            ip->quad_cen = upload(ip->T.qcen.data(), ip->T.qcen.size(), s);
+            ip->quad_nodes = upload(ip->T.quads.data(), ip->T.quads.size(), s);
+            ip->tri6 = upload(ip->T.tri6.data(), ip->T.tri6.size(), s);
+            ip->quad_child = upload(ip->QB.child.data(), ip->QB.child.size(), s);
+            ip->quad_prim = upload(ip->QB.prim.data(), ip->QB.prim.size(), s);
+            ip->quad_cbox = upload(ip->QB.cbox.data(), ip->QB.cbox.size(), s);
+        }
         IPCHECK(hipStreamSynchronize(s));
         return FGPU_OK;
     });
 }
 
-// SolutionReader::calc_full_interpolation / calc_interpolation for dim 2, rank 1
-// (src/SolutionReader.cpp:43-65, 91-101, 136-163): faces located with the chained guess,
-// values interpolated in the tet LinearTriangles::interp_solution picks
-static int surface_run(fgpu_interp *ip, int n, const double *xyz, int *cells, double *out, bool locate) {
-    if (!ip || n < 0 || (n > 0 && (!xyz || !cells || !out))) return FGPU_EINVAL;
+// SolutionReader::calc_full_interpolation / calc_interpolation for dim 2 (src/SolutionReader.cpp:
+// 43-123, 136-190): faces (rank 1, 2) or quads (rank 3) located with the chained guess and the
+// values interpolated in the tet or hex the face picks; with centroids, the exact centroid
+// match and the cell's node average
+static int surface_run(fgpu_interp *ip, int n, const double *xyz, int rank, int centroids, int *cells, double *out,
+                       bool locate) {
+    if (!ip || n < 0 || (n > 0 && (!xyz || !cells || !out)) || rank < 1 || rank > 3) return FGPU_EINVAL;
     if (n == 0) return FGPU_OK;
-    if (ip->T.n == 0) {
-        if (ip->ctx) ip->ctx->err = "no surface triangles set";
+    auto fail = [&](const char *msg) {
+        if (ip->ctx) ip->ctx->err = msg;
         return FGPU_EINVAL;
-    }
+    };
+    if (ip->T.n == 0) return fail("no surface triangles set");
+    if ((rank == 3 || (rank == 2 && centroids)) && ip->T.quads.empty())
+        return fail("rank 3 and the rank 2 centroids need the quadrangles of the faces");
+    if (rank >= 2 && !centroids && ip->n_hexes == 0) return fail("ranks 2 and 3 need the hexes of the mesh");
+    const int n_cells = rank == 3 ? 3 * ip->T.n : ip->T.n;
     return guard(ip, [&] {
         if (!locate)
             for (int i = 0; i < n; ++i)
-                if (cells[i] <= -ip->T.n || cells[i] >= ip->T.n) throw std::invalid_argument("cell index out of range");
+                if (cells[i] <= -n_cells || cells[i] >= n_cells || (centroids && cells[i] < 0))
+                    throw std::invalid_argument("cell index out of range");
         IPCHECK(hipSetDevice(ip->ctx->device));
         hipStream_t st = ip->ctx->stream;
         ensure_points(ip, n);
         IPCHECK(hipMemcpyAsync(ip->pts, xyz, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
-        tet_candidates(ip, n, ip->cand);
-        if (locate)
-            hipLaunchKernelGGL(k_candidates<TriCells>, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, ip->TB.n_leaves,
-                               ip->tri_child, ip->tri_box, ip->tri_cbox, ip->tri_leafbox, ip->tri_prim,
-                               TriCells{ip->tri_tab}, ip->tri_cen, ip->cand2);
-        IPCHECK(hipGetLastError());
-        std::vector<int4> tc(n), fc(locate ? n : 0);
-        IPCHECK(hipMemcpyAsync(tc.data(), ip->cand, n * sizeof(int4), hipMemcpyDeviceToHost, st));
-        if (locate) IPCHECK(hipMemcpyAsync(fc.data(), ip->cand2, n * sizeof(int4), hipMemcpyDeviceToHost, st));
-        IPCHECK(hipStreamSynchronize(st));
-        for (int i = 0; i < n; ++i)
-            if (tc[i].w || (locate && fc[i].w)) throw std::runtime_error("point search stack overflow");
-        if (locate) resolve_tri_chain(ip->T, n, xyz, fc.data(), cells);
-        std::vector<int> tet(n);
-        for (int i = 0; i < n; ++i) tet[i] = surface_tet(ip->T, ip->L, xyz + 3 * (size_t)i, std::abs(cells[i]), tc[i]);
-        IPCHECK(hipMemcpyAsync(ip->tet, tet.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_interp, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, 1, 0, ip->tet, ip->det0, ip->det,
-                           ip->tets, ip->c10, ip->hexes, ip->xyz, ip->sol, ip->cells, ip->out);
+        std::vector<int> use(n);
+        if (centroids) {
+            if (locate) {
+                const bool q = rank == 3;
+                hipLaunchKernelGGL(k_centroid_match, dim3(grid(n)), dim3(256), 0, st, n, ip->pts,
+                                   q ? ip->QB.n_leaves : ip->TB.n_leaves, q ? ip->quad_child : ip->tri_child,
+                                   q ? ip->quad_cbox : ip->tri_cbox, q ? ip->quad_prim : ip->tri_prim,
+                                   q ? ip->quad_cen : ip->tri_cen, ip->cand2);
+                IPCHECK(hipGetLastError());
+                std::vector<int4> mc(n);
+                IPCHECK(hipMemcpyAsync(mc.data(), ip->cand2, n * sizeof(int4), hipMemcpyDeviceToHost, st));
+                IPCHECK(hipStreamSynchronize(st));
+                for (int i = 0; i < n; ++i)
+                    if (mc[i].w) throw std::runtime_error("point search stack overflow");
+                resolve_centroid_chain(q ? ip->T.qcen : ip->T.centroid, n, xyz, mc.data(), cells, use.data());
+            } else {
+                use.assign(cells, cells + n);
+            }
+            IPCHECK(hipMemcpyAsync(ip->tet, use.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+            const int nn = rank == 3 ? 4 : (rank == 2 ? 6 : 3);
+            const int *nodes = rank == 3 ? ip->quad_nodes : (rank == 2 ? ip->tri6 : ip->tri_nodes);
+            hipLaunchKernelGGL(k_interp_nodes, dim3(grid(n)), dim3(256), 0, st, n, ip->tet, nodes, nn, 1.0 / nn,
+                               ip->sol, ip->out);
+        } else {
+            tet_candidates(ip, n, ip->cand);
+            if (locate)
+                hipLaunchKernelGGL(k_candidates<TriCells>, dim3(grid(n)), dim3(256), 0, st, n, ip->pts,
+                                   ip->TB.n_leaves, ip->tri_child, ip->tri_box, ip->tri_cbox, ip->tri_leafbox,
+                                   ip->tri_prim, TriCells{ip->tri_tab}, ip->tri_cen, ip->cand2);
+            IPCHECK(hipGetLastError());
+            std::vector<int4> tc(n), fc(locate ? n : 0);
+            IPCHECK(hipMemcpyAsync(tc.data(), ip->cand, n * sizeof(int4), hipMemcpyDeviceToHost, st));
+            if (locate) IPCHECK(hipMemcpyAsync(fc.data(), ip->cand2, n * sizeof(int4), hipMemcpyDeviceToHost, st));
+            IPCHECK(hipStreamSynchronize(st));
+            for (int i = 0; i < n; ++i)
+                if (tc[i].w || (locate && fc[i].w)) throw std::runtime_error("point search stack overflow");
+            if (locate) resolve_tri_chain(ip->T, n, xyz, fc.data(), rank == 3, cells);
+            for (int i = 0; i < n; ++i) {
+                const double *p = xyz + 3 * (size_t)i;
+                use[i] = rank == 3 ? surface_hex(ip->T, ip->L, p, std::abs(cells[i]), tc[i])
+                                   : surface_tet(ip->T, ip->L, p, std::abs(cells[i]), tc[i]);
+            }
+            IPCHECK(hipMemcpyAsync(ip->tet, use.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_interp, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, rank, 0, ip->tet, ip->det0,
+                               ip->det, ip->tets, ip->c10, ip->hexes, ip->xyz, ip->sol, ip->cells, ip->out);
+        }
         IPCHECK(hipGetLastError());
         IPCHECK(hipMemcpyAsync(out, ip->out, 5 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
         IPCHECK(hipStreamSynchronize(st));
@@ -904,10 +1038,12 @@ static int surface_run(fgpu_interp *ip, int n, const double *xyz, int *cells, do
     });
 }
 
-extern "C" int fgpu_interp_surface_points(fgpu_interp *ip, int n, const double *xyz, int *cells, double *out) {
-    return surface_run(ip, n, xyz, cells, out, true);
+extern "C" int fgpu_interp_surface_points(fgpu_interp *ip, int n, const double *xyz, int rank, int centroids,
+                                          int *cells, double *out) {
+    return surface_run(ip, n, xyz, rank, centroids, cells, out, true);
 }
 
-extern "C" int fgpu_interp_surface_cells(fgpu_interp *ip, int n, const double *xyz, const int *cells, double *out) {
-    return surface_run(ip, n, xyz, const_cast<int *>(cells), out, false);
+extern "C" int fgpu_interp_surface_cells(fgpu_interp *ip, int n, const double *xyz, int rank, int centroids,
+                                         const int *cells, double *out) {
+    return surface_run(ip, n, xyz, rank, centroids, const_cast<int *>(cells), out, false);
 }

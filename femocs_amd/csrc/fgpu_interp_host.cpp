@@ -112,6 +112,14 @@ void build_lintet(LinTetHost &L, int n_nodes, const double *xyz, int nt, const i
     }
 }
 
+void LinTetHost::bcc(const double p[3], int t, double b[4]) const {
+    const double *D = &det[16 * (size_t)t];
+    for (int k = 0; k < 4; ++k) {
+        const double *d = D + 4 * k;
+        b[k] = kZero + det0[t] * (p[0] * d[0] + p[1] * d[1] + p[2] * d[2] + 1.0 * d[3]);
+    }
+}
+
 bool LinTetHost::point_in_cell(const double p[3], int t) const {
     const double *D = &det[16 * (size_t)t];
     const double d0 = det0[t];
@@ -144,6 +152,7 @@ void build_lintri(LinTriHost &T, int n_nodes, const double *xyz, int nf, const i
     T.tab.assign(16 * (size_t)nf, 0.0);
     T.centroid.assign(3 * (size_t)nf, 0.0);
     T.tri2tet.assign(tri2tet, tri2tet + 2 * (size_t)nf);
+    T.tris.assign(tris, tris + 3 * (size_t)nf);
     std::vector<int> ptr(n_nodes + 1, 0), n2f(3 * (size_t)nf);
     for (size_t i = 0; i < 3 * (size_t)nf; ++i) ptr[tris[i] + 1]++;
     for (int i = 0; i < n_nodes; ++i) ptr[i + 1] += ptr[i];
@@ -190,6 +199,61 @@ bool LinTriHost::point_in_cell(const double p[3], int f) const {
     const double v = dot(q, at(t + 12));
     if (v < -kZero || u + v > 1 + kZero) return false;
     return std::fabs(dot(q, at(t + 6))) < t[15];
+}
+
+void LinTriHost::shape_functions(const double p[3], int f, double w[3]) const {
+    const double *t = &tab[16 * (size_t)f];
+    const V3 tv = vsub({p[0], p[1], p[2]}, at(t));
+    const V3 q = cross(tv, at(t + 3));
+    const double v = dot(tv, at(t + 9));
+    const double ww = dot(q, at(t + 12));
+    const double u = 1.0 - v - ww;
+    w[0] = kZero + u;
+    w[1] = kZero + v;
+    w[2] = kZero + ww;
+}
+
+void set_quads(LinTriHost &T, const double *xyz, const int *node_kind, const int *tets, const int *quads) {
+    const int nq = 3 * T.n;
+    T.quads.assign(quads, quads + 4 * (size_t)nq);
+    T.qcen.assign(3 * (size_t)nq, 0.0);
+    T.quad2hex.assign(2 * (size_t)nq, 0);
+    T.tri6.assign(6 * (size_t)T.n, 0);
+    for (int q = 0; q < nq; ++q) {
+        double c[3] = {0, 0, 0};
+        for (int k = 0; k < 4; ++k)
+            for (int a = 0; a < 3; ++a) c[a] += xyz[3 * (size_t)quads[4 * (size_t)q + k] + a];
+        for (int a = 0; a < 3; ++a) T.qcen[3 * (size_t)q + a] = c[a] * (1.0 / 4);
+        const int f = q / 3, v = T.tris[3 * (size_t)f + q % 3];
+        if (quads[4 * (size_t)q] != v) throw std::invalid_argument("quad 3f+k must start at vertex k of face f");
+        for (int r = 0; r < 2; ++r) {
+            const int t = T.tri2tet[2 * (size_t)f + r];
+            int j = 0;
+            while (j < 4 && tets[4 * (size_t)t + j] != v) ++j;
+            if (j == 4) throw std::invalid_argument("a face vertex is not a vertex of the face's tets");
+            T.quad2hex[2 * (size_t)q + r] = 4 * t + j;
+        }
+    }
+    for (int f = 0; f < T.n; ++f) {
+        int e[3][4], ne[3] = {0, 0, 0};
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 4; ++k) {
+                const int node = quads[4 * (size_t)(3 * f + i) + k];
+                if (node_kind[node] == 2) e[i][ne[i]++] = node;
+            }
+        for (int k = 0; k < 3; ++k) T.tri6[6 * (size_t)f + k] = T.tris[3 * (size_t)f + k];
+        for (int i = 0; i < 3; ++i) {
+            const int j = (i + 1) % 3;
+            int c = -1;
+            for (int a = 0; a < ne[i] && c < 0; ++a)
+                for (int b = 0; b < ne[j]; ++b)
+                    if (e[i][a] == e[j][b]) {
+                        c = e[i][a];
+                        break;
+                    }
+            T.tri6[6 * (size_t)f + 3 + i] = c;
+        }
+    }
 }
 
 double LinTriHost::fast_distance(const double p[3], int f) const {
@@ -398,12 +462,12 @@ void resolve_chain(const LinTetHost &L, int n, const double *pts, const int4 *ca
     }
 }
 
-void resolve_tri_chain(const LinTriHost &T, int n, const double *pts, const int4 *cand, int *tri) {
+void resolve_tri_chain(const LinTriHost &T, int n, const double *pts, const int4 *cand, bool quads, int *cell) {
     int prev = -1;
     for (int i = 0; i < n; ++i) {
         const double *p = pts + 3 * (size_t)i;
         const int4 c = cand[i];
-        const int g = std::abs(prev);
+        const int g = quads ? std::abs(prev) / 3 : std::abs(prev);
         int r;
         if (c.y == 0) {
             r = -c.z;
@@ -424,8 +488,35 @@ void resolve_tri_chain(const LinTriHost &T, int n, const double *pts, const int4
             }
             if (r == INT_MIN) r = c.x;
         }
-        tri[i] = r;
+        if (quads) {
+            const int sign = r < 0 ? -1 : 1, f = std::abs(r);
+            double b[3];
+            T.shape_functions(p, f, b);
+            if (b[0] >= b[1] && b[0] >= b[2]) r = sign * (3 * f);
+            else if (b[1] >= b[0] && b[1] >= b[2]) r = sign * (3 * f + 1);
+            else if (b[2] >= b[0] && b[2] >= b[1]) r = sign * (3 * f + 2);
+            else r = -1;
+        }
+        cell[i] = r;
         prev = r;
+    }
+}
+
+void resolve_centroid_chain(const std::vector<double> &cen, int n, const double *pts, const int4 *cand, int *cell,
+                            int *use) {
+    int prev = -1;
+    for (int i = 0; i < n; ++i) {
+        const double *p = pts + 3 * (size_t)i;
+        int r = -1;
+        if (prev >= 0) {
+            const double xx = p[0] - cen[3 * (size_t)prev], yy = p[1] - cen[3 * (size_t)prev + 1],
+                         zz = p[2] - cen[3 * (size_t)prev + 2];
+            if (xx * xx + yy * yy + zz * zz < kZero) r = prev;
+        }
+        if (r < 0) r = cand[i].x;
+        if (r >= 0) prev = r;
+        cell[i] = prev;
+        use[i] = r;
     }
 }
 
@@ -436,6 +527,31 @@ int locate_tet(const LinTetHost &L, const double p[3], int g, int4 c) {
             if (L.point_in_cell(p, L.nbr[e])) return L.nbr[e];
     }
     return c.x >= 0 ? c.x : -c.z;
+}
+
+int surface_hex(const LinTriHost &T, const LinTetHost &L, const double p[3], int quad, int4 tet_cand) {
+    const int h0 = T.quad2hex[2 * (size_t)quad], h1 = T.quad2hex[2 * (size_t)quad + 1];
+    if (std::fabs(T.fast_distance(p, quad / 3)) <= 100.0 * kZero) return h0;
+    // LinearHexahedra::point_in_cell (src/InterpolatorCells.cpp:1507-1527)
+    double b[4];
+    L.bcc(p, h0 / 4, b);
+    if (b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[3] >= 0) {
+        const int k = h0 % 4;
+        bool top = true;
+        for (int j = 0; j < 4; ++j)
+            if (j != k && !(b[k] >= b[j])) top = false;
+        if (top) return h0;
+    }
+    // LinearHexahedra::locate_cell from h1 (src/InterpolatorCells.cpp:1536-1562)
+    int t = locate_tet(L, p, h1 / 4, tet_cand);
+    const int sign = t < 0 ? -1 : 1;
+    t = std::abs(t);
+    L.bcc(p, t, b);
+    if (b[0] >= b[1] && b[0] >= b[2] && b[0] >= b[3]) return sign * (4 * t);
+    if (b[1] >= b[0] && b[1] >= b[2] && b[1] >= b[3]) return sign * (4 * t + 1);
+    if (b[2] >= b[0] && b[2] >= b[1] && b[2] >= b[3]) return sign * (4 * t + 2);
+    if (b[3] >= b[0] && b[3] >= b[1] && b[3] >= b[2]) return sign * (4 * t + 3);
+    return -1;
 }
 
 int surface_tet(const LinTriHost &T, const LinTetHost &L, const double p[3], int tri, int4 tet_cand) {

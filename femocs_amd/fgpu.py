@@ -88,9 +88,9 @@ API = [
     ("fgpu_interp_set_nodal", C.c_int, [_P, _D]),
     ("fgpu_interp_points", C.c_int, [_P, C.c_int, _D, C.c_int, _I, _D]),
     ("fgpu_interp_cells", C.c_int, [_P, C.c_int, _D, C.c_int, _I, _D]),
-    ("fgpu_interp_set_triangles", C.c_int, [_P, C.c_int, _I, _I]),
-    ("fgpu_interp_surface_points", C.c_int, [_P, C.c_int, _D, _I, _D]),
-    ("fgpu_interp_surface_cells", C.c_int, [_P, C.c_int, _D, _I, _D]),
+    ("fgpu_interp_set_triangles", C.c_int, [_P, C.c_int, _I, _I, C.c_void_p]),
+    ("fgpu_interp_surface_points", C.c_int, [_P, C.c_int, _D, C.c_int, C.c_int, _I, _D]),
+    ("fgpu_interp_surface_cells", C.c_int, [_P, C.c_int, _D, C.c_int, C.c_int, _I, _D]),
     ("fgpu_sys_heat_terms", C.c_int, [_P, _P, C.c_double, C.c_double]),
     ("fgpu_sys_apply32", C.c_int, [_P, _D, _D]),
     ("fgpu_mesh_use_affine", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),

@@ -84,28 +84,38 @@ class Interpolator:
         return out.reshape(-1, 5)
 
 
-    def set_triangles(self, tris, tri2tet):
-        """LinearTriangles on the surface faces (3 nodes and 2 tets per face)."""
+    def set_triangles(self, tris, tri2tet, quads=None):
+        """LinearTriangles on the surface faces (3 nodes and 2 tets per face); quads (3 per
+        face, quad 3f+k from vertex k) for rank 3 and the rank 2 centroids."""
         t = np.ascontiguousarray(tris, np.int32).reshape(-1)
         t2 = np.ascontiguousarray(tri2tet, np.int32).reshape(-1)
         if 2 * t.size != 3 * t2.size:
             raise ValueError("3 nodes and 2 tets per face")
-        self.ctx.check(self.ctx.lib.fgpu_interp_set_triangles(self.h, t.size // 3, t, t2), "fgpu_interp_set_triangles")
+        q = None
+        if quads is not None:
+            q = np.ascontiguousarray(quads, np.int32).reshape(-1)
+            if q.size != 4 * t.size:
+                raise ValueError("3 quadrangles of 4 nodes per face")
+        self._quads = q
+        self.ctx.check(self.ctx.lib.fgpu_interp_set_triangles(
+            self.h, t.size // 3, t, t2, None if q is None else q.ctypes.data), "fgpu_interp_set_triangles")
 
-    def surface_points(self, xyz):
+    def surface_points(self, xyz, rank: int = 1, centroids: bool = False):
         pts = np.ascontiguousarray(xyz, np.float64).reshape(-1)
         n = pts.size // 3
         cells = np.zeros(n, np.int32)
         out = np.zeros(5 * n)
-        self.ctx.check(self.ctx.lib.fgpu_interp_surface_points(self.h, n, pts, cells, out), "fgpu_interp_surface_points")
+        self.ctx.check(self.ctx.lib.fgpu_interp_surface_points(self.h, n, pts, int(rank), int(bool(centroids)), cells,
+                                                               out), "fgpu_interp_surface_points")
         return cells, out.reshape(-1, 5)
 
-    def surface_cells(self, xyz, cells):
+    def surface_cells(self, xyz, cells, rank: int = 1, centroids: bool = False):
         pts = np.ascontiguousarray(xyz, np.float64).reshape(-1)
         n = pts.size // 3
         c = np.ascontiguousarray(cells, np.int32)
         out = np.zeros(5 * n)
-        self.ctx.check(self.ctx.lib.fgpu_interp_surface_cells(self.h, n, pts, c, out), "fgpu_interp_surface_cells")
+        self.ctx.check(self.ctx.lib.fgpu_interp_surface_cells(self.h, n, pts, int(rank), int(bool(centroids)), c, out),
+                       "fgpu_interp_surface_cells")
         return out.reshape(-1, 5)
 
 
@@ -113,19 +123,25 @@ class SolutionReader:
     """SolutionReader for dim 3 without atom sorting (src/SolutionReader.cpp:43-190): the first
     call maps the atoms to cells (markers), later calls interpolate in the mapped cells."""
 
-    def __init__(self, interpolator: Interpolator, rank: int = 1, dim: int = 3):
-        if rank not in (1, 2, 3):
-            raise ValueError("rank must be 1, 2 or 3")
-        if dim not in (2, 3) or (dim == 2 and rank != 1):
-            raise ValueError("dim 3 with ranks 1-3, or dim 2 with rank 1")
+    def __init__(self, interpolator: Interpolator, rank: int = 1, dim: int = 3, interp_centroids: bool = False):
         self.interpolator = interpolator
-        self.rank = rank
-        self.dim = dim
         self.markers = None
+        self.set_preferences(False, dim, rank, interp_centroids)
+
+    def set_preferences(self, sort_atoms: bool, dim: int, rank: int, interp_centroids: bool = False):
+        """SolutionReader::set_preferences (include/SolutionReader.h:60-67).  Atom sorting
+        (sort_spatial) is not offered: the caller orders the atoms as the guess chain should run."""
+        if sort_atoms:
+            raise ValueError("atom sorting is not supported")
+        if rank not in (1, 2, 3) or dim not in (2, 3):
+            raise ValueError("dim 2 or 3, rank 1, 2 or 3")
+        if interp_centroids and dim != 2:
+            raise ValueError("centroid interpolation is offered on the surface (dim 2)")
+        self.dim, self.rank, self.interp_centroids = dim, rank, bool(interp_centroids)
 
     def calc_full_interpolation(self, xyz) -> np.ndarray:
         if self.dim == 2:
-            self.markers, sol = self.interpolator.surface_points(xyz)
+            self.markers, sol = self.interpolator.surface_points(xyz, self.rank, self.interp_centroids)
         else:
             self.markers, sol = self.interpolator.points(xyz, self.rank)
         return sol
@@ -135,5 +151,5 @@ class SolutionReader:
         if self.markers is None or len(self.markers) != n:
             return self.calc_full_interpolation(xyz)
         if self.dim == 2:
-            return self.interpolator.surface_cells(xyz, self.markers)
+            return self.interpolator.surface_cells(xyz, self.markers, self.rank, self.interp_centroids)
         return self.interpolator.in_cells(xyz, self.rank, self.markers)

@@ -274,6 +274,27 @@ class TipMesh:
         tris = tets[t[:, None], opp[k]]
         return tris.astype(np.int32), np.stack([t, other], 1).astype(np.int32)
 
+    def surface_quads(self, tris):
+        """The 3 quadrangles each face splits into (quad 3f+k around face vertex k): (vertex k,
+        midpoint of edge (k, k+1), face centroid, midpoint of edge (k, k+2)), as hex-mesh nodes."""
+        nv = self.n_tetnodes
+        _, ev, _, fv = _tet_entities(nv, self.tets)
+        ekey = ev[:, 0] * nv + ev[:, 1]
+        fkey = (fv[:, 0] * nv + fv[:, 1]) * nv + fv[:, 2]
+        tris = np.asarray(tris, np.int64)
+        fs = np.sort(tris, 1)
+        face = len(ev) + nv + np.searchsorted(fkey, (fs[:, 0] * nv + fs[:, 1]) * nv + fs[:, 2])
+
+        def edge(a, b):
+            lo, hi = np.minimum(a, b), np.maximum(a, b)
+            return nv + np.searchsorted(ekey, lo * nv + hi)
+
+        q = np.empty((len(tris), 3, 4), np.int64)
+        for k in range(3):
+            a, b, c = tris[:, k], tris[:, (k + 1) % 3], tris[:, (k + 2) % 3]
+            q[:, k] = np.stack([a, edge(a, b), face, edge(a, c)], 1)
+        return q.reshape(-1, 4).astype(np.int32)
+
     @property
     def n_tets(self) -> int:
         return len(self.tets)

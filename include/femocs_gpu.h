@@ -218,15 +218,26 @@ int fgpu_interp_cells(fgpu_interp *interp, int n, const double *xyz, int rank, c
  * vacuum and the bulk (src/InterpolatorCells.cpp:1585-1637; faces and their two tets as
  * TetgenFaces holds them, src/TetgenCells.cpp:387-409).  tris: 3 nodes per face; tri2tet: the
  * face's 2 tets in TetgenFaces::to_tets order (the first is tried first), both tets of the
- * mesh.  n_tris = 0 drops the faces. */
-int fgpu_interp_set_triangles(fgpu_interp *interp, int n_tris, const int *tris, const int *tri2tet);
-/* SolutionReader::calc_full_interpolation, dim 2, rank 1 (src/SolutionReader.cpp:43-65,
- * 136-163; LinearTriangles::interp_solution, src/InterpolatorCells.cpp:1662-1686): each point
- * located among the faces with the chained guess (negative: nearest centroid), then
- * interpolated in the face's tet. */
-int fgpu_interp_surface_points(fgpu_interp *interp, int n, const double *xyz, int *cells, double *out);
-/* SolutionReader::calc_interpolation, dim 2, rank 1, atoms already mapped to faces */
-int fgpu_interp_surface_cells(fgpu_interp *interp, int n, const double *xyz, const int *cells, double *out);
+ * mesh.  quads (may be NULL): 3 quadrangles per face, 4 nodes each, quad 3f+k starting at
+ * vertex k of face f (Tethex convert_triangles, src/Tethex.cpp:1104-1170); needed by rank 3
+ * (LinearQuadrangles) and the rank 2 centroids (QuadraticTriangles cells,
+ * src/InterpolatorCells.cpp:1866-1886).  n_tris = 0 drops the faces. */
+int fgpu_interp_set_triangles(fgpu_interp *interp, int n_tris, const int *tris, const int *tri2tet,
+                              const int *quads);
+/* SolutionReader::calc_full_interpolation, dim 2 (src/SolutionReader.cpp:43-90, 136-163).
+ * rank 1 LinearTriangles, rank 2 QuadraticTriangles, rank 3 LinearQuadrangles
+ * (src/InterpolatorCells.cpp:1662-1686, 1838-1860, 1925-1992): each point located among the
+ * faces (quads for rank 3) with the chained guess (negative: nearest centroid), then
+ * interpolated in the tet (hex) the face picks.  centroids != 0 (interp_centroids):
+ * locate_interp_centroid (:309-356, 453-461), an exact centroid match (squared distance
+ * < 1e-15) giving the mean of the cell's nodes, else -1 in all 5 values with the previous
+ * cell kept. */
+int fgpu_interp_surface_points(fgpu_interp *interp, int n, const double *xyz, int rank, int centroids, int *cells,
+                               double *out);
+/* SolutionReader::calc_interpolation, dim 2, atoms already mapped to faces (quads); with
+ * centroids every cell must be >= 0 (InterpolatorCells::interp_centroid) */
+int fgpu_interp_surface_cells(fgpu_interp *interp, int n, const double *xyz, int rank, int centroids,
+                              const int *cells, double *out);
 
 /* HeatSolver::assemble cell terms, implicit Euler (src/CurrentHeatSolver.cpp:116-152, 347-393;
  * PhysicalQuantities::sigma/kappa src/PhysicalQuantities.cpp:31-64, 173-185): with T_old the

@@ -164,7 +164,12 @@ void orc_interp_cells(const orc_lintet *L, const double *xyz, const int *tets, c
 typedef struct {
     int n;
     double *v0, *e1, *e2, *pv, *nrm, *maxd, *cen;   /* 3 per triangle (maxd 1) */
-    int *nbr_ptr, *nbr, *tri2tet;
+    int *nbr_ptr, *nbr, *tri2tet, *tris;
+    /* set by orc_lintri_set_quads: 3 quadrangles per triangle (quad 3f+k around vertex k),
+     * their centroids, the two hexes of each quad (TetgenMesh::calc_quad2hex2quad_mapping,
+     * hex 4t+j around vertex j of tet t) and the 6-node triangles of QuadraticTriangles */
+    int *quads, *quad2hex, *tri6;
+    double *qcen;
 } orc_lintri;
 int  orc_lintri_build(orc_lintri *T, int n_nodes, const double *xyz, int n_tris, const int *tris, const int *tri2tet);
 void orc_lintri_free(orc_lintri *T);
@@ -172,13 +177,22 @@ int  orc_lintri_point_in_cell(const orc_lintri *T, const double p[3], int tri);
 void orc_lintri_shape_functions(const orc_lintri *T, const double p[3], int tri, double w[3]);
 double orc_lintri_fast_distance(const orc_lintri *T, const double p[3], int tri);
 int  orc_lintri_locate(const orc_lintri *T, const double p[3], int guess);
-/* SolutionReader::calc_full_interpolation, dim 2 rank 1: chained triangle locate, then
- * LinearTriangles::interp_solution (the point is interpolated in a tet: tets[0] when on the
- * triangle's plane or inside tets[0], else LinearTetrahedra::locate_cell from tets[1]) */
-void orc_interp_surface(const orc_lintri *T, const orc_lintet *L, const int *tets, const double *sol5, int n_points,
-                        const double *pts, int *cells, double *out);
-void orc_interp_surface_cells(const orc_lintri *T, const orc_lintet *L, const int *tets, const double *sol5,
-                              int n_points, const double *pts, const int *cells, double *out);
+/* Quadrangles of the surface triangles (Tethex convert_triangles, src/Tethex.cpp:1104-1170) with
+ * the tets of the mesh and the node kinds (edge centroids = 2) */
+int  orc_lintri_set_quads(orc_lintri *T, int n_nodes, const double *xyz, const int *node_kind, const int *tets,
+                          const int *tris, const int *quads);
+/* SolutionReader::calc_full_interpolation, dim 2 (src/SolutionReader.cpp:43-90, 136-163).
+ * rank 1 LinearTriangles, rank 2 QuadraticTriangles, rank 3 LinearQuadrangles
+ * (src/InterpolatorCells.cpp:1662-1686, 1838-1860, 1925-1992): chained face (or quad) locate,
+ * then interpolation in the tet (hex) the face picks: tets[0] when on the face plane or inside
+ * tets[0], else the tet (hex) search from tets[1].  centroids != 0: locate_interp_centroid
+ * (:309-356, 453-461): exact centroid match, else Solution(-1) and the previous cell kept. */
+void orc_interp_surface(const orc_lintri *T, const orc_lintet *L, const double *xyz, const int *tets,
+                        const int *cells10, const int *hexes, const double *sol5, int n_points, const double *pts,
+                        int rank, int centroids, int *cells, double *out);
+void orc_interp_surface_cells(const orc_lintri *T, const orc_lintet *L, const double *xyz, const int *tets,
+                              const int *cells10, const int *hexes, const double *sol5, int n_points,
+                              const double *pts, int rank, int centroids, const int *cells, double *out);
 
 /* physical quantities: resistivity table, sigma, kappa (src/PhysicalQuantities.cpp:31-64,
  * 173-185; table src/PhysicalQuantityData.cpp:47-76) */

@@ -58,7 +58,8 @@ _U8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 class _Lintri(C.Structure):
     _fields_ = [("n", C.c_int)] + [(k, C.POINTER(C.c_double)) for k in ("v0", "e1", "e2", "pv", "nrm", "maxd", "cen")] + \
-               [(k, C.POINTER(C.c_int)) for k in ("nbr_ptr", "nbr", "tri2tet")]
+               [(k, C.POINTER(C.c_int)) for k in ("nbr_ptr", "nbr", "tri2tet", "tris", "quads", "quad2hex", "tri6")] + \
+               [("qcen", C.POINTER(C.c_double))]
 
 
 def _declare(L):
@@ -100,8 +101,11 @@ def _declare(L):
     L.orc_lintri_shape_functions.argtypes = [C.POINTER(_Lintri), _D, C.c_int, _D]
     L.orc_lintri_fast_distance.argtypes = [C.POINTER(_Lintri), _D, C.c_int]
     L.orc_lintri_fast_distance.restype = C.c_double
-    L.orc_interp_surface.argtypes = [C.POINTER(_Lintri), C.POINTER(_Lintet), _I, _D, C.c_int, _D, _I, _D]
-    L.orc_interp_surface_cells.argtypes = [C.POINTER(_Lintri), C.POINTER(_Lintet), _I, _D, C.c_int, _D, _I, _D]
+    L.orc_lintri_set_quads.argtypes = [C.POINTER(_Lintri), C.c_int, _D, _I, _I, _I, _I]
+    L.orc_interp_surface.argtypes = [C.POINTER(_Lintri), C.POINTER(_Lintet), _D, _I, _I, _I, _D, C.c_int, _D,
+                                     C.c_int, C.c_int, _I, _D]
+    L.orc_interp_surface_cells.argtypes = [C.POINTER(_Lintri), C.POINTER(_Lintet), _D, _I, _I, _I, _D, C.c_int, _D,
+                                           C.c_int, C.c_int, _I, _D]
     for f in ("orc_resistivity", "orc_sigma"):
         getattr(L, f).argtypes = [C.c_double]
         getattr(L, f).restype = C.c_double
@@ -383,7 +387,7 @@ def interp_cells(L: "LinTet", nodes, tets, cells10, hexes, sol5, points, rank, c
 class LinTri:
     """LinearTriangles on the surface triangles (3 nodes each) with their two tets."""
 
-    def __init__(self, nodes, tris, tri2tet):
+    def __init__(self, nodes, tris, tri2tet, quads=None, node_kind=None, tets=None):
         self._T = _Lintri()
         lib().orc_lintri_build(C.byref(self._T), len(nodes), _f64(nodes), len(tris), _i32(tris), _i32(tri2tet))
         T = self._T
@@ -391,6 +395,14 @@ class LinTri:
         self.centroid = _arr(T.cen, 3 * T.n, np.float64).reshape(-1, 3)
         self.nbr_ptr = _arr(T.nbr_ptr, T.n + 1, np.int32)
         self.nbr = _arr(T.nbr, int(self.nbr_ptr[-1]), np.int32)
+        if quads is not None:
+            rc = lib().orc_lintri_set_quads(C.byref(self._T), len(nodes), _f64(nodes), _i32(node_kind), _i32(tets),
+                                            _i32(tris), _i32(quads))
+            if rc != 0:
+                raise ValueError("a quad's vertex is not a vertex of the face's tets")
+            self.quad_centroid = _arr(T.qcen, 9 * T.n, np.float64).reshape(-1, 3)
+            self.quad2hex = _arr(T.quad2hex, 6 * T.n, np.int32).reshape(-1, 2)
+            self.tri6 = _arr(T.tri6, 6 * T.n, np.int32).reshape(-1, 6)
 
     def __del__(self):
         try:
@@ -412,17 +424,24 @@ class LinTri:
     def fast_distance(self, p, tri):
         return lib().orc_lintri_fast_distance(C.byref(self._T), _f64(p), tri)
 
-    def interp(self, L: "LinTet", tets, sol5, points, cells=None):
-        """dim 2 rank 1: (cells, Solution rows); with cells given, the mapped-atom path."""
+    def interp(self, L: "LinTet", tets, sol5, points, cells=None, rank=1, centroids=False, xyz=None, cells10=None,
+               hexes=None):
+        """dim 2: (cells, Solution rows); with cells given, the mapped-atom path.  Ranks 2 and 3
+        need the quadratic tets (cells10) and the hexes with their nodes (xyz)."""
         pts = _f64(points)
         n = len(pts) // 3
         out = np.zeros(5 * n)
+        z = np.zeros(1)
+        zi = np.zeros(1, np.int32)
+        args = (C.byref(self._T), C.byref(L._L), z if xyz is None else _f64(xyz), _i32(tets),
+                zi if cells10 is None else _i32(cells10), zi if hexes is None else _i32(hexes), _f64(sol5), n, pts,
+                int(rank), int(bool(centroids)))
         if cells is None:
             cells = np.zeros(n, np.int32)
-            lib().orc_interp_surface(C.byref(self._T), C.byref(L._L), _i32(tets), _f64(sol5), n, pts, cells, out)
+            lib().orc_interp_surface(*args, cells, out)
         else:
             cells = _i32(cells)
-            lib().orc_interp_surface_cells(C.byref(self._T), C.byref(L._L), _i32(tets), _f64(sol5), n, pts, cells, out)
+            lib().orc_interp_surface_cells(*args, cells, out)
         return cells, out.reshape(-1, 5)
 
 

@@ -512,6 +512,8 @@ int orc_lintri_build(orc_lintri *T, int n_nodes, const double *xyz, int nt, cons
     T->nrm = malloc(sizeof(double) * 3 * m); T->cen = malloc(sizeof(double) * 3 * m);
     T->maxd = malloc(sizeof(double) * m); T->tri2tet = malloc(sizeof(int) * 2 * m);
     memcpy(T->tri2tet, tri2tet, sizeof(int) * 2 * nt);
+    T->tris = malloc(sizeof(int) * 3 * m);
+    memcpy(T->tris, tris, sizeof(int) * 3 * nt);
     /* node -> triangles, increasing */
     int *cnt = calloc(n_nodes + 1, sizeof(int));
     for (int i = 0; i < 3 * nt; ++i) cnt[tris[i] + 1]++;
@@ -566,7 +568,8 @@ int orc_lintri_build(orc_lintri *T, int n_nodes, const double *xyz, int nt, cons
 
 void orc_lintri_free(orc_lintri *T) {
     free(T->v0); free(T->e1); free(T->e2); free(T->pv); free(T->nrm); free(T->maxd); free(T->cen);
-    free(T->nbr_ptr); free(T->nbr); free(T->tri2tet);
+    free(T->nbr_ptr); free(T->nbr); free(T->tri2tet); free(T->tris);
+    free(T->quads); free(T->quad2hex); free(T->tri6); free(T->qcen);
     memset(T, 0, sizeof(*T));
 }
 
@@ -616,7 +619,49 @@ int orc_lintri_locate(const orc_lintri *T, const double p[3], int guess) {
     return -best_i;
 }
 
-/* LinearTriangles::interp_solution: the tet the point is interpolated in */
+/* quadrangles of the surface faces; hex of tet t sharing the quad = 4t + position of the face
+ * vertex in t (the only hex of t holding that vertex) */
+int orc_lintri_set_quads(orc_lintri *T, int n_nodes, const double *xyz, const int *node_kind, const int *tets,
+                         const int *tris, const int *quads) {
+    (void)n_nodes;
+    const int nq = 3 * T->n;
+    const size_t m = nq ? nq : 1;
+    T->quads = malloc(sizeof(int) * 4 * m);
+    T->quad2hex = malloc(sizeof(int) * 2 * m);
+    T->tri6 = malloc(sizeof(int) * 6 * (T->n ? T->n : 1));
+    T->qcen = malloc(sizeof(double) * 3 * m);
+    memcpy(T->quads, quads, sizeof(int) * 4 * nq);
+    for (int q = 0; q < nq; ++q) {
+        /* TetgenCells<4>::get_centroid: sum from zero, times 1/4 */
+        double c[3] = {0, 0, 0};
+        for (int k = 0; k < 4; ++k)
+            for (int a = 0; a < 3; ++a) c[a] += xyz[3 * quads[4 * q + k] + a];
+        for (int a = 0; a < 3; ++a) T->qcen[3 * q + a] = c[a] * (1.0 / 4);
+        const int f = q / 3, v = tris[3 * f + q % 3];
+        for (int r = 0; r < 2; ++r) {
+            const int t = T->tri2tet[2 * f + r];
+            int j = 0;
+            while (j < 4 && tets[4 * t + j] != v) ++j;
+            if (j == 4) return -1;
+            T->quad2hex[2 * q + r] = 4 * t + j;
+        }
+    }
+    /* QuadraticTriangles::calc_cell: edge-centroid nodes shared by quads (0,1), (1,2), (2,0) */
+    for (int f = 0; f < T->n; ++f) {
+        int e[3][4], ne[3] = {0, 0, 0};
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 4; ++k) {
+                const int node = quads[4 * (3 * f + i) + k];
+                if (node_kind[node] == 2) e[i][ne[i]++] = node;
+            }
+        for (int k = 0; k < 3; ++k) T->tri6[6 * f + k] = tris[3 * f + k];
+        for (int i = 0; i < 3; ++i) T->tri6[6 * f + 3 + i] = common_entry(e[i], ne[i], e[(i + 1) % 3], ne[(i + 1) % 3]);
+    }
+    return 0;
+}
+
+/* LinearTriangles/QuadraticTriangles::interp_solution: the tet the point is interpolated in
+ * (the quadratic tets share the linear tets' predicates and search) */
 static int surface_tet(const orc_lintri *T, const orc_lintet *L, const double p[3], int tri) {
     const int t0 = T->tri2tet[2 * tri], t1 = T->tri2tet[2 * tri + 1];
     if (fabs(orc_lintri_fast_distance(T, p, tri)) <= 100.0 * ZERO) return t0;
@@ -624,22 +669,107 @@ static int surface_tet(const orc_lintri *T, const orc_lintet *L, const double p[
     return orc_lintet_locate(L, p, t1);
 }
 
-void orc_interp_surface(const orc_lintri *T, const orc_lintet *L, const int *tets, const double *sol5, int np,
-                        const double *pts, int *cells, double *out) {
-    int cell = -1;
-    for (int i = 0; i < np; ++i) {
-        const double *p = pts + 3 * i;
-        cell = orc_lintri_locate(T, p, abs(cell));
-        cells[i] = cell;
-        interp_one(L, NULL, tets, NULL, NULL, sol5, p, 1, surface_tet(T, L, p, abs(cell)), out + 5 * i);
+/* LinearHexahedra::point_in_cell (src/InterpolatorCells.cpp:1507-1527) */
+static int linhex_point_in_cell(const orc_lintet *L, const double p[3], int hex) {
+    double b[4];
+    orc_lintet_bcc(L, p, hex / 4, b);
+    if (!(b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[3] >= 0)) return 0;
+    switch (hex % 4) {
+        case 0: return b[0] >= b[1] && b[0] >= b[2] && b[0] >= b[3];
+        case 1: return b[1] >= b[0] && b[1] >= b[2] && b[1] >= b[3];
+        case 2: return b[2] >= b[0] && b[2] >= b[1] && b[2] >= b[3];
+        default: return b[3] >= b[0] && b[3] >= b[1] && b[3] >= b[2];
     }
 }
 
-void orc_interp_surface_cells(const orc_lintri *T, const orc_lintet *L, const int *tets, const double *sol5, int np,
-                              const double *pts, const int *cells, double *out) {
+/* LinearQuadrangles::interp_solution: the hex the point is interpolated in */
+static int surface_hex(const orc_lintri *T, const orc_lintet *L, const double p[3], int quad) {
+    const int h0 = T->quad2hex[2 * quad], h1 = T->quad2hex[2 * quad + 1];
+    if (fabs(orc_lintri_fast_distance(T, p, quad / 3)) <= 100.0 * ZERO) return h0;
+    if (linhex_point_in_cell(L, p, h0)) return h0;
+    return orc_linhex_locate(L, p, h1);
+}
+
+/* LinearQuadrangles::locate_cell (src/InterpolatorCells.cpp:1963-1992) */
+static int linquad_locate(const orc_lintri *T, const double p[3], int guess) {
+    int tri = orc_lintri_locate(T, p, guess / 3);
+    const int sign = tri < 0 ? -1 : 1;
+    tri = abs(tri);
+    double b[3];
+    orc_lintri_shape_functions(T, p, tri, b);
+    if (b[0] >= b[1] && b[0] >= b[2]) return sign * (3 * tri);
+    if (b[1] >= b[0] && b[1] >= b[2]) return sign * (3 * tri + 1);
+    if (b[2] >= b[0] && b[2] >= b[1]) return sign * (3 * tri + 2);
+    return -1;
+}
+
+/* InterpolatorCells::locate_centroid (src/InterpolatorCells.cpp:309-333) */
+static int locate_centroid(const double *cen, int n, const double p[3], int guess) {
+    if (guess >= 0 && dist2(p, cen + 3 * guess) < ZERO) return guess;
+    for (int c = 0; c < n; ++c)
+        if (dist2(p, cen + 3 * c) < ZERO) return c;
+    return -1;
+}
+
+/* InterpolatorCells::interp_centroid (:336-356): nodes weighted 1/n */
+static void interp_centroid(const int *nodes, int nn, const double *sol5, double *out) {
+    const double w = 1.0 / nn;
+    double vx = 0, vy = 0, vz = 0, s1 = 0, s2 = 0;
+    for (int i = 0; i < nn; ++i) {
+        const double *s = sol5 + 5 * nodes[i];
+        vx += s[0] * w;
+        vy += s[1] * w;
+        vz += s[2] * w;
+        s1 += s[3] * w;
+        s2 += s[4] * w;
+    }
+    out[0] = vx; out[1] = vy; out[2] = vz; out[3] = s1; out[4] = s2;
+}
+
+static void surface_value(const orc_lintri *T, const orc_lintet *L, const double *xyz, const int *tets,
+                          const int *cells10, const int *hexes, const double *sol5, const double p[3], int rank,
+                          int cell, double *out) {
+    const int c = abs(cell);
+    if (rank == 3) interp_one(L, xyz, tets, cells10, hexes, sol5, p, 3, surface_hex(T, L, p, c), out);
+    else interp_one(L, xyz, tets, cells10, hexes, sol5, p, rank, surface_tet(T, L, p, c), out);
+}
+
+static void centroid_value(const orc_lintri *T, const double *sol5, int rank, int cell, double *out) {
+    if (rank == 3) interp_centroid(T->quads + 4 * cell, 4, sol5, out);
+    else if (rank == 2) interp_centroid(T->tri6 + 6 * cell, 6, sol5, out);
+    else interp_centroid(T->tris + 3 * cell, 3, sol5, out);
+}
+
+void orc_interp_surface(const orc_lintri *T, const orc_lintet *L, const double *xyz, const int *tets,
+                        const int *cells10, const int *hexes, const double *sol5, int np, const double *pts,
+                        int rank, int centroids, int *cells, double *out) {
+    int cell = -1;
     for (int i = 0; i < np; ++i) {
         const double *p = pts + 3 * i;
-        interp_one(L, NULL, tets, NULL, NULL, sol5, p, 1, surface_tet(T, L, p, abs(cells[i])), out + 5 * i);
+        double *o = out + 5 * i;
+        if (centroids) {
+            const double *cen = rank == 3 ? T->qcen : T->cen;
+            const int r = locate_centroid(cen, rank == 3 ? 3 * T->n : T->n, p, cell);
+            if (r >= 0) {
+                cell = r;
+                centroid_value(T, sol5, rank, cell, o);
+            } else {
+                for (int k = 0; k < 5; ++k) o[k] = -1.0;
+            }
+        } else {
+            cell = rank == 3 ? linquad_locate(T, p, abs(cell)) : orc_lintri_locate(T, p, abs(cell));
+            surface_value(T, L, xyz, tets, cells10, hexes, sol5, p, rank, cell, o);
+        }
+        cells[i] = cell;
+    }
+}
+
+void orc_interp_surface_cells(const orc_lintri *T, const orc_lintet *L, const double *xyz, const int *tets,
+                              const int *cells10, const int *hexes, const double *sol5, int np,
+                              const double *pts, int rank, int centroids, const int *cells, double *out) {
+    for (int i = 0; i < np; ++i) {
+        if (centroids) centroid_value(T, sol5, rank, cells[i], out + 5 * i);
+        else surface_value(T, L, xyz, tets, cells10, hexes, sol5, pts + 3 * i, rank, cells[i], out + 5 * i);
     }
 }
 

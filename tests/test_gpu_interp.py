@@ -127,30 +127,60 @@ def _surface_points(tip, tris, n, seed):
     return pts[rng.permutation(len(pts))] if seed % 2 else pts
 
 
-def test_surface_faces_bit_exact(ctx, tip, solved):
-    """dim 2: located faces and values against the oracle's LinearTriangles, bit for bit."""
+@pytest.fixture(scope="module")
+def surface(ctx, tip, solved):
     _, s, _, _ = solved
     tris, t2t = tip.surface_triangles()
+    quads = tip.surface_quads(tris)
     ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
     ip.extract_solution(s, smoothen=True)
-    ip.set_triangles(tris, t2t)
-    sol = ip.nodal()
+    ip.set_triangles(tris, t2t, quads)
     L = O.LinTet(tip.nodes, tip.tets, tip.tet_nbr, tip.tet_marker, 3)
-    T = O.LinTri(tip.nodes, tris, t2t)
+    T = O.LinTri(tip.nodes, tris, t2t, quads, tip.node_kind, tip.tets)
+    q10 = O.quadtet_cells(tip.tets, tip.hexes, tip.node_kind)
+    return ip, T, L, dict(xyz=tip.nodes, cells10=q10, hexes=tip.hexes), tris
+
+
+@pytest.mark.parametrize("rank", [1, 2, 3])
+def test_surface_faces_bit_exact(tip, surface, rank):
+    """dim 2: located faces (quads for rank 3) and values against the oracle, bit for bit."""
+    ip, T, L, kw, tris = surface
+    sol = ip.nodal()
     for seed in (1, 2):
-        pts = _surface_points(tip, tris, 400, seed)
-        cells_o, out_o = T.interp(L, tip.tets, sol, pts)
-        cells_g, out_g = ip.surface_points(pts)
+        pts = _surface_points(tip, tris, 400, seed + 10 * rank)
+        cells_o, out_o = T.interp(L, tip.tets, sol, pts, rank=rank, **kw)
+        cells_g, out_g = ip.surface_points(pts, rank)
         assert np.array_equal(cells_g, cells_o)
         assert np.array_equal(out_g, out_o)
-        assert np.array_equal(ip.surface_cells(pts, cells_g), out_o)
+        assert np.array_equal(ip.surface_cells(pts, cells_g, rank), out_o)
         assert (cells_o < 0).any() and (cells_o > 0).any()
-    r = interp.SolutionReader(ip, rank=1, dim=2)
+    r = interp.SolutionReader(ip, rank=rank, dim=2)
     pts = _surface_points(tip, tris, 200, 7)
     got = r.calc_interpolation(pts)
-    cells_o, out_o = T.interp(L, tip.tets, sol, pts)
+    cells_o, out_o = T.interp(L, tip.tets, sol, pts, rank=rank, **kw)
     assert np.array_equal(r.markers, cells_o) and np.array_equal(got, out_o)
     assert np.array_equal(r.calc_interpolation(pts), out_o)
+
+
+@pytest.mark.parametrize("rank", [1, 2, 3])
+def test_surface_centroids_bit_exact(tip, surface, rank):
+    """interp_centroids: exact centroid matches (within 1e-9), misses keep the previous cell."""
+    ip, T, L, kw, tris = surface
+    sol = ip.nodal()
+    rng = np.random.default_rng(40 + rank)
+    cen = T.quad_centroid if rank == 3 else T.centroid
+    idx = rng.choice(len(cen), 300, replace=False)
+    pts = cen[idx] + rng.uniform(-1e-9, 1e-9, (300, 3))
+    pts[::5] += 1e-6
+    pts[:3] += 1.0                                   # leading misses: cell -1 until the first match
+    cells_o, out_o = T.interp(L, tip.tets, sol, pts, rank=rank, centroids=True, **kw)
+    cells_g, out_g = ip.surface_points(pts, rank, centroids=True)
+    assert np.array_equal(cells_g, cells_o) and np.array_equal(out_g, out_o)
+    assert cells_o[0] == -1 and (out_o == -1.0).all(1).sum() >= 60
+    r = interp.SolutionReader(ip, rank=rank, dim=2, interp_centroids=True)
+    assert np.array_equal(r.calc_interpolation(pts[3:]), out_o[3:])
+    assert np.array_equal(r.calc_interpolation(pts[3:]),
+                          T.interp(L, tip.tets, sol, pts[3:], r.markers, rank=rank, centroids=True, **kw)[1])
 
 
 def test_surface_faces_invalid_calls(ctx, tip):
@@ -166,3 +196,11 @@ def test_surface_faces_invalid_calls(ctx, tip):
     assert ip.surface_points(np.zeros((0, 3)))[0].size == 0
     with pytest.raises(fgpu.FgpuError):
         ip.surface_cells(np.zeros((1, 3)), np.array([len(tris)]))
+    with pytest.raises(fgpu.FgpuError):
+        ip.surface_points(np.zeros((1, 3)), rank=3)       # no quadrangles set
+    quads = tip.surface_quads(tris)
+    with pytest.raises(fgpu.FgpuError):
+        ip.set_triangles(tris, t2t, quads[:, ::-1])      # quads must start at their face vertex
+    ip.set_triangles(tris, t2t, quads)
+    with pytest.raises(fgpu.FgpuError):
+        ip.surface_cells(np.zeros((1, 3)), np.array([-2]), rank=3, centroids=True)

@@ -180,3 +180,86 @@ def test_surface_interpolation_of_linear_fields(tip, surface):
     ref = _solution(pts, lambda x: x @ A + 0.25)
     assert np.abs(out - ref).max() <= 1e-10 * np.abs(ref).max()
     assert np.array_equal(T.interp(L, m.tets, sol, pts, cells)[1], out)
+
+
+@pytest.fixture(scope="module")
+def surface_quads(tip):
+    m, L, q10 = tip
+    tris, t2t = m.surface_triangles()
+    quads = m.surface_quads(tris)
+    T = O.LinTri(m.nodes, tris, t2t, quads, m.node_kind, m.tets)
+    return tris, t2t, quads, T
+
+
+def test_surface_quads_and_quadratic_faces(tip, surface_quads):
+    """Quads: vertex, edge midpoint, face centroid, edge midpoint, all in both hexes mapped to
+    them; the 6-node faces carry the edge midpoints (0,1), (1,2), (2,0)."""
+    m, _, _ = tip
+    tris, t2t, quads, T = surface_quads
+    x = m.nodes
+    assert np.array_equal(m.node_kind[quads], np.tile([M.KIND_TETNODE, M.KIND_EDGE, M.KIND_FACE, M.KIND_EDGE],
+                                                      (len(quads), 1)))
+    assert np.array_equal(quads[:, 0], tris.ravel())
+    assert np.allclose(x[quads[:, 2]], x[np.repeat(tris, 3, 0)].mean(1), atol=1e-12)
+    for r in range(2):
+        hx = m.hexes[T.quad2hex[:, r]]
+        assert np.all((hx[:, :, None] == quads[:, None, :]).any(1))
+        assert np.array_equal(T.quad2hex[:, r] // 4, np.repeat(t2t[:, r], 3))
+    for k, (a, b) in enumerate([(0, 1), (1, 2), (2, 0)]):
+        assert np.array_equal(x[T.tri6[:, 3 + k]], 0.5 * (x[tris[:, a]] + x[tris[:, b]]))
+    assert np.array_equal(T.quad_centroid, x[quads].sum(1) * 0.25)
+
+
+@pytest.mark.parametrize("rank", [2, 3])
+def test_surface_interpolation_higher_ranks(tip, surface_quads, rank):
+    """Rank 2 reproduces quadratics and rank 3 linear fields at atoms near the surface; the
+    located quads are the faces' sub-quads of the largest barycentric weight."""
+    m, L, q10 = tip
+    tris, t2t, quads, T = surface_quads
+    rng = np.random.default_rng(20 + rank)
+    f = rng.choice(len(tris), 200)
+    pts = np.einsum("nk,nkd->nd", rng.dirichlet(np.ones(3), 200), m.nodes[tris[f]])
+    pts = pts + rng.normal(0, 0.05, pts.shape)
+    fn = (lambda x: 1e-3 * (x[..., 0] ** 2 - 2 * x[..., 1] * x[..., 2]) + x @ A) if rank == 2 else \
+        (lambda x: x @ A + 0.25)
+    sol = _solution(m.nodes, fn)
+    kw = dict(rank=rank, xyz=m.nodes, cells10=q10, hexes=m.hexes)
+    cells, out = T.interp(L, m.tets, sol, pts, **kw)
+    ref = _solution(pts, fn)
+    assert np.abs(out - ref).max() <= 1e-10 * np.abs(ref).max()
+    assert np.array_equal(T.interp(L, m.tets, sol, pts, cells, **kw)[1], out)
+    if rank == 3:
+        prev = -1
+        for p, c in zip(pts, cells):
+            t = T.locate(p, abs(prev) // 3)
+            w = T.shape_functions(p, abs(t))
+            assert c == np.sign(t if t else 1) * (3 * abs(t) + int(np.argmax(w)))
+            prev = c
+
+
+@pytest.mark.parametrize("rank", [1, 2, 3])
+def test_surface_centroid_mode(tip, surface_quads, rank):
+    """Atoms on (within 1e-9 of) a face or quad centroid get the mean of its nodes; others
+    get -1 and keep the previous cell."""
+    m, L, q10 = tip
+    tris, t2t, quads, T = surface_quads
+    rng = np.random.default_rng(30 + rank)
+    cen = T.quad_centroid if rank == 3 else T.centroid
+    nodes = quads if rank == 3 else (T.tri6 if rank == 2 else tris)
+    idx = rng.choice(len(cen), 100, replace=False)
+    pts = cen[idx] + rng.uniform(-1e-9, 1e-9, (100, 3))
+    pts[::7] += 1e-6                                     # off by more than sqrt(1e-15)
+    sol = rng.normal(size=(len(m.nodes), 5))
+    kw = dict(rank=rank, centroids=True, xyz=m.nodes, cells10=q10, hexes=m.hexes)
+    cells, out = T.interp(L, m.tets, sol, pts, **kw)
+    miss = np.zeros(100, bool)
+    miss[::7] = True
+    assert np.all(out[miss] == -1.0)
+    assert np.array_equal(cells[~miss], idx[~miss])
+    assert np.abs(out[~miss] - sol[nodes[idx[~miss]]].mean(1)).max() < 1e-14 * np.abs(sol).max() * 10
+    # a miss keeps the previous cell
+    prev = np.maximum.accumulate(np.where(miss, -1, np.arange(100)))
+    expect = np.where(prev >= 0, idx[np.maximum(prev, 0)], -1)
+    assert np.array_equal(cells, expect)
+    c2 = np.where(cells < 0, 0, cells)
+    assert np.array_equal(T.interp(L, m.tets, sol, pts, c2, **kw)[1][~miss], out[~miss])
