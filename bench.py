@@ -8,12 +8,13 @@ Workload: 16 MD clients, each with its own Cu nanotip (radius 3-12 nm, aspect ra
 exponential think time after its previous result came back, 200 requests/s offered in all.
 One step = one served solve; `value` is solves/s over the K timed requests.
 
-* headline: mixed precision (FP32 operator and CG inside FP64 defect correction), continuous
-  batching, meshes and last solutions resident on the GPU (warm starts);
+* headline: mixed precision (FP32 operator and CG inside FP64 defect correction), two-level
+  preconditioner (Jacobi + a coarse correction on 1024-dof blocks, built once per resident
+  mesh), continuous batching, meshes and last solutions resident on the GPU (warm starts);
 * `plain_fp64`: the same request stream through the plain form: FP64 CG, one solve at a time,
   from phi = 0, in arrival order.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--small]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--small] [--jacobi]
 """
 from __future__ import annotations
 
@@ -64,7 +65,7 @@ def make_clients(n: int, small: bool):
         return pool.map(_make_client, [(i, n, small) for i in range(n)])
 
 
-def serve(ctx, clients, precision, batched, warm, steps, warmup, tuned, seed):
+def serve(ctx, clients, precision, batched, warm, steps, warmup, tuned, seed, coarse=False):
     from femocs_amd import fgpu, serving
     o = fgpu.opts(tol=1e-9, relative=True, precision=precision, inner_tol=0.1, use_graph=False)
     cells, ppb = tuned
@@ -76,7 +77,7 @@ def serve(ctx, clients, precision, batched, warm, steps, warmup, tuned, seed):
     for c in clients:
         c.seq = 0
         c.rng = np.random.default_rng(1000 + c.cid)
-    srv = serving.Server(ctx, clients, o, batched=batched, warm=warm, configure=configure)
+    srv = serving.Server(ctx, clients, o, batched=batched, warm=warm, configure=configure, coarse=coarse)
     serving.run_clients(srv, rate=RATE, duration=1e9, max_requests=warmup, seed=seed)
     srv.stats = serving.ServeStats()
     ctx.sync()
@@ -135,6 +136,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
     ap.add_argument("--small", action="store_true", help="4 small tips instead (quick check)")
+    ap.add_argument("--jacobi", action="store_true", help="mixed run without the two-level preconditioner")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the field solver has no multi-GPU path "
@@ -165,7 +167,8 @@ def main():
     if not args.no_plain:
         _, plain = serve(ctx, clients, fgpu.FGPU_FP64, False, False, args.steps, args.warmup, tuned["fp64"], 1)
         log("plain fp64", plain)
-    srv, mixed = serve(ctx, clients, fgpu.FGPU_MIXED, True, True, args.steps, args.warmup, tuned["mixed"], 1)
+    srv, mixed = serve(ctx, clients, fgpu.FGPU_MIXED, True, True, args.steps, args.warmup, tuned["mixed"], 1,
+                       coarse=not args.jacobi)
     log("mixed batched", mixed)
 
     # dominant kernel: the FP32 element apply over the whole batch (all clients resident).
@@ -184,12 +187,14 @@ def main():
     byts = 32 * H + (24 * H if affine else 12 * N) + 8 * N + 4 * N
     gbs = byts / (ms * 1e-3) / 1e9
     gather_ms = b.time_kernel(1, reps=30)
+    coarse_ms = b.time_kernel(2, reps=30) if not args.jacobi else None
     del b
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_patch<float>",
                 "avg_us": round(ms * 1e3, 2), "algorithmic_bytes": byts, "cells": H, "dofs": N,
                 "tet_split_form": affine,
-                "gather_avg_us": round(gather_ms * 1e3, 2)}
+                "gather_avg_us": round(gather_ms * 1e3, 2),
+                "coarse_avg_us": round(coarse_ms * 1e3, 2) if coarse_ms is not None else None}
     log("roofline", roofline)
 
     errs = fp64_errors(srv)
@@ -218,7 +223,9 @@ def main():
         "config": {"workload": f"{n} MD clients, Cu nanotips r=3-12 nm, {min(hexes)}-{max(hexes)} Q1 hexes each, "
                                f"Poisson solve per timestep, closed-loop arrivals {RATE:.0f} req/s offered",
                    "clients": n, "total_dofs": int(sum(dofs)), "total_hexes": int(sum(hexes)),
-                   "tol_rel": 1e-9, "precond": "jacobi", "parallelism": "single GPU"},
+                   "tol_rel": 1e-9,
+                   "precond": "jacobi" if args.jacobi else "two-level: jacobi + 1024-dof block aggregates",
+                   "precond_plain_fp64": "jacobi", "parallelism": "single GPU"},
         "latency_ms": {"p50": round(mixed["p50_ms"], 2), "p99": round(mixed["p99_ms"], 2)},
         "mixed_batched": mixed,
         "plain_fp64": plain,

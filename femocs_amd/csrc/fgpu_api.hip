@@ -121,7 +121,7 @@ fgpu::Sys::~Sys() {
     dfree(b); dfree(x); dfree(r); dfree(z); dfree(p); dfree(q); dfree(Ye); dfree(invd); dfree(diag);
     dfree(g); dfree(tmp); dfree(kq); dfree(gq); dfree(fslot); dfree(fval); dfree(d_dmask);
     dfree(r32); dfree(z32); dfree(p32); dfree(q32); dfree(d32); dfree(Ye32); dfree(invd32);
-    dfree(kq32); dfree(gq32); dfree(scal); dfree(part);
+    dfree(kq32); dfree(gq32); dfree(scal); dfree(part); dfree(cinv); dfree(cc);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -584,8 +584,8 @@ extern "C" int fgpu_sys_device_bytes(const fgpu_sys *s, size_t *bytes) {
     if (!s || !bytes) return FGPU_EINVAL;
     const fgpu_mesh *m = static_cast<const fgpu_mesh *>(s->mesh);
     const size_t nd = m->n_dofs, ns = m->n_slots, nb = m->bf_cell.size(), nc = m->n_cells;
-    *bytes = nd * (11 * 8 + 1 + 6 * 4) + ns * (8 + 4) + nb * 5 * 8 + 8 * (size_t)s->nb_dof * 8 + 2 * kScal * 8 +
-             (s->kq ? nc * 8 * (8 + 8 + 4 + 4) : 0);
+    *bytes = nd * (11 * 8 + 1 + 6 * 4) + ns * (8 + 4) + nb * 5 * 8 + 2 * kPartRegions * (size_t)s->nb_dof * 8 +
+             2 * kScal * 8 + (s->kq ? nc * 8 * (8 + 8 + 4 + 4) : 0) + (size_t)s->nc * (s->nc + 1) * 4;
     return FGPU_OK;
 }
 
@@ -681,7 +681,9 @@ static void ensure_metric(fgpu_mesh *m) {
     m->metric_gen = m->coord_gen;
 }
 
-static Desc make_desc32(fgpu_sys *s) {
+// coarse: the two-level preconditioner of the system's FP32 CG, when set up (only the CG
+// iteration descriptors carry it; plain applies must not add the coarse correction)
+static Desc make_desc32(fgpu_sys *s, bool coarse = false) {
     ensure_layout(s);
     Desc d{};
     fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
@@ -721,7 +723,12 @@ static Desc make_desc32(fgpu_sys *s) {
     d.invdiag = s->invd32;
     d.b = nullptr;
     d.scal = s->scal + kScal;
-    d.part = s->part + 4 * (size_t)s->nb_dof;
+    d.part = s->part + kPartRegions * (size_t)s->nb_dof;
+    if (coarse && s->coarse) {
+        d.cc = s->cc;
+        d.cinv = s->cinv;
+        d.nc = s->nc;
+    }
     return d;
 }
 
@@ -746,9 +753,10 @@ extern "C" int fgpu_sys_create(fgpu_mesh *m, fgpu_sys **out) {
         s->d32 = dalloc<float>(n); s->invd32 = dalloc<float>(n);
         ensure_layout(s);
         s->scal = dalloc<double>(2 * kScal);
-        s->part = dalloc<double>(8 * (size_t)s->nb_dof);
+        s->part = dalloc<double>(2 * kPartRegions * (size_t)s->nb_dof);
         hipStream_t st = m->ctx->stream;
         HIPCHECK(hipMemsetAsync(s->scal, 0, 2 * kScal * sizeof(double), st));
+        HIPCHECK(hipMemsetAsync(s->part, 0, 2 * kPartRegions * (size_t)s->nb_dof * sizeof(double), st));
         HIPCHECK(hipMemsetAsync(s->b, 0, n * sizeof(double), st));
         HIPCHECK(hipMemsetAsync(s->x, 0, n * sizeof(double), st));
         HIPCHECK(hipMemsetAsync(s->g, 0, n * sizeof(double), st));
@@ -809,8 +817,8 @@ static void *ensure_tab(fgpu_ctx *c, size_t bytes) {
 // [ndesc_sets x n Desc][elem table][dof table]
 struct BatchTabs {
     std::vector<const Desc *> sets;  // device pointers of each descriptor set
-    const int2 *etab = nullptr, *dtab = nullptr;
-    int ne = 0, nd = 0;
+    const int2 *etab = nullptr, *dtab = nullptr, *ctab = nullptr;
+    int ne = 0, nd = 0, nct = 0;   // nct: coarse-product workgroups (systems with a coarse level)
     size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the element kernel, largest patch of the batch
 };
 
@@ -822,21 +830,25 @@ static size_t patch_lds(int node_cap, int cells, size_t tsize, bool affine = fal
 
 static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> &sets, const std::vector<fgpu_sys *> &sys,
                               void **own_buf = nullptr, size_t *own_cap = nullptr) {
-    std::vector<int2> et, dt;
+    std::vector<int2> et, dt, ct;
     for (size_t i = 0; i < sys.size(); ++i) {
         append_table(et, (int)i, sys[i]->nb_cell);
         append_table(dt, (int)i, sys[i]->nb_dof);
+        if (sys[i]->coarse)
+            for (int b = 0; b < (sys[i]->nc + kCoarseRows - 1) / kCoarseRows; ++b) ct.push_back(make_int2((int)i, b));
     }
     const size_t n = sys.size();
     const size_t dbytes = sets.size() * n * sizeof(Desc);
     const size_t align = 256;
     const size_t off_e = (dbytes + align - 1) / align * align;
     const size_t off_d = off_e + (et.size() * sizeof(int2) + align - 1) / align * align;
-    const size_t total = off_d + dt.size() * sizeof(int2);
+    const size_t off_c = off_d + (dt.size() * sizeof(int2) + align - 1) / align * align;
+    const size_t total = off_c + ct.size() * sizeof(int2);
     std::vector<char> host(total);
     for (size_t k = 0; k < sets.size(); ++k) std::memcpy(host.data() + k * n * sizeof(Desc), sets[k].data(), n * sizeof(Desc));
     std::memcpy(host.data() + off_e, et.data(), et.size() * sizeof(int2));
     std::memcpy(host.data() + off_d, dt.data(), dt.size() * sizeof(int2));
+    if (!ct.empty()) std::memcpy(host.data() + off_c, ct.data(), ct.size() * sizeof(int2));
     char *dev;
     if (own_buf) {
         if (total > *own_cap) {
@@ -858,8 +870,10 @@ static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> 
     for (size_t k = 0; k < sets.size(); ++k) t.sets.push_back(reinterpret_cast<const Desc *>(dev + k * n * sizeof(Desc)));
     t.etab = reinterpret_cast<const int2 *>(dev + off_e);
     t.dtab = reinterpret_cast<const int2 *>(dev + off_d);
+    t.ctab = reinterpret_cast<const int2 *>(dev + off_c);
     t.ne = (int)et.size();
     t.nd = (int)dt.size();
+    t.nct = (int)ct.size();
     for (auto *x : sys) {
         const fgpu_mesh *m = static_cast<const fgpu_mesh *>(x->mesh);
         t.lds64 = std::max(t.lds64, patch_lds(m->patches.node_cap, m->patches.cells, sizeof(double)));
@@ -1186,6 +1200,173 @@ extern "C" int fgpu_sys_apply(fgpu_sys *s, const double *x, double *y) {
     });
 }
 
+// ------------------------------------------------------------------------------------------
+// two-level preconditioner of the mixed solver
+// ------------------------------------------------------------------------------------------
+// E = Z^T A Z for the system's current operator and constraints, Z the free-dof indicator of
+// the blocks of kDofBlock internal dofs.  The block graph (blocks sharing a cell) is coloured
+// so that no two blocks of a colour are within distance 2; A applied to the indicator of all
+// blocks of one colour then gives, summed over the free dofs of a block b1, E[b1][b2] for the
+// only block b2 of that colour in b1's closed neighbourhood.  One FP64 apply per colour, all
+// on the device; E^-1 by Gauss-Jordan sweeps on the device, stored in FP32.
+static int build_coarse(fgpu_sys *s) {
+    fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+    hipStream_t st = s->ctx->stream;
+    const int n = m->n_dofs;
+    const int nc = (n + kDofBlock - 1) / kDofBlock;
+    // block graph
+    std::vector<int64_t> pairs;
+    for (int c = 0; c < m->n_cells; ++c) {
+        int b[8], nb = 0;
+        for (int k = 0; k < 8; ++k) {
+            const int x = m->idof_of_dof[m->cells_lex[8 * (size_t)c + k]] / kDofBlock;
+            bool seen = false;
+            for (int j = 0; j < nb; ++j) seen = seen || b[j] == x;
+            if (!seen) b[nb++] = x;
+        }
+        for (int k = 0; k < nb; ++k)
+            for (int l = 0; l < nb; ++l)
+                if (k != l) pairs.push_back((int64_t)b[k] * nc + b[l]);
+    }
+    std::sort(pairs.begin(), pairs.end());
+    pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+    std::vector<int> nptr(nc + 1, 0), nbr;
+    for (int64_t pr : pairs) nptr[pr / nc + 1]++;
+    for (int b = 0; b < nc; ++b) nptr[b + 1] += nptr[b];
+    nbr.resize(pairs.size());
+    for (size_t i = 0; i < pairs.size(); ++i) nbr[i] = (int)(pairs[i] % nc);   // sorted: rows in order
+    // greedy colouring of the square of the graph
+    std::vector<int> color(nc, -1), stamp;
+    int ncol = 0;
+    for (int b = 0; b < nc; ++b) {
+        auto mark = [&](int x) {
+            if (color[x] >= 0) {
+                if ((int)stamp.size() <= color[x]) stamp.resize(color[x] + 1, -1);
+                stamp[color[x]] = b;
+            }
+        };
+        mark(b);
+        for (int e = nptr[b]; e < nptr[b + 1]; ++e) {
+            const int a = nbr[e];
+            mark(a);
+            for (int f = nptr[a]; f < nptr[a + 1]; ++f) mark(nbr[f]);
+        }
+        int c = 0;
+        while (c < (int)stamp.size() && stamp[c] == b) ++c;
+        color[b] = c;
+        ncol = std::max(ncol, c + 1);
+    }
+    // block sums of A (indicator of each colour)
+    int *d_color = dalloc<int>(nc);
+    double *d_sums = dalloc<double>((size_t)ncol * nc);
+    h2d(d_color, color.data(), nc, st);
+    Desc d = make_desc64(s, true);
+    d.ua = s->tmp;
+    d.q = s->q;
+    BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
+    for (int c = 0; c < ncol; ++c) {
+        launch_coarse_indicator(st, n, s->d_dmask, d_color, c, s->tmp);
+        launch_elem<double>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds64);
+        launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, false);
+        launch_coarse_sums(st, n, s->d_dmask, s->q, d_sums + (size_t)c * nc);
+    }
+    HIPCHECK(hipGetLastError());
+    std::vector<double> sums((size_t)ncol * nc);
+    d2h(sums.data(), d_sums, sums.size(), st);
+    std::vector<unsigned char> mk(n);
+    d2h(mk.data(), s->d_dmask, n, st);
+    HIPCHECK(hipStreamSynchronize(st));
+    dfree(d_color);
+    dfree(d_sums);
+    std::vector<int> nfree(nc, 0);
+    for (int i = 0; i < n; ++i) nfree[i / kDofBlock] += mk[i] ? 0 : 1;
+    std::vector<double> E((size_t)nc * nc, 0.0);
+    for (int b1 = 0; b1 < nc; ++b1) {
+        E[(size_t)b1 * nc + b1] = sums[(size_t)color[b1] * nc + b1];
+        for (int e = nptr[b1]; e < nptr[b1 + 1]; ++e) {
+            const int b2 = nbr[e];
+            E[(size_t)b1 * nc + b2] = sums[(size_t)color[b2] * nc + b1];
+        }
+    }
+    for (int b1 = 0; b1 < nc; ++b1)
+        for (int b2 = b1 + 1; b2 < nc; ++b2) {
+            const double v = 0.5 * (E[(size_t)b1 * nc + b2] + E[(size_t)b2 * nc + b1]);
+            E[(size_t)b1 * nc + b2] = E[(size_t)b2 * nc + b1] = v;
+        }
+    // blocks without free dofs: a decoupled unit row (their block sums of r are 0)
+    for (int b = 0; b < nc; ++b)
+        if (nfree[b] == 0) {
+            for (int j = 0; j < nc; ++j) E[(size_t)b * nc + j] = E[(size_t)j * nc + b] = 0.0;
+            E[(size_t)b * nc + b] = 1.0;
+        } else if (!(E[(size_t)b * nc + b] > 0.0)) {
+            throw std::runtime_error("coarse operator is not positive definite");
+        }
+    double *dA = dalloc<double>((size_t)nc * nc), *dB = dalloc<double>((size_t)nc * nc);
+    h2d(dA, E.data(), E.size(), st);
+    for (int k = 0; k < nc; ++k) {
+        launch_gj_step(st, nc, k, dA, dB);
+        std::swap(dA, dB);
+    }
+    // the inverse of an SPD matrix has a positive diagonal
+    std::vector<double> dg(nc);
+    HIPCHECK(hipMemcpy2DAsync(dg.data(), sizeof(double), dA, (nc + 1) * sizeof(double), sizeof(double), nc,
+                              hipMemcpyDeviceToHost, st));
+    dfree(s->cinv);
+    dfree(s->cc);
+    s->cinv = dalloc<float>((size_t)nc * nc);
+    s->cc = dalloc<float>(nc);
+    launch_to_float(st, (size_t)nc * nc, dA, s->cinv);
+    HIPCHECK(hipMemsetAsync(s->cc, 0, nc * sizeof(float), st));
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    dfree(dA);
+    dfree(dB);
+    for (double v : dg)
+        if (!(v > 0.0) || !std::isfinite(v)) {
+            dfree(s->cinv);
+            dfree(s->cc);
+            throw std::runtime_error("coarse operator inversion failed");
+        }
+    s->nc = nc;
+    s->coarse = true;
+    return ncol;
+}
+
+extern "C" int fgpu_sys_set_coarse(fgpu_sys *s, int enable, int *colors) {
+    if (!s) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        if (s->batches > 0) throw std::invalid_argument("the system is in a running batch");
+        HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+        s->coarse = false;
+        s->nc = 0;
+        dfree(s->cinv);
+        dfree(s->cc);
+        if (colors) *colors = 0;
+        if (!enable) return FGPU_OK;
+        if (!s->lifted) throw std::invalid_argument("apply the Dirichlet conditions before building the coarse level");
+        const int ncol = build_coarse(s);
+        if (colors) *colors = ncol;
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_sys_get_coarse(fgpu_sys *s, int *nc, int *block, float *einv) {
+    if (!s || !nc) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        const fgpu_mesh *m = static_cast<const fgpu_mesh *>(s->mesh);
+        *nc = s->coarse ? s->nc : 0;
+        if (block)
+            for (int d = 0; d < m->n_dofs; ++d) block[d] = m->idof_of_dof[d] / kDofBlock;
+        if (einv && s->coarse) {
+            HIPCHECK(hipSetDevice(s->ctx->device));
+            d2h(einv, s->cinv, (size_t)s->nc * s->nc, s->ctx->stream);
+            HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+        }
+        return FGPU_OK;
+    });
+}
+
 extern "C" int fgpu_sys_minmax(fgpu_sys *s, double *mn, double *mx) {
     if (!s) return FGPU_EINVAL;
     return guarded(s->ctx, [&] {
@@ -1251,6 +1432,7 @@ static void cg_iter(hipStream_t st, const Desc *D, const BatchTabs &t, int n, bo
     launch_gather_cg<T>(st, D, t.dtab, t.nd);
     launch_scal_alpha(st, D, n);
     launch_update_cg<T>(st, D, t.dtab, t.nd);
+    if (sizeof(T) == 4) launch_coarse(st, D, nullptr, t.ctab, t.nct, 0);
     launch_scal_beta(st, D, n, reliable);
 }
 
@@ -1268,6 +1450,7 @@ static void reliable_update(hipStream_t st, const Desc *D32, const Desc *R64, co
     launch_mixed_acc(st, D32, R64, t.dtab, t.nd);
     fp64_residual(st, R64, D32, t, n, coef, 1);
     launch_mixed_restart(st, D32, R64, t.dtab, t.nd, 0);
+    launch_coarse(st, D32, R64, t.ctab, t.nct, 2);
     launch_scal_restart(st, D32, R64, n, 0);
 }
 
@@ -1305,7 +1488,7 @@ static void make_sets(const std::vector<fgpu_sys *> &sys, std::vector<std::vecto
     for (auto *s : sys) {
         sets[0].push_back(make_desc64(s, false));
         sets[1].push_back(make_desc64(s, true));
-        sets[2].push_back(make_desc32(s));
+        sets[2].push_back(make_desc32(s, true));
     }
 }
 
@@ -1337,6 +1520,7 @@ static void start_solves(fgpu_ctx *c, const std::vector<fgpu_sys *> &sys, const 
     fp64_residual(c->stream, R64, D32, t, n, coef, mixed ? 1 : 0);
     if (mixed) {
         launch_mixed_restart(c->stream, D32, R64, t.dtab, t.nd, 1);
+        launch_coarse(c->stream, D32, R64, t.ctab, t.nct, 1);
         launch_scal_restart(c->stream, D32, R64, n, 1);
     }
     HIPCHECK(hipStreamSynchronize(c->stream));  // h is consumed
@@ -1441,6 +1625,7 @@ extern "C" void fgpu_batch_destroy(fgpu_batch *b) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     batch_drop_graph(b);
+    for (auto *s : b->sys) s->batches--;
     if (b->dtab) (void)hipFree(b->dtab);
     if (b->dflags) (void)hipFree(b->dflags);
     delete b;
@@ -1465,6 +1650,7 @@ extern "C" int fgpu_batch_add(fgpu_batch *b, fgpu_sys *s) {
         start_solves(c, {s}, &b->o, t, coef);
         b->coef = coef;
         b->sys.push_back(s);
+        s->batches++;
         b->dirty = true;
         return FGPU_OK;
     });
@@ -1546,6 +1732,7 @@ extern "C" int fgpu_batch_retire(fgpu_batch *b, int max, fgpu_sys **out, fgpu_so
             HIPCHECK(hipMemcpy(&done, s->scal + S_DONE, sizeof(double), hipMemcpyDeviceToHost));
             if (done != 0.0 && *n_out < max) {
                 out[*n_out] = s;
+                s->batches--;
                 if (stats) read_stats(s, mixed, &stats[*n_out]);
                 ++*n_out;
             } else {
@@ -1575,11 +1762,15 @@ extern "C" int fgpu_sys_solve(fgpu_sys *s, const fgpu_solve_opts *o, fgpu_solve_
 // kernel timing (benchmark helper): average duration of `reps` back-to-back launches of one
 // kernel of the solve, measured with HIP events on the context stream
 // ------------------------------------------------------------------------------------------
-// average duration of `reps` back-to-back launches of the element apply (which 0) or of the
-// dof gather (which 1) over descriptor set D, HIP events on the context's stream
+// average duration of `reps` back-to-back launches of the element apply (which 0), the dof
+// gather (which 1) or the coarse product of the two-level FP32 CG (which 2) over descriptor
+// set D, HIP events on the context's stream
 static double time_launches(fgpu_ctx *c, const Desc *D, const BatchTabs &t, bool coef, int which, int precision, int reps) {
+    if (which == 2 && (precision != FGPU_MIXED || t.nct == 0)) throw std::invalid_argument("no coarse level in the batch");
     auto one = [&] {
-        if (which == 0) {
+        if (which == 2) {
+            launch_coarse(c->stream, D, nullptr, t.ctab, t.nct, 0);
+        } else if (which == 0) {
             if (precision == FGPU_MIXED) launch_elem<float>(c->stream, D, t.etab, t.ne, coef, false, false, S_BETA, t.lds32);
             else launch_elem<double>(c->stream, D, t.etab, t.ne, coef, false, false, S_BETA, t.lds64);
         } else {

@@ -81,9 +81,16 @@ struct Desc {
     const void *invdiag;          // T: 1/diag, 0 on constrained dofs
     const double *b;              // FP64 rhs (residual mode)
     double *scal;                 // kScal doubles
-    double *part;                 // 4 regions of nb_dof per-block partial sums
+    double *part;                 // kPartRegions regions of nb_dof per-block partial sums
     const float *metric;          // FP32 apply of tet-split meshes: 6 per cell, or null
+    // two-level preconditioner of the FP32 CG (null: Jacobi only): coarse correction c per
+    // dof block (T), nc x nc inverse of the coarse operator (row-major), nc = dof blocks
+    void *cc;
+    const float *cinv;
+    int nc;
 };
+constexpr int kPartRegions = 6;   // per descriptor set: p.q, r.z, r.r, b.b, block sums of r, s.c
+constexpr int kCoarseRows = 4;    // coarse rows per workgroup of the coarse product
 
 struct Mesh {
     fgpu_ctx *ctx = nullptr;
@@ -154,7 +161,7 @@ struct Sys {
     float *Ye32 = nullptr, *invd32 = nullptr, *kq32 = nullptr, *gq32 = nullptr;
     // scalars and reduction scratch: [0] FP64 CG / outer, [1] FP32 inner
     double *scal = nullptr;       // 2*kScal
-    double *part = nullptr;       // 2 * 4*nb_dof
+    double *part = nullptr;       // 2 * kPartRegions*nb_dof
     int nb_cell = 0, nb_dof = 0;
     int ppb = kPatchPerBlock;     // patches per element workgroup (tunable)
     int patch_gen = -1;           // mesh patch generation the slot buffers are sized for
@@ -168,6 +175,12 @@ struct Sys {
     };
     std::vector<RhsTerm> rhs_terms;
     bool lifted = false;          // fgpu_sys_apply_dirichlet ran since the last reset
+    // two-level preconditioner of the mixed solver (fgpu_sys_set_coarse)
+    bool coarse = false;
+    int batches = 0;              // running batches holding the system (their tables point here)
+    int nc = 0;                   // coarse size: dof blocks of kDofBlock internal dofs
+    float *cinv = nullptr;        // nc x nc
+    float *cc = nullptr;          // nc
     bool desc32_affine = false;   // the last FP32 descriptor built uses the tet-split form
     void clear_terms();
     ~Sys();

@@ -205,6 +205,16 @@ template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 }
 
 // the dof's element outputs sit contiguously in the slot buffer, in ascending cell order
+// coarse part of the preconditioned residual at dof i under the two-level FP32 CG: the
+// coarse correction of the dof's block, 0 on constrained dofs (Desc::cc must be set)
+template <class T> __device__ __forceinline__ T coarse_term(const Desc &d, int i) {
+    // both loads unconditional and independent (a select on the mask would make the second
+    // wait for the first); c * 1 and c * 0 are exact
+    const T c = reinterpret_cast<const T *>(d.cc)[i / kDofBlock];
+    const T keep = (T)(1 - (int)d.dmask[i]);
+    return c * keep;
+}
+
 template <class T> __device__ __forceinline__ T gather_slots(const Desc &d, int i) {
     const T *Ye = reinterpret_cast<const T *>(d.Ye);
     const int e = d.gptr[i + 1];
@@ -421,6 +431,8 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
                 }
                 if (MODE == EM_APPLY) {
                     T v = ua[node];
+                    // two-level FP32 CG: z carries its coarse part where it is read
+                    if (d.cc) v += coarse_term<T>(d, node);
                     if (ub) v += beta * ub[node];
                     if (MASKIN && d.dmask[node]) v = 0;
                     su[j] = v;
@@ -890,6 +902,62 @@ void launch_lift(hipStream_t s, int n, const unsigned char *mask, const double *
                  double *b, double *x) {
     if (n > 0) hipLaunchKernelGGL(k_lift, dim3(grid_for(n)), dim3(kBlock), 0, s, n, mask, g, Ag, diag, b, x);
 }
+// ------------------------------------------------------------------------------------------
+// coarse operator of the two-level preconditioner: E = Z^T A Z from A applied to the
+// indicators of one colour of dof blocks at a time, then E^-1 by Gauss-Jordan sweeps
+// ------------------------------------------------------------------------------------------
+__global__ void k_coarse_indicator(int n, const unsigned char *__restrict__ mask, const int *__restrict__ color, int c,
+                                   double *__restrict__ u) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] = (!mask[i] && color[i / kDofBlock] == c) ? 1.0 : 0.0;
+}
+
+// out[b] = sum of q over the free dofs of block b (one workgroup per block, fixed order)
+__global__ __launch_bounds__(kBlock) void k_coarse_sums(int n, const unsigned char *__restrict__ mask,
+                                                         const double *__restrict__ q, double *__restrict__ out) {
+    double v[1] = {0.0};
+    for (int m = 0, i = blockIdx.x * kDofBlock + threadIdx.x; m < kDofIter; ++m, i += kBlock)
+        if (i < n && !mask[i]) v[0] += q[i];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+}
+
+// one Gauss-Jordan exchange step on pivot k (no pivoting: E is SPD), out of place:
+// B_kk = 1/a_kk, B_kj = a_kj/a_kk, B_ik = -a_ik/a_kk, B_ij = a_ij - a_ik a_kj / a_kk; after the
+// steps k = 0..n-1 the matrix holds the inverse
+__global__ void k_gj_step(int n, int k, const double *__restrict__ A, double *__restrict__ B) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double piv = A[(size_t)k * n + k];
+    double v;
+    if (i == k && j == k) v = 1.0 / piv;
+    else if (i == k) v = A[e] / piv;
+    else if (j == k) v = -A[e] / piv;
+    else v = A[e] - A[(size_t)i * n + k] * A[(size_t)k * n + j] / piv;
+    B[e] = v;
+}
+
+__global__ void k_to_float(size_t n, const double *__restrict__ a, float *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)a[i];
+}
+
+void launch_coarse_indicator(hipStream_t s, int n, const unsigned char *mask, const int *color, int c, double *u) {
+    if (n > 0) hipLaunchKernelGGL(k_coarse_indicator, dim3((n + 255) / 256), dim3(256), 0, s, n, mask, color, c, u);
+}
+void launch_coarse_sums(hipStream_t s, int n, const unsigned char *mask, const double *q, double *out) {
+    const int nb = (n + kDofBlock - 1) / kDofBlock;
+    if (nb > 0) hipLaunchKernelGGL(k_coarse_sums, dim3(nb), dim3(kBlock), 0, s, n, mask, q, out);
+}
+void launch_gj_step(hipStream_t s, int n, int k, const double *A, double *B) {
+    const size_t e = (size_t)n * n;
+    if (e) hipLaunchKernelGGL(k_gj_step, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, n, k, A, B);
+}
+void launch_to_float(hipStream_t s, size_t n, const double *a, float *out) {
+    if (n) hipLaunchKernelGGL(k_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, out);
+}
+
 void launch_invdiag(hipStream_t s, int n, const unsigned char *mask, const double *diag, double *inv64, float *inv32) {
     if (n > 0) hipLaunchKernelGGL(k_invdiag, dim3(grid_for(n)), dim3(kBlock), 0, s, n, mask, diag, inv64, inv32);
 }

@@ -93,6 +93,8 @@ API = [
     ("fgpu_interp_surface_cells", C.c_int, [_P, C.c_int, _D, C.c_int, C.c_int, _I, _D]),
     ("fgpu_sys_heat_terms", C.c_int, [_P, _P, C.c_double, C.c_double]),
     ("fgpu_sys_apply32", C.c_int, [_P, _D, _D]),
+    ("fgpu_sys_set_coarse", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    ("fgpu_sys_get_coarse", C.c_int, [_P, C.POINTER(C.c_int), C.c_void_p, C.c_void_p]),
     ("fgpu_mesh_use_affine", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
     ("fgpu_sys_minmax", C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("fgpu_sys_solution_grad", C.c_int, [_P, _D]),
@@ -329,6 +331,25 @@ class System:
         y = np.zeros(self.mesh.n_dofs)
         self._c(self.ctx.lib.fgpu_sys_apply(self.h, np.ascontiguousarray(x, np.float64), y), "apply")
         return y
+
+    def set_coarse(self, enable: bool = True) -> int:
+        """Two-level preconditioner of the mixed solver (coarse blocks of 1024 dofs); returns
+        the number of operator applications the build took."""
+        n = C.c_int(0)
+        self._c(self.ctx.lib.fgpu_sys_set_coarse(self.h, int(bool(enable)), C.byref(n)), "set_coarse")
+        return n.value
+
+    def coarse(self):
+        """(block of every dof, E^-1) of the two-level preconditioner, or None without one."""
+        nc = C.c_int(0)
+        self._c(self.ctx.lib.fgpu_sys_get_coarse(self.h, C.byref(nc), None, None), "get_coarse")
+        if nc.value == 0:
+            return None
+        block = np.zeros(self.mesh.n_dofs, np.int32)
+        einv = np.zeros((nc.value, nc.value), np.float32)
+        self._c(self.ctx.lib.fgpu_sys_get_coarse(self.h, C.byref(nc), block.ctypes.data, einv.ctypes.data),
+                "get_coarse")
+        return block, einv
 
     def apply32(self, x):
         y = np.zeros(self.mesh.n_dofs)

@@ -156,7 +156,8 @@ class ServeStats:
 class Server:
     def __init__(self, ctx: fgpu.Context, clients: list[Client], opts: fgpu.SolveOpts,
                  budget_bytes: int = 64 << 30, batched: bool = True, warm: bool = True,
-                 step_iterations: int = 32, max_batch: int = 64, record_inputs: bool = False, configure=None):
+                 step_iterations: int = 32, max_batch: int = 64, record_inputs: bool = False, configure=None,
+                 coarse: bool = False):
         self.ctx = ctx
         self.clients = clients
         self.opts = opts
@@ -167,6 +168,7 @@ class Server:
         self.max_batch = max_batch
         self.record_inputs = record_inputs
         self.configure = configure                    # called as configure(mesh, system) on import
+        self.coarse = coarse                          # two-level preconditioner, built at residency
         self.resident: OrderedDict[int, _Resident] = OrderedDict()   # LRU order
         self.known_bytes: dict[int, int] = {}
         self.pending: deque[Request] = deque()
@@ -202,6 +204,9 @@ class Server:
             self.configure(m, s)
         s.fill_solution(0.0)
         b = m.device_bytes() + s.device_bytes()
+        if self.coarse:
+            nc = (m.n_dofs + 1023) // 1024
+            b += nc * (nc + 1) * 4
         self.known_bytes[c.cid] = b
         r = _Resident(m, s, b)
         self.resident[c.cid] = r
@@ -227,6 +232,9 @@ class Server:
             s.add_neumann(8, None, c.field)
             s.apply_dirichlet()
             s.fill_solution(0.0)
+            if self.coarse:
+                # built once per residency: it stays a valid preconditioner as the mesh moves
+                s.set_coarse(True)
             r.fresh = False
         else:
             # same boundary terms, new geometry: rebuilt on the device

@@ -171,9 +171,10 @@ int fgpu_batch_size(const fgpu_batch *batch);
 int fgpu_batch_add(fgpu_batch *batch, fgpu_sys *sys);
 int fgpu_batch_step(fgpu_batch *batch, int iterations, int *n_running);
 int fgpu_batch_retire(fgpu_batch *batch, int max, fgpu_sys **out, fgpu_solve_stats *stats, int *n_out);
-/* average duration (ms) of one launch of the element apply (which 0) or of the dof gather
- * (which 1) over all members, in the batch's precision; for measurement, no counterpart in
- * the reference.  Overwrites the members' slot buffers. */
+/* average duration (ms) of one launch of the element apply (which 0), the dof gather
+ * (which 1) or the coarse product of the two-level preconditioner (which 2, mixed batches
+ * with a coarse level) over all members, in the batch's precision; for measurement, no
+ * counterpart in the reference.  Overwrites the members' slot buffers and coarse corrections. */
 int fgpu_batch_time_kernel(fgpu_batch *batch, int which, int reps, double *avg_ms);
 /* ---- transfer to atoms (Interpolator, InterpolatorCells, SolutionReader) ---------------
  * An interpolator holds the tet mesh, its Tethex hexes (4 per tet, hex 4t+k around vertex k,
@@ -246,6 +247,19 @@ int fgpu_interp_surface_cells(fgpu_interp *interp, int n, const double *xyz, int
  * int phi_i (gamma T_old + sigma(T_old) |grad phi_c|^2) to the right-hand side.  Call after
  * fgpu_sys_reset; gamma = 3.4496e-24 / dt [J/(K A^3 s)], lorentz [W Ohm/K^2] (2.44e-8). */
 int fgpu_sys_heat_terms(fgpu_sys *heat, const fgpu_sys *current, double gamma, double lorentz);
+/* Two-level preconditioner for the FP32 CG of the mixed solver, no counterpart in the
+ * reference (which uses SSOR, DealSolver::solve_cg, src/DealSolver.cpp:443-458):
+ * M^-1 r = D^-1 r + Z E^-1 Z^T r, Z the indicator of blocks of 1024 consecutive internal dofs
+ * (free dofs only), E = Z^T A Z.  enable != 0 builds E from the system's current operator and
+ * Dirichlet set (call after fgpu_sys_apply_dirichlet) and stores E^-1 in FP32 (nc^2 * 4 bytes,
+ * nc = ceil(n_dofs / 1024)); the FP64 solver and fgpu_sys_apply ignore it.  E may go stale as
+ * the mesh moves or the conditions change: the preconditioner stays symmetric positive
+ * definite, so the solve converges to the same tolerance, in more iterations; call again to
+ * rebuild.  colors (may be NULL): the number of operator applications the build took. */
+int fgpu_sys_set_coarse(fgpu_sys *sys, int enable, int *colors);
+/* for checks: nc (0 without a coarse level), the block of every dof (API order, n_dofs ints,
+ * may be NULL) and E^-1 (nc*nc floats, row-major, may be NULL) */
+int fgpu_sys_get_coarse(fgpu_sys *sys, int *nc, int *block, float *einv);
 /* y = A x with the FP32 operator of the mixed solver (inputs rounded to FP32, result widened);
  * for checks, no counterpart in the reference */
 int fgpu_sys_apply32(fgpu_sys *sys, const double *x, double *y);
