@@ -819,6 +819,7 @@ struct BatchTabs {
     std::vector<const Desc *> sets;  // device pointers of each descriptor set
     const int2 *etab = nullptr, *dtab = nullptr, *ctab = nullptr;
     int ne = 0, nd = 0, nct = 0;   // nct: coarse-product workgroups (systems with a coarse level)
+    bool affine32 = true;          // every system's FP32 descriptor uses the tet-split form
     size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the element kernel, largest patch of the batch
 };
 
@@ -880,6 +881,7 @@ static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> 
         // the tet-split FP32 form needs no coordinates: as decided by the system's FP32
         // descriptor, which every caller builds before uploading it
         t.lds32 = std::max(t.lds32, patch_lds(m->patches.node_cap, m->patches.cells, sizeof(float), x->desc32_affine));
+        t.affine32 = t.affine32 && x->desc32_affine;
     }
     return t;
 }
@@ -1170,7 +1172,7 @@ extern "C" int fgpu_sys_apply32(fgpu_sys *s, const double *x, double *y) {
         d.ub = nullptr;
         d.q = s->q32;
         BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
-        launch_elem<float>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds32);
+        launch_elem<float>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds32, t.affine32);
         launch_gather_plain<float>(st, t.sets[0], t.dtab, t.nd, false);
         std::vector<float> o(m->n_dofs);
         d2h(o.data(), s->q32, o.size(), st);
@@ -1428,7 +1430,7 @@ static bool any_flag(fgpu_ctx *c, const Desc *D, int n, int slot, double want, d
 // one CG iteration of every system of the set
 template <class T>
 static void cg_iter(hipStream_t st, const Desc *D, const BatchTabs &t, int n, bool coef, int reliable) {
-    launch_elem<T>(st, D, t.etab, t.ne, coef, false, true, S_BETA, sizeof(T) == 8 ? t.lds64 : t.lds32);
+    launch_elem<T>(st, D, t.etab, t.ne, coef, false, true, S_BETA, sizeof(T) == 8 ? t.lds64 : t.lds32, t.affine32);
     launch_gather_cg<T>(st, D, t.dtab, t.nd);
     launch_scal_alpha(st, D, n);
     launch_update_cg<T>(st, D, t.dtab, t.nd);
@@ -1771,7 +1773,8 @@ static double time_launches(fgpu_ctx *c, const Desc *D, const BatchTabs &t, bool
         if (which == 2) {
             launch_coarse(c->stream, D, nullptr, t.ctab, t.nct, 0);
         } else if (which == 0) {
-            if (precision == FGPU_MIXED) launch_elem<float>(c->stream, D, t.etab, t.ne, coef, false, false, S_BETA, t.lds32);
+            if (precision == FGPU_MIXED)
+                launch_elem<float>(c->stream, D, t.etab, t.ne, coef, false, false, S_BETA, t.lds32, t.affine32);
             else launch_elem<double>(c->stream, D, t.etab, t.ne, coef, false, false, S_BETA, t.lds64);
         } else {
             if (precision == FGPU_MIXED) launch_gather_plain<float>(c->stream, D, t.dtab, t.nd, false);

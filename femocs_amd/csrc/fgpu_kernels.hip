@@ -381,7 +381,9 @@ void launch_metric(hipStream_t s, int nc, const int *cells, const double *X, flo
                            reinterpret_cast<const double4 *>(X), metric, devmax);
 }
 
-template <class T, bool COEF, bool MASKIN, bool ACTIVE_ONLY, int MODE>
+// AFF: every system of the launch has the tet-split FP32 form (the general path is not
+// compiled in, which frees registers for more resident waves)
+template <class T, bool COEF, bool MASKIN, bool ACTIVE_ONLY, int MODE, bool AFF = false>
 __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const Desc *__restrict__ D, const int2 *__restrict__ tab, int scal_slot,
                                                      const double *__restrict__ fq) {
     constexpr int kRounds = (kPatchNodes + kBlock - 1) / kBlock;
@@ -390,7 +392,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
     if (ACTIVE_ONLY && d.scal[S_ACTIVE] == 0.0) return;
     using V4 = typename Vec4T<T>::type;
     // FP32 apply of a tet-split mesh: per-cell metric, no coordinates (uniform per workgroup)
-    const bool affine = sizeof(T) == 4 && !COEF && MODE == EM_APPLY && d.metric != nullptr;
+    const bool affine = AFF || (sizeof(T) == 4 && !COEF && MODE == EM_APPLY && d.metric != nullptr);
     // dynamic LDS: node coordinates (not for the tet-split form) and inputs (node_cap each),
     // cell outputs, entry lists
     T *sx = reinterpret_cast<T *>(fg_smem);
@@ -407,49 +409,79 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
     for (int p = t.y * d.ppb; p < p_end; ++p) {
         const int c0 = d.pcell0[p], nc = d.pcell0[p + 1] - c0;
         const int n0 = d.pnode0[p], np = d.pnode0[p + 1] - n0;
-        // phase 1: every global load of the patch is issued here, independent of each other
-        uint4 packed = make_uint4(0, 0, 0, 0);
-        if (tid < nc) {
-            packed = reinterpret_cast<const uint4 *>(d.plidx)[c0 + tid];
-            slent[tid] = reinterpret_cast<const uint4 *>(d.plent)[c0 + tid];
+        // phase 1: every global load of the patch is issued here, in two waves: the patch
+        // tables (cell and node lists), then what they index (inputs, coordinates, masks).
+        // Indices past the patch are clamped to its first node or cell, so every load is
+        // unconditional and none waits for another of its wave.
+        const int ct = tid < nc ? tid : 0;
+        const uint4 packed = reinterpret_cast<const uint4 *>(d.plidx)[c0 + ct];
+        const uint4 lent = reinterpret_cast<const uint4 *>(d.plent)[c0 + ct];
+        float2 mq[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+        if constexpr (sizeof(T) == 4 && !COEF && MODE == EM_APPLY) {
+            if (affine) {
+                const float2 *mp = reinterpret_cast<const float2 *>(d.metric + 6 * (size_t)(c0 + ct));
+                mq[0] = mp[0];
+                mq[1] = mp[1];
+                mq[2] = mp[2];
+            }
         }
-        int eb[kRounds], ee[kRounds], os[kRounds];
+        int eb[kRounds], ee[kRounds], os[kRounds], nd[kRounds];
 #pragma unroll
         for (int r = 0; r < kRounds; ++r) {
             const int j = tid + r * kBlock;
-            eb[r] = ee[r] = os[r] = 0;
+            const int jj = j < np ? j : 0;
+            eb[r] = d.plptr[n0 + p + jj];
+            ee[r] = d.plptr[n0 + p + jj + 1];
+            os[r] = d.pslot[n0 + jj];
+            nd[r] = d.pnodes[n0 + jj];
+        }
+        // the node lists had to arrive before the second wave is addressed: store the cell
+        // entry lists meanwhile (keeps their load in the first wave)
+        if (tid < nc) slent[tid] = lent;
+        T va[kRounds], vb[kRounds], vc[kRounds];
+        unsigned char mk[kRounds];
+        V4 xs[kRounds];
+        const T *ubp = ub ? ub : ua;                                  // never read through null
+        const T *ccp = d.cc ? reinterpret_cast<const T *>(d.cc) : ua;
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            if (!affine) xs[r] = X[nd[r]];
+            if (MODE == EM_APPLY) {
+                va[r] = ua[nd[r]];
+                vb[r] = ubp[nd[r]];
+                vc[r] = ccp[nd[r] / kDofBlock];
+                mk[r] = d.dmask[nd[r]];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            const int j = tid + r * kBlock;
             if (j < np) {
-                eb[r] = d.plptr[n0 + p + j];
-                ee[r] = d.plptr[n0 + p + j + 1];
-                os[r] = d.pslot[n0 + j];
-                const int node = d.pnodes[n0 + j];
                 if (!affine) {
-                    const V4 x = X[node];
-                    sx[j] = x.x;
-                    sy[j] = x.y;
-                    sz[j] = x.z;
+                    sx[j] = xs[r].x;
+                    sy[j] = xs[r].y;
+                    sz[j] = xs[r].z;
                 }
                 if (MODE == EM_APPLY) {
-                    T v = ua[node];
+                    T v = va[r];
                     // two-level FP32 CG: z carries its coarse part where it is read
-                    if (d.cc) v += coarse_term<T>(d, node);
-                    if (ub) v += beta * ub[node];
-                    if (MASKIN && d.dmask[node]) v = 0;
+                    if (d.cc) v += vc[r] * (T)(1 - (int)mk[r]);
+                    if (ub) v += beta * vb[r];
+                    if (MASKIN && mk[r]) v = 0;
                     su[j] = v;
                 }
+            } else {
+                eb[r] = ee[r] = 0;
             }
         }
         __syncthreads();
         // phase 2: one cell per thread, from LDS
         if constexpr (sizeof(T) == 4 && !COEF && MODE == EM_APPLY) {
             if (affine && tid < nc) {
-                const int c = c0 + tid;
                 const int lid[8] = {(int)(packed.x & 0xffff), (int)(packed.x >> 16), (int)(packed.y & 0xffff),
                                     (int)(packed.y >> 16), (int)(packed.z & 0xffff), (int)(packed.z >> 16),
                                     (int)(packed.w & 0xffff), (int)(packed.w >> 16)};
-                const float2 *mp = reinterpret_cast<const float2 *>(d.metric + 6 * (size_t)c);
-                const float2 m0 = mp[0], m1 = mp[1], m2 = mp[2];
-                const float M[6] = {m0.x, m0.y, m1.x, m1.y, m2.x, m2.y};
+                const float M[6] = {mq[0].x, mq[0].y, mq[1].x, mq[1].y, mq[2].x, mq[2].y};
                 float u[8], y[8];
 #pragma unroll
                 for (int n = 0; n < 8; ++n) u[n] = su[lid[n]];
@@ -458,7 +490,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
                 for (int n = 0; n < 8; ++n) sout[8 * tid + n] = y[n];
             }
         }
-        if (tid < nc && !affine) {
+        if (!AFF && tid < nc && !affine) {
             const int c = c0 + tid;
             const int lid[8] = {(int)(packed.x & 0xffff), (int)(packed.x >> 16), (int)(packed.y & 0xffff), (int)(packed.y >> 16),
                                 (int)(packed.z & 0xffff), (int)(packed.z >> 16), (int)(packed.w & 0xffff), (int)(packed.w >> 16)};
@@ -866,9 +898,14 @@ __global__ void k_grad_export(int n, const int *__restrict__ v2c, const int *__r
 // ---------------------------------------------------------------------------------------
 template <class T>
 void launch_elem(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef, bool maskin, bool active_only, int scal_slot,
-                 size_t lds) {
+                 size_t lds, bool affine_all) {
     if (nblk <= 0) return;
     const dim3 g(nblk), b(kBlock);
+    if (sizeof(T) == 4 && affine_all && !coef && !maskin) {
+        if (active_only) hipLaunchKernelGGL((k_patch<T, false, false, true, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr);
+        else hipLaunchKernelGGL((k_patch<T, false, false, false, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr);
+        return;
+    }
 #define FG_ELEM(C, M, A) hipLaunchKernelGGL((k_patch<T, C, M, A, EM_APPLY>), g, b, lds, s, D, tab, scal_slot, nullptr)
     if (coef) {
         if (maskin) { if (active_only) FG_ELEM(true, true, true); else FG_ELEM(true, true, false); }
@@ -879,8 +916,8 @@ void launch_elem(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool c
     }
 #undef FG_ELEM
 }
-template void launch_elem<double>(hipStream_t, const Desc *, const int2 *, int, bool, bool, bool, int, size_t);
-template void launch_elem<float>(hipStream_t, const Desc *, const int2 *, int, bool, bool, bool, int, size_t);
+template void launch_elem<double>(hipStream_t, const Desc *, const int2 *, int, bool, bool, bool, int, size_t, bool);
+template void launch_elem<float>(hipStream_t, const Desc *, const int2 *, int, bool, bool, bool, int, size_t, bool);
 
 template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef, size_t lds) {
     if (nblk <= 0) return;
