@@ -817,7 +817,8 @@ static void *ensure_tab(fgpu_ctx *c, size_t bytes) {
 // [ndesc_sets x n Desc][elem table][dof table]
 struct BatchTabs {
     std::vector<const Desc *> sets;  // device pointers of each descriptor set
-    const int2 *etab = nullptr, *dtab = nullptr, *ctab = nullptr;
+    const ETab *etab = nullptr;
+    const int2 *dtab = nullptr, *ctab = nullptr;
     int ne = 0, nd = 0, nct = 0;   // nct: coarse-product workgroups (systems with a coarse level)
     bool affine32 = true;          // every system's FP32 descriptor uses the tet-split form
     size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the element kernel, largest patch of the batch
@@ -831,9 +832,23 @@ static size_t patch_lds(int node_cap, int cells, size_t tsize, bool affine = fal
 
 static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> &sets, const std::vector<fgpu_sys *> &sys,
                               void **own_buf = nullptr, size_t *own_cap = nullptr) {
-    std::vector<int2> et, dt, ct;
+    std::vector<int2> et0, dt, ct;
+    std::vector<ETab> et;
     for (size_t i = 0; i < sys.size(); ++i) {
-        append_table(et, (int)i, sys[i]->nb_cell);
+        et0.clear();
+        append_table(et0, (int)i, sys[i]->nb_cell);
+        const Patches &P = static_cast<const fgpu_mesh *>(sys[i]->mesh)->patches;
+        for (const int2 &e : et0) {
+            const int p = e.y * sys[i]->ppb;
+            ETab w{(int)i, p, 0, 0, 0, 0, 0, 0};
+            if (p < P.n) {
+                w.c0 = P.cell0[p];
+                w.nc = P.cell0[p + 1] - w.c0;
+                w.n0 = P.node0[p];
+                w.np = P.node0[p + 1] - w.n0;
+            }
+            et.push_back(w);
+        }
         append_table(dt, (int)i, sys[i]->nb_dof);
         if (sys[i]->coarse)
             for (int b = 0; b < (sys[i]->nc + kCoarseRows - 1) / kCoarseRows; ++b) ct.push_back(make_int2((int)i, b));
@@ -842,12 +857,12 @@ static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> 
     const size_t dbytes = sets.size() * n * sizeof(Desc);
     const size_t align = 256;
     const size_t off_e = (dbytes + align - 1) / align * align;
-    const size_t off_d = off_e + (et.size() * sizeof(int2) + align - 1) / align * align;
+    const size_t off_d = off_e + (et.size() * sizeof(ETab) + align - 1) / align * align;
     const size_t off_c = off_d + (dt.size() * sizeof(int2) + align - 1) / align * align;
     const size_t total = off_c + ct.size() * sizeof(int2);
     std::vector<char> host(total);
     for (size_t k = 0; k < sets.size(); ++k) std::memcpy(host.data() + k * n * sizeof(Desc), sets[k].data(), n * sizeof(Desc));
-    std::memcpy(host.data() + off_e, et.data(), et.size() * sizeof(int2));
+    std::memcpy(host.data() + off_e, et.data(), et.size() * sizeof(ETab));
     std::memcpy(host.data() + off_d, dt.data(), dt.size() * sizeof(int2));
     if (!ct.empty()) std::memcpy(host.data() + off_c, ct.data(), ct.size() * sizeof(int2));
     char *dev;
@@ -869,7 +884,7 @@ static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> 
     HIPCHECK(hipStreamSynchronize(c->stream));
     BatchTabs t;
     for (size_t k = 0; k < sets.size(); ++k) t.sets.push_back(reinterpret_cast<const Desc *>(dev + k * n * sizeof(Desc)));
-    t.etab = reinterpret_cast<const int2 *>(dev + off_e);
+    t.etab = reinterpret_cast<const ETab *>(dev + off_e);
     t.dtab = reinterpret_cast<const int2 *>(dev + off_d);
     t.ctab = reinterpret_cast<const int2 *>(dev + off_c);
     t.ne = (int)et.size();

@@ -89,6 +89,12 @@ struct Desc {
     const float *cinv;
     int nc;
 };
+// Element-kernel work item: one workgroup, starting at patch p of system sys, with that
+// patch's first cell and node and its cell and node counts (nc = 0: nothing to do), so the
+// first patch needs no table lookups before its cell and node lists are read.
+struct ETab {
+    int sys, p, c0, n0, nc, np, pad0, pad1;
+};
 constexpr int kPartRegions = 6;   // per descriptor set: p.q, r.z, r.r, b.b, block sums of r, s.c
 constexpr int kCoarseRows = 4;    // coarse rows per workgroup of the coarse product
 

@@ -4,10 +4,10 @@
 
 namespace fgpu {
 template <class T>
-void launch_elem(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef, bool maskin, bool active_only,
+void launch_elem(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool coef, bool maskin, bool active_only,
                  int scal_slot, size_t lds, bool affine_all = false);
-template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef, size_t lds);
-void launch_elem_rhs(hipStream_t s, const Desc *D, const int2 *tab, int nblk, const double *fq, size_t lds);
+template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool coef, size_t lds);
+void launch_elem_rhs(hipStream_t s, const Desc *D, const ETab *tab, int nblk, const double *fq, size_t lds);
 template <class T> void launch_gather_cg(hipStream_t s, const Desc *D, const int2 *tab, int nblk);
 template <class T> void launch_update_cg(hipStream_t s, const Desc *D, const int2 *tab, int nblk);
 template <class T> void launch_gather_res(hipStream_t s, const Desc *D, const int2 *tab, int nblk);

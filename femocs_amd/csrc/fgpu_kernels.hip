@@ -205,21 +205,53 @@ template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 }
 
 // the dof's element outputs sit contiguously in the slot buffer, in ascending cell order
+// Global-memory views of the descriptor's pointers: loads and stores through them are
+// global_load/global_store (scalar base + vector offset) instead of flat accesses, which also
+// count against the LDS counter and would make every LDS wait wait for them too.
+template <class T> using gcp = const __attribute__((address_space(1))) T *;
+template <class T> using gwp = __attribute__((address_space(1))) T *;
+template <class T> __device__ __forceinline__ gcp<T> G(const T *p) { return (gcp<T>)p; }
+template <class T> __device__ __forceinline__ gwp<T> GW(T *p) { return (gwp<T>)p; }
+// HIP's vector structs cannot be copied out of an address-space-qualified reference: vector
+// loads go through clang vector types
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+template <class T> struct ExtV4;
+template <> struct ExtV4<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct ExtV4<double> { typedef double type __attribute__((ext_vector_type(4))); };
+__device__ __forceinline__ uint4 ldg(const uint4 *p) {
+    const u32x4 v = G(reinterpret_cast<const u32x4 *>(p))[0];
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 ldg(const float2 *p) {
+    const f32x2 v = G(reinterpret_cast<const f32x2 *>(p))[0];
+    return make_float2(v.x, v.y);
+}
+template <class V4, class T> __device__ __forceinline__ V4 ldg4(const V4 *p) {
+    const typename ExtV4<T>::type v = G(reinterpret_cast<const typename ExtV4<T>::type *>(p))[0];
+    V4 r;
+    r.x = v.x;
+    r.y = v.y;
+    r.z = v.z;
+    r.w = v.w;
+    return r;
+}
+
 // coarse part of the preconditioned residual at dof i under the two-level FP32 CG: the
 // coarse correction of the dof's block, 0 on constrained dofs (Desc::cc must be set)
 template <class T> __device__ __forceinline__ T coarse_term(const Desc &d, int i) {
     // both loads unconditional and independent (a select on the mask would make the second
     // wait for the first); c * 1 and c * 0 are exact
-    const T c = reinterpret_cast<const T *>(d.cc)[i / kDofBlock];
-    const T keep = (T)(1 - (int)d.dmask[i]);
+    const T c = G(reinterpret_cast<const T *>(d.cc))[i / kDofBlock];
+    const T keep = (T)(1 - (int)G(d.dmask)[i]);
     return c * keep;
 }
 
 template <class T> __device__ __forceinline__ T gather_slots(const Desc &d, int i) {
-    const T *Ye = reinterpret_cast<const T *>(d.Ye);
-    const int e = d.gptr[i + 1];
+    const auto Ye = G(reinterpret_cast<const T *>(d.Ye));
+    const int e = G(d.gptr)[i + 1];
     T s = 0;
-    for (int k = d.gptr[i]; k < e; ++k) s += Ye[k];
+    for (int k = G(d.gptr)[i]; k < e; ++k) s += Ye[k];
     return s;
 }
 
@@ -235,6 +267,8 @@ template <class T> __device__ __forceinline__ T gather_slots(const Desc &d, int 
 enum { EM_APPLY = 0, EM_DIAG = 1, EM_RHS = 2 };
 
 extern __shared__ __attribute__((aligned(16))) unsigned char fg_smem[];
+
+
 
 // ---------------------------------------------------------------------------------------
 // tet-split meshes: every hex of the Tethex split of a tet is the image of one reference
@@ -384,12 +418,14 @@ void launch_metric(hipStream_t s, int nc, const int *cells, const double *X, flo
 // AFF: every system of the launch has the tet-split FP32 form (the general path is not
 // compiled in, which frees registers for more resident waves)
 template <class T, bool COEF, bool MASKIN, bool ACTIVE_ONLY, int MODE, bool AFF = false>
-__global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const Desc *__restrict__ D, const int2 *__restrict__ tab, int scal_slot,
+__global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const Desc *__restrict__ D, const ETab *__restrict__ tab, int scal_slot,
                                                      const double *__restrict__ fq) {
     constexpr int kRounds = (kPatchNodes + kBlock - 1) / kBlock;
-    const int2 t = tab[blockIdx.x];
-    const Desc &d = D[t.x];
-    if (ACTIVE_ONLY && d.scal[S_ACTIVE] == 0.0) return;
+    const ETab t = tab[blockIdx.x];
+    if (t.nc == 0) return;
+    const Desc &d = D[t.sys];
+    // read now, tested once the first patch's lists are in flight (one latency fewer)
+    const double active = ACTIVE_ONLY ? G(d.scal)[S_ACTIVE] : 1.0;
     using V4 = typename Vec4T<T>::type;
     // FP32 apply of a tet-split mesh: per-cell metric, no coordinates (uniform per workgroup)
     const bool affine = AFF || (sizeof(T) == 4 && !COEF && MODE == EM_APPLY && d.metric != nullptr);
@@ -400,29 +436,29 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
     T *su = affine ? sx : sz + d.node_cap, *sout = su + d.node_cap;
     uint4 *slent = reinterpret_cast<uint4 *>(sout + 8 * d.pcells);
     const V4 *X = reinterpret_cast<const V4 *>(d.X);
-    const T *ua = reinterpret_cast<const T *>(d.ua);
+    const auto ua = G(reinterpret_cast<const T *>(d.ua));
     const T *ub = reinterpret_cast<const T *>(d.ub);
-    const T beta = (MODE == EM_APPLY && ub) ? (T)d.scal[scal_slot] : (T)0;
-    T *Ye = reinterpret_cast<T *>(d.Ye);
+    // converted where it is used, after the first loads are in flight
+    const double beta64 = MODE == EM_APPLY ? G(d.scal)[scal_slot] : 0.0;   // used only with ub
+    const auto Ye = GW(reinterpret_cast<T *>(d.Ye));
     const int tid = threadIdx.x;
-    const int p_end = min(d.n_patch, (t.y + 1) * d.ppb);
-    for (int p = t.y * d.ppb; p < p_end; ++p) {
-        const int c0 = d.pcell0[p], nc = d.pcell0[p + 1] - c0;
-        const int n0 = d.pnode0[p], np = d.pnode0[p + 1] - n0;
+    // one patch; the first one (check) leaves the workgroup early when the system is idle,
+    // tested only once that patch's lists are in flight.  Returns false to leave.
+    auto patch = [&](const int p, const int c0, const int nc, const int n0, const int np, const bool check) -> bool {
         // phase 1: every global load of the patch is issued here, in two waves: the patch
         // tables (cell and node lists), then what they index (inputs, coordinates, masks).
         // Indices past the patch are clamped to its first node or cell, so every load is
         // unconditional and none waits for another of its wave.
         const int ct = tid < nc ? tid : 0;
-        const uint4 packed = reinterpret_cast<const uint4 *>(d.plidx)[c0 + ct];
-        const uint4 lent = reinterpret_cast<const uint4 *>(d.plent)[c0 + ct];
+        const uint4 packed = ldg(reinterpret_cast<const uint4 *>(d.plidx) + c0 + ct);
+        const uint4 lent = ldg(reinterpret_cast<const uint4 *>(d.plent) + c0 + ct);
         float2 mq[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
         if constexpr (sizeof(T) == 4 && !COEF && MODE == EM_APPLY) {
             if (affine) {
                 const float2 *mp = reinterpret_cast<const float2 *>(d.metric + 6 * (size_t)(c0 + ct));
-                mq[0] = mp[0];
-                mq[1] = mp[1];
-                mq[2] = mp[2];
+                mq[0] = ldg(mp);
+                mq[1] = ldg(mp + 1);
+                mq[2] = ldg(mp + 2);
             }
         }
         int eb[kRounds], ee[kRounds], os[kRounds], nd[kRounds];
@@ -430,27 +466,29 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
         for (int r = 0; r < kRounds; ++r) {
             const int j = tid + r * kBlock;
             const int jj = j < np ? j : 0;
-            eb[r] = d.plptr[n0 + p + jj];
-            ee[r] = d.plptr[n0 + p + jj + 1];
-            os[r] = d.pslot[n0 + jj];
-            nd[r] = d.pnodes[n0 + jj];
+            eb[r] = G(d.plptr)[n0 + p + jj];
+            ee[r] = G(d.plptr)[n0 + p + jj + 1];
+            os[r] = G(d.pslot)[n0 + jj];
+            nd[r] = G(d.pnodes)[n0 + jj];
         }
         // the node lists had to arrive before the second wave is addressed: store the cell
         // entry lists meanwhile (keeps their load in the first wave)
         if (tid < nc) slent[tid] = lent;
+        if (ACTIVE_ONLY && check && active == 0.0) return false;   // uniform: the workgroup leaves
+        const T beta = (T)beta64;
         T va[kRounds], vb[kRounds], vc[kRounds];
         unsigned char mk[kRounds];
         V4 xs[kRounds];
-        const T *ubp = ub ? ub : ua;                                  // never read through null
-        const T *ccp = d.cc ? reinterpret_cast<const T *>(d.cc) : ua;
+        const auto ubp = G(ub ? ub : reinterpret_cast<const T *>(d.ua));   // never read through null
+        const auto ccp = G(d.cc ? reinterpret_cast<const T *>(d.cc) : reinterpret_cast<const T *>(d.ua));
 #pragma unroll
         for (int r = 0; r < kRounds; ++r) {
-            if (!affine) xs[r] = X[nd[r]];
+            if (!affine) xs[r] = ldg4<V4, T>(X + nd[r]);
             if (MODE == EM_APPLY) {
                 va[r] = ua[nd[r]];
                 vb[r] = ubp[nd[r]];
                 vc[r] = ccp[nd[r] / kDofBlock];
-                mk[r] = d.dmask[nd[r]];
+                mk[r] = G(d.dmask)[nd[r]];
             }
         }
 #pragma unroll
@@ -503,8 +541,8 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
             }
             T kq[8], gq[8];
             if (COEF) {
-                const T *k = reinterpret_cast<const T *>(d.kq) + 8 * (size_t)c;
-                const T *g = reinterpret_cast<const T *>(d.gq) + 8 * (size_t)c;
+                const auto k = G(reinterpret_cast<const T *>(d.kq) + 8 * (size_t)c);
+                const auto g = G(reinterpret_cast<const T *>(d.gq) + 8 * (size_t)c);
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     kq[q] = k[q];
@@ -539,7 +577,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
                     const T det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                                   J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                                   J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-                    const T jxw = (T)0.125 * det * (T)fq[8 * (size_t)c + q];
+                    const T jxw = (T)0.125 * det * (T)G(fq)[8 * (size_t)c + q];
                     for (int n = 0; n < 8; ++n)
                         y[n] += Wt<T>(a, n & 1) * Wt<T>(b, (n >> 1) & 1) * Wt<T>(cc, (n >> 2) & 1) * jxw;
                 }
@@ -560,6 +598,13 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
             }
         }
         __syncthreads();
+        return true;
+    };
+    if (!patch(t.p, t.c0, t.nc, t.n0, t.np, true)) return;
+    const int p_end = min(d.n_patch, t.p + d.ppb);
+    for (int p = t.p + 1; p < p_end; ++p) {
+        const int c0 = G(d.pcell0)[p], n0 = G(d.pnode0)[p];
+        patch(p, c0, G(d.pcell0)[p + 1] - c0, n0, G(d.pnode0)[p + 1] - n0, false);
     }
 }
 
@@ -897,7 +942,7 @@ __global__ void k_grad_export(int n, const int *__restrict__ v2c, const int *__r
 // explicit instantiations through launch helpers
 // ---------------------------------------------------------------------------------------
 template <class T>
-void launch_elem(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef, bool maskin, bool active_only, int scal_slot,
+void launch_elem(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool coef, bool maskin, bool active_only, int scal_slot,
                  size_t lds, bool affine_all) {
     if (nblk <= 0) return;
     const dim3 g(nblk), b(kBlock);
@@ -916,17 +961,17 @@ void launch_elem(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool c
     }
 #undef FG_ELEM
 }
-template void launch_elem<double>(hipStream_t, const Desc *, const int2 *, int, bool, bool, bool, int, size_t, bool);
-template void launch_elem<float>(hipStream_t, const Desc *, const int2 *, int, bool, bool, bool, int, size_t, bool);
+template void launch_elem<double>(hipStream_t, const Desc *, const ETab *, int, bool, bool, bool, int, size_t, bool);
+template void launch_elem<float>(hipStream_t, const Desc *, const ETab *, int, bool, bool, bool, int, size_t, bool);
 
-template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const int2 *tab, int nblk, bool coef, size_t lds) {
+template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool coef, size_t lds) {
     if (nblk <= 0) return;
     if (coef) hipLaunchKernelGGL((k_patch<T, true, false, false, EM_DIAG>), dim3(nblk), dim3(kBlock), lds, s, D, tab, 0, nullptr);
     else hipLaunchKernelGGL((k_patch<T, false, false, false, EM_DIAG>), dim3(nblk), dim3(kBlock), lds, s, D, tab, 0, nullptr);
 }
-template void launch_elem_diag<double>(hipStream_t, const Desc *, const int2 *, int, bool, size_t);
+template void launch_elem_diag<double>(hipStream_t, const Desc *, const ETab *, int, bool, size_t);
 
-void launch_elem_rhs(hipStream_t s, const Desc *D, const int2 *tab, int nblk, const double *fq, size_t lds) {
+void launch_elem_rhs(hipStream_t s, const Desc *D, const ETab *tab, int nblk, const double *fq, size_t lds) {
     if (nblk > 0) hipLaunchKernelGGL((k_patch<double, false, false, false, EM_RHS>), dim3(nblk), dim3(kBlock), lds, s, D, tab, 0, fq);
 }
 
