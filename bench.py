@@ -3,10 +3,12 @@ concurrent molecular-dynamics clients.
 
 Workload: 16 MD clients, each with its own Cu nanotip (radius 3-12 nm, aspect ratio 4-7,
 112k-1.6M vacuum Q1 hexes from the tet split, ~10M dofs in all), whose surface atoms vibrate
-(0.1 A Gaussian noise per step).  A client submits the Laplace solve of its next timestep
-(Dirichlet 0 on the copper surface, Neumann field on the top; |b - Ax| / |b| <= 1e-9) an
-exponential think time after its previous result came back, 200 requests/s offered in all.
-One step = one served solve; `value` is solves/s over the K timed requests.
+(0.1 A Gaussian noise per step).  The clients submit the Laplace solves of their next
+timesteps (Dirichlet 0 on the copper surface, Neumann field on the top; |b - Ax| / |b| <=
+1e-9) as a Poisson process of 200 requests/s in all (--closed-loop: each client waits an
+exponential think time after its previous result instead).  A client's solves run in order.
+One step = one served solve; `value` is solves/s over the K timed requests, from the first
+timed arrival to the last timed result; latency runs from arrival to result.
 
 * headline: mixed precision (FP32 operator and CG inside FP64 defect correction), two-level
   preconditioner (Jacobi + a coarse correction on 1024-dof blocks, built once per resident
@@ -65,7 +67,7 @@ def make_clients(n: int, small: bool):
         return pool.map(_make_client, [(i, n, small) for i in range(n)])
 
 
-def serve(ctx, clients, precision, batched, warm, steps, warmup, tuned, seed, coarse=False):
+def serve(ctx, clients, precision, batched, warm, steps, warmup, tuned, seed, coarse=False, arrivals="poisson"):
     from femocs_amd import fgpu, serving
     o = fgpu.opts(tol=1e-9, relative=True, precision=precision, inner_tol=0.1, use_graph=False)
     cells, ppb = tuned
@@ -78,10 +80,11 @@ def serve(ctx, clients, precision, batched, warm, steps, warmup, tuned, seed, co
         c.seq = 0
         c.rng = np.random.default_rng(1000 + c.cid)
     srv = serving.Server(ctx, clients, o, batched=batched, warm=warm, configure=configure, coarse=coarse)
-    serving.run_clients(srv, rate=RATE, duration=1e9, max_requests=warmup, seed=seed)
+    srv.make_resident()                      # every tip imported and assembled before the clock runs
+    serving.run_clients(srv, rate=RATE, duration=1e9, max_requests=warmup, seed=seed, arrivals=arrivals)
     srv.stats = serving.ServeStats()
     ctx.sync()
-    serving.run_clients(srv, rate=RATE, duration=1e9, max_requests=steps, seed=seed + 1)
+    serving.run_clients(srv, rate=RATE, duration=1e9, max_requests=steps, seed=seed + 1, arrivals=arrivals)
     ctx.sync()
     summ = srv.stats.summary()
     bad = [r for r in srv.stats.requests if r.iterations <= 0]
@@ -137,6 +140,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
     ap.add_argument("--small", action="store_true", help="4 small tips instead (quick check)")
     ap.add_argument("--jacobi", action="store_true", help="mixed run without the two-level preconditioner")
+    ap.add_argument("--closed-loop", action="store_true",
+                    help="clients submit an exponential think time after each result instead of Poisson arrivals")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the field solver has no multi-GPU path "
@@ -163,12 +168,14 @@ def main():
     del gs, gm
     log("autotuned (cells per patch, patches per workgroup):", tuned)
 
+    arrivals = "closed" if args.closed_loop else "poisson"
     plain = None
     if not args.no_plain:
-        _, plain = serve(ctx, clients, fgpu.FGPU_FP64, False, False, args.steps, args.warmup, tuned["fp64"], 1)
+        _, plain = serve(ctx, clients, fgpu.FGPU_FP64, False, False, args.steps, args.warmup, tuned["fp64"], 1,
+                         arrivals=arrivals)
         log("plain fp64", plain)
     srv, mixed = serve(ctx, clients, fgpu.FGPU_MIXED, True, True, args.steps, args.warmup, tuned["mixed"], 1,
-                       coarse=not args.jacobi)
+                       coarse=not args.jacobi, arrivals=arrivals)
     log("mixed batched", mixed)
 
     # dominant kernel: the FP32 element apply over the whole batch (all clients resident).
@@ -221,7 +228,9 @@ def main():
         "dtype": "f32 inner / f64 outer",
         "data": "synthetic",
         "config": {"workload": f"{n} MD clients, Cu nanotips r=3-12 nm, {min(hexes)}-{max(hexes)} Q1 hexes each, "
-                               f"Poisson solve per timestep, closed-loop arrivals {RATE:.0f} req/s offered",
+                               f"Poisson solve per timestep, "
+                               + (f"closed-loop arrivals {RATE:.0f} req/s offered" if args.closed_loop else
+                                  f"Poisson arrivals at {RATE:.0f} req/s total"),
                    "clients": n, "total_dofs": int(sum(dofs)), "total_hexes": int(sum(hexes)),
                    "tol_rel": 1e-9,
                    "precond": "jacobi" if args.jacobi else "two-level: jacobi + 1024-dof block aggregates",

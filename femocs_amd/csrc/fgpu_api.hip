@@ -158,6 +158,10 @@ static void ctx_free(fgpu_ctx *c) {
     }
     if (c->d_tab) (void)hipFree(c->d_tab);
     if (c->h_flags) (void)hipHostFree(c->h_flags);
+    for (int k = 0; k < fgpu_ctx::kStage; ++k) {
+        if (c->stage_host[k]) (void)hipHostFree(c->stage_host[k]);
+        if (c->stage_ev[k]) (void)hipEventDestroy(c->stage_ev[k]);
+    }
     delete c;
 }
 
@@ -800,6 +804,29 @@ struct OneSys {
     int ne, nd;
 };
 
+// Asynchronous host-to-device copy from host data that may die when the caller returns: the
+// bytes go through one of the context's pinned staging buffers, on the context's stream
+static void stage_h2d(fgpu_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!bytes) return;
+    const int k = c->stage_next;
+    c->stage_next = (k + 1) % fgpu_ctx::kStage;
+    if (c->stage_busy[k]) HIPCHECK(hipEventSynchronize(c->stage_ev[k]));
+    c->stage_busy[k] = false;
+    if (bytes > c->stage_cap[k]) {
+        if (c->stage_host[k]) HIPCHECK(hipHostFree(c->stage_host[k]));
+        c->stage_host[k] = nullptr;
+        c->stage_cap[k] = 0;
+        const size_t cap = std::max<size_t>(2 * bytes, 1 << 16);
+        HIPCHECK(hipHostMalloc(&c->stage_host[k], cap, hipHostMallocDefault));
+        c->stage_cap[k] = cap;
+    }
+    if (!c->stage_ev[k]) HIPCHECK(hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming));
+    std::memcpy(c->stage_host[k], src, bytes);
+    HIPCHECK(hipMemcpyAsync(dst, c->stage_host[k], bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipEventRecord(c->stage_ev[k], c->stream));
+    c->stage_busy[k] = true;
+}
+
 static void *ensure_tab(fgpu_ctx *c, size_t bytes) {
     if (bytes > c->tab_bytes) {
         HIPCHECK(hipStreamSynchronize(c->stream));
@@ -879,9 +906,8 @@ static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> 
     } else {
         dev = static_cast<char *>(ensure_tab(c, total));
     }
-    HIPCHECK(hipMemcpyAsync(dev, host.data(), total, hipMemcpyHostToDevice, c->stream));
-    // the host buffer dies at return: make the copy complete first
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    // stream-ordered: kernels queued before still read the previous tables of this buffer
+    stage_h2d(c, dev, host.data(), total);
     BatchTabs t;
     for (size_t k = 0; k < sets.size(); ++k) t.sets.push_back(reinterpret_cast<const Desc *>(dev + k * n * sizeof(Desc)));
     t.etab = reinterpret_cast<const ETab *>(dev + off_e);
@@ -911,7 +937,6 @@ static void ensure_diag(fgpu_sys *s) {
     launch_elem_diag<double>(c->stream, t.sets[0], t.etab, t.ne, s->have_coef, t.lds64);
     launch_gather_plain<double>(c->stream, t.sets[0], t.dtab, t.nd, false);
     launch_invdiag(c->stream, m->n_dofs, s->d_dmask, s->diag, s->invd, s->invd32);
-    HIPCHECK(hipStreamSynchronize(c->stream));
     s->diag_valid = true;
     s->diag_gen = m->coord_gen;
 }
@@ -1066,7 +1091,6 @@ static void lift(fgpu_sys *s) {
     launch_elem<double>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds64);
     launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, false);
     launch_lift(st, n, s->d_dmask, s->g, s->tmp, s->diag, s->b, s->x);
-    HIPCHECK(hipStreamSynchronize(st));
 }
 
 // Rebuilds the right-hand side, the diagonal and the lifting for the current vertex positions
@@ -1094,7 +1118,7 @@ extern "C" int fgpu_sys_reassemble(fgpu_sys *s) {
             }
         }
         if (s->lifted) lift(s);
-        HIPCHECK(hipStreamSynchronize(st));
+        HIPCHECK(hipGetLastError());
         return FGPU_OK;
     });
 }
@@ -1116,6 +1140,7 @@ extern "C" int fgpu_sys_apply_dirichlet(fgpu_sys *s) {
         h2d(s->d_dmask, mk.data(), n, st);
         h2d(s->g, gv.data(), n, st);
         lift(s);
+        HIPCHECK(hipStreamSynchronize(st));   // mk and gv are read until the copies end
         s->lifted = true;
         return FGPU_OK;
     });
@@ -1529,7 +1554,7 @@ static void start_solves(fgpu_ctx *c, const std::vector<fgpu_sys *> &sys, const 
     }
     for (int i = 0; i < n; ++i) {
         fgpu_sys *s = sys[i];
-        h2d(s->scal, &h[(size_t)2 * kScal * i], 2 * kScal, c->stream);
+        stage_h2d(c, s->scal, &h[(size_t)2 * kScal * i], 2 * kScal * sizeof(double));
         fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
         launch_enforce_dirichlet(c->stream, m->n_dofs, s->d_dmask, s->g, s->x);
     }
@@ -1540,7 +1565,7 @@ static void start_solves(fgpu_ctx *c, const std::vector<fgpu_sys *> &sys, const 
         launch_coarse(c->stream, D32, R64, t.ctab, t.nct, 1);
         launch_scal_restart(c->stream, D32, R64, n, 1);
     }
-    HIPCHECK(hipStreamSynchronize(c->stream));  // h is consumed
+    HIPCHECK(hipGetLastError());
 }
 
 static void read_stats(fgpu_sys *s, bool mixed, fgpu_solve_stats *st) {

@@ -206,6 +206,14 @@ struct fgpu_ctx {
     int refs = 0;                 // live meshes and systems
     bool closing = false;         // destroy requested while children were alive
     bool affine_table = false;    // reference sub-hex constants uploaded to this device
+    // pinned staging buffers of asynchronous uploads from short-lived host data, each reused
+    // once the copy that last read it has completed (its event)
+    static constexpr int kStage = 4;
+    void *stage_host[kStage] = {};
+    size_t stage_cap[kStage] = {};
+    hipEvent_t stage_ev[kStage] = {};
+    bool stage_busy[kStage] = {};
+    int stage_next = 0;
 };
 
 struct fgpu_mesh : fgpu::Mesh {};

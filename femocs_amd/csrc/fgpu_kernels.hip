@@ -174,6 +174,54 @@ __device__ __forceinline__ void q1_cell(const T X[8][3], const T u[8], const T *
     }
 }
 
+// y_n = K_e(n,n) for one cell in one pass over the Gauss points:
+// sum_q JxW (kappa_q |grad phi_n|^2 + gamma_q phi_n^2), with det J^{-1} rows as in q1_cell and
+// the reference gradient of phi_n (n = i + 2j + 4k) at (a,b,c): (+-1) Wt(b,j) Wt(c,k) along xi,
+// (+-1) Wt(a,i) Wt(c,k) along eta, (+-1) Wt(a,i) Wt(b,j) along zeta
+template <class T, bool COEF>
+__device__ __forceinline__ void q1_cell_diag(const T X[8][3], const T *kq, const T *gq, T y[8]) {
+    T J0[3][2][2], J1[3][2][2], J2[3][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T f[8];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) f[n] = X[n][c];
+        dref<T>(f, 0, J0[c]);
+        dref<T>(f, 1, J1[c]);
+        dref<T>(f, 2, J2[c]);
+    }
+#pragma unroll
+    for (int n = 0; n < 8; ++n) y[n] = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int a = q & 1, b = (q >> 1) & 1, c = (q >> 2) & 1;
+        const T c0x = J0[0][b][c], c0y = J0[1][b][c], c0z = J0[2][b][c];
+        const T c1x = J1[0][a][c], c1y = J1[1][a][c], c1z = J1[2][a][c];
+        const T c2x = J2[0][a][b], c2y = J2[1][a][b], c2z = J2[2][a][b];
+        const T a0x = c1y * c2z - c1z * c2y, a0y = c1z * c2x - c1x * c2z, a0z = c1x * c2y - c1y * c2x;
+        const T a1x = c2y * c0z - c2z * c0y, a1y = c2z * c0x - c2x * c0z, a1z = c2x * c0y - c2y * c0x;
+        const T a2x = c0y * c1z - c0z * c1y, a2y = c0z * c1x - c0x * c1z, a2z = c0x * c1y - c0y * c1x;
+        const T det = c0x * a0x + c0y * a0y + c0z * a0z;
+        T s = (T)0.125 / det;
+        if (COEF) s *= kq[q];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const int i = n & 1, j = (n >> 1) & 1, k = (n >> 2) & 1;
+            const T w0 = (i ? (T)1 : (T)-1) * Wt<T>(b, j) * Wt<T>(c, k);
+            const T w1 = (j ? (T)1 : (T)-1) * Wt<T>(a, i) * Wt<T>(c, k);
+            const T w2 = (k ? (T)1 : (T)-1) * Wt<T>(a, i) * Wt<T>(b, j);
+            const T vx = w0 * a0x + w1 * a1x + w2 * a2x;
+            const T vy = w0 * a0y + w1 * a1y + w2 * a2y;
+            const T vz = w0 * a0z + w1 * a1z + w2 * a2z;
+            y[n] += s * (vx * vx + vy * vy + vz * vz);
+            if (COEF) {
+                const T phi = Wt<T>(a, i) * Wt<T>(b, j) * Wt<T>(c, k);
+                y[n] += (T)0.125 * det * gq[q] * phi * phi;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------------------
@@ -556,12 +604,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
                 for (int n = 0; n < 8; ++n) u[n] = su[lid[n]];
                 q1_cell<T, COEF>(Xe, u, kq, gq, y);
             } else if (MODE == EM_DIAG) {
-                for (int n = 0; n < 8; ++n) {
-                    T u[8], yy[8];
-                    for (int m = 0; m < 8; ++m) u[m] = m == n ? (T)1 : (T)0;
-                    q1_cell<T, COEF>(Xe, u, kq, gq, yy);
-                    y[n] = yy[n];
-                }
+                q1_cell_diag<T, COEF>(Xe, kq, gq, y);
             } else {
                 for (int n = 0; n < 8; ++n) y[n] = 0;
                 for (int q = 0; q < 8; ++q) {

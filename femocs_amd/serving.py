@@ -118,7 +118,6 @@ class _Resident:
     mesh: fgpu.Mesh
     sys: fgpu.System
     bytes: int
-    fresh: bool = True                       # solution still holds the initial fill
 
 
 @dataclass
@@ -202,7 +201,16 @@ class Server:
         s = fgpu.System(m)
         if self.configure is not None:
             self.configure(m, s)
+        # the system is assembled once here, on the unmoved tip; requests move it and replay
+        # the same terms on the device
+        s.reset()
+        s.add_dirichlet(2, 0.0)
+        s.add_neumann(8, None, c.field)
+        s.apply_dirichlet()
         s.fill_solution(0.0)
+        if self.coarse:
+            # built once per residency: it stays a valid preconditioner as the mesh moves
+            s.set_coarse(True)
         b = m.device_bytes() + s.device_bytes()
         if self.coarse:
             nc = (m.n_dofs + 1023) // 1024
@@ -226,26 +234,23 @@ class Server:
             # the step would need a new mesh; this request is solved on the last valid one
             self.stats.rejected_moves += 1
         s = r.sys
-        if r.fresh:
-            s.reset()
-            s.add_dirichlet(2, 0.0)
-            s.add_neumann(8, None, c.field)
-            s.apply_dirichlet()
+        # same boundary terms, new geometry: rebuilt on the device
+        s.reassemble()
+        if not self.warm:
             s.fill_solution(0.0)
-            if self.coarse:
-                # built once per residency: it stays a valid preconditioner as the mesh moves
-                s.set_coarse(True)
-            r.fresh = False
-        else:
-            # same boundary terms, new geometry: rebuilt on the device
-            s.reassemble()
-            if not self.warm:
-                s.fill_solution(0.0)
         if self.record_inputs:
             xyz, vod = r.mesh.dofs()
             req.positions = c.X0.copy()
             req.positions[vod] = xyz
             req.x_init = s.solution()
+
+    def make_resident(self, cids=None) -> int:
+        """Imports and assembles the clients' meshes ahead of their requests (while the budget
+        allows); returns how many are resident."""
+        for cid in (range(len(self.clients)) if cids is None else cids):
+            if self._make_resident(self.clients[cid]) is None:
+                break
+        return len(self.resident)
 
     # ---- request flow
     def submit(self, req: Request):
@@ -323,34 +328,47 @@ class Server:
 
 
 def run_clients(server: Server, rate: float, duration: float, seed: int = 0, max_requests: int | None = None,
-                clock=time.perf_counter) -> ServeStats:
-    """Drives the server with closed-loop MD clients: each client submits its next timestep's
-    solve an exponential think time (total rate `rate` over all clients) after its previous
-    result came back.  Runs `duration` seconds (or until `max_requests` were submitted) and
-    then drains what is in flight; throughput counts the solves finished inside the window."""
+                clock=time.perf_counter, arrivals: str = "poisson") -> ServeStats:
+    """Drives the server with MD clients for `duration` seconds (or until `max_requests` were
+    submitted), then drains what is in flight; throughput counts the solves finished inside the
+    window.
+
+    arrivals "poisson": the clients' requests form one Poisson process of total rate `rate`
+    (each client a Poisson process of rate / n); a request's latency runs from its arrival
+    time, and a client's requests are solved in arrival order.
+    arrivals "closed": each client submits its next request an exponential think time (mean
+    n / rate) after its previous result came back."""
+    if arrivals not in ("poisson", "closed"):
+        raise ValueError("arrivals must be 'poisson' or 'closed'")
     rng = np.random.default_rng(seed)
     n = len(server.clients)
     mean_think = n / rate
     t0 = clock()
     server.stats.t_start = t0
-    events = [(t0 + rng.exponential(mean_think), cid) for cid in range(n)]
+    if arrivals == "closed":
+        events = [(t0 + rng.exponential(mean_think), cid) for cid in range(n)]
+    else:
+        events = [(t0 + rng.exponential(1.0 / rate), int(rng.integers(n)))]
     heapq.heapify(events)
     submitted = 0
     while True:
         now = clock()
         in_window = now - t0 < duration and (max_requests is None or submitted < max_requests)
         while in_window and events and events[0][0] <= now:
-            _, cid = heapq.heappop(events)
+            ta, cid = heapq.heappop(events)
             c = server.clients[cid]
-            server.submit(Request(cid, c.seq, now))
+            server.submit(Request(cid, c.seq, ta if arrivals == "poisson" else now))
             c.seq += 1
             submitted += 1
+            if arrivals == "poisson":
+                heapq.heappush(events, (ta + rng.exponential(1.0 / rate), int(rng.integers(n))))
             if max_requests is not None and submitted >= max_requests:
                 break
         finished = server.step(clock)
         now = clock()
-        for req in finished:
-            heapq.heappush(events, (now + rng.exponential(mean_think), req.client))
+        if arrivals == "closed":
+            for req in finished:
+                heapq.heappush(events, (now + rng.exponential(mean_think), req.client))
         if not in_window and server.idle():
             break
         if not finished and server.idle() and events:

@@ -104,3 +104,25 @@ def test_current_heat_solver_and_transfer(ctx, tip):
     outside = np.ones(len(tip.nodes), bool)
     outside[vod] = False
     assert np.all(sol[outside, :4] == 0.0) and np.all(sol[outside, 4] == 300.0)
+
+
+def test_heat_diagonal_matches_operator(ctx, tip):
+    """The one-pass element diagonal (stiffness kappa(T) + mass gamma) equals e_i^T A e_i of
+    the operator, seen through the lifting b_i = A_ii g_i on constrained rows."""
+    gm = fgpu.Mesh(ctx, tip.nodes, tip.bulk_cells(), *MARKS)
+    cur = _gpu_current(ctx, gm, _face_values(gm, 2, 5e-3, 6))
+    xyz, _ = gm.dofs()
+    heat = fgpu.System(gm)
+    heat.set_solution(300.0 + 100.0 * np.exp(-((xyz[:, 2] - xyz[:, 2].max()) / 20.0) ** 2))
+    heat.reset()
+    heat.heat_terms(cur, O.CU_RHO_CP / DT, O.LORENTZ)
+    heat.add_dirichlet(7, 1.0)
+    heat.apply_dirichlet()
+    b = heat.rhs()
+    fixed = np.nonzero(heat.solution() == 1.0)[0]
+    assert len(fixed) > 10
+    for i in fixed[:: max(1, len(fixed) // 6)][:6]:
+        e = np.zeros(gm.n_dofs)
+        e[i] = 1.0
+        aii = heat.apply(e)[i]
+        assert abs(b[i] - aii) <= 1e-13 * abs(aii), (i, b[i], aii)

@@ -86,3 +86,22 @@ def test_plain_server_solves_cold(ctx, clients):
     for r in srv.stats.requests:
         assert not np.any(r.x_init)
     _check(ctx, srv, o)
+
+
+def test_serving_two_level(ctx, clients):
+    """Mixed serving with the coarse level built at residency (it goes stale as the tips
+    move): every solve reaches the tolerance and matches the oracle on its moved mesh."""
+    o = fgpu.opts(tol=1e-9, relative=True, precision=fgpu.FGPU_MIXED, inner_tol=0.1, use_graph=False)
+    for c in clients:
+        c.seq = 0
+    srv = serving.Server(ctx, clients, o, step_iterations=16, record_inputs=True, coarse=True)
+    serving.run_clients(srv, rate=400.0, duration=60.0, max_requests=9, seed=4)
+    assert srv.idle()
+    done = [r for r in srv.stats.requests if r.t_done >= 0]
+    assert len(done) == 9 and all(r.iterations > 0 for r in done)
+    for r in [q for q in done if q.seq >= 1][:2]:
+        c = clients[r.client]
+        _, xo, it, _, _ = O.poisson_laplace(r.positions, c.cells, c.field, tol=1e-13, relative=True,
+                                            precond="ssor")
+        assert it > 0
+        assert np.linalg.norm(r.x_final - xo) <= 1e-8 * np.linalg.norm(xo), (r.client, r.seq)
