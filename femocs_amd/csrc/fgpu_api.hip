@@ -1641,6 +1641,8 @@ struct fgpu_batch {
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_k = 0;
+    std::vector<double> done;     // S_DONE of each member after the last step (valid while
+    bool done_valid = false;      // the membership has not changed since)
 };
 
 static void batch_drop_graph(fgpu_batch *b) {
@@ -1694,6 +1696,7 @@ extern "C" int fgpu_batch_add(fgpu_batch *b, fgpu_sys *s) {
         b->sys.push_back(s);
         s->batches++;
         b->dirty = true;
+        b->done_valid = false;
         return FGPU_OK;
     });
 }
@@ -1742,6 +1745,8 @@ extern "C" int fgpu_batch_step(fgpu_batch *b, int iterations, int *n_running) {
         int running = 0;
         for (int i = 0; i < n; ++i) running += h[i] == 0.0;
         if (n_running) *n_running = running;
+        b->done.assign(h, h + n);
+        b->done_valid = true;
         return FGPU_OK;
     });
 }
@@ -1769,9 +1774,12 @@ extern "C" int fgpu_batch_retire(fgpu_batch *b, int max, fgpu_sys **out, fgpu_so
         *n_out = 0;
         std::vector<fgpu_sys *> keep;
         const bool mixed = b->o.precision == FGPU_MIXED;
-        for (auto *s : b->sys) {
+        const bool known = b->done_valid && b->done.size() == b->sys.size();
+        for (size_t i = 0; i < b->sys.size(); ++i) {
+            fgpu_sys *s = b->sys[i];
             double done = 0;
-            HIPCHECK(hipMemcpy(&done, s->scal + S_DONE, sizeof(double), hipMemcpyDeviceToHost));
+            if (known) done = b->done[i];
+            else HIPCHECK(hipMemcpy(&done, s->scal + S_DONE, sizeof(double), hipMemcpyDeviceToHost));
             if (done != 0.0 && *n_out < max) {
                 out[*n_out] = s;
                 s->batches--;
@@ -1784,6 +1792,7 @@ extern "C" int fgpu_batch_retire(fgpu_batch *b, int max, fgpu_sys **out, fgpu_so
         if (keep.size() != b->sys.size()) {
             b->sys.swap(keep);
             b->dirty = true;
+            b->done_valid = false;
             if (b->sys.empty()) b->coef = -1;
         }
         return FGPU_OK;

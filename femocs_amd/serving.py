@@ -138,12 +138,15 @@ class ServeStats:
         span = max(self.t_end - self.t_start, 1e-12)
         its = np.array([r.iterations for r in done]) if done else np.zeros(1)
         in_window = sum(1 for r in done if r.t_done <= self.t_end)
+        # mean number of solves on the GPU at once (admission to result, over the span)
+        busy = sum(r.t_done - r.t_admit for r in done if r.t_admit >= 0)
         return {
             "solves": in_window,
             "solves_per_s": in_window / span,
             "p50_ms": float(np.percentile(lat, 50)) if done else None,
             "p99_ms": float(np.percentile(lat, 99)) if done else None,
             "mean_iterations": float(its.mean()),
+            "mean_in_flight": round(busy / span, 2),
             "evictions": self.evictions,
             "rejected_moves": self.rejected_moves,
             "prep_seconds": round(self.prep_seconds, 4),
