@@ -69,7 +69,8 @@ def make_clients(n: int, small: bool):
 
 def serve(ctx, clients, precision, batched, warm, steps, warmup, tuned, seed, coarse=False, arrivals="poisson"):
     from femocs_amd import fgpu, serving
-    o = fgpu.opts(tol=1e-9, relative=True, precision=precision, inner_tol=0.1, use_graph=False)
+    # inner_tol: the FP32 CG runs until |r| dropped 100-fold, then the FP64 residual replaces it
+    o = fgpu.opts(tol=1e-9, relative=True, precision=precision, inner_tol=0.01, use_graph=False)
     cells, ppb = tuned
 
     def configure(mesh, system):
@@ -134,7 +135,8 @@ def cpu_baseline(client):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=160, help="timed requests")
+    ap.add_argument("--steps", type=int, default=480, help="timed requests (steady state: the queue of the "
+                    "busiest clients drains for a while after the last arrival)")
     ap.add_argument("--warmup", type=int, default=16, help="untimed requests (meshes become resident)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")

@@ -295,6 +295,41 @@ template <class T> __device__ __forceinline__ T coarse_term(const Desc &d, int i
     return c * keep;
 }
 
+// Slot sums of the kDofIter dofs a thread of a dof workgroup handles (base + threadIdx.x +
+// m * kBlock), in the same order as gather_slots.  The slot offsets of all of them are loaded
+// first, then up to four slots each in one wave (indices clamped into the dof's own range, the
+// extra values masked to 0), so a thread waits twice instead of once per slot; dofs with more
+// than four slots (rare) finish in a loop.
+template <class T> __device__ __forceinline__ void gather_slots4(const Desc &d, int base, T (&q)[kDofIter]) {
+    const auto Ye = G(reinterpret_cast<const T *>(d.Ye));
+    const auto gp = G(d.gptr);
+    int b[kDofIter], e[kDofIter];
+#pragma unroll
+    for (int m = 0; m < kDofIter; ++m) {
+        const int i = base + threadIdx.x + m * kBlock;
+        const int ii = i < d.n_dofs ? i : d.n_dofs - 1;
+        b[m] = gp[ii];
+        e[m] = gp[ii + 1];
+    }
+    T v[kDofIter][4];
+#pragma unroll
+    for (int m = 0; m < kDofIter; ++m)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int idx = b[m] + k < e[m] ? b[m] + k : b[m];
+            const T y = Ye[idx];
+            v[m][k] = b[m] + k < e[m] ? y : (T)0;
+        }
+#pragma unroll
+    for (int m = 0; m < kDofIter; ++m) {
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s += v[m][k];
+        for (int k = b[m] + 4; k < e[m]; ++k) s += Ye[k];
+        q[m] = s;
+    }
+}
+
 template <class T> __device__ __forceinline__ T gather_slots(const Desc &d, int i) {
     const auto Ye = G(reinterpret_cast<const T *>(d.Ye));
     const int e = G(d.gptr)[i + 1];
