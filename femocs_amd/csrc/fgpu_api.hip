@@ -732,6 +732,7 @@ static Desc make_desc32(fgpu_sys *s, bool coarse = false) {
         d.cc = s->cc;
         d.cinv = s->cinv;
         d.nc = s->nc;
+        d.agg_shift = s->agg_shift;
     }
     return d;
 }
@@ -847,6 +848,7 @@ struct BatchTabs {
     const ETab *etab = nullptr;
     const int2 *dtab = nullptr, *ctab = nullptr;
     int ne = 0, nd = 0, nct = 0;   // nct: coarse-product workgroups (systems with a coarse level)
+    int max_nc = 0;                // largest coarse size of the batch (LDS of the coarse product)
     bool affine32 = true;          // every system's FP32 descriptor uses the tet-split form
     size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the element kernel, largest patch of the batch
 };
@@ -860,6 +862,7 @@ static size_t patch_lds(int node_cap, int cells, size_t tsize, bool affine = fal
 static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> &sets, const std::vector<fgpu_sys *> &sys,
                               void **own_buf = nullptr, size_t *own_cap = nullptr) {
     std::vector<int2> et0, dt, ct;
+    int max_nc_b = 0;
     std::vector<ETab> et;
     for (size_t i = 0; i < sys.size(); ++i) {
         et0.clear();
@@ -877,8 +880,10 @@ static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> 
             et.push_back(w);
         }
         append_table(dt, (int)i, sys[i]->nb_dof);
-        if (sys[i]->coarse)
+        if (sys[i]->coarse) {
             for (int b = 0; b < (sys[i]->nc + kCoarseRows - 1) / kCoarseRows; ++b) ct.push_back(make_int2((int)i, b));
+            max_nc_b = std::max(max_nc_b, sys[i]->nc);
+        }
     }
     const size_t n = sys.size();
     const size_t dbytes = sets.size() * n * sizeof(Desc);
@@ -916,6 +921,7 @@ static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> 
     t.ne = (int)et.size();
     t.nd = (int)dt.size();
     t.nct = (int)ct.size();
+    t.max_nc = max_nc_b;
     for (auto *x : sys) {
         const fgpu_mesh *m = static_cast<const fgpu_mesh *>(x->mesh);
         t.lds64 = std::max(t.lds64, patch_lds(m->patches.node_cap, m->patches.cells, sizeof(double)));
@@ -1084,6 +1090,10 @@ static void lift(fgpu_sys *s) {
     const int n = m->n_dofs;
     ensure_diag(s);
     launch_invdiag(st, n, s->d_dmask, s->diag, s->invd, s->invd32);
+    if (s->g_zero) {
+        launch_lift(st, n, s->d_dmask, s->g, nullptr, s->diag, s->b, s->x);
+        return;
+    }
     Desc d = make_desc64(s, true);
     d.ua = s->g;
     d.q = s->tmp;
@@ -1137,6 +1147,8 @@ extern "C" int fgpu_sys_apply_dirichlet(fgpu_sys *s) {
             mk[i] = s->dmask[d];
             gv[i] = s->dmask[d] ? s->gval[d] : 0.0;
         }
+        s->g_zero = true;
+        for (int i = 0; i < n; ++i) s->g_zero = s->g_zero && gv[i] == 0.0 && !std::signbit(gv[i]);
         h2d(s->d_dmask, mk.data(), n, st);
         h2d(s->g, gv.data(), n, st);
         lift(s);
@@ -1246,22 +1258,25 @@ extern "C" int fgpu_sys_apply(fgpu_sys *s, const double *x, double *y) {
 // two-level preconditioner of the mixed solver
 // ------------------------------------------------------------------------------------------
 // E = Z^T A Z for the system's current operator and constraints, Z the free-dof indicator of
-// the blocks of kDofBlock internal dofs.  The block graph (blocks sharing a cell) is coloured
+// the aggregates of 2^shift internal dofs.  The block graph (blocks sharing a cell) is coloured
 // so that no two blocks of a colour are within distance 2; A applied to the indicator of all
 // blocks of one colour then gives, summed over the free dofs of a block b1, E[b1][b2] for the
 // only block b2 of that colour in b1's closed neighbourhood.  One FP64 apply per colour, all
 // on the device; E^-1 by Gauss-Jordan sweeps on the device, stored in FP32.
-static int build_coarse(fgpu_sys *s) {
+static int build_coarse(fgpu_sys *s, int shift) {
     fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
     hipStream_t st = s->ctx->stream;
     const int n = m->n_dofs;
-    const int nc = (n + kDofBlock - 1) / kDofBlock;
+    const int agg = 1 << shift, g = agg / kAggMin;
+    const int nc = (n + agg - 1) / agg;
+    const int nq = (n + kAggMin - 1) / kAggMin;   // 256-dof rounds
+    if (nc > kCoarseMax) throw std::invalid_argument("too many aggregates: choose larger ones");
     // block graph
     std::vector<int64_t> pairs;
     for (int c = 0; c < m->n_cells; ++c) {
         int b[8], nb = 0;
         for (int k = 0; k < 8; ++k) {
-            const int x = m->idof_of_dof[m->cells_lex[8 * (size_t)c + k]] / kDofBlock;
+            const int x = m->idof_of_dof[m->cells_lex[8 * (size_t)c + k]] >> shift;
             bool seen = false;
             for (int j = 0; j < nb; ++j) seen = seen || b[j] == x;
             if (!seen) b[nb++] = x;
@@ -1300,28 +1315,36 @@ static int build_coarse(fgpu_sys *s) {
     }
     // block sums of A (indicator of each colour)
     int *d_color = dalloc<int>(nc);
-    double *d_sums = dalloc<double>((size_t)ncol * nc);
+    double *d_sums = dalloc<double>((size_t)ncol * nq);
     h2d(d_color, color.data(), nc, st);
     Desc d = make_desc64(s, true);
     d.ua = s->tmp;
     d.q = s->q;
     BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
     for (int c = 0; c < ncol; ++c) {
-        launch_coarse_indicator(st, n, s->d_dmask, d_color, c, s->tmp);
+        launch_coarse_indicator(st, n, s->d_dmask, d_color, shift, c, s->tmp);
         launch_elem<double>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds64);
         launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, false);
-        launch_coarse_sums(st, n, s->d_dmask, s->q, d_sums + (size_t)c * nc);
+        launch_coarse_sums(st, n, s->d_dmask, s->q, d_sums + (size_t)c * nq);
     }
     HIPCHECK(hipGetLastError());
-    std::vector<double> sums((size_t)ncol * nc);
-    d2h(sums.data(), d_sums, sums.size(), st);
+    std::vector<double> qsums((size_t)ncol * nq);
+    d2h(qsums.data(), d_sums, qsums.size(), st);
     std::vector<unsigned char> mk(n);
     d2h(mk.data(), s->d_dmask, n, st);
     HIPCHECK(hipStreamSynchronize(st));
     dfree(d_color);
     dfree(d_sums);
     std::vector<int> nfree(nc, 0);
-    for (int i = 0; i < n; ++i) nfree[i / kDofBlock] += mk[i] ? 0 : 1;
+    for (int i = 0; i < n; ++i) nfree[i >> shift] += mk[i] ? 0 : 1;
+    // per colour, the sums over each aggregate's rounds in round order
+    std::vector<double> sums((size_t)ncol * nc, 0.0);
+    for (int c = 0; c < ncol; ++c)
+        for (int a = 0; a < nc; ++a) {
+            double v = 0.0;
+            for (int u = 0; u < g && a * g + u < nq; ++u) v += qsums[(size_t)c * nq + a * g + u];
+            sums[(size_t)c * nc + a] = v;
+        }
     std::vector<double> E((size_t)nc * nc, 0.0);
     for (int b1 = 0; b1 < nc; ++b1) {
         E[(size_t)b1 * nc + b1] = sums[(size_t)color[b1] * nc + b1];
@@ -1370,6 +1393,7 @@ static int build_coarse(fgpu_sys *s) {
             throw std::runtime_error("coarse operator inversion failed");
         }
     s->nc = nc;
+    s->agg_shift = shift;
     s->coarse = true;
     return ncol;
 }
@@ -1387,7 +1411,17 @@ extern "C" int fgpu_sys_set_coarse(fgpu_sys *s, int enable, int *colors) {
         if (colors) *colors = 0;
         if (!enable) return FGPU_OK;
         if (!s->lifted) throw std::invalid_argument("apply the Dirichlet conditions before building the coarse level");
-        const int ncol = build_coarse(s);
+        int shift;
+        if (enable == 1) {
+            // the coarse product reads nc^2 floats per iteration: aggregates grow with the mesh
+            const int n = static_cast<fgpu_mesh *>(s->mesh)->n_dofs;
+            shift = n <= 700000 ? 8 : (n <= 1500000 ? 9 : 10);
+        } else if (enable == 256 || enable == 512 || enable == 1024) {
+            shift = enable == 256 ? 8 : (enable == 512 ? 9 : 10);
+        } else {
+            throw std::invalid_argument("aggregate size must be 256, 512 or 1024 (or 1: by mesh size)");
+        }
+        const int ncol = build_coarse(s, shift);
         if (colors) *colors = ncol;
         return FGPU_OK;
     });
@@ -1399,7 +1433,7 @@ extern "C" int fgpu_sys_get_coarse(fgpu_sys *s, int *nc, int *block, float *einv
         const fgpu_mesh *m = static_cast<const fgpu_mesh *>(s->mesh);
         *nc = s->coarse ? s->nc : 0;
         if (block)
-            for (int d = 0; d < m->n_dofs; ++d) block[d] = m->idof_of_dof[d] / kDofBlock;
+            for (int d = 0; d < m->n_dofs; ++d) block[d] = m->idof_of_dof[d] >> s->agg_shift;
         if (einv && s->coarse) {
             HIPCHECK(hipSetDevice(s->ctx->device));
             d2h(einv, s->cinv, (size_t)s->nc * s->nc, s->ctx->stream);
@@ -1474,7 +1508,7 @@ static void cg_iter(hipStream_t st, const Desc *D, const BatchTabs &t, int n, bo
     launch_gather_cg<T>(st, D, t.dtab, t.nd);
     launch_scal_alpha(st, D, n);
     launch_update_cg<T>(st, D, t.dtab, t.nd);
-    if (sizeof(T) == 4) launch_coarse(st, D, nullptr, t.ctab, t.nct, 0);
+    if (sizeof(T) == 4) launch_coarse(st, D, nullptr, t.ctab, t.nct, 0, t.max_nc);
     launch_scal_beta(st, D, n, reliable);
 }
 
@@ -1492,7 +1526,7 @@ static void reliable_update(hipStream_t st, const Desc *D32, const Desc *R64, co
     launch_mixed_acc(st, D32, R64, t.dtab, t.nd);
     fp64_residual(st, R64, D32, t, n, coef, 1);
     launch_mixed_restart(st, D32, R64, t.dtab, t.nd, 0);
-    launch_coarse(st, D32, R64, t.ctab, t.nct, 2);
+    launch_coarse(st, D32, R64, t.ctab, t.nct, 2, t.max_nc);
     launch_scal_restart(st, D32, R64, n, 0);
 }
 
@@ -1562,7 +1596,7 @@ static void start_solves(fgpu_ctx *c, const std::vector<fgpu_sys *> &sys, const 
     fp64_residual(c->stream, R64, D32, t, n, coef, mixed ? 1 : 0);
     if (mixed) {
         launch_mixed_restart(c->stream, D32, R64, t.dtab, t.nd, 1);
-        launch_coarse(c->stream, D32, R64, t.ctab, t.nct, 1);
+        launch_coarse(c->stream, D32, R64, t.ctab, t.nct, 1, t.max_nc);
         launch_scal_restart(c->stream, D32, R64, n, 1);
     }
     HIPCHECK(hipGetLastError());
@@ -1820,7 +1854,7 @@ static double time_launches(fgpu_ctx *c, const Desc *D, const BatchTabs &t, bool
     if (which == 2 && (precision != FGPU_MIXED || t.nct == 0)) throw std::invalid_argument("no coarse level in the batch");
     auto one = [&] {
         if (which == 2) {
-            launch_coarse(c->stream, D, nullptr, t.ctab, t.nct, 0);
+            launch_coarse(c->stream, D, nullptr, t.ctab, t.nct, 0, t.max_nc);
         } else if (which == 0) {
             if (precision == FGPU_MIXED)
                 launch_elem<float>(c->stream, D, t.etab, t.ne, coef, false, false, S_BETA, t.lds32, t.affine32);

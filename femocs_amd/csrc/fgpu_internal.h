@@ -15,6 +15,10 @@ namespace fgpu {
 constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
 constexpr int kDofIter = 4;   // dofs per thread in the per-dof kernels
 constexpr int kDofBlock = kBlock * kDofIter;
+// coarse aggregates of the two-level preconditioner: 2^agg_shift consecutive internal dofs, a
+// multiple of kAggMin (one round of a dof workgroup: its sums of r are kept per round)
+constexpr int kAggMin = kBlock;
+constexpr int kAggPerBlock = kDofIter;
 constexpr int kPatchCells = 256;  // largest patch: cells per element workgroup (one per thread)
 constexpr int kPatchNodes = 640;  // largest number of distinct nodes a patch may touch
 constexpr int kPatchPerBlock = 4; // default patches processed in turn by one element workgroup
@@ -87,7 +91,7 @@ struct Desc {
     // dof block (T), nc x nc inverse of the coarse operator (row-major), nc = dof blocks
     void *cc;
     const float *cinv;
-    int nc;
+    int nc, agg_shift;            // aggregates and log2 of their size
 };
 // Element-kernel work item: one workgroup, starting at patch p of system sys, with that
 // patch's first cell and node and its cell and node counts (nc = 0: nothing to do), so the
@@ -95,8 +99,10 @@ struct Desc {
 struct ETab {
     int sys, p, c0, n0, nc, np, pad0, pad1;
 };
-constexpr int kPartRegions = 6;   // per descriptor set: p.q, r.z, r.r, b.b, block sums of r, s.c
-constexpr int kCoarseRows = 4;    // coarse rows per workgroup of the coarse product
+constexpr int kPartRegions = 5 + kAggPerBlock;   // per descriptor set: p.q, r.z, r.r, b.b, s.c,
+                                                 // aggregate sums of r (kAggPerBlock regions)
+constexpr int kCoarseRows = 16;   // coarse rows per workgroup of the coarse product
+constexpr int kCoarseMax = 8000;  // aggregates per system (their sums are staged in LDS)
 
 struct Mesh {
     fgpu_ctx *ctx = nullptr;
@@ -181,10 +187,12 @@ struct Sys {
     };
     std::vector<RhsTerm> rhs_terms;
     bool lifted = false;          // fgpu_sys_apply_dirichlet ran since the last reset
+    bool g_zero = false;          // every Dirichlet value is +0.0 (the lifting's A g vanishes)
     // two-level preconditioner of the mixed solver (fgpu_sys_set_coarse)
     bool coarse = false;
     int batches = 0;              // running batches holding the system (their tables point here)
-    int nc = 0;                   // coarse size: dof blocks of kDofBlock internal dofs
+    int nc = 0;                   // coarse size: aggregates of 2^agg_shift internal dofs
+    int agg_shift = 10;
     float *cinv = nullptr;        // nc x nc
     float *cc = nullptr;          // nc
     bool desc32_affine = false;   // the last FP32 descriptor built uses the tet-split form

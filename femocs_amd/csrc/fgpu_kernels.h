@@ -19,8 +19,9 @@ void launch_scal_restart(hipStream_t s, const Desc *D32, const Desc *D64, int n,
 void launch_mixed_restart(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk, int first);
 void launch_mixed_acc(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk);
 void launch_collect(hipStream_t s, const Desc *D, int n, int slot, double *out);
-void launch_coarse(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk, int mode);
-void launch_coarse_indicator(hipStream_t s, int n, const unsigned char *mask, const int *color, int c, double *u);
+void launch_coarse(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk, int mode, int max_nc);
+void launch_coarse_indicator(hipStream_t s, int n, const unsigned char *mask, const int *color, int shift, int c,
+                             double *u);
 void launch_coarse_sums(hipStream_t s, int n, const unsigned char *mask, const double *q, double *out);
 void launch_gj_step(hipStream_t s, int n, int k, const double *A, double *B);
 void launch_to_float(hipStream_t s, size_t n, const double *a, float *out);

@@ -290,7 +290,7 @@ template <class V4, class T> __device__ __forceinline__ V4 ldg4(const V4 *p) {
 template <class T> __device__ __forceinline__ T coarse_term(const Desc &d, int i) {
     // both loads unconditional and independent (a select on the mask would make the second
     // wait for the first); c * 1 and c * 0 are exact
-    const T c = G(reinterpret_cast<const T *>(d.cc))[i / kDofBlock];
+    const T c = G(reinterpret_cast<const T *>(d.cc))[i >> d.agg_shift];
     const T keep = (T)(1 - (int)G(d.dmask)[i]);
     return c * keep;
 }
@@ -570,7 +570,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
             if (MODE == EM_APPLY) {
                 va[r] = ua[nd[r]];
                 vb[r] = ubp[nd[r]];
-                vc[r] = ccp[nd[r] / kDofBlock];
+                vc[r] = ccp[nd[r] >> d.agg_shift];
                 mk[r] = G(d.dmask)[nd[r]];
             }
         }
@@ -704,7 +704,7 @@ __global__ void k_lift(int n, const unsigned char *__restrict__ mask, const doub
     if (mask[i]) {
         b[i] = g[i] * diag[i];
         x[i] = g[i];
-    } else {
+    } else if (Ag) {       // null: g is zero, so is A g
         b[i] -= Ag[i];
     }
 }
@@ -1066,20 +1066,31 @@ void launch_lift(hipStream_t s, int n, const unsigned char *mask, const double *
 // coarse operator of the two-level preconditioner: E = Z^T A Z from A applied to the
 // indicators of one colour of dof blocks at a time, then E^-1 by Gauss-Jordan sweeps
 // ------------------------------------------------------------------------------------------
-__global__ void k_coarse_indicator(int n, const unsigned char *__restrict__ mask, const int *__restrict__ color, int c,
-                                   double *__restrict__ u) {
+__global__ void k_coarse_indicator(int n, const unsigned char *__restrict__ mask, const int *__restrict__ color,
+                                   int shift, int c, double *__restrict__ u) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) u[i] = (!mask[i] && color[i / kDofBlock] == c) ? 1.0 : 0.0;
+    if (i < n) u[i] = (!mask[i] && color[i >> shift] == c) ? 1.0 : 0.0;
 }
 
-// out[b] = sum of q over the free dofs of block b (one workgroup per block, fixed order)
+// out[q] = sum of q over the free dofs of the 256-dof round q (one workgroup per dof block of
+// kAggPerBlock rounds, fixed order)
 __global__ __launch_bounds__(kBlock) void k_coarse_sums(int n, const unsigned char *__restrict__ mask,
                                                          const double *__restrict__ q, double *__restrict__ out) {
-    double v[1] = {0.0};
-    for (int m = 0, i = blockIdx.x * kDofBlock + threadIdx.x; m < kDofIter; ++m, i += kBlock)
-        if (i < n && !mask[i]) v[0] += q[i];
-    block_sum<1>(v);
-    if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+    double v[kAggPerBlock];
+#pragma unroll
+    for (int a = 0; a < kAggPerBlock; ++a) v[a] = 0.0;
+#pragma unroll
+    for (int m = 0; m < kDofIter; ++m) {
+        const int i = blockIdx.x * kDofBlock + threadIdx.x + m * kBlock;
+        if (i < n && !mask[i]) v[m / (kDofIter / kAggPerBlock)] += q[i];
+    }
+    block_sum<kAggPerBlock>(v);
+    if (threadIdx.x == 0) {
+        const int na = (n + kAggMin - 1) / kAggMin;
+#pragma unroll
+        for (int a = 0; a < kAggPerBlock; ++a)
+            if (blockIdx.x * kAggPerBlock + a < na) out[blockIdx.x * kAggPerBlock + a] = v[a];
+    }
 }
 
 // one Gauss-Jordan exchange step on pivot k (no pivoting: E is SPD), out of place:
@@ -1103,8 +1114,9 @@ __global__ void k_to_float(size_t n, const double *__restrict__ a, float *__rest
     if (i < n) out[i] = (float)a[i];
 }
 
-void launch_coarse_indicator(hipStream_t s, int n, const unsigned char *mask, const int *color, int c, double *u) {
-    if (n > 0) hipLaunchKernelGGL(k_coarse_indicator, dim3((n + 255) / 256), dim3(256), 0, s, n, mask, color, c, u);
+void launch_coarse_indicator(hipStream_t s, int n, const unsigned char *mask, const int *color, int shift, int c,
+                             double *u) {
+    if (n > 0) hipLaunchKernelGGL(k_coarse_indicator, dim3((n + 255) / 256), dim3(256), 0, s, n, mask, color, shift, c, u);
 }
 void launch_coarse_sums(hipStream_t s, int n, const unsigned char *mask, const double *q, double *out) {
     const int nb = (n + kDofBlock - 1) / kDofBlock;

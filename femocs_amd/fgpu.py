@@ -332,11 +332,13 @@ class System:
         self._c(self.ctx.lib.fgpu_sys_apply(self.h, np.ascontiguousarray(x, np.float64), y), "apply")
         return y
 
-    def set_coarse(self, enable: bool = True) -> int:
-        """Two-level preconditioner of the mixed solver (coarse blocks of 1024 dofs); returns
-        the number of operator applications the build took."""
+    def set_coarse(self, enable=True) -> int:
+        """Two-level preconditioner of the mixed solver: False removes it, True picks the
+        aggregate size by mesh size, 256 / 512 / 1024 sets it.  Returns the number of operator
+        applications the build took."""
+        mode = int(enable) if not isinstance(enable, bool) else int(enable)
         n = C.c_int(0)
-        self._c(self.ctx.lib.fgpu_sys_set_coarse(self.h, int(bool(enable)), C.byref(n)), "set_coarse")
+        self._c(self.ctx.lib.fgpu_sys_set_coarse(self.h, mode, C.byref(n)), "set_coarse")
         return n.value
 
     def coarse(self):

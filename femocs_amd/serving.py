@@ -170,7 +170,9 @@ class Server:
         self.max_batch = max_batch
         self.record_inputs = record_inputs
         self.configure = configure                    # called as configure(mesh, system) on import
-        self.coarse = coarse                          # two-level preconditioner, built at residency
+        # two-level preconditioner, built at residency: False, True (aggregate size by mesh
+        # size), 256 / 512 / 1024, or a function of the dof count returning one of those
+        self.coarse = coarse
         self.resident: OrderedDict[int, _Resident] = OrderedDict()   # LRU order
         self.known_bytes: dict[int, int] = {}
         self.pending: deque[Request] = deque()
@@ -213,10 +215,11 @@ class Server:
         s.fill_solution(0.0)
         if self.coarse:
             # built once per residency: it stays a valid preconditioner as the mesh moves
-            s.set_coarse(True)
+            s.set_coarse(self._aggregate(m.n_dofs))
         b = m.device_bytes() + s.device_bytes()
-        if self.coarse:
-            nc = (m.n_dofs + 1023) // 1024
+        agg = self._aggregate(m.n_dofs)
+        if agg:
+            nc = (m.n_dofs + agg - 1) // agg
             b += nc * (nc + 1) * 4
         self.known_bytes[c.cid] = b
         r = _Resident(m, s, b)
@@ -246,6 +249,13 @@ class Server:
             req.positions = c.X0.copy()
             req.positions[vod] = xyz
             req.x_init = s.solution()
+
+    def _aggregate(self, n_dofs: int) -> int:
+        """Aggregate size of the coarse level for a mesh of n_dofs (0: none)."""
+        c = self.coarse(n_dofs) if callable(self.coarse) else self.coarse
+        if c is True:
+            return 256 if n_dofs <= 700000 else (512 if n_dofs <= 1500000 else 1024)
+        return int(c) if c else 0
 
     def make_resident(self, cids=None) -> int:
         """Imports and assembles the clients' meshes ahead of their requests (while the budget

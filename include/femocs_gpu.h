@@ -249,13 +249,15 @@ int fgpu_interp_surface_cells(fgpu_interp *interp, int n, const double *xyz, int
 int fgpu_sys_heat_terms(fgpu_sys *heat, const fgpu_sys *current, double gamma, double lorentz);
 /* Two-level preconditioner for the FP32 CG of the mixed solver, no counterpart in the
  * reference (which uses SSOR, DealSolver::solve_cg, src/DealSolver.cpp:443-458):
- * M^-1 r = D^-1 r + Z E^-1 Z^T r, Z the indicator of blocks of 1024 consecutive internal dofs
- * (free dofs only), E = Z^T A Z.  enable != 0 builds E from the system's current operator and
- * Dirichlet set (call after fgpu_sys_apply_dirichlet) and stores E^-1 in FP32 (nc^2 * 4 bytes,
- * nc = ceil(n_dofs / 1024)); the FP64 solver and fgpu_sys_apply ignore it.  E may go stale as
- * the mesh moves or the conditions change: the preconditioner stays symmetric positive
- * definite, so the solve converges to the same tolerance, in more iterations; call again to
- * rebuild.  colors (may be NULL): the number of operator applications the build took. */
+ * M^-1 r = D^-1 r + Z E^-1 Z^T r, Z the indicator of aggregates of consecutive internal dofs
+ * (free dofs only), E = Z^T A Z.  enable: 0 removes it; 256, 512 or 1024 sets the aggregate
+ * size; 1 picks it by mesh size (256 up to 700k dofs, 512 up to 1.5M, else 1024).  Builds E
+ * from the system's current operator and Dirichlet set (call after fgpu_sys_apply_dirichlet)
+ * and stores E^-1 in FP32 (nc^2 * 4 bytes, nc = aggregates); the FP64 solver and
+ * fgpu_sys_apply ignore it.  E may go stale as the mesh moves or the conditions change: the
+ * preconditioner stays symmetric positive definite, so the solve converges to the same
+ * tolerance, in more iterations; call again to rebuild.  colors (may be NULL): the number of
+ * operator applications the build took. */
 int fgpu_sys_set_coarse(fgpu_sys *sys, int enable, int *colors);
 /* for checks: nc (0 without a coarse level), the block of every dof (API order, n_dofs ints,
  * may be NULL) and E^-1 (nc*nc floats, row-major, may be NULL) */

@@ -30,12 +30,13 @@ def meshes(small_mesh):
     return [small_mesh, m2, m3]
 
 
-def test_coarse_operator_matches_assembled(ctx, meshes):
+@pytest.mark.parametrize("agg", [256, 1024])
+def test_coarse_operator_matches_assembled(ctx, meshes, agg):
     """E^-1 from the device equals the inverse of Z^T A Z of the oracle's CSR matrix."""
     import scipy.sparse as sp
     m = meshes[2]
     s = _system(ctx, m)
-    ncol = s.set_coarse(True)
+    ncol = s.set_coarse(agg)
     assert ncol > 1
     block, einv = s.coarse()
     om = O.Mesh(m.nodes, m.vacuum_cells(), *MARKS)
@@ -47,7 +48,7 @@ def test_coarse_operator_matches_assembled(ctx, meshes):
     rp, ci, va = om.csr()
     A = sp.csr_matrix((va, ci, rp), shape=(om.n_dofs, om.n_dofs))
     nc = einv.shape[0]
-    assert nc == (om.n_dofs + 1023) // 1024 and block.max() == nc - 1
+    assert nc == (om.n_dofs + agg - 1) // agg and block.max() == nc - 1
     free = mask == 0
     Z = sp.csr_matrix((free.astype(float), (np.arange(om.n_dofs), block)), shape=(om.n_dofs, nc))
     E = (Z.T @ (A @ Z)).toarray()
