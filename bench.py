@@ -37,6 +37,20 @@ N_CLIENTS = 16
 RATE = 200.0
 
 
+def measured_traffic():
+    """HBM bytes per full-batch launch of the element apply from the committed counter summary
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE passes over `bench.py --traffic-probe`), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_k_patch_serving_16_clients.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -132,6 +146,50 @@ def cpu_baseline(client):
                       f"SSOR(1.2)-PCG {it} iterations {t2 - t1:.2f} s, 1 thread"}
 
 
+def traffic_probe(n_launches: int = 5) -> None:
+    """Counter target for the roofline's `traffic`: every client's system resident in one
+    mixed batch (as served), `n_launches` element applies over the whole batch, then two device
+    copies of 1 GiB (the calibration kernel of known size).  Run under
+    rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE)."""
+    import ctypes as C
+    from femocs_amd import fgpu, serving
+    # in this process, no worker pool: forked workers would inherit the profiler's state
+    clients = []
+    for i in range(N_CLIENTS):
+        clients.append(_make_client((i, N_CLIENTS, False)))
+        log(f"client {i} ready ({clients[-1].n_dofs} dofs)")
+    ctx = fgpu.Context(0)
+    o = fgpu.opts(tol=1e-9, relative=True, precision=fgpu.FGPU_MIXED, inner_tol=0.01, use_graph=False)
+
+    def configure(mesh, system):
+        mesh.set_patch_cells(256)
+        system.set_patches_per_block(1)
+
+    srv = serving.Server(ctx, clients, o, configure=configure, coarse=True)
+    for i in range(N_CLIENTS):
+        srv.make_resident([i])
+        log(f"client {i} resident")
+    b = fgpu.Batch(ctx, o)
+    for r in srv.resident.values():
+        b.add(r.sys)
+    log("element apply avg us", round(b.time_kernel(0, reps=n_launches) * 1e3, 2))
+    hip = C.CDLL("libamdhip64.so")
+    n = 1 << 30
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(n)) or hip.hipMalloc(C.byref(dst), C.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, C.c_size_t(n)):
+        raise RuntimeError("hipMemset failed")
+    for _ in range(2):
+        if hip.hipMemcpy(dst, src, C.c_size_t(n), 3):          # device to device
+            raise RuntimeError("hipMemcpy failed")
+    if hip.hipDeviceSynchronize():
+        raise RuntimeError("hipDeviceSynchronize failed")
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log("copies done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,7 +202,12 @@ def main():
     ap.add_argument("--jacobi", action="store_true", help="mixed run without the two-level preconditioner")
     ap.add_argument("--closed-loop", action="store_true",
                     help="clients submit an exponential think time after each result instead of Poisson arrivals")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only the counter target of roofline.traffic (see traffic_probe)")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the field solver has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
@@ -199,7 +262,7 @@ def main():
     coarse_ms = b.time_kernel(2, reps=30) if not args.jacobi else None
     del b
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_patch<float>",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(), "kernel": "k_patch<float>",
                 "avg_us": round(ms * 1e3, 2), "algorithmic_bytes": byts, "cells": H, "dofs": N,
                 "tet_split_form": affine,
                 "gather_avg_us": round(gather_ms * 1e3, 2),
