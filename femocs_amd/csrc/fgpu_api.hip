@@ -114,6 +114,7 @@ void fgpu::Sys::clear_terms() {
     for (auto &t : rhs_terms) dfree(t.dval);
     rhs_terms.clear();
     lifted = false;
+    rho_saved = false;
 }
 
 fgpu::Sys::~Sys() {
@@ -121,7 +122,7 @@ fgpu::Sys::~Sys() {
     dfree(b); dfree(x); dfree(r); dfree(z); dfree(p); dfree(q); dfree(Ye); dfree(invd); dfree(diag);
     dfree(g); dfree(tmp); dfree(kq); dfree(gq); dfree(fslot); dfree(fval); dfree(d_dmask);
     dfree(r32); dfree(z32); dfree(p32); dfree(q32); dfree(d32); dfree(Ye32); dfree(invd32);
-    dfree(kq32); dfree(gq32); dfree(scal); dfree(part); dfree(cinv); dfree(cc);
+    dfree(kq32); dfree(gq32); dfree(scal); dfree(part); dfree(cinv); dfree(cc); dfree(rho);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1103,6 +1104,33 @@ static void lift(fgpu_sys *s) {
     launch_lift(st, n, s->d_dmask, s->g, s->tmp, s->diag, s->b, s->x);
 }
 
+// rho = b / dof volume from the right-hand side as assembled (before the lifting); the dof
+// volumes come from the element kernel's volume mode gathered like any slot buffer
+static void save_charge(fgpu_sys *s) {
+    fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+    hipStream_t st = s->ctx->stream;
+    if (!s->rho) s->rho = dalloc<double>(std::max(1, m->n_dofs));
+    Desc d = make_desc64(s, true);
+    d.q = s->tmp;
+    BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
+    launch_elem_vol(st, t.sets[0], t.etab, t.ne, t.lds64);
+    launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, false);
+    launch_charge(st, m->n_dofs, s->b, s->tmp, s->rho);
+    s->rho_saved = true;
+}
+
+extern "C" int fgpu_sys_save_charge_density(fgpu_sys *s) {
+    if (!s) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        if (s->lifted) throw std::invalid_argument("the charge density is taken before fgpu_sys_apply_dirichlet");
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        save_charge(s);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+        return FGPU_OK;
+    });
+}
+
 // Rebuilds the right-hand side, the diagonal and the lifting for the current vertex positions
 // from the terms recorded since the last reset (same kernels, same order, so the result is
 // the one a reset and the same calls would give)
@@ -1127,6 +1155,7 @@ extern "C" int fgpu_sys_reassemble(fgpu_sys *s) {
                 launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, true);
             }
         }
+        if (s->rho_saved) save_charge(s);
         if (s->lifted) lift(s);
         HIPCHECK(hipGetLastError());
         return FGPU_OK;
@@ -1207,6 +1236,15 @@ extern "C" int fgpu_sys_get_solution(fgpu_sys *s, double *x) {
 extern "C" int fgpu_sys_get_rhs(fgpu_sys *s, double *b) {
     if (!s || !b) return FGPU_EINVAL;
     return get_vec(s, s->b, b);
+}
+
+extern "C" int fgpu_sys_export_charge_density(fgpu_sys *s, double *rho) {
+    if (!s || !rho) return FGPU_EINVAL;
+    if (s->rho_saved) return get_vec(s, s->rho, rho);
+    return guarded(s->ctx, [&] {
+        std::fill(rho, rho + static_cast<fgpu_mesh *>(s->mesh)->n_dofs, 0.0);
+        return FGPU_OK;
+    });
 }
 
 extern "C" int fgpu_sys_apply32(fgpu_sys *s, const double *x, double *y) {

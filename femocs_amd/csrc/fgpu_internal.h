@@ -188,6 +188,9 @@ struct Sys {
     std::vector<RhsTerm> rhs_terms;
     bool lifted = false;          // fgpu_sys_apply_dirichlet ran since the last reset
     bool g_zero = false;          // every Dirichlet value is +0.0 (the lifting's A g vanishes)
+    // charge density of a write step (fgpu_sys_save_charge_density), internal order
+    double *rho = nullptr;
+    bool rho_saved = false;
     // two-level preconditioner of the mixed solver (fgpu_sys_set_coarse)
     bool coarse = false;
     int batches = 0;              // running batches holding the system (their tables point here)

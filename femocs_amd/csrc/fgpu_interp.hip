@@ -455,7 +455,8 @@ __global__ __launch_bounds__(256) void k_interp_nodes(int n, const int *__restri
 __global__ __launch_bounds__(256) void k_extract(int n_nodes, const int *__restrict__ node_idof,
                                                  const double *__restrict__ u, const int *__restrict__ n2c_ptr,
                                                  const int *__restrict__ n2c, const int *__restrict__ hexes,
-                                                 const double *__restrict__ xyz, double empty, double *__restrict__ sol) {
+                                                 const double *__restrict__ xyz, const double *__restrict__ rho,
+                                                 double empty, double *__restrict__ sol) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
     double *s = sol + 5 * (size_t)n;
@@ -505,7 +506,7 @@ __global__ __launch_bounds__(256) void k_extract(int n_nodes, const int *__restr
     s[0] = mean.x;
     s[1] = mean.y;
     s[2] = mean.z;
-    s[3] = 0.0;
+    s[3] = rho ? rho[id] : 0.0;   // charge density of a write step, else zeros
     s[4] = u[id];
 }
 
@@ -761,7 +762,7 @@ extern "C" int fgpu_interp_extract(fgpu_interp *ip, fgpu_sys *s, int smoothen) {
         hipStream_t st = ip->ctx->stream;
         node_map(ip, m);
         hipLaunchKernelGGL(k_extract, dim3(grid(ip->n_nodes)), dim3(256), 0, st, ip->n_nodes, ip->node_idof, s->x,
-                           ip->n2c_ptr, ip->n2c, ip->hexes, ip->xyz, ip->empty, ip->sol);
+                           ip->n2c_ptr, ip->n2c, ip->hexes, ip->xyz, s->rho_saved ? s->rho : nullptr, ip->empty, ip->sol);
         if (smoothen && ip->n_tetnodes > 0)
             hipLaunchKernelGGL(k_smooth, dim3(grid(ip->n_tetnodes)), dim3(256), 0, st, ip->n_tetnodes, ip->vor_ptr,
                                ip->vor, ip->xyz, ip->decay, ip->sol);

@@ -8,6 +8,8 @@ void launch_elem(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool c
                  int scal_slot, size_t lds, bool affine_all = false);
 template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool coef, size_t lds);
 void launch_elem_rhs(hipStream_t s, const Desc *D, const ETab *tab, int nblk, const double *fq, size_t lds);
+void launch_elem_vol(hipStream_t s, const Desc *D, const ETab *tab, int nblk, size_t lds);
+void launch_charge(hipStream_t s, int n, const double *b, const double *vol, double *rho);
 template <class T> void launch_gather_cg(hipStream_t s, const Desc *D, const int2 *tab, int nblk);
 template <class T> void launch_update_cg(hipStream_t s, const Desc *D, const int2 *tab, int nblk);
 template <class T> void launch_gather_res(hipStream_t s, const Desc *D, const int2 *tab, int nblk);

@@ -347,7 +347,7 @@ template <class T> __device__ __forceinline__ T gather_slots(const Desc &d, int 
 // each node's contributions are then summed in a fixed order and written once to the
 // node's slot for this patch.
 //   MODE 0: y = K_e u          MODE 1: y_n = K_e(n,n)          MODE 2: y_n = int phi_n f
-enum { EM_APPLY = 0, EM_DIAG = 1, EM_RHS = 2 };
+enum { EM_APPLY = 0, EM_DIAG = 1, EM_RHS = 2, EM_VOL = 3 };
 
 extern __shared__ __attribute__((aligned(16))) unsigned char fg_smem[];
 
@@ -655,9 +655,15 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
                     const T det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                                   J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                                   J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-                    const T jxw = (T)0.125 * det * (T)G(fq)[8 * (size_t)c + q];
-                    for (int n = 0; n < 8; ++n)
-                        y[n] += Wt<T>(a, n & 1) * Wt<T>(b, (n >> 1) & 1) * Wt<T>(cc, (n >> 2) & 1) * jxw;
+                    if (MODE == EM_VOL) {
+                        // JxW(q) to every node of the cell, not shape-weighted
+                        const T jxw = (T)0.125 * det;
+                        for (int n = 0; n < 8; ++n) y[n] += jxw;
+                    } else {
+                        const T jxw = (T)0.125 * det * (T)G(fq)[8 * (size_t)c + q];
+                        for (int n = 0; n < 8; ++n)
+                            y[n] += Wt<T>(a, n & 1) * Wt<T>(b, (n >> 1) & 1) * Wt<T>(cc, (n >> 2) & 1) * jxw;
+                    }
                 }
             }
 #pragma unroll
@@ -1053,7 +1059,21 @@ void launch_elem_rhs(hipStream_t s, const Desc *D, const ETab *tab, int nblk, co
     if (nblk > 0) hipLaunchKernelGGL((k_patch<double, false, false, false, EM_RHS>), dim3(nblk), dim3(kBlock), lds, s, D, tab, 0, fq);
 }
 
+// per slot the volume of its cell (the dof volumes once gathered)
+void launch_elem_vol(hipStream_t s, const Desc *D, const ETab *tab, int nblk, size_t lds) {
+    if (nblk > 0) hipLaunchKernelGGL((k_patch<double, false, false, false, EM_VOL>), dim3(nblk), dim3(kBlock), lds, s, D, tab, 0, nullptr);
+}
+
 static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
+
+// rho = b / vol (PoissonSolver::assemble, src/PoissonSolver.cpp:196-207)
+__global__ void k_charge(int n, const double *__restrict__ b, const double *__restrict__ vol, double *__restrict__ rho) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rho[i] = b[i] / vol[i];
+}
+void launch_charge(hipStream_t s, int n, const double *b, const double *vol, double *rho) {
+    if (n > 0) hipLaunchKernelGGL(k_charge, dim3(grid_for(n)), dim3(kBlock), 0, s, n, b, vol, rho);
+}
 
 void launch_enforce_dirichlet(hipStream_t s, int n, const unsigned char *mask, const double *g, double *x) {
     if (n > 0) hipLaunchKernelGGL(k_enforce_dirichlet, dim3(grid_for(n)), dim3(kBlock), 0, s, n, mask, g, x);

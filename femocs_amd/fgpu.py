@@ -107,6 +107,8 @@ API = [
     ("fgpu_mesh_set_motion", C.c_int, [_P, C.c_int, _I, _I, _I, _D, C.c_int]),
     ("fgpu_mesh_displace", C.c_int, [_P, _D]),
     ("fgpu_sys_reassemble", C.c_int, [_P]),
+    ("fgpu_sys_save_charge_density", C.c_int, [_P]),
+    ("fgpu_sys_export_charge_density", C.c_int, [_P, _D]),
     ("fgpu_mesh_device_bytes", C.c_int, [_P, C.POINTER(C.c_size_t)]),
     ("fgpu_sys_device_bytes", C.c_int, [_P, C.POINTER(C.c_size_t)]),
     ("fgpu_batch_create", C.c_int, [_P, C.POINTER(SolveOpts), C.POINTER(_P)]),
@@ -307,6 +309,14 @@ class System:
 
     def heat_terms(self, current: "System", gamma: float, lorentz: float = 2.44e-8):
         self._c(self.ctx.lib.fgpu_sys_heat_terms(self.h, current.h, float(gamma), float(lorentz)), "heat_terms")
+
+    def save_charge_density(self):
+        self._c(self.ctx.lib.fgpu_sys_save_charge_density(self.h), "save_charge_density")
+
+    def charge_density(self):
+        rho = np.zeros(self.mesh.n_dofs)
+        self._c(self.ctx.lib.fgpu_sys_export_charge_density(self.h, rho), "export_charge_density")
+        return rho
 
     def reassemble(self):
         self._c(self.ctx.lib.fgpu_sys_reassemble(self.h), "reassemble")

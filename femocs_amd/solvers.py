@@ -67,6 +67,12 @@ class DealSolver:
         self.inner_tol = 1e-4
         self.use_graph = True
         self.last_stats = None
+        #: FileWriter::write_time (src/FileWriter.cpp:23-26) is the caller's clock here: set
+        #: True before an assembly whose write-step quantities are wanted
+        self.write_step = False
+
+    def write_time(self) -> bool:
+        return self.write_step
 
     def import_mesh(self, vertices, cells) -> bool:
         """DealSolver::import_mesh (src/DealSolver.cpp:192-209): False on a bad mesh."""
@@ -147,7 +153,14 @@ class PoissonSolver(DealSolver):
             s.add_dirichlet(BID_VACUUM_TOP, self.applied_potential)
         if self.conf.mode != "laplace":
             raise NotImplementedError("space charge (PIC) assembly is outside this path")
+        # charge density of a write step, before the Dirichlet conditions; zeros otherwise
+        if self.write_time():
+            s.save_charge_density()
         s.apply_dirichlet()
+
+    def export_charge_dens(self) -> np.ndarray:
+        """PoissonSolver::export_charge_dens (src/PoissonSolver.cpp:141-149), per vertex."""
+        return self.system.charge_density()
 
     def solve(self) -> int:
         """PoissonSolver::solve (include/PoissonSolver.h:54)."""

@@ -136,6 +136,15 @@ int fgpu_sys_add_neumann(fgpu_sys *sys, int bid, const double *face_values, doub
 int fgpu_sys_add_cell_rhs(fgpu_sys *sys, const double *f_q);
 /* DealSolver::append_dirichlet(bid, value) (src/DealSolver.cpp:433-435) */
 int fgpu_sys_add_dirichlet(fgpu_sys *sys, int bid, double value);
+/* PoissonSolver::assemble on a write step (src/PoissonSolver.cpp:196-207): charge density =
+ * the right-hand side assembled so far divided per dof by its volume (DealSolver::calc_dof_volumes,
+ * src/DealSolver.cpp:344-366: the volumes of the cells holding the dof).  Call before
+ * fgpu_sys_apply_dirichlet (FGPU_EINVAL after it); fgpu_sys_reassemble recomputes it until the
+ * next fgpu_sys_reset, which clears it */
+int fgpu_sys_save_charge_density(fgpu_sys *sys);
+/* PoissonSolver::export_charge_dens (src/PoissonSolver.cpp:141-149), dof order: the saved
+ * charge density, zeros when none was saved since the last reset (charge_density.reinit) */
+int fgpu_sys_export_charge_density(fgpu_sys *sys, double *rho);
 /* DealSolver::apply_dirichlet (src/DealSolver.cpp:438-440): lifting with column elimination;
  * sets solution = value on the constrained dofs */
 int fgpu_sys_apply_dirichlet(fgpu_sys *sys);
@@ -195,7 +204,7 @@ int fgpu_interp_create(fgpu_ctx *ctx, int n_nodes, const double *xyz, const int 
 void fgpu_interp_destroy(fgpu_interp *interp);
 /* Interpolator::extract_solution(PoissonSolver&, smoothen) (src/Interpolator.cpp:142-190): from
  * a solved system whose mesh was imported with these nodes as vertices: Solution{nodal field
- * (store_elfield), charge density 0 (field_mode laplace), potential}; nodes outside the
+ * (store_elfield), charge density (fgpu_sys_export_charge_density), potential}; nodes outside the
  * region get empty_value; smoothen: average_nodal_fields over the pseudo-Voronoi cells */
 int fgpu_interp_extract(fgpu_interp *interp, fgpu_sys *sys, int smoothen);
 /* Interpolator::extract_solution(CurrentHeatSolver&) (src/Interpolator.cpp:207-216): with

@@ -62,6 +62,13 @@ void orc_assemble(orc_csr *A, const orc_mesh *m, const double *kappa_q, const do
 /* b_i += sum_q JxW phi_i f_q (per cell and quadrature point) */
 void orc_assemble_cell_rhs(double *b, const orc_mesh *m, const double *f_q);
 
+/* DealSolver::calc_dof_volumes (src/DealSolver.cpp:344-366): vol_i = sum over the cells of
+ * dof i of the cell volume (sum_q JxW, not shape-weighted).  orc_charge_density: the
+ * write-step charge density of PoissonSolver::assemble (src/PoissonSolver.cpp:196-207),
+ * rho_i = b_i / vol_i with b the right-hand side before the Dirichlet conditions. */
+void orc_dof_volumes(const orc_mesh *m, double *vol);
+void orc_charge_density(const orc_mesh *m, const double *b, double *rho);
+
 /* DealSolver::assemble_rhs (src/DealSolver.cpp:390-430): Neumann load on faces with id bid.
  * face_values: one value per matching face in cell x face order, or NULL with a constant. */
 void orc_assemble_face_rhs(double *b, const orc_mesh *m, int bid, const double *face_values,

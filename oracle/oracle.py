@@ -70,6 +70,8 @@ def _declare(L):
     L.orc_csr_spmv.argtypes = [C.POINTER(_Csr), _D, _D]
     L.orc_assemble.argtypes = [C.POINTER(_Csr), C.POINTER(_Mesh), C.c_void_p, C.c_void_p]
     L.orc_assemble_cell_rhs.argtypes = [_D, C.POINTER(_Mesh), _D]
+    L.orc_dof_volumes.argtypes = [C.POINTER(_Mesh), _D]
+    L.orc_charge_density.argtypes = [C.POINTER(_Mesh), _D, _D]
     L.orc_assemble_face_rhs.argtypes = [_D, C.POINTER(_Mesh), C.c_int, C.c_void_p, C.c_double]
     L.orc_dirichlet_mask.argtypes = [C.POINTER(_Mesh), C.c_int, _U8]
     L.orc_apply_dirichlet.argtypes = [C.POINTER(_Csr), _D, _D, _U8, C.c_void_p]
@@ -173,6 +175,16 @@ class Mesh:
         b = np.zeros(self.n_dofs)
         lib().orc_assemble_cell_rhs(b, C.byref(self._m), np.ascontiguousarray(f_q, np.float64))
         return b
+
+    def dof_volumes(self):
+        v = np.zeros(self.n_dofs)
+        lib().orc_dof_volumes(C.byref(self._m), v)
+        return v
+
+    def charge_density(self, b):
+        rho = np.zeros(self.n_dofs)
+        lib().orc_charge_density(C.byref(self._m), np.ascontiguousarray(b, np.float64), rho)
+        return rho
 
     def face_rhs(self, bid, values=None, constant=0.0, b=None):
         b = np.zeros(self.n_dofs) if b is None else b

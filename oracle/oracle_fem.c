@@ -364,6 +364,28 @@ void orc_assemble_cell_rhs(double *b, const orc_mesh *m, const double *f_q) {
     }
 }
 
+/* DealSolver::calc_dof_volumes (src/DealSolver.cpp:344-366): every quadrature weight JxW(q)
+ * of a cell is added to each of the cell's eight dofs, unweighted by the shape functions */
+void orc_dof_volumes(const orc_mesh *m, double *vol) {
+    for (int i = 0; i < m->n_dofs; ++i) vol[i] = 0.0;
+    for (int c = 0; c < m->n_cells; ++c) {
+        double xe[8][3];
+        cell_coords(m, c, xe);
+        for (int q = 0; q < 8; ++q) {
+            double phi[8], g[8][3];
+            const double jxw = qpoint(xe, q, phi, g);
+            for (int i = 0; i < 8; ++i) vol[m->cells[8 * c + i]] += jxw;
+        }
+    }
+}
+
+/* PoissonSolver::assemble, write-step branch (src/PoissonSolver.cpp:196-207): the right-hand
+ * side before the Dirichlet conditions, divided by the dof volumes */
+void orc_charge_density(const orc_mesh *m, const double *b, double *rho) {
+    orc_dof_volumes(m, rho);
+    for (int i = 0; i < m->n_dofs; ++i) rho[i] = b[i] / rho[i];
+}
+
 void orc_assemble_face_rhs(double *b, const orc_mesh *m, int bid, const double *fv, double cst) {
     int k = 0;
     for (int i = 0; i < m->n_bfaces; ++i) {

@@ -67,6 +67,35 @@ def test_nodal_solution_matches_oracle(ctx, tip, solved, smoothen):
         assert np.array_equal(ip.nodal(), ref)
 
 
+def test_nodal_solution_carries_charge_density(ctx, tip):
+    """A system assembled as a write step hands its charge density to the nodal solution
+    (Interpolator::extract_solution, src/Interpolator.cpp:172-179)."""
+    gm = fgpu.Mesh(ctx, tip.nodes, tip.vacuum_cells(), *MARKS)
+    s = fgpu.System(gm)
+    s.reset()
+    s.add_dirichlet(2, 0.0)
+    s.add_neumann(8, None, -1.0)
+    s.save_charge_density()
+    s.apply_dirichlet()
+    s.fill_solution(0.0)
+    assert s.solve(fgpu.opts(tol=1e-12, relative=True)).iterations > 0
+    _, vod = gm.dofs()
+    phi = np.zeros(len(tip.nodes))
+    rho = np.zeros(len(tip.nodes))
+    used = np.zeros(len(tip.nodes), np.uint8)
+    phi[vod] = s.solution()
+    rho[vod] = s.charge_density()
+    used[vod] = 1
+    assert np.count_nonzero(rho) > 0
+    om = O.Mesh(tip.nodes, tip.vacuum_cells(), *MARKS)
+    ref_rho = om.charge_density(om.face_rhs(8, constant=-1.0))
+    assert np.abs(rho[vod] - ref_rho).max() <= 1e-12 * np.abs(ref_rho).max()
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM, empty_value=-3.5)
+    ip.extract_solution(s, smoothen=False)
+    ref = O.extract_nodal(tip.nodes, tip.hexes, tip.hex_marker, 1, used, phi, rho=rho, empty=-3.5)
+    assert np.array_equal(ip.nodal(), ref)
+
+
 @pytest.mark.parametrize("rank", [1, 2, 3])
 def test_located_cells_and_values_bit_exact(ctx, tip, solved, rank):
     _, s, phi, used = solved

@@ -56,6 +56,76 @@ def test_neumann_and_lifting_match_oracle(ctx, small_mesh):
     assert np.linalg.norm(bg - b) <= 1e-13 * np.linalg.norm(b)
 
 
+def test_charge_density_matches_oracle(ctx, small_mesh):
+    """Write-step charge density: the right-hand side before the Dirichlet conditions (Neumann
+    load and a cell source) over the dof volumes; zeros until saved and after a reset; refused
+    once the conditions are applied."""
+    om, gm = _pair(ctx, small_mesh)
+    rng = np.random.default_rng(4)
+    f_q = rng.uniform(-1.0, 1.0, 8 * om.n_cells)
+    b = om.face_rhs(8, constant=-1.0) + om.cell_rhs(f_q)
+    ref = om.charge_density(b)
+    s = fgpu.System(gm)
+    s.reset()
+    s.add_dirichlet(2, 0.0)
+    s.add_neumann(8, None, -1.0)
+    s.add_cell_rhs(f_q)
+    assert np.array_equal(s.charge_density(), np.zeros(om.n_dofs))
+    s.save_charge_density()
+    rho = s.charge_density()
+    assert np.abs(rho - ref).max() <= 1e-12 * np.abs(ref).max()
+    s.apply_dirichlet()
+    assert np.array_equal(s.charge_density(), rho)
+    with pytest.raises(fgpu.FgpuError):
+        s.save_charge_density()
+    s.reset()
+    assert np.array_equal(s.charge_density(), np.zeros(om.n_dofs))
+
+
+def test_solver_mirror_write_step(ctx, small_mesh):
+    """PoissonSolver.assemble keeps the charge density on a write step only, as
+    PoissonSolver::assemble does (src/PoissonSolver.cpp:196-207)."""
+    from femocs_amd import solvers
+    p = solvers.PoissonSolver(ctx=ctx)
+    assert p.import_mesh(small_mesh.nodes, small_mesh.vacuum_cells())
+    p.setup(-1.0, 0.0)
+    p.assemble()
+    assert not p.export_charge_dens().any()
+    p.write_step = True
+    p.assemble()
+    om = O.Mesh(small_mesh.nodes, small_mesh.vacuum_cells(), *MARKS)
+    ref = om.charge_density(om.face_rhs(8, constant=-1.0))
+    rho = p.export_charge_dens()
+    assert np.abs(rho - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert p.solve() > 0
+
+
+def test_charge_density_follows_reassembly(ctx, small_mesh):
+    """After a vertex move, fgpu_sys_reassemble recomputes a saved charge density for the moved
+    mesh: equal to the oracle's on a fresh import of the moved vertices."""
+    from femocs_amd import serving
+    c = serving.Client(0, small_mesh, sigma=0.1, seed=7)
+    gm = fgpu.Mesh(ctx, small_mesh.nodes, small_mesh.vacuum_cells(), *MARKS)
+    s = fgpu.System(gm)
+    s.reset()
+    s.add_dirichlet(2, 0.0)
+    s.add_neumann(8, None, -1.0)
+    f_q = np.random.default_rng(8).uniform(-1.0, 1.0, 8 * gm.n_cells)
+    s.add_cell_rhs(f_q)                     # charge on the dofs around the moved vertices too
+    s.save_charge_density()
+    s.apply_dirichlet()
+    before = s.charge_density()
+    ids, xyz = c.next_move()
+    gm.move_vertices(ids, xyz)
+    s.reassemble()
+    moved = c.positions_after((ids, xyz))
+    om = O.Mesh(moved, small_mesh.vacuum_cells(), *MARKS)
+    ref = om.charge_density(om.face_rhs(8, constant=-1.0) + om.cell_rhs(f_q))
+    rho = s.charge_density()
+    assert not np.array_equal(rho, before)
+    assert np.abs(rho - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("precision", [fgpu.FGPU_FP64, fgpu.FGPU_MIXED])
 def test_poisson_solution_matches_oracle(ctx, small_mesh, precision):
     om, xo, it, res, _ = O.poisson_laplace(small_mesh.nodes, small_mesh.vacuum_cells(), -1.0,

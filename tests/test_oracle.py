@@ -128,6 +128,73 @@ def test_neumann_load_sums_to_flux():
     assert np.all(b[~top] == 0)
 
 
+def grid_hexes(n=(3, 4, 2), h=0.5, shear=0.0):
+    """Structured grid of cubes of side h in UCD vertex order; shear maps (x, y, z) to
+    (x + shear z, y, z), so the cells are congruent parallelepipeds."""
+    nx, ny, nz = n
+    xs = [np.arange(n[d] + 1) * h for d in range(3)]
+    P = np.stack(np.meshgrid(*xs, indexing="ij"), -1).reshape(-1, 3)
+    P[:, 0] += shear * P[:, 2]
+    idx = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k
+    lex_to_ucd = [0, 1, 4, 5, 3, 2, 7, 6]
+    cells = []
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                ucd = [0] * 8
+                for l in range(8):
+                    ucd[lex_to_ucd[l]] = idx(i + (l & 1), j + ((l >> 1) & 1), k + (l >> 2))
+                cells.append(ucd)
+    return P, np.array(cells)
+
+
+def test_dof_volumes_count_the_cells_of_each_dof():
+    """Congruent cells of volume h^3: a dof's volume is h^3 times the number of cells it
+    belongs to (1 at a corner, 2 on an edge, 4 on a face, 8 inside)."""
+    h = 0.5
+    P, cells = grid_hexes(h=h, shear=0.3)
+    m = O.Mesh(P, cells, 8, 2, 4, 2)
+    count = np.bincount(m.cells.reshape(-1), minlength=m.n_dofs)
+    assert set(count.tolist()) == {1, 2, 4, 8}
+    assert np.allclose(m.dof_volumes(), h ** 3 * count, rtol=1e-14, atol=0)
+
+
+def test_dof_volumes_sum_to_eight_mesh_volumes():
+    nodes, _, hexes = box_hexes()                        # jittered 1 x 1 x 2 box
+    m = O.Mesh(nodes, hexes, 8, 2, 4, 2)
+    v = m.dof_volumes()
+    assert np.all(v > 0) and abs(v.sum() - 8 * 2.0) < 1e-12
+
+
+def test_charge_density_of_a_uniform_source():
+    """Each cell gives every one of its dofs the load c*V/8 of a constant source c on a
+    parallelepiped (the integral of a Q1 shape function) and the volume V: rho = c/8 at every
+    dof.  On a curved mesh, rho_i vol_i still sums to c times the mesh volume."""
+    c = 3.0
+    P, cells = grid_hexes(shear=0.3)
+    m = O.Mesh(P, cells, 8, 2, 4, 2)
+    rho = m.charge_density(m.cell_rhs(np.full(8 * m.n_cells, c)))
+    assert np.allclose(rho, c / 8, rtol=1e-13, atol=0)
+    nodes, _, hexes = box_hexes()
+    m = O.Mesh(nodes, hexes, 8, 2, 4, 2)
+    rho = m.charge_density(m.cell_rhs(np.full(8 * m.n_cells, c)))
+    assert abs((rho * m.dof_volumes()).sum() - c * 2.0) < 1e-12
+    assert not np.allclose(rho, c / 8)
+
+
+def test_charge_density_of_a_neumann_load():
+    """The top-face Neumann load of the field solve shows up as charge on the top dofs only:
+    on the grid a top dof of k top faces of area h^2 and flux q gets k q h^2/4 over the k h^3
+    of the k cells under those faces."""
+    h, q = 0.5, 2.5
+    P, cells = grid_hexes(h=h)
+    m = O.Mesh(P, cells, 8, 2, 4, 2)
+    rho = m.charge_density(m.face_rhs(8, constant=q))
+    top = m.x[:, 2] > m.x[:, 2].max() - 1e-12
+    assert np.all(rho[~top] == 0)
+    assert np.allclose(rho[top], q / (4 * h), rtol=1e-13, atol=0)
+
+
 # ------------------------------------------------------------------------------------------
 # solutions
 # ------------------------------------------------------------------------------------------
