@@ -17,6 +17,7 @@ import numpy as np
 from . import fgpu
 
 TYPES_BULK = 1
+TYPES_SURFACE = 2
 TYPES_VACUUM = 3
 
 
@@ -119,14 +120,97 @@ class Interpolator:
         return out.reshape(-1, 5)
 
 
-class SolutionReader:
-    """SolutionReader for dim 3 without atom sorting (src/SolutionReader.cpp:43-190): the first
-    call maps the atoms to cells (markers), later calls interpolate in the mapped cells."""
+# labels of the interpolated data (Labels, include/Globals.h:134-165)
+ELFIELD, POTENTIAL, CHARGE_DENSITY = "elfield", "potential", "charge_density"
+RHO, TEMPERATURE = "rho", "temperature"
 
-    def __init__(self, interpolator: Interpolator, rank: int = 1, dim: int = 3, interp_centroids: bool = False):
+
+class SolutionReader:
+    """SolutionReader without atom sorting (src/SolutionReader.cpp:43-190, 269-436): the first
+    call maps the atoms to cells (markers), later calls interpolate in the mapped cells.  The
+    last result is kept as ``interpolation`` (n, 5): vector x y z, scalar1, scalar2, with the
+    atom ``ids`` that export_results writes them to."""
+
+    def __init__(self, interpolator: Interpolator, rank: int = 1, dim: int = 3, interp_centroids: bool = False,
+                 vec_label: str = "vec", scalar1_label: str = "scalar1", scalar2_label: str = "scalar2"):
         self.interpolator = interpolator
         self.markers = None
+        self.vec_label, self.vec_norm_label = vec_label, vec_label + "_norm"
+        self.scalar1_label, self.scalar2_label = scalar1_label, scalar2_label
+        self.interpolation = np.zeros((0, 5))
+        self.ids = np.zeros(0, np.int64)
+        self.points = np.zeros((0, 3))
         self.set_preferences(False, dim, rank, interp_centroids)
+
+    def size(self) -> int:
+        return len(self.interpolation)
+
+    def interpolate(self, xyz, ids=None) -> np.ndarray:
+        """SolutionReader::interpolate(n_points, x, y, z) (src/SolutionReader.cpp:428-436):
+        atoms with ids 0..n-1 unless given."""
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        self.points = xyz
+        self.ids = np.arange(len(xyz)) if ids is None else np.asarray(ids, np.int64).reshape(-1)
+        if len(self.ids) != len(xyz):
+            raise ValueError("one id per point")
+        self.interpolation = self.calc_interpolation(xyz)
+        return self.interpolation
+
+    def contains(self, data_label: str) -> int:
+        """SolutionReader::contains (src/SolutionReader.cpp:269-285): 1..4 for the vector, its
+        norm, scalar1, scalar2 under their own (lower-case) label, negated when the label was
+        given with upper-case letters, 0 if absent."""
+        labels = (self.vec_label, self.vec_norm_label, self.scalar1_label, self.scalar2_label)
+        for k, lab in enumerate(labels):
+            if data_label == lab:
+                return k + 1
+        low = data_label.lower()
+        for k, lab in enumerate(labels):
+            if low == lab:
+                return -(k + 1)
+        return 0
+
+    def export_results(self, n_points: int, data_type: str, data: np.ndarray) -> int:
+        """SolutionReader::export_results (src/SolutionReader.cpp:303-421) into ``data`` in place:
+        the label's own spelling adds to what ``data`` holds, an upper-case spelling clears
+        ``data`` first (3*n_points values for the vector, x1 y1 z1 x2 ...); atoms whose id is
+        outside 0..n_points-1 are skipped.  Returns 0; 1 when nothing was interpolated."""
+        if self.size() == 0:
+            return 1
+        dtype = self.contains(data_type)
+        if dtype == 0:
+            raise ValueError(f"{type(self).__name__} does not contain {data_type}")
+        append = dtype > 0
+        width = 3 if abs(dtype) == 1 else 1
+        out = np.asarray(data).reshape(-1)
+        if out.size < width * n_points:
+            raise ValueError("data holds fewer values than the points need")
+        if not append:
+            out[: width * n_points] = 0.0
+        ok = (self.ids >= 0) & (self.ids < n_points)
+        ids, sol = self.ids[ok], self.interpolation[ok]
+        if abs(dtype) == 1:
+            vals, idx = sol[:, :3].reshape(-1), (3 * ids[:, None] + np.arange(3)).reshape(-1)
+        elif abs(dtype) == 2:
+            vals, idx = np.linalg.norm(sol[:, :3], axis=1), ids
+        else:
+            vals, idx = sol[:, abs(dtype)], ids
+        if append:
+            np.add.at(out, idx, vals)
+        else:
+            out[idx] = vals                  # a repeated id keeps its last atom's value
+        return 0
+
+    def interpolate_results(self, n_points: int, data_type: str, x, y, z, data: np.ndarray) -> int:
+        """SolutionReader::interpolate_results (src/SolutionReader.cpp:396-414): a fresh reader
+        with these preferences interpolates at the points and exports one quantity."""
+        if self.size() == 0:
+            return 1
+        sr = SolutionReader(self.interpolator, self.rank, self.dim, self.interp_centroids, self.vec_label,
+                            self.scalar1_label, self.scalar2_label)
+        sr.interpolate(np.stack([np.asarray(x, np.float64)[:n_points], np.asarray(y, np.float64)[:n_points],
+                                 np.asarray(z, np.float64)[:n_points]], axis=1))
+        return sr.export_results(n_points, data_type, data)
 
     def set_preferences(self, sort_atoms: bool, dim: int, rank: int, interp_centroids: bool = False):
         """SolutionReader::set_preferences (include/SolutionReader.h:60-67).  Atom sorting
@@ -153,3 +237,126 @@ class SolutionReader:
         if self.dim == 2:
             return self.interpolator.surface_cells(xyz, self.markers, self.rank, self.interp_centroids)
         return self.interpolator.in_cells(xyz, self.rank, self.markers)
+
+
+class FieldReader(SolutionReader):
+    """FieldReader (src/SolutionReader.cpp:465-597): (E, charge density, potential) at the
+    atoms, the field norms and their maximum, and the analytic check against a hemisphere
+    (or hemi-ellipsoid) on a plane in the uniform field E0."""
+
+    def __init__(self, interpolator: Interpolator, rank: int = 1, dim: int = 3, interp_centroids: bool = False):
+        super().__init__(interpolator, rank, dim, interp_centroids, ELFIELD, CHARGE_DENSITY, POTENTIAL)
+        self.E0 = self.radius1 = self.radius2 = self.beta = self.E_max = 0.0
+        self.limit_min = self.limit_max = 0.0
+        self.field_norm = np.zeros(0)
+
+    def interpolate(self, xyz, ids=None) -> np.ndarray:
+        """FieldReader::interpolate(Surface) (:469-484) and calc_interpolation (:486-496)."""
+        sol = super().interpolate(xyz, ids)
+        self.field_norm = np.linalg.norm(sol[:, :3], axis=1)
+        self.E_max = float(self.field_norm.max()) if len(sol) else -1e100
+        return sol
+
+    def get_elfield(self, i: int) -> np.ndarray:
+        return self.interpolation[i, :3]
+
+    def get_elfield_norm(self, i: int) -> float:
+        return float(self.field_norm[i])
+
+    def get_charge_dens(self, i: int) -> float:
+        return float(self.interpolation[i, 3])
+
+    def get_potential(self, i: int) -> float:
+        return float(self.interpolation[i, 4])
+
+    def get_analyt_potential(self, i: int, origin=(0.0, 0.0, 0.0)) -> float:
+        """phi = -E0 z (1 - (R/r)^3) around the hemisphere centre ``origin`` (:498-505)."""
+        if not 0 <= i < self.size():
+            raise IndexError(f"Invalid index: {i}")
+        p = self.points[i] - np.asarray(origin, np.float64)
+        r = float(np.linalg.norm(p))
+        return -self.E0 * p[2] * (1.0 - (self.radius1 / r) ** 3)
+
+    def get_analyt_field(self, i: int, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+        """E = -grad phi of the same solution (:507-521)."""
+        if not 0 <= i < self.size():
+            raise IndexError(f"Invalid index: {i}")
+        p = self.points[i] - np.asarray(origin, np.float64)
+        r5 = float(p @ p) ** 2.5
+        r3 = self.radius1 ** 3
+        f = p[0] * p[0] + p[1] * p[1] - 2.0 * p[2] * p[2]
+        return np.array([3 * self.E0 * r3 * p[0] * p[2] / r5, 3 * self.E0 * r3 * p[1] * p[2] / r5,
+                         self.E0 * (1.0 - r3 * f / r5)])
+
+    def get_analyt_enhancement(self) -> float:
+        """Apex enhancement of a hemisphere (3) or of a hemi-ellipsoid with semi-axes
+        radius1 < radius2 (:523-534)."""
+        if self.radius1 <= 0:
+            raise ValueError(f"Invalid nanotip minor semi-axis: {self.radius1}")
+        if self.radius2 <= self.radius1:
+            return 3.0
+        nu = self.radius2 / self.radius1
+        zeta = np.sqrt(nu * nu - 1)
+        return float(zeta ** 3 / (nu * np.log(zeta + nu) - zeta))
+
+    def max_field(self, solutions=None) -> float:
+        """:555-565: the largest |E| of the given (n, 5) rows, else of the last interpolation."""
+        if solutions is not None:
+            sol = np.asarray(solutions, np.float64).reshape(-1, 5)
+            return float(np.sqrt((sol[:, :3] ** 2).sum(1).max())) if len(sol) else float("nan")
+        return self.E_max
+
+    def check_limits(self, solutions=None, verbose: bool = False) -> bool:
+        """:536-553: beta = |E_max / E0| / analytic enhancement; True when outside the limits
+        (never when the limits are equal)."""
+        if self.limit_min == self.limit_max:
+            return False
+        gamma1 = abs(self.max_field(solutions) / self.E0)
+        gamma2 = self.get_analyt_enhancement()
+        self.beta = abs(gamma1 / gamma2)
+        if verbose:
+            print(f"field enhancements:  (M)easured:{gamma1:.3f}  (A)nalyt:{gamma2:.3f}  M/A:{gamma1 / gamma2:.3f}")
+        return self.beta < self.limit_min or self.beta > self.limit_max
+
+    def get_beta(self) -> float:
+        return self.beta
+
+    def set_check_params(self, conf, radius: float, tip_height: float, box_height: float,
+                         field_min: float = 0.1, field_max: float = 5.0):
+        """:568-589: E0 from the Neumann field or V0 / box height; the ellipsoid semi-axes from
+        the tip height and radius.  conf: anode_BC, E0, V0 (FieldConfig); the tolerances
+        default to the reference's (src/Config.cpp:60-61)."""
+        if conf.anode_BC not in ("neumann", "dirichlet"):
+            raise ValueError("Invalid anode boundary condition: " + conf.anode_BC)
+        self.E0 = conf.E0 if conf.anode_BC == "neumann" else conf.V0 / box_height
+        self.limit_min, self.limit_max = field_min, field_max
+        if tip_height > radius:
+            self.radius2, self.radius1 = tip_height, float(np.sqrt(tip_height * radius))
+        else:
+            self.radius2 = self.radius1 = radius
+
+
+class HeatReader(SolutionReader):
+    """HeatReader (src/SolutionReader.cpp:591-683): (current density, potential, temperature)
+    at the atoms; interpolate_dofs hands the temperatures at the heat solver's vertices back
+    to the solver."""
+
+    def __init__(self, interpolator: Interpolator, rank: int = 1, dim: int = 3):
+        super().__init__(interpolator, rank, dim, False, RHO, POTENTIAL, TEMPERATURE)
+
+    def interpolate_atoms(self, xyz, markers, ids=None) -> np.ndarray:
+        """HeatReader::interpolate(AtomReader) (:655-668): the atoms marked SURFACE or BULK."""
+        xyz = np.asarray(xyz, np.float64).reshape(-1, 3)
+        markers = np.asarray(markers).reshape(-1)
+        ids = np.arange(len(xyz)) if ids is None else np.asarray(ids, np.int64).reshape(-1)
+        keep = (markers == TYPES_SURFACE) | (markers == TYPES_BULK)
+        return self.interpolate(xyz[keep], ids[keep])
+
+    def interpolate_dofs(self, heat_system: fgpu.System) -> np.ndarray:
+        """HeatReader::interpolate_dofs (:670-683) without sorting: temperatures interpolated at
+        the heat system's vertices become its solution."""
+        xyz, _ = heat_system.mesh.dofs()
+        sol = self.interpolate(xyz)
+        temperatures = np.ascontiguousarray(sol[:, 4])
+        heat_system.set_solution(temperatures)
+        return temperatures
