@@ -115,6 +115,7 @@ void fgpu::Sys::clear_terms() {
     rhs_terms.clear();
     lifted = false;
     rho_saved = false;
+    space_charge = false;
 }
 
 fgpu::Sys::~Sys() {
@@ -1137,6 +1138,7 @@ extern "C" int fgpu_sys_save_charge_density(fgpu_sys *s) {
 extern "C" int fgpu_sys_reassemble(fgpu_sys *s) {
     if (!s) return FGPU_EINVAL;
     return guarded(s->ctx, [&] {
+        if (s->space_charge) throw std::invalid_argument("particles are deposited again after a move: reset and assemble");
         fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
         hipStream_t st = s->ctx->stream;
         HIPCHECK(hipSetDevice(s->ctx->device));

@@ -191,6 +191,7 @@ struct Sys {
     // charge density of a write step (fgpu_sys_save_charge_density), internal order
     double *rho = nullptr;
     bool rho_saved = false;
+    bool space_charge = false;    // particles deposited since the last reset (not replayable)
     // two-level preconditioner of the mixed solver (fgpu_sys_set_coarse)
     bool coarse = false;
     int batches = 0;              // running batches holding the system (their tables point here)

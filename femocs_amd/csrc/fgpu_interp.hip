@@ -5,7 +5,9 @@
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <climits>
 #include <cstring>
 #include <stdexcept>
@@ -52,6 +54,14 @@ struct fgpu_interp {
     int *quad_nodes = nullptr, *tri6 = nullptr, *quad_child = nullptr, *quad_prim = nullptr;
     float *quad_cbox = nullptr;
     std::vector<int> hkind, htets;
+    // the region's hexes in hex order: solver cell c is hex region_hex[c]
+    // (Interpolator::deal2femocs); host copy and device table
+    std::vector<int> region_hex;
+    int *d_region_hex = nullptr;
+    // space-charge deposit: API cell -> internal cell of mesh `sc_mesh`, checked against
+    // the region's hexes once per mesh
+    int *sc_icell = nullptr;
+    unsigned long long sc_mesh = 0;
 };
 
 namespace {
@@ -592,7 +602,7 @@ void free_triangles(fgpu_interp *ip) {
 void free_interp(fgpu_interp *ip) {
     for (double **p : {&ip->xyz, &ip->det0, &ip->det, &ip->cen, &ip->sol, &ip->pts, &ip->out}) release(*p);
     for (int **p : {&ip->tets, &ip->c10, &ip->hexes, &ip->marker, &ip->child, &ip->prim, &ip->n2c_ptr, &ip->n2c,
-                    &ip->vor_ptr, &ip->vor, &ip->node_idof, &ip->tet, &ip->cells})
+                    &ip->vor_ptr, &ip->vor, &ip->node_idof, &ip->tet, &ip->cells, &ip->d_region_hex, &ip->sc_icell})
         release(*p);
     for (float **p : {&ip->box, &ip->cbox, &ip->leafbox}) release(*p);
     release(ip->cand);
@@ -675,6 +685,9 @@ extern "C" int fgpu_interp_create(fgpu_ctx *c, int n_nodes, const double *xyz, c
         std::vector<int> vptr(tn + 1, 0), vor;
         if (n_hexes > 0) {
             auto in_region = [&](int h) { return search_region == 3 ? hex_marker[h] > 0 : hex_marker[h] < 0; };
+            for (int h = 0; h < n_hexes; ++h)
+                if (in_region(h)) ip->region_hex.push_back(h);
+            ip->d_region_hex = upload(ip->region_hex.data(), ip->region_hex.size(), s);
             for (int h = 0; h < n_hexes; ++h)
                 if (in_region(h))
                     for (int k = 0; k < 8; ++k) nptr[hexes[8 * (size_t)h + k] + 1]++;
@@ -822,6 +835,124 @@ extern "C" int fgpu_interp_extract_heat(fgpu_interp *ip, fgpu_sys *cur, fgpu_sys
         (void)hipFree(grad);
         IPCHECK(e);
         IPCHECK(e2);
+        return FGPU_OK;
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// space-charge deposit (PoissonSolver::assemble_space_charge_fast, src/PoissonSolver.cpp:300-319)
+// ------------------------------------------------------------------------------------------
+// per particle p: the shape functions of its cell's hex at x_p (LinearHexahedra::shape_functions,
+// Newton inversion) in the solver cell's lexicographic order (shape_funs_dealii,
+// src/InterpolatorCells.cpp:1355-1358), times factor, keyed by internal dof; val = 8p + k
+// keeps the reference's accumulation order through a stable sort
+__global__ __launch_bounds__(256) void k_sc_shape(int n, const double *__restrict__ pts, const int *__restrict__ cells,
+                                                  const int *__restrict__ region_hex, const int *__restrict__ hexes,
+                                                  const double *__restrict__ xyz, const int *__restrict__ icell,
+                                                  const int *__restrict__ mcells, double factor,
+                                                  double *__restrict__ contrib, int *__restrict__ key,
+                                                  int *__restrict__ val) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    constexpr int kUcdOfLex[8] = {0, 1, 4, 5, 3, 2, 7, 6};
+    const int c = cells[p];
+    const int *h = hexes + 8 * (size_t)region_hex[c];
+    D3 x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = ldx(xyz, h[k]);
+    double sf[8];
+    linhex_sf(x, pts + 3 * (size_t)p, sf);
+    const int *cl = mcells + 8 * (size_t)icell[c];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const size_t j = 8 * (size_t)p + k;
+        contrib[j] = sf[kUcdOfLex[k]] * factor;
+        key[j] = cl[k];
+        val[j] = (int)j;
+    }
+}
+
+// b[dof] += its contributions in particle order, one thread per run of equal sorted keys
+__global__ __launch_bounds__(256) void k_sc_fold(int m, const int *__restrict__ key, const int *__restrict__ val,
+                                                 const double *__restrict__ contrib, double *__restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m || (i > 0 && key[i] == key[i - 1])) return;
+    const int d = key[i];
+    double acc = b[d];
+    for (int j = i; j < m && key[j] == d; ++j) acc += contrib[val[j]];
+    b[d] = acc;
+}
+
+extern "C" int fgpu_sys_add_space_charge(fgpu_sys *s, fgpu_interp *ip, int n, const double *xyz, const int *cells,
+                                         double charge_factor) {
+    if (!s || !ip || n < 0 || (n > 0 && (!xyz || !cells))) return FGPU_EINVAL;
+    if (n == 0) return FGPU_OK;
+    if (n > INT_MAX / 8) return FGPU_EINVAL;
+    return guard(ip, [&] {
+        IPCHECK(hipSetDevice(ip->ctx->device));
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
+        if (s->ctx != ip->ctx) throw std::invalid_argument("system of another context");
+        if (s->lifted) throw std::invalid_argument("space charge is added before fgpu_sys_apply_dirichlet");
+        if (m->n_vertices != ip->n_nodes || (int)ip->region_hex.size() != m->n_cells)
+            throw std::invalid_argument("the system's mesh was not imported from the interpolator's region hexes");
+        for (int p = 0; p < n; ++p)
+            if (cells[p] < 0 || cells[p] >= m->n_cells) throw std::invalid_argument("particle cell out of range");
+        hipStream_t st = ip->ctx->stream;
+        if (ip->sc_mesh != m->uid) {
+            // solver cell c must hold the dofs of hex region_hex[c]'s nodes (as sets: the
+            // import may have flipped the cell)
+            std::vector<int> hx(8 * (size_t)ip->n_hexes);
+            IPCHECK(hipMemcpy(hx.data(), ip->hexes, hx.size() * sizeof(int), hipMemcpyDeviceToHost));
+            for (int c = 0; c < m->n_cells; ++c) {
+                int a[8], b[8];
+                for (int k = 0; k < 8; ++k) {
+                    a[k] = m->dof_of_vertex[hx[8 * (size_t)ip->region_hex[c] + k]];
+                    b[k] = m->cells_lex[8 * (size_t)c + k];
+                }
+                std::sort(a, a + 8);
+                std::sort(b, b + 8);
+                if (!std::equal(a, a + 8, b)) throw std::invalid_argument("solver cell and region hex differ");
+            }
+            release(ip->sc_icell);
+            ip->sc_icell = upload(m->icell_of_cell.data(), m->icell_of_cell.size(), st);
+            IPCHECK(hipStreamSynchronize(st));
+            ip->sc_mesh = m->uid;
+        }
+        const int nk = 8 * n;
+        int bits = 1;
+        while (bits < 31 && (1 << bits) < m->n_dofs) ++bits;
+        double *d_pts = nullptr, *contrib = nullptr;
+        int *d_cells = nullptr, *key = nullptr, *val = nullptr, *skey = nullptr, *sval = nullptr;
+        void *tmp = nullptr;
+        size_t tmp_bytes = 0;
+        auto cleanup = [&] {
+            (void)hipStreamSynchronize(st);
+            for (void *q : {(void *)d_pts, (void *)contrib, (void *)d_cells, (void *)key, (void *)val, (void *)skey,
+                            (void *)sval, tmp})
+                if (q) (void)hipFree(q);
+        };
+        try {
+            d_pts = upload(xyz, 3 * (size_t)n, st);
+            d_cells = upload(cells, (size_t)n, st);
+            for (int **q : {&key, &val, &skey, &sval}) IPCHECK(hipMalloc(q, nk * sizeof(int)));
+            IPCHECK(hipMalloc(&contrib, nk * sizeof(double)));
+            hipLaunchKernelGGL(k_sc_shape, dim3(grid(n)), dim3(256), 0, st, n, d_pts, d_cells, ip->d_region_hex,
+                               ip->hexes, ip->xyz, ip->sc_icell, m->d_cells, charge_factor, contrib, key, val);
+            IPCHECK(hipGetLastError());
+            // LSD radix sort: stable, so equal dofs keep the particle order
+            unsigned *ukey = reinterpret_cast<unsigned *>(key), *uskey = reinterpret_cast<unsigned *>(skey);
+            IPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, ukey, uskey, val, sval, nk, 0, bits, st));
+            IPCHECK(hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+            IPCHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, ukey, uskey, val, sval, nk, 0, bits, st));
+            hipLaunchKernelGGL(k_sc_fold, dim3(grid(nk)), dim3(256), 0, st, nk, skey, sval, contrib, s->b);
+            IPCHECK(hipGetLastError());
+            IPCHECK(hipStreamSynchronize(st));   // the host inputs are read until the copies end
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+        s->space_charge = true;
         return FGPU_OK;
     });
 }

@@ -108,6 +108,7 @@ API = [
     ("fgpu_mesh_displace", C.c_int, [_P, _D]),
     ("fgpu_sys_reassemble", C.c_int, [_P]),
     ("fgpu_sys_save_charge_density", C.c_int, [_P]),
+    ("fgpu_sys_add_space_charge", C.c_int, [_P, _P, C.c_int, _D, _I, C.c_double]),
     ("fgpu_sys_export_charge_density", C.c_int, [_P, _D]),
     ("fgpu_mesh_device_bytes", C.c_int, [_P, C.POINTER(C.c_size_t)]),
     ("fgpu_sys_device_bytes", C.c_int, [_P, C.POINTER(C.c_size_t)]),
@@ -309,6 +310,16 @@ class System:
 
     def heat_terms(self, current: "System", gamma: float, lorentz: float = 2.44e-8):
         self._c(self.ctx.lib.fgpu_sys_heat_terms(self.h, current.h, float(gamma), float(lorentz)), "heat_terms")
+
+    def add_space_charge(self, interpolator, xyz, cells, charge_factor: float):
+        """b += the particles' charge (PoissonSolver::assemble_space_charge_fast); cells are the
+        solver cells of the particles, interpolator the one over this mesh's region hexes."""
+        x = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        c = np.ascontiguousarray(cells, np.int32).reshape(-1)
+        if x.size != 3 * c.size:
+            raise ValueError("one cell per particle")
+        self._c(self.ctx.lib.fgpu_sys_add_space_charge(self.h, interpolator.h, c.size, x, c, float(charge_factor)),
+                "add_space_charge")
 
     def save_charge_density(self):
         self._c(self.ctx.lib.fgpu_sys_save_charge_density(self.h), "save_charge_density")

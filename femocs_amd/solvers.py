@@ -119,6 +119,16 @@ class DealSolver:
         return st.iterations
 
 
+@dataclass
+class ParticleSpecies:
+    """The part of ParticleSpecies the field solve reads (include/ParticleSpecies.h): super-
+    particle positions and solver cells, charge over eps0 and the super-particle weight."""
+    pos: np.ndarray
+    cell: np.ndarray
+    q_over_eps0: float
+    Wsp: float
+
+
 class PoissonSolver(DealSolver):
     """PoissonSolver<3> (src/PoissonSolver.cpp)."""
 
@@ -152,11 +162,23 @@ class PoissonSolver(DealSolver):
         else:
             s.add_dirichlet(BID_VACUUM_TOP, self.applied_potential)
         if self.conf.mode != "laplace":
-            raise NotImplementedError("space charge (PIC) assembly is outside this path")
+            self.assemble_space_charge()
         # charge density of a write step, before the Dirichlet conditions; zeros otherwise
         if self.write_time():
             s.save_charge_density()
         s.apply_dirichlet()
+
+    def assemble_space_charge(self):
+        """PoissonSolver::assemble_space_charge(_fast) (src/PoissonSolver.cpp:267-319): each
+        particle's charge q/eps0 * W_sp spread over its cell's dofs by the shape functions.
+        ``particles``: pos (n, 3), cell (n) solver cells, q_over_eps0, Wsp; ``interpolator``:
+        the Interpolator over this mesh's region hexes.  Nothing without particles."""
+        P = self.particles
+        if P is None or len(P.cell) == 0:
+            return
+        if self.interpolator is None:
+            raise ValueError("space charge needs the interpolator of the solver's hexes")
+        self.system.add_space_charge(self.interpolator, P.pos, P.cell, P.q_over_eps0 * P.Wsp)
 
     def export_charge_dens(self) -> np.ndarray:
         """PoissonSolver::export_charge_dens (src/PoissonSolver.cpp:141-149), per vertex."""

@@ -41,6 +41,7 @@ extern "C" {
 typedef struct fgpu_ctx fgpu_ctx;
 typedef struct fgpu_mesh fgpu_mesh;
 typedef struct fgpu_sys fgpu_sys;
+typedef struct fgpu_interp fgpu_interp;
 
 typedef struct {
     int max_it;        /* max CG iterations (inner + outer for FGPU_MIXED)                    */
@@ -142,6 +143,17 @@ int fgpu_sys_add_dirichlet(fgpu_sys *sys, int bid, double value);
  * fgpu_sys_apply_dirichlet (FGPU_EINVAL after it); fgpu_sys_reassemble recomputes it until the
  * next fgpu_sys_reset, which clears it */
 int fgpu_sys_save_charge_density(fgpu_sys *sys);
+/* PoissonSolver::assemble_space_charge_fast (src/PoissonSolver.cpp:300-319), the field mode with
+ * particles: b_i += factor * (shape function of dof i of the particle's cell at its position),
+ * added in particle order (bit-identical to the sequential loop).  cells: the solver cell of
+ * each particle (deal.II active cell index = cell index of the imported mesh); the shape
+ * functions are those of the interpolator's hex for that cell (Interpolator::deal2femocs,
+ * LinearHexahedra::shape_funs_dealii), so the system's mesh must have been imported from the
+ * interpolator's region hexes in hex order.  factor = q/eps0 * W_sp.  Before
+ * fgpu_sys_apply_dirichlet; fgpu_sys_reassemble refuses a system holding a deposit (reset and
+ * deposit again after a move).  FGPU_EINVAL for a cell out of range. */
+int fgpu_sys_add_space_charge(fgpu_sys *sys, fgpu_interp *interp, int n, const double *xyz, const int *cells,
+                              double charge_factor);
 /* PoissonSolver::export_charge_dens (src/PoissonSolver.cpp:141-149), dof order: the saved
  * charge density, zeros when none was saved since the last reset (charge_density.reinit) */
 int fgpu_sys_export_charge_density(fgpu_sys *sys, double *rho);
@@ -191,7 +203,6 @@ int fgpu_batch_time_kernel(fgpu_batch *batch, int which, int reps, double *avg_m
  * scalar1, scalar2). */
 #define FGPU_TYPE_BULK 1
 #define FGPU_TYPE_VACUUM 3
-typedef struct fgpu_interp fgpu_interp;
 /* Interpolator::initialize (src/Interpolator.cpp:28-77) with LinearTetrahedra /
  * QuadraticTetrahedra / LinearHexahedra::precompute (src/InterpolatorCells.cpp:523-629,
  * 761-781, 1151-1174, 1205-1267): node_kind 1 tet vertex, 2 edge, 3 face, 4 tet centre;

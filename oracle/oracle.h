@@ -129,6 +129,13 @@ int  orc_linhex_locate(const orc_lintet *L, const double p[3], int guess);
  * (src/InterpolatorCells.cpp:1237-1244, 1272-1317, 1334-1353) */
 void orc_linhex_shape_functions(const double xe[8][3], const double p[3], double sf[8]);
 
+/* PoissonSolver::assemble_space_charge_fast (src/PoissonSolver.cpp:300-319): particles in order,
+ * b[dofs of solver cell c_p] += shape functions of femocs hex deal2femocs[c_p] at x_p, in the
+ * cell's lexicographic order (shape_funs_dealii, src/InterpolatorCells.cpp:1355-1358), times
+ * factor (q/eps0 * W_sp).  nodes: the femocs node coordinates the hexes index (UCD order) */
+void orc_space_charge(const orc_mesh *m, const double *nodes, const int *hexes, const int *deal2femocs, int n,
+                      const double *pts, const int *cells, double factor, double *b);
+
 /* QuadraticTetrahedra::calc_cell (src/InterpolatorCells.cpp:1151-1174): per tet its four
  * vertices and the six edge-centre nodes n5..n10 found as the first common EDGECENTROID node
  * (node_kind 2) of the hexes 4t+i, 4t+j (pairs 01, 12, 20, 03, 13, 23).  cells10: 10 per tet */

@@ -88,6 +88,7 @@ def _declare(L):
     L.orc_lintet_bcc.argtypes = [C.POINTER(_Lintet), _D, C.c_int, _D]
     L.orc_locate_chain.argtypes = [C.POINTER(_Lintet), C.c_int, _D, C.c_int, _I]
     L.orc_linhex_shape_functions.argtypes = [_D, _D, _D]
+    L.orc_space_charge.argtypes = [C.POINTER(_Mesh), _D, _I, _I, C.c_int, _D, _I, C.c_double, _D]
     L.orc_qp_fields.argtypes = [C.POINTER(_Mesh), _D, C.c_void_p, C.c_void_p]
     L.orc_quadtet_cells.argtypes = [C.c_int, _I, _I, _I, _I]
     L.orc_extract_nodal.argtypes = [C.c_int, _D, C.c_int, _I, _I, C.c_int, _U8, _D, C.c_void_p, C.c_double, _D]
@@ -462,6 +463,15 @@ def linhex_shape_functions(xe_ucd, p):
     lib().orc_linhex_shape_functions(np.ascontiguousarray(xe_ucd, np.float64).reshape(-1),
                                      np.ascontiguousarray(p, np.float64), sf)
     return sf
+
+
+def space_charge(m: "Mesh", nodes, hexes, deal2femocs, points, cells, factor, b=None):
+    """b += the particles' charge deposit (PoissonSolver::assemble_space_charge_fast)."""
+    pts = _f64(points)
+    b = np.zeros(m.n_dofs) if b is None else b
+    lib().orc_space_charge(C.byref(m._m), _f64(nodes), _i32(hexes), _i32(deal2femocs), len(pts) // 3, pts,
+                           _i32(cells), float(factor), b)
+    return b
 
 
 def resistivity(T):

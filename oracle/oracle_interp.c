@@ -777,6 +777,23 @@ void orc_interp_surface_cells(const orc_lintri *T, const orc_lintet *L, const do
 /* physical quantities                                                                    */
 /* ------------------------------------------------------------------------------------- */
 /* resistivity of bulk Cu, (T [K], rho [Ohm nm]) pairs, src/PhysicalQuantityData.cpp:47-76 */
+/* PoissonSolver::assemble_space_charge_fast (src/PoissonSolver.cpp:300-319) */
+void orc_space_charge(const orc_mesh *m, const double *nodes, const int *hexes, const int *deal2femocs, int n,
+                      const double *pts, const int *cells, double factor, double *b) {
+    /* LinearHexahedra::shape_funs_dealii (src/InterpolatorCells.cpp:1355-1358): femocs (UCD)
+     * order to the solver cell's lexicographic order */
+    static const int ucd_of_lex[8] = {0, 1, 4, 5, 3, 2, 7, 6};
+    for (int p = 0; p < n; ++p) {
+        const int c = cells[p];
+        const int *h = hexes + 8 * deal2femocs[c];
+        double xe[8][3], sf[8];
+        for (int k = 0; k < 8; ++k)
+            for (int d = 0; d < 3; ++d) xe[k][d] = nodes[3 * h[k] + d];
+        orc_linhex_shape_functions(xe, pts + 3 * p, sf);
+        for (int k = 0; k < 8; ++k) b[m->cells[8 * c + k]] += sf[ucd_of_lex[k]] * factor;
+    }
+}
+
 static const double RHO_T[26] = {200, 250, 273.15, 300, 350, 400, 450, 500, 550, 600, 650, 700, 750,
                                  800, 850, 900, 950, 1000, 1050, 1100, 1150, 1200, 1250, 1300, 1350, 1400};
 static const double RHO_V[26] = {

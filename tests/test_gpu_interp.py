@@ -233,3 +233,73 @@ def test_surface_faces_invalid_calls(ctx, tip):
     ip.set_triangles(tris, t2t, quads)
     with pytest.raises(fgpu.FgpuError):
         ip.surface_cells(np.zeros((1, 3)), np.array([-2]), rank=3, centroids=True)
+
+
+def _particles(gm_cells_lex, x, n, seed, pool=None):
+    """Particles at random natural coordinates of random cells (drawn from ``pool`` cells when
+    given, so that many share dofs and the accumulation order is exercised)."""
+    rng = np.random.default_rng(seed)
+    pool = np.arange(len(gm_cells_lex)) if pool is None else pool
+    cells = rng.choice(pool, n)
+    a = rng.uniform(0.02, 0.98, (n, 3))
+    lex = np.array([[k & 1, (k >> 1) & 1, k >> 2] for k in range(8)])
+    w = np.prod(np.where(lex[None] == 1, a[:, None, :], 1 - a[:, None, :]), axis=2)
+    return np.einsum("nk,nkd->nd", w, x[gm_cells_lex[cells]]), cells.astype(np.int32)
+
+
+def test_space_charge_deposit_bit_exact(ctx, tip):
+    """PoissonSolver::assemble_space_charge_fast on the device: the deposit equals the oracle's
+    sequential loop bit for bit, with thousands of particles sharing a few cells."""
+    om = O.Mesh(tip.nodes, tip.vacuum_cells(), *MARKS)
+    gm = fgpu.Mesh(ctx, tip.nodes, tip.vacuum_cells(), *MARKS)
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+    d2f = np.nonzero(tip.hex_marker > 0)[0].astype(np.int32)
+    rng = np.random.default_rng(9)
+    pts, cells = _particles(om.cells, om.x, 4000, 10, pool=rng.choice(om.n_cells, 40, replace=False))
+    pts2, cells2 = _particles(om.cells, om.x, 3000, 11)
+    f = -0.0721
+    s = fgpu.System(gm)
+    s.reset()
+    s.add_space_charge(ip, pts, cells, f)
+    s.add_space_charge(ip, pts2, cells2, 2 * f)
+    ref = O.space_charge(om, tip.nodes, tip.hexes, d2f, pts, cells, f)
+    ref = O.space_charge(om, tip.nodes, tip.hexes, d2f, pts2, cells2, 2 * f, b=ref)
+    assert np.array_equal(s.rhs(), ref)
+    # on top of the Neumann load, as the assembly orders them
+    s.reset()
+    s.add_neumann(8, None, -1.0)
+    s.add_space_charge(ip, pts, cells, f)
+    ref = O.space_charge(om, tip.nodes, tip.hexes, d2f, pts, cells, f, b=om.face_rhs(8, constant=-1.0))
+    assert np.abs(s.rhs() - ref).max() <= 1e-13 * np.abs(ref).max()
+    with pytest.raises(fgpu.FgpuError):
+        s.add_space_charge(ip, pts[:1], np.array([gm.n_cells], np.int32), f)
+    with pytest.raises(fgpu.FgpuError):
+        s.reassemble()
+    s.add_dirichlet(2, 0.0)
+    s.apply_dirichlet()
+    with pytest.raises(fgpu.FgpuError):
+        s.add_space_charge(ip, pts, cells, f)
+
+
+def test_space_charge_through_the_solver_mirror(ctx, tip):
+    """A field-mode-with-particles assembly: the particles' charge shows in the write-step charge
+    density (rho_i vol_i sums to the deposited charge plus the Neumann load) and the system
+    solves."""
+    from femocs_amd import solvers
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+    om = O.Mesh(tip.nodes, tip.vacuum_cells(), *MARKS)
+    pts, cells = _particles(om.cells, om.x, 2000, 12)
+    P = solvers.ParticleSpecies(pos=pts, cell=cells, q_over_eps0=-0.18, Wsp=0.05)
+    p = solvers.PoissonSolver(particles=P, conf=solvers.FieldConfig(mode="transient"), interpolator=ip, ctx=ctx)
+    assert p.import_mesh(tip.nodes, tip.vacuum_cells())
+    p.setup(-1.0, 0.0)
+    p.write_step = True
+    p.assemble()
+    rho = p.export_charge_dens()
+    b = om.face_rhs(8, constant=-1.0)
+    b = O.space_charge(om, tip.nodes, tip.hexes, np.nonzero(tip.hex_marker > 0)[0].astype(np.int32), pts, cells,
+                       -0.18 * 0.05, b=b)
+    ref = om.charge_density(b)
+    assert np.abs(rho - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert abs((rho * om.dof_volumes()).sum() - b.sum()) <= 1e-9 * np.abs(b).sum()
+    assert p.solve() > 0

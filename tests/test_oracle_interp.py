@@ -263,3 +263,47 @@ def test_surface_centroid_mode(tip, surface_quads, rank):
     assert np.array_equal(cells, expect)
     c2 = np.where(cells < 0, 0, cells)
     assert np.array_equal(T.interp(L, m.tets, sol, pts, c2, **kw)[1][~miss], out[~miss])
+
+
+# ------------------------------------------------------------------------------------------
+# space charge deposit (PIC field mode)
+# ------------------------------------------------------------------------------------------
+def _particles_in_cells(m, n, seed):
+    """Points at random natural coordinates of random solver cells (trilinear map of the
+    cell's lexicographic vertices)."""
+    rng = np.random.default_rng(seed)
+    cells = rng.integers(0, m.n_cells, n)
+    a = rng.uniform(0.02, 0.98, (n, 3))
+    lex = np.array([[k & 1, (k >> 1) & 1, k >> 2] for k in range(8)])
+    w = np.prod(np.where(lex[None] == 1, a[:, None, :], 1 - a[:, None, :]), axis=2)      # (n, 8)
+    pts = np.einsum("nk,nkd->nd", w, m.x[m.cells[cells]])
+    return pts, cells.astype(np.int32)
+
+
+def test_space_charge_deposit_conserves_charge_and_first_moment(tip):
+    """Shape functions sum to one and reproduce linear functions: the deposit carries the total
+    charge and its first moment (sum_i b_i x_i = factor * sum_p x_p) -- a wrong order between
+    the femocs hex and the solver cell breaks the second."""
+    tip = tip[0]
+    m = O.Mesh(tip.nodes, tip.vacuum_cells(), 8, 2, 4, 2)
+    d2f = np.nonzero(tip.hex_marker > 0)[0].astype(np.int32)
+    pts, cells = _particles_in_cells(m, 500, 3)
+    f = 0.37
+    b = O.space_charge(m, tip.nodes, tip.hexes, d2f, pts, cells, f)
+    assert abs(b.sum() - f * len(pts)) < 1e-12 * len(pts)
+    assert np.allclose(m.x.T @ b, f * pts.sum(0), rtol=1e-10, atol=1e-9)
+    assert np.count_nonzero(b) <= 8 * len(pts)
+
+
+def test_space_charge_at_a_vertex_lands_on_its_dof(tip):
+    tip = tip[0]
+    m = O.Mesh(tip.nodes, tip.vacuum_cells(), 8, 2, 4, 2)
+    d2f = np.nonzero(tip.hex_marker > 0)[0].astype(np.int32)
+    rng = np.random.default_rng(4)
+    for c in rng.integers(0, m.n_cells, 5):
+        for k in (0, 3, 6):
+            dof = m.cells[c][k]
+            b = O.space_charge(m, tip.nodes, tip.hexes, d2f, m.x[dof][None], np.array([c], np.int32), 2.0)
+            e = np.zeros(m.n_dofs)
+            e[dof] = 2.0
+            assert np.abs(b - e).max() < 1e-12
