@@ -266,9 +266,9 @@ __global__ __launch_bounds__(256) void k_candidates(int n, const double *__restr
     cand[i] = make_int4(t0 == INT_MAX ? -1 : t0, cnt, amin, overflow);
 }
 
-// LinearHexahedra::shape_functions via project_to_nat_coords (src/InterpolatorCells.cpp:
-// 1237-1244, 1272-1317, 1334-1353), vertices in femocs order
-__device__ void linhex_sf(const D3 x[8], const double *pp, double sf[8]) {
+// LinearHexahedra::project_to_nat_coords (src/InterpolatorCells.cpp:1237-1244, 1272-1317):
+// natural coordinates of a point by Newton iterations, vertices in femocs order
+__device__ void linhex_uvw(const D3 x[8], const double *pp, double &u_out, double &v_out, double &w_out) {
     const D3 f0s = dvd(add(add(add(add(add(add(add(x[0], x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
     const D3 f1 = dvd(sub(add(add(sub(sub(add(add(scl(x[0], -1), x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
     const D3 f2 = dvd(add(add(sub(sub(add(add(sub(scl(x[0], -1), x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
@@ -296,6 +296,15 @@ __device__ void linhex_sf(const D3 x[8], const double *pp, double sf[8]) {
         w += dw;
         if (du * du + dv * dv + dw * dw < kZero) break;
     }
+    u_out = u;
+    v_out = v;
+    w_out = w;
+}
+
+// LinearHexahedra::shape_functions (src/InterpolatorCells.cpp:1334-1353)
+__device__ void linhex_sf(const D3 x[8], const double *pp, double sf[8]) {
+    double u, v, w;
+    linhex_uvw(x, pp, u, v, w);
     sf[0] = (1 - u) * (1 - v) * (1 - w) / 8.0;
     sf[1] = (1 + u) * (1 - v) * (1 - w) / 8.0;
     sf[2] = (1 + u) * (1 + v) * (1 - w) / 8.0;
@@ -304,6 +313,63 @@ __device__ void linhex_sf(const D3 x[8], const double *pp, double sf[8]) {
     sf[5] = (1 + u) * (1 - v) * (1 + w) / 8.0;
     sf[6] = (1 + u) * (1 + v) * (1 + w) / 8.0;
     sf[7] = (1 - u) * (1 + v) * (1 + w) / 8.0;
+}
+
+// LinearHexahedra::shape_fun_grads (src/InterpolatorCells.cpp:1363-1416): gradients of the
+// femocs-order shape functions at a point, through the Jacobian at its natural coordinates
+__device__ void linhex_sfg(const D3 x[8], const double *pp, D3 sfg[8]) {
+    double u, v, w;
+    linhex_uvw(x, pp, u, v, w);
+    D3 dN[8] = {mk(-(1 - v) * (1 - w), -(1 - u) * (1 - w), -(1 - u) * (1 - v)),
+                mk((1 - v) * (1 - w), -(1 + u) * (1 - w), -(1 + u) * (1 - v)),
+                mk((1 + v) * (1 - w), (1 + u) * (1 - w), -(1 + u) * (1 + v)),
+                mk(-(1 + v) * (1 - w), (1 - u) * (1 - w), -(1 - u) * (1 + v)),
+                mk(-(1 - v) * (1 + w), -(1 - u) * (1 + w), (1 - u) * (1 - v)),
+                mk((1 - v) * (1 + w), -(1 + u) * (1 + w), (1 + u) * (1 - v)),
+                mk((1 + v) * (1 + w), (1 + u) * (1 + w), (1 + u) * (1 + v)),
+                mk(-(1 + v) * (1 + w), (1 - u) * (1 + w), (1 - u) * (1 + v))};
+    for (int i = 0; i < 8; ++i) dN[i] = scl(dN[i], 0.125);
+    // J[i] = sum_k x_k dN_k,i
+    D3 J[3] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
+    for (int k = 0; k < 8; ++k) {
+        J[0] = add(J[0], scl(x[k], dN[k].x));
+        J[1] = add(J[1], scl(x[k], dN[k].y));
+        J[2] = add(J[2], scl(x[k], dN[k].z));
+    }
+    double Jdet = det3(J[0], J[1], J[2]);
+    Jdet = 1.0 / Jdet;
+    const D3 Ji[3] = {scl(mk(J[1].y * J[2].z - J[2].y * J[1].z, J[2].x * J[1].z - J[1].x * J[2].z,
+                             J[1].x * J[2].y - J[2].x * J[1].y), Jdet),
+                      scl(mk(J[2].y * J[0].z - J[0].y * J[2].z, J[0].x * J[2].z - J[2].x * J[0].z,
+                             J[2].x * J[0].y - J[0].x * J[2].y), Jdet),
+                      scl(mk(J[0].y * J[1].z - J[1].y * J[0].z, J[1].x * J[0].z - J[0].x * J[1].z,
+                             J[0].x * J[1].y - J[1].x * J[0].y), Jdet)};
+    for (int k = 0; k < 8; ++k) {
+        D3 g = mk(0, 0, 0);
+        g = add(g, scl(Ji[0], dN[k].x));
+        g = add(g, scl(Ji[1], dN[k].y));
+        g = add(g, scl(Ji[2], dN[k].z));
+        sfg[k] = g;
+    }
+}
+
+// InterpolatorCells::interp_gradient(point, cell) of LinearHexahedra (src/InterpolatorCells.cpp:
+// 407-421): minus the gradient of the nodal scalar2 at the point, in the given hex
+__global__ __launch_bounds__(256) void k_hex_gradient(int n, const double *__restrict__ pts, const int *__restrict__ hex,
+                                                      const int *__restrict__ hexes, const double *__restrict__ xyz,
+                                                      const double *__restrict__ sol, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int *h = hexes + 8 * (size_t)hex[i];
+    D3 x[8], g[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = ldx(xyz, h[k]);
+    linhex_sfg(x, pts + 3 * (size_t)i, g);
+    D3 e = mk(0, 0, 0);
+    for (int k = 0; k < 8; ++k) e = sub(e, scl(g[k], sol[5 * (size_t)h[k] + 4]));
+    out[3 * (size_t)i] = e.x;
+    out[3 * (size_t)i + 1] = e.y;
+    out[3 * (size_t)i + 2] = e.z;
 }
 
 // InterpolatorCells::interp_solution (src/InterpolatorCells.cpp:331-355) for ranks 1-3.
@@ -953,6 +1019,27 @@ extern "C" int fgpu_sys_add_space_charge(fgpu_sys *s, fgpu_interp *ip, int n, co
         }
         cleanup();
         s->space_charge = true;
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_interp_hex_gradient(fgpu_interp *ip, int n, const double *xyz, const int *hexes, double *grad) {
+    if (!ip || n < 0 || (n > 0 && (!xyz || !hexes || !grad))) return FGPU_EINVAL;
+    if (n == 0) return FGPU_OK;
+    if (ip->n_hexes == 0) return FGPU_EINVAL;
+    return guard(ip, [&] {
+        for (int i = 0; i < n; ++i)
+            if (hexes[i] < 0 || hexes[i] >= ip->n_hexes) throw std::invalid_argument("hex index out of range");
+        IPCHECK(hipSetDevice(ip->ctx->device));
+        hipStream_t st = ip->ctx->stream;
+        ensure_points(ip, n);
+        IPCHECK(hipMemcpyAsync(ip->pts, xyz, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+        IPCHECK(hipMemcpyAsync(ip->cells, hexes, (size_t)n * sizeof(int), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_hex_gradient, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, ip->cells, ip->hexes, ip->xyz,
+                           ip->sol, ip->out);
+        IPCHECK(hipGetLastError());
+        IPCHECK(hipMemcpyAsync(grad, ip->out, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        IPCHECK(hipStreamSynchronize(st));
         return FGPU_OK;
     });
 }

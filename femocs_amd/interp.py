@@ -62,6 +62,17 @@ class Interpolator:
         self.ctx.check(self.ctx.lib.fgpu_interp_get_nodal(self.h, out), "fgpu_interp_get_nodal")
         return out.reshape(-1, 5)
 
+    def hex_gradient(self, xyz, hexes) -> np.ndarray:
+        """LinearHexahedra::interp_gradient(point, hex) (src/InterpolatorCells.cpp:407-421):
+        minus the gradient of the nodal scalar2 at each point in its hex, (n, 3)."""
+        x = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        h = np.ascontiguousarray(hexes, np.int32).reshape(-1)
+        if x.size != 3 * h.size:
+            raise ValueError("one hex per point")
+        out = np.zeros(3 * h.size)
+        self.ctx.check(self.ctx.lib.fgpu_interp_hex_gradient(self.h, h.size, x, h, out), "fgpu_interp_hex_gradient")
+        return out.reshape(-1, 3)
+
     def set_nodal(self, sol5):
         s = np.ascontiguousarray(sol5, np.float64).reshape(-1)
         if s.size != 5 * self.n_nodes:

@@ -225,6 +225,11 @@ int fgpu_interp_extract(fgpu_interp *interp, fgpu_sys *sys, int smoothen);
 int fgpu_interp_extract_heat(fgpu_interp *interp, fgpu_sys *current, fgpu_sys *heat);
 /* the nodal solution (5 * n_nodes), as Interpolator::store_solution / nodes hold it */
 int fgpu_interp_get_nodal(fgpu_interp *interp, double *sol5);
+/* LinearHexahedra::interp_gradient(point, hex) (src/InterpolatorCells.cpp:407-421, shape_fun_grads
+ * :1363-1416): grad = -sum_i grad N_i(x) * scalar2 of node i, for each point in the given hex
+ * (femocs hex index: 4t + k), 3 per point; the particle field of the PIC push
+ * (src/Pic.cpp:39,204).  FGPU_EINVAL for a hex out of range. */
+int fgpu_interp_hex_gradient(fgpu_interp *interp, int n, const double *xyz, const int *hexes, double *grad);
 int fgpu_interp_set_nodal(fgpu_interp *interp, const double *sol5);
 /* SolutionReader::calc_full_interpolation, dim 3 (src/SolutionReader.cpp:43-165): every point
  * located with the guess chained from the previous one (starting at -1), then interpolated;

@@ -128,6 +128,10 @@ int  orc_linhex_locate(const orc_lintet *L, const double p[3], int guess);
 /* trilinear hex shape functions in femocs order via Newton inversion
  * (src/InterpolatorCells.cpp:1237-1244, 1272-1317, 1334-1353) */
 void orc_linhex_shape_functions(const double xe[8][3], const double p[3], double sf[8]);
+/* LinearHexahedra::interp_gradient(point, hex) (src/InterpolatorCells.cpp:407-421, 1363-1416):
+ * out = -sum_i grad N_i(p) * scalar2 of node i, for point q in hex hex[q] (3 per point) */
+void orc_linhex_gradient(const double *nodes, const int *hexes, const double *sol5, int n, const double *pts,
+                         const int *hex, double *out);
 
 /* PoissonSolver::assemble_space_charge_fast (src/PoissonSolver.cpp:300-319): particles in order,
  * b[dofs of solver cell c_p] += shape functions of femocs hex deal2femocs[c_p] at x_p, in the

@@ -88,6 +88,7 @@ def _declare(L):
     L.orc_lintet_bcc.argtypes = [C.POINTER(_Lintet), _D, C.c_int, _D]
     L.orc_locate_chain.argtypes = [C.POINTER(_Lintet), C.c_int, _D, C.c_int, _I]
     L.orc_linhex_shape_functions.argtypes = [_D, _D, _D]
+    L.orc_linhex_gradient.argtypes = [_D, _I, _D, C.c_int, _D, _I, _D]
     L.orc_space_charge.argtypes = [C.POINTER(_Mesh), _D, _I, _I, C.c_int, _D, _I, C.c_double, _D]
     L.orc_qp_fields.argtypes = [C.POINTER(_Mesh), _D, C.c_void_p, C.c_void_p]
     L.orc_quadtet_cells.argtypes = [C.c_int, _I, _I, _I, _I]
@@ -463,6 +464,15 @@ def linhex_shape_functions(xe_ucd, p):
     lib().orc_linhex_shape_functions(np.ascontiguousarray(xe_ucd, np.float64).reshape(-1),
                                      np.ascontiguousarray(p, np.float64), sf)
     return sf
+
+
+def linhex_gradient(nodes, hexes, sol5, points, hex):
+    """-grad of the nodal scalar2 at the points in the given hexes (interp_gradient)."""
+    pts = _f64(points)
+    n = len(pts) // 3
+    out = np.zeros(3 * n)
+    lib().orc_linhex_gradient(_f64(nodes), _i32(hexes), _f64(sol5), n, pts, _i32(hex), out)
+    return out.reshape(-1, 3)
 
 
 def space_charge(m: "Mesh", nodes, hexes, deal2femocs, points, cells, factor, b=None):

@@ -201,9 +201,7 @@ void orc_locate_chain(const orc_lintet *L, int np, const double *xyz, int rank, 
 /* ------------------------------------------------------------------------------------- */
 /* linear hexahedra                                                                       */
 /* ------------------------------------------------------------------------------------- */
-void orc_linhex_shape_functions(const double xe[8][3], const double p[3], double sf[8]) {
-    v3 x[8];
-    for (int i = 0; i < 8; ++i) x[i] = mk(xe[i][0], xe[i][1], xe[i][2]);
+static void linhex_uvw(const v3 x[8], const double p[3], double *u_out, double *v_out, double *w_out) {
     /* LinearHexahedra::precompute, src/InterpolatorCells.cpp:1237-1244 */
     v3 f0s = dvd(add(add(add(add(add(add(add(x[0], x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
     v3 f1 = dvd(sub(add(add(sub(sub(add(add(scl(x[0], -1), x[1]), x[2]), x[3]), x[4]), x[5]), x[6]), x[7]), 8.0);
@@ -232,6 +230,17 @@ void orc_linhex_shape_functions(const double xe[8][3], const double p[3], double
         w += dw;
         if (du * du + dv * dv + dw * dw < ZERO) break;
     }
+    *u_out = u;
+    *v_out = v;
+    *w_out = w;
+}
+
+void orc_linhex_shape_functions(const double xe[8][3], const double p[3], double sf[8]) {
+    v3 x[8];
+    double u, v, w;
+    for (int i = 0; i < 8; ++i) x[i] = mk(xe[i][0], xe[i][1], xe[i][2]);
+    linhex_uvw(x, p, &u, &v, &w);
+    /* shape_functions, src/InterpolatorCells.cpp:1334-1353 */
     sf[0] = (1 - u) * (1 - v) * (1 - w) / 8.0;
     sf[1] = (1 + u) * (1 - v) * (1 - w) / 8.0;
     sf[2] = (1 + u) * (1 + v) * (1 - w) / 8.0;
@@ -240,6 +249,51 @@ void orc_linhex_shape_functions(const double xe[8][3], const double p[3], double
     sf[5] = (1 + u) * (1 - v) * (1 + w) / 8.0;
     sf[6] = (1 + u) * (1 + v) * (1 + w) / 8.0;
     sf[7] = (1 - u) * (1 + v) * (1 + w) / 8.0;
+}
+
+/* LinearHexahedra::shape_fun_grads(point, hex) (src/InterpolatorCells.cpp:1363-1416) and
+ * interp_gradient(point, cell) (:407-421): E = -sum_i grad N_i(p) scalar2_i */
+void orc_linhex_gradient(const double *nodes, const int *hexes, const double *sol5, int n, const double *pts,
+                         const int *hex, double *out) {
+    for (int q = 0; q < n; ++q) {
+        const int *h = hexes + 8 * hex[q];
+        v3 x[8], dN[8], J[3], Ji[3], e = mk(0, 0, 0);
+        double u, v, w;
+        for (int k = 0; k < 8; ++k) x[k] = mk(nodes[3 * h[k]], nodes[3 * h[k] + 1], nodes[3 * h[k] + 2]);
+        linhex_uvw(x, pts + 3 * q, &u, &v, &w);
+        dN[0] = mk(-(1 - v) * (1 - w), -(1 - u) * (1 - w), -(1 - u) * (1 - v));
+        dN[1] = mk((1 - v) * (1 - w), -(1 + u) * (1 - w), -(1 + u) * (1 - v));
+        dN[2] = mk((1 + v) * (1 - w), (1 + u) * (1 - w), -(1 + u) * (1 + v));
+        dN[3] = mk(-(1 + v) * (1 - w), (1 - u) * (1 - w), -(1 - u) * (1 + v));
+        dN[4] = mk(-(1 - v) * (1 + w), -(1 - u) * (1 + w), (1 - u) * (1 - v));
+        dN[5] = mk((1 - v) * (1 + w), -(1 + u) * (1 + w), (1 + u) * (1 - v));
+        dN[6] = mk((1 + v) * (1 + w), (1 + u) * (1 + w), (1 + u) * (1 + v));
+        dN[7] = mk(-(1 + v) * (1 + w), (1 - u) * (1 + w), (1 - u) * (1 + v));
+        for (int k = 0; k < 8; ++k) dN[k] = scl(dN[k], 0.125);
+        J[0] = J[1] = J[2] = mk(0, 0, 0);
+        for (int k = 0; k < 8; ++k) {
+            J[0] = add(J[0], scl(x[k], dN[k].x));
+            J[1] = add(J[1], scl(x[k], dN[k].y));
+            J[2] = add(J[2], scl(x[k], dN[k].z));
+        }
+        const double Jdet = 1.0 / det3(J[0], J[1], J[2]);
+        Ji[0] = scl(mk(J[1].y * J[2].z - J[2].y * J[1].z, J[2].x * J[1].z - J[1].x * J[2].z,
+                       J[1].x * J[2].y - J[2].x * J[1].y), Jdet);
+        Ji[1] = scl(mk(J[2].y * J[0].z - J[0].y * J[2].z, J[0].x * J[2].z - J[2].x * J[0].z,
+                       J[2].x * J[0].y - J[0].x * J[2].y), Jdet);
+        Ji[2] = scl(mk(J[0].y * J[1].z - J[1].y * J[0].z, J[1].x * J[0].z - J[0].x * J[1].z,
+                       J[0].x * J[1].y - J[1].x * J[0].y), Jdet);
+        for (int k = 0; k < 8; ++k) {
+            v3 g = mk(0, 0, 0);
+            g = add(g, scl(Ji[0], dN[k].x));
+            g = add(g, scl(Ji[1], dN[k].y));
+            g = add(g, scl(Ji[2], dN[k].z));
+            e = sub(e, scl(g, sol5[5 * h[k] + 4]));
+        }
+        out[3 * q] = e.x;
+        out[3 * q + 1] = e.y;
+        out[3 * q + 2] = e.z;
+    }
 }
 
 /* shape_fun_grads(hex, node), src/InterpolatorCells.cpp:1319-1332, 1462-1505 */

@@ -303,3 +303,24 @@ def test_space_charge_through_the_solver_mirror(ctx, tip):
     assert np.abs(rho - ref).max() <= 1e-12 * np.abs(ref).max()
     assert abs((rho * om.dof_volumes()).sum() - b.sum()) <= 1e-9 * np.abs(b).sum()
     assert p.solve() > 0
+
+
+def test_hex_gradient_bit_exact(ctx, tip, solved):
+    """interp_gradient(point, hex) of the PIC push, on the device, equals the oracle bit for bit
+    for points in random vacuum hexes and at their vertices; out-of-range hexes are refused."""
+    _, s, phi, used = solved
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+    ip.extract_solution(s, smoothen=False)
+    sol = ip.nodal()
+    rng = np.random.default_rng(31)
+    hx = rng.choice(np.nonzero(tip.hex_marker > 0)[0], 3000).astype(np.int32)
+    uvw = rng.uniform(-1.0, 1.0, (len(hx), 3))
+    UVW = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]])
+    w = np.prod(1 + UVW[None] * uvw[:, None, :], axis=2) / 8.0
+    pts = np.einsum("nk,nkd->nd", w, tip.nodes[tip.hexes[hx]])
+    pts[:100] = tip.nodes[tip.hexes[hx[:100], 6]]
+    g = ip.hex_gradient(pts, hx)
+    ref = O.linhex_gradient(tip.nodes, tip.hexes, sol, pts, hx)
+    assert np.array_equal(g, ref)
+    with pytest.raises(fgpu.FgpuError):
+        ip.hex_gradient(pts[:1], np.array([len(tip.hexes)], np.int32))

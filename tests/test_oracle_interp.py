@@ -307,3 +307,42 @@ def test_space_charge_at_a_vertex_lands_on_its_dof(tip):
             e = np.zeros(m.n_dofs)
             e[dof] = 2.0
             assert np.abs(b - e).max() < 1e-12
+
+
+_UVW = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]])
+
+
+def _points_in_hexes(m, n, seed):
+    """Points at random natural coordinates of random vacuum hexes (femocs vertex order)."""
+    rng = np.random.default_rng(seed)
+    hx = rng.choice(np.nonzero(m.hex_marker > 0)[0], n).astype(np.int32)
+    uvw = rng.uniform(-0.9, 0.9, (n, 3))
+    w = np.prod(1 + _UVW[None] * uvw[:, None, :], axis=2) / 8.0
+    return np.einsum("nk,nkd->nd", w, m.nodes[m.hexes[hx]]), hx
+
+
+def test_hex_gradient_of_linear_field_is_exact(tip):
+    m = tip[0]
+    sol = np.zeros((len(m.nodes), 5))
+    sol[:, 4] = m.nodes @ A + 0.25
+    pts, hx = _points_in_hexes(m, 300, 21)
+    g = O.linhex_gradient(m.nodes, m.hexes, sol, pts, hx)
+    assert np.abs(g + A).max() < 1e-11
+
+
+def test_hex_gradient_matches_differentiated_shape_functions(tip):
+    """-grad of the trilinear interpolant by central differences of the shape functions (which
+    are pinned on their own), for a field that is not linear."""
+    m = tip[0]
+    rng = np.random.default_rng(22)
+    sol = np.zeros((len(m.nodes), 5))
+    sol[:, 4] = rng.standard_normal(len(m.nodes))
+    pts, hx = _points_in_hexes(m, 60, 23)
+    g = O.linhex_gradient(m.nodes, m.hexes, sol, pts, hx)
+    h = 1e-6
+    for p, c, gi in zip(pts, hx, g):
+        xe = m.nodes[m.hexes[c]]
+        phi = sol[m.hexes[c], 4]
+        fd = [(O.linhex_shape_functions(xe, p + h * e) @ phi - O.linhex_shape_functions(xe, p - h * e) @ phi) / (2 * h)
+              for e in np.eye(3)]
+        assert np.allclose(gi, -np.array(fd), rtol=1e-5, atol=1e-6 * np.abs(phi).max())
