@@ -115,7 +115,7 @@ void fgpu::Sys::clear_terms() {
     rhs_terms.clear();
     lifted = false;
     rho_saved = false;
-    space_charge = false;
+    no_replay = nullptr;
 }
 
 fgpu::Sys::~Sys() {
@@ -949,6 +949,28 @@ static void ensure_diag(fgpu_sys *s) {
     s->diag_gen = m->coord_gen;
 }
 
+extern "C" int fgpu_sys_sigma_terms(fgpu_sys *cur, const fgpu_sys *heat, double sigma0, double alpha) {
+    if (!cur || !heat || !(sigma0 > 0) || !std::isfinite(alpha)) return FGPU_EINVAL;
+    return guarded(cur->ctx, [&] {
+        fgpu_mesh *m = static_cast<fgpu_mesh *>(cur->mesh);
+        if (heat->mesh != cur->mesh) throw std::invalid_argument("current and heat systems must share the mesh");
+        if (cur->lifted) throw std::invalid_argument("coefficients are set before fgpu_sys_apply_dirichlet");
+        hipStream_t st = cur->ctx->stream;
+        HIPCHECK(hipSetDevice(cur->ctx->device));
+        const size_t nc = m->n_cells;
+        if (!cur->kq) {
+            cur->kq = dalloc<double>(8 * nc); cur->gq = dalloc<double>(8 * nc);
+            cur->kq32 = dalloc<float>(8 * nc); cur->gq32 = dalloc<float>(8 * nc);
+        }
+        launch_sigma_terms(st, (int)nc, m->d_cells, heat->x, sigma0, alpha, cur->kq, cur->gq, cur->kq32, cur->gq32);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(st));
+        cur->have_coef = true;
+        cur->diag_valid = false;
+        return FGPU_OK;
+    });
+}
+
 extern "C" int fgpu_sys_set_coefficients(fgpu_sys *s, const double *kq, const double *gq) {
     if (!s) return FGPU_EINVAL;
     return guarded(s->ctx, [&] {
@@ -985,8 +1007,13 @@ extern "C" int fgpu_sys_set_coefficients(fgpu_sys *s, const double *kq, const do
 }
 
 extern "C" int fgpu_sys_heat_terms(fgpu_sys *h, const fgpu_sys *cur, double gamma, double lorentz) {
-    if (!h || !cur || !(gamma >= 0)) return FGPU_EINVAL;
+    return fgpu_sys_heat_terms_theta(h, cur, gamma, lorentz, 1.0);
+}
+
+extern "C" int fgpu_sys_heat_terms_theta(fgpu_sys *h, const fgpu_sys *cur, double gamma, double lorentz, double theta) {
+    if (!h || !cur || !(gamma >= 0) || !(theta > 0 && theta <= 1)) return FGPU_EINVAL;
     return guarded(h->ctx, [&] {
+        if (h->lifted) throw std::invalid_argument("heat terms are added before fgpu_sys_apply_dirichlet");
         fgpu_mesh *m = static_cast<fgpu_mesh *>(h->mesh);
         if (cur->mesh != h->mesh) throw std::invalid_argument("heat and current systems must share the mesh");
         hipStream_t st = h->ctx->stream;
@@ -998,7 +1025,9 @@ extern "C" int fgpu_sys_heat_terms(fgpu_sys *h, const fgpu_sys *cur, double gamm
         }
         double *f = dalloc<double>(8 * nc);
         h->rhs_terms.push_back({1, 0, f});
-        launch_heat_terms(st, (int)nc, m->d_cells, m->d_X64, h->x, cur->x, gamma, lorentz, h->kq, h->gq, h->kq32, h->gq32, f);
+        double *kx = theta < 1 ? dalloc<double>(8 * nc) : nullptr;
+        launch_heat_terms(st, (int)nc, m->d_cells, m->d_X64, h->x, cur->x, gamma, lorentz, theta, h->kq, h->gq, h->kq32,
+                          h->gq32, f, kx);
         h->have_coef = true;
         h->diag_valid = false;
         Desc d = make_desc64(h, true);
@@ -1006,6 +1035,26 @@ extern "C" int fgpu_sys_heat_terms(fgpu_sys *h, const fgpu_sys *cur, double gamm
         BatchTabs t = upload_batch(h->ctx, {{d}}, {h});
         launch_elem_rhs(st, t.sets[0], t.etab, t.ne, f, t.lds64);
         launch_gather_plain<double>(st, t.sets[0], t.dtab, t.nd, true);
+        if (kx) {
+            // explicit part of the theta step: b -= (1 - theta) K_kappa(T_old) T_old, every dof
+            double *gz = dalloc<double>(8 * nc);
+            launch_set_const(st, (int)(8 * nc), 0.0, gz);
+            Desc dx = make_desc64(h, true);
+            dx.kq = kx;
+            dx.gq = gz;
+            dx.ua = h->x;
+            dx.ub = nullptr;
+            dx.q = h->b;
+            BatchTabs tx = upload_batch(h->ctx, {{dx}}, {h});
+            launch_elem<double>(st, tx.sets[0], tx.etab, tx.ne, true, false, false, S_BETA, tx.lds64);
+            launch_gather_plain<double>(st, tx.sets[0], tx.dtab, tx.nd, true);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipStreamSynchronize(st));
+            dfree(gz);
+            dfree(kx);
+            h->no_replay = "the explicit stiffness term of a theta step depends on the previous temperature: reset "
+                           "and assemble again";
+        }
         HIPCHECK(hipStreamSynchronize(st));
         return FGPU_OK;
     });
@@ -1138,7 +1187,7 @@ extern "C" int fgpu_sys_save_charge_density(fgpu_sys *s) {
 extern "C" int fgpu_sys_reassemble(fgpu_sys *s) {
     if (!s) return FGPU_EINVAL;
     return guarded(s->ctx, [&] {
-        if (s->space_charge) throw std::invalid_argument("particles are deposited again after a move: reset and assemble");
+        if (s->no_replay) throw std::invalid_argument(s->no_replay);
         fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
         hipStream_t st = s->ctx->stream;
         HIPCHECK(hipSetDevice(s->ctx->device));

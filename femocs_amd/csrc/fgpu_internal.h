@@ -191,7 +191,9 @@ struct Sys {
     // charge density of a write step (fgpu_sys_save_charge_density), internal order
     double *rho = nullptr;
     bool rho_saved = false;
-    bool space_charge = false;    // particles deposited since the last reset (not replayable)
+    // a right-hand-side term since the last reset that fgpu_sys_reassemble cannot replay
+    // (particle deposits, the explicit stiffness term of a theta step): the reason, or null
+    const char *no_replay = nullptr;
     // two-level preconditioner of the mixed solver (fgpu_sys_set_coarse)
     bool coarse = false;
     int batches = 0;              // running batches holding the system (their tables point here)

@@ -1018,7 +1018,7 @@ extern "C" int fgpu_sys_add_space_charge(fgpu_sys *s, fgpu_interp *ip, int n, co
             throw;
         }
         cleanup();
-        s->space_charge = true;
+        s->no_replay = "particles are deposited again after a move: reset and assemble";
         return FGPU_OK;
     });
 }

@@ -41,7 +41,10 @@ hipError_t upload_affine_table(hipStream_t s);
 int metric_blocks(int nc);
 void launch_metric(hipStream_t s, int nc, const int *cells, const double *X, float *metric, double *devmax);
 void launch_heat_terms(hipStream_t s, int nc, const int *cells, const double *X, const double *T, const double *phic,
-                       double gamma, double lorentz, double *kq, double *gq, float *kq32, float *gq32, double *f);
+                       double gamma, double lorentz, double theta, double *kq, double *gq, float *kq32, float *gq32,
+                       double *f, double *kx);
+void launch_sigma_terms(hipStream_t s, int nc, const int *cells, const double *T, double sigma0, double alpha, double *kq,
+                        double *gq, float *kq32, float *gq32);
 void launch_motion(hipStream_t s, int rows, const int *idof, const int *ptr, const int *ctrl, const double *w,
                    const double *base, const double *disp, double *X64, float *X32);
 void launch_scatter_coords(hipStream_t s, int n, const int *idx, const double *xyz, double *X64, float *X32);

@@ -914,11 +914,13 @@ __device__ inline double pq_sigma(double T) {
 
 // HeatSolver::assemble_local_cell (src/CurrentHeatSolver.cpp:347-393) per Gauss point q of
 // cell c (internal order): T_old and grad phi_c interpolated at q; stiffness coefficient
-// kappa(T_old), mass coefficient gamma, source gamma T_old + sigma(T_old) |grad phi_c|^2
+// theta kappa(T_old), mass coefficient gamma, source gamma T_old + sigma(T_old) |grad phi_c|^2.
+// theta < 1 (the theta scheme, Crank-Nicolson at 1/2): kx = -(1 - theta) kappa(T_old), the
+// coefficient of the explicit stiffness term applied to T_old
 __global__ void k_heat_terms(int nc, const int *__restrict__ cells, const double4 *__restrict__ X,
                              const double *__restrict__ T, const double *__restrict__ phic, double gamma, double lorentz,
-                             double *__restrict__ kq, double *__restrict__ gq, float *__restrict__ kq32,
-                             float *__restrict__ gq32, double *__restrict__ f) {
+                             double theta, double *__restrict__ kq, double *__restrict__ gq, float *__restrict__ kq32,
+                             float *__restrict__ gq32, double *__restrict__ f, double *__restrict__ kx) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const int *cl = cells + 8 * (size_t)c;
@@ -961,19 +963,54 @@ __global__ void k_heat_terms(int nc, const int *__restrict__ cells, const double
         const double Tc = Tq < kRhoT[0] ? kRhoT[0] : (Tq > kRhoT[25] ? kRhoT[25] : Tq);
         const double kap = lorentz * Tc * pq_sigma(Tc);
         const size_t o = 8 * (size_t)c + q;
-        kq[o] = kap;
+        kq[o] = theta * kap;
         gq[o] = gamma;
-        kq32[o] = (float)kap;
+        kq32[o] = (float)(theta * kap);
         gq32[o] = (float)gamma;
         f[o] = gamma * Tq + sg * (vx * vx + vy * vy + vz * vz);
+        if (kx) kx[o] = -(1.0 - theta) * kap;
     }
 }
 
 void launch_heat_terms(hipStream_t s, int nc, const int *cells, const double *X, const double *T, const double *phic,
-                       double gamma, double lorentz, double *kq, double *gq, float *kq32, float *gq32, double *f) {
+                       double gamma, double lorentz, double theta, double *kq, double *gq, float *kq32, float *gq32,
+                       double *f, double *kx) {
     if (nc > 0)
         hipLaunchKernelGGL(k_heat_terms, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nc, cells,
-                           reinterpret_cast<const double4 *>(X), T, phic, gamma, lorentz, kq, gq, kq32, gq32, f);
+                           reinterpret_cast<const double4 *>(X), T, phic, gamma, lorentz, theta, kq, gq, kq32, gq32, f,
+                           kx);
+}
+
+// temperature-dependent conductivity of the current solve, per Gauss point of cell c
+// (internal order): sigma = sigma0 / (1 + alpha (T - 300 K)) with T interpolated at q;
+// no mass term
+__global__ void k_sigma_terms(int nc, const int *__restrict__ cells, const double *__restrict__ T, double sigma0,
+                              double alpha, double *__restrict__ kq, double *__restrict__ gq, float *__restrict__ kq32,
+                              float *__restrict__ gq32) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int *cl = cells + 8 * (size_t)c;
+    double t[8];
+    for (int n = 0; n < 8; ++n) t[n] = T[cl[n]];
+    for (int q = 0; q < 8; ++q) {
+        const int a = q & 1, b = (q >> 1) & 1, cc = (q >> 2) & 1;
+        double Tq = 0;
+        for (int n = 0; n < 8; ++n)
+            Tq += Wt<double>(a, n & 1) * Wt<double>(b, (n >> 1) & 1) * Wt<double>(cc, (n >> 2) & 1) * t[n];
+        const double sg = sigma0 / (1.0 + alpha * (Tq - 300.0));
+        const size_t o = 8 * (size_t)c + q;
+        kq[o] = sg;
+        gq[o] = 0.0;
+        kq32[o] = (float)sg;
+        gq32[o] = 0.0f;
+    }
+}
+
+void launch_sigma_terms(hipStream_t s, int nc, const int *cells, const double *T, double sigma0, double alpha, double *kq,
+                        double *gq, float *kq32, float *gq32) {
+    if (nc > 0)
+        hipLaunchKernelGGL(k_sigma_terms, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nc, cells, T, sigma0,
+                           alpha, kq, gq, kq32, gq32);
 }
 
 // -grad u at the Gauss point v2n of cell v2c (DealSolver::export_solution_grad)

@@ -93,6 +93,8 @@ API = [
     ("fgpu_interp_surface_points", C.c_int, [_P, C.c_int, _D, C.c_int, C.c_int, _I, _D]),
     ("fgpu_interp_surface_cells", C.c_int, [_P, C.c_int, _D, C.c_int, C.c_int, _I, _D]),
     ("fgpu_sys_heat_terms", C.c_int, [_P, _P, C.c_double, C.c_double]),
+    ("fgpu_sys_heat_terms_theta", C.c_int, [_P, _P, C.c_double, C.c_double, C.c_double]),
+    ("fgpu_sys_sigma_terms", C.c_int, [_P, _P, C.c_double, C.c_double]),
     ("fgpu_sys_apply32", C.c_int, [_P, _D, _D]),
     ("fgpu_sys_set_coarse", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
     ("fgpu_sys_get_coarse", C.c_int, [_P, C.POINTER(C.c_int), C.c_void_p, C.c_void_p]),
@@ -309,8 +311,18 @@ class System:
     def apply_dirichlet(self):
         self._c(self.ctx.lib.fgpu_sys_apply_dirichlet(self.h), "apply_dirichlet")
 
-    def heat_terms(self, current: "System", gamma: float, lorentz: float = 2.44e-8):
-        self._c(self.ctx.lib.fgpu_sys_heat_terms(self.h, current.h, float(gamma), float(lorentz)), "heat_terms")
+    def heat_terms(self, current: "System", gamma: float, lorentz: float = 2.44e-8, theta: float = 1.0):
+        """Heat-step terms from T_old (this system's solution) and the current potential; theta
+        1 is implicit Euler, 1/2 Crank-Nicolson."""
+        if theta == 1.0:
+            self._c(self.ctx.lib.fgpu_sys_heat_terms(self.h, current.h, float(gamma), float(lorentz)), "heat_terms")
+        else:
+            self._c(self.ctx.lib.fgpu_sys_heat_terms_theta(self.h, current.h, float(gamma), float(lorentz),
+                                                           float(theta)), "heat_terms_theta")
+
+    def sigma_terms(self, heat: "System", sigma0: float, alpha: float):
+        """sigma(T) = sigma0 / (1 + alpha (T - 300)) as this current system's conductivity."""
+        self._c(self.ctx.lib.fgpu_sys_sigma_terms(self.h, heat.h, float(sigma0), float(alpha)), "sigma_terms")
 
     def add_space_charge(self, interpolator, xyz, cells, charge_factor: float):
         """b += the particles' charge (PoissonSolver::assemble_space_charge_fast); cells are the

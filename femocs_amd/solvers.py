@@ -198,6 +198,11 @@ class HeatingConfig:
     cg_tolerance: float = 1e-9
     ssor_param: float = 1.2
     delta_time: float = 10.0          # fs
+    #: extensions (not in the reference): the heat step's theta (1 implicit Euler as the
+    #: reference, 1/2 Crank-Nicolson) and sigma(T) = sigma0 / (1 + alpha (T - 300 K)) in the
+    #: current solve instead of sigma = 1 (None)
+    theta: float = 1.0
+    sigma_T: tuple | None = None      # (sigma0, alpha)
 
 
 CU_RHO_CP = 3.4496e-24                # J/(K A^3), include/CurrentHeatSolver.h:110
@@ -227,6 +232,8 @@ class CurrentSolver(_BulkSolver):
     def assemble(self):
         s = self.system
         s.reset()
+        if self.conf.sigma_T is not None:
+            s.sigma_terms(self.owner.heat.system, *self.conf.sigma_T)
         s.add_neumann(BID_COPPER_SURFACE, self.bc_values, 0.0)
         s.add_dirichlet(BID_COPPER_BOTTOM, self.dirichlet_bc_value)
         s.apply_dirichlet()
@@ -247,7 +254,7 @@ class HeatSolver(_BulkSolver):
             raise ValueError(f"Invalid delta time: {delta_time}")
         s = self.system
         s.reset()
-        s.heat_terms(self.owner.current.system, CU_RHO_CP / delta_time, self.conf.lorentz)
+        s.heat_terms(self.owner.current.system, CU_RHO_CP / delta_time, self.conf.lorentz, self.conf.theta)
         if self.bc_values is not None:
             s.add_neumann(BID_COPPER_SURFACE, self.bc_values, 0.0)
         s.add_dirichlet(BID_COPPER_BOTTOM, self.dirichlet_bc_value)

@@ -272,6 +272,17 @@ int fgpu_interp_surface_cells(fgpu_interp *interp, int n, const double *xyz, int
  * int phi_i (gamma T_old + sigma(T_old) |grad phi_c|^2) to the right-hand side.  Call after
  * fgpu_sys_reset; gamma = 3.4496e-24 / dt [J/(K A^3 s)], lorentz [W Ohm/K^2] (2.44e-8). */
 int fgpu_sys_heat_terms(fgpu_sys *heat, const fgpu_sys *current, double gamma, double lorentz);
+/* The theta scheme of the same step (extension: the reference's HeatSolver::assemble_crank_nicolson,
+ * src/CurrentHeatSolver.cpp:155-272, is disabled): stiffness coefficient theta kappa(T_old), and
+ * b -= (1 - theta) K_kappa(T_old) T_old on top of the terms above; theta = 1/2 is Crank-Nicolson,
+ * theta = 1 is fgpu_sys_heat_terms.  0 < theta <= 1.  A system holding the explicit term is not
+ * replayed by fgpu_sys_reassemble. */
+int fgpu_sys_heat_terms_theta(fgpu_sys *heat, const fgpu_sys *current, double gamma, double lorentz, double theta);
+/* Temperature-dependent conductivity of the current solve (extension: the reference's current
+ * matrix has sigma = 1, src/CurrentHeatSolver.cpp:469): stiffness coefficient sigma0 /
+ * (1 + alpha (T - 300 K)) at every Gauss point, T the heat system's solution on the same mesh.
+ * Before fgpu_sys_apply_dirichlet. */
+int fgpu_sys_sigma_terms(fgpu_sys *current, const fgpu_sys *heat, double sigma0, double alpha);
 /* Two-level preconditioner for the FP32 CG of the mixed solver, no counterpart in the
  * reference (which uses SSOR, DealSolver::solve_cg, src/DealSolver.cpp:443-458):
  * M^-1 r = D^-1 r + Z E^-1 Z^T r, Z the indicator of aggregates of consecutive internal dofs

@@ -260,11 +260,12 @@ LORENTZ = 2.44e-8           # Lorenz number [W Ohm/K^2], src/Config.cpp:75
 BID_COPPER_BOTTOM = 7
 
 
-def current_solve(vertices, bulk_cells, face_j, tol=1e-12, precond="ssor", max_it=10000):
-    """CurrentSolver::assemble + solve (src/CurrentHeatSolver.cpp:421-484): sigma = 1, Neumann
-    flux face_j on the copper surface (one value per id-2 face), phi = 0 on the bottom (id 7)."""
+def current_solve(vertices, bulk_cells, face_j, tol=1e-12, precond="ssor", max_it=10000, sigma_q=None):
+    """CurrentSolver::assemble + solve (src/CurrentHeatSolver.cpp:421-484): sigma = 1 (or the
+    given conductivity per cell and Gauss point), Neumann flux face_j on the copper surface (one
+    value per id-2 face), phi = 0 on the bottom (id 7)."""
     m = Mesh(vertices, bulk_cells, BID_COPPER_SURFACE, BID_COPPER_BOTTOM, BID_SIDES, BID_COPPER_SURFACE)
-    m.assemble()
+    m.assemble(kappa_q=None if sigma_q is None else np.ascontiguousarray(sigma_q, np.float64).reshape(-1))
     b = m.face_rhs(BID_COPPER_SURFACE, values=face_j)
     x = np.zeros(m.n_dofs)
     m.apply_dirichlet(b, x, m.dirichlet_mask(BID_COPPER_BOTTOM), None)
@@ -272,18 +273,27 @@ def current_solve(vertices, bulk_cells, face_j, tol=1e-12, precond="ssor", max_i
     return m, x, it
 
 
-def heat_step(m: "Mesh", T_old, phi_c, dt, face_nottingham, T_amb, tol=1e-12, lorentz=LORENTZ, precond="ssor"):
+def heat_step(m: "Mesh", T_old, phi_c, dt, face_nottingham, T_amb, tol=1e-12, lorentz=LORENTZ, precond="ssor",
+              theta=1.0):
     """HeatSolver::assemble (implicit Euler, src/CurrentHeatSolver.cpp:116-152, 347-393) and solve
     on the bulk mesh m: A = gamma M + K_kappa(T_old), b = int phi_i (gamma T_old + sigma(T_old)
-    |grad phi_c|^2) + Nottingham flux on id 2, T = T_amb on id 7; T_old is the initial guess."""
+    |grad phi_c|^2) + Nottingham flux on id 2, T = T_amb on id 7; T_old is the initial guess.
+    theta < 1 (extension, Crank-Nicolson at 1/2): A = gamma M + theta K, and b gains
+    -(1 - theta) K T_old."""
     gamma = CU_RHO_CP / dt
     Tq, _ = m.qp_fields(T_old)
     _, gq = m.qp_fields(phi_c)
     kq = np.vectorize(lambda t: kappa(t, lorentz))(Tq)
     sq = np.vectorize(sigma)(Tq)
-    m.assemble(kappa_q=kq.reshape(-1), gamma_q=np.full(Tq.size, gamma))
+    explicit = None
+    if theta < 1.0:
+        m.assemble(kappa_q=kq.reshape(-1), gamma_q=np.zeros(Tq.size))
+        explicit = m.spmv(np.ascontiguousarray(T_old, np.float64))
+    m.assemble(kappa_q=(theta * kq).reshape(-1), gamma_q=np.full(Tq.size, gamma))
     f = gamma * Tq + sq * (gq ** 2).sum(-1)
     b = m.cell_rhs(f.reshape(-1))
+    if explicit is not None:
+        b -= (1.0 - theta) * explicit
     m.face_rhs(BID_COPPER_SURFACE, values=face_nottingham, b=b)
     x = np.ascontiguousarray(T_old, np.float64).copy()
     mask = m.dirichlet_mask(BID_COPPER_BOTTOM)
@@ -291,6 +301,12 @@ def heat_step(m: "Mesh", T_old, phi_c, dt, face_nottingham, T_amb, tol=1e-12, lo
     m.apply_dirichlet(b, x, mask, vals)
     it, res = m.pcg(b, x, tol=tol, relative=True, precond=precond)
     return x, it
+
+
+def sigma_T(Tq, sigma0, alpha):
+    """Temperature-dependent conductivity of the current solve (extension):
+    sigma0 / (1 + alpha (T - 300 K))."""
+    return sigma0 / (1.0 + alpha * (np.asarray(Tq) - 300.0))
 
 
 def nodal_field(nodes, hexes, hex_marker, region_sign, phi_nodes):

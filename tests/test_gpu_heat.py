@@ -126,3 +126,75 @@ def test_heat_diagonal_matches_operator(ctx, tip):
         e[i] = 1.0
         aii = heat.apply(e)[i]
         assert abs(b[i] - aii) <= 1e-13 * abs(aii), (i, b[i], aii)
+
+
+@pytest.mark.parametrize("precision", [fgpu.FGPU_FP64, fgpu.FGPU_MIXED])
+def test_coupled_crank_nicolson_with_sigma_T(ctx, tip, precision):
+    """The coupled extension: each step a current solve with sigma(T) = sigma0 / (1 + alpha
+    (T - 300 K)) from the previous temperature, then a Crank-Nicolson heat step (theta = 1/2)
+    with its Joule source, dt = 4 fs, 20 steps; the temperature of every step within 1e-9 of
+    the FP64 oracle's."""
+    sigma0, alpha = 1.0, 3.9e-3
+    gm = fgpu.Mesh(ctx, tip.nodes, tip.bulk_cells(), *MARKS)
+    J = _face_values(gm, 2, 3e-5, 4)         # larger currents run away as sigma(T) falls
+    N = _face_values(gm, 2, 1e-9, 5)
+    om = O.Mesh(tip.nodes, tip.bulk_cells(), *MARKS)
+    xyz, _ = gm.dofs()
+    T0 = 300.0 + 400.0 * np.exp(-((xyz[:, 2] - xyz[:, 2].max()) / 20.0) ** 2)
+    cur, heat = fgpu.System(gm), fgpu.System(gm)
+    heat.set_solution(T0)
+    cur.fill_solution(0.0)
+    To = T0.copy()
+    o = fgpu.opts(tol=1e-13, relative=True, precision=precision)
+    worst = 0.0
+    for step in range(20):
+        cur.reset()
+        cur.sigma_terms(heat, sigma0, alpha)
+        cur.add_neumann(2, J, 0.0)
+        cur.add_dirichlet(7, 0.0)
+        cur.apply_dirichlet()
+        assert cur.solve(o).iterations >= 0, step
+        heat.reset()
+        heat.heat_terms(cur, O.CU_RHO_CP / DT, O.LORENTZ, theta=0.5)
+        heat.add_neumann(2, N, 0.0)
+        heat.add_dirichlet(7, 300.0)
+        heat.apply_dirichlet()
+        assert heat.solve(o).iterations > 0, step
+        Tq, _ = om.qp_fields(To)
+        _, phic, _ = O.current_solve(tip.nodes, tip.bulk_cells(), J, tol=1e-14, sigma_q=O.sigma_T(Tq, sigma0, alpha))
+        To, it = O.heat_step(om, To, phic, DT, N, 300.0, tol=1e-14, theta=0.5)
+        T = heat.solution()
+        err = np.linalg.norm(T - To) / np.linalg.norm(To)
+        worst = max(worst, err)
+        assert err <= 1e-9, (step, err)
+    assert np.abs(T - T0).max() > 10.0              # the steps did change the temperature
+    with pytest.raises(fgpu.FgpuError):
+        heat.reassemble()                            # the explicit term is not replayable
+
+
+def test_solver_mirror_crank_nicolson_sigma_T(ctx, tip):
+    """CurrentHeatSolver with the extensions switched on in its config (theta = 1/2,
+    sigma(T)) gives the oracle's coupled steps."""
+    conf = solvers.HeatingConfig(theta=0.5, sigma_T=(1.0, 3.9e-3), cg_tolerance=1e-13)
+    chs = solvers.CurrentHeatSolver(conf=conf, ctx=ctx)
+    assert chs.import_mesh(tip.nodes, tip.bulk_cells())
+    chs.setup(300.0)
+    J = _face_values(chs.mesh, 2, 3e-5, 7)
+    N = _face_values(chs.mesh, 2, 1e-9, 8)
+    chs.current.bc_values, chs.heat.bc_values = J, N
+    chs.current.relative = chs.heat.relative = True
+    om = O.Mesh(tip.nodes, tip.bulk_cells(), *MARKS)
+    xyz, _ = chs.mesh.dofs()
+    T0 = 300.0 + 200.0 * np.exp(-((xyz[:, 2] - xyz[:, 2].max()) / 20.0) ** 2)
+    chs.heat.import_solution(T0)
+    To = T0.copy()
+    for step in range(3):
+        chs.current.assemble()
+        assert chs.current.solve() >= 0
+        chs.heat.assemble(DT)
+        assert chs.heat.solve() > 0
+        Tq, _ = om.qp_fields(To)
+        _, phic, _ = O.current_solve(tip.nodes, tip.bulk_cells(), J, tol=1e-14, sigma_q=O.sigma_T(Tq, 1.0, 3.9e-3))
+        To, _ = O.heat_step(om, To, phic, DT, N, 300.0, tol=1e-14, theta=0.5)
+        T = chs.heat.export_solution()
+        assert np.linalg.norm(T - To) <= 1e-9 * np.linalg.norm(To), step

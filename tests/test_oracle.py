@@ -412,3 +412,44 @@ def test_qp_fields_of_linear_function():
     pts = np.array([[t(q & 1), t((q >> 1) & 1), t((q >> 2) & 1)] for q in range(8)])
     assert np.allclose(gr[0], a, atol=1e-14)
     assert np.allclose(v[0], pts @ a + 2.0, atol=1e-14)
+
+
+# ------------------------------------------------------------------------------------------
+# heat step: implicit Euler and the theta scheme on a discrete eigenmode
+# ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("theta", [1.0, 0.5])
+def test_heat_step_amplifies_an_eigenmode_by_the_theta_factor(theta):
+    """Below 200 K the conductivity table is clamped, so kappa is one constant and the step is
+    linear.  Starting from T = eps v, v a generalized eigenvector of the free block
+    (K v = lambda gamma M v, from scipy, with T = 0 on the Dirichlet bottom), one step gives
+    g(lambda) eps v with g = (1 - (1 - theta) lambda) / (1 + theta lambda): 1/(1 + lambda) for
+    implicit Euler, the Crank-Nicolson factor for theta = 1/2."""
+    import scipy.linalg
+    nodes, _, hexes = box_hexes(n=(2, 2, 2), L=(1.0, 1.0, 1.0), jitter=0.05)
+    m = O.Mesh(nodes, hexes, 2, 7, 4, 2)
+    dt = 4e-15
+    gamma = O.CU_RHO_CP / dt
+    k0 = O.kappa(0.0)
+    nq = 8 * m.n_cells
+
+    def dense(kq, gq):
+        m.assemble(kappa_q=np.full(nq, kq), gamma_q=np.full(nq, gq))
+        rp, col, val = m.csr()
+        A = np.zeros((m.n_dofs, m.n_dofs))
+        for i in range(m.n_dofs):
+            A[i, col[rp[i]:rp[i + 1]]] = val[rp[i]:rp[i + 1]]
+        return A
+
+    K, Mm = dense(k0, 0.0), dense(0.0, 1.0)
+    free = m.dirichlet_mask(7) == 0
+    lam, V = scipy.linalg.eigh(K[np.ix_(free, free)], gamma * Mm[np.ix_(free, free)])
+    n_face = int(np.sum(np.asarray(m.bface_id) == 2))
+    for j in (0, len(lam) // 3, len(lam) - 1):
+        v = np.zeros(m.n_dofs)
+        v[free] = V[:, j] / np.abs(V[:, j]).max()
+        eps = 50.0
+        T, it = O.heat_step(m, eps * v, np.zeros(m.n_dofs), dt, np.zeros(n_face), 0.0, tol=1e-14, theta=theta)
+        assert it > 0
+        g = (1 - (1 - theta) * lam[j]) / (1 + theta * lam[j])
+        assert np.abs(T - g * eps * v).max() <= 1e-9 * eps, (j, lam[j])
+    assert lam[-1] > 10                       # stiff modes: the two schemes differ in sign there
