@@ -58,6 +58,8 @@ struct fgpu_interp {
     // (Interpolator::deal2femocs); host copy and device table
     std::vector<int> region_hex;
     int *d_region_hex = nullptr;
+    int *d_hex_cell = nullptr;              // inverse: solver cell of each hex, -1 outside
+    int *d_nbr_ptr = nullptr, *d_nbr = nullptr;   // tet neighbour lists (LinearTetrahedra)
     // space-charge deposit: API cell -> internal cell of mesh `sc_mesh`, checked against
     // the region's hexes once per mesh
     int *sc_icell = nullptr;
@@ -668,7 +670,8 @@ void free_triangles(fgpu_interp *ip) {
 void free_interp(fgpu_interp *ip) {
     for (double **p : {&ip->xyz, &ip->det0, &ip->det, &ip->cen, &ip->sol, &ip->pts, &ip->out}) release(*p);
     for (int **p : {&ip->tets, &ip->c10, &ip->hexes, &ip->marker, &ip->child, &ip->prim, &ip->n2c_ptr, &ip->n2c,
-                    &ip->vor_ptr, &ip->vor, &ip->node_idof, &ip->tet, &ip->cells, &ip->d_region_hex, &ip->sc_icell})
+                    &ip->vor_ptr, &ip->vor, &ip->node_idof, &ip->tet, &ip->cells, &ip->d_region_hex, &ip->sc_icell,
+                    &ip->d_hex_cell, &ip->d_nbr_ptr, &ip->d_nbr})
         release(*p);
     for (float **p : {&ip->box, &ip->cbox, &ip->leafbox}) release(*p);
     release(ip->cand);
@@ -717,6 +720,8 @@ extern "C" int fgpu_interp_create(fgpu_ctx *c, int n_nodes, const double *xyz, c
         ip->hxyz.assign(xyz, xyz + 3 * (size_t)n_nodes);
         ip->hkind.assign(node_kind, node_kind + n_nodes);
         ip->htets.assign(tets, tets + 4 * (size_t)n_tets);
+        ip->d_nbr_ptr = upload(ip->L.nbr_ptr.data(), ip->L.nbr_ptr.size(), s);
+        ip->d_nbr = upload(ip->L.nbr.data(), ip->L.nbr.size(), s);
         // LinearTetrahedra::precompute: decay factor of the smoothing, -1 / longest tet edge
         double emax = -1e308;
         for (int t = 0; t < n_tets; ++t) {
@@ -754,6 +759,10 @@ extern "C" int fgpu_interp_create(fgpu_ctx *c, int n_nodes, const double *xyz, c
             for (int h = 0; h < n_hexes; ++h)
                 if (in_region(h)) ip->region_hex.push_back(h);
             ip->d_region_hex = upload(ip->region_hex.data(), ip->region_hex.size(), s);
+            std::vector<int> hex_cell(n_hexes, -1);
+            for (size_t c = 0; c < ip->region_hex.size(); ++c) hex_cell[ip->region_hex[c]] = (int)c;
+            ip->d_hex_cell = upload(hex_cell.data(), hex_cell.size(), s);
+            IPCHECK(hipStreamSynchronize(s));       // hex_cell is read until the copy ends
             for (int h = 0; h < n_hexes; ++h)
                 if (in_region(h))
                     for (int k = 0; k < 8; ++k) nptr[hexes[8 * (size_t)h + k] + 1]++;
@@ -1019,6 +1028,87 @@ extern "C" int fgpu_sys_add_space_charge(fgpu_sys *s, fgpu_interp *ip, int n, co
         }
         cleanup();
         s->no_replay = "particles are deposited again after a move: reset and assemble";
+        return FGPU_OK;
+    });
+}
+
+// Pic::update_point_cell (src/Pic.cpp:185-195): each particle located from its own previous
+// cell, LinearHexahedra::locate_cell (src/InterpolatorCells.cpp:1536-1562) over
+// LinearTetrahedra::locate_cell (:269-307): the guess tet, its neighbours in order, else the
+// first containing region tet in index order (cand.x) or, if none, minus the nearest centroid
+// (cand.z); the hex by the largest barycentric coordinate; back to the solver cell, -1 when
+// the point lies outside every region tet
+__global__ __launch_bounds__(256) void k_update_cells(int n, const double *__restrict__ pts, const int4 *__restrict__ cand,
+                                                      const int *__restrict__ region_hex, const int *__restrict__ hex_cell,
+                                                      const int *__restrict__ nbr_ptr, const int *__restrict__ nbr,
+                                                      const double *__restrict__ det0, const double *__restrict__ det,
+                                                      int n_tets, int *__restrict__ cells) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double p[3] = {pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]};
+    const int4 c = cand[i];
+    if (c.w) {                 // the tree walk overflowed: the candidates are incomplete
+        cells[i] = INT_MIN;
+        return;
+    }
+    const int g = region_hex[cells[i]] / 4;
+    int r;
+    if (c.y == 0) {
+        r = -c.z;
+    } else if (c.y == 1 && c.x >= 0) {
+        r = c.x;
+    } else {
+        r = INT_MIN;
+        if (g >= 0 && g < n_tets) {
+            if (pic(det0, det, p, g)) {
+                r = g;
+            } else {
+                for (int e = nbr_ptr[g]; e < nbr_ptr[g + 1]; ++e)
+                    if (pic(det0, det, p, nbr[e])) {
+                        r = nbr[e];
+                        break;
+                    }
+            }
+        }
+        if (r == INT_MIN) r = c.x >= 0 ? c.x : -c.z;
+    }
+    if (r < 0) {
+        cells[i] = -1;
+        return;
+    }
+    double b[4];
+    bcc(det0, det, p, r, b);
+    int k = -1;
+    if (b[0] >= b[1] && b[0] >= b[2] && b[0] >= b[3]) k = 0;
+    else if (b[1] >= b[0] && b[1] >= b[2] && b[1] >= b[3]) k = 1;
+    else if (b[2] >= b[0] && b[2] >= b[1] && b[2] >= b[3]) k = 2;
+    else if (b[3] >= b[0] && b[3] >= b[1] && b[3] >= b[2]) k = 3;
+    cells[i] = k < 0 ? -1 : hex_cell[4 * r + k];
+}
+
+extern "C" int fgpu_interp_update_cells(fgpu_interp *ip, int n, const double *xyz, int *cells) {
+    if (!ip || n < 0 || (n > 0 && (!xyz || !cells))) return FGPU_EINVAL;
+    if (n == 0) return FGPU_OK;
+    if (ip->region_hex.empty()) return FGPU_EINVAL;
+    return guard(ip, [&] {
+        for (int i = 0; i < n; ++i)
+            if (cells[i] < 0 || cells[i] >= (int)ip->region_hex.size())
+                throw std::invalid_argument("particle cell out of range");
+        IPCHECK(hipSetDevice(ip->ctx->device));
+        hipStream_t st = ip->ctx->stream;
+        ensure_points(ip, n);
+        IPCHECK(hipMemcpyAsync(ip->pts, xyz, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+        IPCHECK(hipMemcpyAsync(ip->cells, cells, (size_t)n * sizeof(int), hipMemcpyHostToDevice, st));
+        tet_candidates(ip, n, ip->cand);
+        hipLaunchKernelGGL(k_update_cells, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, ip->cand, ip->d_region_hex,
+                           ip->d_hex_cell, ip->d_nbr_ptr, ip->d_nbr, ip->det0, ip->det, ip->n_tets, ip->cells);
+        IPCHECK(hipGetLastError());
+        std::vector<int> res(n);
+        IPCHECK(hipMemcpyAsync(res.data(), ip->cells, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, st));
+        IPCHECK(hipStreamSynchronize(st));
+        for (int v : res)
+            if (v == INT_MIN) throw std::runtime_error("point search stack overflow");
+        std::copy(res.begin(), res.end(), cells);
         return FGPU_OK;
     });
 }

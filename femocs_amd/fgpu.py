@@ -86,6 +86,7 @@ API = [
     ("fgpu_interp_extract_heat", C.c_int, [_P, _P, _P]),
     ("fgpu_interp_get_nodal", C.c_int, [_P, _D]),
     ("fgpu_interp_hex_gradient", C.c_int, [_P, C.c_int, _D, _I, _D]),
+    ("fgpu_interp_update_cells", C.c_int, [_P, C.c_int, _D, _I]),
     ("fgpu_interp_set_nodal", C.c_int, [_P, _D]),
     ("fgpu_interp_points", C.c_int, [_P, C.c_int, _D, C.c_int, _I, _D]),
     ("fgpu_interp_cells", C.c_int, [_P, C.c_int, _D, C.c_int, _I, _D]),

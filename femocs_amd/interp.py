@@ -73,6 +73,16 @@ class Interpolator:
         self.ctx.check(self.ctx.lib.fgpu_interp_hex_gradient(self.h, h.size, x, h, out), "fgpu_interp_hex_gradient")
         return out.reshape(-1, 3)
 
+    def update_cells(self, xyz, cells) -> np.ndarray:
+        """Pic::update_point_cell (src/Pic.cpp:185-195) for every particle: the solver cell
+        holding each point, located from the particle's previous cell; -1 outside the region."""
+        x = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        c = np.array(cells, np.int32).reshape(-1)
+        if x.size != 3 * c.size:
+            raise ValueError("one cell per particle")
+        self.ctx.check(self.ctx.lib.fgpu_interp_update_cells(self.h, c.size, x, c), "fgpu_interp_update_cells")
+        return c
+
     def set_nodal(self, sol5):
         s = np.ascontiguousarray(sol5, np.float64).reshape(-1)
         if s.size != 5 * self.n_nodes:

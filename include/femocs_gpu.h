@@ -230,6 +230,12 @@ int fgpu_interp_get_nodal(fgpu_interp *interp, double *sol5);
  * (femocs hex index: 4t + k), 3 per point; the particle field of the PIC push
  * (src/Pic.cpp:39,204).  FGPU_EINVAL for a hex out of range. */
 int fgpu_interp_hex_gradient(fgpu_interp *interp, int n, const double *xyz, const int *hexes, double *grad);
+/* Pic::update_point_cell (src/Pic.cpp:185-195): cells[i] holds particle i's solver cell
+ * (deal.II index, the region's hexes in hex order) and is replaced by the cell holding xyz[i],
+ * found by LinearHexahedra::locate_cell from that guess (src/InterpolatorCells.cpp:1536-1562,
+ * 269-307), or -1 when the point is outside every tet of the region.  Each particle uses its
+ * own guess (no chain).  FGPU_EINVAL for a cell out of range. */
+int fgpu_interp_update_cells(fgpu_interp *interp, int n, const double *xyz, int *cells);
 int fgpu_interp_set_nodal(fgpu_interp *interp, const double *sol5);
 /* SolutionReader::calc_full_interpolation, dim 3 (src/SolutionReader.cpp:43-165): every point
  * located with the guess chained from the previous one (starting at -1), then interpolated;

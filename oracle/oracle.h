@@ -133,6 +133,11 @@ void orc_linhex_shape_functions(const double xe[8][3], const double p[3], double
 void orc_linhex_gradient(const double *nodes, const int *hexes, const double *sol5, int n, const double *pts,
                          const int *hex, double *out);
 
+/* Pic::update_point_cell (src/Pic.cpp:185-195) for each particle: cells[i] (solver cell) ->
+ * LinearHexahedra::locate_cell from hex deal2femocs[cells[i]] -> femocs2deal, -1 outside */
+void orc_update_cells(const orc_lintet *L, const int *deal2femocs, const int *femocs2deal, int n, const double *pts,
+                      int *cells);
+
 /* PoissonSolver::assemble_space_charge_fast (src/PoissonSolver.cpp:300-319): particles in order,
  * b[dofs of solver cell c_p] += shape functions of femocs hex deal2femocs[c_p] at x_p, in the
  * cell's lexicographic order (shape_funs_dealii, src/InterpolatorCells.cpp:1355-1358), times

@@ -89,6 +89,7 @@ def _declare(L):
     L.orc_locate_chain.argtypes = [C.POINTER(_Lintet), C.c_int, _D, C.c_int, _I]
     L.orc_linhex_shape_functions.argtypes = [_D, _D, _D]
     L.orc_linhex_gradient.argtypes = [_D, _I, _D, C.c_int, _D, _I, _D]
+    L.orc_update_cells.argtypes = [C.POINTER(_Lintet), _I, _I, C.c_int, _D, _I]
     L.orc_space_charge.argtypes = [C.POINTER(_Mesh), _D, _I, _I, C.c_int, _D, _I, C.c_double, _D]
     L.orc_qp_fields.argtypes = [C.POINTER(_Mesh), _D, C.c_void_p, C.c_void_p]
     L.orc_quadtet_cells.argtypes = [C.c_int, _I, _I, _I, _I]
@@ -489,6 +490,14 @@ def linhex_gradient(nodes, hexes, sol5, points, hex):
     out = np.zeros(3 * n)
     lib().orc_linhex_gradient(_f64(nodes), _i32(hexes), _f64(sol5), n, pts, _i32(hex), out)
     return out.reshape(-1, 3)
+
+
+def update_cells(L: "LinTet", deal2femocs, femocs2deal, points, cells):
+    """Pic::update_point_cell for every particle (solver cells in, solver cells or -1 out)."""
+    pts = _f64(points)
+    c = np.array(cells, np.int32).reshape(-1)
+    lib().orc_update_cells(C.byref(L._L), _i32(deal2femocs), _i32(femocs2deal), len(c), pts, c)
+    return c
 
 
 def space_charge(m: "Mesh", nodes, hexes, deal2femocs, points, cells, factor, b=None):

@@ -831,6 +831,16 @@ void orc_interp_surface_cells(const orc_lintri *T, const orc_lintet *L, const do
 /* physical quantities                                                                    */
 /* ------------------------------------------------------------------------------------- */
 /* resistivity of bulk Cu, (T [K], rho [Ohm nm]) pairs, src/PhysicalQuantityData.cpp:47-76 */
+/* Pic::update_point_cell (src/Pic.cpp:185-195): deal cell -> femocs hex guess -> locate -> deal
+ * cell, -1 outside */
+void orc_update_cells(const orc_lintet *L, const int *deal2femocs, const int *femocs2deal, int n, const double *pts,
+                      int *cells) {
+    for (int i = 0; i < n; ++i) {
+        const int h = orc_linhex_locate(L, pts + 3 * i, deal2femocs[cells[i]]);
+        cells[i] = h < 0 ? -1 : femocs2deal[h];
+    }
+}
+
 /* PoissonSolver::assemble_space_charge_fast (src/PoissonSolver.cpp:300-319) */
 void orc_space_charge(const orc_mesh *m, const double *nodes, const int *hexes, const int *deal2femocs, int n,
                       const double *pts, const int *cells, double factor, double *b) {

@@ -324,3 +324,26 @@ def test_hex_gradient_bit_exact(ctx, tip, solved):
     assert np.array_equal(g, ref)
     with pytest.raises(fgpu.FgpuError):
         ip.hex_gradient(pts[:1], np.array([len(tip.hexes)], np.int32))
+
+
+def test_particle_cell_update_bit_exact(ctx, tip):
+    """Pic::update_point_cell for particles that stayed, moved to a neighbouring cell, moved far
+    (a random guess), left into the bulk or left the domain: the device result equals the
+    oracle's for every particle; out-of-range cells are refused."""
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+    d2f = np.nonzero(tip.hex_marker > 0)[0].astype(np.int32)
+    f2d = -np.ones(len(tip.hexes), np.int32)
+    f2d[d2f] = np.arange(len(d2f))
+    om = O.Mesh(tip.nodes, tip.vacuum_cells(), *MARKS)
+    pts, cells = _particles(om.cells, om.x, 3000, 51)
+    rng = np.random.default_rng(52)
+    pts[:1000] += rng.normal(0.0, 2.0, (1000, 3))                 # small moves
+    cells[1000:1500] = rng.integers(0, len(d2f), 500)             # stale guesses
+    pts[1500:1800] = _points(tip, 300, 53)[:300]                   # anywhere, outside included
+    got = ip.update_cells(pts, cells)
+    L = O.LinTet(tip.nodes, tip.tets, tip.tet_nbr, tip.tet_marker, 3)
+    ref = O.update_cells(L, d2f, f2d, pts, cells)
+    assert np.array_equal(got, ref)
+    assert (got == -1).any() and (got >= 0).mean() > 0.8
+    with pytest.raises(fgpu.FgpuError):
+        ip.update_cells(pts[:1], np.array([len(d2f)], np.int32))

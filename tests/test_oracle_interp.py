@@ -346,3 +346,28 @@ def test_hex_gradient_matches_differentiated_shape_functions(tip):
         fd = [(O.linhex_shape_functions(xe, p + h * e) @ phi - O.linhex_shape_functions(xe, p - h * e) @ phi) / (2 * h)
               for e in np.eye(3)]
         assert np.allclose(gi, -np.array(fd), rtol=1e-5, atol=1e-6 * np.abs(phi).max())
+
+
+def test_particle_cell_update_against_brute_force(tip):
+    """Pic::update_point_cell: the returned solver cell is the region hex whose tet holds the
+    point and whose vertex has the largest barycentric coordinate; points outside every
+    region tet get -1; a point still inside its own cell keeps it."""
+    m, L, _ = tip
+    d2f = np.nonzero(m.hex_marker > 0)[0].astype(np.int32)
+    f2d = -np.ones(len(m.hexes), np.int32)
+    f2d[d2f] = np.arange(len(d2f))
+    rng = np.random.default_rng(41)
+    pts, ts = _inside_points(m, 300, 42)
+    guess = rng.integers(0, len(d2f), len(pts)).astype(np.int32)
+    out = np.array([[0.0, 0.0, -1e4]])                  # far below the slab: outside
+    cells = O.update_cells(L, d2f, f2d, np.vstack([pts, out]), np.concatenate([guess, [0]]))
+    assert cells[-1] == -1
+    for p, c in zip(pts, cells[:-1]):
+        assert c >= 0
+        h = d2f[c]
+        t, k = h // 4, h % 4
+        b = L.bcc(p, t)
+        assert min(b) > -1e-12                           # the tet holds the point
+        assert b[k] == max(b)
+    again = O.update_cells(L, d2f, f2d, pts, cells[:-1])
+    assert np.array_equal(again, cells[:-1])
