@@ -190,6 +190,138 @@ def traffic_probe(n_launches: int = 5) -> None:
     log("copies done")
 
 
+def _best_of(f, reps=3):
+    """Smallest wall time of reps calls of f (each synchronises before it returns)."""
+    best = float("inf")
+    out = None
+    for _ in range(reps):
+        t = time.perf_counter()
+        out = f()
+        best = min(best, time.perf_counter() - t)
+    return best, out
+
+
+def paths_bench(no_cpu: bool):
+    """The paths around the field solve, measured on the larger tip (r = 10 nm, h = 60 nm on a
+    120 x 120 nm slab): the plain FP64 solve, the transfer of the solution to every surface
+    atom (ranks 1-3, located and mapped), the PIC pieces (space-charge deposit, particle cell
+    update, particle field) for 1M superparticles, and one coupled sigma(T) current +
+    Crank-Nicolson heat step on the r = 3 nm tip's bulk.  Host arrays cross PCIe inside these
+    timings (the calls take and return host buffers).  CPU baselines: the oracle on bounded
+    samples, one core.  One JSON line."""
+    from femocs_amd import fgpu, interp
+    from femocs_amd import mesh as M
+    out = {"metric": "secondary paths: transfer to atoms, PIC pieces, coupled heat step (1 GPU)",
+           "data": "synthetic"}
+    tip = M.make_tip_mesh(M.tip_r10nm_geometry())
+    ctx = fgpu.Context(0)
+    gm = fgpu.Mesh(ctx, tip.nodes, tip.vacuum_cells(), 8, 2, 4, 2)
+    s = fgpu.System(gm)
+    s.reset()
+    s.add_dirichlet(2, 0.0)
+    s.add_neumann(8, None, -1.0)
+    s.apply_dirichlet()
+
+    def cold(precision):
+        s.fill_solution(0.0)
+        return s.solve(fgpu.opts(tol=1e-9, relative=True, precision=precision, inner_tol=0.01))
+
+    t64, st64 = _best_of(lambda: cold(fgpu.FGPU_FP64))
+    tmx, stmx = _best_of(lambda: cold(fgpu.FGPU_MIXED))
+    out["solve_r10nm"] = {"hexes": gm.n_cells, "dofs": gm.n_dofs, "fp64_ms": round(t64 * 1e3, 2),
+                          "fp64_iterations": st64.iterations, "mixed_ms": round(tmx * 1e3, 2),
+                          "mixed_iterations": stmx.iterations}
+    log("solve", out["solve_r10nm"])
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+    te, _ = _best_of(lambda: ip.extract_solution(s, smoothen=True))
+    atoms = tip.atoms
+    tr = {"atoms": len(atoms), "extract_ms": round(te * 1e3, 2)}
+    for rank in (1, 2, 3):
+        tl, (cells, _) = _best_of(lambda: ip.points(atoms, rank))
+        tm, _ = _best_of(lambda: ip.in_cells(atoms, rank, cells))
+        tr[f"rank{rank}_locate_atoms_per_s"] = round(len(atoms) / tl)
+        tr[f"rank{rank}_mapped_atoms_per_s"] = round(len(atoms) / tm)
+    out["transfer"] = tr
+    log("transfer", tr)
+    # PIC pieces: superparticles at random natural coordinates of random solver cells
+    rng = np.random.default_rng(7)
+    npart = 1_000_000
+    hx = np.nonzero(tip.hex_marker > 0)[0]
+    pc = rng.integers(0, len(hx), npart).astype(np.int32)
+    UVW = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]])
+    uvw = rng.uniform(-0.95, 0.95, (npart, 3))
+    w = np.prod(1 + UVW[None] * uvw[:, None, :], axis=2) / 8.0
+    pts = np.einsum("nk,nkd->nd", w, tip.nodes[tip.hexes[hx[pc]]])
+    sp = fgpu.System(gm)
+
+    def deposit():
+        sp.reset()
+        sp.add_space_charge(ip, pts, pc, -1e-3)
+
+    td, _ = _best_of(deposit)
+    moved = pts + rng.normal(0.0, 0.5, pts.shape)
+    tu, _ = _best_of(lambda: ip.update_cells(moved, pc))
+    tg, _ = _best_of(lambda: ip.hex_gradient(pts, hx[pc].astype(np.int32)))
+    pic = {"particles": npart, "deposit_per_s": round(npart / td), "cell_update_per_s": round(npart / tu),
+           "field_per_s": round(npart / tg)}
+    out["pic"] = pic
+    log("pic", pic)
+    # one coupled step on the r = 3 nm tip's bulk (mixed precision, as served)
+    small = M.make_tip_mesh(M.tip_r3nm_geometry())
+    bm = fgpu.Mesh(ctx, small.nodes, small.bulk_cells(), 2, 7, 4, 2)
+    cen = bm.face_centroids(2)
+    J = 3e-5 * (1.0 + 0.5 * np.sin(cen[:, 0] / 7.0) * np.cos(cen[:, 2] / 5.0))
+    Nq = 1e-9 * np.ones(len(cen))
+    cur, heat = fgpu.System(bm), fgpu.System(bm)
+    xyz, _ = bm.dofs()
+    heat.set_solution(300.0 + 400.0 * np.exp(-((xyz[:, 2] - xyz[:, 2].max()) / 20.0) ** 2))
+    cur.fill_solution(0.0)
+    o = fgpu.opts(tol=1e-9, relative=True, precision=fgpu.FGPU_MIXED, inner_tol=0.01)
+
+    def coupled_step():
+        cur.reset()
+        cur.sigma_terms(heat, 1.0, 3.9e-3)
+        cur.add_neumann(2, J, 0.0)
+        cur.add_dirichlet(7, 0.0)
+        cur.apply_dirichlet()
+        cur.solve(o)
+        heat.reset()
+        heat.heat_terms(cur, 3.4496e-24 / 4e-15, 2.44e-8, theta=0.5)
+        heat.add_neumann(2, Nq, 0.0)
+        heat.add_dirichlet(7, 300.0)
+        heat.apply_dirichlet()
+        return heat.solve(o)
+
+    th, _ = _best_of(coupled_step, reps=5)
+    out["coupled_step_r3nm"] = {"bulk_hexes": bm.n_cells, "dofs": bm.n_dofs, "ms": round(th * 1e3, 2)}
+    log("coupled", out["coupled_step_r3nm"])
+    if not no_cpu:
+        from oracle import oracle as O
+        cpu = {"cores": 1, "kind": "port"}
+        L = O.LinTet(tip.nodes, tip.tets, tip.tet_nbr, tip.tet_marker, 3)
+        q10 = O.quadtet_cells(tip.tets, tip.hexes, tip.node_kind)
+        sub = atoms[:: max(1, len(atoms) // 2000)]
+        sol = ip.nodal()
+        t = time.perf_counter()
+        O.interp_points(L, tip.nodes, tip.tets, q10, tip.hexes, sol, sub, 2)
+        cpu["rank2_locate_atoms_per_s"] = round(len(sub) / (time.perf_counter() - t), 1)
+        om = O.Mesh(tip.nodes, tip.vacuum_cells(), 8, 2, 4, 2)
+        d2f = hx.astype(np.int32)
+        f2d = -np.ones(len(tip.hexes), np.int32)
+        f2d[d2f] = np.arange(len(d2f))
+        k = 20000
+        t = time.perf_counter()
+        O.space_charge(om, tip.nodes, tip.hexes, d2f, pts[:k], pc[:k], -1e-3)
+        cpu["deposit_per_s"] = round(k / (time.perf_counter() - t))
+        t = time.perf_counter()
+        O.update_cells(L, d2f, f2d, moved[:k], pc[:k])
+        cpu["cell_update_per_s"] = round(k / (time.perf_counter() - t))
+        cpu["sample"] = f"{len(sub)} atoms (every {max(1, len(atoms) // 2000)}th), {k} particles"
+        out["cpu_baseline"] = cpu
+        log("cpu", cpu)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,9 +336,14 @@ def main():
                     help="clients submit an exponential think time after each result instead of Poisson arrivals")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only the counter target of roofline.traffic (see traffic_probe)")
+    ap.add_argument("--paths", action="store_true",
+                    help="the secondary paths instead of the headline (transfer, PIC pieces, coupled heat)")
     args = ap.parse_args()
     if args.traffic_probe:
         traffic_probe()
+        return
+    if args.paths:
+        paths_bench(args.no_cpu)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the field solver has no multi-GPU path "
