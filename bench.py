@@ -222,9 +222,11 @@ def paths_bench(no_cpu: bool):
     s.add_neumann(8, None, -1.0)
     s.apply_dirichlet()
 
+    # direct launches, as in the serving loop (a rocprofv3 kernel trace of graph-captured
+    # solves ended the host process)
     def cold(precision):
         s.fill_solution(0.0)
-        return s.solve(fgpu.opts(tol=1e-9, relative=True, precision=precision, inner_tol=0.01))
+        return s.solve(fgpu.opts(tol=1e-9, relative=True, precision=precision, inner_tol=0.01, use_graph=False))
 
     t64, st64 = _best_of(lambda: cold(fgpu.FGPU_FP64))
     tmx, stmx = _best_of(lambda: cold(fgpu.FGPU_MIXED))
@@ -276,7 +278,7 @@ def paths_bench(no_cpu: bool):
     xyz, _ = bm.dofs()
     heat.set_solution(300.0 + 400.0 * np.exp(-((xyz[:, 2] - xyz[:, 2].max()) / 20.0) ** 2))
     cur.fill_solution(0.0)
-    o = fgpu.opts(tol=1e-9, relative=True, precision=fgpu.FGPU_MIXED, inner_tol=0.01)
+    o = fgpu.opts(tol=1e-9, relative=True, precision=fgpu.FGPU_MIXED, inner_tol=0.01, use_graph=False)
 
     def coupled_step():
         cur.reset()

@@ -21,7 +21,14 @@ constexpr int kAggMin = kBlock;
 constexpr int kAggPerBlock = kDofIter;
 constexpr int kPatchCells = 256;  // largest patch: cells per element workgroup (one per thread)
 constexpr int kPatchNodes = 640;  // largest number of distinct nodes a patch may touch
-constexpr int kPatchPerBlock = 4; // default patches processed in turn by one element workgroup
+constexpr int kPatchPerBlock = 1; // default patches processed in turn by one element workgroup
+// default patch size: kPatchCells, halved (down to 64) while the element launch would have
+// fewer than about a thousand workgroups, so that small meshes still spread over the chip
+inline int default_patch_cells(int n_cells) {
+    int c = kPatchCells;
+    while (c > 64 && (n_cells + c - 1) / c < 1024) c /= 2;
+    return c;
+}
 
 // Cells are processed in patches of consecutive (space-filling-curve ordered) cells.  A patch
 // stages the coordinates and values of its distinct nodes in LDS, applies its cells, sums the

@@ -281,7 +281,7 @@ void build_internal_order(Mesh &m, std::vector<int> &icells, std::vector<double>
     X.assign(4 * (size_t)nd, 0.0);
     for (int i = 0; i < nd; ++i)
         for (int a = 0; a < 3; ++a) X[4 * (size_t)i + a] = m.x[3 * (size_t)m.dof_of_idof[i] + a];
-    build_patches(nc, nd, icells, kPatchCells, m.patches);
+    build_patches(nc, nd, icells, default_patch_cells(nc), m.patches);
     build_slots(nd, m.patches, gptr);
     spos.clear();
     // dof -> boundary-face slots (4 per face, API face order)
