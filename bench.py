@@ -11,7 +11,7 @@ One step = one served solve; `value` is solves/s over the K timed requests, from
 timed arrival to the last timed result; latency runs from arrival to result.
 
 * headline: mixed precision (FP32 operator and CG inside FP64 defect correction), two-level
-  preconditioner (Jacobi + a coarse correction on 1024-dof blocks, built once per resident
+  preconditioner (Jacobi + a coarse correction on blocks of 256-1024 dofs by mesh size, built once per resident
   mesh), continuous batching, meshes and last solutions resident on the GPU (warm starts);
 * `plain_fp64`: the same request stream through the plain form: FP64 CG, one solve at a time,
   from phi = 0, in arrival order.
@@ -437,7 +437,7 @@ def main():
                                   f"Poisson arrivals at {RATE:.0f} req/s total"),
                    "clients": n, "total_dofs": int(sum(dofs)), "total_hexes": int(sum(hexes)),
                    "tol_rel": 1e-9,
-                   "precond": "jacobi" if args.jacobi else "two-level: jacobi + 1024-dof block aggregates",
+                   "precond": "jacobi" if args.jacobi else "two-level: jacobi + aggregates of 256-1024 consecutive dofs by mesh size",
                    "precond_plain_fp64": "jacobi", "parallelism": "single GPU"},
         "latency_ms": {"p50": round(mixed["p50_ms"], 2), "p99": round(mixed["p99_ms"], 2)},
         "mixed_batched": mixed,
