@@ -98,6 +98,7 @@ fgpu::Mesh::~Mesh() {
     dfree(d_plidx);
     dfree(d_plptr);
     dfree(d_plent);
+    dfree(d_layout);
     dfree(d_bf_icell);
     dfree(d_bf_face);
     dfree(d_bf_id);
@@ -225,8 +226,12 @@ static void upload_patches(fgpu_mesh *m, const std::vector<int> &gptr) {
     h2d(m->d_pnodes, P.nodes.data(), P.nodes.size(), s);
     m->d_pslot = dalloc<int>(P.slot.size());
     h2d(m->d_pslot, P.slot.data(), P.slot.size(), s);
-    m->d_plidx = dalloc<uint16_t>(P.lidx.size());
-    h2d(m->d_plidx, P.lidx.data(), P.lidx.size(), s);
+    // bit 15 of a cell's first local index carries its sub-hex layout (local indices < 2^15)
+    std::vector<uint16_t> lidx(P.lidx);
+    for (size_t c = 0; c < m->layout.size() && 8 * c < lidx.size(); ++c)
+        if (m->layout[c]) lidx[8 * c] |= 0x8000;
+    m->d_plidx = dalloc<uint16_t>(lidx.size());
+    h2d(m->d_plidx, lidx.data(), lidx.size(), s);
     m->d_plptr = dalloc<uint16_t>(P.lptr.size());
     h2d(m->d_plptr, P.lptr.data(), P.lptr.size(), s);
     m->d_plent = dalloc<uint16_t>(P.lent.size());
@@ -253,7 +258,10 @@ extern "C" int fgpu_mesh_import(fgpu_ctx *ctx, int nv, const double *xyz, int nc
         std::vector<int> icells, gptr, spos, fgptr, fgidx, bf_icell;
         std::vector<double> X;
         build_internal_order(*m, icells, X, gptr, spos, fgptr, fgidx, bf_icell);
+        cell_layouts(m->n_cells, icells, X, m->layout);
         hipStream_t s = ctx->stream;
+        m->d_layout = dalloc<unsigned char>(m->layout.size());
+        h2d(m->d_layout, m->layout.data(), m->layout.size(), s);
         m->d_cells = dalloc<int>(icells.size());
         h2d(m->d_cells, icells.data(), icells.size(), s);
         HIPCHECK(hipStreamSynchronize(s));
@@ -676,7 +684,7 @@ static void ensure_metric(fgpu_mesh *m) {
     if (!m->d_metric) m->d_metric = dalloc<float>(6 * (size_t)std::max(1, m->n_cells));
     const int nb = metric_blocks(m->n_cells);
     double *dev = dalloc<double>(nb);
-    launch_metric(st, m->n_cells, m->d_cells, m->d_X64, m->d_metric, dev);
+    launch_metric(st, m->n_cells, m->d_cells, m->d_layout, m->d_X64, m->d_metric, dev);
     std::vector<double> h(nb);
     d2h(h.data(), dev, nb, st);
     HIPCHECK(hipStreamSynchronize(st));

@@ -24,6 +24,8 @@ constexpr int kPatchNodes = 640;  // largest number of distinct nodes a patch ma
 constexpr int kPatchPerBlock = 1; // default patches processed in turn by one element workgroup
 // default patch size: kPatchCells, halved (down to 64) while the element launch would have
 // fewer than about a thousand workgroups, so that small meshes still spread over the chip
+// the eta flip of a lexicographic hex (an involution): vertex j <-> kEtaFlip[j]
+constexpr int kEtaFlip[8] = {2, 3, 0, 1, 6, 7, 4, 5};
 inline int default_patch_cells(int n_cells) {
     int c = kPatchCells;
     while (c > 64 && (n_cells + c - 1) / c < 1024) c /= 2;
@@ -148,6 +150,10 @@ struct Mesh {
         bool have_prev = false;
     } motion;
     std::vector<int> icells;               // 8*n_cells, internal order and dofs (host copy)
+    // per internal cell: 1 when its lexicographic vertices hold the tet-split sub-hex with the
+    // eta flip (Tethex's orientation fix swaps its front and back faces), 0 otherwise
+    std::vector<unsigned char> layout;
+    unsigned char *d_layout = nullptr;
     // device arrays (internal numbering)
     int *d_cells = nullptr;                // 8*n_cells
     double *d_X64 = nullptr;               // 4*n_dofs
@@ -254,4 +260,5 @@ void build_internal_order(Mesh &m, std::vector<int> &icells, std::vector<double>
 void build_patches(int nc, int nd, const std::vector<int> &icells, int cells_per_patch, Patches &P);
 // dof-major slot layout of the patch outputs: gptr (n_dofs+1) and Patches::slot
 void build_slots(int nd, Patches &P, std::vector<int> &gptr);
+void cell_layouts(int nc, const std::vector<int> &icells, const std::vector<double> &X, std::vector<unsigned char> &layout);
 }  // namespace fgpu

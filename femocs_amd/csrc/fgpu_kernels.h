@@ -39,7 +39,8 @@ void launch_minmax(hipStream_t s, int n, const double *x, double *out);
 int center_det_blocks(int n);
 hipError_t upload_affine_table(hipStream_t s);
 int metric_blocks(int nc);
-void launch_metric(hipStream_t s, int nc, const int *cells, const double *X, float *metric, double *devmax);
+void launch_metric(hipStream_t s, int nc, const int *cells, const unsigned char *layout, const double *X, float *metric,
+                   double *devmax);
 void launch_heat_terms(hipStream_t s, int nc, const int *cells, const double *X, const double *T, const double *phic,
                        double gamma, double lorentz, double theta, double *kq, double *gq, float *kq32, float *gq32,
                        double *f, double *kx);

@@ -437,14 +437,18 @@ __device__ __forceinline__ void q1_cell_affine(const float M[6], const float u[8
 
 // per cell: the metric of the tet-split form and, per block, the largest deviation of the
 // nodes from that form relative to the frame size
-__global__ __launch_bounds__(kBlock) void k_metric(int nc, const int *__restrict__ cells, const double4 *__restrict__ X,
+__global__ __launch_bounds__(kBlock) void k_metric(int nc, const int *__restrict__ cells,
+                                                   const unsigned char *__restrict__ layout, const double4 *__restrict__ X,
                                                    float *__restrict__ metric, double *__restrict__ devmax) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     double dev = 0;
     if (c < nc) {
+        // an eta-flipped cell is read in the flipped order: the sub-hex form then holds with
+        // the opposite orientation of the frame
+        const bool flip = layout[c] != 0;
         double x[8][3];
         for (int n = 0; n < 8; ++n) {
-            const double4 p = X[cells[8 * (size_t)c + n]];
+            const double4 p = X[cells[8 * (size_t)c + (flip ? kEtaFlip[n] : n)]];
             x[n][0] = p.x;
             x[n][1] = p.y;
             x[n][2] = p.z;
@@ -475,9 +479,10 @@ __global__ __launch_bounds__(kBlock) void k_metric(int nc, const int *__restrict
         const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
         for (int k = 0; k < 6; ++k) {
             const double *a = r[ij[k][0]], *b = r[ij[k][1]];
-            metric[6 * (size_t)c + k] = (float)(-(a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / det);
+            metric[6 * (size_t)c + k] = (float)((a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / fabs(det));
         }
-        if (!(det < 0)) dev = 1e300;     // the reference orientation needs det(A) < 0
+        // a positively oriented cell has det(A) < 0 as laid out, det(A) > 0 when eta-flipped
+        if (!(flip ? det > 0 : det < 0)) dev = 1e300;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) dev = fmax(dev, __shfl_xor(dev, o, 64));
@@ -492,9 +497,10 @@ __global__ __launch_bounds__(kBlock) void k_metric(int nc, const int *__restrict
 }
 
 int metric_blocks(int nc) { return (nc + kBlock - 1) / kBlock; }
-void launch_metric(hipStream_t s, int nc, const int *cells, const double *X, float *metric, double *devmax) {
+void launch_metric(hipStream_t s, int nc, const int *cells, const unsigned char *layout, const double *X, float *metric,
+                   double *devmax) {
     if (nc > 0)
-        hipLaunchKernelGGL(k_metric, dim3(metric_blocks(nc)), dim3(kBlock), 0, s, nc, cells,
+        hipLaunchKernelGGL(k_metric, dim3(metric_blocks(nc)), dim3(kBlock), 0, s, nc, cells, layout,
                            reinterpret_cast<const double4 *>(X), metric, devmax);
 }
 
@@ -599,21 +605,26 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
         // phase 2: one cell per thread, from LDS
         if constexpr (sizeof(T) == 4 && !COEF && MODE == EM_APPLY) {
             if (affine && tid < nc) {
-                const int lid[8] = {(int)(packed.x & 0xffff), (int)(packed.x >> 16), (int)(packed.y & 0xffff),
-                                    (int)(packed.y >> 16), (int)(packed.z & 0xffff), (int)(packed.z >> 16),
-                                    (int)(packed.w & 0xffff), (int)(packed.w >> 16)};
+                // an eta-flipped cell (bit 15 of its first index) is computed in flipped order
+                const bool flip = (packed.x & 0x8000u) != 0;
+                const int l0[8] = {(int)(packed.x & 0x7fff), (int)(packed.x >> 16), (int)(packed.y & 0xffff),
+                                   (int)(packed.y >> 16), (int)(packed.z & 0xffff), (int)(packed.z >> 16),
+                                   (int)(packed.w & 0xffff), (int)(packed.w >> 16)};
+                int lid[8];
+#pragma unroll
+                for (int n = 0; n < 8; ++n) lid[n] = flip ? l0[kEtaFlip[n]] : l0[n];
                 const float M[6] = {mq[0].x, mq[0].y, mq[1].x, mq[1].y, mq[2].x, mq[2].y};
                 float u[8], y[8];
 #pragma unroll
                 for (int n = 0; n < 8; ++n) u[n] = su[lid[n]];
                 q1_cell_affine(M, u, y);
 #pragma unroll
-                for (int n = 0; n < 8; ++n) sout[8 * tid + n] = y[n];
+                for (int n = 0; n < 8; ++n) sout[8 * tid + n] = flip ? y[kEtaFlip[n]] : y[n];
             }
         }
         if (!AFF && tid < nc && !affine) {
             const int c = c0 + tid;
-            const int lid[8] = {(int)(packed.x & 0xffff), (int)(packed.x >> 16), (int)(packed.y & 0xffff), (int)(packed.y >> 16),
+            const int lid[8] = {(int)(packed.x & 0x7fff), (int)(packed.x >> 16), (int)(packed.y & 0xffff), (int)(packed.y >> 16),
                                 (int)(packed.z & 0xffff), (int)(packed.z >> 16), (int)(packed.w & 0xffff), (int)(packed.w >> 16)};
             T Xe[8][3];
 #pragma unroll

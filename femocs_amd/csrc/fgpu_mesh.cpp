@@ -230,6 +230,38 @@ void build_slots(int nd, Patches &P, std::vector<int> &gptr) {
     for (size_t j = 0; j < P.nodes.size(); ++j) P.slot[j] = pos[P.nodes[j]]++;
 }
 
+// Layout of each internal cell against the tet-split sub-hex (nodes of a tet vertex, three edge
+// midpoints, three face centroids and the centroid, lexicographic): 0 as laid out, 1 when the
+// eta flip of the cell's vertices has that form more closely
+void cell_layouts(int nc, const std::vector<int> &icells, const std::vector<double> &X, std::vector<unsigned char> &layout) {
+    static const double kSub[8][3] = {{0, 0, 0}, {0.5, 0, 0}, {0, 0, 0.5}, {1.0 / 3, 0, 1.0 / 3},
+                                      {0, 0.5, 0}, {1.0 / 3, 1.0 / 3, 0}, {0, 1.0 / 3, 1.0 / 3}, {0.25, 0.25, 0.25}};
+    static const int kId[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+    auto dev = [&](const int *l, const int *perm) {
+        double x[8][3];
+        for (int n = 0; n < 8; ++n)
+            for (int a = 0; a < 3; ++a) x[n][a] = X[4 * (size_t)l[perm[n]] + a];
+        double u[3][3];
+        for (int a = 0; a < 3; ++a) {
+            u[0][a] = 2 * (x[1][a] - x[0][a]);
+            u[1][a] = 2 * (x[4][a] - x[0][a]);
+            u[2][a] = 2 * (x[2][a] - x[0][a]);
+        }
+        double size = 0, d = 0;
+        for (int i = 0; i < 3; ++i) size += std::sqrt(u[i][0] * u[i][0] + u[i][1] * u[i][1] + u[i][2] * u[i][2]);
+        for (int n = 0; n < 8; ++n)
+            for (int a = 0; a < 3; ++a)
+                d = std::max(d, std::fabs(x[0][a] + kSub[n][0] * u[0][a] + kSub[n][1] * u[1][a] + kSub[n][2] * u[2][a] -
+                                          x[n][a]));
+        return size > 0 ? d / size : 1e300;
+    };
+    layout.assign(nc, 0);
+    for (int c = 0; c < nc; ++c) {
+        const int *l = &icells[8 * (size_t)c];
+        layout[c] = dev(l, kEtaFlip) < dev(l, kId) ? 1 : 0;
+    }
+}
+
 void build_internal_order(Mesh &m, std::vector<int> &icells, std::vector<double> &X, std::vector<int> &gptr,
                           std::vector<int> &spos, std::vector<int> &fgptr, std::vector<int> &fgidx,
                           std::vector<int> &bf_icell) {
