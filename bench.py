@@ -235,6 +235,12 @@ def paths_bench(no_cpu: bool):
                           "mixed_iterations": stmx.iterations}
     log("solve", out["solve_r10nm"])
     ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+    # the reference's tet -> 4-hex split with its numbering, on the device (host arrays in/out)
+    tris, _ = tip.surface_triangles()
+    tsp, sp_out = _best_of(lambda: fgpu.tethex_split(ctx, tip.nodes[:tip.n_tetnodes], tip.tets, tris))
+    out["tet_split"] = {"tets": len(tip.tets), "boundary_triangles": len(tris), "ms": round(tsp * 1e3, 2),
+                        "nodes": len(sp_out[0])}
+    log("tet split", out["tet_split"])
     te, _ = _best_of(lambda: ip.extract_solution(s, smoothen=True))
     atoms = tip.atoms
     tr = {"atoms": len(atoms), "extract_ms": round(te * 1e3, 2)}
@@ -318,7 +324,11 @@ def paths_bench(no_cpu: bool):
         t = time.perf_counter()
         O.update_cells(L, d2f, f2d, moved[:k], pc[:k])
         cpu["cell_update_per_s"] = round(k / (time.perf_counter() - t))
-        cpu["sample"] = f"{len(sub)} atoms (every {max(1, len(atoms) // 2000)}th), {k} particles"
+        t = time.perf_counter()
+        M.tet_to_hex(tip.nodes[:tip.n_tetnodes], tip.tets)
+        cpu["tet_split_numpy_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+        cpu["sample"] = (f"{len(sub)} atoms (every {max(1, len(atoms) // 2000)}th), {k} particles; the tet split "
+                         "by the generator's numpy version (its own numbering)")
         out["cpu_baseline"] = cpu
         log("cpu", cpu)
     print(json.dumps(out), flush=True)

@@ -87,6 +87,10 @@ API = [
     ("fgpu_interp_get_nodal", C.c_int, [_P, _D]),
     ("fgpu_interp_hex_gradient", C.c_int, [_P, C.c_int, _D, _I, _D]),
     ("fgpu_interp_update_cells", C.c_int, [_P, C.c_int, _D, _I]),
+    ("fgpu_tethex_split", C.c_int, [_P, C.c_int, _D, C.c_int, _I, C.c_int, C.c_void_p, C.POINTER(_P)]),
+    ("fgpu_tethex_sizes", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("fgpu_tethex_export", C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("fgpu_tethex_destroy", None, [_P]),
     ("fgpu_interp_set_nodal", C.c_int, [_P, _D]),
     ("fgpu_interp_points", C.c_int, [_P, C.c_int, _D, C.c_int, _I, _D]),
     ("fgpu_interp_cells", C.c_int, [_P, C.c_int, _D, C.c_int, _I, _D]),
@@ -470,6 +474,32 @@ class Batch:
         for i in range(n.value):
             res.append((self.members.pop(out[i]), st[i]))
         return res
+
+
+def tethex_split(ctx: "Context", nodes, tets, tris=None):
+    """The reference's tet -> 4-hex split on the device (Tethex Mesh::convert_3D): (all nodes
+    (n, 3), node kind (1 vertex, 2 edge, 3 face, 4 tet), hexes (4T, 8) in femocs/UCD order,
+    quadrangles (3F, 4) of the boundary triangles or None, number of edges, number of faces)."""
+    x = np.ascontiguousarray(nodes, np.float64).reshape(-1)
+    t = np.ascontiguousarray(tets, np.int32).reshape(-1)
+    tr = None if tris is None else np.ascontiguousarray(tris, np.int32).reshape(-1)
+    h = _P()
+    ctx.check(ctx.lib.fgpu_tethex_split(ctx.h, x.size // 3, x, t.size // 4, t, 0 if tr is None else tr.size // 3,
+                                        None if tr is None else tr.ctypes.data, C.byref(h)), "fgpu_tethex_split")
+    try:
+        nn, ne, nf = C.c_int(), C.c_int(), C.c_int()
+        ctx.check(ctx.lib.fgpu_tethex_sizes(h, C.byref(nn), C.byref(ne), C.byref(nf)), "fgpu_tethex_sizes")
+        xyz = np.zeros((nn.value, 3))
+        hexes = np.zeros((t.size, 8), np.int32)
+        quads = None if tr is None else np.zeros((tr.size, 4), np.int32)
+        ctx.check(ctx.lib.fgpu_tethex_export(h, xyz.ctypes.data, hexes.ctypes.data,
+                                             None if quads is None else quads.ctypes.data), "fgpu_tethex_export")
+    finally:
+        ctx.lib.fgpu_tethex_destroy(h)
+    nv = x.size // 3
+    kind = np.concatenate([np.full(nv, 1), np.full(ne.value, 2), np.full(nf.value, 3),
+                           np.full(t.size // 4, 4)]).astype(np.int8)
+    return xyz, kind, hexes, quads, ne.value, nf.value
 
 
 def opts(max_it=10000, tol=1e-9, relative=True, precond=FGPU_PC_JACOBI, precision=FGPU_FP64, inner_tol=1e-4,

@@ -42,6 +42,7 @@ typedef struct fgpu_ctx fgpu_ctx;
 typedef struct fgpu_mesh fgpu_mesh;
 typedef struct fgpu_sys fgpu_sys;
 typedef struct fgpu_interp fgpu_interp;
+typedef struct fgpu_tethex fgpu_tethex;
 
 typedef struct {
     int max_it;        /* max CG iterations (inner + outer for FGPU_MIXED)                    */
@@ -323,6 +324,23 @@ int fgpu_sys_solution_grad(fgpu_sys *sys, double *grad);
  * element apply, 1: the dof gather, in the given precision), timed with HIP events on the
  * context stream. */
 int fgpu_sys_time_kernel(fgpu_sys *sys, int which, int precision, int reps, double *avg_ms);
+
+/* ---- the reference's tet -> 4-hex split (Tethex Mesh::convert_3D, src/Tethex.cpp:965-1169) ----
+ * On the device, with the reference's numbering: nodes = the n_vertices vertices, then edge
+ * midpoints (edges ordered by (larger, smaller) vertex, IncidenceMatrix :374-425), face
+ * centroids (faces in order of first encounter over tet, local edge, opposite vertex,
+ * Mesh::face_numeration :1272-1340), tet centroids (Mesh::set_new_vertices :897-919); hex
+ * 4t + k around tet vertex k, femocs (UCD) vertex order, front and back faces swapped where
+ * the deal.II-ordered cell would be negative (Mesh::convert_tetrahedra :1038-1100); three
+ * quadrangles per boundary triangle (Mesh::convert_triangles :1104-1169).  tets: 4 per tet,
+ * tris: 3 per boundary triangle (faces of the tets; may be 0).  At most 2097151 vertices.
+ * fgpu_tethex_export writes 3 * n_nodes coordinates, 32 * n_tets hex vertices and
+ * 12 * n_tris quadrangle vertices (any pointer may be NULL). */
+int fgpu_tethex_split(fgpu_ctx *ctx, int n_vertices, const double *xyz, int n_tets, const int *tets, int n_tris,
+                      const int *tris, fgpu_tethex **out);
+int fgpu_tethex_sizes(const fgpu_tethex *split, int *n_nodes, int *n_edges, int *n_faces);
+int fgpu_tethex_export(const fgpu_tethex *split, double *xyz, int *hexes, int *quads);
+void fgpu_tethex_destroy(fgpu_tethex *split);
 
 #ifdef __cplusplus
 }
