@@ -4,8 +4,10 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <utility>
 
 #include "fgpu_internal.h"
 
@@ -73,33 +75,54 @@ int build_topology(Mesh &m, int nv, const double *xyz, int nc, const int *ucd, i
             std::swap(l[5], l[7]);
         }
     }
-    // boundary faces = faces of exactly one cell
+    // boundary faces = faces of exactly one cell.  Faces are bucketed by their smallest vertex
+    // (a counting sort) and matched within each small bucket on their other three vertices
     struct FaceKey {
-        std::array<int, 4> v;
-        int slot;
+        int v1, v2, v3;
+        int64_t slot;
     };
     std::vector<FaceKey> keys((size_t)6 * nc);
+    std::vector<int64_t> bstart((size_t)nd + 1, 0);
     for (int c = 0; c < nc; ++c)
         for (int f = 0; f < 6; ++f) {
-            FaceKey &k = keys[6 * (size_t)c + f];
-            for (int i = 0; i < 4; ++i) k.v[i] = m.cells_lex[8 * (size_t)c + kFaceVerts[f][i]];
-            std::sort(k.v.begin(), k.v.end());
-            k.slot = 6 * c + f;
+            int v[4];
+            for (int i = 0; i < 4; ++i) v[i] = m.cells_lex[8 * (size_t)c + kFaceVerts[f][i]];
+            std::sort(v, v + 4);
+            keys[6 * (size_t)c + f] = {v[1], v[2], v[3], v[0]};   // slot holds v0 until bucketed
+            bstart[v[0] + 1]++;
         }
-    std::sort(keys.begin(), keys.end(), [](const FaceKey &a, const FaceKey &b) {
-        return a.v != b.v ? a.v < b.v : a.slot < b.slot;
-    });
-    std::vector<unsigned char> is_bd((size_t)6 * nc, 0);
-    for (size_t i = 0; i < keys.size();) {
-        size_t j = i + 1;
-        while (j < keys.size() && keys[j].v == keys[i].v) ++j;
-        if (j - i == 1) is_bd[keys[i].slot] = 1;
-        else if (j - i > 2) {
-            err = "mesh import: a face is shared by more than two cells";
-            return FGPU_EINVAL;
+    for (int d = 0; d < nd; ++d) bstart[d + 1] += bstart[d];
+    std::vector<FaceKey> bucket(keys.size());
+    {
+        std::vector<int64_t> fill(bstart.begin(), bstart.end() - 1);
+        for (size_t i = 0; i < keys.size(); ++i) {
+            const int64_t v0 = keys[i].slot;
+            bucket[fill[v0]++] = {keys[i].v1, keys[i].v2, keys[i].v3, (int64_t)i};
         }
-        i = j;
     }
+    std::vector<unsigned char> is_bd((size_t)6 * nc, 0);
+    auto less = [](const FaceKey &a, const FaceKey &b) {
+        if (a.v1 != b.v1) return a.v1 < b.v1;
+        if (a.v2 != b.v2) return a.v2 < b.v2;
+        if (a.v3 != b.v3) return a.v3 < b.v3;
+        return a.slot < b.slot;
+    };
+    for (int d = 0; d < nd; ++d) {
+        FaceKey *b0 = bucket.data() + bstart[d], *b1 = bucket.data() + bstart[d + 1];
+        std::sort(b0, b1, less);
+        for (FaceKey *i = b0; i < b1;) {
+            FaceKey *j = i + 1;
+            while (j < b1 && j->v1 == i->v1 && j->v2 == i->v2 && j->v3 == i->v3) ++j;
+            if (j - i == 1) is_bd[i->slot] = 1;
+            else if (j - i > 2) {
+                err = "mesh import: a face is shared by more than two cells";
+                return FGPU_EINVAL;
+            }
+            i = j;
+        }
+    }
+    bucket.clear();
+    bucket.shrink_to_fit();
     keys.clear();
     keys.shrink_to_fit();
     std::vector<double> ctr;
@@ -288,10 +311,12 @@ void build_internal_order(Mesh &m, std::vector<int> &icells, std::vector<double>
         }
         key[c] = k;
     }
+    // (key, cell) pairs sorted: equal keys keep the cell order
+    std::vector<std::pair<uint64_t, int>> kc(nc);
+    for (int c = 0; c < nc; ++c) kc[c] = {key[c], c};
+    std::sort(kc.begin(), kc.end());
     m.cell_of_icell.resize(nc);
-    std::iota(m.cell_of_icell.begin(), m.cell_of_icell.end(), 0);
-    std::stable_sort(m.cell_of_icell.begin(), m.cell_of_icell.end(),
-                     [&](int a, int b) { return key[a] < key[b]; });
+    for (int i = 0; i < nc; ++i) m.cell_of_icell[i] = kc[i].second;
     m.icell_of_cell.resize(nc);
     for (int i = 0; i < nc; ++i) m.icell_of_cell[m.cell_of_icell[i]] = i;
     // dofs by first touch along the sorted cells
