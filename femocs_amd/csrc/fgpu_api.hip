@@ -1774,6 +1774,17 @@ struct fgpu_batch {
     int graph_k = 0;
     std::vector<double> done;     // S_DONE of each member after the last step (valid while
     bool done_valid = false;      // the membership has not changed since)
+    // an asynchronous step in flight: its members and their S_DONE, copied to pinned memory
+    // when the step ends (event); members that finished are known once it is waited for
+    bool pending = false;
+    std::vector<fgpu_sys *> pend_sys;
+    double *h_pend = nullptr;
+    int pend_cap = 0;
+    hipEvent_t pend_ev = nullptr;
+    std::vector<std::pair<fgpu_sys *, double>> last;   // (member, S_DONE) of the last waited step
+    bool last_valid = false;
+    std::vector<fgpu_sys *> joined;   // added while a step was in flight (not its members, even
+                                      // at the address of one that left)
 };
 
 static void batch_drop_graph(fgpu_batch *b) {
@@ -1803,6 +1814,8 @@ extern "C" void fgpu_batch_destroy(fgpu_batch *b) {
     for (auto *s : b->sys) s->batches--;
     if (b->dtab) (void)hipFree(b->dtab);
     if (b->dflags) (void)hipFree(b->dflags);
+    if (b->h_pend) (void)hipHostFree(b->h_pend);
+    if (b->pend_ev) (void)hipEventDestroy(b->pend_ev);
     delete b;
     ctx_release(c);
 }
@@ -1823,6 +1836,9 @@ extern "C" int fgpu_batch_add(fgpu_batch *b, fgpu_sys *s) {
         make_sets({s}, sets);
         BatchTabs t = upload_batch(c, sets, {s});
         start_solves(c, {s}, &b->o, t, coef);
+        for (auto &e : b->last)
+            if (e.first == s) e.first = nullptr;
+        if (b->pending) b->joined.push_back(s);
         b->coef = coef;
         b->sys.push_back(s);
         s->batches++;
@@ -1832,14 +1848,11 @@ extern "C" int fgpu_batch_add(fgpu_batch *b, fgpu_sys *s) {
     });
 }
 
-extern "C" int fgpu_batch_step(fgpu_batch *b, int iterations, int *n_running) {
-    if (!b || iterations <= 0) return FGPU_EINVAL;
+// enqueues one step of every member and the collection of their S_DONE into b->dflags
+static void batch_launch(fgpu_batch *b, int iterations) {
     fgpu_ctx *c = b->ctx;
-    return guarded(c, [&] {
-        HIPCHECK(hipSetDevice(c->device));
-        const int n = (int)b->sys.size();
-        if (n_running) *n_running = 0;
-        if (n == 0) return FGPU_OK;
+    const int n = (int)b->sys.size();
+    {
         const bool mixed = b->o.precision == FGPU_MIXED;
         const int k = mixed ? (iterations + 7) / 8 * 8 : iterations;
         if (b->dirty) {
@@ -1869,8 +1882,23 @@ extern "C" int fgpu_batch_step(fgpu_batch *b, int iterations, int *n_running) {
         } else {
             chunk(c->stream, a);
         }
-        double *h = flags_buffer(c, n);
         launch_collect(c->stream, b->t.sets[1], n, S_DONE, b->dflags);
+    }
+}
+
+static void batch_wait(fgpu_batch *b);
+
+extern "C" int fgpu_batch_step(fgpu_batch *b, int iterations, int *n_running) {
+    if (!b || iterations <= 0) return FGPU_EINVAL;
+    fgpu_ctx *c = b->ctx;
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        batch_wait(b);
+        const int n = (int)b->sys.size();
+        if (n_running) *n_running = 0;
+        if (n == 0) return FGPU_OK;
+        batch_launch(b, iterations);
+        double *h = flags_buffer(c, n);
         HIPCHECK(hipMemcpyAsync(h, b->dflags, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
         int running = 0;
@@ -1878,6 +1906,63 @@ extern "C" int fgpu_batch_step(fgpu_batch *b, int iterations, int *n_running) {
         if (n_running) *n_running = running;
         b->done.assign(h, h + n);
         b->done_valid = true;
+        b->last_valid = false;
+        return FGPU_OK;
+    });
+}
+
+// waits for the asynchronous step in flight, if any, and keeps its members' S_DONE
+static void batch_wait(fgpu_batch *b) {
+    if (!b->pending) return;
+    HIPCHECK(hipEventSynchronize(b->pend_ev));
+    b->pending = false;
+    b->last.clear();
+    for (size_t i = 0; i < b->pend_sys.size(); ++i)
+        if (std::find(b->joined.begin(), b->joined.end(), b->pend_sys[i]) == b->joined.end())
+            b->last.emplace_back(b->pend_sys[i], b->h_pend[i]);
+    b->joined.clear();
+    b->last_valid = true;
+}
+
+extern "C" int fgpu_batch_step_async(fgpu_batch *b, int iterations) {
+    if (!b || iterations <= 0) return FGPU_EINVAL;
+    fgpu_ctx *c = b->ctx;
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        if (b->pending) throw std::invalid_argument("a step is in flight: fgpu_batch_wait first");
+        const int n = (int)b->sys.size();
+        if (n == 0) return FGPU_OK;
+        if (n > b->pend_cap) {
+            if (b->h_pend) HIPCHECK(hipHostFree(b->h_pend));
+            b->h_pend = nullptr;
+            b->pend_cap = 0;
+            HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&b->h_pend), 2 * n * sizeof(double), hipHostMallocDefault));
+            b->pend_cap = 2 * n;
+        }
+        if (!b->pend_ev) HIPCHECK(hipEventCreateWithFlags(&b->pend_ev, hipEventDisableTiming));
+        batch_launch(b, iterations);
+        HIPCHECK(hipMemcpyAsync(b->h_pend, b->dflags, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipEventRecord(b->pend_ev, c->stream));
+        b->pend_sys = b->sys;
+        b->joined.clear();
+        b->pending = true;
+        b->done_valid = false;
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_batch_wait(fgpu_batch *b, int *n_running) {
+    if (!b) return FGPU_EINVAL;
+    fgpu_ctx *c = b->ctx;
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        const bool had = b->pending;
+        batch_wait(b);
+        if (n_running) {
+            *n_running = 0;
+            if (had)
+                for (auto &e : b->last) *n_running += e.second == 0.0;
+        }
         return FGPU_OK;
     });
 }
@@ -1909,11 +1994,20 @@ extern "C" int fgpu_batch_retire(fgpu_batch *b, int max, fgpu_sys **out, fgpu_so
         for (size_t i = 0; i < b->sys.size(); ++i) {
             fgpu_sys *s = b->sys[i];
             double done = 0;
-            if (known) done = b->done[i];
-            else HIPCHECK(hipMemcpy(&done, s->scal + S_DONE, sizeof(double), hipMemcpyDeviceToHost));
+            if (known) {
+                done = b->done[i];
+            } else if (b->last_valid) {
+                // the last waited asynchronous step; a member that joined after it is running
+                for (auto &e : b->last)
+                    if (e.first == s) done = e.second;
+            } else {
+                HIPCHECK(hipMemcpy(&done, s->scal + S_DONE, sizeof(double), hipMemcpyDeviceToHost));
+            }
             if (done != 0.0 && *n_out < max) {
                 out[*n_out] = s;
                 s->batches--;
+                for (auto &e : b->last)
+                    if (e.first == s) e.first = nullptr;   // a later add of s starts afresh
                 if (stats) read_stats(s, mixed, &stats[*n_out]);
                 ++*n_out;
             } else {

@@ -126,6 +126,8 @@ API = [
     ("fgpu_batch_add", C.c_int, [_P, _P]),
     ("fgpu_batch_step", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
     ("fgpu_batch_retire", C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(SolveStats), C.POINTER(C.c_int)]),
+    ("fgpu_batch_step_async", C.c_int, [_P, C.c_int]),
+    ("fgpu_batch_wait", C.c_int, [_P, C.POINTER(C.c_int)]),
 ]
 
 
@@ -455,6 +457,16 @@ class Batch:
     def step(self, iterations: int = 32) -> int:
         n = C.c_int()
         self.ctx.check(self.ctx.lib.fgpu_batch_step(self.h, int(iterations), C.byref(n)), "fgpu_batch_step")
+        return n.value
+
+    def step_async(self, iterations: int = 32):
+        """Enqueues one step of every member and returns at once."""
+        self.ctx.check(self.ctx.lib.fgpu_batch_step_async(self.h, int(iterations)), "fgpu_batch_step_async")
+
+    def wait(self) -> int:
+        """Waits for the step in flight; returns how many of its members still run."""
+        n = C.c_int()
+        self.ctx.check(self.ctx.lib.fgpu_batch_wait(self.h, C.byref(n)), "fgpu_batch_wait")
         return n.value
 
     def time_kernel(self, which=0, reps=50) -> float:

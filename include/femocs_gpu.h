@@ -193,6 +193,14 @@ int fgpu_batch_size(const fgpu_batch *batch);
 int fgpu_batch_add(fgpu_batch *batch, fgpu_sys *sys);
 int fgpu_batch_step(fgpu_batch *batch, int iterations, int *n_running);
 int fgpu_batch_retire(fgpu_batch *batch, int max, fgpu_sys **out, fgpu_solve_stats *stats, int *n_out);
+/* Pipelined steps (no reference counterpart): fgpu_batch_step_async enqueues one step of every
+ * member and returns at once; fgpu_batch_wait waits for it and gives the number still running.
+ * Between the two, the caller may retire the members a previous waited step finished and add
+ * new ones (they join the next step); fgpu_batch_retire then uses the finished flags of the
+ * last waited step.  Members that finished keep their results while later steps run (their
+ * kernels skip finished systems).  One step in flight at a time (FGPU_EINVAL otherwise). */
+int fgpu_batch_step_async(fgpu_batch *batch, int iterations);
+int fgpu_batch_wait(fgpu_batch *batch, int *n_running);
 /* average duration (ms) of one launch of the element apply (which 0), the dof gather
  * (which 1) or the coarse product of the two-level preconditioner (which 2, mixed batches
  * with a coarse level) over all members, in the batch's precision; for measurement, no

@@ -39,12 +39,15 @@ def _check(ctx, srv, o):
         assert np.array_equal(x, r.x_final), (r.client, r.seq)
 
 
-@pytest.mark.parametrize("precision", [fgpu.FGPU_FP64, fgpu.FGPU_MIXED])
-def test_serving_results_equal_alone(ctx, clients, precision):
+@pytest.mark.parametrize("precision,pipelined", [(fgpu.FGPU_FP64, True), (fgpu.FGPU_MIXED, True),
+                                                 (fgpu.FGPU_MIXED, False)])
+def test_serving_results_equal_alone(ctx, clients, precision, pipelined):
+    """Each served result equals the same solve alone, bit for bit, with steps pipelined (the
+    next step queued before the host retires and admits) or synchronous."""
     o = fgpu.opts(tol=1e-9, relative=True, precision=precision, inner_tol=0.1, use_graph=False)
     for c in clients:
         c.seq = 0
-    srv = serving.Server(ctx, clients, o, step_iterations=16, record_inputs=True)
+    srv = serving.Server(ctx, clients, o, step_iterations=16, record_inputs=True, pipelined=pipelined)
     serving.run_clients(srv, rate=400.0, duration=60.0, max_requests=9, seed=1)
     assert srv.idle()
     _check(ctx, srv, o)
