@@ -234,7 +234,24 @@ def paths_bench(no_cpu: bool):
                           "fp64_iterations": st64.iterations, "mixed_ms": round(tmx * 1e3, 2),
                           "mixed_iterations": stmx.iterations}
     log("solve", out["solve_r10nm"])
+    # mesh import (DealSolver::import_mesh + mark_boundary and the kernels' internal order):
+    # tables built on the device against the same tables built on the host (host arrays in)
+    vc = tip.vacuum_cells()
+
+    def imp(host):
+        m = fgpu.Mesh(ctx, tip.nodes, vc, 8, 2, 4, 2, host_import=host)
+        ctx.sync()
+        return m
+
+    tid, md = _best_of(lambda: imp(False))
+    tih, mh = _best_of(lambda: imp(True))
+    out["mesh_import"] = {"hexes": md.n_cells, "device_ms": round(tid * 1e3, 2), "host_ms": round(tih * 1e3, 2),
+                          "same_tables": md.digest() == mh.digest()}
+    del md, mh
+    log("mesh import", out["mesh_import"])
+    t = time.perf_counter()
     ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+    out["interpolator_create_ms"] = round((time.perf_counter() - t) * 1e3, 1)
     # the reference's tet -> 4-hex split with its numbering, on the device (host arrays in/out)
     tris, _ = tip.surface_triangles()
     tsp, sp_out = _best_of(lambda: fgpu.tethex_split(ctx, tip.nodes[:tip.n_tetnodes], tip.tets, tris))

@@ -17,10 +17,10 @@ LIB = os.path.join(HERE, "libfemocs_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["fgpu_mesh.cpp", "fgpu_kernels.hip", "fgpu_api.hip", "fgpu_interp_host.cpp", "fgpu_interp.hip",
-           "fgpu_tethex.hip"]
+           "fgpu_tethex.hip", "fgpu_import.hip"]
 # the transfer to atoms reproduces the reference's rounding: no floating-point contraction
 EXTRA = {"fgpu_interp_host.cpp": ["-ffp-contract=off"], "fgpu_interp.hip": ["-ffp-contract=off"],
-         "fgpu_tethex.hip": ["-ffp-contract=off"]}
+         "fgpu_tethex.hip": ["-ffp-contract=off"], "fgpu_import.hip": ["-ffp-contract=off"]}
 HEADERS = ["fgpu_internal.h", "fgpu_kernels.h", "fgpu_interp.h", "fgpu_cg.inc"]
 
 # -packed-fp32-ops: the compiler emits no packed-FP32 (v_pk_*_f32) instructions

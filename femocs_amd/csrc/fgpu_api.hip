@@ -238,17 +238,38 @@ static void upload_patches(fgpu_mesh *m, const std::vector<int> &gptr) {
     h2d(m->d_plent, P.lent.data(), P.lent.size(), s);
     HIPCHECK(hipStreamSynchronize(s));
     m->patch_gen++;
+    // the device holds the lists now (as after a device import): keep only the patch bounds
+    Patches &Q = m->patches;
+    std::vector<int>().swap(Q.nodes);
+    std::vector<int>().swap(Q.slot);
+    std::vector<uint16_t>().swap(Q.lidx);
+    std::vector<uint16_t>().swap(Q.lptr);
+    std::vector<uint16_t>().swap(Q.lent);
 }
 
 extern "C" int fgpu_mesh_import(fgpu_ctx *ctx, int nv, const double *xyz, int nc, const int *ucd, int top, int bottom,
                                 int sides, int other, fgpu_mesh **out) {
-    if (!ctx || !out) return FGPU_EINVAL;
+    return fgpu_mesh_import_ex(ctx, nv, xyz, nc, ucd, top, bottom, sides, other, 0, out);
+}
+
+extern "C" int fgpu_mesh_import_ex(fgpu_ctx *ctx, int nv, const double *xyz, int nc, const int *ucd, int top, int bottom,
+                                   int sides, int other, int flags, fgpu_mesh **out) {
+    if (!ctx || !out || (flags & ~FGPU_IMPORT_HOST)) return FGPU_EINVAL;
     *out = nullptr;
     fgpu_mesh *m = new (std::nothrow) fgpu_mesh();
     if (!m) return FGPU_ENOMEM;
     m->ctx = ctx;
+    static std::atomic<unsigned long long> next_uid{1};
+    // the device builds the tables of any non-empty mesh; the host path gives the same bits
+    const bool on_device = !(flags & FGPU_IMPORT_HOST) && nv > 0 && nc > 0 && xyz && ucd;
     const int rc = guarded(ctx, [&] {
         HIPCHECK(hipSetDevice(ctx->device));
+        if (on_device) {
+            const int r = device_import(*m, ctx->stream, nv, xyz, nc, ucd, top, bottom, sides, other, ctx->err);
+            if (r != FGPU_OK) return r;
+            m->uid = next_uid++;
+            return FGPU_OK;
+        }
         const int r = build_topology(*m, nv, xyz, nc, ucd, top, bottom, sides, other, ctx->err);
         if (r != FGPU_OK) return r;
         if (m->n_cells == 0 || m->n_dofs == 0) {
@@ -265,7 +286,6 @@ extern "C" int fgpu_mesh_import(fgpu_ctx *ctx, int nv, const double *xyz, int nc
         m->d_cells = dalloc<int>(icells.size());
         h2d(m->d_cells, icells.data(), icells.size(), s);
         HIPCHECK(hipStreamSynchronize(s));
-        static std::atomic<unsigned long long> next_uid{1};
         m->uid = next_uid++;
         m->d_X64 = dalloc<double>(X.size());
         m->d_X32 = dalloc<float>(X.size());
@@ -289,6 +309,8 @@ extern "C" int fgpu_mesh_import(fgpu_ctx *ctx, int nv, const double *xyz, int nc
             v2c[i] = m->icell_of_cell[m->v2c_cell[d]];
             v2n[i] = m->v2c_node[d];
         }
+        std::vector<int>().swap(m->v2c_cell);   // read only here
+        std::vector<int>().swap(m->v2c_node);
         m->d_v2c = dalloc<int>(m->n_dofs);
         h2d(m->d_v2c, v2c.data(), v2c.size(), s);
         m->d_v2n = dalloc<int>(m->n_dofs);
@@ -578,6 +600,11 @@ extern "C" int fgpu_mesh_set_patch_cells(fgpu_mesh *m, int cells) {
         HIPCHECK(hipSetDevice(m->ctx->device));
         if (cells == m->patches.cells) return FGPU_OK;
         std::vector<int> gptr;
+        if (m->icells.empty()) {   // a device import keeps the internal cells on the device only
+            m->icells.resize(8 * (size_t)m->n_cells);
+            d2h(m->icells.data(), m->d_cells, m->icells.size(), m->ctx->stream);
+            HIPCHECK(hipStreamSynchronize(m->ctx->stream));
+        }
         build_patches(m->n_cells, m->n_dofs, m->icells, cells, m->patches);
         build_slots(m->n_dofs, m->patches, gptr);
         upload_patches(m, gptr);
@@ -589,8 +616,10 @@ extern "C" int fgpu_mesh_device_bytes(const fgpu_mesh *m, size_t *bytes) {
     if (!m || !bytes) return FGPU_EINVAL;
     const size_t nd = m->n_dofs, nc = m->n_cells, ns = m->n_slots, nb = m->bf_cell.size();
     const Patches &P = m->patches;
-    *bytes = nd * (32 + 16 + 4 + 4 + 4) + nc * 32 + (P.cell0.size() + P.node0.size()) * 4 + P.nodes.size() * 8 +
-             (P.lidx.size() + P.lptr.size() + P.lent.size()) * 2 + nb * 8 + (nd + 1 + 4 * nb) * 4 + ns * 0;
+    // patch lists: nodes and slots (4 + 4 per slot), local indices and entries (2 + 2 per cell
+    // vertex), entry pointers (2 per slot and patch)
+    *bytes = nd * (32 + 16 + 4 + 4 + 4) + nc * 32 + (P.cell0.size() + P.node0.size()) * 4 + ns * 8 +
+             (16 * nc + ns + P.n) * 2 + nb * 8 + (nd + 1 + 4 * nb) * 4;
     return FGPU_OK;
 }
 

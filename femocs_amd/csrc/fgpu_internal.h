@@ -41,6 +41,7 @@ struct Patches {
     int node_cap = 0;             // largest node count of a patch, rounded up to 8
     std::vector<int> cell0;       // n+1: patch p covers internal cells [cell0[p], cell0[p+1])
     std::vector<int> node0;       // n+1: its nodes are nodes[node0[p] .. node0[p+1])
+    // the lists below are built on the host, uploaded, then released (the device keeps them)
     std::vector<int> nodes;       // internal dofs, first-appearance order within each patch
     std::vector<int> slot;        // slot-buffer position of every patch node
     std::vector<uint16_t> lidx;   // 8 per cell: patch-local index of each cell vertex
@@ -261,4 +262,9 @@ void build_patches(int nc, int nd, const std::vector<int> &icells, int cells_per
 // dof-major slot layout of the patch outputs: gptr (n_dofs+1) and Patches::slot
 void build_slots(int nd, Patches &P, std::vector<int> &gptr);
 void cell_layouts(int nc, const std::vector<int> &icells, const std::vector<double> &X, std::vector<unsigned char> &layout);
+// the same import on the device (fgpu_import.hip): fills every host table above and the mesh's
+// device arrays (cells, coordinates, patches, slots, boundary faces, gradient export, layouts);
+// nv > 0 and nc > 0
+int device_import(Mesh &m, hipStream_t st, int nv, const double *xyz, int nc, const int *ucd, int top, int bottom,
+                  int sides, int other, std::string &err);
 }  // namespace fgpu

@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(HERE, "libfemocs_gpu.so")
 
 FGPU_FP64, FGPU_MIXED = 0, 1
 FGPU_PC_NONE, FGPU_PC_JACOBI = 0, 1
+FGPU_IMPORT_HOST = 1
+FGPU_MESH_DIGEST_TABLES = 32
 BID_COPPER_SURFACE, BID_VACUUM_TOP, BID_COPPER_BOTTOM, BID_SIDES = 2, 8, 7, 4
 
 _lib = None
@@ -57,6 +59,9 @@ API = [
     ("fgpu_ctx_error", C.c_char_p, [_P]),
     ("fgpu_ctx_sync", C.c_int, [_P]),
     ("fgpu_mesh_import", C.c_int, [_P, C.c_int, _D, C.c_int, _I, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
+    ("fgpu_mesh_import_ex", C.c_int, [_P, C.c_int, _D, C.c_int, _I, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(_P)]),
+    ("fgpu_mesh_digest", C.c_int, [_P, C.c_int, C.POINTER(C.c_ulonglong)]),
     ("fgpu_mesh_destroy", None, [_P]),
     ("fgpu_mesh_sizes", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("fgpu_mesh_export_dofs", C.c_int, [_P, _P, _P]),
@@ -190,13 +195,15 @@ class Context:
 class Mesh:
     """Device mesh: DealSolver::import_mesh + mark_boundary."""
 
-    def __init__(self, ctx: Context, vertices, ucd_cells, top, bottom, sides, other):
+    def __init__(self, ctx: Context, vertices, ucd_cells, top, bottom, sides, other, host_import=False):
+        """The tables are built on the device; host_import=True builds the same bits on the host."""
         self.ctx = ctx
         v = np.ascontiguousarray(vertices, np.float64).reshape(-1)
         c = np.ascontiguousarray(ucd_cells, np.int32).reshape(-1)
         h = _P()
-        ctx.check(ctx.lib.fgpu_mesh_import(ctx.h, len(v) // 3, v, len(c) // 8, c, top, bottom, sides, other,
-                                           C.byref(h)), "fgpu_mesh_import")
+        flags = FGPU_IMPORT_HOST if host_import else 0
+        ctx.check(ctx.lib.fgpu_mesh_import_ex(ctx.h, len(v) // 3, v, len(c) // 8, c, top, bottom, sides, other,
+                                              flags, C.byref(h)), "fgpu_mesh_import_ex")
         self.h = h
         nd, nc, nb = C.c_int(), C.c_int(), C.c_int()
         ctx.lib.fgpu_mesh_sizes(h, C.byref(nd), C.byref(nc), C.byref(nb))
@@ -232,6 +239,12 @@ class Mesh:
         b = C.c_size_t()
         self.ctx.lib.fgpu_mesh_device_bytes(self.h, C.byref(b))
         return b.value
+
+    def digest(self):
+        """FNV-1a hash of every import table (host mirrors, then device arrays)."""
+        out = (C.c_ulonglong * FGPU_MESH_DIGEST_TABLES)()
+        self.ctx.check(self.ctx.lib.fgpu_mesh_digest(self.h, FGPU_MESH_DIGEST_TABLES, out), "fgpu_mesh_digest")
+        return list(out)
 
     def set_motion(self, ids, W):
         """Rows ids follow the control points through the sparse weights W (len(ids) x n_ctrl)."""

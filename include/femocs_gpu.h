@@ -77,6 +77,20 @@ int fgpu_ctx_sync(fgpu_ctx *ctx);
 int fgpu_mesh_import(fgpu_ctx *ctx, int n_vertices, const double *xyz, int n_cells,
                      const int *ucd_cells, int top, int bottom, int sides, int other,
                      fgpu_mesh **out);
+/* the same import with flags.  The tables (compaction, lexicographic cells, orientation,
+ * boundary faces and ids, vertex -> cell, the internal order, patches and slots) are built on
+ * the device by default; FGPU_IMPORT_HOST builds them on the host.  Both give the same bits
+ * (fgpu_mesh_digest).  No counterpart in the reference, which imports through deal.II
+ * (DealSolver::import_mesh, src/DealSolver.cpp:192-209). */
+#define FGPU_IMPORT_HOST 1
+int fgpu_mesh_import_ex(fgpu_ctx *ctx, int n_vertices, const double *xyz, int n_cells,
+                        const int *ucd_cells, int top, int bottom, int sides, int other, int flags,
+                        fgpu_mesh **out);
+/* 64-bit FNV-1a hashes of every import table, host mirrors and device arrays (n >=
+ * FGPU_MESH_DIGEST_TABLES): two imports of one mesh agree table by table.  No counterpart in
+ * the reference. */
+#define FGPU_MESH_DIGEST_TABLES 32
+int fgpu_mesh_digest(const fgpu_mesh *mesh, int n, unsigned long long *out);
 void fgpu_mesh_destroy(fgpu_mesh *mesh);
 /* DealSolver::size / get_n_cells; n_bfaces: boundary faces */
 int fgpu_mesh_sizes(const fgpu_mesh *mesh, int *n_dofs, int *n_cells, int *n_bfaces);
