@@ -249,9 +249,19 @@ def paths_bench(no_cpu: bool):
                           "same_tables": md.digest() == mh.digest()}
     del md, mh
     log("mesh import", out["mesh_import"])
-    t = time.perf_counter()
-    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
-    out["interpolator_create_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+    # interpolator tables (tet tables, neighbour lists, search tree, node lists): device and host
+    def icreate(host):
+        r = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM, host_build=host)
+        ctx.sync()
+        return r
+
+    tcd, ipd = _best_of(lambda: icreate(False))
+    tch, iph = _best_of(lambda: icreate(True), reps=1)
+    out["interpolator_create"] = {"tets": len(tip.tets), "device_ms": round(tcd * 1e3, 1),
+                                  "host_ms": round(tch * 1e3, 1), "same_tables": ipd.digest() == iph.digest()}
+    del iph
+    log("interpolator create", out["interpolator_create"])
+    ip = ipd
     # the reference's tet -> 4-hex split with its numbering, on the device (host arrays in/out)
     tris, _ = tip.surface_triangles()
     tsp, sp_out = _best_of(lambda: fgpu.tethex_split(ctx, tip.nodes[:tip.n_tetnodes], tip.tets, tris))

@@ -60,6 +60,8 @@ struct fgpu_interp {
     int *d_region_hex = nullptr;
     int *d_hex_cell = nullptr;              // inverse: solver cell of each hex, -1 outside
     int *d_nbr_ptr = nullptr, *d_nbr = nullptr;   // tet neighbour lists (LinearTetrahedra)
+    int n_nbr = 0, n_n2c = 0, n_vor = 0;          // lengths of the neighbour, node -> hex, Voronoi lists
+    bool L_on_host = false;                       // host copy L of the tet tables is filled
     // space-charge deposit: API cell -> internal cell of mesh `sc_mesh`, checked against
     // the region's hexes once per mesh
     int *sc_icell = nullptr;
@@ -679,6 +681,8 @@ void free_interp(fgpu_interp *ip) {
     free_triangles(ip);
 }
 
+#include "fgpu_interp_build.inc"
+
 void tet_candidates(fgpu_interp *ip, int n, int4 *cand) {
     hipLaunchKernelGGL(k_candidates<TetCells>, dim3(grid(n)), dim3(256), 0, ip->ctx->stream, n, ip->pts,
                        ip->B.n_leaves, ip->child, ip->box, ip->cbox, ip->leafbox, ip->prim,
@@ -694,8 +698,17 @@ extern "C" int fgpu_interp_create(fgpu_ctx *c, int n_nodes, const double *xyz, c
                                   const int *tets, const int *tet_nbr, const int *tet_marker, int n_hexes,
                                   const int *hexes, const int *hex_marker, int search_region, double empty_value,
                                   fgpu_interp **out) {
+    return fgpu_interp_create_ex(c, n_nodes, xyz, node_kind, n_tets, tets, tet_nbr, tet_marker, n_hexes, hexes,
+                                 hex_marker, search_region, empty_value, 0, out);
+}
+
+extern "C" int fgpu_interp_create_ex(fgpu_ctx *c, int n_nodes, const double *xyz, const int *node_kind, int n_tets,
+                                     const int *tets, const int *tet_nbr, const int *tet_marker, int n_hexes,
+                                     const int *hexes, const int *hex_marker, int search_region, double empty_value,
+                                     int flags, fgpu_interp **out) {
     if (!c || !out || n_nodes <= 0 || n_tets <= 0 || !xyz || !node_kind || !tets || !tet_nbr || !tet_marker ||
-        (n_hexes > 0 && (!hexes || !hex_marker)) || (n_hexes != 0 && n_hexes != 4 * n_tets))
+        (n_hexes > 0 && (!hexes || !hex_marker)) || (n_hexes != 0 && n_hexes != 4 * n_tets) ||
+        (flags & ~FGPU_INTERP_HOST))
         return FGPU_EINVAL;
     *out = nullptr;
     fgpu_interp *ip = new (std::nothrow) fgpu_interp();
@@ -708,6 +721,8 @@ extern "C" int fgpu_interp_create(fgpu_ctx *c, int n_nodes, const double *xyz, c
             if (tets[i] < 0 || tets[i] >= n_nodes) throw std::invalid_argument("tet node out of range");
         for (size_t i = 0; i < 8 * (size_t)n_hexes; ++i)
             if (hexes[i] < 0 || hexes[i] >= n_nodes) throw std::invalid_argument("hex node out of range");
+        for (size_t i = 0; i < 4 * (size_t)n_tets; ++i)
+            if (tet_nbr[i] < -1 || tet_nbr[i] >= n_tets) throw std::invalid_argument("tet neighbour out of range");
         ip->n_nodes = n_nodes;
         ip->n_tets = n_tets;
         ip->n_hexes = n_hexes;
@@ -716,102 +731,114 @@ extern "C" int fgpu_interp_create(fgpu_ctx *c, int n_nodes, const double *xyz, c
         int tn = 0;
         for (size_t i = 0; i < 4 * (size_t)n_tets; ++i) tn = std::max(tn, tets[i] + 1);
         ip->n_tetnodes = tn;
-        build_lintet(ip->L, n_nodes, xyz, n_tets, tets, tet_nbr, tet_marker, search_region);
         ip->hxyz.assign(xyz, xyz + 3 * (size_t)n_nodes);
         ip->hkind.assign(node_kind, node_kind + n_nodes);
         ip->htets.assign(tets, tets + 4 * (size_t)n_tets);
-        ip->d_nbr_ptr = upload(ip->L.nbr_ptr.data(), ip->L.nbr_ptr.size(), s);
-        ip->d_nbr = upload(ip->L.nbr.data(), ip->L.nbr.size(), s);
-        // LinearTetrahedra::precompute: decay factor of the smoothing, -1 / longest tet edge
-        double emax = -1e308;
-        for (int t = 0; t < n_tets; ++t) {
-            const double *v[4];
-            for (int k = 0; k < 4; ++k) v[k] = xyz + 3 * (size_t)tets[4 * (size_t)t + k];
-            const int P[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
-            for (auto &pq : P) {
-                const double xx = v[pq[0]][0] - v[pq[1]][0], yy = v[pq[0]][1] - v[pq[1]][1], zz = v[pq[0]][2] - v[pq[1]][2];
-                emax = std::max(emax, xx * xx + yy * yy + zz * zz);
-            }
-        }
-        ip->decay = -1.0 / std::sqrt(emax);
-        // bounding boxes of the tets for the search tree
-        std::vector<double> lo(3 * (size_t)n_tets), hi(3 * (size_t)n_tets);
-        for (int t = 0; t < n_tets; ++t)
-            for (int a = 0; a < 3; ++a) {
-                double l = 1e300, h = -1e300;
-                for (int k = 0; k < 4; ++k) {
-                    const double x = xyz[3 * (size_t)tets[4 * (size_t)t + k] + a];
-                    l = std::min(l, x);
-                    h = std::max(h, x);
-                }
-                lo[3 * (size_t)t + a] = l;
-                hi[3 * (size_t)t + a] = h;
-            }
-        build_bvh(n_tets, lo, hi, ip->L.centroid, ip->B);
-        std::vector<int> c10;
-        if (n_hexes > 0) build_quadtet(n_tets, tets, hexes, node_kind, c10);
-        else c10.assign(10 * (size_t)n_tets, 0);
-        // node -> (hex, corner) of the region's hexes in hex order (Interpolator::initialize)
-        std::vector<int> nptr(n_nodes + 1, 0), n2c;
-        std::vector<int> vptr(tn + 1, 0), vor;
+        ip->xyz = upload(xyz, 3 * (size_t)n_nodes, s);
+        ip->tets = upload(tets, 4 * (size_t)n_tets, s);
+        ip->hexes = upload(hexes, 8 * (size_t)n_hexes, s);
+        // the region's hexes in hex order and their solver cells (Interpolator::deal2femocs)
+        auto in_region = [&](int h) { return search_region == 3 ? hex_marker[h] > 0 : hex_marker[h] < 0; };
         if (n_hexes > 0) {
-            auto in_region = [&](int h) { return search_region == 3 ? hex_marker[h] > 0 : hex_marker[h] < 0; };
             for (int h = 0; h < n_hexes; ++h)
                 if (in_region(h)) ip->region_hex.push_back(h);
             ip->d_region_hex = upload(ip->region_hex.data(), ip->region_hex.size(), s);
             std::vector<int> hex_cell(n_hexes, -1);
-            for (size_t c = 0; c < ip->region_hex.size(); ++c) hex_cell[ip->region_hex[c]] = (int)c;
+            for (size_t k = 0; k < ip->region_hex.size(); ++k) hex_cell[ip->region_hex[k]] = (int)k;
             ip->d_hex_cell = upload(hex_cell.data(), hex_cell.size(), s);
             IPCHECK(hipStreamSynchronize(s));       // hex_cell is read until the copy ends
-            for (int h = 0; h < n_hexes; ++h)
-                if (in_region(h))
-                    for (int k = 0; k < 8; ++k) nptr[hexes[8 * (size_t)h + k] + 1]++;
-            for (int i = 0; i < n_nodes; ++i) nptr[i + 1] += nptr[i];
-            n2c.resize(nptr[n_nodes]);
-            std::vector<int> pos(nptr.begin(), nptr.end());
-            for (int h = 0; h < n_hexes; ++h)
-                if (in_region(h))
-                    for (int k = 0; k < 8; ++k) n2c[pos[hexes[8 * (size_t)h + k]]++] = 8 * h + k;
-            // TetgenMesh::calc_pseudo_3D_vorocells (src/TetgenMesh.cpp:819-837)
-            const int mult = search_region == 3 ? 1 : -1;
-            for (int pass = 0; pass < 2; ++pass) {
-                std::vector<int> vpos(vptr.begin(), vptr.end());
-                for (int h = 0; h < n_hexes; ++h) {
-                    int t = mult * hex_marker[h];
-                    if (t <= 0) continue;
-                    t -= 1;
-                    if (t >= tn) continue;
-                    for (int k = 0; k < 8; ++k) {
-                        const int node = hexes[8 * (size_t)h + k];
-                        if (node != t && node_kind[node] >= 2) {
-                            if (pass == 0) vptr[t + 1]++;
-                            else vor[vpos[t]++] = node;
-                        }
-                    }
-                }
-                if (pass == 0) {
-                    for (int i = 0; i < tn; ++i) vptr[i + 1] += vptr[i];
-                    vor.resize(vptr[tn]);
+        }
+        if (!(flags & FGPU_INTERP_HOST)) {
+            device_build(ip, tet_nbr, tet_marker, node_kind, hex_marker);
+        } else {
+            build_lintet(ip->L, n_nodes, xyz, n_tets, tets, tet_nbr, tet_marker, search_region);
+            ip->L_on_host = true;
+            ip->n_nbr = (int)ip->L.nbr.size();
+            ip->d_nbr_ptr = upload(ip->L.nbr_ptr.data(), ip->L.nbr_ptr.size(), s);
+            ip->d_nbr = upload(ip->L.nbr.data(), ip->L.nbr.size(), s);
+            // LinearTetrahedra::precompute: decay factor of the smoothing, -1 / longest tet edge
+            double emax = -1e308;
+            for (int t = 0; t < n_tets; ++t) {
+                const double *v[4];
+                for (int k = 0; k < 4; ++k) v[k] = xyz + 3 * (size_t)tets[4 * (size_t)t + k];
+                const int P[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+                for (auto &pq : P) {
+                    const double xx = v[pq[0]][0] - v[pq[1]][0], yy = v[pq[0]][1] - v[pq[1]][1], zz = v[pq[0]][2] - v[pq[1]][2];
+                    emax = std::max(emax, xx * xx + yy * yy + zz * zz);
                 }
             }
+            ip->decay = -1.0 / std::sqrt(emax);
+            // bounding boxes of the tets for the search tree
+            std::vector<double> lo(3 * (size_t)n_tets), hi(3 * (size_t)n_tets);
+            for (int t = 0; t < n_tets; ++t)
+                for (int a = 0; a < 3; ++a) {
+                    double l = 1e300, h = -1e300;
+                    for (int k = 0; k < 4; ++k) {
+                        const double x = xyz[3 * (size_t)tets[4 * (size_t)t + k] + a];
+                        l = std::min(l, x);
+                        h = std::max(h, x);
+                    }
+                    lo[3 * (size_t)t + a] = l;
+                    hi[3 * (size_t)t + a] = h;
+                }
+            build_bvh(n_tets, lo, hi, ip->L.centroid, ip->B);
+            std::vector<int> c10;
+            if (n_hexes > 0) build_quadtet(n_tets, tets, hexes, node_kind, c10);
+            else c10.assign(10 * (size_t)n_tets, 0);
+            // node -> (hex, corner) of the region's hexes in hex order (Interpolator::initialize)
+            std::vector<int> nptr(n_nodes + 1, 0), n2c;
+            std::vector<int> vptr(tn + 1, 0), vor;
+            if (n_hexes > 0) {
+                for (int h = 0; h < n_hexes; ++h)
+                    if (in_region(h))
+                        for (int k = 0; k < 8; ++k) nptr[hexes[8 * (size_t)h + k] + 1]++;
+                for (int i = 0; i < n_nodes; ++i) nptr[i + 1] += nptr[i];
+                n2c.resize(nptr[n_nodes]);
+                std::vector<int> pos(nptr.begin(), nptr.end());
+                for (int h = 0; h < n_hexes; ++h)
+                    if (in_region(h))
+                        for (int k = 0; k < 8; ++k) n2c[pos[hexes[8 * (size_t)h + k]]++] = 8 * h + k;
+                // TetgenMesh::calc_pseudo_3D_vorocells (src/TetgenMesh.cpp:819-837)
+                const int mult = search_region == 3 ? 1 : -1;
+                for (int pass = 0; pass < 2; ++pass) {
+                    std::vector<int> vpos(vptr.begin(), vptr.end());
+                    for (int h = 0; h < n_hexes; ++h) {
+                        int t = mult * hex_marker[h];
+                        if (t <= 0) continue;
+                        t -= 1;
+                        if (t >= tn) continue;
+                        for (int k = 0; k < 8; ++k) {
+                            const int node = hexes[8 * (size_t)h + k];
+                            if (node != t && node_kind[node] >= 2) {
+                                if (pass == 0) vptr[t + 1]++;
+                                else vor[vpos[t]++] = node;
+                            }
+                        }
+                    }
+                    if (pass == 0) {
+                        for (int i = 0; i < tn; ++i) vptr[i + 1] += vptr[i];
+                        vor.resize(vptr[tn]);
+                    }
+                }
+            }
+            ip->n_n2c = (int)n2c.size();
+            ip->n_vor = (int)vor.size();
+            ip->det0 = upload(ip->L.det0.data(), ip->L.det0.size(), s);
+            ip->det = upload(ip->L.det.data(), ip->L.det.size(), s);
+            ip->cen = upload(ip->L.centroid.data(), ip->L.centroid.size(), s);
+            ip->marker = upload(ip->L.marker.data(), ip->L.marker.size(), s);
+            ip->c10 = upload(c10.data(), c10.size(), s);
+            ip->child = upload(ip->B.child.data(), ip->B.child.size(), s);
+            ip->box = upload(ip->B.box.data(), ip->B.box.size(), s);
+            ip->cbox = upload(ip->B.cbox.data(), ip->B.cbox.size(), s);
+            ip->leafbox = upload(ip->B.leafbox.data(), ip->B.leafbox.size(), s);
+            ip->prim = upload(ip->B.prim.data(), ip->B.prim.size(), s);
+            ip->n2c_ptr = upload(nptr.data(), nptr.size(), s);
+            ip->n2c = upload(n2c.data(), n2c.size(), s);
+            ip->vor_ptr = upload(vptr.data(), vptr.size(), s);
+            ip->vor = upload(vor.data(), vor.size(), s);
+            IPCHECK(hipStreamSynchronize(s));   // the host vectors are read until the copies end
         }
-        ip->xyz = upload(xyz, 3 * (size_t)n_nodes, s);
-        ip->det0 = upload(ip->L.det0.data(), ip->L.det0.size(), s);
-        ip->det = upload(ip->L.det.data(), ip->L.det.size(), s);
-        ip->cen = upload(ip->L.centroid.data(), ip->L.centroid.size(), s);
-        ip->marker = upload(ip->L.marker.data(), ip->L.marker.size(), s);
-        ip->tets = upload(tets, 4 * (size_t)n_tets, s);
-        ip->c10 = upload(c10.data(), c10.size(), s);
-        ip->hexes = upload(hexes, 8 * (size_t)n_hexes, s);
-        ip->child = upload(ip->B.child.data(), ip->B.child.size(), s);
-        ip->box = upload(ip->B.box.data(), ip->B.box.size(), s);
-        ip->cbox = upload(ip->B.cbox.data(), ip->B.cbox.size(), s);
-        ip->leafbox = upload(ip->B.leafbox.data(), ip->B.leafbox.size(), s);
-        ip->prim = upload(ip->B.prim.data(), ip->B.prim.size(), s);
-        ip->n2c_ptr = upload(nptr.data(), nptr.size(), s);
-        ip->n2c = upload(n2c.data(), n2c.size(), s);
-        ip->vor_ptr = upload(vptr.data(), vptr.size(), s);
-        ip->vor = upload(vor.data(), vor.size(), s);
         std::vector<double> sol(5 * (size_t)n_nodes, empty_value);
         ip->sol = upload(sol.data(), sol.size(), s);
         IPCHECK(hipStreamSynchronize(s));
@@ -825,6 +852,61 @@ extern "C" int fgpu_interp_create(fgpu_ctx *c, int n_nodes, const double *xyz, c
     c->refs++;
     *out = ip;
     return FGPU_OK;
+}
+
+namespace {
+unsigned long long fnv64(const void *p, size_t bytes) {
+    unsigned long long h = 1469598103934665603ULL;
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < bytes; ++i) {
+        h ^= b[i];
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+template <class T> unsigned long long fnv_host(const std::vector<T> &v) { return fnv64(v.data(), v.size() * sizeof(T)); }
+template <class T> unsigned long long fnv_device(const T *d, size_t n, hipStream_t st) {
+    std::vector<T> h(n);
+    if (n) IPCHECK(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, st));
+    IPCHECK(hipStreamSynchronize(st));
+    return fnv_host(h);
+}
+}  // namespace
+
+extern "C" int fgpu_interp_digest(fgpu_interp *ip, int n, unsigned long long *out) {
+    if (!ip || !out || n < FGPU_INTERP_DIGEST_TABLES) return FGPU_EINVAL;
+    return guard(ip, [&] {
+        IPCHECK(hipSetDevice(ip->ctx->device));
+        hipStream_t st = ip->ctx->stream;
+        ensure_host_tets(ip);
+        const size_t nt = ip->n_tets, ni = nt > 0 ? nt - 1 : 0;
+        const LinTetHost &L = ip->L;
+        int k = 0;
+        out[k++] = fnv_host(L.det0);
+        out[k++] = fnv_host(L.det);
+        out[k++] = fnv_host(L.centroid);
+        out[k++] = fnv_host(L.marker);
+        out[k++] = fnv_host(L.nbr_ptr);
+        out[k++] = fnv_host(L.nbr);
+        out[k++] = fnv_device(ip->det0, nt, st);
+        out[k++] = fnv_device(ip->det, 16 * nt, st);
+        out[k++] = fnv_device(ip->cen, 3 * nt, st);
+        out[k++] = fnv_device(ip->marker, nt, st);
+        out[k++] = fnv_device(ip->d_nbr_ptr, nt + 1, st);
+        out[k++] = fnv_device(ip->d_nbr, (size_t)ip->n_nbr, st);
+        out[k++] = fnv_device(ip->child, 2 * ni, st);
+        out[k++] = fnv_device(ip->box, 6 * ni, st);
+        out[k++] = fnv_device(ip->cbox, 6 * ni, st);
+        out[k++] = fnv_device(ip->leafbox, 6 * nt, st);
+        out[k++] = fnv_device(ip->prim, nt, st);
+        out[k++] = fnv_device(ip->c10, 10 * nt, st);
+        out[k++] = fnv_device(ip->n2c_ptr, (size_t)ip->n_nodes + 1, st);
+        out[k++] = fnv_device(ip->n2c, (size_t)ip->n_n2c, st);
+        out[k++] = fnv_device(ip->vor_ptr, (size_t)ip->n_tetnodes + 1, st);
+        out[k++] = fnv_device(ip->vor, (size_t)ip->n_vor, st);
+        out[k++] = fnv64(&ip->decay, sizeof(double));
+        return FGPU_OK;
+    });
 }
 
 extern "C" void fgpu_interp_destroy(fgpu_interp *ip) {
@@ -1174,6 +1256,7 @@ static int interp_run(fgpu_interp *ip, int n, const double *xyz, int rank, int *
             for (int i = 0; i < n; ++i)
                 if (cand[i].w) throw std::runtime_error("point search stack overflow");
             std::vector<int> tet(n);
+            ensure_host_tets(ip);
             resolve_chain(ip->L, n, xyz, cand.data(), rank, tet.data());
             IPCHECK(hipMemcpyAsync(ip->tet, tet.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
             hipLaunchKernelGGL(k_interp, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, rank, rank == 3 ? 1 : 0, ip->tet,
@@ -1331,6 +1414,7 @@ static int surface_run(fgpu_interp *ip, int n, const double *xyz, int rank, int 
             for (int i = 0; i < n; ++i)
                 if (tc[i].w || (locate && fc[i].w)) throw std::runtime_error("point search stack overflow");
             if (locate) resolve_tri_chain(ip->T, n, xyz, fc.data(), rank == 3, cells);
+            ensure_host_tets(ip);
             for (int i = 0; i < n; ++i) {
                 const double *p = xyz + 3 * (size_t)i;
                 use[i] = rank == 3 ? surface_hex(ip->T, ip->L, p, std::abs(cells[i]), tc[i])

@@ -17,6 +17,8 @@ FGPU_FP64, FGPU_MIXED = 0, 1
 FGPU_PC_NONE, FGPU_PC_JACOBI = 0, 1
 FGPU_IMPORT_HOST = 1
 FGPU_MESH_DIGEST_TABLES = 32
+FGPU_INTERP_HOST = 1
+FGPU_INTERP_DIGEST_TABLES = 23
 BID_COPPER_SURFACE, BID_VACUUM_TOP, BID_COPPER_BOTTOM, BID_SIDES = 2, 8, 7, 4
 
 _lib = None
@@ -86,6 +88,9 @@ API = [
     ("fgpu_batch_time_kernel", C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     ("fgpu_interp_create", C.c_int, [_P, C.c_int, _D, _I, C.c_int, _I, _I, _I, C.c_int, _I, _I, C.c_int,
                                      C.c_double, C.POINTER(_P)]),
+    ("fgpu_interp_create_ex", C.c_int, [_P, C.c_int, _D, _I, C.c_int, _I, _I, _I, C.c_int, _I, _I, C.c_int,
+                                        C.c_double, C.c_int, C.POINTER(_P)]),
+    ("fgpu_interp_digest", C.c_int, [_P, C.c_int, C.POINTER(C.c_ulonglong)]),
     ("fgpu_interp_destroy", None, [_P]),
     ("fgpu_interp_extract", C.c_int, [_P, _P, C.c_int]),
     ("fgpu_interp_extract_heat", C.c_int, [_P, _P, _P]),

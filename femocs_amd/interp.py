@@ -25,20 +25,23 @@ class Interpolator:
     """Interpolator::initialize (src/Interpolator.cpp:28-77) on a tet mesh and its tet-split
     hexes (hex 4t+k around vertex k of tet t)."""
 
-    def __init__(self, ctx: fgpu.Context, tip, search_region: int = TYPES_VACUUM, empty_value: float = 0.0):
+    def __init__(self, ctx: fgpu.Context, tip, search_region: int = TYPES_VACUUM, empty_value: float = 0.0,
+                 host_build: bool = False):
+        """The tables are built on the device; host_build=True builds the same bits on the host."""
         self.ctx = ctx
         self.n_nodes = len(tip.nodes)
         self.search_region = search_region
         h = fgpu._P()
-        ctx.check(ctx.lib.fgpu_interp_create(
+        ctx.check(ctx.lib.fgpu_interp_create_ex(
             ctx.h, self.n_nodes, np.ascontiguousarray(tip.nodes, np.float64).reshape(-1),
             np.ascontiguousarray(tip.node_kind, np.int32), len(tip.tets),
             np.ascontiguousarray(tip.tets, np.int32).reshape(-1),
             np.ascontiguousarray(tip.tet_nbr, np.int32).reshape(-1),
             np.ascontiguousarray(tip.tet_marker, np.int32), len(tip.hexes),
             np.ascontiguousarray(tip.hexes, np.int32).reshape(-1),
-            np.ascontiguousarray(tip.hex_marker, np.int32), int(search_region), float(empty_value), C.byref(h)),
-            "fgpu_interp_create")
+            np.ascontiguousarray(tip.hex_marker, np.int32), int(search_region), float(empty_value),
+            fgpu.FGPU_INTERP_HOST if host_build else 0, C.byref(h)),
+            "fgpu_interp_create_ex")
         self.h = h
 
     def __del__(self):
@@ -48,6 +51,13 @@ class Interpolator:
                 self.h = None
         except Exception:
             pass
+
+    def digest(self):
+        """FNV-1a hash of every table (host copy of the tet tables, device tables, decay)."""
+        out = (C.c_ulonglong * fgpu.FGPU_INTERP_DIGEST_TABLES)()
+        self.ctx.check(self.ctx.lib.fgpu_interp_digest(self.h, fgpu.FGPU_INTERP_DIGEST_TABLES, out),
+                       "fgpu_interp_digest")
+        return list(out)
 
     def extract_solution(self, system: fgpu.System, smoothen: bool = True):
         """Interpolator::extract_solution(PoissonSolver&, smoothen) (src/Interpolator.cpp:172-190)."""

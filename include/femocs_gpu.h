@@ -235,6 +235,21 @@ int fgpu_batch_time_kernel(fgpu_batch *batch, int which, int reps, double *avg_m
 int fgpu_interp_create(fgpu_ctx *ctx, int n_nodes, const double *xyz, const int *node_kind, int n_tets,
                        const int *tets, const int *tet_nbr, const int *tet_marker, int n_hexes, const int *hexes,
                        const int *hex_marker, int search_region, double empty_value, fgpu_interp **out);
+/* the same with flags.  The tables (tet determinants and centroids, neighbour lists, search
+ * tree, 10-node tets, node -> hex and pseudo-Voronoi lists) are built on the device by default;
+ * FGPU_INTERP_HOST builds them on the host.  Both give the same bits (fgpu_interp_digest).  No
+ * counterpart in the reference, whose InterpolatorCells::precompute runs on the host
+ * (src/InterpolatorCells.cpp:523-742). */
+#define FGPU_INTERP_HOST 1
+int fgpu_interp_create_ex(fgpu_ctx *ctx, int n_nodes, const double *xyz, const int *node_kind, int n_tets,
+                          const int *tets, const int *tet_nbr, const int *tet_marker, int n_hexes, const int *hexes,
+                          const int *hex_marker, int search_region, double empty_value, int flags,
+                          fgpu_interp **out);
+/* 64-bit FNV-1a hashes of every interpolator table (host copy of the tet tables, device
+ * tables, the smoothing decay), n >= FGPU_INTERP_DIGEST_TABLES.  No counterpart in the
+ * reference. */
+#define FGPU_INTERP_DIGEST_TABLES 23
+int fgpu_interp_digest(fgpu_interp *interp, int n, unsigned long long *out);
 void fgpu_interp_destroy(fgpu_interp *interp);
 /* Interpolator::extract_solution(PoissonSolver&, smoothen) (src/Interpolator.cpp:142-190): from
  * a solved system whose mesh was imported with these nodes as vertices: Solution{nodal field
