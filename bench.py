@@ -260,6 +260,13 @@ def paths_bench(no_cpu: bool):
     out["interpolator_create"] = {"tets": len(tip.tets), "device_ms": round(tcd * 1e3, 1),
                                   "host_ms": round(tch * 1e3, 1), "same_tables": ipd.digest() == iph.digest()}
     del iph
+    # the first located transfer of a new interpolator also fetches the host copy of its tet
+    # tables (the chained guess reads them)
+    ipf = icreate(False)
+    t = time.perf_counter()
+    ipf.points(tip.atoms, 1)
+    out["interpolator_create"]["first_locate_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+    del ipf
     log("interpolator create", out["interpolator_create"])
     ip = ipd
     # the reference's tet -> 4-hex split with its numbering, on the device (host arrays in/out)

@@ -24,6 +24,7 @@ BID_COPPER_SURFACE, BID_VACUUM_TOP, BID_COPPER_BOTTOM, BID_SIDES = 2, 8, 7, 4
 _lib = None
 
 
+FGPU_OK, FGPU_EINVAL = 0, -1
 FGPU_EMESH = -5
 
 

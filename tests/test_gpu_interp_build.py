@@ -53,3 +53,43 @@ def test_transfer_is_identical_on_either_build(ctx, small_mesh):
 def test_large_tip(ctx):
     tip = M.make_tip_mesh(M.tip_r10nm_geometry(), with_atoms=False)
     _same(ctx, tip, interp.TYPES_VACUUM)
+
+
+def _create(ctx, tip, nbr, n_hexes, flags):
+    import ctypes as C
+    h = fgpu._P()
+    hexes = np.ascontiguousarray(tip.hexes if n_hexes else np.zeros((0, 8)), np.int32).reshape(-1)
+    hm = np.ascontiguousarray(tip.hex_marker if n_hexes else np.zeros(0), np.int32)
+    rc = ctx.lib.fgpu_interp_create_ex(
+        ctx.h, len(tip.nodes), np.ascontiguousarray(tip.nodes, np.float64).reshape(-1),
+        np.ascontiguousarray(tip.node_kind, np.int32), len(tip.tets), np.ascontiguousarray(tip.tets, np.int32).reshape(-1),
+        np.ascontiguousarray(nbr, np.int32).reshape(-1), np.ascontiguousarray(tip.tet_marker, np.int32),
+        n_hexes, hexes, hm, interp.TYPES_VACUUM, 0.0, flags, C.byref(h))
+    return rc, h
+
+
+def _digest(ctx, h):
+    import ctypes as C
+    out = (C.c_ulonglong * fgpu.FGPU_INTERP_DIGEST_TABLES)()
+    ctx.check(ctx.lib.fgpu_interp_digest(h, fgpu.FGPU_INTERP_DIGEST_TABLES, out), "fgpu_interp_digest")
+    return list(out)
+
+
+def test_tets_only_and_invalid_neighbours(ctx, small_mesh):
+    """Without hexes (no 10-node tets, no node lists) both builds agree; a neighbour index that
+    is neither -1 nor a tet is refused before anything is built."""
+    hs = []
+    for flags in (0, fgpu.FGPU_INTERP_HOST):
+        rc, h = _create(ctx, small_mesh, small_mesh.tet_nbr, 0, flags)
+        assert rc == fgpu.FGPU_OK
+        hs.append(h)
+    try:
+        assert _digest(ctx, hs[0]) == _digest(ctx, hs[1])
+    finally:
+        for h in hs:
+            ctx.lib.fgpu_interp_destroy(h)
+    bad = np.array(small_mesh.tet_nbr, np.int32)
+    bad[5, 2] = len(small_mesh.tets)
+    for flags in (0, fgpu.FGPU_INTERP_HOST):
+        rc, h = _create(ctx, small_mesh, bad, len(small_mesh.hexes), flags)
+        assert rc == fgpu.FGPU_EINVAL and not h
