@@ -371,3 +371,45 @@ def test_particle_cell_update_against_brute_force(tip):
         assert b[k] == max(b)
     again = O.update_cells(L, d2f, f2d, pts, cells[:-1])
     assert np.array_equal(again, cells[:-1])
+
+
+def test_smoothing_weights_on_one_regular_tet():
+    """One regular tet (edge a) split around its vertices: vertex 0's pseudo-Voronoi neighbours
+    are its three edge midpoints (distance a/2), three face centroids (a/sqrt 3) and the tet
+    centroid (a sqrt 6 / 4), so average_nodal_fields (src/Interpolator.cpp:142-170) with
+    weights exp(decay * distance) has a closed form from the geometry alone.  Checks the
+    neighbour set, the weight law and its sign, the normalisation, decay = 0 (plain mean) and
+    that edge/face/centre nodes and the scalar slots are left alone."""
+    a = 2.0
+    V = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], float) * (a / np.sqrt(8))
+    edges = [(i, j) for i in range(4) for j in range(i + 1, 4)]
+    faces = [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)]
+    nodes = np.concatenate([V, [(V[i] + V[j]) / 2 for i, j in edges], [V[list(f)].mean(0) for f in faces],
+                            [V.mean(0)]])
+    kind = np.array([1] * 4 + [2] * 6 + [3] * 4 + [4], np.int32)
+    hexes, marker = [], []
+    for v in range(4):   # node order inside a hex does not enter the average
+        hexes.append([v] + [4 + e for e, (i, j) in enumerate(edges) if v in (i, j)]
+                     + [10 + f for f, fv in enumerate(faces) if v in fv] + [14])
+        marker.append(v + 1)
+    hexes, marker = np.array(hexes, np.int32), np.array(marker, np.int32)
+    assert hexes.shape == (4, 8)
+    value = np.array([0.0] * 4 + [1.0] * 6 + [2.0] * 4 + [5.0])
+    sol = np.zeros((15, 5))
+    sol[:, 0] = value
+    sol[:, 1] = 7.0
+    sol[:, 3] = np.arange(15)
+    sol[:, 4] = -1.0
+    for decay in (-1.0 / a, 0.0):
+        out = O.smooth_nodal(sol, nodes, 4, kind, hexes, marker, True, decay)
+        we, wf, wc = np.exp(decay * a / 2), np.exp(decay * a / np.sqrt(3)), np.exp(decay * a * np.sqrt(6) / 4)
+        expect = (3 * we * 1.0 + 3 * wf * 2.0 + wc * 5.0) / (3 * we + 3 * wf + wc)
+        assert out[0, 0] == pytest.approx(expect, rel=1e-14)
+        assert np.allclose(out[:4, 0], expect, rtol=1e-14)       # every vertex sees the same
+        assert np.allclose(out[:4, 1], 7.0, rtol=1e-15)          # constants are kept
+        assert np.array_equal(out[4:], sol[4:])                  # non-vertex nodes untouched
+        assert np.array_equal(out[:, 3:], sol[:, 3:])            # scalars untouched
+    # the nearer classes weigh more with the negative decay: the mean moves towards 1
+    plain = (3 * 1.0 + 3 * 2.0 + 5.0) / 7
+    near = O.smooth_nodal(sol, nodes, 4, kind, hexes, marker, True, -1.0 / a)[0, 0]
+    assert near < plain
