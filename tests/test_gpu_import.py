@@ -103,3 +103,24 @@ def test_solve_on_either_import_is_identical(ctx, small_mesh):
         out.append((st.iterations, s.solution()))
     assert out[0][0] == out[1][0] > 0
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_patch_rebuild_after_either_import(ctx, small_mesh):
+    """A new patch size rebuilds the patches from the internal cells, which a device import keeps
+    on the device only: both imports still agree table by table and solve to the same bits."""
+    dev, host = _same(ctx, small_mesh.nodes, small_mesh.vacuum_cells())
+    out = []
+    for m in (dev, host):
+        m.set_patch_cells(128)
+    assert dev.digest() == host.digest()
+    for m in (dev, host):
+        s = fgpu.System(m)
+        s.reset()
+        s.add_dirichlet(2, 0.0)
+        s.add_neumann(8, None, -1.0)
+        s.apply_dirichlet()
+        s.fill_solution(0.0)
+        st = s.solve(fgpu.opts(tol=1e-9, relative=True))
+        out.append((st.iterations, s.solution()))
+    assert out[0][0] == out[1][0] > 0
+    assert np.array_equal(out[0][1], out[1][1])
