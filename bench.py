@@ -368,6 +368,50 @@ def paths_bench(no_cpu: bool):
     print(json.dumps(out), flush=True)
 
 
+def setup_bench(reps: int):
+    """One remesh of the r = 10 nm tip on the device, `reps` times: the Tethex split of its tets
+    (host arrays in and out), the import of the vacuum hexes and the interpolator tables; then
+    the same import and tables once on the host, and whether both give the same tables.  One
+    JSON line; host buffers cross PCIe inside these timings."""
+    from femocs_amd import fgpu, interp
+    from femocs_amd import mesh as M
+    tip = M.make_tip_mesh(M.tip_r10nm_geometry(), with_atoms=False)
+    ctx = fgpu.Context(0)
+    tris, _ = tip.surface_triangles()
+    vc = tip.vacuum_cells()
+    steps = {"tet_split": [], "mesh_import": [], "interpolator": []}
+    md = ipd = None
+    for r in range(reps + 1):          # the first pass loads the kernels and is not kept
+        t0 = time.perf_counter()
+        fgpu.tethex_split(ctx, tip.nodes[:tip.n_tetnodes], tip.tets, tris)
+        t1 = time.perf_counter()
+        md = fgpu.Mesh(ctx, tip.nodes, vc, 8, 2, 4, 2)
+        ctx.sync()
+        t2 = time.perf_counter()
+        ipd = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+        ctx.sync()
+        t3 = time.perf_counter()
+        if r > 0:
+            for k, v in zip(steps, (t1 - t0, t2 - t1, t3 - t2)):
+                steps[k].append(v * 1e3)
+        log("setup pass", r, [round(v * 1e3, 1) for v in (t1 - t0, t2 - t1, t3 - t2)])
+    t = time.perf_counter()
+    mh = fgpu.Mesh(ctx, tip.nodes, vc, 8, 2, 4, 2, host_import=True)
+    t_imp = time.perf_counter() - t
+    t = time.perf_counter()
+    iph = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM, host_build=True)
+    t_int = time.perf_counter() - t
+    out = {"metric": "remesh setup on the device: tet split, mesh import, interpolator tables (1 GPU)",
+           "unit": "ms", "data": "synthetic", "reps": reps,
+           "config": {"workload": "r = 10 nm tip", "tets": len(tip.tets), "vacuum_hexes": md.n_cells,
+                      "dofs": md.n_dofs},
+           "device_ms": {k: {"best": round(min(v), 2), "mean": round(sum(v) / len(v), 2)} for k, v in steps.items()},
+           "host_ms": {"mesh_import": round(t_imp * 1e3, 1), "interpolator": round(t_int * 1e3, 1)},
+           "same_tables": {"mesh_import": md.digest() == mh.digest(), "interpolator": ipd.digest() == iph.digest()}}
+    out["device_ms"]["total_best"] = round(sum(v["best"] for v in out["device_ms"].values()), 2)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,7 +428,12 @@ def main():
                     help="only the counter target of roofline.traffic (see traffic_probe)")
     ap.add_argument("--paths", action="store_true",
                     help="the secondary paths instead of the headline (transfer, PIC pieces, coupled heat)")
+    ap.add_argument("--setup", action="store_true",
+                    help="only the remesh setup on the device (tet split, import, interpolator tables)")
     args = ap.parse_args()
+    if args.setup:
+        setup_bench(max(1, min(args.steps, 20)))
+        return
     if args.traffic_probe:
         traffic_probe()
         return
