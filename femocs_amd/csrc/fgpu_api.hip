@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -128,6 +129,31 @@ fgpu::Sys::~Sys() {
 }
 
 // ------------------------------------------------------------------------------------------
+// scratch of the no-allocation entry points: one array per GPU, in the library's code object.
+// Its address on a device comes from hipGetSymbolAddress with that device current (the name
+// itself is only a host-side stand-in); every call on a device shares that device's array.
+// ------------------------------------------------------------------------------------------
+constexpr int kScratchMinBlocks = 65536;   // per-block minima of fgpu_mesh_displace_noalloc
+constexpr int kScratchDoubles = kScratchMinBlocks + 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double *g_scratch[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// the device address of `scratch` on the current device `device` (once per device)
+static double *scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) throw std::invalid_argument("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("hipGetSymbolAddress gave no address for the scratch array");
+        g_scratch[device] = static_cast<double *>(p);
+    }
+    return g_scratch[device];
+}
+
+// ------------------------------------------------------------------------------------------
 // context
 // ------------------------------------------------------------------------------------------
 extern "C" int fgpu_ctx_create(int device, fgpu_ctx **out) {
@@ -142,6 +168,7 @@ extern "C" int fgpu_ctx_create(int device, fgpu_ctx **out) {
         c->device = device;
         HIPCHECK(hipSetDevice(device));
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->scratch = scratch_ptr(device);
         return FGPU_OK;
     });
     if (rc != FGPU_OK) {
@@ -567,6 +594,42 @@ extern "C" int fgpu_mesh_displace(fgpu_mesh *m, const double *disp) {
         const double mn = min_center_det(m);
         if (!(mn >= 0.0)) {
             // back to the last accepted displacement (the base positions before the first)
+            if (!M.have_prev) launch_set_const(st, 3 * M.n_ctrl, 0.0, M.prev);
+            launch_motion(st, M.rows, M.idof, M.ptr, M.ctrl, M.w, M.base, M.prev, m->d_X64, m->d_X32);
+            HIPCHECK(hipStreamSynchronize(st));
+            throw MeshError("displacement turns a cell inside out (centre Jacobian " + std::to_string(mn) +
+                            "); coordinates left unchanged, the mesh must be rebuilt");
+        }
+        std::swap(M.disp, M.prev);
+        M.have_prev = true;
+        m->x_stale = true;
+        m->coord_gen++;
+        return FGPU_OK;
+    });
+}
+
+// fgpu_mesh_displace without allocation: the per-block minima of the centre Jacobian and their
+// minimum live in the device's scratch array (kScratchMinBlocks blocks of kBlock cells at
+// most).  Calls that overlap on one GPU race for that array.
+extern "C" int fgpu_mesh_displace_noalloc(fgpu_mesh *m, const double *disp) {
+    if (!m || !disp) return FGPU_EINVAL;
+    return guarded(m->ctx, [&] {
+        HIPCHECK(hipSetDevice(m->ctx->device));
+        hipStream_t st = m->ctx->stream;
+        Mesh::Motion &M = m->motion;
+        if (!M.idof) throw std::invalid_argument("no motion map set (fgpu_mesh_set_motion)");
+        const int nb = center_det_blocks(m->n_cells);
+        if (nb > kScratchMinBlocks)
+            throw std::invalid_argument("mesh too large for fgpu_mesh_displace_noalloc (more than 16777216 cells)");
+        double *sc = m->ctx->scratch;
+        h2d(M.disp, disp, 3 * (size_t)M.n_ctrl, st);
+        launch_motion(st, M.rows, M.idof, M.ptr, M.ctrl, M.w, M.base, M.disp, m->d_X64, m->d_X32);
+        launch_center_det_min(st, m->n_cells, m->d_cells, m->d_X64, sc);
+        launch_min_reduce(st, nb, sc, sc + kScratchMinBlocks);
+        double mn = 0.0;
+        d2h(&mn, sc + kScratchMinBlocks, 1, st);
+        HIPCHECK(hipStreamSynchronize(st));
+        if (!(mn >= 0.0)) {
             if (!M.have_prev) launch_set_const(st, 3 * M.n_ctrl, 0.0, M.prev);
             launch_motion(st, M.rows, M.idof, M.ptr, M.ctrl, M.w, M.base, M.prev, m->d_X64, m->d_X32);
             HIPCHECK(hipStreamSynchronize(st));

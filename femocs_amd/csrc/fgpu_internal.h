@@ -225,6 +225,7 @@ struct Sys {
 struct fgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    double *scratch = nullptr;    // device address of the library's scratch array on `device`
     std::string err;
     // batch tables reused between solves (grown on demand)
     void *d_tab = nullptr;

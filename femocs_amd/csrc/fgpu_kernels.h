@@ -50,6 +50,7 @@ void launch_motion(hipStream_t s, int rows, const int *idof, const int *ptr, con
                    const double *base, const double *disp, double *X64, float *X32);
 void launch_scatter_coords(hipStream_t s, int n, const int *idx, const double *xyz, double *X64, float *X32);
 void launch_center_det_min(hipStream_t s, int n, const int *cells, const double *X, double *out);
+void launch_min_reduce(hipStream_t s, int n, const double *in, double *out);
 void launch_grad_export(hipStream_t s, int n, const int *v2c, const int *v2n, const int *cells, const double *X,
                         const double *u, double *grad);
 }  // namespace fgpu

@@ -895,6 +895,25 @@ __global__ __launch_bounds__(kBlock) void k_center_det_min(int n, const int *__r
     }
 }
 
+// out[0] = min of in[0..n) (one workgroup; min is exact, so the order does not matter)
+__global__ __launch_bounds__(kBlock) void k_min_reduce(int n, const double *__restrict__ in, double *__restrict__ out) {
+    double v = 1e300;
+    for (int i = threadIdx.x; i < n; i += kBlock) v = fmin(v, in[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    __shared__ double s[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; ++k) v = fmin(v, s[k]);
+        v = fmin(v, s[0]);
+        out[0] = v;
+    }
+}
+void launch_min_reduce(hipStream_t s, int n, const double *in, double *out) {
+    hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(kBlock), 0, s, n, in, out);
+}
+
 // resistivity of bulk Cu, (T [K], rho [Ohm nm]) pairs of the reference's table
 // (src/PhysicalQuantityData.cpp:47-76); rho(T) = 10 * linear interpolation [Ohm A]
 __constant__ double kRhoT[26] = {200, 250, 273.15, 300, 350, 400, 450, 500, 550, 600, 650, 700, 750,

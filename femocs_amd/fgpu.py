@@ -125,6 +125,7 @@ API = [
     ("fgpu_mesh_move_vertices", C.c_int, [_P, C.c_int, _I, _D]),
     ("fgpu_mesh_set_motion", C.c_int, [_P, C.c_int, _I, _I, _I, _D, C.c_int]),
     ("fgpu_mesh_displace", C.c_int, [_P, _D]),
+    ("fgpu_mesh_displace_noalloc", C.c_int, [_P, _D]),
     ("fgpu_sys_reassemble", C.c_int, [_P]),
     ("fgpu_sys_save_charge_density", C.c_int, [_P]),
     ("fgpu_sys_add_space_charge", C.c_int, [_P, _P, C.c_int, _D, _I, C.c_double]),
@@ -270,6 +271,14 @@ class Mesh:
         if d.size != 3 * self.n_ctrl:
             raise ValueError("3 displacement components per control point")
         self.ctx.check(self.ctx.lib.fgpu_mesh_displace(self.h, d), "fgpu_mesh_displace")
+
+    def displace_noalloc(self, disp):
+        """displace() through the entry point that allocates nothing (the library's fixed
+        scratch array holds the validity check's partial minima)."""
+        d = np.ascontiguousarray(disp, np.float64).reshape(-1)
+        if d.size != 3 * self.n_ctrl:
+            raise ValueError("3 displacement components per control point")
+        self.ctx.check(self.ctx.lib.fgpu_mesh_displace_noalloc(self.h, d), "fgpu_mesh_displace_noalloc")
 
     def use_affine(self, allow: bool = True) -> bool:
         """Tet-split FP32 operator on or off; returns whether it is in use."""

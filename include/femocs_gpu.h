@@ -119,6 +119,14 @@ int fgpu_mesh_move_vertices(fgpu_mesh *mesh, int n, const int *vertex_ids, const
 int fgpu_mesh_set_motion(fgpu_mesh *mesh, int rows, const int *vertex_ids, const int *ptr, const int *ctrl,
                          const double *w, int n_ctrl);
 int fgpu_mesh_displace(fgpu_mesh *mesh, const double *disp);
+/* fgpu_mesh_displace without any allocation, no counterpart in the reference: for callers that
+ * cannot allocate device memory (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same inputs, results and errors as fgpu_mesh_displace; the partial
+ * minima of the validity check live in a fixed device array of the library (512 KiB per GPU,
+ * held until the process ends), so meshes of more than 16777216 cells are refused with
+ * FGPU_EINVAL.  Calls on one GPU share that array: calls that overlap on one GPU (several
+ * host threads on different contexts of the same device) race. */
+int fgpu_mesh_displace_noalloc(fgpu_mesh *mesh, const double *disp);
 /* device memory held by a mesh / a system (bytes), for admission by memory budget; no
  * counterpart in the reference */
 int fgpu_mesh_device_bytes(const fgpu_mesh *mesh, size_t *bytes);
