@@ -1,4 +1,4 @@
-"""Autotuner of the element kernel's work decomposition.
+"""Autotuner of the element kernel's work decomposition and of the CG's launch unrolling.
 
 The matrix-free apply processes patches of consecutive cells per workgroup (their nodes
 staged in LDS); two knobs trade LDS reuse against parallelism and latency hiding:
@@ -6,10 +6,18 @@ cells per patch (the elements-per-block of the element kernel) and patches handl
 by one workgroup.  The best pair depends on the mesh size and on the precision, so it is
 measured on the system's own data: each candidate times the element apply plus the dof
 gather that consumes its output (the patch size changes the gather's work too).
+
+The third knob is the unroll of a batched solve: how many CG iterations are captured in one
+hipGraph between two host convergence checks (more: fewer host round trips; fewer: less
+work after a system has converged).  It is timed on whole solves of the batch itself.
 """
 from __future__ import annotations
 
+import time
+
 from . import fgpu
+
+UNROLLS = (8, 16, 32, 64)
 
 CELLS = (64, 128, 256)
 PATCHES_PER_BLOCK = (1, 2, 4, 8)
@@ -32,3 +40,21 @@ def autotune(system: fgpu.System, precision: int = fgpu.FGPU_MIXED, cells=CELLS,
         mesh.set_patch_cells(best[0])
         system.set_patches_per_block(best[1])
     return best, table
+
+
+def autotune_unroll(ctx: fgpu.Context, systems, opts: fgpu.SolveOpts, unrolls=UNROLLS):
+    """Times a solve of the batch from zero for every unroll (CG iterations per captured graph);
+    returns (best unroll, {unroll: seconds}).  The systems are left at their last solution."""
+    table = {}
+    for u in unrolls:
+        opts.check_every = int(u)
+        opts.use_graph = 1
+        for s in systems:
+            s.fill_solution(0.0)
+        ctx.sync()
+        t = time.perf_counter()
+        st = ctx.solve_batch(systems, opts)
+        table[u] = time.perf_counter() - t
+        if min(x.iterations for x in st) <= 0:
+            raise RuntimeError("a batched solve did not converge while tuning the unroll")
+    return min(table, key=table.get), table

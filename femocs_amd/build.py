@@ -17,11 +17,11 @@ LIB = os.path.join(HERE, "libfemocs_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["fgpu_mesh.cpp", "fgpu_kernels.hip", "fgpu_api.hip", "fgpu_interp_host.cpp", "fgpu_interp.hip",
-           "fgpu_tethex.hip", "fgpu_import.hip"]
+           "fgpu_tethex.hip", "fgpu_import.hip", "fgpu_q2.hip"]
 # the transfer to atoms reproduces the reference's rounding: no floating-point contraction
 EXTRA = {"fgpu_interp_host.cpp": ["-ffp-contract=off"], "fgpu_interp.hip": ["-ffp-contract=off"],
          "fgpu_tethex.hip": ["-ffp-contract=off"], "fgpu_import.hip": ["-ffp-contract=off"]}
-HEADERS = ["fgpu_internal.h", "fgpu_kernels.h", "fgpu_interp.h", "fgpu_cg.inc", "fgpu_interp_build.inc"]
+HEADERS = ["fgpu_internal.h", "fgpu_kernels.h", "fgpu_interp.h", "fgpu_cg.inc", "fgpu_interp_build.inc", "fgpu_cheb.h"]
 
 # -packed-fp32-ops: the compiler emits no packed-FP32 (v_pk_*_f32) instructions
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",

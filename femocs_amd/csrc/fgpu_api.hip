@@ -13,6 +13,7 @@
 
 #include "fgpu_internal.h"
 #include "fgpu_kernels.h"
+#include "fgpu_cheb.h"
 
 using namespace fgpu;
 
@@ -126,6 +127,7 @@ fgpu::Sys::~Sys() {
     dfree(g); dfree(tmp); dfree(kq); dfree(gq); dfree(fslot); dfree(fval); dfree(d_dmask);
     dfree(r32); dfree(z32); dfree(p32); dfree(q32); dfree(d32); dfree(Ye32); dfree(invd32);
     dfree(kq32); dfree(gq32); dfree(scal); dfree(part); dfree(cinv); dfree(cc); dfree(rho);
+    dfree(cc64); dfree(cd); dfree(cres); dfree(ccoef); dfree(hist);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -213,7 +215,34 @@ static void ctx_release(fgpu_ctx *c) {
 // for the other translation units holding a context reference
 void fgpu_ctx_release_ref(fgpu_ctx *c) { ctx_release(c); }
 
+static void mesh_free(fgpu_mesh *m);
+// for the other translation units holding a mesh reference (a live Q2 system)
+void fgpu_mesh_release_ref(fgpu_mesh *m) {
+    if (--m->refs == 0 && m->closing) mesh_free(m);
+}
+
 extern "C" const char *fgpu_ctx_error(const fgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+// device memory for the caller (e.g. checkpoints of solutions), freed with fgpu_ctx_free
+extern "C" int fgpu_ctx_alloc(fgpu_ctx *c, size_t bytes, void **dev) {
+    if (!c || !dev) return FGPU_EINVAL;
+    *dev = nullptr;
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        HIPCHECK(hipMalloc(dev, std::max<size_t>(bytes, 1)));
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_ctx_free(fgpu_ctx *c, void *dev) {
+    if (!c) return FGPU_EINVAL;
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (dev) HIPCHECK(hipFree(dev));
+        return FGPU_OK;
+    });
+}
 
 extern "C" int fgpu_ctx_sync(fgpu_ctx *c) {
     if (!c) return FGPU_EINVAL;
@@ -761,6 +790,7 @@ static Desc make_desc64(fgpu_sys *s, bool residual) {
     d.b = s->b;
     d.scal = s->scal;
     d.part = s->part;
+    d.pc = FGPU_PC_JACOBI;
     return d;
 }
 
@@ -835,7 +865,9 @@ static Desc make_desc32(fgpu_sys *s, bool coarse = false) {
         d.cinv = s->cinv;
         d.nc = s->nc;
         d.agg_shift = s->agg_shift;
+        d.cc_in = 1;
     }
+    d.pc = FGPU_PC_JACOBI;
     return d;
 }
 
@@ -889,15 +921,30 @@ extern "C" int fgpu_sys_set_patches_per_block(fgpu_sys *s, int ppb) {
     return FGPU_OK;
 }
 
-extern "C" void fgpu_sys_destroy(fgpu_sys *s) {
-    if (!s) return;
+static void sys_free(fgpu_sys *s) {
     fgpu_ctx *c = s->ctx;
     fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    std::vector<Sys *> srcs;
+    for (auto &t : s->rhs_terms)
+        if (t.src) srcs.push_back(t.src);
     delete s;
+    for (Sys *src : srcs)
+        if (--src->term_refs == 0 && src->closing) sys_free(static_cast<fgpu_sys *>(src));
     if (--m->refs == 0 && m->closing) mesh_free(m);
     ctx_release(c);
+}
+
+// a system whose solution a recorded heat term of another system reads stays alive (marked)
+// until that term is dropped
+extern "C" void fgpu_sys_destroy(fgpu_sys *s) {
+    if (!s) return;
+    if (s->term_refs > 0) {
+        s->closing = true;
+        return;
+    }
+    sys_free(s);
 }
 
 // single-system tables for setup launches: [Desc][elem tab][dof tab] in a scratch buffer
@@ -1124,7 +1171,17 @@ extern "C" int fgpu_sys_heat_terms_theta(fgpu_sys *h, const fgpu_sys *cur, doubl
             h->kq32 = dalloc<float>(8 * nc); h->gq32 = dalloc<float>(8 * nc);
         }
         double *f = dalloc<double>(8 * nc);
-        h->rhs_terms.push_back({1, 0, f});
+        if (theta < 1) {
+            h->rhs_terms.push_back({1, 0, f});
+        } else {
+            // replayed by fgpu_sys_reassemble from the solution and potential at that time
+            Sys::RhsTerm term{2, 0, f};
+            term.src = const_cast<fgpu_sys *>(cur);
+            term.gamma = gamma;
+            term.lorentz = lorentz;
+            h->rhs_terms.push_back(term);
+            term.src->term_refs++;
+        }
         double *kx = theta < 1 ? dalloc<double>(8 * nc) : nullptr;
         launch_heat_terms(st, (int)nc, m->d_cells, m->d_X64, h->x, cur->x, gamma, lorentz, theta, h->kq, h->gq, h->kq32,
                           h->gq32, f, kx);
@@ -1169,7 +1226,12 @@ extern "C" int fgpu_sys_reset(fgpu_sys *s) {
         std::fill(s->dmask.begin(), s->dmask.end(), 0);
         std::fill(s->gval.begin(), s->gval.end(), 0.0);
         HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+        std::vector<Sys *> srcs;
+        for (auto &t : s->rhs_terms)
+            if (t.src) srcs.push_back(t.src);
         s->clear_terms();
+        for (Sys *src : srcs)
+            if (--src->term_refs == 0 && src->closing) sys_free(static_cast<fgpu_sys *>(src));
         return FGPU_OK;
     });
 }
@@ -1299,6 +1361,14 @@ extern "C" int fgpu_sys_reassemble(fgpu_sys *s) {
                                  m->d_X64, s->fslot);
                 launch_face_gather(st, m->n_dofs, m->d_fgptr, m->d_fgidx, s->fslot, s->b);
             } else {
+                if (term.kind == 2) {
+                    // heat step: coefficients and source from the present solution (the
+                    // previous temperature) and the present current potential
+                    if (term.src->mesh != s->mesh) throw std::invalid_argument("heat term of another mesh");
+                    launch_heat_terms(st, m->n_cells, m->d_cells, m->d_X64, s->x, term.src->x, term.gamma, term.lorentz,
+                                      1.0, s->kq, s->gq, s->kq32, s->gq32, term.dval, nullptr);
+                    s->diag_valid = false;
+                }
                 Desc d = make_desc64(s, true);
                 d.q = s->b;
                 BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
@@ -1375,6 +1445,28 @@ static int get_vec(fgpu_sys *s, const double *dev, double *out) {
         d2h(v.data(), dev, v.size(), s->ctx->stream);
         HIPCHECK(hipStreamSynchronize(s->ctx->stream));
         for (int i = 0; i < m->n_dofs; ++i) out[m->dof_of_idof[i]] = v[i];
+        return FGPU_OK;
+    });
+}
+
+// device-to-device copies of the solution in the library's internal dof order
+extern "C" int fgpu_sys_solution_to_device(fgpu_sys *s, double *dev) {
+    if (!s || !dev) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        const int n = static_cast<fgpu_mesh *>(s->mesh)->n_dofs;
+        HIPCHECK(hipMemcpyAsync(dev, s->x, n * sizeof(double), hipMemcpyDeviceToDevice, s->ctx->stream));
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_sys_solution_from_device(fgpu_sys *s, const double *dev) {
+    if (!s || !dev) return FGPU_EINVAL;
+    return guarded(s->ctx, [&] {
+        if (s->batches > 0) throw std::invalid_argument("the system is in a running batch");
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        const int n = static_cast<fgpu_mesh *>(s->mesh)->n_dofs;
+        HIPCHECK(hipMemcpyAsync(s->x, dev, n * sizeof(double), hipMemcpyDeviceToDevice, s->ctx->stream));
         return FGPU_OK;
     });
 }
@@ -1569,6 +1661,8 @@ static int build_coarse(fgpu_sys *s, int shift) {
     dfree(s->cc);
     s->cinv = dalloc<float>((size_t)nc * nc);
     s->cc = dalloc<float>(nc);
+    dfree(s->cc64);
+    s->cc64 = dalloc<double>(nc);
     launch_to_float(st, (size_t)nc * nc, dA, s->cinv);
     HIPCHECK(hipMemsetAsync(s->cc, 0, nc * sizeof(float), st));
     HIPCHECK(hipGetLastError());
@@ -1690,49 +1784,78 @@ static bool any_flag(fgpu_ctx *c, const Desc *D, int n, int slot, double want, d
     return false;
 }
 
+// preconditioner pieces of one solve's CG sets
+struct SolveKind {
+    bool mixed = false, coef = false;
+    int cheb_deg = 0;        // FP64 CG with the Chebyshev-Jacobi preconditioner of this degree
+    bool coarse64 = false;   // FP64 CG with the two-level correction
+};
+
+// the deg - 1 element applies and updates of the Chebyshev preconditioner (FP64 CG; D: the
+// systems' CG or residual descriptors, DC: the same with the Chebyshev search vector as input)
+static void cheb_steps(hipStream_t st, const Desc *D, const Desc *DC, const BatchTabs &t, bool coef, int deg) {
+    for (int j = 1; j < deg; ++j) {
+        launch_elem<double>(st, DC, t.etab, t.ne, coef, false, true, S_BETA, t.lds64);
+        launch_gather_cheb(st, D, t.dtab, t.nd, j);
+    }
+}
+
 // one CG iteration of every system of the set
 template <class T>
-static void cg_iter(hipStream_t st, const Desc *D, const BatchTabs &t, int n, bool coef, int reliable) {
-    launch_elem<T>(st, D, t.etab, t.ne, coef, false, true, S_BETA, sizeof(T) == 8 ? t.lds64 : t.lds32, t.affine32);
+static void cg_iter(hipStream_t st, const Desc *D, const Desc *DC, const BatchTabs &t, int n, const SolveKind &k,
+                    int reliable) {
+    launch_elem<T>(st, D, t.etab, t.ne, k.coef, false, true, S_BETA, sizeof(T) == 8 ? t.lds64 : t.lds32, t.affine32);
     launch_gather_cg<T>(st, D, t.dtab, t.nd);
     launch_scal_alpha(st, D, n);
     launch_update_cg<T>(st, D, t.dtab, t.nd);
-    if (sizeof(T) == 4) launch_coarse(st, D, nullptr, t.ctab, t.nct, 0, t.max_nc);
+    if (sizeof(T) == 4) {
+        launch_coarse(st, D, nullptr, t.ctab, t.nct, 0, t.max_nc);
+    } else {
+        if (k.coarse64) launch_coarse64(st, D, t.ctab, t.nct, t.max_nc);
+        if (k.cheb_deg > 1) cheb_steps(st, D, DC, t, k.coef, k.cheb_deg);
+    }
     launch_scal_beta(st, D, n, reliable);
 }
 
 // FP64 residual of the lifted system (R64: residual descriptors, u = x with the constrained
-// dofs zeroed) and the resulting state
-static void fp64_residual(hipStream_t st, const Desc *R64, const Desc *D32, const BatchTabs &t, int n, bool coef, int mixed) {
-    launch_elem<double>(st, R64, t.etab, t.ne, coef, true, true, S_BETA, t.lds64);
+// dofs zeroed) and the resulting state; in an FP64 solve also the first preconditioned
+// residual (coarse correction or Chebyshev steps)
+static void fp64_residual(hipStream_t st, const Desc *R64, const Desc *D32, const Desc *DC, const BatchTabs &t, int n,
+                          const SolveKind &k, int mixed) {
+    launch_elem<double>(st, R64, t.etab, t.ne, k.coef, true, true, S_BETA, t.lds64);
     launch_gather_res<double>(st, R64, t.dtab, t.nd);
+    if (!mixed) {
+        if (k.coarse64) launch_coarse64(st, R64, t.ctab, t.nct, t.max_nc);
+        if (k.cheb_deg > 1) cheb_steps(st, R64, DC, t, k.coef, k.cheb_deg);
+    }
     launch_scal_res(st, R64, D32, n, mixed);
 }
 
 // reliable update of the systems whose FP32 CG waits: fold the FP32 correction into x64,
 // recompute the FP64 residual, resume the FP32 CG from it
-static void reliable_update(hipStream_t st, const Desc *D32, const Desc *R64, const BatchTabs &t, int n, bool coef) {
+static void reliable_update(hipStream_t st, const Desc *D32, const Desc *R64, const BatchTabs &t, int n,
+                            const SolveKind &k) {
     launch_mixed_acc(st, D32, R64, t.dtab, t.nd);
-    fp64_residual(st, R64, D32, t, n, coef, 1);
+    fp64_residual(st, R64, D32, nullptr, t, n, k, 1);
     launch_mixed_restart(st, D32, R64, t.dtab, t.nd, 0);
     launch_coarse(st, D32, R64, t.ctab, t.nct, 2, t.max_nc);
     launch_scal_restart(st, D32, R64, n, 0);
 }
 
 struct ChunkArgs {
-    bool mixed, coef;
+    SolveKind kind;
     int k, u, n;
-    const Desc *D64, *R64, *D32;
+    const Desc *D64, *R64, *D32, *DC;
     const BatchTabs *t;
 };
 
 static void chunk(hipStream_t st, const ChunkArgs &a) {
     for (int j = 0; j < a.k; ++j) {
-        if (!a.mixed) {
-            cg_iter<double>(st, a.D64, *a.t, a.n, a.coef, 0);
+        if (!a.kind.mixed) {
+            cg_iter<double>(st, a.D64, a.DC, *a.t, a.n, a.kind, 0);
         } else {
-            cg_iter<float>(st, a.D32, *a.t, a.n, a.coef, 1);
-            if ((j + 1) % a.u == 0) reliable_update(st, a.D32, a.R64, *a.t, a.n, a.coef);
+            cg_iter<float>(st, a.D32, nullptr, *a.t, a.n, a.kind, 1);
+            if ((j + 1) % a.u == 0) reliable_update(st, a.D32, a.R64, *a.t, a.n, a.kind);
         }
     }
 }
@@ -1748,21 +1871,70 @@ static bool batch_coef(const std::vector<fgpu_sys *> &sys, fgpu_ctx *c) {
     return coef;
 }
 
-static void make_sets(const std::vector<fgpu_sys *> &sys, std::vector<std::vector<Desc>> &sets) {
-    sets.assign(3, {});
+static void check_precond(const fgpu_solve_opts *o) {
+    if (o->precond != FGPU_PC_NONE && o->precond != FGPU_PC_JACOBI && o->precond != FGPU_PC_CHEBYSHEV)
+        throw std::invalid_argument("unknown preconditioner");
+    if (o->precond == FGPU_PC_CHEBYSHEV && o->precision != FGPU_FP64)
+        throw std::invalid_argument("the Chebyshev-Jacobi preconditioner is for FGPU_FP64 solves");
+}
+
+// Descriptor sets of a solve: [0] FP64 CG (input z + beta p), [1] FP64 residual (input x),
+// [2] FP32 CG of the mixed solver, [3] FP64 Chebyshev apply (input: the Chebyshev search
+// vector).  The FP64 sets carry the solve's preconditioner: identity, Jacobi (plus the
+// two-level correction of systems that have one) or Chebyshev-Jacobi of degree cheb_deg; the
+// FP32 CG uses Jacobi plus the two-level correction unless no preconditioner is asked for.
+// hist: record the first CG coefficients (Lanczos estimate).
+static SolveKind make_sets(const std::vector<fgpu_sys *> &sys, std::vector<std::vector<Desc>> &sets,
+                           const fgpu_solve_opts *o, int cheb_deg = 0, bool hist = false) {
+    SolveKind k;
+    k.mixed = o->precision == FGPU_MIXED;
+    sets.assign(4, {});
     for (auto *s : sys) {
-        sets[0].push_back(make_desc64(s, false));
-        sets[1].push_back(make_desc64(s, true));
-        sets[2].push_back(make_desc32(s, true));
+        Desc d0 = make_desc64(s, false), d1 = make_desc64(s, true), d2 = make_desc32(s, true), dc = make_desc64(s, false);
+        if (!k.mixed) {
+            d0.pc = d1.pc = o->precond;
+            if (o->precond == FGPU_PC_CHEBYSHEV) {
+                for (Desc *d : {&d0, &d1}) {
+                    d->cd = s->cd;
+                    d->cres = s->cres;
+                    d->ccoef = s->ccoef;
+                    d->cheb_deg = cheb_deg;
+                }
+                k.cheb_deg = cheb_deg;
+            }
+            if (o->precond == FGPU_PC_JACOBI && s->coarse) {
+                for (Desc *d : {&d0, &d1}) {
+                    d->cc = s->cc64;
+                    d->cinv = s->cinv;
+                    d->nc = s->nc;
+                    d->agg_shift = s->agg_shift;
+                }
+                d0.cc_in = 1;          // the residual set's element input is x, not z
+                k.coarse64 = true;
+            }
+            if (hist) d0.hist = s->hist;
+        } else if (o->precond == FGPU_PC_NONE) {
+            d2.pc = FGPU_PC_NONE;
+            d2.cc = nullptr;
+        }
+        dc.ua = s->cd;
+        dc.ub = nullptr;
+        dc.cc = nullptr;
+        sets[0].push_back(d0);
+        sets[1].push_back(d1);
+        sets[2].push_back(d2);
+        sets[3].push_back(dc);
     }
+    return k;
 }
 
 // Starts the solves of a set of systems: scalars, x = g on constrained dofs, FP64 residual of
-// the lifted system and (mixed) the FP32 CG state.  t: the set's uploaded tables.
+// the lifted system and the first preconditioned residual (FP64), or the FP32 CG state
+// (mixed).  t: the set's uploaded tables.
 static void start_solves(fgpu_ctx *c, const std::vector<fgpu_sys *> &sys, const fgpu_solve_opts *o, const BatchTabs &t,
-                         bool coef) {
+                         const SolveKind &k) {
     const int n = (int)sys.size();
-    const bool mixed = o->precision == FGPU_MIXED;
+    const bool mixed = k.mixed;
     const int max_it = std::max(0, o->max_it);
     const double delta = o->inner_tol > 0 && o->inner_tol < 1 ? o->inner_tol : 0.1;
     std::vector<double> h((size_t)2 * kScal * n, 0.0);
@@ -1781,8 +1953,8 @@ static void start_solves(fgpu_ctx *c, const std::vector<fgpu_sys *> &sys, const 
         fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
         launch_enforce_dirichlet(c->stream, m->n_dofs, s->d_dmask, s->g, s->x);
     }
-    const Desc *R64 = t.sets[1], *D32 = t.sets[2];
-    fp64_residual(c->stream, R64, D32, t, n, coef, mixed ? 1 : 0);
+    const Desc *R64 = t.sets[1], *D32 = t.sets[2], *DC = t.sets[3];
+    fp64_residual(c->stream, R64, D32, DC, t, n, k, mixed ? 1 : 0);
     if (mixed) {
         launch_mixed_restart(c->stream, D32, R64, t.dtab, t.nd, 1);
         launch_coarse(c->stream, D32, R64, t.ctab, t.nct, 1, t.max_nc);
@@ -1802,21 +1974,23 @@ static void read_stats(fgpu_sys *s, bool mixed, fgpu_solve_stats *st) {
     st->rhs_norm = std::sqrt(std::max(0.0, s64[S_BB]));
 }
 
-static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_solve_opts *o, fgpu_solve_stats *stats) {
-    HIPCHECK(hipSetDevice(c->device));
-    std::vector<fgpu_sys *> sys(systems, systems + n);
+// CG of every system to its own convergence, chunks of iterations between host checks
+static void run_solves(fgpu_ctx *c, const std::vector<fgpu_sys *> &sys, const fgpu_solve_opts *o, int cheb_deg, bool hist,
+                       fgpu_solve_stats *stats) {
+    const int n = (int)sys.size();
+    if (n == 0) return;
     const bool coef = batch_coef(sys, c);
-    const bool mixed = o->precision == FGPU_MIXED;
     const int k = std::max(2, o->check_every > 0 ? o->check_every : 32);
     const int max_it = std::max(0, o->max_it);
     for (auto *s : sys) ensure_diag(s);
     std::vector<std::vector<Desc>> sets;
-    make_sets(sys, sets);
+    SolveKind kind = make_sets(sys, sets, o, cheb_deg, hist);
+    kind.coef = coef;
     BatchTabs t = upload_batch(c, sets, sys);
-    start_solves(c, sys, o, t, coef);
+    start_solves(c, sys, o, t, kind);
     double *dflags = nullptr;
     HIPCHECK(hipMalloc(&dflags, std::max(1, n) * sizeof(double)));
-    ChunkArgs a{mixed, coef, k, 8, n, t.sets[0], t.sets[1], t.sets[2], &t};
+    ChunkArgs a{kind, k, 8, n, t.sets[0], t.sets[1], t.sets[2], t.sets[3], &t};
     // a system waiting for its reliable update idles at most u iterations per update
     const int max_chunks = 2 * (max_it / k) + 16;
     hipGraphExec_t exec = nullptr;
@@ -1841,7 +2015,98 @@ static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_
     HIPCHECK(hipStreamSynchronize(c->stream));
     HIPCHECK(hipFree(dflags));
     if (stats)
-        for (int i = 0; i < n; ++i) read_stats(sys[i], mixed, &stats[i]);
+        for (int i = 0; i < n; ++i) read_stats(sys[i], kind.mixed, &stats[i]);
+}
+
+static int solve_batch(fgpu_ctx *c, int n, fgpu_sys *const *systems, const fgpu_solve_opts *o, fgpu_solve_stats *stats) {
+    HIPCHECK(hipSetDevice(c->device));
+    check_precond(o);
+    std::vector<fgpu_sys *> sys(systems, systems + n);
+    if (o->precond != FGPU_PC_CHEBYSHEV) {
+        run_solves(c, sys, o, 0, false, stats);
+        return FGPU_OK;
+    }
+    // Chebyshev-Jacobi: lambda_max(D^-1 A) of every system whose coordinates changed since its
+    // last estimate, from the Lanczos matrix of kLanczosSteps Jacobi-PCG iterations (which
+    // also start its solve), then the CG with the Chebyshev preconditioner from that iterate
+    std::vector<fgpu_solve_stats> st(n), pre(n);
+    std::vector<char> done(n, 0);
+    std::vector<int> est;
+    for (int i = 0; i < n; ++i) {
+        fgpu_sys *s = sys[i];
+        const fgpu_mesh *m = static_cast<const fgpu_mesh *>(s->mesh);
+        if (!(s->cheb_lmax > 0) || s->cheb_gen != m->coord_gen || !s->diag_valid) est.push_back(i);
+        if (!s->cd) {
+            s->cd = dalloc<double>(m->n_dofs);
+            s->cres = dalloc<double>(m->n_dofs);
+            s->ccoef = dalloc<double>(2 * kChebMax + 1);
+        }
+    }
+    if (!est.empty()) {
+        std::vector<fgpu_sys *> es;
+        for (int i : est) {
+            if (!sys[i]->hist) sys[i]->hist = dalloc<double>(2 * kHist);
+            es.push_back(sys[i]);
+        }
+        fgpu_solve_opts oj = *o;
+        oj.precond = FGPU_PC_JACOBI;
+        oj.max_it = std::min(o->max_it, kLanczosSteps);
+        oj.check_every = kLanczosSteps;
+        std::vector<fgpu_solve_stats> es_st(es.size());
+        run_solves(c, es, &oj, 0, true, es_st.data());
+        for (size_t e = 0; e < es.size(); ++e) {
+            const int i = est[e];
+            fgpu_sys *s = sys[i];
+            pre[i] = es_st[e];
+            const int m_its = std::abs(es_st[e].iterations);
+            if (es_st[e].iterations > 0 || m_its >= o->max_it) {
+                done[i] = 1;       // converged (or out of iterations) in the Jacobi steps
+                continue;
+            }
+            double h[2 * kHist];
+            HIPCHECK(hipMemcpy(h, s->hist, sizeof(h), hipMemcpyDeviceToHost));
+            const double lm = lanczos_lmax(h, m_its);
+            if (!(lm > 0) || !std::isfinite(lm)) throw std::runtime_error("Lanczos estimate of lambda_max failed");
+            s->cheb_lmax = lm;
+            s->cheb_gen = static_cast<const fgpu_mesh *>(s->mesh)->coord_gen;
+        }
+    }
+    std::vector<fgpu_sys *> rs;
+    std::vector<int> ri;
+    int nmax = 0;
+    for (int i = 0; i < n; ++i)
+        if (!done[i]) {
+            rs.push_back(sys[i]);
+            ri.push_back(i);
+            nmax = std::max(nmax, static_cast<const fgpu_mesh *>(sys[i]->mesh)->n_dofs);
+        }
+    const int deg = o->cheb_degree > 0 ? std::min(o->cheb_degree, kChebMax) : cheb_auto_degree(nmax);
+    for (fgpu_sys *s : rs) {
+        std::vector<double> cf;
+        cheb_coefficients(s->cheb_lmax, deg, cf);
+        stage_h2d(c, s->ccoef, cf.data(), cf.size() * sizeof(double));
+    }
+    if (!rs.empty()) {
+        fgpu_solve_opts oc = *o;
+        int used = 0;
+        for (int i : ri) used = std::max(used, std::abs(pre[i].iterations));
+        oc.max_it = std::max(1, o->max_it - used);
+        std::vector<fgpu_solve_stats> rs_st(rs.size());
+        run_solves(c, rs, &oc, deg, false, rs_st.data());
+        for (size_t e = 0; e < rs.size(); ++e) st[ri[e]] = rs_st[e];
+    }
+    if (stats)
+        for (int i = 0; i < n; ++i) {
+            if (done[i]) {
+                stats[i] = pre[i];
+                continue;
+            }
+            fgpu_solve_stats r = st[i];
+            const int p = std::abs(pre[i].iterations);
+            r.iterations = r.iterations > 0 ? r.iterations + p : r.iterations - p;
+            r.outer += pre[i].outer;
+            stats[i] = r;
+        }
     return FGPU_OK;
 }
 
@@ -1877,6 +2142,7 @@ struct fgpu_batch {
     bool last_valid = false;
     std::vector<fgpu_sys *> joined;   // added while a step was in flight (not its members, even
                                       // at the address of one that left)
+    SolveKind kind;                   // preconditioner pieces of the uploaded sets
 };
 
 static void batch_drop_graph(fgpu_batch *b) {
@@ -1888,6 +2154,11 @@ static void batch_drop_graph(fgpu_batch *b) {
 
 extern "C" int fgpu_batch_create(fgpu_ctx *c, const fgpu_solve_opts *o, fgpu_batch **out) {
     if (!c || !o || !out) return FGPU_EINVAL;
+    if (o->precond != FGPU_PC_NONE && o->precond != FGPU_PC_JACOBI) {
+        c->err = "continuous batches take FGPU_PC_NONE or FGPU_PC_JACOBI (the two-level correction of members "
+                 "that have one included)";
+        return FGPU_EINVAL;
+    }
     fgpu_batch *b = new (std::nothrow) fgpu_batch();
     if (!b) return FGPU_ENOMEM;
     b->ctx = c;
@@ -1925,9 +2196,10 @@ extern "C" int fgpu_batch_add(fgpu_batch *b, fgpu_sys *s) {
         if (b->coef >= 0 && (int)coef != b->coef) throw std::invalid_argument("a batch mixes coefficient and plain systems");
         ensure_diag(s);
         std::vector<std::vector<Desc>> sets;
-        make_sets({s}, sets);
+        SolveKind kind = make_sets({s}, sets, &b->o);
+        kind.coef = coef;
         BatchTabs t = upload_batch(c, sets, {s});
-        start_solves(c, {s}, &b->o, t, coef);
+        start_solves(c, {s}, &b->o, t, kind);
         for (auto &e : b->last)
             if (e.first == s) e.first = nullptr;
         if (b->pending) b->joined.push_back(s);
@@ -1950,7 +2222,8 @@ static void batch_launch(fgpu_batch *b, int iterations) {
         if (b->dirty) {
             batch_drop_graph(b);
             std::vector<std::vector<Desc>> sets;
-            make_sets(b->sys, sets);
+            b->kind = make_sets(b->sys, sets, &b->o);
+            b->kind.coef = b->coef > 0;
             b->t = upload_batch(c, sets, b->sys, &b->dtab, &b->cap);
             if (n > b->flags_cap) {
                 if (b->dflags) HIPCHECK(hipFree(b->dflags));
@@ -1960,7 +2233,7 @@ static void batch_launch(fgpu_batch *b, int iterations) {
             }
             b->dirty = false;
         }
-        ChunkArgs a{mixed, b->coef > 0, k, 8, n, b->t.sets[0], b->t.sets[1], b->t.sets[2], &b->t};
+        ChunkArgs a{b->kind, k, 8, n, b->t.sets[0], b->t.sets[1], b->t.sets[2], b->t.sets[3], &b->t};
         if (b->o.use_graph) {
             if (!b->exec || b->graph_k != k) {
                 batch_drop_graph(b);
@@ -2066,7 +2339,7 @@ extern "C" int fgpu_batch_time_kernel(fgpu_batch *b, int which, int reps, double
         HIPCHECK(hipSetDevice(c->device));
         if (b->sys.empty()) throw std::invalid_argument("empty batch");
         std::vector<std::vector<Desc>> sets;
-        make_sets(b->sys, sets);
+        make_sets(b->sys, sets, &b->o);
         BatchTabs t = upload_batch(c, sets, b->sys);
         const bool mixed = b->o.precision == FGPU_MIXED;
         *avg_ms = time_launches(c, mixed ? t.sets[2] : t.sets[1], t, b->coef > 0, which, b->o.precision, reps);

@@ -102,6 +102,13 @@ struct Desc {
     void *cc;
     const float *cinv;
     int nc, agg_shift;            // aggregates and log2 of their size
+    int cc_in;                    // the element input is z: add its coarse part (CG sets only;
+                                  // a residual set's input is the iterate)
+    // preconditioner of this descriptor's CG: 0 identity, 1 Jacobi, 2 Chebyshev-Jacobi
+    int pc, cheb_deg;
+    double *hist;                 // alpha_j, beta_j of the first kHist iterations, or null
+    double *cd, *cres;            // FP64 Chebyshev search vector and residual
+    const double *ccoef;          // 1/theta, then (c1_j, c2_j) for j = 1..cheb_deg-1
 };
 // Element-kernel work item: one workgroup, starting at patch p of system sys, with that
 // patch's first cell and node and its cell and node counts (nc = 0: nothing to do), so the
@@ -109,6 +116,8 @@ struct Desc {
 struct ETab {
     int sys, p, c0, n0, nc, np, pad0, pad1;
 };
+constexpr int kHist = 32;         // CG coefficients recorded per system for the Lanczos estimate
+constexpr int kChebMax = 12;      // largest Chebyshev degree
 constexpr int kPartRegions = 5 + kAggPerBlock;   // per descriptor set: p.q, r.z, r.r, b.b, s.c,
                                                  // aggregate sums of r (kAggPerBlock regions)
 constexpr int kCoarseRows = 16;   // coarse rows per workgroup of the coarse product
@@ -195,9 +204,13 @@ struct Sys {
     // right-hand-side terms since the last reset, replayed by fgpu_sys_reassemble after the
     // vertices moved: Neumann face values (one per boundary face) or cell values (8 per cell)
     struct RhsTerm {
-        int kind;                 // 0: Neumann on boundary id `bid`, 1: cell source
+        int kind;                 // 0: Neumann on boundary id `bid`, 1: cell source, 2: heat terms
         int bid;
-        double *dval;
+        double *dval;             // face values (kind 0) or cell source, 8 per cell (kinds 1, 2)
+        // kind 2 (implicit-Euler heat step): coefficients and Joule source recomputed from the
+        // system's own solution (the previous temperature) and the current potential of `src`
+        Sys *src = nullptr;
+        double gamma = 0, lorentz = 0;
     };
     std::vector<RhsTerm> rhs_terms;
     bool lifted = false;          // fgpu_sys_apply_dirichlet ran since the last reset
@@ -216,6 +229,19 @@ struct Sys {
     float *cinv = nullptr;        // nc x nc
     float *cc = nullptr;          // nc
     bool desc32_affine = false;   // the last FP32 descriptor built uses the tet-split form
+    // replayable heat terms of other systems that read this one's solution: while any is
+    // recorded, fgpu_sys_destroy only marks the system and the last such term frees it
+    int term_refs = 0;
+    bool closing = false;
+    // two-level preconditioner in the FP64 CG (coarse correction in FP64, same E^-1)
+    double *cc64 = nullptr;
+    // Chebyshev-Jacobi preconditioner of the FP64 CG: search vectors d and the Chebyshev
+    // residual, coefficients [1/theta, (c1_j, c2_j) for j = 1..degree-1] on the device, the
+    // estimate of lambda_max(D^-1 A) and the coordinate generation it belongs to
+    double *cd = nullptr, *cres = nullptr, *ccoef = nullptr;
+    double *hist = nullptr;       // CG alpha_j, beta_j of the first kHist iterations (Lanczos)
+    double cheb_lmax = 0;
+    int cheb_gen = -1;
     void clear_terms();
     ~Sys();
 };

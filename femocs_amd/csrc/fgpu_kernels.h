@@ -22,6 +22,8 @@ void launch_mixed_restart(hipStream_t s, const Desc *D32, const Desc *D64, const
 void launch_mixed_acc(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk);
 void launch_collect(hipStream_t s, const Desc *D, int n, int slot, double *out);
 void launch_coarse(hipStream_t s, const Desc *D32, const Desc *D64, const int2 *tab, int nblk, int mode, int max_nc);
+void launch_coarse64(hipStream_t s, const Desc *D, const int2 *tab, int nblk, int max_nc);
+void launch_gather_cheb(hipStream_t s, const Desc *D, const int2 *tab, int nblk, int j);
 void launch_coarse_indicator(hipStream_t s, int n, const unsigned char *mask, const int *color, int shift, int c,
                              double *u);
 void launch_coarse_sums(hipStream_t s, int n, const unsigned char *mask, const double *q, double *out);

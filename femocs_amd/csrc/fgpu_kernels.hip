@@ -569,14 +569,15 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
         unsigned char mk[kRounds];
         V4 xs[kRounds];
         const auto ubp = G(ub ? ub : reinterpret_cast<const T *>(d.ua));   // never read through null
-        const auto ccp = G(d.cc ? reinterpret_cast<const T *>(d.cc) : reinterpret_cast<const T *>(d.ua));
+        const bool cin = d.cc != nullptr && d.cc_in != 0;
+        const auto ccp = G(cin ? reinterpret_cast<const T *>(d.cc) : reinterpret_cast<const T *>(d.ua));
 #pragma unroll
         for (int r = 0; r < kRounds; ++r) {
             if (!affine) xs[r] = ldg4<V4, T>(X + nd[r]);
             if (MODE == EM_APPLY) {
                 va[r] = ua[nd[r]];
                 vb[r] = ubp[nd[r]];
-                vc[r] = ccp[nd[r] >> d.agg_shift];
+                vc[r] = ccp[cin ? nd[r] >> d.agg_shift : 0];
                 mk[r] = G(d.dmask)[nd[r]];
             }
         }
@@ -592,7 +593,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
                 if (MODE == EM_APPLY) {
                     T v = va[r];
                     // two-level FP32 CG: z carries its coarse part where it is read
-                    if (d.cc) v += vc[r] * (T)(1 - (int)mk[r]);
+                    if (cin) v += vc[r] * (T)(1 - (int)mk[r]);
                     if (ub) v += beta * vb[r];
                     if (MASKIN && mk[r]) v = 0;
                     su[j] = v;

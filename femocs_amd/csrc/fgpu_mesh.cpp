@@ -191,8 +191,15 @@ void build_patches(int nc, int nd, const std::vector<int> &icells, int cells_per
         const int p = P.n, c_start = c;
         int np = 0;
         while (c < nc && c - c_start < cells_per_patch) {
+            // distinct vertices of the cell not yet in the patch (a vertex repeated inside a
+            // collapsed cell counts once, as in the device import)
             int fresh = 0;
-            for (int k = 0; k < 8; ++k) fresh += stamp[icells[8 * (size_t)c + k]] != p;
+            for (int k = 0; k < 8; ++k) {
+                const int d = icells[8 * (size_t)c + k];
+                bool repeat = false;
+                for (int j = 0; j < k; ++j) repeat = repeat || icells[8 * (size_t)c + j] == d;
+                fresh += stamp[d] != p && !repeat;
+            }
             if (np + fresh > cap) break;
             for (int k = 0; k < 8; ++k) {
                 const int d = icells[8 * (size_t)c + k];

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfemocs_gpu.so")
 
 FGPU_FP64, FGPU_MIXED = 0, 1
-FGPU_PC_NONE, FGPU_PC_JACOBI = 0, 1
+FGPU_PC_NONE, FGPU_PC_JACOBI, FGPU_PC_CHEBYSHEV = 0, 1, 2
 FGPU_IMPORT_HOST = 1
 FGPU_MESH_DIGEST_TABLES = 32
 FGPU_INTERP_HOST = 1
@@ -41,7 +41,7 @@ class MeshInvertedError(FgpuError):
 class SolveOpts(C.Structure):
     _fields_ = [("max_it", C.c_int), ("tol", C.c_double), ("relative", C.c_int), ("precond", C.c_int),
                 ("precision", C.c_int), ("inner_tol", C.c_double), ("check_every", C.c_int),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("cheb_degree", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -61,6 +61,8 @@ API = [
     ("fgpu_ctx_destroy", None, [_P]),
     ("fgpu_ctx_error", C.c_char_p, [_P]),
     ("fgpu_ctx_sync", C.c_int, [_P]),
+    ("fgpu_ctx_alloc", C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    ("fgpu_ctx_free", C.c_int, [_P, _P]),
     ("fgpu_mesh_import", C.c_int, [_P, C.c_int, _D, C.c_int, _I, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
     ("fgpu_mesh_import_ex", C.c_int, [_P, C.c_int, _D, C.c_int, _I, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.POINTER(_P)]),
@@ -82,6 +84,8 @@ API = [
     ("fgpu_sys_fill_solution", C.c_int, [_P, C.c_double]),
     ("fgpu_sys_set_solution", C.c_int, [_P, _D]),
     ("fgpu_sys_get_solution", C.c_int, [_P, _D]),
+    ("fgpu_sys_solution_to_device", C.c_int, [_P, C.c_void_p]),
+    ("fgpu_sys_solution_from_device", C.c_int, [_P, C.c_void_p]),
     ("fgpu_sys_get_rhs", C.c_int, [_P, _D]),
     ("fgpu_sys_apply", C.c_int, [_P, _D, _D]),
     ("fgpu_sys_solve", C.c_int, [_P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
@@ -140,6 +144,20 @@ API = [
     ("fgpu_batch_retire", C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(SolveStats), C.POINTER(C.c_int)]),
     ("fgpu_batch_step_async", C.c_int, [_P, C.c_int]),
     ("fgpu_batch_wait", C.c_int, [_P, C.POINTER(C.c_int)]),
+    ("fgpu_q2_create", C.c_int, [_P, C.POINTER(_P)]),
+    ("fgpu_q2_destroy", None, [_P]),
+    ("fgpu_q2_sizes", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("fgpu_q2_export_dofs", C.c_int, [_P, C.c_void_p, C.c_void_p]),
+    ("fgpu_q2_reset", C.c_int, [_P]),
+    ("fgpu_q2_add_dirichlet", C.c_int, [_P, C.c_int, C.c_double]),
+    ("fgpu_q2_add_neumann", C.c_int, [_P, C.c_int, C.c_double]),
+    ("fgpu_q2_apply_dirichlet", C.c_int, [_P]),
+    ("fgpu_q2_fill_solution", C.c_int, [_P, C.c_double]),
+    ("fgpu_q2_solve", C.c_int, [_P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    ("fgpu_q2_get_solution", C.c_int, [_P, _D]),
+    ("fgpu_q2_get_rhs", C.c_int, [_P, _D]),
+    ("fgpu_q2_apply", C.c_int, [_P, _D, _D]),
+    ("fgpu_q2_time_apply", C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
 ]
 
 
@@ -180,6 +198,10 @@ class Context:
     def sync(self):
         self.check(self.lib.fgpu_ctx_sync(self.h), "sync")
 
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        """Device memory on the context's device, freed when the buffer is dropped."""
+        return DeviceBuffer(self, nbytes)
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.fgpu_ctx_destroy(self.h)
@@ -197,6 +219,25 @@ class Context:
         stats = (SolveStats * n)()
         self.check(self.lib.fgpu_solve_batch(self.h, n, arr, C.byref(opts), stats), "fgpu_solve_batch")
         return [stats[i] for i in range(n)]
+
+
+class DeviceBuffer:
+    """A block of device memory from fgpu_ctx_alloc (ptr: its address)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        p = _P()
+        ctx.check(ctx.lib.fgpu_ctx_alloc(ctx.h, C.c_size_t(int(nbytes)), C.byref(p)), "fgpu_ctx_alloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self.ctx.lib.fgpu_ctx_free(self.ctx.h, _P(self.ptr))
+                self.ptr = None
+        except Exception:
+            pass
 
 
 class Mesh:
@@ -391,6 +432,15 @@ class System:
         self._c(self.ctx.lib.fgpu_sys_get_solution(self.h, x), "get_solution")
         return x
 
+    def solution_to_device(self, dev_ptr: int):
+        """Copies the solution (internal dof order) to device memory at dev_ptr (n_dofs
+        doubles), ordered on the context's stream."""
+        self._c(self.ctx.lib.fgpu_sys_solution_to_device(self.h, C.c_void_p(int(dev_ptr))), "solution_to_device")
+
+    def solution_from_device(self, dev_ptr: int):
+        """Sets the solution from device memory written by solution_to_device on this mesh."""
+        self._c(self.ctx.lib.fgpu_sys_solution_from_device(self.h, C.c_void_p(int(dev_ptr))), "solution_from_device")
+
     def rhs(self):
         b = np.zeros(self.mesh.n_dofs)
         self._c(self.ctx.lib.fgpu_sys_get_rhs(self.h, b), "get_rhs")
@@ -516,6 +566,79 @@ class Batch:
         return res
 
 
+class Q2System:
+    """The Laplace system with Q2 elements on a mesh's hexes (fgpu_q2_*): dofs are the mesh's
+    vertices (same order as Mesh.dofs), then edge midpoints, face centres and cell centres."""
+
+    def __init__(self, mesh: Mesh):
+        self.ctx = mesh.ctx
+        self.mesh = mesh
+        h = _P()
+        self.ctx.check(self.ctx.lib.fgpu_q2_create(mesh.h, C.byref(h)), "fgpu_q2_create")
+        self.h = h
+        n, c = C.c_int(), C.c_int()
+        self.ctx.lib.fgpu_q2_sizes(h, C.byref(n), C.byref(c))
+        self.n_dofs, self.n_cells = n.value, c.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.fgpu_q2_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _c(self, rc, what):
+        self.ctx.check(rc, what)
+
+    def dofs(self):
+        """(support points (n, 3), 27 dofs per cell in import cell order)."""
+        xyz = np.zeros((self.n_dofs, 3))
+        cells = np.zeros((self.n_cells, 27), np.int32)
+        self._c(self.ctx.lib.fgpu_q2_export_dofs(self.h, xyz.ctypes.data, cells.ctypes.data), "fgpu_q2_export_dofs")
+        return xyz, cells
+
+    def reset(self):
+        self._c(self.ctx.lib.fgpu_q2_reset(self.h), "fgpu_q2_reset")
+
+    def add_dirichlet(self, bid, value):
+        self._c(self.ctx.lib.fgpu_q2_add_dirichlet(self.h, int(bid), float(value)), "fgpu_q2_add_dirichlet")
+
+    def add_neumann(self, bid, value):
+        self._c(self.ctx.lib.fgpu_q2_add_neumann(self.h, int(bid), float(value)), "fgpu_q2_add_neumann")
+
+    def apply_dirichlet(self):
+        self._c(self.ctx.lib.fgpu_q2_apply_dirichlet(self.h), "fgpu_q2_apply_dirichlet")
+
+    def fill_solution(self, v):
+        self._c(self.ctx.lib.fgpu_q2_fill_solution(self.h, float(v)), "fgpu_q2_fill_solution")
+
+    def solve(self, opts: SolveOpts) -> SolveStats:
+        st = SolveStats()
+        self._c(self.ctx.lib.fgpu_q2_solve(self.h, C.byref(opts), C.byref(st)), "fgpu_q2_solve")
+        return st
+
+    def solution(self):
+        x = np.zeros(self.n_dofs)
+        self._c(self.ctx.lib.fgpu_q2_get_solution(self.h, x), "fgpu_q2_get_solution")
+        return x
+
+    def rhs(self):
+        b = np.zeros(self.n_dofs)
+        self._c(self.ctx.lib.fgpu_q2_get_rhs(self.h, b), "fgpu_q2_get_rhs")
+        return b
+
+    def apply(self, x):
+        y = np.zeros(self.n_dofs)
+        self._c(self.ctx.lib.fgpu_q2_apply(self.h, np.ascontiguousarray(x, np.float64), y), "fgpu_q2_apply")
+        return y
+
+    def time_apply(self, reps=20) -> float:
+        ms = C.c_double()
+        self._c(self.ctx.lib.fgpu_q2_time_apply(self.h, int(reps), C.byref(ms)), "fgpu_q2_time_apply")
+        return ms.value
+
+
 def tethex_split(ctx: "Context", nodes, tets, tris=None):
     """The reference's tet -> 4-hex split on the device (Tethex Mesh::convert_3D): (all nodes
     (n, 3), node kind (1 vertex, 2 edge, 3 face, 4 tet), hexes (4T, 8) in femocs/UCD order,
@@ -543,5 +666,6 @@ def tethex_split(ctx: "Context", nodes, tets, tris=None):
 
 
 def opts(max_it=10000, tol=1e-9, relative=True, precond=FGPU_PC_JACOBI, precision=FGPU_FP64, inner_tol=1e-4,
-         check_every=32, use_graph=True) -> SolveOpts:
-    return SolveOpts(max_it, tol, int(relative), precond, precision, inner_tol, check_every, int(use_graph))
+         check_every=32, use_graph=True, cheb_degree=0) -> SolveOpts:
+    return SolveOpts(max_it, tol, int(relative), precond, precision, inner_tol, check_every, int(use_graph),
+                     int(cheb_degree))

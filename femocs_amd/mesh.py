@@ -439,6 +439,16 @@ def tip_r10nm_geometry(**kw) -> TipGeometry:
     return TipGeometry(radius=100.0, height=600.0, slab_width=1200.0, slab_thickness=60.0, **kw)
 
 
+def batched_tip_geometry(i: int) -> TipGeometry:
+    """Tip i of the batched workload of 64 tips: radius U[2, 12] nm, aspect ratio U[3, 10],
+    slab 12 radii wide, its own lattice jitter seed (65k-2.3M vacuum hexes)."""
+    rng = np.random.default_rng(4242 + 7919 * i)
+    r = rng.uniform(20.0, 120.0)
+    a = rng.uniform(3.0, 10.0)
+    return TipGeometry(radius=r, height=r * a, slab_width=12 * r, slab_thickness=30.0 if r < 60 else 60.0,
+                       seed=20000 + i)
+
+
 def small_geometry(**kw) -> TipGeometry:
     """A small tip for tests that run the CPU oracle in seconds."""
     for k, v in (("h_min", 6.0), ("growth", 0.5), ("radius", 12.0), ("height", 30.0),

@@ -3,7 +3,12 @@
 The reference solves one Laplace problem per MD timestep, inline, with the previous solution as
 the initial guess only inside a PIC run (src/ProjectRunaway.cpp:422-431, 455-498; the solution
 vector survives between `PoissonSolver::assemble` calls, src/DealSolver.cpp:369-387 resets it
-only in `setup_system`).  Here many MD clients share one GPU:
+only in `setup_system`), then, with heating on, the current and the heat step on the bulk mesh
+(ProjectRunaway::solve_heat, src/ProjectRunaway.cpp:535-571: current.assemble + solve,
+heat.assemble(dt) + solve).  Here many MD clients share one GPU; a request is one timestep:
+the Laplace solve, and with `heat` the current solve and the implicit-Euler heat step after it,
+in that order (the current density of the reference comes from the field: the stages of one
+request run one after another, requests of different clients overlap):
 
 * every client keeps its tip mesh and its last solution resident in device memory, so a new
   timestep only uploads moved vertex coordinates and starts CG from the previous potential;
@@ -31,7 +36,14 @@ from . import mesh as M
 # Poisson boundary ids for the vacuum mesh: top 8 (Neumann field), bottom/sides/other
 # as src/PoissonSolver.cpp:52-55
 POISSON_MARKS = (8, 2, 4, 2)
+# bulk mesh: top and interface 2 (current and heat fluxes), bottom 7 (Dirichlet), sides 4, as
+# CurrentHeatSolver::mark_mesh (src/CurrentHeatSolver.cpp:535-538)
+HEAT_MARKS = (2, 7, 4, 2)
 BYTES_PER_DOF_ESTIMATE = 480          # mesh + system device bytes per dof before the first import
+CU_RHO_CP = 3.4496e-24                # J/(K A^3), include/CurrentHeatSolver.h:110
+LORENTZ = 2.44e-8                     # src/Config.cpp:75
+T_AMBIENT = 300.0
+STAGES = ("poisson", "current", "heat")
 
 
 @dataclass
@@ -41,10 +53,16 @@ class Request:
     t_submit: float
     t_admit: float = -1.0
     t_done: float = -1.0
-    iterations: int = 0
+    iterations: int = 0                     # CG iterations of the Laplace solve
     residual: float = 0.0
+    stage: int = 0                          # 0 Laplace, 1 current, 2 heat (in flight)
+    stage_iterations: list = field(default_factory=list)
+    disp: np.ndarray | None = None          # the surface displacement of this timestep
+    moved: bool = True                      # the displacement was accepted (else the last mesh)
+    snaps: list = field(default_factory=list)   # per stage: the solution's checkpoint handle
     x_init: np.ndarray | None = None        # kept only when the server records inputs
     positions: np.ndarray | None = None
+    bpositions: np.ndarray | None = None    # bulk mesh positions (heat on)
     x_final: np.ndarray | None = None
 
     @property
@@ -75,7 +93,7 @@ class Client:
     0.1 A turns the thinnest Delaunay slivers inside out."""
 
     def __init__(self, cid: int, tip: M.TipMesh, field: float = -1.0, sigma: float = 0.1, seed: int = 0,
-                 cap: float = 0.15):
+                 cap: float = 0.15, heat: bool = False, dt_fs: float = 4.0):
         self.cid = cid
         self.tip = tip
         self.field = field
@@ -84,6 +102,11 @@ class Client:
         self.cells = np.ascontiguousarray(tip.vacuum_cells(), np.int32)
         self.X0 = np.ascontiguousarray(tip.nodes, np.float64)
         self.n_dofs = int(np.unique(self.cells).size)
+        # current and heat on the bulk mesh (solve_heat), time step dt_fs femtoseconds
+        self.heat = heat
+        self.dt = dt_fs * 1e-15
+        self.bulk_cells = np.ascontiguousarray(tip.bulk_cells(), np.int32) if heat else None
+        self.n_bulk_dofs = int(np.unique(self.bulk_cells).size) if heat else 0
         W = M.hex_node_weights(tip.n_tetnodes, tip.tets)
         ns = tip.n_surface_nodes
         self.W_surf = W[:, :ns].tocsr()
@@ -93,6 +116,7 @@ class Client:
         self.W_moved = self.W_surf[self.moved_ids]
         self.limit = cap * vertex_altitudes(tip.nodes[: tip.n_tetnodes], tip.tets)[:ns]
         self.seq = 0
+        self.last_disp = None                 # displacement the resident meshes hold (None: 0)
 
     def displacements(self) -> np.ndarray:
         d = self.rng.normal(0.0, self.sigma, size=(self.tip.n_surface_nodes, 3))
@@ -112,12 +136,34 @@ class Client:
         X[move[0]] = move[1]
         return X
 
+    def surface_loads(self, centroids: np.ndarray):
+        """Per copper-surface face of the bulk mesh (face order of DealSolver::assemble_rhs):
+        the current density and the Nottingham heat flux.  GETELEC computes both from the
+        field in the reference (src/EmissionReader.cpp:199-228); it is absent here, so they are
+        seeded smooth fields over the face centroids of the unmoved tip."""
+        rng = np.random.default_rng(5000 + self.cid)
+        c = centroids
+        shape = 1.0 + 0.5 * np.sin(c[:, 0] / 7.0) * np.cos(c[:, 2] / 5.0) + 0.1 * rng.uniform(size=len(c))
+        J = 1e-3 * shape
+        N = 1e-9 * (1.0 + 0.3 * np.cos(c[:, 1] / 5.0))
+        return J, N
+
+    def initial_temperature(self, xyz: np.ndarray) -> np.ndarray:
+        return T_AMBIENT + 100.0 * np.exp(-((xyz[:, 2] - xyz[:, 2].max()) / 20.0) ** 2)
+
 
 @dataclass
 class _Resident:
     mesh: fgpu.Mesh
     sys: fgpu.System
     bytes: int
+    bmesh: fgpu.Mesh | None = None          # bulk mesh, current and heat systems (heat on)
+    cur: fgpu.System | None = None
+    heat: fgpu.System | None = None
+    T0: np.ndarray | None = None            # temperature before the first step (dof order)
+
+    def stage_system(self, stage: int) -> fgpu.System:
+        return (self.sys, self.cur, self.heat)[stage]
 
 
 @dataclass
@@ -140,12 +186,18 @@ class ServeStats:
         in_window = sum(1 for r in done if r.t_done <= self.t_end)
         # mean number of solves on the GPU at once (admission to result, over the span)
         busy = sum(r.t_done - r.t_admit for r in done if r.t_admit >= 0)
+        stage_its = {}
+        for k, name in enumerate(STAGES):
+            v = [r.stage_iterations[k] for r in done if len(r.stage_iterations) > k]
+            if v:
+                stage_its[name] = round(float(np.mean(v)), 1)
         return {
             "solves": in_window,
             "solves_per_s": in_window / span,
             "p50_ms": float(np.percentile(lat, 50)) if done else None,
             "p99_ms": float(np.percentile(lat, 99)) if done else None,
             "mean_iterations": float(its.mean()),
+            "mean_stage_iterations": stage_its,
             "mean_in_flight": round(busy / span, 2),
             "evictions": self.evictions,
             "rejected_moves": self.rejected_moves,
@@ -159,10 +211,12 @@ class Server:
     def __init__(self, ctx: fgpu.Context, clients: list[Client], opts: fgpu.SolveOpts,
                  budget_bytes: int = 64 << 30, batched: bool = True, warm: bool = True,
                  step_iterations: int = 32, max_batch: int = 64, record_inputs: bool = False, configure=None,
-                 coarse: bool = False, pipelined: bool = False):
+                 coarse: bool = False, pipelined: bool = False, heat_opts: fgpu.SolveOpts | None = None,
+                 snapshot=None):
         self.ctx = ctx
         self.clients = clients
         self.opts = opts
+        self.heat_opts = heat_opts if heat_opts is not None else opts
         self.budget = int(budget_bytes)
         self.batched = batched
         self.warm = warm
@@ -170,20 +224,30 @@ class Server:
         self.max_batch = max_batch
         self.record_inputs = record_inputs
         self.configure = configure                    # called as configure(mesh, system) on import
+        # snapshot(system) -> handle: called when a stage's solve ends (checkpoints of results)
+        self.snapshot = snapshot
         # two-level preconditioner, built at residency: False, True (aggregate size by mesh
         # size), 256 / 512 / 1024, or a function of the dof count returning one of those
         self.coarse = coarse
+        self.heat = any(c.heat for c in clients)
+        if self.heat and not all(c.heat for c in clients):
+            raise ValueError("either every client or none runs the heat stages")
         self.resident: OrderedDict[int, _Resident] = OrderedDict()   # LRU order
         self.known_bytes: dict[int, int] = {}
         self.pending: deque[Request] = deque()
         self.busy: dict[int, Request] = {}                            # client -> in-flight request
         self.by_sys: dict[int, Request] = {}
         self.stats = ServeStats()
+        # Laplace and current solves share one batch; heat systems carry per-cell coefficients
+        # (kappa(T), the mass term) and run in a second one
         self.batch = fgpu.Batch(ctx, opts) if batched else None
-        # pipelined steps: the next step is queued before the host retires (requests join one
-        # step later).  Off by default: admission synchronises with the stream (the validity
-        # check of a displacement, the metric of the moved mesh), so the queued step only
-        # delays it; measured 154 against 160 solves/s on the serving workload
+        self.hbatch = fgpu.Batch(ctx, self.heat_opts) if batched and self.heat else None
+        # pipelined steps (Laplace only): the next step is queued before the host retires
+        # (requests join one step later).  Off by default: admission synchronises with the
+        # stream (the validity check of a displacement, the metric of the moved mesh), so the
+        # queued step only delays it; measured 154 against 160 solves/s on the serving workload
+        if pipelined and self.heat:
+            raise ValueError("pipelined steps serve Laplace-only requests")
         self.pipelined = pipelined
         self._inflight = False
 
@@ -191,12 +255,15 @@ class Server:
     def used_bytes(self) -> int:
         return sum(r.bytes for r in self.resident.values())
 
+    def _estimate(self, c: Client) -> int:
+        return BYTES_PER_DOF_ESTIMATE * (c.n_dofs + 2 * c.n_bulk_dofs)
+
     def _make_resident(self, c: Client) -> _Resident | None:
         r = self.resident.get(c.cid)
         if r is not None:
             self.resident.move_to_end(c.cid)
             return r
-        need = self.known_bytes.get(c.cid, BYTES_PER_DOF_ESTIMATE * c.n_dofs)
+        need = self.known_bytes.get(c.cid, self._estimate(c))
         if need > self.budget:
             raise ValueError(f"client {c.cid} needs {need} bytes, more than the whole budget")
         for cid in list(self.resident):
@@ -209,6 +276,7 @@ class Server:
             return None
         m = fgpu.Mesh(self.ctx, c.X0, c.cells, *POISSON_MARKS)
         m.set_motion(c.moved_ids, c.W_moved)
+        c.last_disp = None                            # a fresh import holds the unmoved tip
         s = fgpu.System(m)
         if self.configure is not None:
             self.configure(m, s)
@@ -222,15 +290,54 @@ class Server:
         if self.coarse:
             # built once per residency: it stays a valid preconditioner as the mesh moves
             s.set_coarse(self._aggregate(m.n_dofs))
-        b = m.device_bytes() + s.device_bytes()
-        agg = self._aggregate(m.n_dofs)
-        if agg:
-            nc = (m.n_dofs + agg - 1) // agg
-            b += nc * (nc + 1) * 4
+        b = m.device_bytes() + s.device_bytes() + self._coarse_bytes(m.n_dofs)
+        r = _Resident(m, s, 0)
+        if c.heat:
+            bm = fgpu.Mesh(self.ctx, c.X0, c.bulk_cells, *HEAT_MARKS)
+            bm.set_motion(c.moved_ids, c.W_moved)             # rows outside the bulk are dropped
+            cur, heat = fgpu.System(bm), fgpu.System(bm)
+            if self.configure is not None:
+                self.configure(bm, cur)
+                self.configure(bm, heat)
+            J, N = c.surface_loads(bm.face_centroids(2))
+            # CurrentSolver::assemble (src/CurrentHeatSolver.cpp:421-484): sigma = 1, the
+            # current density on the copper surface, phi = 0 on the bottom
+            cur.reset()
+            cur.add_neumann(2, J, 0.0)
+            cur.add_dirichlet(7, 0.0)
+            cur.apply_dirichlet()
+            cur.fill_solution(0.0)
+            if self.coarse:
+                cur.set_coarse(self._aggregate(bm.n_dofs))
+            # HeatSolver::assemble (src/CurrentHeatSolver.cpp:116-152): implicit Euler from the
+            # present temperature, Joule heat of the present current, Nottingham flux, T_amb on
+            # the bottom; every request replays it with the new temperature and current
+            xyz, _ = bm.dofs()
+            r.T0 = c.initial_temperature(xyz)
+            heat.set_solution(r.T0)
+            heat.reset()
+            heat.heat_terms(cur, CU_RHO_CP / c.dt, LORENTZ)
+            heat.add_neumann(2, N, 0.0)
+            heat.add_dirichlet(7, T_AMBIENT)
+            heat.apply_dirichlet()
+            r.T0 = heat.solution()        # the first step's temperature: T_amb on the bottom
+            if self.coarse:
+                # from the first step's kappa(T); kappa drifts slowly and the mass term stays, so
+                # the level remains a good preconditioner over the steps
+                heat.set_coarse(self._aggregate(bm.n_dofs))
+            r.bmesh, r.cur, r.heat = bm, cur, heat
+            b += bm.device_bytes() + cur.device_bytes() + heat.device_bytes() + 2 * self._coarse_bytes(bm.n_dofs)
+        r.bytes = b
         self.known_bytes[c.cid] = b
-        r = _Resident(m, s, b)
         self.resident[c.cid] = r
         return r
+
+    def _coarse_bytes(self, n_dofs: int) -> int:
+        agg = self._aggregate(n_dofs)
+        if not agg:
+            return 0
+        nc = (n_dofs + agg - 1) // agg
+        return nc * (nc + 1) * 4 + nc * 8
 
     def _assemble(self, c: Client, r: _Resident, req: Request):
         t0 = time.perf_counter()
@@ -240,11 +347,18 @@ class Server:
             self.stats.prep_seconds += time.perf_counter() - t0
 
     def _assemble_timed(self, c: Client, r: _Resident, req: Request):
+        d = c.displacements()
         try:
-            r.mesh.displace(c.displacements())
+            r.mesh.displace_noalloc(d)
+            if r.bmesh is not None:
+                r.bmesh.displace_noalloc(d)
+            c.last_disp = d
         except fgpu.MeshInvertedError:
             # the step would need a new mesh; this request is solved on the last valid one
             self.stats.rejected_moves += 1
+            req.moved = False
+        # the displacement the meshes now hold (relative to the unmoved tip)
+        req.disp = c.last_disp if c.last_disp is not None else np.zeros_like(d)
         s = r.sys
         # same boundary terms, new geometry: rebuilt on the device
         s.reassemble()
@@ -255,6 +369,10 @@ class Server:
             req.positions = c.X0.copy()
             req.positions[vod] = xyz
             req.x_init = s.solution()
+            if r.bmesh is not None:
+                xyz, vod = r.bmesh.dofs()
+                req.bpositions = c.X0.copy()
+                req.bpositions[vod] = xyz
 
     def _aggregate(self, n_dofs: int) -> int:
         """Aggregate size of the coarse level for a mesh of n_dofs (0: none)."""
@@ -298,13 +416,32 @@ class Server:
         self.pending = waiting
         return admitted
 
-    def _finish(self, req: Request, st: fgpu.SolveStats, now: float):
+    def _next_stage(self, req: Request) -> fgpu.System:
+        """Prepares the next stage of a request on the device: the current solve on the moved
+        bulk mesh, or the heat step from the new current and the last temperature."""
+        t0 = time.perf_counter()
+        r = self.resident[req.client]
+        req.stage += 1
+        nxt = r.stage_system(req.stage)
+        nxt.reassemble()
+        if req.stage == 1 and not self.warm:
+            nxt.fill_solution(0.0)
+        self.stats.prep_seconds += time.perf_counter() - t0
+        return nxt
+
+    def _stage_done(self, req: Request, s: fgpu.System, st: fgpu.SolveStats):
+        req.stage_iterations.append(st.iterations)
+        if req.stage == 0:
+            req.iterations = st.iterations
+            req.residual = st.residual
+            if self.record_inputs:
+                req.x_final = s.solution()
+        if self.snapshot is not None:
+            req.snaps.append(self.snapshot(s))
+
+    def _finish(self, req: Request, now: float):
         req.t_done = now
-        req.iterations = st.iterations
-        req.residual = st.residual
         del self.busy[req.client]
-        if self.record_inputs:
-            req.x_final = self.resident[req.client].sys.solution()
 
     def step(self, clock=time.perf_counter) -> list[Request]:
         """Admits what fits, advances the GPU work by one step and returns finished requests."""
@@ -322,18 +459,27 @@ class Server:
             req.t_admit = clock()
             self.busy[req.client] = req
             t0 = time.perf_counter()
-            st = r.sys.solve(self.opts)
-            self._finish(req, st, clock())
+            self._stage_done(req, r.sys, r.sys.solve(self.opts))
+            while c.heat and req.stage < 2:
+                s = self._next_stage(req)
+                self._stage_done(req, s, s.solve(self.heat_opts if req.stage == 2 else self.opts))
+            self._finish(req, clock())
             self.stats.step_seconds += time.perf_counter() - t0
             return [req]
         self._admit(clock())
         if not self.pipelined:
-            if len(self.batch) == 0:
+            active = [b for b in (self.batch, self.hbatch) if b is not None and len(b) > 0]
+            if not active:
                 return done
             t0 = time.perf_counter()
-            self.batch.step(self.step_iterations)
+            for b in active:
+                b.step_async(self.step_iterations)
+            for b in active:
+                b.wait()
             now = clock()
-            retired = self.batch.retire()
+            retired = []
+            for b in active:
+                retired += b.retire()
         else:
             if len(self.batch) == 0 and not self._inflight:
                 return done
@@ -349,7 +495,13 @@ class Server:
             retired = self.batch.retire() if waited else []
         for s, st in retired:
             req = self.by_sys.pop(s.h.value)
-            self._finish(req, st, now)
+            self._stage_done(req, s, st)
+            if self.heat and req.stage < 2:
+                nxt = self._next_stage(req)
+                self.by_sys[nxt.h.value] = req
+                (self.hbatch if req.stage == 2 else self.batch).add(nxt)
+                continue
+            self._finish(req, now)
             done.append(req)
         self.stats.step_seconds += time.perf_counter() - t0
         return done

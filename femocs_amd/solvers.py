@@ -66,6 +66,9 @@ class DealSolver:
         self.relative = False
         self.inner_tol = 1e-4
         self.use_graph = True
+        # GPU preconditioner standing in for SSOR: FGPU_PC_JACOBI (with the two-level
+        # correction of a system that has one) or FGPU_PC_CHEBYSHEV (FP64)
+        self.preconditioner = fgpu.FGPU_PC_JACOBI
         self.last_stats = None
         #: FileWriter::write_time (src/FileWriter.cpp:23-26) is the caller's clock here: set
         #: True before an assembly whose write-step quantities are wanted
@@ -110,8 +113,12 @@ class DealSolver:
         self.stat.sol_min, self.stat.sol_max = self.system.minmax()
         return self.stat.sol_min < low_limit or self.stat.sol_max > high_limit
 
-    def solve_cg(self, n_steps: int, tolerance: float, precond: int = fgpu.FGPU_PC_JACOBI) -> int:
-        """DealSolver::solve_cg (src/DealSolver.cpp:443-458): iterations, negative on failure."""
+    def solve_cg(self, n_steps: int, tolerance: float, ssor_param: float = 1.2) -> int:
+        """DealSolver::solve_cg (src/DealSolver.cpp:443-458): iterations, negative on failure.
+        As the reference, ssor_param <= 0 solves without a preconditioner
+        (src/DealSolver.cpp:447-452); otherwise the GPU preconditioner `self.preconditioner`
+        (Jacobi by default, FGPU_PC_CHEBYSHEV for FP64) stands in for SSOR(ssor_param)."""
+        precond = self.preconditioner if ssor_param > 0 else fgpu.FGPU_PC_NONE
         o = fgpu.opts(max_it=n_steps, tol=tolerance, relative=self.relative, precond=precond,
                       precision=self.precision, inner_tol=self.inner_tol, use_graph=self.use_graph)
         st = self.system.solve(o)
@@ -186,7 +193,7 @@ class PoissonSolver(DealSolver):
 
     def solve(self) -> int:
         """PoissonSolver::solve (include/PoissonSolver.h:54)."""
-        return self.solve_cg(self.conf.n_cg, self.conf.cg_tolerance)
+        return self.solve_cg(self.conf.n_cg, self.conf.cg_tolerance, self.conf.ssor_param)
 
 
 @dataclass
@@ -218,7 +225,7 @@ class _BulkSolver(DealSolver):
 
     def solve(self) -> int:
         """EmissionSolver::solve (include/CurrentHeatSolver.h:35)."""
-        return self.solve_cg(self.conf.n_cg, self.conf.cg_tolerance)
+        return self.solve_cg(self.conf.n_cg, self.conf.cg_tolerance, self.conf.ssor_param)
 
 
 class CurrentSolver(_BulkSolver):

@@ -13,6 +13,8 @@
 #ifndef FEMOCS_GPU_H
 #define FEMOCS_GPU_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -34,9 +36,16 @@ extern "C" {
 #define FGPU_FP64 0   /* FP64 CG (the reference's arithmetic)                            */
 #define FGPU_MIXED 1  /* FP32 operator and CG inside FP64 defect correction               */
 
-/* preconditioners */
+/* preconditioners.  FGPU_PC_NONE: identity, as PreconditionIdentity when ssor_param <= 0
+ * (src/DealSolver.cpp:447-452).  FGPU_PC_JACOBI: the diagonal, plus the two-level coarse
+ * correction of a system that has one (fgpu_sys_set_coarse), in either precision.
+ * FGPU_PC_CHEBYSHEV (FGPU_FP64 only): Chebyshev-Jacobi polynomial preconditioner
+ * z = p(D^-1 A) D^-1 r on [lambda_max / 100, lambda_max], lambda_max from a Lanczos estimate
+ * (the first Jacobi-PCG iterations of the solve, repeated when the system changed).  All
+ * three stand in for the SSOR preconditioner of the reference, whose sweeps are sequential. */
 #define FGPU_PC_NONE 0
 #define FGPU_PC_JACOBI 1
+#define FGPU_PC_CHEBYSHEV 2
 
 typedef struct fgpu_ctx fgpu_ctx;
 typedef struct fgpu_mesh fgpu_mesh;
@@ -53,6 +62,7 @@ typedef struct {
     double inner_tol;  /* FGPU_MIXED: relative reduction asked of each FP32 inner solve        */
     int check_every;   /* iterations between host convergence checks (0: default)             */
     int use_graph;     /* capture the iterations between checks in a hipGraph                 */
+    int cheb_degree;   /* FGPU_PC_CHEBYSHEV: polynomial degree (0: chosen by the system size)  */
 } fgpu_solve_opts;
 
 typedef struct {
@@ -67,6 +77,10 @@ int fgpu_ctx_create(int device, fgpu_ctx **out);
 void fgpu_ctx_destroy(fgpu_ctx *ctx);
 const char *fgpu_ctx_error(const fgpu_ctx *ctx);
 int fgpu_ctx_sync(fgpu_ctx *ctx);
+/* device memory owned by the caller on the context's device (e.g. for
+ * fgpu_sys_solution_to_device checkpoints); no counterpart in the reference */
+int fgpu_ctx_alloc(fgpu_ctx *ctx, size_t bytes, void **dev);
+int fgpu_ctx_free(fgpu_ctx *ctx, void *dev);
 
 /* ---- mesh ---------------------------------------------------------------------------
  * DealSolver::import_mesh(vector<Point<3>>, vector<CellData<3>>) (src/DealSolver.cpp:192-209)
@@ -193,6 +207,12 @@ int fgpu_sys_fill_solution(fgpu_sys *sys, double value);
 /* DealSolver::import_solution / export_solution (src/DealSolver.cpp:270-315), dof order */
 int fgpu_sys_set_solution(fgpu_sys *sys, const double *x);
 int fgpu_sys_get_solution(fgpu_sys *sys, double *x);
+/* device-to-device copies of the solution (n_dofs doubles in the library's internal dof
+ * order, meaningful only for the same mesh) to and from caller-owned device memory, ordered on
+ * the context's stream: checkpoints of served results without host copies.  No counterpart in
+ * the reference. */
+int fgpu_sys_solution_to_device(fgpu_sys *sys, double *dev);
+int fgpu_sys_solution_from_device(fgpu_sys *sys, const double *dev);
 /* lifted right-hand side (dof order), for checks */
 int fgpu_sys_get_rhs(fgpu_sys *sys, double *b);
 /* y = A x with the un-lifted operator (the matrix before apply_dirichlet), FP64 */
@@ -386,6 +406,38 @@ int fgpu_tethex_split(fgpu_ctx *ctx, int n_vertices, const double *xyz, int n_te
 int fgpu_tethex_sizes(const fgpu_tethex *split, int *n_nodes, int *n_edges, int *n_faces);
 int fgpu_tethex_export(const fgpu_tethex *split, double *xyz, int *hexes, int *quads);
 void fgpu_tethex_destroy(fgpu_tethex *split);
+
+/* ---- Q2 Laplace solve --------------------------------------------------------------
+ * The reference's Poisson path with the element degree raised to 2: a recompile of
+ * include/DealSolver.h:130-131 with shape_degree = 2 (FE_Q(2), QGauss(3) cells, QGauss<2>(3)
+ * faces, trilinear geometry) on the hexes of an imported mesh; no such build exists in the
+ * reference.  Dofs in the API order: the mesh's dofs (vertices) first, in fgpu_mesh_export_dofs
+ * order, then edge midpoints, face centres and cell centres.  The mesh stays alive while a Q2
+ * system built on it does. */
+typedef struct fgpu_q2 fgpu_q2;
+int fgpu_q2_create(fgpu_mesh *mesh, fgpu_q2 **out);
+void fgpu_q2_destroy(fgpu_q2 *q2);
+int fgpu_q2_sizes(const fgpu_q2 *q2, int *n_dofs, int *n_cells);
+/* support points (3*n_dofs: vertices, then the mean of the entity's vertices) and the 27 dofs
+ * of every cell (import cell order, lexicographic 3x3x3 nodes i + 3j + 9k); either may be NULL */
+int fgpu_q2_export_dofs(const fgpu_q2 *q2, double *xyz, int *cells27);
+/* DealSolver::assemble pieces (src/DealSolver.cpp:390-440, src/PoissonSolver.cpp:170-210):
+ * clear, Dirichlet value on every dof of the faces with an id, constant Neumann load on the
+ * faces with an id (QGauss<2>(3)), the lifting; then the solution from its current value */
+int fgpu_q2_reset(fgpu_q2 *q2);
+int fgpu_q2_add_dirichlet(fgpu_q2 *q2, int bid, double value);
+int fgpu_q2_add_neumann(fgpu_q2 *q2, int bid, double value);
+int fgpu_q2_apply_dirichlet(fgpu_q2 *q2);
+int fgpu_q2_fill_solution(fgpu_q2 *q2, double value);
+/* DealSolver::solve_cg (src/DealSolver.cpp:443-458): FGPU_FP64 only; FGPU_PC_NONE, _JACOBI
+ * or _CHEBYSHEV */
+int fgpu_q2_solve(fgpu_q2 *q2, const fgpu_solve_opts *opts, fgpu_solve_stats *stats);
+int fgpu_q2_get_solution(fgpu_q2 *q2, double *x);
+int fgpu_q2_get_rhs(fgpu_q2 *q2, double *b);
+/* y = A x with the un-lifted Q2 operator (API dof order) */
+int fgpu_q2_apply(fgpu_q2 *q2, const double *x, double *y);
+/* average duration of reps element applies (HIP events on the context's stream), benchmark */
+int fgpu_q2_time_apply(fgpu_q2 *q2, int reps, double *avg_ms);
 
 #ifdef __cplusplus
 }

@@ -85,6 +85,11 @@ void orc_apply_dirichlet(orc_csr *A, double *b, double *x, const unsigned char *
  * Returns iterations, negative on non-convergence. final residual in *res. */
 int  orc_pcg(const orc_csr *A, const double *b, double *x, int max_it, double tol, int relative,
              int precond, double omega, double *res);
+/* orc_pcg with the matrix-vector product and vector operations on nthreads OpenMP threads
+ * (SSOR sweeps sequential), for the CPU reference timing (refcpu.c) */
+int  orc_pcg_omp(const orc_csr *A, const double *b, double *x, int max_it, double tol, int relative,
+                 int precond, double omega, int nthreads, double *res_out);
+int  orc_omp_max_threads(void);
 
 /* -grad u at every vertex, taken at the Gauss point "vertex2node" of the last cell touching it
  * (DealSolver::export_solution_grad, src/DealSolver.cpp:281-301). */
@@ -222,6 +227,7 @@ void orc_interp_surface_cells(const orc_lintri *T, const orc_lintet *L, const do
 double orc_resistivity(double T);
 double orc_sigma(double T);
 double orc_kappa(double T, double lorentz);
+void orc_sigma_kappa_array(int n, const double *T, double lorentz, double *sigma, double *kappa);
 
 #ifdef __cplusplus
 }

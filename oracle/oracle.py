@@ -78,6 +78,10 @@ def _declare(L):
     L.orc_pcg.argtypes = [C.POINTER(_Csr), _D, _D, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
                           C.POINTER(C.c_double)]
     L.orc_pcg.restype = C.c_int
+    L.orc_pcg_omp.argtypes = [C.POINTER(_Csr), _D, _D, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
+                              C.POINTER(C.c_double)]
+    L.orc_pcg_omp.restype = C.c_int
+    L.orc_omp_max_threads.restype = C.c_int
     L.orc_export_solution_grad.argtypes = [C.POINTER(_Mesh), _D, _D]
     L.orc_element_matrix.argtypes = [_D, C.c_void_p, C.c_void_p, _D]
     L.orc_nodal_field.argtypes = [C.c_int, _D, C.c_int, _I, _I, C.c_int, _D, _D]
@@ -116,6 +120,7 @@ def _declare(L):
         getattr(L, f).restype = C.c_double
     L.orc_kappa.argtypes = [C.c_double, C.c_double]
     L.orc_kappa.restype = C.c_double
+    L.orc_sigma_kappa_array.argtypes = [C.c_int, _D, C.c_double, _D, _D]
 
 
 def _arr(ptr, n, dtype):
@@ -144,6 +149,18 @@ class Mesh:
         self.bface_face = _arr(m.bface_face, m.n_bfaces, np.int32)
         self.bface_id = _arr(m.bface_id, m.n_bfaces, np.int32)
         self._csr = None
+
+    # vertices of deal.II face f (lexicographic cell numbering): the 4 cell vertices with bit
+    # f // 2 equal to f % 2
+    FACE_VERTICES = [[k for k in range(8) if (k >> (f // 2)) & 1 == f % 2] for f in range(6)]
+
+    def face_centroids(self, bid):
+        """DealSolver::export_surface_centroids (src/DealSolver.cpp:229-245): the centre (mean of
+        the 4 vertices) of every boundary face with id bid, in active-cell x face order."""
+        sel = np.nonzero(self.bface_id == bid)[0]
+        fv = np.array(self.FACE_VERTICES)[self.bface_face[sel]]
+        dofs = self.cells[self.bface_cell[sel][:, None], fv]
+        return self.x[dofs].mean(axis=1)
 
     def __del__(self):
         try:
@@ -210,6 +227,15 @@ class Mesh:
         it = lib().orc_pcg(C.byref(self._csr), b, x, max_it, tol, int(relative), kind, omega, C.byref(res))
         return it, res.value
 
+    def pcg_threads(self, b, x, threads, max_it=10000, tol=1e-9, relative=False, precond="ssor", omega=1.2):
+        """pcg() with the matrix-vector product and vector operations on `threads` OpenMP
+        threads (SSOR sweeps sequential); for the CPU reference timing."""
+        kind = {"none": 0, "jacobi": 1, "ssor": 2}[precond]
+        res = C.c_double()
+        it = lib().orc_pcg_omp(C.byref(self._csr), b, x, max_it, tol, int(relative), kind, omega, int(threads),
+                               C.byref(res))
+        return it, res.value
+
     def qp_fields(self, u):
         """(values 8 per cell, gradients 8x3 per cell) of a dof vector at the Gauss points."""
         val = np.zeros(8 * self.n_cells)
@@ -232,8 +258,15 @@ def element_matrix(xe_lex, kappa_q=None, gamma_q=None):
     return K.reshape(8, 8)
 
 
+def _pcg(m, b, x, threads, **kw):
+    """m.pcg, or its multi-threaded form (CPU reference timing) when threads is given."""
+    if threads:
+        return m.pcg_threads(b, x, threads, **kw)
+    return m.pcg(b, x, **kw)
+
+
 def poisson_laplace(vertices, ucd_cells, applied_field, tol=1e-9, relative=False, precond="ssor",
-                    omega=1.2, max_it=10000, anode="neumann", potential=0.0):
+                    omega=1.2, max_it=10000, anode="neumann", potential=0.0, threads=None):
     """PoissonSolver setup/assemble/solve for field_mode = laplace (src/PoissonSolver.cpp:162-210,
     src/DealSolver.cpp:443-458): phi = 0 on the copper surface (id 2), Neumann -E0 on the top
     (id 8).  Returns (mesh, solution per dof, iterations, residual)."""
@@ -252,7 +285,7 @@ def poisson_laplace(vertices, ucd_cells, applied_field, tol=1e-9, relative=False
         vals[top > 0] = potential
         vals[m.dirichlet_mask(BID_COPPER_SURFACE) > 0] = 0.0
     m.apply_dirichlet(b, x, mask, vals)
-    it, res = m.pcg(b, x, max_it=max_it, tol=tol, relative=relative, precond=precond, omega=omega)
+    it, res = _pcg(m, b, x, threads, max_it=max_it, tol=tol, relative=relative, precond=precond, omega=omega)
     return m, x, it, res, b
 
 
@@ -261,7 +294,7 @@ LORENTZ = 2.44e-8           # Lorenz number [W Ohm/K^2], src/Config.cpp:75
 BID_COPPER_BOTTOM = 7
 
 
-def current_solve(vertices, bulk_cells, face_j, tol=1e-12, precond="ssor", max_it=10000, sigma_q=None):
+def current_solve(vertices, bulk_cells, face_j, tol=1e-12, precond="ssor", max_it=10000, sigma_q=None, threads=None):
     """CurrentSolver::assemble + solve (src/CurrentHeatSolver.cpp:421-484): sigma = 1 (or the
     given conductivity per cell and Gauss point), Neumann flux face_j on the copper surface (one
     value per id-2 face), phi = 0 on the bottom (id 7)."""
@@ -270,12 +303,12 @@ def current_solve(vertices, bulk_cells, face_j, tol=1e-12, precond="ssor", max_i
     b = m.face_rhs(BID_COPPER_SURFACE, values=face_j)
     x = np.zeros(m.n_dofs)
     m.apply_dirichlet(b, x, m.dirichlet_mask(BID_COPPER_BOTTOM), None)
-    it, res = m.pcg(b, x, max_it=max_it, tol=tol, relative=True, precond=precond)
+    it, res = _pcg(m, b, x, threads, max_it=max_it, tol=tol, relative=True, precond=precond)
     return m, x, it
 
 
 def heat_step(m: "Mesh", T_old, phi_c, dt, face_nottingham, T_amb, tol=1e-12, lorentz=LORENTZ, precond="ssor",
-              theta=1.0):
+              theta=1.0, threads=None):
     """HeatSolver::assemble (implicit Euler, src/CurrentHeatSolver.cpp:116-152, 347-393) and solve
     on the bulk mesh m: A = gamma M + K_kappa(T_old), b = int phi_i (gamma T_old + sigma(T_old)
     |grad phi_c|^2) + Nottingham flux on id 2, T = T_amb on id 7; T_old is the initial guess.
@@ -284,8 +317,10 @@ def heat_step(m: "Mesh", T_old, phi_c, dt, face_nottingham, T_amb, tol=1e-12, lo
     gamma = CU_RHO_CP / dt
     Tq, _ = m.qp_fields(T_old)
     _, gq = m.qp_fields(phi_c)
-    kq = np.vectorize(lambda t: kappa(t, lorentz))(Tq)
-    sq = np.vectorize(sigma)(Tq)
+    flat = np.ascontiguousarray(Tq, np.float64).reshape(-1)
+    kq, sq = np.empty(flat.size), np.empty(flat.size)
+    lib().orc_sigma_kappa_array(flat.size, flat, lorentz, sq, kq)
+    kq, sq = kq.reshape(Tq.shape), sq.reshape(Tq.shape)
     explicit = None
     if theta < 1.0:
         m.assemble(kappa_q=kq.reshape(-1), gamma_q=np.zeros(Tq.size))
@@ -300,7 +335,7 @@ def heat_step(m: "Mesh", T_old, phi_c, dt, face_nottingham, T_amb, tol=1e-12, lo
     mask = m.dirichlet_mask(BID_COPPER_BOTTOM)
     vals = np.where(mask > 0, T_amb, 0.0)
     m.apply_dirichlet(b, x, mask, vals)
-    it, res = m.pcg(b, x, tol=tol, relative=True, precond=precond)
+    it, res = _pcg(m, b, x, threads, tol=tol, relative=True, precond=precond)
     return x, it
 
 

@@ -890,3 +890,11 @@ double orc_kappa(double T, double lorentz) {
     T = clampT(T);
     return lorentz * T * orc_sigma(T);
 }
+
+/* element-wise orc_sigma / orc_kappa over n temperatures (same arithmetic per element) */
+void orc_sigma_kappa_array(int n, const double *T, double lorentz, double *sigma, double *kappa) {
+    for (int i = 0; i < n; ++i) {
+        if (sigma) sigma[i] = orc_sigma(T[i]);
+        if (kappa) kappa[i] = orc_kappa(T[i], lorentz);
+    }
+}

@@ -124,3 +124,12 @@ def test_patch_rebuild_after_either_import(ctx, small_mesh):
         out.append((st.iterations, s.solution()))
     assert out[0][0] == out[1][0] > 0
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_collapsed_cells_where_the_patch_cap_binds(ctx):
+    """Cells with a repeated vertex (one edge collapsed) count each distinct vertex once in the
+    patch node cap on both imports: the same tables where the cap binds."""
+    tip = M.make_tip_mesh(M.tip_r10nm_geometry(), with_atoms=False)
+    cells = np.array(tip.vacuum_cells(), copy=True)
+    cells[::997, 1] = cells[::997, 0]              # collapse the edge 0-1 of every 997th cell
+    _same(ctx, tip.nodes, cells)

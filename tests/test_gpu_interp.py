@@ -67,6 +67,24 @@ def test_nodal_solution_matches_oracle(ctx, tip, solved, smoothen):
         assert np.array_equal(ip.nodal(), ref)
 
 
+def test_nodal_field_of_gpu_solve_matches_oracle_solve(ctx, tip, solved):
+    """The nodal potential and field of the GPU solve (1e-12) against those of the oracle's own
+    solve (SSOR-PCG to 1e-13), each extracted by its own path: the chain solve -> nodal field
+    compared end to end, not only the extraction."""
+    _, s, phi, used = solved
+    _, xo, it, _, _ = O.poisson_laplace(tip.nodes, tip.vacuum_cells(), -1.0, tol=1e-13, relative=True)
+    assert it > 0
+    om = O.Mesh(tip.nodes, tip.vacuum_cells(), *MARKS)
+    phi_o = np.zeros(len(tip.nodes))
+    phi_o[om.vertex_of_dof] = xo
+    ref = O.extract_nodal(tip.nodes, tip.hexes, tip.hex_marker, 1, used, phi_o, empty=0.0)
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM, empty_value=0.0)
+    ip.extract_solution(s, smoothen=False)
+    got = ip.nodal()
+    assert np.linalg.norm(got[:, 3] - ref[:, 3]) <= 1e-10 * np.linalg.norm(ref[:, 3])
+    assert np.linalg.norm(got[:, :3] - ref[:, :3]) <= 1e-9 * np.linalg.norm(ref[:, :3])
+
+
 def test_nodal_solution_carries_charge_density(ctx, tip):
     """A system assembled as a write step hands its charge density to the nodal solution
     (Interpolator::extract_solution, src/Interpolator.cpp:172-179)."""

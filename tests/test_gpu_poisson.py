@@ -415,3 +415,32 @@ def test_reference_ordered_hexes_keep_the_tet_split_fp32_operator(ctx, small_mes
     st = s.solve(fgpu.opts(tol=1e-11, relative=True, precision=fgpu.FGPU_MIXED))
     assert st.iterations > 0
     assert np.linalg.norm(s.solution() - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def test_minmax_and_check_limits(ctx, small_mesh):
+    """DealSolver::check_limits (src/DealSolver.cpp:158-167) through fgpu_sys_minmax: the
+    extremes of the solution, and out of limits on either side."""
+    from femocs_amd import solvers
+    p = solvers.PoissonSolver(ctx=ctx)
+    assert p.import_mesh(small_mesh.nodes, small_mesh.vacuum_cells())
+    p.setup(-1.0, 0.0)
+    p.assemble()
+    assert p.solve() > 0
+    x = p.export_solution()
+    mn, mx = p.system.minmax()
+    assert mn == x.min() and mx == x.max()
+    assert not p.check_limits(mn, mx)
+    assert p.stat.sol_min == mn and p.stat.sol_max == mx
+    assert p.check_limits(mn + 1e-9 * abs(mx - mn), mx)          # minimum below the low limit
+    assert p.check_limits(mn, mx - 1e-9 * abs(mx - mn))          # maximum above the high limit
+
+
+@pytest.mark.parametrize("bid", [2, 8])
+def test_face_centroids_match_oracle(ctx, small_mesh, bid):
+    """DealSolver::export_surface_centroids (src/DealSolver.cpp:229-245): the centres of the
+    faces with an id, in the cell x face order of the Neumann load."""
+    om, gm = _pair(ctx, small_mesh)
+    c = gm.face_centroids(bid)
+    ref = om.face_centroids(bid)
+    assert c.shape == ref.shape and len(c) > 0
+    assert np.abs(c - ref).max() <= 1e-13 * np.abs(ref).max()

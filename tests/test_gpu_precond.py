@@ -1,0 +1,201 @@
+"""FP64 solves under each preconditioner the library offers in place of the reference's SSOR
+(src/DealSolver.cpp:443-458): none (ssor_param <= 0), Jacobi, Jacobi with the two-level
+correction, Chebyshev-Jacobi with a Lanczos estimate of lambda_max; each solution against the
+oracle's.  Also the replay of an implicit-Euler heat step by fgpu_sys_reassemble and the
+device-to-device solution copies."""
+import numpy as np
+import pytest
+
+from femocs_amd import fgpu, serving, solvers
+from femocs_amd import mesh as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+MARKS = (8, 2, 4, 2)
+
+
+def _system(ctx, mesh, field=-1.0):
+    gm = fgpu.Mesh(ctx, mesh.nodes, mesh.vacuum_cells(), *MARKS)
+    s = fgpu.System(gm)
+    s.reset()
+    s.add_dirichlet(2, 0.0)
+    s.add_neumann(8, None, field)
+    s.apply_dirichlet()
+    s.fill_solution(0.0)
+    return gm, s
+
+
+@pytest.fixture(scope="module")
+def r3nm():
+    """Config 1's own tip (r = 3 nm, h = 15 nm on a 30 x 30 nm slab) and the oracle's FP64
+    Laplace solution to 1e-13."""
+    tip = M.make_tip_mesh(M.tip_r3nm_geometry(), with_atoms=False)
+    print(f"r3nm tip: {len(tip.vacuum_cells())} vacuum hexes", flush=True)
+    _, xo, it, _, _ = O.poisson_laplace(tip.nodes, tip.vacuum_cells(), -1.0, tol=1e-13, relative=True, precond="ssor")
+    assert it > 0
+    return tip, xo
+
+
+@pytest.fixture(scope="module")
+def small_ref(small_mesh):
+    _, xo, it, _, _ = O.poisson_laplace(small_mesh.nodes, small_mesh.vacuum_cells(), -1.0, tol=1e-13, relative=True,
+                                        precond="ssor")
+    assert it > 0
+    return xo
+
+
+def _err(x, xo):
+    return np.linalg.norm(x - xo) / np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("degree", [0, 2, 5])
+def test_chebyshev_matches_oracle(ctx, small_mesh, small_ref, degree):
+    _, s = _system(ctx, small_mesh)
+    st = s.solve(fgpu.opts(tol=1e-12, relative=True, precond=fgpu.FGPU_PC_CHEBYSHEV, cheb_degree=degree))
+    assert st.iterations > 0
+    assert _err(s.solution(), small_ref) <= 1e-10
+    # the Jacobi solve needs more CG steps than the Chebyshev one (16 Jacobi steps included)
+    _, sj = _system(ctx, small_mesh)
+    sj_st = sj.solve(fgpu.opts(tol=1e-12, relative=True))
+    assert st.iterations < sj_st.iterations
+
+
+def test_chebyshev_r3nm_and_reuse(ctx, r3nm):
+    """Config 1's mesh: Chebyshev-Jacobi FP64 within 1e-10 of the oracle; a second solve of the
+    unchanged system reuses the lambda_max estimate (no Jacobi steps: fewer iterations)."""
+    tip, xo = r3nm
+    _, s = _system(ctx, tip)
+    o = fgpu.opts(tol=1e-12, relative=True, precond=fgpu.FGPU_PC_CHEBYSHEV)
+    st = s.solve(o)
+    assert st.iterations > 0
+    assert _err(s.solution(), xo) <= 1e-10
+    s.fill_solution(0.0)
+    st2 = s.solve(o)
+    assert 0 < st2.iterations < st.iterations
+    assert _err(s.solution(), xo) <= 1e-10
+
+
+def test_no_preconditioner_and_solver_mirror(ctx, small_mesh, small_ref):
+    """ssor_param <= 0 selects the identity, as the reference (src/DealSolver.cpp:447-452)."""
+    _, s = _system(ctx, small_mesh)
+    st = s.solve(fgpu.opts(tol=1e-12, relative=True, precond=fgpu.FGPU_PC_NONE))
+    _, sj = _system(ctx, small_mesh)
+    stj = sj.solve(fgpu.opts(tol=1e-12, relative=True))
+    assert st.iterations > stj.iterations > 0
+    assert _err(s.solution(), small_ref) <= 1e-10
+    p = solvers.PoissonSolver(ctx=ctx)
+    assert p.import_mesh(small_mesh.nodes, small_mesh.vacuum_cells())
+    p.setup(-1.0, 0.0)
+    p.relative = True
+    p.conf.cg_tolerance = 1e-12
+    p.assemble()
+    p.conf.ssor_param = 0.0
+    it_none = p.solve()
+    p.system.fill_solution(0.0)
+    p.conf.ssor_param = 1.2
+    it_pc = p.solve()
+    assert it_none == st.iterations and it_pc == stj.iterations
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_two_level_fp64_matches_oracle(ctx, r3nm, mixed):
+    """The two-level correction in the FP64 CG (and, as before, in the FP32 CG of the mixed
+    solver): same solution as the oracle in fewer iterations than Jacobi."""
+    tip, xo = r3nm
+    _, s = _system(ctx, tip)
+    s.set_coarse(True)
+    prec = fgpu.FGPU_MIXED if mixed else fgpu.FGPU_FP64
+    st = s.solve(fgpu.opts(tol=1e-12, relative=True, precision=prec, inner_tol=0.01))
+    assert st.iterations > 0
+    assert _err(s.solution(), xo) <= 1e-10
+    _, sj = _system(ctx, tip)
+    stj = sj.solve(fgpu.opts(tol=1e-12, relative=True))
+    assert st.iterations < stj.iterations
+
+
+def test_two_level_fp64_batch_equals_alone(ctx, small_mesh):
+    """FP64 two-level systems give the same bits in a continuous batch as alone."""
+    m2 = M.make_tip_mesh(M.small_geometry(radius=10.0, height=24.0, seed=7), with_atoms=False)
+    o = fgpu.opts(tol=1e-10, relative=True, use_graph=False)
+    alone = []
+    for m in (small_mesh, m2):
+        _, s = _system(ctx, m)
+        s.set_coarse(True)
+        st = s.solve(o)
+        alone.append((s.solution(), st.iterations))
+    b = fgpu.Batch(ctx, o)
+    syss = []
+    for m in (small_mesh, m2):
+        _, s = _system(ctx, m)
+        s.set_coarse(True)
+        syss.append(s)
+        b.add(s)
+    done = {}
+    for _ in range(200):
+        if not len(b):
+            break
+        b.step(16)
+        for s, st in b.retire():
+            done[id(s)] = st
+    for (xa, ita), s in zip(alone, syss):
+        assert done[id(s)].iterations == ita
+        assert np.array_equal(s.solution(), xa)
+
+
+def test_heat_step_replay_equals_fresh_assembly(ctx, small_mesh):
+    """fgpu_sys_reassemble of a heat system recomputes kappa(T_old), the mass term and the Joule
+    source from its present solution and the present current potential: after a mesh move it
+    gives the right-hand side and solution of a reset and fresh assembly, bit for bit."""
+    cl = serving.Client(0, small_mesh, seed=3)
+    disp = cl.displacements()                 # capped: no cell turns inside out
+    o = fgpu.opts(tol=1e-12, relative=True, use_graph=False)
+    out = []
+    for replay in (True, False):
+        gm = fgpu.Mesh(ctx, small_mesh.nodes, small_mesh.bulk_cells(), 2, 7, 4, 2)
+        gm.set_motion(cl.moved_ids, cl.W_moved)
+        cen = gm.face_centroids(2)
+        J = 5e-3 * (1.0 + 0.5 * np.sin(cen[:, 0] / 7.0))
+        N = 1e-9 * (1.0 + 0.3 * np.cos(cen[:, 1] / 5.0))
+        cur, heat = fgpu.System(gm), fgpu.System(gm)
+        xyz, _ = gm.dofs()
+        heat.set_solution(300.0 + 300.0 * np.exp(-((xyz[:, 2] - xyz[:, 2].max()) / 20.0) ** 2))
+        cur.reset()
+        cur.add_neumann(2, J, 0.0)
+        cur.add_dirichlet(7, 0.0)
+        cur.apply_dirichlet()
+        cur.fill_solution(0.0)
+        assert cur.solve(o).iterations > 0
+
+        def fresh():
+            heat.reset()
+            heat.heat_terms(cur, O.CU_RHO_CP / 4e-15, O.LORENTZ)
+            heat.add_neumann(2, N, 0.0)
+            heat.add_dirichlet(7, 300.0)
+            heat.apply_dirichlet()
+
+        fresh()
+        assert heat.solve(o).iterations > 0
+        gm.displace(disp)
+        cur.reassemble()
+        assert cur.solve(o).iterations > 0
+        if replay:
+            heat.reassemble()
+        else:
+            fresh()
+        b = heat.rhs()
+        assert heat.solve(o).iterations > 0
+        out.append((b, heat.solution()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_solution_device_copies(ctx, small_mesh):
+    gm, s = _system(ctx, small_mesh)
+    assert s.solve(fgpu.opts(tol=1e-10, relative=True)).iterations > 0
+    x = s.solution()
+    buf = ctx.alloc(8 * gm.n_dofs)
+    s.solution_to_device(buf.ptr)
+    s.fill_solution(0.0)
+    assert not s.solution().any()
+    s.solution_from_device(buf.ptr)
+    assert np.array_equal(s.solution(), x)

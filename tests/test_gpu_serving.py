@@ -108,3 +108,86 @@ def test_serving_two_level(ctx, clients):
                                             precond="ssor")
         assert it > 0
         assert np.linalg.norm(r.x_final - xo) <= 1e-8 * np.linalg.norm(xo), (r.client, r.seq)
+
+
+@pytest.fixture(scope="module")
+def heat_clients():
+    geos = [M.small_geometry(), M.small_geometry(radius=10.0, height=24.0, seed=7)]
+    return [serving.Client(i, M.make_tip_mesh(g, with_atoms=False), field=-1.0 - 0.1 * i, seed=i, heat=True)
+            for i, g in enumerate(geos)]
+
+
+def test_serving_heat_timesteps_equal_alone_and_oracle(ctx, heat_clients):
+    """Timesteps with the current and heat stages (solve_heat after solve_laplace): every
+    stage of every served timestep equals the same solve alone (fresh imports of the moved
+    meshes, fresh assembly, same initial guess) bit for bit, and the first client's second
+    timestep matches the oracle stage by stage (each stage from the served inputs)."""
+    o = fgpu.opts(tol=1e-9, relative=True, precision=fgpu.FGPU_MIXED, inner_tol=0.1, use_graph=False)
+    for c in heat_clients:
+        c.seq = 0
+    srv = serving.Server(ctx, heat_clients, o, step_iterations=16, record_inputs=True, coarse=True,
+                         snapshot=lambda s: s.solution())
+    serving.run_clients(srv, rate=400.0, duration=60.0, max_requests=6, seed=11)
+    assert srv.idle()
+    done = sorted([r for r in srv.stats.requests if r.t_done >= 0], key=lambda r: (r.client, r.seq))
+    assert len(done) == 6 and all(len(r.snaps) == 3 and min(r.stage_iterations) > 0 for r in done)
+    prev = {}
+    for r in done:
+        c = heat_clients[r.client]
+        res = srv.resident[r.client]
+        # Laplace stage alone: a fresh import assembled and given its coarse level on the
+        # unmoved tip (as at residency), then moved to the served positions
+        m = fgpu.Mesh(ctx, c.X0, c.cells, *serving.POISSON_MARKS)
+        s = fgpu.System(m)
+        s.reset()
+        s.add_dirichlet(2, 0.0)
+        s.add_neumann(8, None, c.field)
+        s.apply_dirichlet()
+        s.set_coarse(srv._aggregate(m.n_dofs))
+        m.update_vertices(r.positions)
+        s.reassemble()
+        s.set_solution(r.x_init)
+        st = s.solve(o)
+        assert st.iterations == r.stage_iterations[0] and np.array_equal(s.solution(), r.snaps[0])
+        # current and heat stages alone: both systems set up on the unmoved tip as at residency
+        # (the heat step from the initial temperature and a zero current), then moved and
+        # reassembled from the served inputs
+        bm = fgpu.Mesh(ctx, c.X0, c.bulk_cells, *serving.HEAT_MARKS)
+        J, N = c.surface_loads(bm.face_centroids(2))
+        agg = srv._aggregate(bm.n_dofs)
+        cur = fgpu.System(bm)
+        cur.reset()
+        cur.add_neumann(2, J, 0.0)
+        cur.add_dirichlet(7, 0.0)
+        cur.apply_dirichlet()
+        cur.fill_solution(0.0)
+        cur.set_coarse(agg)
+        heat = fgpu.System(bm)
+        xyz, _ = bm.dofs()
+        heat.set_solution(c.initial_temperature(xyz))
+        heat.reset()
+        heat.heat_terms(cur, serving.CU_RHO_CP / c.dt, serving.LORENTZ)
+        heat.add_neumann(2, N, 0.0)
+        heat.add_dirichlet(7, serving.T_AMBIENT)
+        heat.apply_dirichlet()
+        assert np.array_equal(heat.solution(), res.T0)
+        heat.set_coarse(agg)
+        bm.update_vertices(r.bpositions)
+        cur.reassemble()
+        p = prev.get(r.client)
+        cur.set_solution(p[1] if p else np.zeros(bm.n_dofs))
+        st = cur.solve(o)
+        assert st.iterations == r.stage_iterations[1] and np.array_equal(cur.solution(), r.snaps[1])
+        T_old = p[2] if p else res.T0
+        heat.set_solution(T_old)
+        heat.reassemble()
+        st = heat.solve(o)
+        assert st.iterations == r.stage_iterations[2] and np.array_equal(heat.solution(), r.snaps[2])
+        if r.client == 0 and r.seq == 1:
+            _, xo, it, _, _ = O.poisson_laplace(r.positions, c.cells, c.field, tol=1e-13, relative=True)
+            assert it > 0 and np.linalg.norm(r.snaps[0] - xo) <= 1e-8 * np.linalg.norm(xo)
+            om, co, it = O.current_solve(r.bpositions, c.bulk_cells, J, tol=1e-13)
+            assert it > 0 and np.linalg.norm(r.snaps[1] - co) <= 1e-8 * np.linalg.norm(co)
+            To, it = O.heat_step(om, T_old, r.snaps[1], c.dt, N, serving.T_AMBIENT, tol=1e-13)
+            assert it > 0 and np.linalg.norm(r.snaps[2] - To) <= 1e-8 * np.linalg.norm(To)
+        prev[r.client] = r.snaps

@@ -453,3 +453,26 @@ def test_heat_step_amplifies_an_eigenmode_by_the_theta_factor(theta):
         g = (1 - (1 - theta) * lam[j]) / (1 + theta * lam[j])
         assert np.abs(T - g * eps * v).max() <= 1e-9 * eps, (j, lam[j])
     assert lam[-1] > 10                       # stiff modes: the two schemes differ in sign there
+
+
+def test_face_centroids_of_a_box_grid():
+    """Mesh.face_centroids (DealSolver::export_surface_centroids): on a regular grid the top
+    faces' centres are (x_c, y_c, z_top) of their cells, listed in cell order; a sheared grid
+    moves them with the shear."""
+    n, h = (3, 2, 4), 0.5
+    ijk = np.stack(np.meshgrid(*[np.arange(k) for k in n], indexing="ij"), -1).reshape(-1, 3)
+    vid = lambda i, j, k: (i * (n[1] + 1) + j) * (n[2] + 1) + k
+    lex = np.stack([vid(ijk[:, 0] + (l & 1), ijk[:, 1] + ((l >> 1) & 1), ijk[:, 2] + (l >> 2)) for l in range(8)], 1)
+    ucd = np.empty_like(lex)
+    ucd[:, M.UCD_TO_LEX] = lex
+    P = np.stack(np.meshgrid(*[np.arange(k + 1) * h for k in n], indexing="ij"), -1).reshape(-1, 3)
+    for shear in (0.0, 0.3):
+        X = P.copy()
+        X[:, 0] += shear * X[:, 2]
+        m = O.Mesh(X, ucd, 8, 2, 4, 2)
+        top = ijk[:, 2] == n[2] - 1
+        zt = n[2] * h
+        ref = np.stack([(ijk[top, 0] + 0.5) * h + shear * zt, (ijk[top, 1] + 0.5) * h, np.full(top.sum(), zt)], 1)
+        c = m.face_centroids(8)
+        assert c.shape == ref.shape
+        assert np.abs(c - ref).max() <= 1e-14

@@ -86,3 +86,46 @@ def test_boundary_quadrangles(split):
             assert np.allclose(out[q[2]], nodes[tr].mean(0), atol=1e-12)
             xy = out[[q[0], q[1], q[3], q[2]], :2]
             assert tethex._measure_2d(xy) >= 0 or np.isclose(tethex._measure_2d(xy), 0.0, atol=1e-9)
+
+
+def _volume_lex(x):
+    """Exact volume of a trilinear hex (lexicographic 8 x 3): det J is of degree <= 2 in each
+    reference coordinate, so QGauss(2) integrates it exactly.  This is the quantity the
+    reference's cell_measure_3D (src/Tethex.cpp:1603-1630) evaluates in closed form."""
+    g = (0.5 - 0.5 / np.sqrt(3.0), 0.5 + 0.5 / np.sqrt(3.0))
+    vol = 0.0
+    for z in g:
+        for y in g:
+            for xx in g:
+                J = np.zeros((3, 3))
+                for n in range(8):
+                    a, b, c = n & 1, (n >> 1) & 1, n >> 2
+                    N = (xx if a else 1 - xx, y if b else 1 - y, z if c else 1 - z)
+                    d = (1 if a else -1, 1 if b else -1, 1 if c else -1)
+                    J += np.outer(x[n], [d[0] * N[1] * N[2], N[0] * d[1] * N[2], N[0] * N[1] * d[2]])
+                vol += np.linalg.det(J) / 8.0
+    return vol
+
+
+def test_orientation_predicate_agrees_with_cell_measure():
+    """The split orients each hex by the sign of its centre Jacobian; the reference swaps the
+    front and back faces when cell_measure_3D < 0.  A sub-hex of the split is the image of one
+    reference sub-hex under the affine map of its tet, so both signs are sign(det A) for every
+    tet, slivers included: checked on random tets, flat slivers (height 1e-7 of the edge) and
+    needles, before (both orders) and after the orientation fix."""
+    rng = np.random.default_rng(17)
+    tets_x = [rng.normal(size=(4, 3)) for _ in range(40)]
+    for h in (1e-3, 1e-5, 1e-7):
+        base = rng.normal(size=(3, 3))
+        base[:, 2] = 0.0
+        apex = base.mean(0) + [0.1, -0.2, h]
+        tets_x.append(np.vstack([base, apex]))                  # sliver: apex near the base plane
+        tets_x.append(np.array([[0, 0, 0], [1, h, 0], [1, 0, h], [1, h, h]], float))   # needle
+    for P in tets_x:
+        nodes, kind, hexes, _, _, _ = tethex.split(P, np.array([[0, 1, 2, 3]]))
+        for hx in hexes:
+            x = nodes[hx[LEX]]
+            c, v = tethex._center_det_lex(x), _volume_lex(x)
+            assert c > 0 and v > 0
+            xf = x[[4, 5, 6, 7, 0, 1, 2, 3]]                     # front and back swapped
+            assert tethex._center_det_lex(xf) < 0 and _volume_lex(xf) < 0
