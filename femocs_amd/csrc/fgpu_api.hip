@@ -803,7 +803,7 @@ static void ensure_metric(fgpu_mesh *m) {
         HIPCHECK(upload_affine_table(st));
         m->ctx->affine_table = true;
     }
-    if (!m->d_metric) m->d_metric = dalloc<float>(6 * (size_t)std::max(1, m->n_cells));
+    if (!m->d_metric) m->d_metric = dalloc<float>(7 * (size_t)std::max(1, m->n_cells));
     const int nb = metric_blocks(m->n_cells);
     double *dev = dalloc<double>(nb);
     launch_metric(st, m->n_cells, m->d_cells, m->d_layout, m->d_X64, m->d_metric, dev);
@@ -824,7 +824,7 @@ static Desc make_desc32(fgpu_sys *s, bool coarse = false) {
     Desc d{};
     fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
     ensure_metric(m);
-    d.metric = (m->affine && m->affine_allowed && !s->have_coef) ? m->d_metric : nullptr;
+    d.metric = (m->affine && m->affine_allowed) ? m->d_metric : nullptr;
     s->desc32_affine = d.metric != nullptr;   // sizes the LDS of the launches using it
     d.n_dofs = m->n_dofs;
     d.n_cells = m->n_cells;
@@ -1002,9 +1002,10 @@ struct BatchTabs {
     size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the element kernel, largest patch of the batch
 };
 
-// coordinates and inputs per node (inputs only for the tet-split FP32 form), outputs per
-// cell, entry lists
+// coordinates and inputs per node (inputs only for the tet-split FP32 form); FP32: one 64-bit
+// fixed-point sum per node and four floats; FP64: outputs per cell, entry lists
 static size_t patch_lds(int node_cap, int cells, size_t tsize, bool affine = false) {
+    if (tsize == 4) return (affine ? 1 : 4) * (size_t)node_cap * 4 + 8 * (size_t)node_cap + 16;
     return ((affine ? 1 : 4) * (size_t)node_cap + 8 * (size_t)cells) * tsize + 16 * (size_t)cells;
 }
 

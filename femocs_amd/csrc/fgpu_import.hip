@@ -47,32 +47,13 @@ struct ImpHipErr : std::runtime_error {
         if (e_ != hipSuccess) throw ImpHipErr(std::string(#call) + ": " + hipGetErrorString(e_), e_); \
     } while (0)
 
-// temporaries of one import: a bump allocator over a few large blocks, freed together (one
-// allocation instead of dozens; each hipFree costs a fraction of a millisecond)
-struct Bufs {
-    std::vector<void *> blocks;
-    char *cur = nullptr;
-    size_t left = 0, chunk;
-    explicit Bufs(size_t chunk_bytes) : chunk(chunk_bytes) {}
-    template <class T> T *get(size_t n) {
-        const size_t bytes = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
-        if (bytes > left) {
-            const size_t b = std::max(bytes, blocks.empty() ? chunk : chunk / 4);
-            void *q = nullptr;
-            IMCHECK(hipMalloc(&q, b));
-            blocks.push_back(q);
-            cur = static_cast<char *>(q);
-            left = b;
-        }
-        T *r = reinterpret_cast<T *>(cur);
-        cur += bytes;
-        left -= bytes;
-        return r;
-    }
-    ~Bufs() {
-        for (void *q : blocks) (void)hipFree(q);
+// temporaries of one import (allocation failures raise ImpHipErr)
+struct ImpFail {
+    [[noreturn]] static void raise(hipError_t e, const char *what) {
+        throw ImpHipErr(std::string(what) + ": " + hipGetErrorString(e), e);
     }
 };
+using Bufs = DeviceArena<ImpFail>;
 // device arrays the mesh keeps (freed by the mesh)
 template <class T> T *keep(size_t n) {
     void *q = nullptr;
@@ -582,13 +563,13 @@ template <class K, class V>
 void sort_pairs(Bufs &B, const K *kin, K *kout, const V *vin, V *vout, int n, int end_bit, hipStream_t st) {
     size_t tb = 0;
     IMCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, n, 0, end_bit, st));
-    void *tmp = B.get<unsigned char>(tb);
+    void *tmp = B.scratch(tb);
     IMCHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, vin, vout, n, 0, end_bit, st));
 }
 template <class T> void exclusive_sum(Bufs &B, const T *in, T *out, int n, hipStream_t st) {
     size_t tb = 0;
     IMCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, st));
-    void *tmp = B.get<unsigned char>(tb);
+    void *tmp = B.scratch(tb);
     IMCHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, st));
 }
 int read_int(const int *d, hipStream_t st) {

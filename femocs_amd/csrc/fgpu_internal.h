@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -96,7 +97,7 @@ struct Desc {
     const double *b;              // FP64 rhs (residual mode)
     double *scal;                 // kScal doubles
     double *part;                 // kPartRegions regions of nb_dof per-block partial sums
-    const float *metric;          // FP32 apply of tet-split meshes: 6 per cell, or null
+    const float *metric;          // FP32 apply of tet-split meshes: 6 per cell, then |det A| per cell, or null
     // two-level preconditioner of the FP32 CG (null: Jacobi only): coarse correction c per
     // dof block (T), nc x nc inverse of the coarse operator (row-major), nc = dof blocks
     void *cc;
@@ -146,7 +147,7 @@ struct Mesh {
     // every cell an affine image of the tet-split reference sub-hex: the FP32 operator reads
     // a per-cell metric instead of the node coordinates (valid for coordinate generation
     // metric_gen)
-    float *d_metric = nullptr;             // 6 per cell (internal order)
+    float *d_metric = nullptr;             // 6 per cell, then |det A| per cell (internal order)
     int metric_gen = -1;
     bool affine = false;
     bool affine_allowed = true;
@@ -270,6 +271,52 @@ struct fgpu_ctx {
     bool stage_busy[kStage] = {};
     int stage_next = 0;
 };
+
+namespace fgpu {
+// Temporaries of one device build (import, tet split, interpolator tables): a bump allocator
+// over a few large device blocks freed together (one allocation instead of dozens; each
+// hipFree costs a fraction of a millisecond), plus one reusable region for the temporary
+// storage of the stream-ordered hipcub sorts and scans (each call may reuse what the previous
+// one used).  Fail::raise(hipError_t, const char *) reports a failed allocation.
+template <class Fail> struct DeviceArena {
+    std::vector<void *> blocks;
+    char *cur = nullptr;
+    size_t left = 0, chunk;
+    void *scr = nullptr;
+    size_t scr_bytes = 0;
+    explicit DeviceArena(size_t chunk_bytes) : chunk(chunk_bytes) {}
+    void *alloc(size_t b) {
+        void *q = nullptr;
+        const hipError_t e = hipMalloc(&q, b);
+        if (e != hipSuccess) Fail::raise(e, "hipMalloc (device arena block)");
+        blocks.push_back(q);
+        return q;
+    }
+    template <class T> T *get(size_t n) {
+        const size_t bytes = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
+        if (bytes > left) {
+            const size_t b = std::max(bytes, blocks.empty() ? chunk : chunk / 4);
+            cur = static_cast<char *>(alloc(b));
+            left = b;
+        }
+        T *r = reinterpret_cast<T *>(cur);
+        cur += bytes;
+        left -= bytes;
+        return r;
+    }
+    // hipcub temporary storage of at least `bytes` (valid until the next call)
+    void *scratch(size_t bytes) {
+        if (bytes > scr_bytes) {
+            scr_bytes = std::max(bytes, 2 * scr_bytes);
+            scr = alloc(scr_bytes);
+        }
+        return scr;
+    }
+    ~DeviceArena() {
+        for (void *q : blocks) (void)hipFree(q);
+    }
+};
+}  // namespace fgpu
 
 struct fgpu_mesh : fgpu::Mesh {};
 struct fgpu_sys : fgpu::Sys {};

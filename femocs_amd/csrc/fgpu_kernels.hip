@@ -365,12 +365,14 @@ extern __shared__ __attribute__((aligned(16))) unsigned char fg_smem[];
 static const double kSubHex[8][3] = {{0, 0, 0}, {0.5, 0, 0}, {0, 0, 0.5}, {1.0 / 3, 0, 1.0 / 3},
                                      {0, 0.5, 0}, {1.0 / 3, 1.0 / 3, 0}, {0, 1.0 / 3, 1.0 / 3}, {0.25, 0.25, 0.25}};
 __constant__ float kAffP[8][9];
+// mass weights of the reference sub-hex: w_q |det J_ref,q| (the cell's |det A| scales them)
+__constant__ float kAffW[8];
 __constant__ double kSubHexD[8][3] = {{0, 0, 0}, {0.5, 0, 0}, {0, 0, 0.5}, {1.0 / 3, 0, 1.0 / 3},
                                       {0, 0.5, 0}, {1.0 / 3, 1.0 / 3, 0}, {0, 1.0 / 3, 1.0 / 3}, {0.25, 0.25, 0.25}};
 
 hipError_t upload_affine_table(hipStream_t s) {
     const double g[2] = {FG_G0, FG_G1};
-    float P[8][9];
+    float P[8][9], Wm[8];
     for (int q = 0; q < 8; ++q) {
         const double xi[3] = {g[q & 1], g[(q >> 1) & 1], g[(q >> 2) & 1]};
         double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -401,9 +403,12 @@ hipError_t upload_affine_table(hipStream_t s) {
         const double sc = std::sqrt(std::fabs(0.125 * det)) / det;
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) P[q][3 * i + j] = (float)(C[i][j] * sc);
+        Wm[q] = (float)std::fabs(0.125 * det);
     }
     (void)s;
-    return hipMemcpyToSymbol(HIP_SYMBOL(kAffP), P, sizeof(P), 0, hipMemcpyHostToDevice);
+    const hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(kAffP), P, sizeof(P), 0, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(kAffW), Wm, sizeof(Wm), 0, hipMemcpyHostToDevice);
 }
 
 // y = K_e u for a tet-split hex with metric M (m00 m01 m02 m11 m12 m22)
@@ -435,8 +440,77 @@ __device__ __forceinline__ void q1_cell_affine(const float M[6], const float u[8
     dref_t<float>(S2, 2, y);
 }
 
-// per cell: the metric of the tet-split form and, per block, the largest deviation of the
-// nodes from that form relative to the frame size
+// y = K_e u for a tet-split hex with metric M and per-point stiffness and mass coefficients
+// kq, gq (lexicographic Gauss points): sum_q kq (P_q g)^T M (P_q g) + gq |det A| w_q |det J_ref,q|
+// phi phi^T.  |det A| is passed in: det M equals it, but the FP32 metric of a flat cell does not
+// fix its determinant (cancellation)
+__device__ __forceinline__ void q1_cell_affine_coef(const float M[6], const float detA, const float kq[8], const float gq[8],
+                                                    const float u[8], float y[8]) {
+    float du0[2][2], du1[2][2], du2[2][2];
+    dref<float>(u, 0, du0);
+    dref<float>(u, 1, du1);
+    dref<float>(u, 2, du2);
+    // values at the Gauss points (three 1D interpolation sweeps)
+    float a1[8], a2[8], uq[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int a = q & 1, r = q & 6;
+        a1[q] = Wt<float>(a, 0) * u[r] + Wt<float>(a, 1) * u[r + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int b = (q >> 1) & 1, r = q & 5;
+        a2[q] = Wt<float>(b, 0) * a1[r] + Wt<float>(b, 1) * a1[r + 2];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int c = (q >> 2) & 1, r = q & 3;
+        uq[q] = Wt<float>(c, 0) * a2[r] + Wt<float>(c, 1) * a2[r + 4];
+    }
+    float S0[2][2] = {{0, 0}, {0, 0}}, S1[2][2] = {{0, 0}, {0, 0}}, S2[2][2] = {{0, 0}, {0, 0}};
+    float mq[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int a = q & 1, b = (q >> 1) & 1, c = (q >> 2) & 1;
+        const float g0 = du0[b][c], g1 = du1[a][c], g2 = du2[a][b];
+        const float *P = kAffP[q];
+        const float s0 = P[0] * g0 + P[1] * g1 + P[2] * g2;
+        const float s1 = P[3] * g0 + P[4] * g1 + P[5] * g2;
+        const float s2 = P[6] * g0 + P[7] * g1 + P[8] * g2;
+        const float k = kq[q];
+        const float t0 = k * (M[0] * s0 + M[1] * s1 + M[2] * s2);
+        const float t1 = k * (M[1] * s0 + M[3] * s1 + M[4] * s2);
+        const float t2 = k * (M[2] * s0 + M[4] * s1 + M[5] * s2);
+        S0[b][c] += P[0] * t0 + P[3] * t1 + P[6] * t2;
+        S1[a][c] += P[1] * t0 + P[4] * t1 + P[7] * t2;
+        S2[a][b] += P[2] * t0 + P[5] * t1 + P[8] * t2;
+        mq[q] = gq[q] * detA * kAffW[q] * uq[q];
+    }
+#pragma unroll
+    for (int n = 0; n < 8; ++n) y[n] = 0;
+    dref_t<float>(S0, 0, y);
+    dref_t<float>(S1, 1, y);
+    dref_t<float>(S2, 2, y);
+    float b2[8], b1[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int k = (r >> 2) & 1, base = r & 3;
+        b2[r] = Wt<float>(0, k) * mq[base] + Wt<float>(1, k) * mq[base + 4];
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int j = (r >> 1) & 1, base = r & 5;
+        b1[r] = Wt<float>(0, j) * b2[base] + Wt<float>(1, j) * b2[base + 2];
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int i = r & 1, base = r & 6;
+        y[r] += Wt<float>(0, i) * b1[base] + Wt<float>(1, i) * b1[base + 1];
+    }
+}
+
+// per cell: the metric of the tet-split form (metric[6 c ..]), |det A| (metric[6 nc + c]) and,
+// per block, the largest deviation of the nodes from that form relative to the frame size
 __global__ __launch_bounds__(kBlock) void k_metric(int nc, const int *__restrict__ cells,
                                                    const unsigned char *__restrict__ layout, const double4 *__restrict__ X,
                                                    float *__restrict__ metric, double *__restrict__ devmax) {
@@ -481,6 +555,7 @@ __global__ __launch_bounds__(kBlock) void k_metric(int nc, const int *__restrict
             const double *a = r[ij[k][0]], *b = r[ij[k][1]];
             metric[6 * (size_t)c + k] = (float)((a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / fabs(det));
         }
+        metric[6 * (size_t)nc + c] = (float)fabs(det);
         // a positively oriented cell has det(A) < 0 as laid out, det(A) > 0 when eta-flipped
         if (!(flip ? det > 0 : det < 0)) dev = 1e300;
     }
@@ -517,13 +592,21 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
     const double active = ACTIVE_ONLY ? G(d.scal)[S_ACTIVE] : 1.0;
     using V4 = typename Vec4T<T>::type;
     // FP32 apply of a tet-split mesh: per-cell metric, no coordinates (uniform per workgroup)
-    const bool affine = AFF || (sizeof(T) == 4 && !COEF && MODE == EM_APPLY && d.metric != nullptr);
-    // dynamic LDS: node coordinates (not for the tet-split form) and inputs (node_cap each),
-    // cell outputs, entry lists
+    // (with COEF: per-point coefficients on top of the metric)
+    const bool affine = AFF || (sizeof(T) == 4 && MODE == EM_APPLY && d.metric != nullptr);
+    // FP32: the per-node sums are exact integer sums of fixed-point contributions (LDS 64-bit
+    // atomics, scale per patch), so no entry lists are read and the order does not matter;
+    // FP64: the cell outputs are summed per node along the entry lists in a fixed order
+    constexpr bool FX = sizeof(T) == 4;
+    // dynamic LDS: node coordinates (not for the tet-split form) and inputs (node_cap each);
+    // FP32: fixed-point node sums (node_cap) and the patch's largest output per wave;
+    // FP64: cell outputs, entry lists
     T *sx = reinterpret_cast<T *>(fg_smem);
     T *sy = sx + d.node_cap, *sz = sy + d.node_cap;
     T *su = affine ? sx : sz + d.node_cap, *sout = su + d.node_cap;
     uint4 *slent = reinterpret_cast<uint4 *>(sout + 8 * d.pcells);
+    unsigned long long *sacc = reinterpret_cast<unsigned long long *>(su + d.node_cap);
+    float *smax = reinterpret_cast<float *>(sacc + d.node_cap);
     const V4 *X = reinterpret_cast<const V4 *>(d.X);
     const auto ua = G(reinterpret_cast<const T *>(d.ua));
     const T *ub = reinterpret_cast<const T *>(d.ub);
@@ -540,14 +623,26 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
         // unconditional and none waits for another of its wave.
         const int ct = tid < nc ? tid : 0;
         const uint4 packed = ldg(reinterpret_cast<const uint4 *>(d.plidx) + c0 + ct);
-        const uint4 lent = ldg(reinterpret_cast<const uint4 *>(d.plent) + c0 + ct);
+        uint4 lent = make_uint4(0, 0, 0, 0);
+        if constexpr (!FX) lent = ldg(reinterpret_cast<const uint4 *>(d.plent) + c0 + ct);
         float2 mq[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
-        if constexpr (sizeof(T) == 4 && !COEF && MODE == EM_APPLY) {
+        float4 kc[2] = {}, gc[2] = {};
+        float vdet = 0.f;
+        if constexpr (sizeof(T) == 4 && MODE == EM_APPLY) {
             if (affine) {
                 const float2 *mp = reinterpret_cast<const float2 *>(d.metric + 6 * (size_t)(c0 + ct));
                 mq[0] = ldg(mp);
                 mq[1] = ldg(mp + 1);
                 mq[2] = ldg(mp + 2);
+                if constexpr (COEF) {
+                    const float4 *kp = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(d.kq) + 8 * (size_t)(c0 + ct));
+                    const float4 *gp = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(d.gq) + 8 * (size_t)(c0 + ct));
+                    kc[0] = ldg4<float4, float>(kp);
+                    kc[1] = ldg4<float4, float>(kp + 1);
+                    gc[0] = ldg4<float4, float>(gp);
+                    gc[1] = ldg4<float4, float>(gp + 1);
+                    vdet = G(d.metric)[6 * (size_t)d.n_cells + c0 + ct];
+                }
             }
         }
         int eb[kRounds], ee[kRounds], os[kRounds], nd[kRounds];
@@ -555,14 +650,20 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
         for (int r = 0; r < kRounds; ++r) {
             const int j = tid + r * kBlock;
             const int jj = j < np ? j : 0;
-            eb[r] = G(d.plptr)[n0 + p + jj];
-            ee[r] = G(d.plptr)[n0 + p + jj + 1];
+            if constexpr (!FX) {
+                eb[r] = G(d.plptr)[n0 + p + jj];
+                ee[r] = G(d.plptr)[n0 + p + jj + 1];
+            } else {
+                eb[r] = ee[r] = 0;
+            }
             os[r] = G(d.pslot)[n0 + jj];
             nd[r] = G(d.pnodes)[n0 + jj];
         }
         // the node lists had to arrive before the second wave is addressed: store the cell
         // entry lists meanwhile (keeps their load in the first wave)
-        if (tid < nc) slent[tid] = lent;
+        if constexpr (!FX) {
+            if (tid < nc) slent[tid] = lent;
+        }
         if (ACTIVE_ONLY && check && active == 0.0) return false;   // uniform: the workgroup leaves
         const T beta = (T)beta64;
         T va[kRounds], vb[kRounds], vc[kRounds];
@@ -585,6 +686,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
         for (int r = 0; r < kRounds; ++r) {
             const int j = tid + r * kBlock;
             if (j < np) {
+                if constexpr (FX) sacc[j] = 0ull;
                 if (!affine) {
                     sx[j] = xs[r].x;
                     sy[j] = xs[r].y;
@@ -603,8 +705,10 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
             }
         }
         __syncthreads();
-        // phase 2: one cell per thread, from LDS
-        if constexpr (sizeof(T) == 4 && !COEF && MODE == EM_APPLY) {
+        // phase 2: one cell per thread, from LDS (FP32: outputs yc for local nodes al)
+        T yc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int al[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if constexpr (sizeof(T) == 4 && MODE == EM_APPLY) {
             if (affine && tid < nc) {
                 // an eta-flipped cell (bit 15 of its first index) is computed in flipped order
                 const bool flip = (packed.x & 0x8000u) != 0;
@@ -618,9 +722,25 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
                 float u[8], y[8];
 #pragma unroll
                 for (int n = 0; n < 8; ++n) u[n] = su[lid[n]];
-                q1_cell_affine(M, u, y);
+                if constexpr (COEF) {
+                    // the Gauss points of a flipped cell flip with its vertices
+                    const float k0[8] = {kc[0].x, kc[0].y, kc[0].z, kc[0].w, kc[1].x, kc[1].y, kc[1].z, kc[1].w};
+                    const float g0[8] = {gc[0].x, gc[0].y, gc[0].z, gc[0].w, gc[1].x, gc[1].y, gc[1].z, gc[1].w};
+                    float kf[8], gf[8];
 #pragma unroll
-                for (int n = 0; n < 8; ++n) sout[8 * tid + n] = flip ? y[kEtaFlip[n]] : y[n];
+                    for (int q = 0; q < 8; ++q) {
+                        kf[q] = flip ? k0[kEtaFlip[q]] : k0[q];
+                        gf[q] = flip ? g0[kEtaFlip[q]] : g0[q];
+                    }
+                    q1_cell_affine_coef(M, vdet, kf, gf, u, y);
+                } else {
+                    q1_cell_affine(M, u, y);
+                }
+#pragma unroll
+                for (int n = 0; n < 8; ++n) {
+                    yc[n] = y[n];
+                    al[n] = lid[n];
+                }
             }
         }
         if (!AFF && tid < nc && !affine) {
@@ -679,18 +799,60 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
                 }
             }
 #pragma unroll
-            for (int n = 0; n < 8; ++n) sout[8 * tid + n] = y[n];
+            for (int n = 0; n < 8; ++n) {
+                if constexpr (FX) {
+                    yc[n] = y[n];
+                    al[n] = lid[n];
+                } else {
+                    sout[8 * tid + n] = y[n];
+                }
+            }
         }
-        __syncthreads();
-        // phase 3: per patch node, its cell contributions in ascending order
-        const uint16_t *le = reinterpret_cast<const uint16_t *>(slent);
+        if constexpr (FX) {
+            // fixed point: the patch's largest |output| m < 2^e gives the scale 2^(50 - e), so up
+            // to 2^13 contributions of a node add without overflow, each exact to m 2^-50 (far
+            // below FP32 rounding).  A non-finite output makes the patch's outputs NaN.
+            float m = 0.f;
+            if (tid < nc) {
 #pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-            const int j = tid + r * kBlock;
-            if (j < np) {
-                T s = 0;
-                for (int e = eb[r]; e < ee[r]; ++e) s += sout[le[e]];
-                Ye[os[r]] = s;
+                for (int n = 0; n < 8; ++n) {
+                    const float a = fabsf((float)yc[n]);
+                    m = (a <= m) ? m : (a == a ? a : __builtin_inff());
+                }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+            if ((tid & 63) == 0) smax[tid >> 6] = m;
+            __syncthreads();
+            m = fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3]));
+            const bool finite = m < __builtin_inff();
+            int e = 0;
+            (void)frexpf(finite ? m : 0.f, &e);
+            const double sc = (finite && m > 0.f) ? ldexp(1.0, 50 - e) : 1.0;
+            if (tid < nc && finite) {
+#pragma unroll
+                for (int n = 0; n < 8; ++n)
+                    atomicAdd(&sacc[al[n]], (unsigned long long)__double2ll_rn((double)yc[n] * sc));
+            }
+            __syncthreads();
+            const double isc = 1.0 / sc;            // a power of two: exact
+#pragma unroll
+            for (int r = 0; r < kRounds; ++r) {
+                const int j = tid + r * kBlock;
+                if (j < np) Ye[os[r]] = finite ? (T)((double)(long long)sacc[j] * isc) : (T)__builtin_nanf("");
+            }
+        } else {
+            __syncthreads();
+            // phase 3: per patch node, its cell contributions in ascending order
+            const uint16_t *le = reinterpret_cast<const uint16_t *>(slent);
+#pragma unroll
+            for (int r = 0; r < kRounds; ++r) {
+                const int j = tid + r * kBlock;
+                if (j < np) {
+                    T s = 0;
+                    for (int e = eb[r]; e < ee[r]; ++e) s += sout[le[e]];
+                    Ye[os[r]] = s;
+                }
             }
         }
         __syncthreads();
@@ -1098,9 +1260,14 @@ void launch_elem(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool c
                  size_t lds, bool affine_all) {
     if (nblk <= 0) return;
     const dim3 g(nblk), b(kBlock);
-    if (sizeof(T) == 4 && affine_all && !coef && !maskin) {
-        if (active_only) hipLaunchKernelGGL((k_patch<T, false, false, true, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr);
-        else hipLaunchKernelGGL((k_patch<T, false, false, false, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr);
+    if (sizeof(T) == 4 && affine_all && !maskin) {
+        if (coef) {
+            if (active_only) hipLaunchKernelGGL((k_patch<T, true, false, true, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr);
+            else hipLaunchKernelGGL((k_patch<T, true, false, false, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr);
+        } else {
+            if (active_only) hipLaunchKernelGGL((k_patch<T, false, false, true, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr);
+            else hipLaunchKernelGGL((k_patch<T, false, false, false, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr);
+        }
         return;
     }
 #define FG_ELEM(C, M, A) hipLaunchKernelGGL((k_patch<T, C, M, A, EM_APPLY>), g, b, lds, s, D, tab, scal_slot, nullptr)

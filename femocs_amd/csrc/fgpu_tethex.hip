@@ -42,32 +42,13 @@ struct TxErr : std::runtime_error {
         if (e_ != hipSuccess) throw TxErr(std::string(#call) + ": " + hipGetErrorString(e_));     \
     } while (0)
 
-// device buffers of one split: a bump allocator over a few large blocks, freed together (one
-// allocation instead of dozens; each hipFree costs a fraction of a millisecond)
-struct Bufs {
-    std::vector<void *> blocks;
-    char *cur = nullptr;
-    size_t left = 0, chunk;
-    explicit Bufs(size_t chunk_bytes) : chunk(chunk_bytes) {}
-    template <class T> T *get(size_t n) {
-        const size_t bytes = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
-        if (bytes > left) {
-            const size_t b = std::max(bytes, blocks.empty() ? chunk : chunk / 4);
-            void *q = nullptr;
-            TXCHECK(hipMalloc(&q, b));
-            blocks.push_back(q);
-            cur = static_cast<char *>(q);
-            left = b;
-        }
-        T *r = reinterpret_cast<T *>(cur);
-        cur += bytes;
-        left -= bytes;
-        return r;
-    }
-    ~Bufs() {
-        for (void *q : blocks) (void)hipFree(q);
+// device buffers of one split (allocation failures raise TxErr)
+struct TxFail {
+    [[noreturn]] static void raise(hipError_t e, const char *what) {
+        throw TxErr(std::string(what) + ": " + hipGetErrorString(e));
     }
 };
+using Bufs = fgpu::DeviceArena<TxFail>;
 
 constexpr int kLexOfUcd[8] = {0, 1, 4, 5, 3, 2, 7, 6};
 
