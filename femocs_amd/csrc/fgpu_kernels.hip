@@ -296,7 +296,7 @@ template <class T> __device__ __forceinline__ T coarse_term(const Desc &d, int i
 }
 
 // Slot sums of the kDofIter dofs a thread of a dof workgroup handles (base + threadIdx.x +
-// m * kBlock), in the same order as gather_slots.  The slot offsets of all of them are loaded
+// m * kBlock), each in ascending slot order.  The slot offsets of all of them are loaded
 // first, then up to four slots each in one wave (indices clamped into the dof's own range, the
 // extra values masked to 0), so a thread waits twice instead of once per slot; dofs with more
 // than four slots (rare) finish in a loop.
@@ -325,18 +325,21 @@ template <class T> __device__ __forceinline__ void gather_slots4(const Desc &d, 
         T s = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) s += v[m][k];
-        for (int k = b[m] + 4; k < e[m]; ++k) s += Ye[k];
+        // long lists (Q2 vertex dofs touch about two dozen cells): four loads in flight at a
+        // time, summed in the same order
+        int k = b[m] + 4;
+        for (; k + 4 <= e[m]; k += 4) {
+            const T y0 = Ye[k], y1 = Ye[k + 1], y2 = Ye[k + 2], y3 = Ye[k + 3];
+            s += y0;
+            s += y1;
+            s += y2;
+            s += y3;
+        }
+        for (; k < e[m]; ++k) s += Ye[k];
         q[m] = s;
     }
 }
 
-template <class T> __device__ __forceinline__ T gather_slots(const Desc &d, int i) {
-    const auto Ye = G(reinterpret_cast<const T *>(d.Ye));
-    const int e = G(d.gptr)[i + 1];
-    T s = 0;
-    for (int k = G(d.gptr)[i]; k < e; ++k) s += Ye[k];
-    return s;
-}
 
 // ---------------------------------------------------------------------------------------
 // element kernel: one workgroup per patch of up to kPatchCells consecutive cells

@@ -73,6 +73,115 @@ void q2_tables(double B[3][3], double D[3][3], double W[3]) {
 
 __device__ __constant__ double kQG[3] = {0.1127016653792583, 0.5, 0.8872983346207417};
 
+// Tet-split meshes: every hex is the image of one reference sub-hex (frame coordinates of its
+// lexicographic nodes below: tet vertex, three edge midpoints, three face centroids, centroid)
+// under the affine map x = x0 + A lambda, A = [2(x1 - x0), 2(x4 - x0), 2(x2 - x0)].  With
+// J = A J_ref(xi), w |det J| J^-1 J^-T = P_q^T M P_q for the per-cell metric
+// M = |det A| (A^T A)^-1 and the constants P_q = sqrt(w_q |det J_ref,q|) J_ref,q^-T (sign of
+// det J_ref,q included, it cancels): 6 numbers per cell replace the Jacobian at each of the 27
+// Gauss points.  An eta-flipped cell (layout 1) has the form in the order n ^ 2 of its
+// vertices, i.e. with the node grid's middle index reversed.
+static const double kQ2SubHex[8][3] = {{0, 0, 0}, {0.5, 0, 0}, {0, 0, 0.5}, {1.0 / 3, 0, 1.0 / 3},
+                                       {0, 0.5, 0}, {1.0 / 3, 1.0 / 3, 0}, {0, 1.0 / 3, 1.0 / 3}, {0.25, 0.25, 0.25}};
+__constant__ double kQ2P[27][9];
+
+void q2_affine_table(double P[27][9]) {
+    const double g[3] = {0.5 - 0.5 * std::sqrt(0.6), 0.5, 0.5 + 0.5 * std::sqrt(0.6)};
+    const double w[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+    for (int q = 0; q < 27; ++q) {
+        const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
+        const double xi[3] = {g[qi], g[qj], g[qk]};
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int n = 0; n < 8; ++n) {
+            double N[3], dN[3];
+            for (int d = 0; d < 3; ++d) {
+                const int bit = (n >> d) & 1;
+                N[d] = bit ? xi[d] : 1 - xi[d];
+                dN[d] = bit ? 1 : -1;
+            }
+            const double gr[3] = {dN[0] * N[1] * N[2], N[0] * dN[1] * N[2], N[0] * N[1] * dN[2]};
+            for (int i = 0; i < 3; ++i)
+                for (int d = 0; d < 3; ++d) J[i][d] += kQ2SubHex[n][i] * gr[d];
+        }
+        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        // cofactor(J) = det J^-T
+        double C[3][3];
+        C[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        C[0][1] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        C[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        C[1][0] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+        C[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+        C[1][2] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+        C[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+        C[2][1] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+        C[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        const double sc = std::sqrt(std::fabs(w[qi] * w[qj] * w[qk] * det)) / det;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) P[q][3 * i + j] = C[i][j] * sc;
+    }
+}
+
+// per cell (internal order): the FP64 metric of the tet-split form and, per block, the largest
+// deviation of the vertices from that form relative to the frame size (1e300 for a cell whose
+// frame has the wrong orientation for its layout)
+__global__ __launch_bounds__(256) void k_q2_metric(int nc, const int *__restrict__ c27, const unsigned char *__restrict__ layout,
+                                                   const double4 *__restrict__ X, double *__restrict__ qm,
+                                                   double *__restrict__ devmax) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    double dev = 0;
+    if (c < nc) {
+        const bool flip = layout[c] != 0;
+        double x[8][3];
+        for (int n = 0; n < 8; ++n) {
+            const int v = flip ? (n ^ 2) : n;     // vertex (a, b, c) is grid node 2a + 6b + 18c
+            const double4 p = X[c27[27 * (size_t)c + 2 * (v & 1) + 6 * ((v >> 1) & 1) + 18 * (v >> 2)]];
+            x[n][0] = p.x;
+            x[n][1] = p.y;
+            x[n][2] = p.z;
+        }
+        double u[3][3];
+        for (int a = 0; a < 3; ++a) {
+            u[0][a] = 2 * (x[1][a] - x[0][a]);
+            u[1][a] = 2 * (x[4][a] - x[0][a]);
+            u[2][a] = 2 * (x[2][a] - x[0][a]);
+        }
+        const double size = sqrt(u[0][0] * u[0][0] + u[0][1] * u[0][1] + u[0][2] * u[0][2]) +
+                            sqrt(u[1][0] * u[1][0] + u[1][1] * u[1][1] + u[1][2] * u[1][2]) +
+                            sqrt(u[2][0] * u[2][0] + u[2][1] * u[2][1] + u[2][2] * u[2][2]);
+        for (int n = 0; n < 8; ++n) {
+            // the sub-hex node coordinates of the frame (kQ2SubHex, exact in binary but 1/3 and 1/4)
+            const double l0 = (n == 1) ? 0.5 : (n == 3 || n == 5) ? 1.0 / 3 : (n == 7) ? 0.25 : 0.0;
+            const double l1 = (n == 4) ? 0.5 : (n == 5 || n == 6) ? 1.0 / 3 : (n == 7) ? 0.25 : 0.0;
+            const double l2 = (n == 2) ? 0.5 : (n == 3 || n == 6) ? 1.0 / 3 : (n == 7) ? 0.25 : 0.0;
+            for (int a = 0; a < 3; ++a) {
+                const double pr = x[0][a] + l0 * u[0][a] + l1 * u[1][a] + l2 * u[2][a];
+                dev = fmax(dev, fabs(pr - x[n][a]) / size);
+            }
+        }
+        double r[3][3];
+        for (int i = 0; i < 3; ++i) {
+            const double *p = u[(i + 1) % 3], *q = u[(i + 2) % 3];
+            r[i][0] = p[1] * q[2] - p[2] * q[1];
+            r[i][1] = p[2] * q[0] - p[0] * q[2];
+            r[i][2] = p[0] * q[1] - p[1] * q[0];
+        }
+        const double det = u[0][0] * r[0][0] + u[0][1] * r[0][1] + u[0][2] * r[0][2];
+        const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+        for (int k = 0; k < 6; ++k) {
+            const double *a = r[ij[k][0]], *b = r[ij[k][1]];
+            qm[6 * (size_t)c + k] = (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / fabs(det);
+        }
+        if (!(flip ? det > 0 : det < 0)) dev = 1e300;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dev = fmax(dev, __shfl_xor(dev, o, 64));
+    __shared__ double sm[4];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = dev;
+    __syncthreads();
+    if (threadIdx.x == 0) devmax[blockIdx.x] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+}
+
 // columns c0, c1, c2 (d x / d xi_a) of the trilinear map of vertices X[8][3] (lexicographic) at
 // the reference point (x, y, z)
 __device__ __forceinline__ void q2_jacobian(const double (*X)[3], double x, double y, double z, double c[3][3]) {
@@ -114,11 +223,17 @@ struct Q2Args {
     const unsigned char *dmask;  // masked inputs (maskin)
     double *Ye;                  // dof-major slots
     int maskin, active_only;
+    const double *qm;            // tet-split form: 6 per cell (AFF kernels only)
+    const unsigned char *layout; // tet-split form: 1 for an eta-flipped cell
 };
 
-// y = K_c u for every cell, K_c(i, j) = sum_q JxW grad phi_i . grad phi_j (QGauss(3))
+// y = K_c u for every cell, K_c(i, j) = sum_q JxW grad phi_i . grad phi_j (QGauss(3)).
+// AFF: every cell has the tet-split form (per-cell metric, no coordinates); a flipped cell is
+// computed with its node grid's middle index reversed (thread column tb reads and writes row
+// 2 - tb), which relabels the nodes of the same element matrix.
+template <bool AFF>
 __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
-    __shared__ double sX[kQ2Cells][8][3];
+    __shared__ double sX[AFF ? 1 : kQ2Cells][8][3];
     __shared__ double sA[kQ2Cells][81];
     __shared__ double sB[kQ2Cells][81];
     if (a.active_only && a.scal[S_ACTIVE] == 0.0) return;
@@ -130,17 +245,24 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
     const int cc = live ? c : 0;
     const int *cd = a.c27 + 27 * (size_t)cc;
     const double beta = a.ub ? a.scal[S_BETA] : 0.0;
+    // the row of the grid this thread's column reads and writes
+    const int tr = (AFF && a.layout[cc]) ? 2 - tb : tb;
+    double M[6] = {0, 0, 0, 0, 0, 0};
+    if (AFF) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) M[k] = a.qm[6 * (size_t)cc + k];
+    }
     // column (i = ta, j = tb) of the nodal values; vertex coordinates of the cell
     double u[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const int dof = cd[ta + 3 * tb + 9 * k];
+        const int dof = cd[ta + 3 * tr + 9 * k];
         double v = a.ua[dof];
         if (a.ub) v += beta * a.ub[dof];
         if (a.maskin && a.dmask[dof]) v = 0.0;
         u[k] = v;
     }
-    if (inwg && t < 8) {
+    if (!AFF && inwg && t < 8) {
         const double4 p = a.X[cd[q2_vnode(t)]];
         sX[lc][t][0] = p.x;
         sX[lc][t][1] = p.y;
@@ -205,17 +327,31 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
                 g1 += kQB[qi][i] * t2y[i];
                 g2 += kQB[qi][i] * t2z[i];
             }
-            double cj[3][3], ad[3][3];
-            q2_jacobian(sX[lc], kQG[qi], kQG[qj], kQG[qk], cj);
-            const double det = q2_adj(cj, ad);
-            // det * grad u, then w / det * (a_k . det grad u)
-            const double vx = g0 * ad[0][0] + g1 * ad[1][0] + g2 * ad[2][0];
-            const double vy = g0 * ad[0][1] + g1 * ad[1][1] + g2 * ad[2][1];
-            const double vz = g0 * ad[0][2] + g1 * ad[1][2] + g2 * ad[2][2];
-            const double s = kQW[qi] * kQW[qj] * kQW[qk] / det;
-            hx[qi] = s * (ad[0][0] * vx + ad[0][1] * vy + ad[0][2] * vz);
-            hy[qi] = s * (ad[1][0] * vx + ad[1][1] * vy + ad[1][2] * vz);
-            hz[qi] = s * (ad[2][0] * vx + ad[2][1] * vy + ad[2][2] * vz);
+            if (AFF) {
+                // P_q^T M P_q g
+                const double *P = kQ2P[qi + 3 * qj + 9 * qk];
+                const double s0 = P[0] * g0 + P[1] * g1 + P[2] * g2;
+                const double s1 = P[3] * g0 + P[4] * g1 + P[5] * g2;
+                const double s2 = P[6] * g0 + P[7] * g1 + P[8] * g2;
+                const double t0 = M[0] * s0 + M[1] * s1 + M[2] * s2;
+                const double t1 = M[1] * s0 + M[3] * s1 + M[4] * s2;
+                const double t2 = M[2] * s0 + M[4] * s1 + M[5] * s2;
+                hx[qi] = P[0] * t0 + P[3] * t1 + P[6] * t2;
+                hy[qi] = P[1] * t0 + P[4] * t1 + P[7] * t2;
+                hz[qi] = P[2] * t0 + P[5] * t1 + P[8] * t2;
+            } else {
+                double cj[3][3], ad[3][3];
+                q2_jacobian(sX[lc], kQG[qi], kQG[qj], kQG[qk], cj);
+                const double det = q2_adj(cj, ad);
+                // det * grad u, then w / det * (a_k . det grad u)
+                const double vx = g0 * ad[0][0] + g1 * ad[1][0] + g2 * ad[2][0];
+                const double vy = g0 * ad[0][1] + g1 * ad[1][1] + g2 * ad[2][1];
+                const double vz = g0 * ad[0][2] + g1 * ad[1][2] + g2 * ad[2][2];
+                const double s = kQW[qi] * kQW[qj] * kQW[qk] / det;
+                hx[qi] = s * (ad[0][0] * vx + ad[0][1] * vy + ad[0][2] * vz);
+                hy[qi] = s * (ad[1][0] * vx + ad[1][1] * vy + ad[1][2] * vz);
+                hz[qi] = s * (ad[2][0] * vx + ad[2][1] * vy + ad[2][2] * vz);
+            }
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -270,7 +406,7 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
             double y = 0;
 #pragma unroll
             for (int q = 0; q < 3; ++q) y += kQB[q][k] * s2[q] + kQD[q][k] * s2z[q];
-            a.Ye[sp[i + 3 * j + 9 * k]] = y;
+            a.Ye[sp[i + 3 * tr + 9 * k]] = y;
         }
     }
 }
@@ -367,7 +503,12 @@ struct fgpu_q2 {
     std::vector<unsigned char> dmask;
     std::vector<double> gval;
     bool lifted = false, diag_valid = false;
+    int diag_gen = -1;                    // coordinate generation of the diagonal
     double cheb_lmax = 0;
+    // tet-split form: per-cell FP64 metric, valid for the mesh's coordinate generation metric_gen
+    double *d_qm = nullptr;
+    bool affine = false;
+    int metric_gen = -1;
     Desc *d_desc = nullptr;
     int2 *d_dtab = nullptr;
     ~fgpu_q2() {
@@ -375,6 +516,7 @@ struct fgpu_q2 {
                            &dflag, &fslot})
             q2free(*v);
         q2free(invd32);
+        q2free(d_qm);
         q2free(d_dmask);
         q2free(d_c27);
         q2free(d_spos);
@@ -478,11 +620,13 @@ void q2_upload_tables() {
     int dev = 0;
     Q2CHECK(hipGetDevice(&dev));
     if (dev < 64 && done[dev]) return;
-    double B[3][3], D[3][3], W[3];
+    double B[3][3], D[3][3], W[3], P[27][9];
     q2_tables(B, D, W);
+    q2_affine_table(P);
     Q2CHECK(hipMemcpyToSymbol(HIP_SYMBOL(kQB), B, sizeof(B), 0, hipMemcpyHostToDevice));
     Q2CHECK(hipMemcpyToSymbol(HIP_SYMBOL(kQD), D, sizeof(D), 0, hipMemcpyHostToDevice));
     Q2CHECK(hipMemcpyToSymbol(HIP_SYMBOL(kQW), W, sizeof(W), 0, hipMemcpyHostToDevice));
+    Q2CHECK(hipMemcpyToSymbol(HIP_SYMBOL(kQ2P), P, sizeof(P), 0, hipMemcpyHostToDevice));
     if (dev < 64) done[dev] = true;
 }
 
@@ -516,15 +660,50 @@ void q2_upload_desc(fgpu_q2 *s, const Desc &d) {
     Q2CHECK(hipStreamSynchronize(s->ctx->stream));
 }
 
+// the tet-split metric for the mesh's current coordinates, and whether every cell has the form
+// (the mesh's use-affine switch applies to Q2 too); synchronises when the coordinates changed
+void q2_ensure_metric(fgpu_q2 *s) {
+    fgpu_mesh *m = s->mesh;
+    if (s->metric_gen == m->coord_gen) return;
+    hipStream_t st = s->ctx->stream;
+    const int nc = s->n_cells, nb = (nc + 255) / 256;
+    if (!s->d_qm) s->d_qm = q2alloc<double>(6 * (size_t)std::max(nc, 1));
+    double *dev = q2alloc<double>(std::max(nb, 1));
+    double mx = 0;
+    try {
+        if (nc > 0 && m->d_layout) {
+            hipLaunchKernelGGL(k_q2_metric, dim3(nb), dim3(256), 0, st, nc, s->d_c27, m->d_layout,
+                               reinterpret_cast<const double4 *>(m->d_X64), s->d_qm, dev);
+            Q2CHECK(hipGetLastError());
+            std::vector<double> h(nb);
+            Q2CHECK(hipMemcpyAsync(h.data(), dev, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+            Q2CHECK(hipStreamSynchronize(st));
+            for (double v : h) mx = std::max(mx, v);
+        } else {
+            mx = 1e300;
+        }
+    } catch (...) {
+        q2free(dev);
+        throw;
+    }
+    q2free(dev);
+    s->affine = nc > 0 && mx <= 1e-9;
+    s->metric_gen = m->coord_gen;
+}
+
 void q2_apply(fgpu_q2 *s, const double *ua, const double *ub, bool maskin, bool active_only) {
+    q2_ensure_metric(s);
+    const bool aff = s->affine && s->mesh->affine_allowed;
     Q2Args a{s->n_cells, s->d_c27, s->d_spos, reinterpret_cast<const double4 *>(s->mesh->d_X64), ua, ub, s->scal,
-             s->d_dmask, s->Ye, maskin ? 1 : 0, active_only ? 1 : 0};
+             s->d_dmask, s->Ye, maskin ? 1 : 0, active_only ? 1 : 0, s->d_qm, s->mesh->d_layout};
     const int grid = (s->n_cells + kQ2Cells - 1) / kQ2Cells;
-    if (grid > 0) hipLaunchKernelGGL(k_q2_apply, dim3(grid), dim3(256), 0, s->ctx->stream, a);
+    if (grid <= 0) return;
+    if (aff) hipLaunchKernelGGL(k_q2_apply<true>, dim3(grid), dim3(256), 0, s->ctx->stream, a);
+    else hipLaunchKernelGGL(k_q2_apply<false>, dim3(grid), dim3(256), 0, s->ctx->stream, a);
 }
 
 void q2_ensure_diag(fgpu_q2 *s) {
-    if (s->diag_valid) return;
+    if (s->diag_valid && s->diag_gen == s->mesh->coord_gen) return;
     hipStream_t st = s->ctx->stream;
     const size_t ne = 27 * (size_t)s->n_cells;
     if (ne > 0)
@@ -537,6 +716,7 @@ void q2_ensure_diag(fgpu_q2 *s) {
     launch_invdiag(st, s->n_dofs, s->d_dmask, s->diag, s->invd, s->invd32);
     Q2CHECK(hipGetLastError());
     s->diag_valid = true;
+    s->diag_gen = s->mesh->coord_gen;
 }
 
 }  // namespace
@@ -956,6 +1136,16 @@ extern "C" int fgpu_q2_solve(fgpu_q2 *s, const fgpu_solve_opts *o, fgpu_solve_st
             st.iterations = st.iterations > 0 ? st.iterations + used : st.iterations - used;
         }
         if (stats) *stats = st;
+        return FGPU_OK;
+    });
+}
+
+extern "C" int fgpu_q2_uses_tet_split(fgpu_q2 *s, int *on) {
+    if (!s || !on) return FGPU_EINVAL;
+    return q2_guarded(s->ctx, [&] {
+        Q2CHECK(hipSetDevice(s->ctx->device));
+        q2_ensure_metric(s);
+        *on = (s->affine && s->mesh->affine_allowed) ? 1 : 0;
         return FGPU_OK;
     });
 }

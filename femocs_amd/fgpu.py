@@ -158,6 +158,7 @@ API = [
     ("fgpu_q2_get_rhs", C.c_int, [_P, _D]),
     ("fgpu_q2_apply", C.c_int, [_P, _D, _D]),
     ("fgpu_q2_time_apply", C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
+    ("fgpu_q2_uses_tet_split", C.c_int, [_P, C.POINTER(C.c_int)]),
 ]
 
 
@@ -637,6 +638,12 @@ class Q2System:
         ms = C.c_double()
         self._c(self.ctx.lib.fgpu_q2_time_apply(self.h, int(reps), C.byref(ms)), "fgpu_q2_time_apply")
         return ms.value
+
+    def uses_tet_split(self) -> bool:
+        """Whether the apply runs the tet-split form (per-cell metric)."""
+        on = C.c_int()
+        self._c(self.ctx.lib.fgpu_q2_uses_tet_split(self.h, C.byref(on)), "fgpu_q2_uses_tet_split")
+        return bool(on.value)
 
 
 def tethex_split(ctx: "Context", nodes, tets, tris=None):

@@ -342,7 +342,10 @@ int fgpu_interp_surface_cells(fgpu_interp *interp, int n, const double *xyz, int
  * heat system's current solution and phi_c the solution of `current` on the same mesh, sets the
  * coefficients kappa(T_old) (stiffness) and gamma (mass) at every Gauss point and adds
  * int phi_i (gamma T_old + sigma(T_old) |grad phi_c|^2) to the right-hand side.  Call after
- * fgpu_sys_reset; gamma = 3.4496e-24 / dt [J/(K A^3 s)], lorentz [W Ohm/K^2] (2.44e-8). */
+ * fgpu_sys_reset; gamma = 3.4496e-24 / dt [J/(K A^3 s)], lorentz [W Ohm/K^2] (2.44e-8).
+ * fgpu_sys_reassemble replays the step: T_old is then the heat system's solution at that time
+ * (the last step's temperature) and phi_c the current system's solution at that time; a
+ * current system destroyed while a heat system holds the term is released with that term. */
 int fgpu_sys_heat_terms(fgpu_sys *heat, const fgpu_sys *current, double gamma, double lorentz);
 /* The theta scheme of the same step (extension: the reference's HeatSolver::assemble_crank_nicolson,
  * src/CurrentHeatSolver.cpp:155-272, is disabled): stiffness coefficient theta kappa(T_old), and
@@ -438,6 +441,10 @@ int fgpu_q2_get_rhs(fgpu_q2 *q2, double *b);
 int fgpu_q2_apply(fgpu_q2 *q2, const double *x, double *y);
 /* average duration of reps element applies (HIP events on the context's stream), benchmark */
 int fgpu_q2_time_apply(fgpu_q2 *q2, int reps, double *avg_ms);
+/* whether the apply runs the tet-split form (every hex the affine image of the reference
+ * sub-hex of a tet split: a per-cell metric instead of the Jacobian at each Gauss point; the
+ * mesh's fgpu_mesh_use_affine switch turns it off); no counterpart in the reference */
+int fgpu_q2_uses_tet_split(fgpu_q2 *q2, int *on);
 
 #ifdef __cplusplus
 }

@@ -112,3 +112,45 @@ def test_q2_on_tip_mesh(ctx, small_mesh):
     assert it > 0
     # the two discretisations differ by the discretisation error, not by more
     assert np.linalg.norm(x2 - x1) <= 0.05 * np.linalg.norm(x1)
+
+
+def test_q2_tet_split_form_matches_general_form(ctx, small_mesh):
+    """The tet-split form of the Q2 apply (a per-cell metric and per-point constants instead of
+    the trilinear Jacobian at each of the 27 Gauss points) gives the general form's result to
+    rounding, on the generator's hexes and on hexes in the reference's vertex order (about half
+    of them eta-flipped, computed with the node grid's middle index reversed); the box mesh
+    (not a tet split) keeps the general form."""
+    from oracle import tethex
+    nodes, _, hexes, _, _, _ = tethex.split(small_mesh.nodes[:small_mesh.n_tetnodes], small_mesh.tets)
+    rng = np.random.default_rng(21)
+    for X, cells in ((small_mesh.nodes, small_mesh.vacuum_cells()), (nodes, hexes[small_mesh.hex_marker > 0])):
+        gm = fgpu.Mesh(ctx, X, cells, *MARKS)
+        s = fgpu.Q2System(gm)
+        assert s.uses_tet_split()
+        for _ in range(2):
+            u = rng.standard_normal(s.n_dofs)
+            assert gm.use_affine(True) and s.uses_tet_split()
+            ya = s.apply(u)
+            assert not gm.use_affine(False) and not s.uses_tet_split()
+            yg = s.apply(u)
+            assert np.linalg.norm(ya - yg) <= 1e-13 * np.linalg.norm(yg)
+        assert gm.use_affine(True)
+        # the solve on the tet-split form equals the general form's solution
+        sols = []
+        for allow in (True, False):
+            gm.use_affine(allow)
+            s.reset()
+            s.add_dirichlet(2, 0.0)
+            s.add_neumann(8, -1.0)
+            s.apply_dirichlet()
+            s.fill_solution(0.0)
+            st = s.solve(fgpu.opts(tol=1e-12, relative=True, precond=fgpu.FGPU_PC_JACOBI))
+            assert st.iterations > 0
+            sols.append(s.solution())
+        assert np.linalg.norm(sols[0] - sols[1]) <= 1e-10 * np.linalg.norm(sols[1])
+
+
+def test_q2_box_keeps_general_form(ctx, box):
+    X, ucd = box
+    s = fgpu.Q2System(fgpu.Mesh(ctx, X, ucd, *MARKS))
+    assert not s.uses_tet_split()
