@@ -367,15 +367,29 @@ extern __shared__ __attribute__((aligned(16))) unsigned char fg_smem[];
 // lambda coordinates of the reference sub-hex nodes in the frame (u1, u2, u3)
 static const double kSubHex[8][3] = {{0, 0, 0}, {0.5, 0, 0}, {0, 0, 0.5}, {1.0 / 3, 0, 1.0 / 3},
                                      {0, 0.5, 0}, {1.0 / 3, 1.0 / 3, 0}, {0, 1.0 / 3, 1.0 / 3}, {0.25, 0.25, 0.25}};
-__constant__ float kAffP[8][9];
-// mass weights of the reference sub-hex: w_q |det J_ref,q| (the cell's |det A| scales them)
-__constant__ float kAffW[8];
+// The sub-hex is symmetric under the permutations of the tet's three edge directions (with the
+// matching permutation of xi: node 1 lies on u1, node 4 on u2, node 2 on u3), which map Gauss
+// points onto Gauss points.  So P_q of the points (1,0,0), (0,1,0), (0,0,1) are row and column
+// permutations of one table, likewise for (1,1,0), (1,0,1), (0,1,1): four tables (points 0, 1,
+// 3, 7) hold all eight, P_q[i][j] = P_rep(q)[kAffRow[q][i]][kAffCol[q][j]] (checked when the
+// tables are built).  Fewer constants keep them in scalar registers.
+__constant__ float kAffP[4][9];
+// mass weights of the reference sub-hex: w_q |det J_ref,q| (the cell's |det A| scales them),
+// per representative point
+__constant__ float kAffW[4];
+constexpr int kAffRep[8] = {0, 1, 1, 2, 1, 2, 2, 3};    // table of point q (0, 1, 3, 7 -> 0..3)
+constexpr int kAffRow[8][3] = {{0, 1, 2}, {0, 1, 2}, {1, 2, 0}, {0, 1, 2}, {1, 0, 2}, {0, 2, 1}, {1, 0, 2}, {0, 1, 2}};
+constexpr int kAffCol[8][3] = {{0, 1, 2}, {0, 1, 2}, {2, 0, 1}, {0, 1, 2}, {2, 1, 0}, {0, 2, 1}, {2, 1, 0}, {0, 1, 2}};
+
+// P_q[i][j] for a compile-time q (the loops over q are unrolled)
+__device__ __forceinline__ float aff_p(int q, int i, int j) { return kAffP[kAffRep[q]][3 * kAffRow[q][i] + kAffCol[q][j]]; }
 __constant__ double kSubHexD[8][3] = {{0, 0, 0}, {0.5, 0, 0}, {0, 0, 0.5}, {1.0 / 3, 0, 1.0 / 3},
                                       {0, 0.5, 0}, {1.0 / 3, 1.0 / 3, 0}, {0, 1.0 / 3, 1.0 / 3}, {0.25, 0.25, 0.25}};
 
 hipError_t upload_affine_table(hipStream_t s) {
     const double g[2] = {FG_G0, FG_G1};
     float P[8][9], Wm[8];
+    static_assert(sizeof(kAffP) == 4 * 9 * sizeof(float), "four representative tables");
     for (int q = 0; q < 8; ++q) {
         const double xi[3] = {g[q & 1], g[(q >> 1) & 1], g[(q >> 2) & 1]};
         double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -409,9 +423,22 @@ hipError_t upload_affine_table(hipStream_t s) {
         Wm[q] = (float)std::fabs(0.125 * det);
     }
     (void)s;
-    const hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(kAffP), P, sizeof(P), 0, hipMemcpyHostToDevice);
+    // the representatives, and the symmetry every other point must satisfy exactly
+    const int rep_q[4] = {0, 1, 3, 7};
+    float R[4][9], Rw[4];
+    for (int r = 0; r < 4; ++r) {
+        for (int k = 0; k < 9; ++k) R[r][k] = P[rep_q[r]][k];
+        Rw[r] = Wm[rep_q[r]];
+    }
+    for (int q = 0; q < 8; ++q) {
+        if (Wm[q] != Rw[kAffRep[q]]) return hipErrorInvalidValue;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                if (P[q][3 * i + j] != R[kAffRep[q]][3 * kAffRow[q][i] + kAffCol[q][j]]) return hipErrorInvalidValue;
+    }
+    const hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(kAffP), R, sizeof(R), 0, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
-    return hipMemcpyToSymbol(HIP_SYMBOL(kAffW), Wm, sizeof(Wm), 0, hipMemcpyHostToDevice);
+    return hipMemcpyToSymbol(HIP_SYMBOL(kAffW), Rw, sizeof(Rw), 0, hipMemcpyHostToDevice);
 }
 
 // y = K_e u for a tet-split hex with metric M (m00 m01 m02 m11 m12 m22)
@@ -425,16 +452,15 @@ __device__ __forceinline__ void q1_cell_affine(const float M[6], const float u[8
     for (int q = 0; q < 8; ++q) {
         const int a = q & 1, b = (q >> 1) & 1, c = (q >> 2) & 1;
         const float g0 = du0[b][c], g1 = du1[a][c], g2 = du2[a][b];
-        const float *P = kAffP[q];
-        const float s0 = P[0] * g0 + P[1] * g1 + P[2] * g2;
-        const float s1 = P[3] * g0 + P[4] * g1 + P[5] * g2;
-        const float s2 = P[6] * g0 + P[7] * g1 + P[8] * g2;
+        const float s0 = aff_p(q, 0, 0) * g0 + aff_p(q, 0, 1) * g1 + aff_p(q, 0, 2) * g2;
+        const float s1 = aff_p(q, 1, 0) * g0 + aff_p(q, 1, 1) * g1 + aff_p(q, 1, 2) * g2;
+        const float s2 = aff_p(q, 2, 0) * g0 + aff_p(q, 2, 1) * g1 + aff_p(q, 2, 2) * g2;
         const float t0 = M[0] * s0 + M[1] * s1 + M[2] * s2;
         const float t1 = M[1] * s0 + M[3] * s1 + M[4] * s2;
         const float t2 = M[2] * s0 + M[4] * s1 + M[5] * s2;
-        S0[b][c] += P[0] * t0 + P[3] * t1 + P[6] * t2;
-        S1[a][c] += P[1] * t0 + P[4] * t1 + P[7] * t2;
-        S2[a][b] += P[2] * t0 + P[5] * t1 + P[8] * t2;
+        S0[b][c] += aff_p(q, 0, 0) * t0 + aff_p(q, 1, 0) * t1 + aff_p(q, 2, 0) * t2;
+        S1[a][c] += aff_p(q, 0, 1) * t0 + aff_p(q, 1, 1) * t1 + aff_p(q, 2, 1) * t2;
+        S2[a][b] += aff_p(q, 0, 2) * t0 + aff_p(q, 1, 2) * t1 + aff_p(q, 2, 2) * t2;
     }
 #pragma unroll
     for (int n = 0; n < 8; ++n) y[n] = 0;
@@ -476,18 +502,17 @@ __device__ __forceinline__ void q1_cell_affine_coef(const float M[6], const floa
     for (int q = 0; q < 8; ++q) {
         const int a = q & 1, b = (q >> 1) & 1, c = (q >> 2) & 1;
         const float g0 = du0[b][c], g1 = du1[a][c], g2 = du2[a][b];
-        const float *P = kAffP[q];
-        const float s0 = P[0] * g0 + P[1] * g1 + P[2] * g2;
-        const float s1 = P[3] * g0 + P[4] * g1 + P[5] * g2;
-        const float s2 = P[6] * g0 + P[7] * g1 + P[8] * g2;
+        const float s0 = aff_p(q, 0, 0) * g0 + aff_p(q, 0, 1) * g1 + aff_p(q, 0, 2) * g2;
+        const float s1 = aff_p(q, 1, 0) * g0 + aff_p(q, 1, 1) * g1 + aff_p(q, 1, 2) * g2;
+        const float s2 = aff_p(q, 2, 0) * g0 + aff_p(q, 2, 1) * g1 + aff_p(q, 2, 2) * g2;
         const float k = kq[q];
         const float t0 = k * (M[0] * s0 + M[1] * s1 + M[2] * s2);
         const float t1 = k * (M[1] * s0 + M[3] * s1 + M[4] * s2);
         const float t2 = k * (M[2] * s0 + M[4] * s1 + M[5] * s2);
-        S0[b][c] += P[0] * t0 + P[3] * t1 + P[6] * t2;
-        S1[a][c] += P[1] * t0 + P[4] * t1 + P[7] * t2;
-        S2[a][b] += P[2] * t0 + P[5] * t1 + P[8] * t2;
-        mq[q] = gq[q] * detA * kAffW[q] * uq[q];
+        S0[b][c] += aff_p(q, 0, 0) * t0 + aff_p(q, 1, 0) * t1 + aff_p(q, 2, 0) * t2;
+        S1[a][c] += aff_p(q, 0, 1) * t0 + aff_p(q, 1, 1) * t1 + aff_p(q, 2, 1) * t2;
+        S2[a][b] += aff_p(q, 0, 2) * t0 + aff_p(q, 1, 2) * t1 + aff_p(q, 2, 2) * t2;
+        mq[q] = gq[q] * detA * kAffW[kAffRep[q]] * uq[q];
     }
 #pragma unroll
     for (int n = 0; n < 8; ++n) y[n] = 0;
@@ -583,8 +608,10 @@ void launch_metric(hipStream_t s, int nc, const int *cells, const unsigned char 
 }
 
 // AFF: every system of the launch has the tet-split FP32 form (the general path is not
-// compiled in, which frees registers for more resident waves)
-template <class T, bool COEF, bool MASKIN, bool ACTIVE_ONLY, int MODE, bool AFF = false>
+// compiled in, which frees registers for more resident waves).  PF (with AFF): workgroups of
+// several patches load the next patch's tables while the present one runs (more registers,
+// fewer resident waves: only for launches whose systems run several patches per workgroup)
+template <class T, bool COEF, bool MASKIN, bool ACTIVE_ONLY, int MODE, bool AFF = false, bool PF = false>
 __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const Desc *__restrict__ D, const ETab *__restrict__ tab, int scal_slot,
                                                      const double *__restrict__ fq) {
     constexpr int kRounds = (kPatchNodes + kBlock - 1) / kBlock;
@@ -617,51 +644,71 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
     const double beta64 = MODE == EM_APPLY ? G(d.scal)[scal_slot] : 0.0;   // used only with ub
     const auto Ye = GW(reinterpret_cast<T *>(d.Ye));
     const int tid = threadIdx.x;
-    // one patch; the first one (check) leaves the workgroup early when the system is idle,
-    // tested only once that patch's lists are in flight.  Returns false to leave.
-    auto patch = [&](const int p, const int c0, const int nc, const int n0, const int np, const bool check) -> bool {
-        // phase 1: every global load of the patch is issued here, in two waves: the patch
-        // tables (cell and node lists), then what they index (inputs, coordinates, masks).
-        // Indices past the patch are clamped to its first node or cell, so every load is
-        // unconditional and none waits for another of its wave.
+    // A patch's first wave of loads: its tables (cell local ids, entry lists, metric and
+    // coefficients, node slots and ids).  Indices past the patch are clamped to its first node
+    // or cell, so every load is unconditional and none waits for another of its wave.
+    struct Tables {
+        uint4 packed, lent;
+        float2 mq[3];
+        float4 kc[2], gc[2];
+        float vdet;
+        int eb[kRounds], ee[kRounds], os[kRounds], nd[kRounds];
+    };
+    auto load_tables = [&](Tables &a, const int p, const int c0, const int nc, const int n0, const int np) {
         const int ct = tid < nc ? tid : 0;
-        const uint4 packed = ldg(reinterpret_cast<const uint4 *>(d.plidx) + c0 + ct);
-        uint4 lent = make_uint4(0, 0, 0, 0);
-        if constexpr (!FX) lent = ldg(reinterpret_cast<const uint4 *>(d.plent) + c0 + ct);
-        float2 mq[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
-        float4 kc[2] = {}, gc[2] = {};
-        float vdet = 0.f;
+        a.packed = ldg(reinterpret_cast<const uint4 *>(d.plidx) + c0 + ct);
+        a.lent = make_uint4(0, 0, 0, 0);
+        if constexpr (!FX) a.lent = ldg(reinterpret_cast<const uint4 *>(d.plent) + c0 + ct);
+        a.mq[0] = a.mq[1] = a.mq[2] = make_float2(0.f, 0.f);
+        a.kc[0] = a.kc[1] = a.gc[0] = a.gc[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        a.vdet = 0.f;
         if constexpr (sizeof(T) == 4 && MODE == EM_APPLY) {
             if (affine) {
                 const float2 *mp = reinterpret_cast<const float2 *>(d.metric + 6 * (size_t)(c0 + ct));
-                mq[0] = ldg(mp);
-                mq[1] = ldg(mp + 1);
-                mq[2] = ldg(mp + 2);
+                a.mq[0] = ldg(mp);
+                a.mq[1] = ldg(mp + 1);
+                a.mq[2] = ldg(mp + 2);
                 if constexpr (COEF) {
                     const float4 *kp = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(d.kq) + 8 * (size_t)(c0 + ct));
                     const float4 *gp = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(d.gq) + 8 * (size_t)(c0 + ct));
-                    kc[0] = ldg4<float4, float>(kp);
-                    kc[1] = ldg4<float4, float>(kp + 1);
-                    gc[0] = ldg4<float4, float>(gp);
-                    gc[1] = ldg4<float4, float>(gp + 1);
-                    vdet = G(d.metric)[6 * (size_t)d.n_cells + c0 + ct];
+                    a.kc[0] = ldg4<float4, float>(kp);
+                    a.kc[1] = ldg4<float4, float>(kp + 1);
+                    a.gc[0] = ldg4<float4, float>(gp);
+                    a.gc[1] = ldg4<float4, float>(gp + 1);
+                    a.vdet = G(d.metric)[6 * (size_t)d.n_cells + c0 + ct];
                 }
             }
         }
-        int eb[kRounds], ee[kRounds], os[kRounds], nd[kRounds];
 #pragma unroll
         for (int r = 0; r < kRounds; ++r) {
             const int j = tid + r * kBlock;
             const int jj = j < np ? j : 0;
             if constexpr (!FX) {
-                eb[r] = G(d.plptr)[n0 + p + jj];
-                ee[r] = G(d.plptr)[n0 + p + jj + 1];
+                a.eb[r] = G(d.plptr)[n0 + p + jj];
+                a.ee[r] = G(d.plptr)[n0 + p + jj + 1];
             } else {
-                eb[r] = ee[r] = 0;
+                a.eb[r] = a.ee[r] = 0;
             }
-            os[r] = G(d.pslot)[n0 + jj];
-            nd[r] = G(d.pnodes)[n0 + jj];
+            a.os[r] = G(d.pslot)[n0 + jj];
+            a.nd[r] = G(d.pnodes)[n0 + jj];
         }
+    };
+    // tet-split FP32 workgroups that run several patches issue the next patch's tables while
+    // the present one is gathered and applied (its latency is then off the critical path)
+    constexpr bool PREFETCH = AFF && PF;
+    // one patch from its loaded tables; the first one (check) leaves the workgroup early when
+    // the system is idle, tested only once that patch's lists are in flight.  nxt (PREFETCH):
+    // the next patch's tables, loaded once this patch's second wave is issued.  Returns false
+    // to leave.
+    auto patch = [&](Tables &a, const int p, const int c0, const int nc, const int n0, const int np, const bool check,
+                     Tables *nxt, const int np_p, const int np_c0, const int np_nc, const int np_n0, const int np_np) -> bool {
+        const uint4 packed = a.packed;
+        const uint4 lent = a.lent;
+        const float2 *mq = a.mq;
+        const float4 *kc = a.kc, *gc = a.gc;
+        const float vdet = a.vdet;
+        int *eb = a.eb, *ee = a.ee, *os = a.os, *nd = a.nd;
+        (void)p;
         // the node lists had to arrive before the second wave is addressed: store the cell
         // entry lists meanwhile (keeps their load in the first wave)
         if constexpr (!FX) {
@@ -684,6 +731,9 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
                 vc[r] = ccp[cin ? nd[r] >> d.agg_shift : 0];
                 mk[r] = G(d.dmask)[nd[r]];
             }
+        }
+        if constexpr (PREFETCH) {
+            if (nxt) load_tables(*nxt, np_p, np_c0, np_nc, np_n0, np_np);
         }
 #pragma unroll
         for (int r = 0; r < kRounds; ++r) {
@@ -815,13 +865,16 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
             // fixed point: the patch's largest |output| m < 2^e gives the scale 2^(50 - e), so up
             // to 2^13 contributions of a node add without overflow, each exact to m 2^-50 (far
             // below FP32 rounding).  A non-finite output makes the patch's outputs NaN.
+            // fmaxf skips NaN, so a non-finite output is caught through the sum of the eight
             float m = 0.f;
             if (tid < nc) {
+                float sum = 0.f;
 #pragma unroll
                 for (int n = 0; n < 8; ++n) {
-                    const float a = fabsf((float)yc[n]);
-                    m = (a <= m) ? m : (a == a ? a : __builtin_inff());
+                    m = fmaxf(m, fabsf((float)yc[n]));
+                    sum += (float)yc[n];
                 }
+                if (!(fabsf(sum) <= 3.0e38f)) m = __builtin_inff();
             }
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
@@ -831,11 +884,19 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
             const bool finite = m < __builtin_inff();
             int e = 0;
             (void)frexpf(finite ? m : 0.f, &e);
+            // scaled outputs |y| 2^(50 - e) < 2^50: rounded to integers by adding 1.5 * 2^52 in
+            // FP64 (round to nearest even, ulp 1 in that binade); the integer is the sum's bit
+            // pattern minus the constant's
             const double sc = (finite && m > 0.f) ? ldexp(1.0, 50 - e) : 1.0;
             if (tid < nc && finite) {
+                constexpr double kMagic = 6755399441055744.0;                  // 1.5 * 2^52
+                constexpr long long kMagicBits = 0x4338000000000000ll;
 #pragma unroll
-                for (int n = 0; n < 8; ++n)
-                    atomicAdd(&sacc[al[n]], (unsigned long long)__double2ll_rn((double)yc[n] * sc));
+                for (int n = 0; n < 8; ++n) {
+                    // y sc is exact (a power of two), so the fused add rounds once
+                    const double t = __fma_rn((double)yc[n], sc, kMagic);
+                    atomicAdd(&sacc[al[n]], (unsigned long long)(__double_as_longlong(t) - kMagicBits));
+                }
             }
             __syncthreads();
             const double isc = 1.0 / sc;            // a power of two: exact
@@ -861,11 +922,29 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
         __syncthreads();
         return true;
     };
-    if (!patch(t.p, t.c0, t.nc, t.n0, t.np, true)) return;
     const int p_end = min(d.n_patch, t.p + d.ppb);
-    for (int p = t.p + 1; p < p_end; ++p) {
-        const int c0 = G(d.pcell0)[p], n0 = G(d.pnode0)[p];
-        patch(p, c0, G(d.pcell0)[p + 1] - c0, n0, G(d.pnode0)[p + 1] - n0, false);
+    Tables cur;
+    load_tables(cur, t.p, t.c0, t.nc, t.n0, t.np);
+    int p = t.p, c0 = t.c0, nc = t.nc, n0 = t.n0, np = t.np;
+    for (bool first = true;; first = false) {
+        const bool more = p + 1 < p_end;
+        int c1 = 0, nc1 = 0, n1 = 0, np1 = 0;
+        if (more) {
+            c1 = G(d.pcell0)[p + 1];
+            n1 = G(d.pnode0)[p + 1];
+            nc1 = G(d.pcell0)[p + 2] - c1;
+            np1 = G(d.pnode0)[p + 2] - n1;
+        }
+        Tables nxt;
+        if (!patch(cur, p, c0, nc, n0, np, first, (PREFETCH && more) ? &nxt : nullptr, p + 1, c1, nc1, n1, np1)) return;
+        if (!more) break;
+        if constexpr (PREFETCH) cur = nxt;
+        else load_tables(cur, p + 1, c1, nc1, n1, np1);
+        p += 1;
+        c0 = c1;
+        nc = nc1;
+        n0 = n1;
+        np = np1;
     }
 }
 
@@ -1260,17 +1339,19 @@ __global__ void k_grad_export(int n, const int *__restrict__ v2c, const int *__r
 // ---------------------------------------------------------------------------------------
 template <class T>
 void launch_elem(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool coef, bool maskin, bool active_only, int scal_slot,
-                 size_t lds, bool affine_all) {
+                 size_t lds, int form) {
     if (nblk <= 0) return;
     const dim3 g(nblk), b(kBlock);
-    if (sizeof(T) == 4 && affine_all && !maskin) {
-        if (coef) {
-            if (active_only) hipLaunchKernelGGL((k_patch<T, true, false, true, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr);
-            else hipLaunchKernelGGL((k_patch<T, true, false, false, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr);
+    if (sizeof(T) == 4 && form > 0 && !maskin) {
+#define FG_AFF(C, A, P) hipLaunchKernelGGL((k_patch<T, C, false, A, EM_APPLY, true, P>), g, b, lds, s, D, tab, scal_slot, nullptr)
+        if (form == 2) {
+            if (coef) { if (active_only) FG_AFF(true, true, true); else FG_AFF(true, false, true); }
+            else { if (active_only) FG_AFF(false, true, true); else FG_AFF(false, false, true); }
         } else {
-            if (active_only) hipLaunchKernelGGL((k_patch<T, false, false, true, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr);
-            else hipLaunchKernelGGL((k_patch<T, false, false, false, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr);
+            if (coef) { if (active_only) FG_AFF(true, true, false); else FG_AFF(true, false, false); }
+            else { if (active_only) FG_AFF(false, true, false); else FG_AFF(false, false, false); }
         }
+#undef FG_AFF
         return;
     }
 #define FG_ELEM(C, M, A) hipLaunchKernelGGL((k_patch<T, C, M, A, EM_APPLY>), g, b, lds, s, D, tab, scal_slot, nullptr)
@@ -1283,8 +1364,8 @@ void launch_elem(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool c
     }
 #undef FG_ELEM
 }
-template void launch_elem<double>(hipStream_t, const Desc *, const ETab *, int, bool, bool, bool, int, size_t, bool);
-template void launch_elem<float>(hipStream_t, const Desc *, const ETab *, int, bool, bool, bool, int, size_t, bool);
+template void launch_elem<double>(hipStream_t, const Desc *, const ETab *, int, bool, bool, bool, int, size_t, int);
+template void launch_elem<float>(hipStream_t, const Desc *, const ETab *, int, bool, bool, bool, int, size_t, int);
 
 template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool coef, size_t lds) {
     if (nblk <= 0) return;

@@ -444,3 +444,30 @@ def test_face_centroids_match_oracle(ctx, small_mesh, bid):
     ref = om.face_centroids(bid)
     assert c.shape == ref.shape and len(c) > 0
     assert np.abs(c - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("precision", [fgpu.FGPU_FP64, fgpu.FGPU_MIXED])
+def test_patches_per_workgroup_give_the_same_bits(ctx, small_mesh, precision):
+    """Workgroups that run several patches in turn (the FP32 tet-split form then loads the next
+    patch's tables ahead) compute every patch as one-patch workgroups do: the applies and the
+    solves are bit-identical for 1, 2 and 3 patches per workgroup (3: a partial last group)."""
+    gm = fgpu.Mesh(ctx, small_mesh.nodes, small_mesh.vacuum_cells(), *MARKS)
+    gm.set_patch_cells(64)
+    s = fgpu.System(gm)
+    rng = np.random.default_rng(31)
+    u = rng.standard_normal(gm.n_dofs)
+    outs = []
+    for ppb in (1, 2, 3):
+        s.set_patches_per_block(ppb)
+        y32, y64 = s.apply32(u), s.apply(u)
+        s.reset()
+        s.add_dirichlet(2, 0.0)
+        s.add_neumann(8, None, -1.0)
+        s.apply_dirichlet()
+        s.fill_solution(0.0)
+        st = s.solve(fgpu.opts(tol=1e-10, relative=True, precision=precision))
+        assert st.iterations > 0
+        outs.append((y32, y64, st.iterations, s.solution()))
+    for y32, y64, it, x in outs[1:]:
+        assert np.array_equal(y32, outs[0][0]) and np.array_equal(y64, outs[0][1])
+        assert it == outs[0][2] and np.array_equal(x, outs[0][3])
