@@ -264,6 +264,7 @@ def cpu_baseline(client):
     except (OSError, IndexError):
         model = "unknown"
     out["cpu_model"] = model
+    out["host_cpus"] = os.cpu_count()          # the host's CPUs; `cores`: those this process may use
     out["sample"] = (f"one timestep of the smallest client ({client.n_dofs} vacuum dofs, {len(client.cells)} vacuum + "
                      f"{len(client.bulk_cells)} bulk hexes): assemble + SSOR(1.2)-PCG for Laplace, current and heat; "
                      f"median of 5 after a warm-up, {cores} threads (value) and 1 thread")

@@ -999,11 +999,7 @@ struct BatchTabs {
     int ne = 0, nd = 0, nct = 0;   // nct: coarse-product workgroups (systems with a coarse level)
     int max_nc = 0;                // largest coarse size of the batch (LDS of the coarse product)
     bool affine32 = true;          // every system's FP32 descriptor uses the tet-split form
-    bool multi = false;            // some system runs several patches per workgroup
     size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the element kernel, largest patch of the batch
-    // element kernel form of the FP32 apply: 0 general, 1 tet-split, 2 tet-split with the next
-    // patch's tables prefetched (workgroups of several patches)
-    int form32() const { return affine32 ? (multi ? 2 : 1) : 0; }
 };
 
 // coordinates and inputs per node (inputs only for the tet-split FP32 form); FP32: one 64-bit
@@ -1083,7 +1079,6 @@ static BatchTabs upload_batch(fgpu_ctx *c, const std::vector<std::vector<Desc>> 
         // descriptor, which every caller builds before uploading it
         t.lds32 = std::max(t.lds32, patch_lds(m->patches.node_cap, m->patches.cells, sizeof(float), x->desc32_affine));
         t.affine32 = t.affine32 && x->desc32_affine;
-        t.multi = t.multi || x->ppb > 1;
     }
     return t;
 }
@@ -1511,7 +1506,7 @@ extern "C" int fgpu_sys_apply32(fgpu_sys *s, const double *x, double *y) {
         d.ub = nullptr;
         d.q = s->q32;
         BatchTabs t = upload_batch(s->ctx, {{d}}, {s});
-        launch_elem<float>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds32, t.form32());
+        launch_elem<float>(st, t.sets[0], t.etab, t.ne, s->have_coef, false, false, S_BETA, t.lds32, t.affine32);
         launch_gather_plain<float>(st, t.sets[0], t.dtab, t.nd, false);
         std::vector<float> o(m->n_dofs);
         d2h(o.data(), s->q32, o.size(), st);
@@ -1810,7 +1805,7 @@ static void cheb_steps(hipStream_t st, const Desc *D, const Desc *DC, const Batc
 template <class T>
 static void cg_iter(hipStream_t st, const Desc *D, const Desc *DC, const BatchTabs &t, int n, const SolveKind &k,
                     int reliable) {
-    launch_elem<T>(st, D, t.etab, t.ne, k.coef, false, true, S_BETA, sizeof(T) == 8 ? t.lds64 : t.lds32, t.form32());
+    launch_elem<T>(st, D, t.etab, t.ne, k.coef, false, true, S_BETA, sizeof(T) == 8 ? t.lds64 : t.lds32, t.affine32);
     launch_gather_cg<T>(st, D, t.dtab, t.nd);
     launch_scal_alpha(st, D, n);
     launch_update_cg<T>(st, D, t.dtab, t.nd);
@@ -2419,7 +2414,7 @@ static double time_launches(fgpu_ctx *c, const Desc *D, const BatchTabs &t, bool
             launch_coarse(c->stream, D, nullptr, t.ctab, t.nct, 0, t.max_nc);
         } else if (which == 0) {
             if (precision == FGPU_MIXED)
-                launch_elem<float>(c->stream, D, t.etab, t.ne, coef, false, false, S_BETA, t.lds32, t.form32());
+                launch_elem<float>(c->stream, D, t.etab, t.ne, coef, false, false, S_BETA, t.lds32, t.affine32);
             else launch_elem<double>(c->stream, D, t.etab, t.ne, coef, false, false, S_BETA, t.lds64);
         } else {
             if (precision == FGPU_MIXED) launch_gather_plain<float>(c->stream, D, t.dtab, t.nd, false);

@@ -5,7 +5,7 @@
 namespace fgpu {
 template <class T>
 void launch_elem(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool coef, bool maskin, bool active_only,
-                 int scal_slot, size_t lds, int form = 0);   // form: 0 general, 1 tet-split FP32, 2 (1 prefetching)
+                 int scal_slot, size_t lds, bool affine_all = false);
 template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool coef, size_t lds);
 void launch_elem_rhs(hipStream_t s, const Desc *D, const ETab *tab, int nblk, const double *fq, size_t lds);
 void launch_elem_vol(hipStream_t s, const Desc *D, const ETab *tab, int nblk, size_t lds);

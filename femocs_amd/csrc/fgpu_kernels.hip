@@ -608,10 +608,8 @@ void launch_metric(hipStream_t s, int nc, const int *cells, const unsigned char 
 }
 
 // AFF: every system of the launch has the tet-split FP32 form (the general path is not
-// compiled in, which frees registers for more resident waves).  PF (with AFF): workgroups of
-// several patches load the next patch's tables while the present one runs (more registers,
-// fewer resident waves: only for launches whose systems run several patches per workgroup)
-template <class T, bool COEF, bool MASKIN, bool ACTIVE_ONLY, int MODE, bool AFF = false, bool PF = false>
+// compiled in, which frees registers for more resident waves)
+template <class T, bool COEF, bool MASKIN, bool ACTIVE_ONLY, int MODE, bool AFF = false>
 __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const Desc *__restrict__ D, const ETab *__restrict__ tab, int scal_slot,
                                                      const double *__restrict__ fq) {
     constexpr int kRounds = (kPatchNodes + kBlock - 1) / kBlock;
@@ -693,15 +691,11 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
             a.nd[r] = G(d.pnodes)[n0 + jj];
         }
     };
-    // tet-split FP32 workgroups that run several patches issue the next patch's tables while
-    // the present one is gathered and applied (its latency is then off the critical path)
-    constexpr bool PREFETCH = AFF && PF;
     // one patch from its loaded tables; the first one (check) leaves the workgroup early when
-    // the system is idle, tested only once that patch's lists are in flight.  nxt (PREFETCH):
-    // the next patch's tables, loaded once this patch's second wave is issued.  Returns false
-    // to leave.
-    auto patch = [&](Tables &a, const int p, const int c0, const int nc, const int n0, const int np, const bool check,
-                     Tables *nxt, const int np_p, const int np_c0, const int np_nc, const int np_n0, const int np_np) -> bool {
+    // the system is idle, tested only once that patch's lists are in flight.  Returns false to
+    // leave.  (Loading the next patch's tables ahead, in workgroups of several patches, was
+    // measured: the extra registers cost more resident waves than the hidden latency gained.)
+    auto patch = [&](Tables &a, const int p, const int c0, const int nc, const int n0, const int np, const bool check) -> bool {
         const uint4 packed = a.packed;
         const uint4 lent = a.lent;
         const float2 *mq = a.mq;
@@ -732,9 +726,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
                 mk[r] = G(d.dmask)[nd[r]];
             }
         }
-        if constexpr (PREFETCH) {
-            if (nxt) load_tables(*nxt, np_p, np_c0, np_nc, np_n0, np_np);
-        }
+
 #pragma unroll
         for (int r = 0; r < kRounds; ++r) {
             const int j = tid + r * kBlock;
@@ -922,29 +914,15 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
         __syncthreads();
         return true;
     };
+    Tables a;
+    load_tables(a, t.p, t.c0, t.nc, t.n0, t.np);
+    if (!patch(a, t.p, t.c0, t.nc, t.n0, t.np, true)) return;
     const int p_end = min(d.n_patch, t.p + d.ppb);
-    Tables cur;
-    load_tables(cur, t.p, t.c0, t.nc, t.n0, t.np);
-    int p = t.p, c0 = t.c0, nc = t.nc, n0 = t.n0, np = t.np;
-    for (bool first = true;; first = false) {
-        const bool more = p + 1 < p_end;
-        int c1 = 0, nc1 = 0, n1 = 0, np1 = 0;
-        if (more) {
-            c1 = G(d.pcell0)[p + 1];
-            n1 = G(d.pnode0)[p + 1];
-            nc1 = G(d.pcell0)[p + 2] - c1;
-            np1 = G(d.pnode0)[p + 2] - n1;
-        }
-        Tables nxt;
-        if (!patch(cur, p, c0, nc, n0, np, first, (PREFETCH && more) ? &nxt : nullptr, p + 1, c1, nc1, n1, np1)) return;
-        if (!more) break;
-        if constexpr (PREFETCH) cur = nxt;
-        else load_tables(cur, p + 1, c1, nc1, n1, np1);
-        p += 1;
-        c0 = c1;
-        nc = nc1;
-        n0 = n1;
-        np = np1;
+    for (int p = t.p + 1; p < p_end; ++p) {
+        const int c0 = G(d.pcell0)[p], n0 = G(d.pnode0)[p];
+        const int nc = G(d.pcell0)[p + 1] - c0, np = G(d.pnode0)[p + 1] - n0;
+        load_tables(a, p, c0, nc, n0, np);
+        patch(a, p, c0, nc, n0, np, false);
     }
 }
 
@@ -1339,18 +1317,13 @@ __global__ void k_grad_export(int n, const int *__restrict__ v2c, const int *__r
 // ---------------------------------------------------------------------------------------
 template <class T>
 void launch_elem(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool coef, bool maskin, bool active_only, int scal_slot,
-                 size_t lds, int form) {
+                 size_t lds, bool affine_all) {
     if (nblk <= 0) return;
     const dim3 g(nblk), b(kBlock);
-    if (sizeof(T) == 4 && form > 0 && !maskin) {
-#define FG_AFF(C, A, P) hipLaunchKernelGGL((k_patch<T, C, false, A, EM_APPLY, true, P>), g, b, lds, s, D, tab, scal_slot, nullptr)
-        if (form == 2) {
-            if (coef) { if (active_only) FG_AFF(true, true, true); else FG_AFF(true, false, true); }
-            else { if (active_only) FG_AFF(false, true, true); else FG_AFF(false, false, true); }
-        } else {
-            if (coef) { if (active_only) FG_AFF(true, true, false); else FG_AFF(true, false, false); }
-            else { if (active_only) FG_AFF(false, true, false); else FG_AFF(false, false, false); }
-        }
+    if (sizeof(T) == 4 && affine_all && !maskin) {
+#define FG_AFF(C, A) hipLaunchKernelGGL((k_patch<T, C, false, A, EM_APPLY, true>), g, b, lds, s, D, tab, scal_slot, nullptr)
+        if (coef) { if (active_only) FG_AFF(true, true); else FG_AFF(true, false); }
+        else { if (active_only) FG_AFF(false, true); else FG_AFF(false, false); }
 #undef FG_AFF
         return;
     }
@@ -1364,8 +1337,8 @@ void launch_elem(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool c
     }
 #undef FG_ELEM
 }
-template void launch_elem<double>(hipStream_t, const Desc *, const ETab *, int, bool, bool, bool, int, size_t, int);
-template void launch_elem<float>(hipStream_t, const Desc *, const ETab *, int, bool, bool, bool, int, size_t, int);
+template void launch_elem<double>(hipStream_t, const Desc *, const ETab *, int, bool, bool, bool, int, size_t, bool);
+template void launch_elem<float>(hipStream_t, const Desc *, const ETab *, int, bool, bool, bool, int, size_t, bool);
 
 template <class T> void launch_elem_diag(hipStream_t s, const Desc *D, const ETab *tab, int nblk, bool coef, size_t lds) {
     if (nblk <= 0) return;

@@ -365,3 +365,53 @@ def test_particle_cell_update_bit_exact(ctx, tip):
     assert (got == -1).any() and (got >= 0).mean() > 0.8
     with pytest.raises(fgpu.FgpuError):
         ip.update_cells(pts[:1], np.array([len(d2f)], np.int32))
+
+
+@pytest.fixture(scope="module")
+def r3nm():
+    """Config 1's own tip (r = 3 nm, h = 15 nm on a 30 x 30 x 3 nm slab)."""
+    tip = M.make_tip_mesh(M.tip_r3nm_geometry())
+    print(f"r3nm tip: {len(tip.vacuum_cells())} vacuum hexes, {len(tip.tets)} tets", flush=True)
+    return tip
+
+
+def test_config1_tip_solution_and_transfer_match_oracle(ctx, r3nm):
+    """On config 1's tip: the GPU Laplace solution within 1e-10 of the oracle's, its nodal
+    solution bit-exact with the oracle's extraction of the same potential, and located cells and
+    values bit-exact for ranks 1-3 (the linear interpolation of E to the surface atoms is
+    config 1's transfer)."""
+    tip = r3nm
+    gm = fgpu.Mesh(ctx, tip.nodes, tip.vacuum_cells(), *MARKS)
+    s = fgpu.System(gm)
+    s.reset()
+    s.add_dirichlet(2, 0.0)
+    s.add_neumann(8, None, -1.0)
+    s.apply_dirichlet()
+    s.fill_solution(0.0)
+    assert s.solve(fgpu.opts(tol=1e-13, relative=True)).iterations > 0
+    _, xo, it, _, _ = O.poisson_laplace(tip.nodes, tip.vacuum_cells(), -1.0, tol=1e-13, relative=True)
+    assert it > 0
+    x = s.solution()
+    assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
+    xyz, vod = gm.dofs()
+    phi = np.zeros(len(tip.nodes))
+    phi[vod] = x
+    used = np.zeros(len(tip.nodes), np.uint8)
+    used[vod] = 1
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+    ip.extract_solution(s, smoothen=False)
+    sol = ip.nodal()
+    assert np.array_equal(sol, O.extract_nodal(tip.nodes, tip.hexes, tip.hex_marker, 1, used, phi))
+    L = O.LinTet(tip.nodes, tip.tets, tip.tet_nbr, tip.tet_marker, 3)
+    q10 = O.quadtet_cells(tip.tets, tip.hexes, tip.node_kind)
+    for rank in (1, 2, 3):
+        pts = _points(tip, 300, 40 + rank)
+        cells_o, out_o = O.interp_points(L, tip.nodes, tip.tets, q10, tip.hexes, sol, pts, rank)
+        cells_g, out_g = ip.points(pts, rank)
+        assert np.array_equal(cells_g, cells_o)
+        assert np.array_equal(out_g, out_o)
+    # the surface atoms (vertices of the tets): rank 1, as config 1 transfers E
+    atoms = tip.nodes[: tip.n_surface_nodes]
+    cells_o, out_o = O.interp_points(L, tip.nodes, tip.tets, q10, tip.hexes, sol, atoms[:2000], 1)
+    cells_g, out_g = ip.points(atoms[:2000], 1)
+    assert np.array_equal(cells_g, cells_o) and np.array_equal(out_g, out_o)
