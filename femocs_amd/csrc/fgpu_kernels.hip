@@ -4,8 +4,8 @@
 // operator is applied matrix-free in two deterministic steps per product:
 //   k_patch   per patch of consecutive cells: node data staged in LDS, one thread per cell
 //             (sum-factorised reference gradients, per-point metric, transposed sum
-//             factorisation), per-node sums in LDS in a fixed order, one store per
-//             (node, patch) into the dof-major slot buffer
+//             factorisation), per-node sums in LDS (FP64: in a fixed order; FP32: exact
+//             fixed-point sums), one store per (node, patch) into the dof-major slot buffer
 //   k_gather  one thread per dof: add the dof's slots in ascending order, fused with the
 //             CG vector work of that step and a block reduction into per-block partials,
 //             which a one-block-per-system scalar kernel sums in block order
@@ -17,6 +17,10 @@
 #include <cmath>
 
 #include "fgpu_kernels.h"
+
+// a * b + c fuses only within one expression, as written: two kernels that evaluate the same
+// element code round identically, whatever the compiler's scheduling around it
+#pragma clang fp contract(on)
 
 namespace fgpu {
 
