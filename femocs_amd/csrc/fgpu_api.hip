@@ -2144,6 +2144,11 @@ struct fgpu_batch {
     std::vector<fgpu_sys *> joined;   // added while a step was in flight (not its members, even
                                       // at the address of one that left)
     SolveKind kind;                   // preconditioner pieces of the uploaded sets
+    // the batch's own stream: its steps run beside other batches' (their kernels fill each
+    // other's idle CUs: scalar kernels, launch tails); set-up work stays on the context's
+    // stream, and every step first waits for what was enqueued there (ev_ctx)
+    hipStream_t st = nullptr;
+    hipEvent_t ev_ctx = nullptr;
 };
 
 static void batch_drop_graph(fgpu_batch *b) {
@@ -2162,6 +2167,17 @@ extern "C" int fgpu_batch_create(fgpu_ctx *c, const fgpu_solve_opts *o, fgpu_bat
     }
     fgpu_batch *b = new (std::nothrow) fgpu_batch();
     if (!b) return FGPU_ENOMEM;
+    const int rc = guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        HIPCHECK(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));      // default priority
+        HIPCHECK(hipEventCreateWithFlags(&b->ev_ctx, hipEventDisableTiming));
+        return FGPU_OK;
+    });
+    if (rc != FGPU_OK) {
+        if (b->st) (void)hipStreamDestroy(b->st);
+        delete b;
+        return rc;
+    }
     b->ctx = c;
     b->o = *o;
     c->refs++;
@@ -2174,8 +2190,11 @@ extern "C" void fgpu_batch_destroy(fgpu_batch *b) {
     fgpu_ctx *c = b->ctx;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (b->st) (void)hipStreamSynchronize(b->st);
     batch_drop_graph(b);
     for (auto *s : b->sys) s->batches--;
+    if (b->st) (void)hipStreamDestroy(b->st);
+    if (b->ev_ctx) (void)hipEventDestroy(b->ev_ctx);
     if (b->dtab) (void)hipFree(b->dtab);
     if (b->dflags) (void)hipFree(b->dflags);
     if (b->h_pend) (void)hipHostFree(b->h_pend);
@@ -2235,20 +2254,23 @@ static void batch_launch(fgpu_batch *b, int iterations) {
             b->dirty = false;
         }
         ChunkArgs a{b->kind, k, 8, n, b->t.sets[0], b->t.sets[1], b->t.sets[2], b->t.sets[3], &b->t};
+        // the members' set-up and the tables were enqueued on the context's stream
+        HIPCHECK(hipEventRecord(b->ev_ctx, c->stream));
+        HIPCHECK(hipStreamWaitEvent(b->st, b->ev_ctx, 0));
         if (b->o.use_graph) {
             if (!b->exec || b->graph_k != k) {
                 batch_drop_graph(b);
-                HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                chunk(c->stream, a);
-                HIPCHECK(hipStreamEndCapture(c->stream, &b->graph));
+                HIPCHECK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+                chunk(b->st, a);
+                HIPCHECK(hipStreamEndCapture(b->st, &b->graph));
                 HIPCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
                 b->graph_k = k;
             }
-            HIPCHECK(hipGraphLaunch(b->exec, c->stream));
+            HIPCHECK(hipGraphLaunch(b->exec, b->st));
         } else {
-            chunk(c->stream, a);
+            chunk(b->st, a);
         }
-        launch_collect(c->stream, b->t.sets[1], n, S_DONE, b->dflags);
+        launch_collect(b->st, b->t.sets[1], n, S_DONE, b->dflags);
     }
 }
 
@@ -2265,8 +2287,8 @@ extern "C" int fgpu_batch_step(fgpu_batch *b, int iterations, int *n_running) {
         if (n == 0) return FGPU_OK;
         batch_launch(b, iterations);
         double *h = flags_buffer(c, n);
-        HIPCHECK(hipMemcpyAsync(h, b->dflags, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipStreamSynchronize(c->stream));
+        HIPCHECK(hipMemcpyAsync(h, b->dflags, n * sizeof(double), hipMemcpyDeviceToHost, b->st));
+        HIPCHECK(hipStreamSynchronize(b->st));
         int running = 0;
         for (int i = 0; i < n; ++i) running += h[i] == 0.0;
         if (n_running) *n_running = running;
@@ -2307,8 +2329,8 @@ extern "C" int fgpu_batch_step_async(fgpu_batch *b, int iterations) {
         }
         if (!b->pend_ev) HIPCHECK(hipEventCreateWithFlags(&b->pend_ev, hipEventDisableTiming));
         batch_launch(b, iterations);
-        HIPCHECK(hipMemcpyAsync(b->h_pend, b->dflags, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipEventRecord(b->pend_ev, c->stream));
+        HIPCHECK(hipMemcpyAsync(b->h_pend, b->dflags, n * sizeof(double), hipMemcpyDeviceToHost, b->st));
+        HIPCHECK(hipEventRecord(b->pend_ev, b->st));
         b->pend_sys = b->sys;
         b->joined.clear();
         b->pending = true;
