@@ -149,8 +149,7 @@ def set_mode(srv, precision, batched, warm, coarse_in_cg, plain_precond=None):
     srv.heat_opts = o
     srv.batched = batched
     srv.warm = warm
-    srv.batch = fgpu.Batch(srv.ctx, o) if batched else None
-    srv.hbatch = fgpu.Batch(srv.ctx, o) if batched else None
+    srv.make_batches()
     if not coarse_in_cg and pc == fgpu.FGPU_PC_JACOBI:
         raise ValueError("the two-level correction is on whenever the systems have one")
 

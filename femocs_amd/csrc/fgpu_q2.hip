@@ -49,8 +49,8 @@ template <class T> void q2free(T *&p) {
     p = nullptr;
 }
 
-constexpr int kQ2Cells = 28;                 // cells per workgroup, 9 threads each
-constexpr int kQ2Threads = 9 * kQ2Cells;     // 252 of the 256 threads
+constexpr int kQ2CellsPerWave = 7;           // 9 threads each: 63 of a wave's 64 lanes
+constexpr int kQ2Cells = 4 * kQ2CellsPerWave; // cells per workgroup of 4 waves
 
 // 1D Q2 basis on the nodes 0, 1/2, 1 at the QGauss(3) points of [0, 1]: B[q][n], derivative
 // D[q][n], weights W[q]
@@ -227,6 +227,13 @@ struct Q2Args {
     const unsigned char *layout; // tet-split form: 1 for an eta-flipped cell
 };
 
+// orders a wave's LDS writes before its lanes' later reads (no other wave takes part)
+__device__ __forceinline__ void q2_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // y = K_c u for every cell, K_c(i, j) = sum_q JxW grad phi_i . grad phi_j (QGauss(3)).
 // AFF: every cell has the tet-split form (per-cell metric, no coordinates); a flipped cell is
 // computed with its node grid's middle index reversed (thread column tb reads and writes row
@@ -238,8 +245,11 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
     __shared__ double sB[kQ2Cells][81];
     if (a.active_only && a.scal[S_ACTIVE] == 0.0) return;
     const int tid = threadIdx.x;
-    const int lc = tid / 9, t = tid % 9, ta = t % 3, tb = t / 3;
-    const bool inwg = tid < kQ2Threads;
+    // a cell's 9 threads lie in one wave, so the exchanges between them need no workgroup
+    // barrier: a wave's own LDS writes are visible to its lanes after a wave-level fence
+    const int lane = tid & 63, wv = tid >> 6;
+    const int lc = wv * kQ2CellsPerWave + lane / 9, t = lane % 9, ta = t % 3, tb = t / 3;
+    const bool inwg = lane < 9 * kQ2CellsPerWave;
     const int c = blockIdx.x * kQ2Cells + lc;
     const bool live = inwg && c < a.nc;
     const int cc = live ? c : 0;
@@ -282,7 +292,7 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
             sA[lc][27 + ta + 3 * tb + 9 * q] = sd;
         }
     }
-    __syncthreads();
+    q2_wave_sync();
     // 2: along y (thread (i, qk)): T2 (B), T2y (D) from T1; T2z (B) from T1d
     if (inwg) {
         const int i = ta, qk = tb;
@@ -306,7 +316,7 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
             sB[lc][54 + i + 3 * q + 9 * qk] = sz;
         }
     }
-    __syncthreads();
+    q2_wave_sync();
     // 3: along x (thread (qj, qk)): reference gradient at (qi, qj, qk), metric, and back along x
     if (inwg) {
         const int qj = ta, qk = tb;
@@ -367,7 +377,7 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
             sA[lc][54 + i + 3 * qj + 9 * qk] = sz;
         }
     }
-    __syncthreads();
+    q2_wave_sync();
     // 4: back along y (thread (i, qk))
     if (inwg) {
         const int i = ta, qk = tb;
@@ -390,7 +400,7 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
             sB[lc][27 + i + 3 * j + 9 * qk] = sz;
         }
     }
-    __syncthreads();
+    q2_wave_sync();
     // 5: back along z (thread (i, j)), one store per node into its slot
     if (live) {
         const int i = ta, j = tb;

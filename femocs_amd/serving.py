@@ -212,7 +212,7 @@ class Server:
                  budget_bytes: int = 64 << 30, batched: bool = True, warm: bool = True,
                  step_iterations: int = 32, max_batch: int = 64, record_inputs: bool = False, configure=None,
                  coarse: bool = False, pipelined: bool = False, heat_opts: fgpu.SolveOpts | None = None,
-                 snapshot=None):
+                 snapshot=None, bulk_iterations: int | None = None):
         self.ctx = ctx
         self.clients = clients
         self.opts = opts
@@ -221,6 +221,9 @@ class Server:
         self.batched = batched
         self.warm = warm
         self.step_iterations = step_iterations
+        # CG iterations per step of the bulk-mesh batches (current, heat): their systems are
+        # smaller, so their steps are shorter than the Laplace batch's they run beside
+        self.bulk_iterations = bulk_iterations if bulk_iterations is not None else step_iterations
         self.max_batch = max_batch
         self.record_inputs = record_inputs
         self.configure = configure                    # called as configure(mesh, system) on import
@@ -238,10 +241,7 @@ class Server:
         self.busy: dict[int, Request] = {}                            # client -> in-flight request
         self.by_sys: dict[int, Request] = {}
         self.stats = ServeStats()
-        # Laplace and current solves share one batch; heat systems carry per-cell coefficients
-        # (kappa(T), the mass term) and run in a second one
-        self.batch = fgpu.Batch(ctx, opts) if batched else None
-        self.hbatch = fgpu.Batch(ctx, self.heat_opts) if batched and self.heat else None
+        self.make_batches()
         # pipelined steps (Laplace only): the next step is queued before the host retires
         # (requests join one step later).  Off by default: admission synchronises with the
         # stream (the validity check of a displacement, the metric of the moved mesh), so the
@@ -250,6 +250,14 @@ class Server:
             raise ValueError("pipelined steps serve Laplace-only requests")
         self.pipelined = pipelined
         self._inflight = False
+
+    def make_batches(self):
+        """One continuous batch per stage: Laplace (vacuum meshes), current and heat (bulk
+        meshes; heat systems carry per-cell coefficients, kappa(T) and the mass term).  Each
+        batch has its own stream, so the three advance side by side on the GPU."""
+        self.batch = fgpu.Batch(self.ctx, self.opts) if self.batched else None
+        self.cbatch = fgpu.Batch(self.ctx, self.opts) if self.batched and self.heat else None
+        self.hbatch = fgpu.Batch(self.ctx, self.heat_opts) if self.batched and self.heat else None
 
     # ---- memory-budget admission
     def used_bytes(self) -> int:
@@ -468,12 +476,15 @@ class Server:
             return [req]
         self._admit(clock())
         if not self.pipelined:
-            active = [b for b in (self.batch, self.hbatch) if b is not None and len(b) > 0]
+            active = []
+            for b in (self.batch, self.cbatch, self.hbatch):
+                if b is not None and len(b) > 0 and all(b is not a for a in active):
+                    active.append(b)
             if not active:
                 return done
             t0 = time.perf_counter()
             for b in active:
-                b.step_async(self.step_iterations)
+                b.step_async(self.step_iterations if b is self.batch else self.bulk_iterations)
             for b in active:
                 b.wait()
             now = clock()
@@ -499,7 +510,7 @@ class Server:
             if self.heat and req.stage < 2:
                 nxt = self._next_stage(req)
                 self.by_sys[nxt.h.value] = req
-                (self.hbatch if req.stage == 2 else self.batch).add(nxt)
+                (self.hbatch if req.stage == 2 else self.cbatch).add(nxt)
                 continue
             self._finish(req, now)
             done.append(req)
