@@ -720,7 +720,7 @@ extern "C" int fgpu_sys_device_bytes(const fgpu_sys *s, size_t *bytes) {
     const fgpu_mesh *m = static_cast<const fgpu_mesh *>(s->mesh);
     const size_t nd = m->n_dofs, ns = m->n_slots, nb = m->bf_cell.size(), nc = m->n_cells;
     *bytes = nd * (11 * 8 + 1 + 6 * 4) + ns * (8 + 4) + nb * 5 * 8 + 2 * kPartRegions * (size_t)s->nb_dof * 8 +
-             2 * kScal * 8 + (s->kq ? nc * 8 * (8 + 8 + 4 + 4) : 0) + (size_t)s->nc * (s->nc + 1) * 4;
+             2 * kScal * 8 + (s->kq ? nc * 8 * (8 + 8 + 4 + 4) : 0) + (size_t)s->nc * s->nc * 2 + (size_t)s->nc * 12;
     return FGPU_OK;
 }
 
@@ -863,6 +863,7 @@ static Desc make_desc32(fgpu_sys *s, bool coarse = false) {
     if (coarse && s->coarse) {
         d.cc = s->cc;
         d.cinv = s->cinv;
+        d.cscale = s->cscale;
         d.nc = s->nc;
         d.agg_shift = s->agg_shift;
         d.cc_in = 1;
@@ -1658,24 +1659,34 @@ static int build_coarse(fgpu_sys *s, int shift) {
     std::vector<double> dg(nc);
     HIPCHECK(hipMemcpy2DAsync(dg.data(), sizeof(double), dA, (nc + 1) * sizeof(double), sizeof(double), nc,
                               hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    double dmax = 0.0;
+    for (double v : dg) {
+        if (!(v > 0.0) || !std::isfinite(v)) {
+            dfree(dA);
+            dfree(dB);
+            throw std::runtime_error("coarse operator inversion failed");
+        }
+        dmax = std::max(dmax, v);
+    }
+    // FP16 storage: every entry of an SPD matrix is at most its largest diagonal entry, which
+    // the power-of-two scale brings into [2^13, 2^14) (FP16 holds up to 65504)
+    int ex = 0;
+    (void)std::frexp(dmax, &ex);
+    const double cscale = std::ldexp(1.0, ex - 14);
     dfree(s->cinv);
     dfree(s->cc);
-    s->cinv = dalloc<float>((size_t)nc * nc);
+    s->cinv = dalloc<_Float16>((size_t)nc * nc);
+    s->cscale = cscale;
     s->cc = dalloc<float>(nc);
     dfree(s->cc64);
     s->cc64 = dalloc<double>(nc);
-    launch_to_float(st, (size_t)nc * nc, dA, s->cinv);
+    launch_to_half(st, (size_t)nc * nc, dA, cscale, s->cinv);
     HIPCHECK(hipMemsetAsync(s->cc, 0, nc * sizeof(float), st));
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
     dfree(dA);
     dfree(dB);
-    for (double v : dg)
-        if (!(v > 0.0) || !std::isfinite(v)) {
-            dfree(s->cinv);
-            dfree(s->cc);
-            throw std::runtime_error("coarse operator inversion failed");
-        }
     s->nc = nc;
     s->agg_shift = shift;
     s->coarse = true;
@@ -1720,8 +1731,10 @@ extern "C" int fgpu_sys_get_coarse(fgpu_sys *s, int *nc, int *block, float *einv
             for (int d = 0; d < m->n_dofs; ++d) block[d] = m->idof_of_dof[d] >> s->agg_shift;
         if (einv && s->coarse) {
             HIPCHECK(hipSetDevice(s->ctx->device));
-            d2h(einv, s->cinv, (size_t)s->nc * s->nc, s->ctx->stream);
+            std::vector<_Float16> h((size_t)s->nc * s->nc);
+            d2h(h.data(), s->cinv, h.size(), s->ctx->stream);
             HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+            for (size_t i = 0; i < h.size(); ++i) einv[i] = (float)((double)(float)h[i] * s->cscale);
         }
         return FGPU_OK;
     });
@@ -1907,6 +1920,7 @@ static SolveKind make_sets(const std::vector<fgpu_sys *> &sys, std::vector<std::
                 for (Desc *d : {&d0, &d1}) {
                     d->cc = s->cc64;
                     d->cinv = s->cinv;
+                    d->cscale = s->cscale;
                     d->nc = s->nc;
                     d->agg_shift = s->agg_shift;
                 }

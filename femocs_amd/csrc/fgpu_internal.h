@@ -101,7 +101,8 @@ struct Desc {
     // two-level preconditioner of the FP32 CG (null: Jacobi only): coarse correction c per
     // dof block (T), nc x nc inverse of the coarse operator (row-major), nc = dof blocks
     void *cc;
-    const float *cinv;
+    const _Float16 *cinv;         // FP16, scaled: the inverse is cinv x cscale
+    double cscale;                // a power of two
     int nc, agg_shift;            // aggregates and log2 of their size
     int cc_in;                    // the element input is z: add its coarse part (CG sets only;
                                   // a residual set's input is the iterate)
@@ -227,7 +228,10 @@ struct Sys {
     int batches = 0;              // running batches holding the system (their tables point here)
     int nc = 0;                   // coarse size: aggregates of 2^agg_shift internal dofs
     int agg_shift = 10;
-    float *cinv = nullptr;        // nc x nc
+    // nc x nc, FP16 scaled by cscale (a power of two: the largest entry, on the diagonal of
+    // the SPD inverse, stays below 2^14): the coarse product reads half the bytes of FP32
+    _Float16 *cinv = nullptr;
+    double cscale = 1.0;
     float *cc = nullptr;          // nc
     bool desc32_affine = false;   // the last FP32 descriptor built uses the tet-split form
     // replayable heat terms of other systems that read this one's solution: while any is

@@ -1425,9 +1425,10 @@ __global__ void k_gj_step(int n, int k, const double *__restrict__ A, double *__
     B[e] = v;
 }
 
-__global__ void k_to_float(size_t n, const double *__restrict__ a, float *__restrict__ out) {
+// out = a / scale in FP16 (scale a power of two: the division is exact before the rounding)
+__global__ void k_to_half(size_t n, const double *__restrict__ a, double inv_scale, _Float16 *__restrict__ out) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (float)a[i];
+    if (i < n) out[i] = (_Float16)(a[i] * inv_scale);
 }
 
 void launch_coarse_indicator(hipStream_t s, int n, const unsigned char *mask, const int *color, int shift, int c,
@@ -1442,8 +1443,8 @@ void launch_gj_step(hipStream_t s, int n, int k, const double *A, double *B) {
     const size_t e = (size_t)n * n;
     if (e) hipLaunchKernelGGL(k_gj_step, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, n, k, A, B);
 }
-void launch_to_float(hipStream_t s, size_t n, const double *a, float *out) {
-    if (n) hipLaunchKernelGGL(k_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, out);
+void launch_to_half(hipStream_t s, size_t n, const double *a, double scale, _Float16 *out) {
+    if (n) hipLaunchKernelGGL(k_to_half, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, 1.0 / scale, out);
 }
 
 void launch_invdiag(hipStream_t s, int n, const unsigned char *mask, const double *diag, double *inv64, float *inv32) {

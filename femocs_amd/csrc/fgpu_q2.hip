@@ -262,6 +262,10 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) M[k] = a.qm[6 * (size_t)cc + k];
     }
+    // the column's output slots, loaded with its inputs (they are stored at the very end)
+    int slot[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) slot[k] = a.spos[27 * (size_t)cc + ta + 3 * tr + 9 * k];
     // column (i = ta, j = tb) of the nodal values; vertex coordinates of the cell
     double u[3];
 #pragma unroll
@@ -410,13 +414,12 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
             s2[q] = sB[lc][i + 3 * j + 9 * q];
             s2z[q] = sB[lc][27 + i + 3 * j + 9 * q];
         }
-        const int *sp = a.spos + 27 * (size_t)c;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             double y = 0;
 #pragma unroll
             for (int q = 0; q < 3; ++q) y += kQB[q][k] * s2[q] + kQD[q][k] * s2z[q];
-            a.Ye[sp[i + 3 * tr + 9 * k]] = y;
+            a.Ye[slot[k]] = y;
         }
     }
 }

@@ -345,7 +345,7 @@ class Server:
         if not agg:
             return 0
         nc = (n_dofs + agg - 1) // agg
-        return nc * (nc + 1) * 4 + nc * 8
+        return nc * nc * 2 + nc * 12             # FP16 inverse, FP32 and FP64 corrections
 
     def _assemble(self, c: Client, r: _Resident, req: Request):
         t0 = time.perf_counter()

@@ -32,7 +32,8 @@ def meshes(small_mesh):
 
 @pytest.mark.parametrize("agg", [256, 1024])
 def test_coarse_operator_matches_assembled(ctx, meshes, agg):
-    """E^-1 from the device equals the inverse of Z^T A Z of the oracle's CSR matrix."""
+    """E^-1 from the device equals the inverse of Z^T A Z of the oracle's CSR matrix, to its FP16
+    storage."""
     import scipy.sparse as sp
     m = meshes[2]
     s = _system(ctx, m)
@@ -57,7 +58,13 @@ def test_coarse_operator_matches_assembled(ctx, meshes, agg):
     E[:, empty] = 0
     E[empty, empty] = 1
     ref = np.linalg.inv(E)
-    assert np.abs(einv - ref).max() <= 1e-6 * np.abs(ref).max()
+    # stored in FP16 with a power-of-two scale that puts the largest (diagonal) entry in
+    # [2^13, 2^14): every entry is the FP16 rounding of the scaled inverse
+    scale = 2.0 ** (np.frexp(np.diag(ref).max())[1] - 14)
+    q = einv.astype(np.float64) / scale
+    assert np.array_equal(q, q.astype(np.float16).astype(np.float64))
+    assert 2 ** 13 <= np.abs(q).max() < 2 ** 14
+    assert np.all(np.abs(einv - ref) <= 2.0 ** -11 * np.abs(ref) + 1e-12 * np.abs(ref).max())
     assert s.set_coarse(False) == 0 and s.coarse() is None
 
 
