@@ -16,6 +16,7 @@ namespace fgpu {
 constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
 constexpr int kDofIter = 4;   // dofs per thread in the per-dof kernels
 constexpr int kDofBlock = kBlock * kDofIter;
+constexpr int kSlotGroup = 64;  // dofs per group of interleaved slots (one wave's dofs)
 // coarse aggregates of the two-level preconditioner: 2^agg_shift consecutive internal dofs, a
 // multiple of kAggMin (one round of a dof workgroup: its sums of r are kept per round)
 constexpr int kAggMin = kBlock;
@@ -81,6 +82,10 @@ struct Desc {
     const int *cells;             // 8 per cell, lexicographic, internal dof ids
     const void *X;                // T4 coordinates per dof
     const int *gptr;              // dof i owns slot-buffer positions [gptr[i], gptr[i+1])
+    // or, with slot counts (interleaved slots): dof i's k-th slot at gptr[i] + k * kSlotGroup,
+    // k < scnt[i] (each group of kSlotGroup consecutive dofs stores its k-th slots side by
+    // side, so a wave reads them coalesced); null for contiguous slots
+    const int *scnt;
     int n_patch;                  // patches (see Patches)
     int pcells, node_cap, ppb;    // patch size, LDS node capacity, patches per workgroup
     const int *pcell0, *pnode0, *pnodes, *pslot;

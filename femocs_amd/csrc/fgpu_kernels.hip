@@ -307,22 +307,24 @@ template <class T> __device__ __forceinline__ T coarse_term(const Desc &d, int i
 template <class T> __device__ __forceinline__ void gather_slots4(const Desc &d, int base, T (&q)[kDofIter]) {
     const auto Ye = G(reinterpret_cast<const T *>(d.Ye));
     const auto gp = G(d.gptr);
-    int b[kDofIter], e[kDofIter];
+    // contiguous slots (stride 1) or interleaved ones (stride kSlotGroup, count scnt)
+    const int stride = d.scnt ? kSlotGroup : 1;
+    int b[kDofIter], n[kDofIter];
 #pragma unroll
     for (int m = 0; m < kDofIter; ++m) {
         const int i = base + threadIdx.x + m * kBlock;
         const int ii = i < d.n_dofs ? i : d.n_dofs - 1;
         b[m] = gp[ii];
-        e[m] = gp[ii + 1];
+        n[m] = d.scnt ? G(d.scnt)[ii] : gp[ii + 1] - b[m];
     }
     T v[kDofIter][4];
 #pragma unroll
     for (int m = 0; m < kDofIter; ++m)
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int idx = b[m] + k < e[m] ? b[m] + k : b[m];
+            const int idx = k < n[m] ? b[m] + k * stride : b[m];
             const T y = Ye[idx];
-            v[m][k] = b[m] + k < e[m] ? y : (T)0;
+            v[m][k] = k < n[m] ? y : (T)0;
         }
 #pragma unroll
     for (int m = 0; m < kDofIter; ++m) {
@@ -331,15 +333,16 @@ template <class T> __device__ __forceinline__ void gather_slots4(const Desc &d, 
         for (int k = 0; k < 4; ++k) s += v[m][k];
         // long lists (Q2 vertex dofs touch about two dozen cells): four loads in flight at a
         // time, summed in the same order
-        int k = b[m] + 4;
-        for (; k + 4 <= e[m]; k += 4) {
-            const T y0 = Ye[k], y1 = Ye[k + 1], y2 = Ye[k + 2], y3 = Ye[k + 3];
+        int k = 4;
+        for (; k + 4 <= n[m]; k += 4) {
+            const int o = b[m] + k * stride;
+            const T y0 = Ye[o], y1 = Ye[o + stride], y2 = Ye[o + 2 * stride], y3 = Ye[o + 3 * stride];
             s += y0;
             s += y1;
             s += y2;
             s += y3;
         }
-        for (; k < e[m]; ++k) s += Ye[k];
+        for (; k < n[m]; ++k) s += Ye[b[m] + k * stride];
         q[m] = s;
     }
 }

@@ -122,6 +122,24 @@ def test_serving_heat_timesteps_equal_alone_and_oracle(ctx, heat_clients):
     stage of every served timestep equals the same solve alone (fresh imports of the moved
     meshes, fresh assembly, same initial guess) bit for bit, and the first client's second
     timestep matches the oracle stage by stage (each stage from the served inputs)."""
+    _heat_timesteps(ctx, heat_clients)
+
+
+@pytest.fixture(scope="module")
+def heat_clients_r3nm():
+    """Config 1's tip (r = 3 nm, h = 15 nm on a 30 x 30 x 3 nm slab) served beside a small one."""
+    geos = [M.tip_r3nm_geometry(), M.small_geometry(radius=10.0, height=24.0, seed=7)]
+    return [serving.Client(i, M.make_tip_mesh(g, with_atoms=False), field=-1.0 - 0.1 * i, seed=i, heat=True)
+            for i, g in enumerate(geos)]
+
+
+def test_serving_heat_timesteps_config1_tip(ctx, heat_clients_r3nm):
+    """The same checks with config 1's own tip as the first client (its second timestep
+    against the oracle, stage by stage)."""
+    _heat_timesteps(ctx, heat_clients_r3nm)
+
+
+def _heat_timesteps(ctx, heat_clients):
     o = fgpu.opts(tol=1e-9, relative=True, precision=fgpu.FGPU_MIXED, inner_tol=0.1, use_graph=False)
     for c in heat_clients:
         c.seq = 0
