@@ -720,7 +720,7 @@ extern "C" int fgpu_sys_device_bytes(const fgpu_sys *s, size_t *bytes) {
     const fgpu_mesh *m = static_cast<const fgpu_mesh *>(s->mesh);
     const size_t nd = m->n_dofs, ns = m->n_slots, nb = m->bf_cell.size(), nc = m->n_cells;
     *bytes = nd * (11 * 8 + 1 + 6 * 4) + ns * (8 + 4) + nb * 5 * 8 + 2 * kPartRegions * (size_t)s->nb_dof * 8 +
-             2 * kScal * 8 + (s->kq ? nc * 8 * (8 + 8 + 4 + 4) : 0) + (size_t)s->nc * s->nc * 2 + (size_t)s->nc * 12;
+             2 * kScal * 8 + (s->kq ? nc * 8 * (8 + 8 + 4 + 4) : 0) + (size_t)s->nc * s->cld * 2 + (size_t)s->nc * 12;
     return FGPU_OK;
 }
 
@@ -864,6 +864,7 @@ static Desc make_desc32(fgpu_sys *s, bool coarse = false) {
         d.cc = s->cc;
         d.cinv = s->cinv;
         d.cscale = s->cscale;
+        d.cld = s->cld;
         d.nc = s->nc;
         d.agg_shift = s->agg_shift;
         d.cc_in = 1;
@@ -1676,12 +1677,14 @@ static int build_coarse(fgpu_sys *s, int shift) {
     const double cscale = std::ldexp(1.0, ex - 14);
     dfree(s->cinv);
     dfree(s->cc);
-    s->cinv = dalloc<_Float16>((size_t)nc * nc);
+    const int ld = (nc + 7) / 8 * 8;
+    s->cinv = dalloc<_Float16>((size_t)nc * ld);
     s->cscale = cscale;
+    s->cld = ld;
     s->cc = dalloc<float>(nc);
     dfree(s->cc64);
     s->cc64 = dalloc<double>(nc);
-    launch_to_half(st, (size_t)nc * nc, dA, cscale, s->cinv);
+    launch_to_half(st, nc, ld, dA, cscale, s->cinv);
     HIPCHECK(hipMemsetAsync(s->cc, 0, nc * sizeof(float), st));
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
@@ -1731,10 +1734,12 @@ extern "C" int fgpu_sys_get_coarse(fgpu_sys *s, int *nc, int *block, float *einv
             for (int d = 0; d < m->n_dofs; ++d) block[d] = m->idof_of_dof[d] >> s->agg_shift;
         if (einv && s->coarse) {
             HIPCHECK(hipSetDevice(s->ctx->device));
-            std::vector<_Float16> h((size_t)s->nc * s->nc);
+            std::vector<_Float16> h((size_t)s->nc * s->cld);
             d2h(h.data(), s->cinv, h.size(), s->ctx->stream);
             HIPCHECK(hipStreamSynchronize(s->ctx->stream));
-            for (size_t i = 0; i < h.size(); ++i) einv[i] = (float)((double)(float)h[i] * s->cscale);
+            for (int r = 0; r < s->nc; ++r)
+                for (int c = 0; c < s->nc; ++c)
+                    einv[(size_t)r * s->nc + c] = (float)((double)(float)h[(size_t)r * s->cld + c] * s->cscale);
         }
         return FGPU_OK;
     });
@@ -1921,6 +1926,7 @@ static SolveKind make_sets(const std::vector<fgpu_sys *> &sys, std::vector<std::
                     d->cc = s->cc64;
                     d->cinv = s->cinv;
                     d->cscale = s->cscale;
+                    d->cld = s->cld;
                     d->nc = s->nc;
                     d->agg_shift = s->agg_shift;
                 }

@@ -16,7 +16,6 @@ namespace fgpu {
 constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
 constexpr int kDofIter = 4;   // dofs per thread in the per-dof kernels
 constexpr int kDofBlock = kBlock * kDofIter;
-constexpr int kSlotGroup = 64;  // dofs per group of interleaved slots (one wave's dofs)
 // coarse aggregates of the two-level preconditioner: 2^agg_shift consecutive internal dofs, a
 // multiple of kAggMin (one round of a dof workgroup: its sums of r are kept per round)
 constexpr int kAggMin = kBlock;
@@ -82,10 +81,6 @@ struct Desc {
     const int *cells;             // 8 per cell, lexicographic, internal dof ids
     const void *X;                // T4 coordinates per dof
     const int *gptr;              // dof i owns slot-buffer positions [gptr[i], gptr[i+1])
-    // or, with slot counts (interleaved slots): dof i's k-th slot at gptr[i] + k * kSlotGroup,
-    // k < scnt[i] (each group of kSlotGroup consecutive dofs stores its k-th slots side by
-    // side, so a wave reads them coalesced); null for contiguous slots
-    const int *scnt;
     int n_patch;                  // patches (see Patches)
     int pcells, node_cap, ppb;    // patch size, LDS node capacity, patches per workgroup
     const int *pcell0, *pnode0, *pnodes, *pslot;
@@ -106,8 +101,9 @@ struct Desc {
     // two-level preconditioner of the FP32 CG (null: Jacobi only): coarse correction c per
     // dof block (T), nc x nc inverse of the coarse operator (row-major), nc = dof blocks
     void *cc;
-    const _Float16 *cinv;         // FP16, scaled: the inverse is cinv x cscale
+    const _Float16 *cinv;         // FP16, scaled: the inverse is cinv x cscale; rows of cld
     double cscale;                // a power of two
+    int cld;                      // row stride of cinv: nc rounded up to 8 (zero padding)
     int nc, agg_shift;            // aggregates and log2 of their size
     int cc_in;                    // the element input is z: add its coarse part (CG sets only;
                                   // a residual set's input is the iterate)
@@ -235,8 +231,9 @@ struct Sys {
     int agg_shift = 10;
     // nc x nc, FP16 scaled by cscale (a power of two: the largest entry, on the diagonal of
     // the SPD inverse, stays below 2^14): the coarse product reads half the bytes of FP32
-    _Float16 *cinv = nullptr;
+    _Float16 *cinv = nullptr;     // nc rows of cld entries (zero past nc)
     double cscale = 1.0;
+    int cld = 0;
     float *cc = nullptr;          // nc
     bool desc32_affine = false;   // the last FP32 descriptor built uses the tet-split form
     // replayable heat terms of other systems that read this one's solution: while any is

@@ -28,7 +28,7 @@ void launch_coarse_indicator(hipStream_t s, int n, const unsigned char *mask, co
                              double *u);
 void launch_coarse_sums(hipStream_t s, int n, const unsigned char *mask, const double *q, double *out);
 void launch_gj_step(hipStream_t s, int n, int k, const double *A, double *B);
-void launch_to_half(hipStream_t s, size_t n, const double *a, double scale, _Float16 *out);
+void launch_to_half(hipStream_t s, int n, int ld, const double *a, double scale, _Float16 *out);
 void launch_enforce_dirichlet(hipStream_t s, int n, const unsigned char *mask, const double *g, double *x);
 void launch_lift(hipStream_t s, int n, const unsigned char *mask, const double *g, const double *Ag, const double *diag,
                  double *b, double *x);

@@ -307,24 +307,22 @@ template <class T> __device__ __forceinline__ T coarse_term(const Desc &d, int i
 template <class T> __device__ __forceinline__ void gather_slots4(const Desc &d, int base, T (&q)[kDofIter]) {
     const auto Ye = G(reinterpret_cast<const T *>(d.Ye));
     const auto gp = G(d.gptr);
-    // contiguous slots (stride 1) or interleaved ones (stride kSlotGroup, count scnt)
-    const int stride = d.scnt ? kSlotGroup : 1;
-    int b[kDofIter], n[kDofIter];
+    int b[kDofIter], e[kDofIter];
 #pragma unroll
     for (int m = 0; m < kDofIter; ++m) {
         const int i = base + threadIdx.x + m * kBlock;
         const int ii = i < d.n_dofs ? i : d.n_dofs - 1;
         b[m] = gp[ii];
-        n[m] = d.scnt ? G(d.scnt)[ii] : gp[ii + 1] - b[m];
+        e[m] = gp[ii + 1];
     }
     T v[kDofIter][4];
 #pragma unroll
     for (int m = 0; m < kDofIter; ++m)
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int idx = k < n[m] ? b[m] + k * stride : b[m];
+            const int idx = b[m] + k < e[m] ? b[m] + k : b[m];
             const T y = Ye[idx];
-            v[m][k] = k < n[m] ? y : (T)0;
+            v[m][k] = b[m] + k < e[m] ? y : (T)0;
         }
 #pragma unroll
     for (int m = 0; m < kDofIter; ++m) {
@@ -333,16 +331,15 @@ template <class T> __device__ __forceinline__ void gather_slots4(const Desc &d, 
         for (int k = 0; k < 4; ++k) s += v[m][k];
         // long lists (Q2 vertex dofs touch about two dozen cells): four loads in flight at a
         // time, summed in the same order
-        int k = 4;
-        for (; k + 4 <= n[m]; k += 4) {
-            const int o = b[m] + k * stride;
-            const T y0 = Ye[o], y1 = Ye[o + stride], y2 = Ye[o + 2 * stride], y3 = Ye[o + 3 * stride];
+        int k = b[m] + 4;
+        for (; k + 4 <= e[m]; k += 4) {
+            const T y0 = Ye[k], y1 = Ye[k + 1], y2 = Ye[k + 2], y3 = Ye[k + 3];
             s += y0;
             s += y1;
             s += y2;
             s += y3;
         }
-        for (; k < n[m]; ++k) s += Ye[b[m] + k * stride];
+        for (; k < e[m]; ++k) s += Ye[k];
         q[m] = s;
     }
 }
@@ -1428,10 +1425,13 @@ __global__ void k_gj_step(int n, int k, const double *__restrict__ A, double *__
     B[e] = v;
 }
 
-// out = a / scale in FP16 (scale a power of two: the division is exact before the rounding)
-__global__ void k_to_half(size_t n, const double *__restrict__ a, double inv_scale, _Float16 *__restrict__ out) {
+// out = a / scale in FP16 (scale a power of two: the division is exact before the rounding);
+// a: n x n, out: n rows of ld >= n entries, zero past n
+__global__ void k_to_half(int n, int ld, const double *__restrict__ a, double inv_scale, _Float16 *__restrict__ out) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (_Float16)(a[i] * inv_scale);
+    if (i >= (size_t)n * ld) return;
+    const int r = (int)(i / ld), c = (int)(i % ld);
+    out[i] = c < n ? (_Float16)(a[(size_t)r * n + c] * inv_scale) : (_Float16)0.0f;
 }
 
 void launch_coarse_indicator(hipStream_t s, int n, const unsigned char *mask, const int *color, int shift, int c,
@@ -1446,8 +1446,9 @@ void launch_gj_step(hipStream_t s, int n, int k, const double *A, double *B) {
     const size_t e = (size_t)n * n;
     if (e) hipLaunchKernelGGL(k_gj_step, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, n, k, A, B);
 }
-void launch_to_half(hipStream_t s, size_t n, const double *a, double scale, _Float16 *out) {
-    if (n) hipLaunchKernelGGL(k_to_half, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, 1.0 / scale, out);
+void launch_to_half(hipStream_t s, int n, int ld, const double *a, double scale, _Float16 *out) {
+    const size_t e = (size_t)n * ld;
+    if (e) hipLaunchKernelGGL(k_to_half, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, n, ld, a, 1.0 / scale, out);
 }
 
 void launch_invdiag(hipStream_t s, int n, const unsigned char *mask, const double *diag, double *inv64, float *inv32) {

@@ -507,8 +507,6 @@ struct fgpu_q2 {
     std::vector<int> api_of_q2, q2_of_api;
     std::vector<unsigned char> face_dofs_mask;
     int *d_c27 = nullptr, *d_spos = nullptr, *d_gptr = nullptr, *d_fgptr = nullptr, *d_fgidx = nullptr;
-    int *d_scnt = nullptr;                // slots per dof (interleaved slot layout)
-    size_t n_slots = 0;
     double *fslot = nullptr;
     // system (internal dof order)
     double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *Ye = nullptr;
@@ -537,7 +535,6 @@ struct fgpu_q2 {
         q2free(d_c27);
         q2free(d_spos);
         q2free(d_gptr);
-        q2free(d_scnt);
         q2free(d_fgptr);
         q2free(d_fgidx);
         q2free(d_desc);
@@ -653,7 +650,6 @@ Desc q2_desc(fgpu_q2 *s) {
     d.n_cells = s->n_cells;
     d.nb_dof = s->nb_dof;
     d.gptr = s->d_gptr;
-    d.scnt = s->d_scnt;
     d.dmask = s->d_dmask;
     d.Ye = s->Ye;
     d.q = s->q;
@@ -762,29 +758,14 @@ extern "C" int fgpu_q2_create(fgpu_mesh *m, fgpu_q2 **out) {
         s->nv = nv;
         s->n_dofs = number_dofs(icells, nv, nc, s->c27);
         const int n = s->n_dofs;
-        // interleaved slots: a dof's (cell, node) entries in ascending order, and each group of
-        // kSlotGroup consecutive dofs stores its k-th entries side by side (entry k of dof i at
-        // gptr[i] + k kSlotGroup), padded to the group's longest list: the gather's lanes then
-        // read consecutive slots (vertex dofs sum about two dozen, interior ones one)
-        std::vector<int> cnt(n, 0), gptr(n + 1, 0), spos(27 * (size_t)nc);
-        for (int d : s->c27) cnt[d]++;
-        size_t total = 0;
-        for (int g0 = 0; g0 < n; g0 += kSlotGroup) {
-            const int g1 = std::min(n, g0 + kSlotGroup);
-            const int mx = *std::max_element(cnt.begin() + g0, cnt.begin() + g1);
-            for (int i = g0; i < g1; ++i) gptr[i] = (int)(total + (size_t)(i - g0));
-            total += (size_t)mx * kSlotGroup;
-        }
-        if (total > (size_t)INT32_MAX) throw std::invalid_argument("mesh too large for the Q2 slot buffer");
-        gptr[n] = (int)total;
+        // dof-major slots: each dof's (cell, node) entries in ascending order
+        std::vector<int> gptr(n + 1, 0), spos(27 * (size_t)nc);
+        for (int d : s->c27) gptr[d + 1]++;
+        for (int i = 0; i < n; ++i) gptr[i + 1] += gptr[i];
         {
-            std::vector<int> k(n, 0);
-            for (size_t e = 0; e < s->c27.size(); ++e) {
-                const int d = s->c27[e];
-                spos[e] = gptr[d] + (k[d]++) * kSlotGroup;
-            }
+            std::vector<int> fill(gptr.begin(), gptr.end() - 1);
+            for (size_t e = 0; e < s->c27.size(); ++e) spos[e] = fill[s->c27[e]]++;
         }
-        s->n_slots = total;
         // boundary faces: 9 slots per face, dof -> face slots in ascending order
         const int nb = (int)m->bf_cell.size();
         s->nb = nb;
@@ -831,9 +812,7 @@ extern "C" int fgpu_q2_create(fgpu_mesh *m, fgpu_q2 **out) {
         for (double **v : {&s->b, &s->x, &s->r, &s->z, &s->p, &s->q, &s->invd, &s->diag, &s->g, &s->tmp, &s->cd, &s->cres})
             *v = q2alloc<double>(n);
         s->invd32 = q2alloc<float>(n);
-        s->Ye = q2alloc<double>(std::max<size_t>(s->n_slots, 1));
-        s->d_scnt = q2alloc<int>(n);
-        Q2CHECK(hipMemcpyAsync(s->d_scnt, cnt.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+        s->Ye = q2alloc<double>(27 * (size_t)nc);
         s->fslot = q2alloc<double>(9 * (size_t)std::max(nb, 1));
         s->ccoef = q2alloc<double>(2 * kChebMax + 1);
         s->hist = q2alloc<double>(2 * kHist);
