@@ -389,8 +389,7 @@ def paths_bench(no_cpu: bool):
     if not out["interpolator_create"]["same_tables"]:
         raise RuntimeError("device and host interpolator tables differ")
     del iph
-    # the first located transfer of a new interpolator also fetches the host copy of its tet
-    # tables (the chained guess reads them)
+    # the first located transfer of a new interpolator (it allocates the point and chain buffers)
     ipf = icreate(False)
     t = time.perf_counter()
     ipf.points(tip.atoms, 1)

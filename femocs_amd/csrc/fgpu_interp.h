@@ -9,6 +9,7 @@ namespace fgpu {
 
 constexpr double kZero = 1e-15;   // InterpolatorCells::zero (include/InterpolatorCells.h:241)
 constexpr int kBvhStack = 96;     // traversal stack of the point queries (tree depth <= 64 + 32)
+constexpr int kChainW = 64;       // possible results per point of the device chained guess
 
 // Linear-tet tables exactly as LinearTetrahedra::precompute builds them
 // (src/InterpolatorCells.cpp:523-629): 1/det0, four Vec4 cofactor rows, centroid, neighbour

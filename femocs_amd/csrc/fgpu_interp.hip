@@ -41,6 +41,12 @@ struct fgpu_interp {
     int4 *cand = nullptr, *cand2 = nullptr;
     int *tet = nullptr, *cells = nullptr;
     size_t cap = 0;
+    // device chained guess (grown on demand): containing cells and tables per point, per
+    // workgroup of 256 points its composition and starting position, the fallback flag
+    int *chain_lst = nullptr, *chain_entry = nullptr, *chain_flag = nullptr;
+    unsigned char *chain_tab = nullptr, *chain_bt = nullptr;
+    size_t chain_cap = 0;
+    bool host_chain = false;                // FGPU_INTERP_HOST_CHAIN
     // surface faces (LinearTriangles), set by fgpu_interp_set_triangles
     std::vector<double> hxyz;
     LinTriHost T;
@@ -185,23 +191,35 @@ struct TriCells {
 
 // Per point: the lowest searchable cell containing it (x, -1 if none), the number of cells of
 // any marker containing it (y), the cell of the nearest centroid, lowest index among equal
-// distances (z, only when x < 0) and a traversal-overflow flag (w).
+// distances (z, only when x < 0) and a traversal-overflow flag (w). With `lst`, also the cells
+// containing the point, ascending, in its row of kChainW entries (the first kChainW of them).
 template <class Cells>
 __global__ __launch_bounds__(256) void k_candidates(int n, const double *__restrict__ pts, int n_leaves,
                                                     const int *__restrict__ child, const float *__restrict__ box,
                                                     const float *__restrict__ cbox, const float *__restrict__ leafbox,
                                                     const int *__restrict__ prim, Cells cells,
-                                                    const double *__restrict__ cen, int4 *__restrict__ cand) {
+                                                    const double *__restrict__ cen, int4 *__restrict__ cand,
+                                                    int *__restrict__ lst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double p[3] = {pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]};
     int t0 = INT_MAX, cnt = 0, overflow = 0;
     int stack[kBvhStack];
     int sp = 0;
+    int *row = lst ? lst + (size_t)i * kChainW : nullptr;
     auto leaf_hit = [&](int leaf) {
         if (!in_box(leafbox + 6 * (size_t)leaf, p)) return;
         const int t = prim[leaf];
         if (cells.contains(p, t)) {
+            if (row && cnt < kChainW) {
+                // insertion into the ascending row
+                int k = cnt;
+                while (k > 0 && row[k - 1] > t) {
+                    row[k] = row[k - 1];
+                    --k;
+                }
+                row[k] = t;
+            }
             ++cnt;
             if (cells.searchable(t) && t < t0) t0 = t;
         }
@@ -268,6 +286,183 @@ __global__ __launch_bounds__(256) void k_candidates(int n, const double *__restr
         }
     }
     cand[i] = make_int4(t0 == INT_MAX ? -1 : t0, cnt, amin, overflow);
+}
+
+// ------------------------------------------------------------------------------------------
+// chained guess on the device
+// ------------------------------------------------------------------------------------------
+// The reference locates point after point, each from the previous point's result
+// (LinearTetrahedra::locate_cell, src/InterpolatorCells.cpp:268-307, driven by
+// src/SolutionReader.cpp:136-165): r_i = f_i(|r_{i-1}|), f_i(g) = g if g contains p_i, else
+// the first neighbour of g (in list order) that contains it, else the lowest searchable cell
+// containing it (cand.x) or, if none, minus the nearest centroid (-cand.z). f_i is constant
+// when no cell contains p_i or exactly one searchable cell does. Otherwise r_i is one of
+// C_i = (the cells containing p_i, ascending) followed by -cand.z when cand.x < 0, and the
+// chain becomes a composition of tables T_i: position in C_{i-1} -> position in C_i (at most
+// kChainW entries each), evaluated as a prefix scan: per workgroup of 256 points, then over
+// the workgroups, then back to every point.
+__device__ __forceinline__ bool chain_fixed(int4 c) { return c.y == 0 || (c.y == 1 && c.x >= 0); }
+__device__ __forceinline__ int chain_size(int4 c) { return chain_fixed(c) ? 1 : c.y + (c.x < 0 ? 1 : 0); }
+// value at position k of C_i (k < min(chain_size, kChainW))
+__device__ __forceinline__ int chain_value(int4 c, const int *row, int k) {
+    if (chain_fixed(c)) return c.y == 0 ? -c.z : c.x;
+    return k < c.y ? row[k] : -c.z;
+}
+// position of t in the ascending row of m entries, -1 if absent
+__device__ __forceinline__ int chain_find(const int *row, int m, int t) {
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (row[mid] < t) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < m && row[lo] == t ? lo : -1;
+}
+
+// T_i for every point whose f_i is not constant (the rows of the others are never read);
+// flag = 1 when a point's candidates are incomplete (traversal overflow, or more than kChainW
+// possible results): the caller then resolves the chain on the host
+__global__ __launch_bounds__(256) void k_chain_tables(int n, const int4 *__restrict__ cand, const int *__restrict__ lst,
+                                                      const int *__restrict__ nbr_ptr, const int *__restrict__ nbr,
+                                                      int n_cells, int g0, unsigned char *__restrict__ tab,
+                                                      int *__restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int4 c = cand[i];
+    if (c.w || chain_size(c) > kChainW) atomicOr(flag, 1);
+    if (chain_fixed(c)) return;
+    unsigned char *T = tab + (size_t)i * kChainW;
+    const int *row = lst + (size_t)i * kChainW;
+    const int m = min(c.y, kChainW);
+    const int fb = c.x >= 0 ? chain_find(row, m, c.x) : m;
+    int nd = 1;
+    int4 cp = make_int4(0, 0, 0, 0);
+    const int *prow = nullptr;
+    if (i > 0) {
+        cp = cand[i - 1];
+        prow = lst + (size_t)(i - 1) * kChainW;
+        nd = min(chain_size(cp), kChainW);
+    }
+    for (int k = 0; k < kChainW; ++k) {
+        int pos = 0;
+        if (k < nd) {
+            const int g = i == 0 ? g0 : abs(chain_value(cp, prow, k));
+            pos = -1;
+            if (g >= 0 && g < n_cells) {
+                pos = chain_find(row, m, g);
+                for (int e = nbr_ptr[g]; pos < 0 && e < nbr_ptr[g + 1]; ++e) pos = chain_find(row, m, nbr[e]);
+            }
+            if (pos < 0) pos = fb;
+        }
+        T[k] = (unsigned char)min(max(pos, 0), kChainW - 1);
+    }
+}
+
+// LDS rows of a table: kChainW bytes, padded to spread consecutive rows over the banks
+constexpr int kChainRow = kChainW + 4;
+
+// inclusive scan of the tables of 256 consecutive functions in LDS (F[cur] in, result in the
+// returned buffer): row j becomes T_j o ... o T_0
+__device__ __forceinline__ int chain_scan256(unsigned char (*F)[256 * kChainRow], int cur) {
+    const int j = threadIdx.x;
+    for (int d = 1; d < 256; d <<= 1) {
+        __syncthreads();
+        const unsigned char *src = F[cur] + j * kChainRow;
+        unsigned *dst = reinterpret_cast<unsigned *>(F[cur ^ 1] + j * kChainRow);
+        if (j >= d) {
+            const unsigned *prev = reinterpret_cast<const unsigned *>(F[cur] + (j - d) * kChainRow);
+#pragma unroll 4
+            for (int w = 0; w < kChainW / 4; ++w) {
+                const unsigned q = prev[w];
+                dst[w] = (unsigned)src[q & 255] | (unsigned)src[(q >> 8) & 255] << 8 |
+                         (unsigned)src[(q >> 16) & 255] << 16 | (unsigned)src[q >> 24] << 24;
+            }
+        } else {
+            const unsigned *s4 = reinterpret_cast<const unsigned *>(src);
+#pragma unroll 4
+            for (int w = 0; w < kChainW / 4; ++w) dst[w] = s4[w];
+        }
+        cur ^= 1;
+    }
+    __syncthreads();
+    return cur;
+}
+
+// per workgroup of 256 points: tab row i becomes the prefix T_i o ... o T_(first point of the
+// workgroup); the workgroup's whole composition goes to bt
+__global__ __launch_bounds__(256) void k_chain_scan(int n, const int4 *__restrict__ cand, unsigned char *__restrict__ tab,
+                                                    unsigned char *__restrict__ bt) {
+    __shared__ __align__(16) unsigned char F[2][256 * kChainRow];
+    const int j = threadIdx.x;
+    const int i = blockIdx.x * 256 + j;
+    unsigned *r = reinterpret_cast<unsigned *>(F[0] + j * kChainRow);
+    if (i >= n) {
+        for (int w = 0; w < kChainW / 4; ++w)      // identity
+            r[w] = (unsigned)(4 * w) | (unsigned)(4 * w + 1) << 8 | (unsigned)(4 * w + 2) << 16 |
+                   (unsigned)(4 * w + 3) << 24;
+    } else if (chain_fixed(cand[i])) {
+        for (int w = 0; w < kChainW / 4; ++w) r[w] = 0;
+    } else {
+        const uint4 *t = reinterpret_cast<const uint4 *>(tab + (size_t)i * kChainW);
+        for (int w = 0; w < kChainW / 16; ++w) {
+            const uint4 v = t[w];
+            r[4 * w] = v.x;
+            r[4 * w + 1] = v.y;
+            r[4 * w + 2] = v.z;
+            r[4 * w + 3] = v.w;
+        }
+    }
+    const int cur = chain_scan256(F, 0);
+    const unsigned *o = reinterpret_cast<const unsigned *>(F[cur] + j * kChainRow);
+    if (i < n) {
+        uint4 *t = reinterpret_cast<uint4 *>(tab + (size_t)i * kChainW);
+        for (int w = 0; w < kChainW / 16; ++w) t[w] = make_uint4(o[4 * w], o[4 * w + 1], o[4 * w + 2], o[4 * w + 3]);
+    }
+    if (j == 255) {
+        uint4 *t = reinterpret_cast<uint4 *>(bt + (size_t)blockIdx.x * kChainW);
+        for (int w = 0; w < kChainW / 16; ++w) t[w] = make_uint4(o[4 * w], o[4 * w + 1], o[4 * w + 2], o[4 * w + 3]);
+    }
+}
+
+// one workgroup: the position each workgroup of points starts from (in C of the point before
+// it), by scanning the workgroups' compositions 256 at a time; the chain starts at position 0
+// of the one-entry domain {initial guess}
+__global__ __launch_bounds__(256) void k_chain_top(int nb, const unsigned char *__restrict__ bt, int *__restrict__ entry) {
+    __shared__ __align__(16) unsigned char F[2][256 * kChainRow];
+    const int j = threadIdx.x;
+    int s = 0;
+    for (int base = 0; base < nb; base += 256) {
+        unsigned *r = reinterpret_cast<unsigned *>(F[0] + j * kChainRow);
+        if (base + j < nb) {
+            const uint4 *t = reinterpret_cast<const uint4 *>(bt + (size_t)(base + j) * kChainW);
+            for (int w = 0; w < kChainW / 16; ++w) {
+                const uint4 v = t[w];
+                r[4 * w] = v.x;
+                r[4 * w + 1] = v.y;
+                r[4 * w + 2] = v.z;
+                r[4 * w + 3] = v.w;
+            }
+        } else {
+            for (int w = 0; w < kChainW / 4; ++w)
+                r[w] = (unsigned)(4 * w) | (unsigned)(4 * w + 1) << 8 | (unsigned)(4 * w + 2) << 16 |
+                       (unsigned)(4 * w + 3) << 24;
+        }
+        const int cur = chain_scan256(F, 0);
+        if (base + j < nb) entry[base + j] = j == 0 ? s : F[cur][(j - 1) * kChainRow + s];
+        s = F[cur][255 * kChainRow + s];
+        __syncthreads();
+    }
+}
+
+// each point's result from its workgroup's starting position and its prefix table
+__global__ __launch_bounds__(256) void k_chain_finish(int n, const int4 *__restrict__ cand, const int *__restrict__ lst,
+                                                      const unsigned char *__restrict__ tab,
+                                                      const int *__restrict__ entry, int *__restrict__ res) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int4 c = cand[i];
+    const int pos = tab[(size_t)i * kChainW + entry[i >> 8]];
+    res[i] = chain_value(c, lst + (size_t)i * kChainW, min(pos, kChainW - 1));
 }
 
 // LinearHexahedra::project_to_nat_coords (src/InterpolatorCells.cpp:1237-1244, 1272-1317):
@@ -658,6 +853,21 @@ void ensure_points(fgpu_interp *ip, size_t n) {
     ip->cap = c;
 }
 
+void ensure_chain(fgpu_interp *ip, size_t n) {
+    if (n <= ip->chain_cap) return;
+    IPCHECK(hipStreamSynchronize(ip->ctx->stream));
+    for (int **p : {&ip->chain_lst, &ip->chain_entry, &ip->chain_flag}) release(*p);
+    release(ip->chain_tab);
+    release(ip->chain_bt);
+    const size_t c = std::max<size_t>(n, 1024), nb = (c + 255) / 256;
+    IPCHECK(hipMalloc(&ip->chain_lst, c * kChainW * sizeof(int)));
+    IPCHECK(hipMalloc(&ip->chain_tab, c * kChainW));
+    IPCHECK(hipMalloc(&ip->chain_bt, nb * kChainW));
+    IPCHECK(hipMalloc(&ip->chain_entry, nb * sizeof(int)));
+    IPCHECK(hipMalloc(&ip->chain_flag, sizeof(int)));
+    ip->chain_cap = c;
+}
+
 void free_triangles(fgpu_interp *ip) {
     for (double **p : {&ip->tri_tab, &ip->tri_cen, &ip->quad_cen}) release(*p);
     for (int **p : {&ip->tri_child, &ip->tri_prim, &ip->tri_nodes, &ip->quad_nodes, &ip->tri6, &ip->quad_child,
@@ -678,15 +888,37 @@ void free_interp(fgpu_interp *ip) {
     for (float **p : {&ip->box, &ip->cbox, &ip->leafbox}) release(*p);
     release(ip->cand);
     release(ip->cand2);
+    for (int **p : {&ip->chain_lst, &ip->chain_entry, &ip->chain_flag}) release(*p);
+    release(ip->chain_tab);
+    release(ip->chain_bt);
+    ip->chain_cap = 0;
     free_triangles(ip);
 }
 
 #include "fgpu_interp_build.inc"
 
-void tet_candidates(fgpu_interp *ip, int n, int4 *cand) {
+void tet_candidates(fgpu_interp *ip, int n, int4 *cand, int *lst = nullptr) {
     hipLaunchKernelGGL(k_candidates<TetCells>, dim3(grid(n)), dim3(256), 0, ip->ctx->stream, n, ip->pts,
                        ip->B.n_leaves, ip->child, ip->box, ip->cbox, ip->leafbox, ip->prim,
-                       TetCells{ip->det0, ip->det, ip->marker}, ip->cen, cand);
+                       TetCells{ip->det0, ip->det, ip->marker}, ip->cen, cand, lst);
+}
+
+// the chained guess over the located points (ip->pts, n of them) into ip->tet, in the
+// reference's numbering of the tets; g0 is the first point's guess. Leaves *ip->chain_flag = 1
+// when some point's candidates are incomplete (the host chain is then needed).
+void device_chain(fgpu_interp *ip, int n, int g0) {
+    hipStream_t st = ip->ctx->stream;
+    ensure_chain(ip, n);
+    const int nb = (n + 255) / 256;
+    IPCHECK(hipMemsetAsync(ip->chain_flag, 0, sizeof(int), st));
+    tet_candidates(ip, n, ip->cand, ip->chain_lst);
+    hipLaunchKernelGGL(k_chain_tables, dim3(grid(n)), dim3(256), 0, st, n, ip->cand, ip->chain_lst, ip->d_nbr_ptr,
+                       ip->d_nbr, ip->n_tets, g0, ip->chain_tab, ip->chain_flag);
+    hipLaunchKernelGGL(k_chain_scan, dim3(nb), dim3(256), 0, st, n, ip->cand, ip->chain_tab, ip->chain_bt);
+    hipLaunchKernelGGL(k_chain_top, dim3(1), dim3(256), 0, st, nb, ip->chain_bt, ip->chain_entry);
+    hipLaunchKernelGGL(k_chain_finish, dim3(grid(n)), dim3(256), 0, st, n, ip->cand, ip->chain_lst, ip->chain_tab,
+                       ip->chain_entry, ip->tet);
+    IPCHECK(hipGetLastError());
 }
 
 }  // namespace
@@ -708,7 +940,7 @@ extern "C" int fgpu_interp_create_ex(fgpu_ctx *c, int n_nodes, const double *xyz
                                      int flags, fgpu_interp **out) {
     if (!c || !out || n_nodes <= 0 || n_tets <= 0 || !xyz || !node_kind || !tets || !tet_nbr || !tet_marker ||
         (n_hexes > 0 && (!hexes || !hex_marker)) || (n_hexes != 0 && n_hexes != 4 * n_tets) ||
-        (flags & ~FGPU_INTERP_HOST))
+        (flags & ~(FGPU_INTERP_HOST | FGPU_INTERP_HOST_CHAIN)))
         return FGPU_EINVAL;
     *out = nullptr;
     fgpu_interp *ip = new (std::nothrow) fgpu_interp();
@@ -748,6 +980,7 @@ extern "C" int fgpu_interp_create_ex(fgpu_ctx *c, int n_nodes, const double *xyz
             ip->d_hex_cell = upload(hex_cell.data(), hex_cell.size(), s);
             IPCHECK(hipStreamSynchronize(s));       // hex_cell is read until the copy ends
         }
+        ip->host_chain = (flags & FGPU_INTERP_HOST_CHAIN) != 0;
         if (!(flags & FGPU_INTERP_HOST)) {
             device_build(ip, tet_nbr, tet_marker, node_kind, hex_marker);
         } else {
@@ -1248,23 +1481,42 @@ static int interp_run(fgpu_interp *ip, int n, const double *xyz, int rank, int *
         ensure_points(ip, n);
         IPCHECK(hipMemcpyAsync(ip->pts, xyz, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
         if (locate) {
-            tet_candidates(ip, n, ip->cand);
-            IPCHECK(hipGetLastError());
-            std::vector<int4> cand(n);
-            IPCHECK(hipMemcpyAsync(cand.data(), ip->cand, n * sizeof(int4), hipMemcpyDeviceToHost, st));
-            IPCHECK(hipStreamSynchronize(st));
-            for (int i = 0; i < n; ++i)
-                if (cand[i].w) throw std::runtime_error("point search stack overflow");
-            std::vector<int> tet(n);
-            ensure_host_tets(ip);
-            resolve_chain(ip->L, n, xyz, cand.data(), rank, tet.data());
-            IPCHECK(hipMemcpyAsync(ip->tet, tet.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_interp, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, rank, rank == 3 ? 1 : 0, ip->tet,
-                               ip->det0, ip->det, ip->tets, ip->c10, ip->hexes, ip->xyz, ip->sol, ip->cells, ip->out);
-            IPCHECK(hipGetLastError());
-            IPCHECK(hipMemcpyAsync(cells, ip->cells, n * sizeof(int), hipMemcpyDeviceToHost, st));
-            IPCHECK(hipMemcpyAsync(out, ip->out, 5 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
-            IPCHECK(hipStreamSynchronize(st));
+            int flag = 1;
+            if (!ip->host_chain) {
+                // the first point's guess: |-1| as a tet, or |-1| / 4 from the hex numbering
+                device_chain(ip, n, rank == 3 ? 0 : 1);
+                hipLaunchKernelGGL(k_interp, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, rank, rank == 3 ? 1 : 0,
+                                   ip->tet, ip->det0, ip->det, ip->tets, ip->c10, ip->hexes, ip->xyz, ip->sol,
+                                   ip->cells, ip->out);
+                IPCHECK(hipGetLastError());
+                IPCHECK(hipMemcpyAsync(&flag, ip->chain_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+                IPCHECK(hipMemcpyAsync(cells, ip->cells, n * sizeof(int), hipMemcpyDeviceToHost, st));
+                IPCHECK(hipMemcpyAsync(out, ip->out, 5 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+                IPCHECK(hipStreamSynchronize(st));
+            } else {
+                tet_candidates(ip, n, ip->cand);
+                IPCHECK(hipGetLastError());
+            }
+            if (flag) {
+                // the chain on the host, point after point (FGPU_INTERP_HOST_CHAIN, or a
+                // traversal overflow, or a point in more than kChainW - 1 cells)
+                std::vector<int4> cand(n);
+                IPCHECK(hipMemcpyAsync(cand.data(), ip->cand, n * sizeof(int4), hipMemcpyDeviceToHost, st));
+                IPCHECK(hipStreamSynchronize(st));
+                for (int i = 0; i < n; ++i)
+                    if (cand[i].w) throw std::runtime_error("point search stack overflow");
+                std::vector<int> tet(n);
+                ensure_host_tets(ip);
+                resolve_chain(ip->L, n, xyz, cand.data(), rank, tet.data());
+                IPCHECK(hipMemcpyAsync(ip->tet, tet.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(k_interp, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, rank, rank == 3 ? 1 : 0,
+                                   ip->tet, ip->det0, ip->det, ip->tets, ip->c10, ip->hexes, ip->xyz, ip->sol,
+                                   ip->cells, ip->out);
+                IPCHECK(hipGetLastError());
+                IPCHECK(hipMemcpyAsync(cells, ip->cells, n * sizeof(int), hipMemcpyDeviceToHost, st));
+                IPCHECK(hipMemcpyAsync(out, ip->out, 5 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+                IPCHECK(hipStreamSynchronize(st));
+            }
         } else {
             const int lim = rank == 3 ? ip->n_hexes : ip->n_tets;
             for (int i = 0; i < n; ++i)
@@ -1405,7 +1657,7 @@ static int surface_run(fgpu_interp *ip, int n, const double *xyz, int rank, int 
             if (locate)
                 hipLaunchKernelGGL(k_candidates<TriCells>, dim3(grid(n)), dim3(256), 0, st, n, ip->pts,
                                    ip->TB.n_leaves, ip->tri_child, ip->tri_box, ip->tri_cbox, ip->tri_leafbox,
-                                   ip->tri_prim, TriCells{ip->tri_tab}, ip->tri_cen, ip->cand2);
+                                   ip->tri_prim, TriCells{ip->tri_tab}, ip->tri_cen, ip->cand2, nullptr);
             IPCHECK(hipGetLastError());
             std::vector<int4> tc(n), fc(locate ? n : 0);
             IPCHECK(hipMemcpyAsync(tc.data(), ip->cand, n * sizeof(int4), hipMemcpyDeviceToHost, st));

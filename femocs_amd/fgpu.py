@@ -18,6 +18,7 @@ FGPU_PC_NONE, FGPU_PC_JACOBI, FGPU_PC_CHEBYSHEV = 0, 1, 2
 FGPU_IMPORT_HOST = 1
 FGPU_MESH_DIGEST_TABLES = 32
 FGPU_INTERP_HOST = 1
+FGPU_INTERP_HOST_CHAIN = 2
 FGPU_INTERP_DIGEST_TABLES = 23
 BID_COPPER_SURFACE, BID_VACUUM_TOP, BID_COPPER_BOTTOM, BID_SIDES = 2, 8, 7, 4
 

@@ -26,8 +26,10 @@ class Interpolator:
     hexes (hex 4t+k around vertex k of tet t)."""
 
     def __init__(self, ctx: fgpu.Context, tip, search_region: int = TYPES_VACUUM, empty_value: float = 0.0,
-                 host_build: bool = False):
-        """The tables are built on the device; host_build=True builds the same bits on the host."""
+                 host_build: bool = False, host_chain: bool = False):
+        """The tables are built on the device; host_build=True builds the same bits on the host.
+        host_chain=True resolves the located transfers' chained guess on the host, point after
+        point (the device resolves it as a prefix scan by default; same cells)."""
         self.ctx = ctx
         self.n_nodes = len(tip.nodes)
         self.search_region = search_region
@@ -40,7 +42,8 @@ class Interpolator:
             np.ascontiguousarray(tip.tet_marker, np.int32), len(tip.hexes),
             np.ascontiguousarray(tip.hexes, np.int32).reshape(-1),
             np.ascontiguousarray(tip.hex_marker, np.int32), int(search_region), float(empty_value),
-            fgpu.FGPU_INTERP_HOST if host_build else 0, C.byref(h)),
+            (fgpu.FGPU_INTERP_HOST if host_build else 0) | (fgpu.FGPU_INTERP_HOST_CHAIN if host_chain else 0),
+            C.byref(h)),
             "fgpu_interp_create_ex")
         self.h = h
 

@@ -269,6 +269,10 @@ int fgpu_interp_create(fgpu_ctx *ctx, int n_nodes, const double *xyz, const int 
  * counterpart in the reference, whose InterpolatorCells::precompute runs on the host
  * (src/InterpolatorCells.cpp:523-742). */
 #define FGPU_INTERP_HOST 1
+/* located transfers resolve the chained guess on the host, point after point as the reference
+ * does (the candidates still come from the device tree search), instead of the device prefix
+ * scan; the same cells either way */
+#define FGPU_INTERP_HOST_CHAIN 2
 int fgpu_interp_create_ex(fgpu_ctx *ctx, int n_nodes, const double *xyz, const int *node_kind, int n_tets,
                           const int *tets, const int *tet_nbr, const int *tet_marker, int n_hexes, const int *hexes,
                           const int *hex_marker, int search_region, double empty_value, int flags,

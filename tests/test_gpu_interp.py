@@ -415,3 +415,29 @@ def test_config1_tip_solution_and_transfer_match_oracle(ctx, r3nm):
     cells_o, out_o = O.interp_points(L, tip.nodes, tip.tets, q10, tip.hexes, sol, atoms[:2000], 1)
     cells_g, out_g = ip.points(atoms[:2000], 1)
     assert np.array_equal(cells_g, cells_o) and np.array_equal(out_g, out_o)
+
+
+def test_located_chain_over_many_workgroups(ctx, r3nm):
+    """The chained guess over 70,400 points: the vertices of the first 17,600 tets of config 1's
+    tip, in tet order. Every point lies in several tets, so each result depends on the previous
+    one (vertices of bulk tets lie only in cells the search excludes). That is 275 workgroups of
+    256 points, so the scan over the workgroups runs in two chunks. Cells and values bit-exact
+    with the oracle's sequential chain, ranks 1 and 3, and with the library's own host chain."""
+    tip = r3nm
+    rng = np.random.default_rng(5)
+    sol = rng.standard_normal((len(tip.nodes), 5))
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+    ip.set_nodal(sol)
+    iph = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM, host_chain=True)
+    iph.set_nodal(sol)
+    L = O.LinTet(tip.nodes, tip.tets, tip.tet_nbr, tip.tet_marker, 3)
+    q10 = O.quadtet_cells(tip.tets, tip.hexes, tip.node_kind)
+    pts = tip.nodes[tip.tets[:17600].ravel()]
+    for rank in (1, 3):
+        cells_o, out_o = O.interp_points(L, tip.nodes, tip.tets, q10, tip.hexes, sol, pts, rank)
+        cells_g, out_g = ip.points(pts, rank)
+        assert np.array_equal(cells_g, cells_o)
+        assert np.array_equal(out_g, out_o)
+        assert (cells_o < 0).any() and (cells_o > 0).any()
+        cells_h, out_h = iph.points(pts, rank)
+        assert np.array_equal(cells_h, cells_o) and np.array_equal(out_h, out_o)
