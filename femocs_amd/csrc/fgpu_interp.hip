@@ -31,6 +31,7 @@ struct fgpu_interp {
     double *xyz = nullptr, *det0 = nullptr, *det = nullptr, *cen = nullptr;
     int *tets = nullptr, *c10 = nullptr, *hexes = nullptr, *marker = nullptr;
     int *child = nullptr, *prim = nullptr;
+    int *esc = nullptr;                     // escape links of the tet tree
     float *box = nullptr, *cbox = nullptr, *leafbox = nullptr;
     int *n2c_ptr = nullptr, *n2c = nullptr, *vor_ptr = nullptr, *vor = nullptr;
     double *sol = nullptr;                  // 5 per node: vector xyz, scalar1, scalar2
@@ -52,7 +53,7 @@ struct fgpu_interp {
     LinTriHost T;
     Bvh TB;
     double *tri_tab = nullptr, *tri_cen = nullptr;
-    int *tri_child = nullptr, *tri_prim = nullptr, *tri_nodes = nullptr;
+    int *tri_child = nullptr, *tri_prim = nullptr, *tri_nodes = nullptr, *tri_esc = nullptr;
     float *tri_box = nullptr, *tri_cbox = nullptr, *tri_leafbox = nullptr;
     // quadrangles: nodes, 6-node faces, centroids and their tree
     Bvh QB;
@@ -189,13 +190,47 @@ struct TriCells {
     __device__ bool searchable(int) const { return true; }
 };
 
+// Escape links of a tree (child pairs of ni internal nodes, leaves as ~leaf): for each node,
+// the node a depth-first walk visits next when it skips that node's subtree (the right sibling
+// of the nearest left child among the node and its ancestors), kBvhEnd after the last one.
+// esc[v] for internal node v, esc[ni + l] for leaf l.  The point queries walk the tree along
+// them without a stack.
+constexpr int kBvhEnd = INT_MAX;
+__global__ __launch_bounds__(256) void kb_parents(int ni, const int *__restrict__ child, int *__restrict__ par) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= ni) return;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int c = child[2 * (size_t)v + k];
+        par[c >= 0 ? c : ni + ~c] = 2 * v + k;      // position in the child array
+    }
+}
+__global__ __launch_bounds__(256) void kb_escape(int ni, int nl, const int *__restrict__ child, const int *__restrict__ par,
+                                                 int *__restrict__ esc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni + nl) return;
+    int pe = t == 0 ? -1 : par[t];
+    int e = kBvhEnd;
+    while (pe >= 0) {
+        if ((pe & 1) == 0) {           // a left child: its right sibling is next
+            e = child[pe + 1];
+            break;
+        }
+        const int v = pe >> 1;
+        pe = v == 0 ? -1 : par[v];
+    }
+    esc[t] = e;
+}
+
 // Per point: the lowest searchable cell containing it (x, -1 if none), the number of cells of
 // any marker containing it (y), the cell of the nearest centroid, lowest index among equal
-// distances (z, only when x < 0) and a traversal-overflow flag (w). With `lst`, also the cells
-// containing the point, ascending, in its row of kChainW entries (the first kChainW of them).
+// distances (z, only when x < 0) and a traversal-overflow flag (w, always 0: the walk along the
+// escape links needs no stack). With `lst`, also the cells containing the point, ascending, in
+// its row of kChainW entries (the first kChainW of them).
 template <class Cells>
 __global__ __launch_bounds__(256) void k_candidates(int n, const double *__restrict__ pts, int n_leaves,
-                                                    const int *__restrict__ child, const float *__restrict__ box,
+                                                    const int *__restrict__ child, const int *__restrict__ esc,
+                                                    const float *__restrict__ box,
                                                     const float *__restrict__ cbox, const float *__restrict__ leafbox,
                                                     const int *__restrict__ prim, Cells cells,
                                                     const double *__restrict__ cen, int4 *__restrict__ cand,
@@ -203,12 +238,27 @@ __global__ __launch_bounds__(256) void k_candidates(int n, const double *__restr
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double p[3] = {pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]};
-    int t0 = INT_MAX, cnt = 0, overflow = 0;
-    int stack[kBvhStack];
-    int sp = 0;
+    const int ni = n_leaves - 1;
+    int t0 = INT_MAX, cnt = 0;
     int *row = lst ? lst + (size_t)i * kChainW : nullptr;
+    // nearest centroid: strict < over increasing indices in the reference's scan, i.e. the
+    // lowest index among the minima
+    double best = 1e100;
+    int amin = 0;
+    auto consider = [&](int leaf) {
+        const int t = prim[leaf];
+        const double xx = p[0] - cen[3 * (size_t)t];
+        const double yy = p[1] - cen[3 * (size_t)t + 1];
+        const double zz = p[2] - cen[3 * (size_t)t + 2];
+        const double d2 = xx * xx + yy * yy + zz * zz;
+        if (d2 < best || (d2 == best && t < amin)) {
+            best = d2;
+            amin = t;
+        }
+    };
     auto leaf_hit = [&](int leaf) {
         if (!in_box(leafbox + 6 * (size_t)leaf, p)) return;
+        consider(leaf);                 // the cells near the point seed the nearest-centroid search
         const int t = prim[leaf];
         if (cells.contains(p, t)) {
             if (row && cnt < kChainW) {
@@ -227,65 +277,34 @@ __global__ __launch_bounds__(256) void k_candidates(int n, const double *__restr
     if (n_leaves == 1) {
         leaf_hit(0);
     } else {
-        stack[sp++] = 0;
-        while (sp > 0) {
-            const int v = stack[--sp];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int c = child[2 * (size_t)v + k];
-                if (c < 0) {
-                    leaf_hit(~c);
-                } else if (in_box(box + 6 * (size_t)c, p)) {
-                    if (sp < kBvhStack) stack[sp++] = c;
-                    else overflow = 1;
-                }
+        int ref = 0;                    // the root is always entered
+        while (ref != kBvhEnd) {
+            if (ref >= 0) {
+                ref = ref == 0 || in_box(box + 6 * (size_t)ref, p) ? child[2 * (size_t)ref] : esc[ref];
+            } else {
+                leaf_hit(~ref);
+                ref = esc[ni + ~ref];
             }
         }
     }
-    int amin = 0;
     if (t0 == INT_MAX) {
-        // nearest centroid: strict < over increasing indices in the reference's scan, i.e. the
-        // lowest index among the minima
-        double best = 1e100;
-        auto consider = [&](int leaf) {
-            const int t = prim[leaf];
-            const double xx = p[0] - cen[3 * (size_t)t];
-            const double yy = p[1] - cen[3 * (size_t)t + 1];
-            const double zz = p[2] - cen[3 * (size_t)t + 2];
-            const double d2 = xx * xx + yy * yy + zz * zz;
-            if (d2 < best || (d2 == best && t < amin)) {
-                best = d2;
-                amin = t;
-            }
-        };
         if (n_leaves == 1) {
             consider(0);
         } else {
-            sp = 0;
-            stack[sp++] = 0;
-            while (sp > 0) {
-                const int v = stack[--sp];
-                if (box_d2(cbox + 6 * (size_t)v, p) > best) continue;
-                const int c0 = child[2 * (size_t)v], c1 = child[2 * (size_t)v + 1];
-                if (c0 < 0) consider(~c0);
-                if (c1 < 0) consider(~c1);
-                const double d0 = c0 >= 0 ? box_d2(cbox + 6 * (size_t)c0, p) : 1e308;
-                const double d1 = c1 >= 0 ? box_d2(cbox + 6 * (size_t)c1, p) : 1e308;
-                // nearer child on top of the stack
-                const int far = d0 <= d1 ? c1 : c0, near = d0 <= d1 ? c0 : c1;
-                const double dfar = d0 <= d1 ? d1 : d0, dnear = d0 <= d1 ? d0 : d1;
-                if (far >= 0 && dfar <= best) {
-                    if (sp < kBvhStack) stack[sp++] = far;
-                    else overflow = 1;
-                }
-                if (near >= 0 && dnear <= best) {
-                    if (sp < kBvhStack) stack[sp++] = near;
-                    else overflow = 1;
+            // subtrees whose centroid box lies farther than the best distance so far are
+            // skipped; a lower bound, so no minimum (and no tie of lower index) is missed
+            int ref = 0;
+            while (ref != kBvhEnd) {
+                if (ref >= 0) {
+                    ref = box_d2(cbox + 6 * (size_t)ref, p) > best ? esc[ref] : child[2 * (size_t)ref];
+                } else {
+                    consider(~ref);
+                    ref = esc[ni + ~ref];
                 }
             }
         }
     }
-    cand[i] = make_int4(t0 == INT_MAX ? -1 : t0, cnt, amin, overflow);
+    cand[i] = make_int4(t0 == INT_MAX ? -1 : t0, cnt, t0 == INT_MAX ? amin : 0, 0);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -871,7 +890,7 @@ void ensure_chain(fgpu_interp *ip, size_t n) {
 void free_triangles(fgpu_interp *ip) {
     for (double **p : {&ip->tri_tab, &ip->tri_cen, &ip->quad_cen}) release(*p);
     for (int **p : {&ip->tri_child, &ip->tri_prim, &ip->tri_nodes, &ip->quad_nodes, &ip->tri6, &ip->quad_child,
-                    &ip->quad_prim})
+                    &ip->quad_prim, &ip->tri_esc})
         release(*p);
     for (float **p : {&ip->tri_box, &ip->tri_cbox, &ip->tri_leafbox, &ip->quad_cbox}) release(*p);
     ip->T = LinTriHost{};
@@ -883,7 +902,7 @@ void free_interp(fgpu_interp *ip) {
     for (double **p : {&ip->xyz, &ip->det0, &ip->det, &ip->cen, &ip->sol, &ip->pts, &ip->out}) release(*p);
     for (int **p : {&ip->tets, &ip->c10, &ip->hexes, &ip->marker, &ip->child, &ip->prim, &ip->n2c_ptr, &ip->n2c,
                     &ip->vor_ptr, &ip->vor, &ip->node_idof, &ip->tet, &ip->cells, &ip->d_region_hex, &ip->sc_icell,
-                    &ip->d_hex_cell, &ip->d_nbr_ptr, &ip->d_nbr})
+                    &ip->d_hex_cell, &ip->d_nbr_ptr, &ip->d_nbr, &ip->esc})
         release(*p);
     for (float **p : {&ip->box, &ip->cbox, &ip->leafbox}) release(*p);
     release(ip->cand);
@@ -899,8 +918,24 @@ void free_interp(fgpu_interp *ip) {
 
 void tet_candidates(fgpu_interp *ip, int n, int4 *cand, int *lst = nullptr) {
     hipLaunchKernelGGL(k_candidates<TetCells>, dim3(grid(n)), dim3(256), 0, ip->ctx->stream, n, ip->pts,
-                       ip->B.n_leaves, ip->child, ip->box, ip->cbox, ip->leafbox, ip->prim,
+                       ip->B.n_leaves, ip->child, ip->esc, ip->box, ip->cbox, ip->leafbox, ip->prim,
                        TetCells{ip->det0, ip->det, ip->marker}, ip->cen, cand, lst);
+}
+
+// the escape links of a tree of nl leaves (device child array) into a new array
+int *tree_escapes(hipStream_t s, int nl, const int *child) {
+    const int ni = std::max(nl - 1, 0);
+    int *esc = nullptr, *par = nullptr;
+    IPCHECK(hipMalloc(&esc, std::max<size_t>((size_t)ni + nl, 1) * sizeof(int)));
+    if (ni > 0) {
+        IPCHECK(hipMalloc(&par, ((size_t)ni + nl) * sizeof(int)));
+        hipLaunchKernelGGL(kb_parents, dim3(grid(ni)), dim3(256), 0, s, ni, child, par);
+        hipLaunchKernelGGL(kb_escape, dim3(grid(ni + nl)), dim3(256), 0, s, ni, nl, child, par, esc);
+        IPCHECK(hipGetLastError());
+        IPCHECK(hipStreamSynchronize(s));
+        IPCHECK(hipFree(par));
+    }
+    return esc;
 }
 
 // the chained guess over the located points (ip->pts, n of them) into ip->tet, in the
@@ -1072,6 +1107,7 @@ extern "C" int fgpu_interp_create_ex(fgpu_ctx *c, int n_nodes, const double *xyz
             ip->vor = upload(vor.data(), vor.size(), s);
             IPCHECK(hipStreamSynchronize(s));   // the host vectors are read until the copies end
         }
+        ip->esc = tree_escapes(s, ip->B.n_leaves, ip->child);
         std::vector<double> sol(5 * (size_t)n_nodes, empty_value);
         ip->sol = upload(sol.data(), sol.size(), s);
         IPCHECK(hipStreamSynchronize(s));
@@ -1589,6 +1625,7 @@ extern "C" int fgpu_interp_set_triangles(fgpu_interp *ip, int n_tris, const int 
         ip->tri_box = upload(ip->TB.box.data(), ip->TB.box.size(), s);
         ip->tri_cbox = upload(ip->TB.cbox.data(), ip->TB.cbox.size(), s);
         ip->tri_leafbox = upload(ip->TB.leafbox.data(), ip->TB.leafbox.size(), s);
+        ip->tri_esc = tree_escapes(s, ip->TB.n_leaves, ip->tri_child);
         if (quads) {
             build_bvh(3 * n_tris, ip->T.qcen, ip->T.qcen, ip->T.qcen, ip->QB);
             ip->quad_cen = upload(ip->T.qcen.data(), ip->T.qcen.size(), s);
@@ -1656,8 +1693,9 @@ static int surface_run(fgpu_interp *ip, int n, const double *xyz, int rank, int 
             tet_candidates(ip, n, ip->cand);
             if (locate)
                 hipLaunchKernelGGL(k_candidates<TriCells>, dim3(grid(n)), dim3(256), 0, st, n, ip->pts,
-                                   ip->TB.n_leaves, ip->tri_child, ip->tri_box, ip->tri_cbox, ip->tri_leafbox,
-                                   ip->tri_prim, TriCells{ip->tri_tab}, ip->tri_cen, ip->cand2, nullptr);
+                                   ip->TB.n_leaves, ip->tri_child, ip->tri_esc, ip->tri_box, ip->tri_cbox,
+                                   ip->tri_leafbox, ip->tri_prim, TriCells{ip->tri_tab}, ip->tri_cen, ip->cand2,
+                                   nullptr);
             IPCHECK(hipGetLastError());
             std::vector<int4> tc(n), fc(locate ? n : 0);
             IPCHECK(hipMemcpyAsync(tc.data(), ip->cand, n * sizeof(int4), hipMemcpyDeviceToHost, st));
