@@ -242,8 +242,9 @@ __device__ __forceinline__ void q2_wave_sync() {
 template <bool AFF>
 __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
     __shared__ double sX[AFF ? 1 : kQ2Cells][8][3];
+    // one exchange row per cell, overwritten stage after stage: each stage reads what it needs
+    // into registers, the wave syncs, then it writes (a cell's row belongs to one wave)
     __shared__ double sA[kQ2Cells][81];
-    __shared__ double sB[kQ2Cells][81];
     if (a.active_only && a.scal[S_ACTIVE] == 0.0) return;
     const int tid = threadIdx.x;
     // a cell's 9 threads lie in one wave, so the exchanges between them need no workgroup
@@ -251,7 +252,12 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
     const int lane = tid & 63, wv = tid >> 6;
     const int lc = wv * kQ2CellsPerWave + lane / 9, t = lane % 9, ta = t % 3, tb = t / 3;
     const bool inwg = lane < 9 * kQ2CellsPerWave;
-    const int c = blockIdx.x * kQ2Cells + lc;
+    // the grid is a multiple of 8 and workgroups are dispatched round-robin over the 8 XCDs:
+    // workgroup w runs cell block (w mod 8) * (grid / 8) + w / 8, so each XCD (its own L2) gets
+    // one contiguous range of cells and the dofs and slot lines they share
+    const int per = gridDim.x >> 3;
+    const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int c = blk * kQ2Cells + lc;
     const bool live = inwg && c < a.nc;
     const int cc = live ? c : 0;
     const int *cd = a.c27 + 27 * (size_t)cc;
@@ -307,6 +313,7 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
             t1[j] = sA[lc][i + 3 * j + 9 * qk];
             t1d[j] = sA[lc][27 + i + 3 * j + 9 * qk];
         }
+        q2_wave_sync();
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             double s = 0, sy = 0, sz = 0;
@@ -316,9 +323,9 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
                 sy += kQD[q][j] * t1[j];
                 sz += kQB[q][j] * t1d[j];
             }
-            sB[lc][i + 3 * q + 9 * qk] = s;
-            sB[lc][27 + i + 3 * q + 9 * qk] = sy;
-            sB[lc][54 + i + 3 * q + 9 * qk] = sz;
+            sA[lc][i + 3 * q + 9 * qk] = s;
+            sA[lc][27 + i + 3 * q + 9 * qk] = sy;
+            sA[lc][54 + i + 3 * q + 9 * qk] = sz;
         }
     }
     q2_wave_sync();
@@ -328,10 +335,11 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
         double t2[3], t2y[3], t2z[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            t2[i] = sB[lc][i + 3 * qj + 9 * qk];
-            t2y[i] = sB[lc][27 + i + 3 * qj + 9 * qk];
-            t2z[i] = sB[lc][54 + i + 3 * qj + 9 * qk];
+            t2[i] = sA[lc][i + 3 * qj + 9 * qk];
+            t2y[i] = sA[lc][27 + i + 3 * qj + 9 * qk];
+            t2z[i] = sA[lc][54 + i + 3 * qj + 9 * qk];
         }
+        q2_wave_sync();
         double hx[3], hy[3], hz[3];
 #pragma unroll
         for (int qi = 0; qi < 3; ++qi) {
@@ -393,6 +401,7 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
             s1y[q] = sA[lc][27 + i + 3 * q + 9 * qk];
             s1z[q] = sA[lc][54 + i + 3 * q + 9 * qk];
         }
+        q2_wave_sync();
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             double s = 0, sz = 0;
@@ -401,8 +410,8 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
                 s += kQB[q][j] * s1[q] + kQD[q][j] * s1y[q];
                 sz += kQB[q][j] * s1z[q];
             }
-            sB[lc][i + 3 * j + 9 * qk] = s;
-            sB[lc][27 + i + 3 * j + 9 * qk] = sz;
+            sA[lc][i + 3 * j + 9 * qk] = s;
+            sA[lc][27 + i + 3 * j + 9 * qk] = sz;
         }
     }
     q2_wave_sync();
@@ -412,8 +421,8 @@ __global__ __launch_bounds__(256) void k_q2_apply(Q2Args a) {
         double s2[3], s2z[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            s2[q] = sB[lc][i + 3 * j + 9 * q];
-            s2z[q] = sB[lc][27 + i + 3 * j + 9 * q];
+            s2[q] = sA[lc][i + 3 * j + 9 * q];
+            s2z[q] = sA[lc][27 + i + 3 * j + 9 * q];
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -710,7 +719,7 @@ void q2_apply(fgpu_q2 *s, const double *ua, const double *ub, bool maskin, bool 
     const bool aff = s->affine && s->mesh->affine_allowed;
     Q2Args a{s->n_cells, s->d_c27, s->d_spos, reinterpret_cast<const double4 *>(s->mesh->d_X64), ua, ub, s->scal,
              s->d_dmask, s->Ye, maskin ? 1 : 0, active_only ? 1 : 0, s->d_qm, s->mesh->d_layout};
-    const int grid = (s->n_cells + kQ2Cells - 1) / kQ2Cells;
+    const int grid = (s->n_cells + 8 * kQ2Cells - 1) / (8 * kQ2Cells) * 8;   // cell blocks, a multiple of 8
     if (grid <= 0) return;
     if (aff) hipLaunchKernelGGL(k_q2_apply<true>, dim3(grid), dim3(256), 0, s->ctx->stream, a);
     else hipLaunchKernelGGL(k_q2_apply<false>, dim3(grid), dim3(256), 0, s->ctx->stream, a);
