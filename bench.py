@@ -411,6 +411,28 @@ def paths_bench(no_cpu: bool):
         tm, _ = _best_of(lambda: ip.in_cells(atoms, rank, cells))
         tr[f"rank{rank}_locate_atoms_per_s"] = round(len(atoms) / tl)
         tr[f"rank{rank}_mapped_atoms_per_s"] = round(len(atoms) / tm)
+    # the same located transfer with the chained guess resolved on the host, point after point
+    # (its first call also fetches the host copy of the tet tables)
+    iph = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM, host_chain=True)
+    iph.set_nodal(ip.nodal())
+    iph.points(atoms, 1)
+    th, (cells_h, _) = _best_of(lambda: iph.points(atoms, 1), reps=2)
+    if not np.array_equal(cells_h, ip.points(atoms, 1)[0]):
+        raise RuntimeError("device and host chains differ")
+    tr["rank1_locate_host_chain_atoms_per_s"] = round(len(atoms) / th)
+    # dim 2: the surface faces (quadrangles for rank 3), then the tet or hex each one picks
+    tris, t2t = tip.surface_triangles()
+    quads = tip.surface_quads(tris)
+    ip.set_triangles(tris, t2t, quads)
+    iph.set_triangles(tris, t2t, quads)
+    for rank in (1, 3):
+        ts, (cs, vs) = _best_of(lambda: ip.surface_points(atoms, rank))
+        tsh, (csh, vsh) = _best_of(lambda: iph.surface_points(atoms, rank), reps=1)
+        if not (np.array_equal(cs, csh) and np.array_equal(vs, vsh)):
+            raise RuntimeError("device and host surface chains differ")
+        tr[f"surface_rank{rank}_locate_atoms_per_s"] = round(len(atoms) / ts)
+        tr[f"surface_rank{rank}_host_chain_atoms_per_s"] = round(len(atoms) / tsh)
+    del iph
     out["transfer"] = tr
     log("transfer", tr)
     # PIC pieces: superparticles at random natural coordinates of random solver cells

@@ -40,7 +40,7 @@ struct fgpu_interp {
     // point buffers (grown on demand)
     double *pts = nullptr, *out = nullptr;
     int4 *cand = nullptr, *cand2 = nullptr;
-    int *tet = nullptr, *cells = nullptr;
+    int *tet = nullptr, *cells = nullptr, *cells2 = nullptr;
     size_t cap = 0;
     // device chained guess (grown on demand): containing cells and tables per point, per
     // workgroup of 256 points its composition and starting position, the fallback flag
@@ -54,6 +54,8 @@ struct fgpu_interp {
     Bvh TB;
     double *tri_tab = nullptr, *tri_cen = nullptr;
     int *tri_child = nullptr, *tri_prim = nullptr, *tri_nodes = nullptr, *tri_esc = nullptr;
+    int *tri_nbr_ptr = nullptr, *tri_nbr = nullptr;   // face neighbour lists (LinearTriangles)
+    int *tri2tet = nullptr, *quad2hex = nullptr;       // the two tets of a face, hexes of a quad
     float *tri_box = nullptr, *tri_cbox = nullptr, *tri_leafbox = nullptr;
     // quadrangles: nodes, 6-node faces, centroids and their tree
     Bvh QB;
@@ -862,6 +864,7 @@ void ensure_points(fgpu_interp *ip, size_t n) {
     release(ip->cand2);
     release(ip->tet);
     release(ip->cells);
+    release(ip->cells2);
     const size_t c = std::max<size_t>(n, 1024);
     IPCHECK(hipMalloc(&ip->cand2, c * sizeof(int4)));
     IPCHECK(hipMalloc(&ip->pts, 3 * c * sizeof(double)));
@@ -869,6 +872,7 @@ void ensure_points(fgpu_interp *ip, size_t n) {
     IPCHECK(hipMalloc(&ip->cand, c * sizeof(int4)));
     IPCHECK(hipMalloc(&ip->tet, c * sizeof(int)));
     IPCHECK(hipMalloc(&ip->cells, c * sizeof(int)));
+    IPCHECK(hipMalloc(&ip->cells2, c * sizeof(int)));
     ip->cap = c;
 }
 
@@ -890,7 +894,7 @@ void ensure_chain(fgpu_interp *ip, size_t n) {
 void free_triangles(fgpu_interp *ip) {
     for (double **p : {&ip->tri_tab, &ip->tri_cen, &ip->quad_cen}) release(*p);
     for (int **p : {&ip->tri_child, &ip->tri_prim, &ip->tri_nodes, &ip->quad_nodes, &ip->tri6, &ip->quad_child,
-                    &ip->quad_prim, &ip->tri_esc})
+                    &ip->quad_prim, &ip->tri_esc, &ip->tri_nbr_ptr, &ip->tri_nbr, &ip->tri2tet, &ip->quad2hex})
         release(*p);
     for (float **p : {&ip->tri_box, &ip->tri_cbox, &ip->tri_leafbox, &ip->quad_cbox}) release(*p);
     ip->T = LinTriHost{};
@@ -901,7 +905,8 @@ void free_triangles(fgpu_interp *ip) {
 void free_interp(fgpu_interp *ip) {
     for (double **p : {&ip->xyz, &ip->det0, &ip->det, &ip->cen, &ip->sol, &ip->pts, &ip->out}) release(*p);
     for (int **p : {&ip->tets, &ip->c10, &ip->hexes, &ip->marker, &ip->child, &ip->prim, &ip->n2c_ptr, &ip->n2c,
-                    &ip->vor_ptr, &ip->vor, &ip->node_idof, &ip->tet, &ip->cells, &ip->d_region_hex, &ip->sc_icell,
+                    &ip->vor_ptr, &ip->vor, &ip->node_idof, &ip->tet, &ip->cells, &ip->cells2, &ip->d_region_hex,
+                    &ip->sc_icell,
                     &ip->d_hex_cell, &ip->d_nbr_ptr, &ip->d_nbr, &ip->esc})
         release(*p);
     for (float **p : {&ip->box, &ip->cbox, &ip->leafbox}) release(*p);
@@ -941,19 +946,26 @@ int *tree_escapes(hipStream_t s, int nl, const int *child) {
 // the chained guess over the located points (ip->pts, n of them) into ip->tet, in the
 // reference's numbering of the tets; g0 is the first point's guess. Leaves *ip->chain_flag = 1
 // when some point's candidates are incomplete (the host chain is then needed).
-void device_chain(fgpu_interp *ip, int n, int g0) {
+// the chain over candidates `cand` with their containing-cell rows in ip->chain_lst (cells
+// with neighbour lists nbr_ptr/nbr, n_cells of them) into res; the flag was cleared before
+void run_chain(fgpu_interp *ip, int n, const int4 *cand, const int *nbr_ptr, const int *nbr, int n_cells, int g0,
+               int *res) {
     hipStream_t st = ip->ctx->stream;
-    ensure_chain(ip, n);
     const int nb = (n + 255) / 256;
-    IPCHECK(hipMemsetAsync(ip->chain_flag, 0, sizeof(int), st));
-    tet_candidates(ip, n, ip->cand, ip->chain_lst);
-    hipLaunchKernelGGL(k_chain_tables, dim3(grid(n)), dim3(256), 0, st, n, ip->cand, ip->chain_lst, ip->d_nbr_ptr,
-                       ip->d_nbr, ip->n_tets, g0, ip->chain_tab, ip->chain_flag);
-    hipLaunchKernelGGL(k_chain_scan, dim3(nb), dim3(256), 0, st, n, ip->cand, ip->chain_tab, ip->chain_bt);
+    hipLaunchKernelGGL(k_chain_tables, dim3(grid(n)), dim3(256), 0, st, n, cand, ip->chain_lst, nbr_ptr, nbr, n_cells,
+                       g0, ip->chain_tab, ip->chain_flag);
+    hipLaunchKernelGGL(k_chain_scan, dim3(nb), dim3(256), 0, st, n, cand, ip->chain_tab, ip->chain_bt);
     hipLaunchKernelGGL(k_chain_top, dim3(1), dim3(256), 0, st, nb, ip->chain_bt, ip->chain_entry);
-    hipLaunchKernelGGL(k_chain_finish, dim3(grid(n)), dim3(256), 0, st, n, ip->cand, ip->chain_lst, ip->chain_tab,
-                       ip->chain_entry, ip->tet);
+    hipLaunchKernelGGL(k_chain_finish, dim3(grid(n)), dim3(256), 0, st, n, cand, ip->chain_lst, ip->chain_tab,
+                       ip->chain_entry, res);
     IPCHECK(hipGetLastError());
+}
+
+void device_chain(fgpu_interp *ip, int n, int g0) {
+    ensure_chain(ip, n);
+    IPCHECK(hipMemsetAsync(ip->chain_flag, 0, sizeof(int), ip->ctx->stream));
+    tet_candidates(ip, n, ip->cand, ip->chain_lst);
+    run_chain(ip, n, ip->cand, ip->d_nbr_ptr, ip->d_nbr, ip->n_tets, g0, ip->tet);
 }
 
 }  // namespace
@@ -1383,6 +1395,105 @@ extern "C" int fgpu_sys_add_space_charge(fgpu_sys *s, fgpu_interp *ip, int n, co
     });
 }
 
+// LinearTriangles::fast_distance (src/InterpolatorCells.cpp:1743-1747) on the 16-double rows
+__device__ inline double tri_fast_distance(const double *tab, const double *p, int f) {
+    const double *t = tab + 16 * (size_t)f;
+    const D3 tv = sub(mk(p[0], p[1], p[2]), mk(t[0], t[1], t[2]));
+    const D3 q = mk(tv.y * t[5] - tv.z * t[4], tv.z * t[3] - tv.x * t[5], tv.x * t[4] - tv.y * t[3]);
+    return t[6] * q.x + t[7] * q.y + t[8] * q.z;
+}
+// LinearTetrahedra::locate_cell from guess g (src/InterpolatorCells.cpp:268-307) with the
+// point's tree candidates standing in for the linear scan
+__device__ inline int tet_locate_from(const double *det0, const double *det, const int *nbr_ptr, const int *nbr,
+                                      int n_tets, const double *p, int g, int4 c) {
+    if (g >= 0 && g < n_tets) {
+        if (pic(det0, det, p, g)) return g;
+        for (int e = nbr_ptr[g]; e < nbr_ptr[g + 1]; ++e)
+            if (pic(det0, det, p, nbr[e])) return nbr[e];
+    }
+    return c.x >= 0 ? c.x : -c.z;
+}
+
+// The cells the surface transfer interpolates in (SolutionReader::calc_interpolation for dim 2,
+// src/SolutionReader.cpp:136-190): per point its face (rank 1, 2) or quadrangle (rank 3) in
+// cells[] (located: the chained face in, the quadrangle around the face's largest barycentric
+// coordinate out, LinearQuadrangles::locate_cell, src/InterpolatorCells.cpp:1969-1990, with
+// LinearTriangles::shape_functions, :1652-1660), then the tet the face picks
+// (LinearTriangles::interp_solution, :1662-1686: the face's first tet when the point lies on its
+// plane or inside that tet, else the search from its second tet) or the hex the quadrangle picks
+// (LinearQuadrangles::interp_solution, :1922-1940: the first hex when on the plane or when it
+// holds the point, else LinearHexahedra::locate_cell from the second).  flag |= 1 when a
+// quadrangle pick has no largest coordinate (NaN weights): the host then resolves the call.
+__global__ __launch_bounds__(256) void k_surface_pick(int n, const double *__restrict__ pts, int rank, int locate,
+                                                      const double *__restrict__ tab, const int *__restrict__ tri2tet,
+                                                      const int *__restrict__ quad2hex, const int4 *__restrict__ tcand,
+                                                      const double *__restrict__ det0, const double *__restrict__ det,
+                                                      const int *__restrict__ nbr_ptr, const int *__restrict__ nbr,
+                                                      int n_tets, int *__restrict__ cells, int *__restrict__ use,
+                                                      int *__restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double p[3] = {pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]};
+    int r = cells[i];
+    if (locate && rank == 3) {
+        // LinearTriangles::shape_functions of the face, the quadrangle around its largest weight
+        const int sign = r < 0 ? -1 : 1, f = abs(r);
+        const double *t = tab + 16 * (size_t)f;
+        const D3 tv = sub(mk(p[0], p[1], p[2]), mk(t[0], t[1], t[2]));
+        const D3 q = mk(tv.y * t[5] - tv.z * t[4], tv.z * t[3] - tv.x * t[5], tv.x * t[4] - tv.y * t[3]);
+        const double v = tv.x * t[9] + tv.y * t[10] + tv.z * t[11];
+        const double ww = q.x * t[12] + q.y * t[13] + q.z * t[14];
+        const double u = 1.0 - v - ww;
+        const double b0 = kZero + u, b1 = kZero + v, b2 = kZero + ww;
+        if (b0 >= b1 && b0 >= b2) r = sign * (3 * f);
+        else if (b1 >= b0 && b1 >= b2) r = sign * (3 * f + 1);
+        else if (b2 >= b0 && b2 >= b1) r = sign * (3 * f + 2);
+        else {
+            r = -1;
+            atomicOr(flag, 1);
+        }
+        cells[i] = r;
+    }
+    const int4 tc = tcand[i];
+    int out;
+    if (rank == 3) {
+        const int qd = abs(r);
+        const int h0 = quad2hex[2 * (size_t)qd], h1 = quad2hex[2 * (size_t)qd + 1];
+        out = INT_MIN;
+        if (fabs(tri_fast_distance(tab, p, qd / 3)) <= 100.0 * kZero) out = h0;
+        double b[4];
+        if (out == INT_MIN) {
+            bcc(det0, det, p, h0 / 4, b);
+            if (b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[3] >= 0) {
+                const int k = h0 % 4;
+                bool top = true;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (j != k && !(b[k] >= b[j])) top = false;
+                if (top) out = h0;
+            }
+        }
+        if (out == INT_MIN) {
+            int tt = tet_locate_from(det0, det, nbr_ptr, nbr, n_tets, p, h1 / 4, tc);
+            const int sign = tt < 0 ? -1 : 1;
+            tt = abs(tt);
+            bcc(det0, det, p, tt, b);
+            if (b[0] >= b[1] && b[0] >= b[2] && b[0] >= b[3]) out = sign * (4 * tt);
+            else if (b[1] >= b[0] && b[1] >= b[2] && b[1] >= b[3]) out = sign * (4 * tt + 1);
+            else if (b[2] >= b[0] && b[2] >= b[1] && b[2] >= b[3]) out = sign * (4 * tt + 2);
+            else if (b[3] >= b[0] && b[3] >= b[1] && b[3] >= b[2]) out = sign * (4 * tt + 3);
+            else out = -1;
+        }
+    } else {
+        const int f = abs(r);
+        const int t0 = tri2tet[2 * (size_t)f], t1 = tri2tet[2 * (size_t)f + 1];
+        if (fabs(tri_fast_distance(tab, p, f)) <= 100.0 * kZero) out = t0;
+        else if (pic(det0, det, p, t0)) out = t0;
+        else out = tet_locate_from(det0, det, nbr_ptr, nbr, n_tets, p, t1, tc);
+    }
+    use[i] = out;
+}
+
 // Pic::update_point_cell (src/Pic.cpp:185-195): each particle located from its own previous
 // cell, LinearHexahedra::locate_cell (src/InterpolatorCells.cpp:1536-1562) over
 // LinearTetrahedra::locate_cell (:269-307): the guess tet, its neighbours in order, else the
@@ -1626,6 +1737,10 @@ extern "C" int fgpu_interp_set_triangles(fgpu_interp *ip, int n_tris, const int 
         ip->tri_cbox = upload(ip->TB.cbox.data(), ip->TB.cbox.size(), s);
         ip->tri_leafbox = upload(ip->TB.leafbox.data(), ip->TB.leafbox.size(), s);
         ip->tri_esc = tree_escapes(s, ip->TB.n_leaves, ip->tri_child);
+        ip->tri_nbr_ptr = upload(ip->T.nbr_ptr.data(), ip->T.nbr_ptr.size(), s);
+        ip->tri_nbr = upload(ip->T.nbr.data(), ip->T.nbr.size(), s);
+        ip->tri2tet = upload(ip->T.tri2tet.data(), ip->T.tri2tet.size(), s);
+        if (quads) ip->quad2hex = upload(ip->T.quad2hex.data(), ip->T.quad2hex.size(), s);
         if (quads) {
             build_bvh(3 * n_tris, ip->T.qcen, ip->T.qcen, ip->T.qcen, ip->QB);
             ip->quad_cen = upload(ip->T.qcen.data(), ip->T.qcen.size(), s);
@@ -1638,6 +1753,37 @@ extern "C" int fgpu_interp_set_triangles(fgpu_interp *ip, int n_tris, const int 
         IPCHECK(hipStreamSynchronize(s));
         return FGPU_OK;
     });
+}
+
+// the face (or quadrangle) chain, the cell picks and the interpolation on the device, out and
+// (located) cells copied back; false, with nothing written, when the host has to resolve it
+static bool device_surface(fgpu_interp *ip, int n, int *cells, double *out, int rank, bool locate) {
+    hipStream_t st = ip->ctx->stream;
+    ensure_chain(ip, n);
+    IPCHECK(hipMemsetAsync(ip->chain_flag, 0, sizeof(int), st));
+    tet_candidates(ip, n, ip->cand);
+    if (locate) {
+        hipLaunchKernelGGL(k_candidates<TriCells>, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, ip->TB.n_leaves,
+                           ip->tri_child, ip->tri_esc, ip->tri_box, ip->tri_cbox, ip->tri_leafbox, ip->tri_prim,
+                           TriCells{ip->tri_tab}, ip->tri_cen, ip->cand2, ip->chain_lst);
+        // the first guess: |-1| as a face, or |-1| / 3 from the quadrangle numbering
+        run_chain(ip, n, ip->cand2, ip->tri_nbr_ptr, ip->tri_nbr, ip->T.n, rank == 3 ? 0 : 1, ip->cells2);
+    } else {
+        IPCHECK(hipMemcpyAsync(ip->cells2, cells, n * sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    hipLaunchKernelGGL(k_surface_pick, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, rank, locate ? 1 : 0, ip->tri_tab,
+                       ip->tri2tet, ip->quad2hex, ip->cand, ip->det0, ip->det, ip->d_nbr_ptr, ip->d_nbr, ip->n_tets,
+                       ip->cells2, ip->tet, ip->chain_flag);
+    hipLaunchKernelGGL(k_interp, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, rank, 0, ip->tet, ip->det0, ip->det,
+                       ip->tets, ip->c10, ip->hexes, ip->xyz, ip->sol, ip->cells, ip->out);
+    IPCHECK(hipGetLastError());
+    // (when the flag is set the host path overwrites cells and out again)
+    int flag = 0;
+    IPCHECK(hipMemcpyAsync(&flag, ip->chain_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (locate) IPCHECK(hipMemcpyAsync(cells, ip->cells2, n * sizeof(int), hipMemcpyDeviceToHost, st));
+    IPCHECK(hipMemcpyAsync(out, ip->out, 5 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+    IPCHECK(hipStreamSynchronize(st));
+    return flag == 0;
 }
 
 // SolutionReader::calc_full_interpolation / calc_interpolation for dim 2 (src/SolutionReader.cpp:
@@ -1689,7 +1835,11 @@ static int surface_run(fgpu_interp *ip, int n, const double *xyz, int rank, int 
             const int *nodes = rank == 3 ? ip->quad_nodes : (rank == 2 ? ip->tri6 : ip->tri_nodes);
             hipLaunchKernelGGL(k_interp_nodes, dim3(grid(n)), dim3(256), 0, st, n, ip->tet, nodes, nn, 1.0 / nn,
                                ip->sol, ip->out);
+        } else if (!ip->host_chain && device_surface(ip, n, cells, out, rank, locate)) {
+            return FGPU_OK;
         } else {
+            // the chain and the cell picks on the host (FGPU_INTERP_HOST_CHAIN, or a point in
+            // more than kChainW - 1 faces, or a quadrangle pick with NaN weights)
             tet_candidates(ip, n, ip->cand);
             if (locate)
                 hipLaunchKernelGGL(k_candidates<TriCells>, dim3(grid(n)), dim3(256), 0, st, n, ip->pts,

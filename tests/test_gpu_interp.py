@@ -209,6 +209,29 @@ def test_surface_faces_bit_exact(tip, surface, rank):
     assert np.array_equal(r.calc_interpolation(pts), out_o)
 
 
+def test_surface_chain_over_many_workgroups(ctx, tip, surface):
+    """The face chain resolved on the device: the vertices of every surface face in face order,
+    then in reverse order, then in face order again (each lies in several faces, so each result
+    depends on the previous one), over 20 workgroups of 256 points; faces, quadrangles and values
+    bit-exact with the oracle's sequential chain and with the library's own host chain, ranks 1
+    and 3."""
+    ip, T, L, kw, tris = surface
+    sol = ip.nodal()
+    _, t2t = tip.surface_triangles()
+    iph = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM, host_chain=True)
+    iph.set_nodal(sol)
+    iph.set_triangles(tris, t2t, tip.surface_quads(tris))
+    v = tris.ravel()
+    pts = tip.nodes[np.concatenate([v, v[::-1], v])]
+    assert len(pts) > 20 * 256
+    for rank in (1, 3):
+        cells_o, out_o = T.interp(L, tip.tets, sol, pts, rank=rank, **kw)
+        cells_g, out_g = ip.surface_points(pts, rank)
+        assert np.array_equal(cells_g, cells_o) and np.array_equal(out_g, out_o)
+        cells_h, out_h = iph.surface_points(pts, rank)
+        assert np.array_equal(cells_h, cells_o) and np.array_equal(out_h, out_o)
+
+
 @pytest.mark.parametrize("rank", [1, 2, 3])
 def test_surface_centroids_bit_exact(tip, surface, rank):
     """interp_centroids: exact centroid matches (within 1e-9), misses keep the previous cell."""
