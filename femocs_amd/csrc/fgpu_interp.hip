@@ -49,7 +49,7 @@ struct fgpu_interp {
     size_t chain_cap = 0;
     bool host_chain = false;                // FGPU_INTERP_HOST_CHAIN
     // surface faces (LinearTriangles), set by fgpu_interp_set_triangles
-    std::vector<double> hxyz;
+    std::vector<double> hxyz;               // host copy of the coordinates, fetched for the faces
     LinTriHost T;
     Bvh TB;
     double *tri_tab = nullptr, *tri_cen = nullptr;
@@ -1010,7 +1010,6 @@ extern "C" int fgpu_interp_create_ex(fgpu_ctx *c, int n_nodes, const double *xyz
         int tn = 0;
         for (size_t i = 0; i < 4 * (size_t)n_tets; ++i) tn = std::max(tn, tets[i] + 1);
         ip->n_tetnodes = tn;
-        ip->hxyz.assign(xyz, xyz + 3 * (size_t)n_nodes);
         ip->hkind.assign(node_kind, node_kind + n_nodes);
         ip->htets.assign(tets, tets + 4 * (size_t)n_tets);
         ip->xyz = upload(xyz, 3 * (size_t)n_nodes, s);
@@ -1120,8 +1119,10 @@ extern "C" int fgpu_interp_create_ex(fgpu_ctx *c, int n_nodes, const double *xyz
             IPCHECK(hipStreamSynchronize(s));   // the host vectors are read until the copies end
         }
         ip->esc = tree_escapes(s, ip->B.n_leaves, ip->child);
-        std::vector<double> sol(5 * (size_t)n_nodes, empty_value);
-        ip->sol = upload(sol.data(), sol.size(), s);
+        // the nodal solution starts at the empty value everywhere (filled on the device)
+        IPCHECK(hipMalloc(&ip->sol, 5 * (size_t)n_nodes * sizeof(double)));
+        launch_set_const(s, 5 * n_nodes, empty_value, ip->sol);
+        IPCHECK(hipGetLastError());
         IPCHECK(hipStreamSynchronize(s));
         return FGPU_OK;
     });
@@ -1707,6 +1708,10 @@ extern "C" int fgpu_interp_set_triangles(fgpu_interp *ip, int n_tris, const int 
         IPCHECK(hipStreamSynchronize(s));
         free_triangles(ip);
         if (n_tris == 0) return FGPU_OK;
+        if (ip->hxyz.empty()) {
+            ip->hxyz.resize(3 * (size_t)ip->n_nodes);
+            IPCHECK(hipMemcpy(ip->hxyz.data(), ip->xyz, ip->hxyz.size() * sizeof(double), hipMemcpyDeviceToHost));
+        }
         LinTriHost T;
         build_lintri(T, ip->n_nodes, ip->hxyz.data(), n_tris, tris, tri2tet);
         if (quads) set_quads(T, ip->hxyz.data(), ip->hkind.data(), ip->htets.data(), quads);
