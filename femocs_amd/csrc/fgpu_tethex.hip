@@ -24,11 +24,20 @@
 #include "femocs_gpu.h"
 #include "fgpu_internal.h"
 
+// the split's results stay on the device until fgpu_tethex_export copies them out
 struct fgpu_tethex {
+    int device = 0;
     int nv = 0, ne = 0, nf = 0, nt = 0, ntri = 0;
-    std::vector<double> xyz;      // 3 per node
-    std::vector<int> hexes;       // 8 per hex
-    std::vector<int> quads;       // 4 per quadrangle
+    double *xyz = nullptr;        // 3 per node
+    int *hexes = nullptr;         // 8 per hex
+    int *quads = nullptr;         // 4 per quadrangle
+    ~fgpu_tethex() {
+        if (!xyz && !hexes && !quads) return;
+        (void)hipSetDevice(device);
+        if (xyz) (void)hipFree(xyz);
+        if (hexes) (void)hipFree(hexes);
+        if (quads) (void)hipFree(quads);
+    }
 };
 
 namespace {
@@ -324,6 +333,9 @@ extern "C" int fgpu_tethex_split(fgpu_ctx *c, int nv, const double *xyz, int nt,
         }
         TXCHECK(hipSetDevice(c->device));
         hipStream_t st = c->stream;
+        fgpu_tethex *h = new fgpu_tethex();
+        std::unique_ptr<fgpu_tethex> guard_h(h);
+        h->device = c->device;
         // about 0.9 KB per tet and 32 bytes per vertex (sort buffers included)
         Bufs B(960 * (size_t)nt + 32 * (size_t)nv + (1 << 20));
         const int n6 = 6 * nt, n4 = 4 * nt;
@@ -377,15 +389,18 @@ extern "C" int fgpu_tethex_split(fgpu_ctx *c, int nv, const double *xyz, int nt,
         hipLaunchKernelGGL(k_face_number, dim3(grid(nf)), dim3(256), 0, st, nf, forder, fnum);
         // new nodes and the cells
         const size_t n_nodes = (size_t)nv + ne + nf + nt;
-        double *all = B.get<double>(3 * n_nodes);
+        TXCHECK(hipMalloc(&h->xyz, 3 * n_nodes * sizeof(double)));
+        TXCHECK(hipMalloc(&h->hexes, 8 * (size_t)n4 * sizeof(int)));
+        TXCHECK(hipMalloc(&h->quads, 12 * (size_t)std::max(1, ntri) * sizeof(int)));
+        double *all = h->xyz;
         TXCHECK(hipMemcpyAsync(all, d_xyz, 3 * (size_t)nv * sizeof(double), hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_edge_nodes, dim3(grid(ne)), dim3(256), 0, st, ne, euk, d_xyz, all + 3 * (size_t)nv);
         hipLaunchKernelGGL(k_face_nodes, dim3(grid(nf)), dim3(256), 0, st, nf, fuk, fnum, d_xyz,
                            all + 3 * ((size_t)nv + ne));
-        int *d_hex = B.get<int>(8 * (size_t)n4);
+        int *d_hex = h->hexes;
         hipLaunchKernelGGL(k_tet_hexes, dim3(grid(nt)), dim3(256), 0, st, nt, nv, ne, nf, d_tets, euk, fuk, fnum, all,
                            all + 3 * ((size_t)nv + ne + nf), d_hex);
-        int *d_quad = B.get<int>(12 * (size_t)std::max(1, ntri));
+        int *d_quad = h->quads;
         int *d_bad = B.get<int>(1);
         TXCHECK(hipMemsetAsync(d_bad, 0, sizeof(int), st));
         if (ntri > 0) {
@@ -395,21 +410,12 @@ extern "C" int fgpu_tethex_split(fgpu_ctx *c, int nv, const double *xyz, int nt,
                                all, d_quad, d_bad);
         }
         TXCHECK(hipGetLastError());
-        fgpu_tethex *h = new fgpu_tethex();
-        std::unique_ptr<fgpu_tethex> guard_h(h);
         h->nv = nv;
         h->ne = ne;
         h->nf = nf;
         h->nt = nt;
         h->ntri = ntri;
-        h->xyz.resize(3 * n_nodes);
-        h->hexes.resize(8 * (size_t)n4);
-        h->quads.resize(12 * (size_t)ntri);
         int bad = 0;
-        TXCHECK(hipMemcpyAsync(h->xyz.data(), all, h->xyz.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-        TXCHECK(hipMemcpyAsync(h->hexes.data(), d_hex, h->hexes.size() * sizeof(int), hipMemcpyDeviceToHost, st));
-        if (ntri > 0)
-            TXCHECK(hipMemcpyAsync(h->quads.data(), d_quad, h->quads.size() * sizeof(int), hipMemcpyDeviceToHost, st));
         TXCHECK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
         TXCHECK(hipStreamSynchronize(st));
         if (bad) throw std::invalid_argument("tet split: a boundary triangle is not a face of the tets");
@@ -428,9 +434,15 @@ extern "C" int fgpu_tethex_sizes(const fgpu_tethex *h, int *n_nodes, int *n_edge
 
 extern "C" int fgpu_tethex_export(const fgpu_tethex *h, double *xyz, int *hexes, int *quads) {
     if (!h) return FGPU_EINVAL;
-    if (xyz) std::copy(h->xyz.begin(), h->xyz.end(), xyz);
-    if (hexes) std::copy(h->hexes.begin(), h->hexes.end(), hexes);
-    if (quads) std::copy(h->quads.begin(), h->quads.end(), quads);
+    const size_t n_nodes = (size_t)h->nv + h->ne + h->nf + h->nt;
+    if (hipSetDevice(h->device) != hipSuccess) return FGPU_EHIP;
+    if (xyz && hipMemcpy(xyz, h->xyz, 3 * n_nodes * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return FGPU_EHIP;
+    if (hexes && hipMemcpy(hexes, h->hexes, 32 * (size_t)h->nt * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return FGPU_EHIP;
+    if (quads && h->ntri > 0 &&
+        hipMemcpy(quads, h->quads, 12 * (size_t)h->ntri * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return FGPU_EHIP;
     return FGPU_OK;
 }
 

@@ -660,9 +660,10 @@ def tethex_split(ctx: "Context", nodes, tets, tris=None):
     try:
         nn, ne, nf = C.c_int(), C.c_int(), C.c_int()
         ctx.check(ctx.lib.fgpu_tethex_sizes(h, C.byref(nn), C.byref(ne), C.byref(nf)), "fgpu_tethex_sizes")
-        xyz = np.zeros((nn.value, 3))
-        hexes = np.zeros((t.size, 8), np.int32)
-        quads = None if tr is None else np.zeros((tr.size, 4), np.int32)
+        # every entry is written by the export
+        xyz = np.empty((nn.value, 3))
+        hexes = np.empty((t.size, 8), np.int32)
+        quads = None if tr is None else np.empty((tr.size, 4), np.int32)
         ctx.check(ctx.lib.fgpu_tethex_export(h, xyz.ctypes.data, hexes.ctypes.data,
                                              None if quads is None else quads.ctypes.data), "fgpu_tethex_export")
     finally:

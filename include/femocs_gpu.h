@@ -407,7 +407,8 @@ int fgpu_sys_time_kernel(fgpu_sys *sys, int which, int precision, int reps, doub
  * quadrangles per boundary triangle (Mesh::convert_triangles :1104-1169).  tets: 4 per tet,
  * tris: 3 per boundary triangle (faces of the tets; may be 0).  At most 2097151 vertices.
  * fgpu_tethex_export writes 3 * n_nodes coordinates, 32 * n_tets hex vertices and
- * 12 * n_tris quadrangle vertices (any pointer may be NULL). */
+ * 12 * n_tris quadrangle vertices (any pointer may be NULL); the results stay in device memory
+ * (about 230 bytes per tet) until fgpu_tethex_destroy. */
 int fgpu_tethex_split(fgpu_ctx *ctx, int n_vertices, const double *xyz, int n_tets, const int *tets, int n_tris,
                       const int *tris, fgpu_tethex **out);
 int fgpu_tethex_sizes(const fgpu_tethex *split, int *n_nodes, int *n_edges, int *n_faces);
