@@ -160,6 +160,31 @@ def test_empty_and_invalid_calls(ctx, tip):
         ip.in_cells(np.zeros((1, 3)), 1, np.array([len(tip.tets) + 5]))
 
 
+def test_located_chain_small_and_boundary_sizes(ctx, tip, solved):
+    """The device chain at the sizes where its workgroups start and end: one point of each kind
+    (interior, on a node, in the bulk, outside) alone, then 255, 256, 257 and 512 points of
+    mesh vertices in tet order (every result depends on the previous one); cells and values
+    bit-exact with the oracle and with the library's host chain, ranks 1 and 3."""
+    _, s, _, _ = solved
+    ip = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM)
+    ip.extract_solution(s, smoothen=False)
+    sol = ip.nodal()
+    iph = interp.Interpolator(ctx, tip, interp.TYPES_VACUUM, host_chain=True)
+    iph.set_nodal(sol)
+    L = O.LinTet(tip.nodes, tip.tets, tip.tet_nbr, tip.tet_marker, 3)
+    q10 = O.quadtet_cells(tip.tets, tip.hexes, tip.node_kind)
+    singles = _points(tip, 8, 4)[[0, 8, 10, 12]]       # even seed: kinds in order
+    verts = tip.nodes[tip.tets.ravel()]
+    sets = [singles[k:k + 1] for k in range(len(singles))] + [verts[:m] for m in (255, 256, 257, 512)]
+    for pts in sets:
+        for rank in (1, 3):
+            cells_o, out_o = O.interp_points(L, tip.nodes, tip.tets, q10, tip.hexes, sol, pts, rank)
+            cells_g, out_g = ip.points(pts, rank)
+            cells_h, out_h = iph.points(pts, rank)
+            assert np.array_equal(cells_g, cells_o) and np.array_equal(out_g, out_o)
+            assert np.array_equal(cells_h, cells_o) and np.array_equal(out_h, out_o)
+
+
 def _surface_points(tip, tris, n, seed):
     """Atoms on, just above and below the surface faces, on surface nodes and far away."""
     rng = np.random.default_rng(seed)
