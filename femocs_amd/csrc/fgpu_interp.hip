@@ -48,6 +48,11 @@ struct fgpu_interp {
     unsigned char *chain_tab = nullptr, *chain_bt = nullptr;
     size_t chain_cap = 0;
     bool host_chain = false;                // FGPU_INTERP_HOST_CHAIN
+    // Morton order of the query points (grown on demand): keys, point indices, sort storage
+    unsigned long long *skey = nullptr, *skey2 = nullptr;
+    int *sidx = nullptr, *sperm = nullptr;
+    void *stmp = nullptr;
+    size_t sort_cap = 0, stmp_bytes = 0;
     // surface faces (LinearTriangles), set by fgpu_interp_set_triangles
     std::vector<double> hxyz;               // host copy of the coordinates, fetched for the faces
     LinTriHost T;
@@ -236,9 +241,12 @@ __global__ __launch_bounds__(256) void k_candidates(int n, const double *__restr
                                                     const float *__restrict__ cbox, const float *__restrict__ leafbox,
                                                     const int *__restrict__ prim, Cells cells,
                                                     const double *__restrict__ cen, int4 *__restrict__ cand,
-                                                    int *__restrict__ lst) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+                                                    int *__restrict__ lst, const int *__restrict__ perm) {
+    // with `perm`, thread s takes point perm[s] (points in Morton order: a wave's points lie
+    // close together and walk the tree alike); results go to the point's own index
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int i = perm ? perm[s] : s;
     const double p[3] = {pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]};
     const int ni = n_leaves - 1;
     int t0 = INT_MAX, cnt = 0;
@@ -916,15 +924,67 @@ void free_interp(fgpu_interp *ip) {
     release(ip->chain_tab);
     release(ip->chain_bt);
     ip->chain_cap = 0;
+    for (unsigned long long **p : {&ip->skey, &ip->skey2}) release(*p);
+    for (int **p : {&ip->sidx, &ip->sperm}) release(*p);
+    release(ip->stmp);
+    ip->sort_cap = ip->stmp_bytes = 0;
     free_triangles(ip);
 }
 
 #include "fgpu_interp_build.inc"
 
+// Morton code of each query point in the box `root` (a tree's root box, 6 floats on the
+// device; points outside it are clamped to its faces)
+__global__ __launch_bounds__(256) void k_point_codes(int n, const double *__restrict__ pts, const float *__restrict__ root,
+                                                     unsigned long long *__restrict__ key, int *__restrict__ idx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long m = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double lo = root[a], hi = root[3 + a];
+        const double t = hi > lo ? (pts[3 * (size_t)i + a] - lo) / (hi - lo) : 0.0;
+        const unsigned long long q = (unsigned long long)fmin(2097151.0, fmax(0.0, t * 2097151.0));
+        m |= spread21_d(q) << (2 - a);
+    }
+    key[i] = m;
+    idx[i] = i;
+}
+
+// the order in which the point searches take the n points of ip->pts (their Morton order in the
+// tree's root box), or nullptr where sorting does not pay (few points, a one-leaf tree)
+const int *point_order(fgpu_interp *ip, int n, int n_leaves, const float *root_box) {
+    if (n < 2048 || n_leaves < 2) return nullptr;
+    hipStream_t st = ip->ctx->stream;
+    if ((size_t)n > ip->sort_cap) {
+        IPCHECK(hipStreamSynchronize(st));
+        for (unsigned long long **p : {&ip->skey, &ip->skey2}) release(*p);
+        for (int **p : {&ip->sidx, &ip->sperm}) release(*p);
+        const size_t c = std::max<size_t>(n, 1024);
+        IPCHECK(hipMalloc(&ip->skey, c * sizeof(unsigned long long)));
+        IPCHECK(hipMalloc(&ip->skey2, c * sizeof(unsigned long long)));
+        IPCHECK(hipMalloc(&ip->sidx, c * sizeof(int)));
+        IPCHECK(hipMalloc(&ip->sperm, c * sizeof(int)));
+        ip->sort_cap = c;
+    }
+    size_t bytes = 0;
+    IPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ip->skey, ip->skey2, ip->sidx, ip->sperm, n, 0, 63, st));
+    if (bytes > ip->stmp_bytes) {
+        IPCHECK(hipStreamSynchronize(st));
+        release(ip->stmp);
+        IPCHECK(hipMalloc(&ip->stmp, bytes));
+        ip->stmp_bytes = bytes;
+    }
+    hipLaunchKernelGGL(k_point_codes, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, root_box, ip->skey, ip->sidx);
+    IPCHECK(hipcub::DeviceRadixSort::SortPairs(ip->stmp, bytes, ip->skey, ip->skey2, ip->sidx, ip->sperm, n, 0, 63, st));
+    return ip->sperm;
+}
+
 void tet_candidates(fgpu_interp *ip, int n, int4 *cand, int *lst = nullptr) {
+    const int *perm = point_order(ip, n, ip->B.n_leaves, ip->box);
     hipLaunchKernelGGL(k_candidates<TetCells>, dim3(grid(n)), dim3(256), 0, ip->ctx->stream, n, ip->pts,
                        ip->B.n_leaves, ip->child, ip->esc, ip->box, ip->cbox, ip->leafbox, ip->prim,
-                       TetCells{ip->det0, ip->det, ip->marker}, ip->cen, cand, lst);
+                       TetCells{ip->det0, ip->det, ip->marker}, ip->cen, cand, lst, perm);
 }
 
 // the escape links of a tree of nl leaves (device child array) into a new array
@@ -1768,9 +1828,10 @@ static bool device_surface(fgpu_interp *ip, int n, int *cells, double *out, int 
     IPCHECK(hipMemsetAsync(ip->chain_flag, 0, sizeof(int), st));
     tet_candidates(ip, n, ip->cand);
     if (locate) {
+        const int *perm = point_order(ip, n, ip->TB.n_leaves, ip->tri_box);
         hipLaunchKernelGGL(k_candidates<TriCells>, dim3(grid(n)), dim3(256), 0, st, n, ip->pts, ip->TB.n_leaves,
                            ip->tri_child, ip->tri_esc, ip->tri_box, ip->tri_cbox, ip->tri_leafbox, ip->tri_prim,
-                           TriCells{ip->tri_tab}, ip->tri_cen, ip->cand2, ip->chain_lst);
+                           TriCells{ip->tri_tab}, ip->tri_cen, ip->cand2, ip->chain_lst, perm);
         // the first guess: |-1| as a face, or |-1| / 3 from the quadrangle numbering
         run_chain(ip, n, ip->cand2, ip->tri_nbr_ptr, ip->tri_nbr, ip->T.n, rank == 3 ? 0 : 1, ip->cells2);
     } else {
@@ -1850,7 +1911,7 @@ static int surface_run(fgpu_interp *ip, int n, const double *xyz, int rank, int 
                 hipLaunchKernelGGL(k_candidates<TriCells>, dim3(grid(n)), dim3(256), 0, st, n, ip->pts,
                                    ip->TB.n_leaves, ip->tri_child, ip->tri_esc, ip->tri_box, ip->tri_cbox,
                                    ip->tri_leafbox, ip->tri_prim, TriCells{ip->tri_tab}, ip->tri_cen, ip->cand2,
-                                   nullptr);
+                                   nullptr, nullptr);
             IPCHECK(hipGetLastError());
             std::vector<int4> tc(n), fc(locate ? n : 0);
             IPCHECK(hipMemcpyAsync(tc.data(), ip->cand, n * sizeof(int4), hipMemcpyDeviceToHost, st));
