@@ -684,7 +684,7 @@ def batched_bench(steps: int):
 
 
 def q2_bench(steps: int):
-    """Config 2's tip (r = 10 nm, h = 60 nm on a 120 x 120 nm slab) with Q2 elements: the FP64
+    """The r = 10 nm tip (h = 60 nm on a 120 x 120 nm slab) with Q2 elements: the FP64
     Laplace solve to 1e-9 (Chebyshev-Jacobi and Jacobi), the Q1 solve of the same mesh beside
     it, and the roofline of the Q2 element apply (HIP events over `reps` launches).  Algorithmic
     bytes per apply: the 27 int32 dofs of every cell, the vertex coordinates (3 x 8 B), the two

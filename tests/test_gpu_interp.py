@@ -417,17 +417,17 @@ def test_particle_cell_update_bit_exact(ctx, tip):
 
 @pytest.fixture(scope="module")
 def r3nm():
-    """Config 1's own tip (r = 3 nm, h = 15 nm on a 30 x 30 x 3 nm slab)."""
+    """The r = 3 nm tip (h = 15 nm on a 30 x 30 x 3 nm slab)."""
     tip = M.make_tip_mesh(M.tip_r3nm_geometry())
     print(f"r3nm tip: {len(tip.vacuum_cells())} vacuum hexes, {len(tip.tets)} tets", flush=True)
     return tip
 
 
-def test_config1_tip_solution_and_transfer_match_oracle(ctx, r3nm):
-    """On config 1's tip: the GPU Laplace solution within 1e-10 of the oracle's, its nodal
+def test_r3nm_tip_solution_and_transfer_match_oracle(ctx, r3nm):
+    """On the r = 3 nm tip: the GPU Laplace solution within 1e-10 of the oracle's, its nodal
     solution bit-exact with the oracle's extraction of the same potential, and located cells and
     values bit-exact for ranks 1-3 (the linear interpolation of E to the surface atoms is
-    config 1's transfer)."""
+    the field transfer of this workload)."""
     tip = r3nm
     gm = fgpu.Mesh(ctx, tip.nodes, tip.vacuum_cells(), *MARKS)
     s = fgpu.System(gm)
@@ -458,7 +458,7 @@ def test_config1_tip_solution_and_transfer_match_oracle(ctx, r3nm):
         cells_g, out_g = ip.points(pts, rank)
         assert np.array_equal(cells_g, cells_o)
         assert np.array_equal(out_g, out_o)
-    # the surface atoms (vertices of the tets): rank 1, as config 1 transfers E
+    # the surface atoms (vertices of the tets): rank 1, as the field is transferred to them
     atoms = tip.nodes[: tip.n_surface_nodes]
     cells_o, out_o = O.interp_points(L, tip.nodes, tip.tets, q10, tip.hexes, sol, atoms[:2000], 1)
     cells_g, out_g = ip.points(atoms[:2000], 1)
@@ -466,7 +466,7 @@ def test_config1_tip_solution_and_transfer_match_oracle(ctx, r3nm):
 
 
 def test_located_chain_over_many_workgroups(ctx, r3nm):
-    """The chained guess over 70,400 points: the vertices of the first 17,600 tets of config 1's
+    """The chained guess over 70,400 points: the vertices of the first 17,600 tets of the r = 3 nm
     tip, in tet order. Every point lies in several tets, so each result depends on the previous
     one (vertices of bulk tets lie only in cells the search excludes). That is 275 workgroups of
     256 points, so the scan over the workgroups runs in two chunks. Cells and values bit-exact

@@ -27,7 +27,7 @@ def _system(ctx, mesh, field=-1.0):
 
 @pytest.fixture(scope="module")
 def r3nm():
-    """Config 1's own tip (r = 3 nm, h = 15 nm on a 30 x 30 nm slab) and the oracle's FP64
+    """The r = 3 nm tip (h = 15 nm on a 30 x 30 nm slab) and the oracle's FP64
     Laplace solution to 1e-13."""
     tip = M.make_tip_mesh(M.tip_r3nm_geometry(), with_atoms=False)
     print(f"r3nm tip: {len(tip.vacuum_cells())} vacuum hexes", flush=True)
@@ -61,7 +61,7 @@ def test_chebyshev_matches_oracle(ctx, small_mesh, small_ref, degree):
 
 
 def test_chebyshev_r3nm_and_reuse(ctx, r3nm):
-    """Config 1's mesh: Chebyshev-Jacobi FP64 within 1e-10 of the oracle; a second solve of the
+    """The r = 3 nm tip's mesh: Chebyshev-Jacobi FP64 within 1e-10 of the oracle; a second solve of the
     unchanged system reuses the lambda_max estimate (no Jacobi steps: fewer iterations)."""
     tip, xo = r3nm
     _, s = _system(ctx, tip)

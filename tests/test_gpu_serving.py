@@ -127,14 +127,14 @@ def test_serving_heat_timesteps_equal_alone_and_oracle(ctx, heat_clients):
 
 @pytest.fixture(scope="module")
 def heat_clients_r3nm():
-    """Config 1's tip (r = 3 nm, h = 15 nm on a 30 x 30 x 3 nm slab) served beside a small one."""
+    """The r = 3 nm tip (h = 15 nm on a 30 x 30 x 3 nm slab) served beside a small one."""
     geos = [M.tip_r3nm_geometry(), M.small_geometry(radius=10.0, height=24.0, seed=7)]
     return [serving.Client(i, M.make_tip_mesh(g, with_atoms=False), field=-1.0 - 0.1 * i, seed=i, heat=True)
             for i, g in enumerate(geos)]
 
 
-def test_serving_heat_timesteps_config1_tip(ctx, heat_clients_r3nm):
-    """The same checks with config 1's own tip as the first client (its second timestep
+def test_serving_heat_timesteps_r3nm_tip(ctx, heat_clients_r3nm):
+    """The same checks with the r = 3 nm tip as the first client (its second timestep
     against the oracle, stage by stage)."""
     _heat_timesteps(ctx, heat_clients_r3nm)
 
