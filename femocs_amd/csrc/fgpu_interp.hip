@@ -769,19 +769,21 @@ __global__ __launch_bounds__(256) void k_extract(int n_nodes, const int *__restr
         for (int k = 0; k < 5; ++k) s[k] = empty;
         return;
     }
-    constexpr int UVW[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
-                               {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
-    constexpr int NN[8][3] = {{1, 3, 4}, {0, 2, 5}, {3, 1, 6}, {2, 0, 7}, {5, 7, 0}, {4, 6, 1}, {7, 5, 2}, {6, 4, 3}};
     D3 mean = mk(0, 0, 0);
     const int e0 = n2c_ptr[n], e1 = n2c_ptr[n + 1];
     for (int e = e0; e < e1; ++e) {
         const int h = n2c[e] >> 3, node = n2c[e] & 7;
         const int *sh = hexes + 8 * (size_t)h;
-        const double uvw[3] = {(double)UVW[node][0], (double)UVW[node][1], (double)UVW[node][2]};
+        // corner `node` of the femocs hex at natural coordinates uvw (corners 0-3 counter-clockwise
+        // at w = -1, 4-7 above them); its neighbours along u, v and w are node ^ 1, ^ 3 and ^ 4
+        const int q = node & 3;
+        const double uvw[3] = {(q == 1 || q == 2) ? 1.0 : -1.0, q >= 2 ? 1.0 : -1.0, node >= 4 ? 1.0 : -1.0};
+        const int nn[3] = {node ^ 1, node ^ 3, node ^ 4};
         const D3 v0 = ldx(xyz, sh[node]);
         double J[3][3];
+#pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const D3 r = scl(sub(v0, ldx(xyz, sh[NN[node][a]])), uvw[a] * 0.5);
+            const D3 r = scl(sub(v0, ldx(xyz, sh[nn[a]])), uvw[a] * 0.5);
             J[a][0] = r.x;
             J[a][1] = r.y;
             J[a][2] = r.z;
@@ -794,16 +796,29 @@ __global__ __launch_bounds__(256) void k_extract(int n_nodes, const int *__restr
                             J[1][0] * J[0][2] - J[0][0] * J[1][2]},
                            {J[1][0] * J[2][1] - J[2][0] * J[1][1], J[2][0] * J[0][1] - J[0][0] * J[2][1],
                             J[0][0] * J[1][1] - J[1][0] * J[0][1]}};
+#pragma unroll
         for (int a = 0; a < 3; ++a)
+#pragma unroll
             for (int b = 0; b < 3; ++b) Ji[a][b] *= Jdet;
-        double g[8][3];
-        for (int k = 0; k < 8; ++k) g[k][0] = g[k][1] = g[k][2] = 0.0;
+        // shape-function gradients at the corner: nonzero for the corner and its three
+        // neighbours only; every corner's term is subtracted in corner order, zero rows included
+        double gn[3], gb[3][3];
+#pragma unroll
         for (int a = 0; a < 3; ++a) {
-            g[node][a] = 0.5 * (Ji[a][0] * uvw[0] + Ji[a][1] * uvw[1] + Ji[a][2] * uvw[2]);
-            for (int b = 0; b < 3; ++b) g[NN[node][b]][a] = -0.5 * Ji[a][b] * uvw[b];
+            gn[a] = 0.5 * (Ji[a][0] * uvw[0] + Ji[a][1] * uvw[1] + Ji[a][2] * uvw[2]);
+#pragma unroll
+            for (int b = 0; b < 3; ++b) gb[b][a] = -0.5 * Ji[a][b] * uvw[b];
         }
         D3 vi = mk(0, 0, 0);
-        for (int k = 0; k < 8; ++k) vi = sub(vi, scl(mk(g[k][0], g[k][1], g[k][2]), u[node_idof[sh[k]]]));
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            D3 gk = mk(0.0, 0.0, 0.0);
+            if (k == node) gk = mk(gn[0], gn[1], gn[2]);
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                if (k == nn[b]) gk = mk(gb[b][0], gb[b][1], gb[b][2]);
+            vi = sub(vi, scl(gk, u[node_idof[sh[k]]]));
+        }
         mean = add(mean, vi);
     }
     if (e1 > e0) mean = scl(mean, 1.0 / (e1 - e0));
