@@ -1174,7 +1174,7 @@ __device__ inline double pq_sigma(double T) {
 // theta kappa(T_old), mass coefficient gamma, source gamma T_old + sigma(T_old) |grad phi_c|^2.
 // theta < 1 (the theta scheme, Crank-Nicolson at 1/2): kx = -(1 - theta) kappa(T_old), the
 // coefficient of the explicit stiffness term applied to T_old
-__global__ void k_heat_terms(int nc, const int *__restrict__ cells, const double4 *__restrict__ X,
+__global__ __launch_bounds__(kBlock) void k_heat_terms(int nc, const int *__restrict__ cells, const double4 *__restrict__ X,
                              const double *__restrict__ T, const double *__restrict__ phic, double gamma, double lorentz,
                              double theta, double *__restrict__ kq, double *__restrict__ gq, float *__restrict__ kq32,
                              float *__restrict__ gq32, double *__restrict__ f, double *__restrict__ kx) {
@@ -1182,6 +1182,7 @@ __global__ void k_heat_terms(int nc, const int *__restrict__ cells, const double
     if (c >= nc) return;
     const int *cl = cells + 8 * (size_t)c;
     double xs[3][8], t[8], u[8];
+#pragma unroll
     for (int n = 0; n < 8; ++n) {
         const double4 p = X[cl[n]];
         xs[0][n] = p.x;
@@ -1191,6 +1192,7 @@ __global__ void k_heat_terms(int nc, const int *__restrict__ cells, const double
         u[n] = phic[cl[n]];
     }
     double J0[3][2][2], J1[3][2][2], J2[3][2][2], d0[2][2], d1[2][2], d2[2][2];
+#pragma unroll
     for (int co = 0; co < 3; ++co) {
         dref<double>(xs[co], 0, J0[co]);
         dref<double>(xs[co], 1, J1[co]);
@@ -1199,10 +1201,13 @@ __global__ void k_heat_terms(int nc, const int *__restrict__ cells, const double
     dref<double>(u, 0, d0);
     dref<double>(u, 1, d1);
     dref<double>(u, 2, d2);
+    // unrolled: the Gauss point's corner indices are constants, so the tables stay in registers
+#pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int a = q & 1, b = (q >> 1) & 1, cc = (q >> 2) & 1;
         // T at the Gauss point: trilinear weights
         double Tq = 0;
+#pragma unroll
         for (int n = 0; n < 8; ++n)
             Tq += Wt<double>(a, n & 1) * Wt<double>(b, (n >> 1) & 1) * Wt<double>(cc, (n >> 2) & 1) * t[n];
         const double c0x = J0[0][b][cc], c0y = J0[1][b][cc], c0z = J0[2][b][cc];
@@ -1241,7 +1246,7 @@ void launch_heat_terms(hipStream_t s, int nc, const int *cells, const double *X,
 // temperature-dependent conductivity of the current solve, per Gauss point of cell c
 // (internal order): sigma = sigma0 / (1 + alpha (T - 300 K)) with T interpolated at q;
 // no mass term
-__global__ void k_sigma_terms(int nc, const int *__restrict__ cells, const double *__restrict__ T, double sigma0,
+__global__ __launch_bounds__(kBlock) void k_sigma_terms(int nc, const int *__restrict__ cells, const double *__restrict__ T, double sigma0,
                               double alpha, double *__restrict__ kq, double *__restrict__ gq, float *__restrict__ kq32,
                               float *__restrict__ gq32) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1271,7 +1276,7 @@ void launch_sigma_terms(hipStream_t s, int nc, const int *cells, const double *T
 }
 
 // -grad u at the Gauss point v2n of cell v2c (DealSolver::export_solution_grad)
-__global__ void k_grad_export(int n, const int *__restrict__ v2c, const int *__restrict__ v2n, const int *__restrict__ cells,
+__global__ __launch_bounds__(kBlock) void k_grad_export(int n, const int *__restrict__ v2c, const int *__restrict__ v2n, const int *__restrict__ cells,
                               const double4 *__restrict__ X, const double *__restrict__ u, double *__restrict__ grad) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
