@@ -81,6 +81,8 @@ struct Desc {
     const int *cells;             // 8 per cell, lexicographic, internal dof ids
     const void *X;                // T4 coordinates per dof
     const int *gptr;              // dof i owns slot-buffer positions [gptr[i], gptr[i+1])
+    int wide_slots;               // dofs own many slots (Q2): each wave gathers its dofs' slot
+                                  // range cooperatively, staged through LDS
     int n_patch;                  // patches (see Patches)
     int pcells, node_cap, ppb;    // patch size, LDS node capacity, patches per workgroup
     const int *pcell0, *pnode0, *pnodes, *pslot;

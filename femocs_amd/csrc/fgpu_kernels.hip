@@ -304,7 +304,47 @@ template <class T> __device__ __forceinline__ T coarse_term(const Desc &d, int i
 // first, then up to four slots each in one wave (indices clamped into the dof's own range, the
 // extra values masked to 0), so a thread waits twice instead of once per slot; dofs with more
 // than four slots (rare) finish in a loop.
+// Slot sums for dofs with long slot lists: for each round of the thread's dofs, the wave's 64
+// consecutive dofs own one contiguous slot range, which the wave loads coalesced into its LDS
+// row (kWideChunk slots per pass); each lane then adds its own dof's slots from LDS in ascending
+// order.  Called by every lane of the wave; a dof past the end has the empty range at gptr[n].
+constexpr int kWideChunk = 256;
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <class T> __device__ __forceinline__ void gather_slots_wide(const Desc &d, int base, T (&q)[kDofIter]) {
+    __shared__ double wide_lds[kBlock / 64][kWideChunk];
+    T *buf = reinterpret_cast<T *>(wide_lds[threadIdx.x >> 6]);
+    const auto Ye = G(reinterpret_cast<const T *>(d.Ye));
+    const auto gp = G(d.gptr);
+    const int lane = threadIdx.x & 63, n = d.n_dofs;
+#pragma unroll
+    for (int m = 0; m < kDofIter; ++m) {
+        const int i = base + threadIdx.x + m * kBlock;
+        const int ic = i < n ? i : n;
+        const int b = gp[ic];
+        const int e = i < n ? gp[ic + 1] : b;
+        const int g0 = __shfl(b, 0, 64), g1 = __shfl(e, 63, 64);
+        T s = 0;
+        for (int c0 = g0; c0 < g1; c0 += kWideChunk) {
+            const int cn = min(kWideChunk, g1 - c0);
+            for (int k = lane; k < cn; k += 64) buf[k] = Ye[c0 + k];
+            wave_lds_sync();
+            const int lo = max(b, c0), hi = min(e, c0 + cn);
+            for (int k = lo; k < hi; ++k) s += buf[k - c0];
+            wave_lds_sync();
+        }
+        q[m] = s;
+    }
+}
+
 template <class T> __device__ __forceinline__ void gather_slots4(const Desc &d, int base, T (&q)[kDofIter]) {
+    if (d.wide_slots) {
+        gather_slots_wide<T>(d, base, q);
+        return;
+    }
     const auto Ye = G(reinterpret_cast<const T *>(d.Ye));
     const auto gp = G(d.gptr);
     int b[kDofIter], e[kDofIter];

@@ -659,6 +659,7 @@ Desc q2_desc(fgpu_q2 *s) {
     d.n_cells = s->n_cells;
     d.nb_dof = s->nb_dof;
     d.gptr = s->d_gptr;
+    d.wide_slots = 1;             // vertex dofs own about two dozen slots each
     d.dmask = s->d_dmask;
     d.Ye = s->Ye;
     d.q = s->q;
