@@ -738,7 +738,9 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 3 : 4) void k_patch(const 
     // one patch from its loaded tables; the first one (check) leaves the workgroup early when
     // the system is idle, tested only once that patch's lists are in flight.  Returns false to
     // leave.  (Loading the next patch's tables ahead, in workgroups of several patches, was
-    // measured: the extra registers cost more resident waves than the hidden latency gained.)
+    // measured: the extra registers cost more resident waves than the hidden latency gained.
+    // Loading only its node ids ahead keeps 8 waves resident and changed nothing: the scattered
+    // node accesses cost memory requests, not latency.)
     auto patch = [&](Tables &a, const int p, const int c0, const int nc, const int n0, const int np, const bool check) -> bool {
         const uint4 packed = a.packed;
         const uint4 lent = a.lent;

@@ -448,9 +448,9 @@ def test_face_centroids_match_oracle(ctx, small_mesh, bid):
 
 @pytest.mark.parametrize("precision", [fgpu.FGPU_FP64, fgpu.FGPU_MIXED])
 def test_patches_per_workgroup_give_the_same_bits(ctx, small_mesh, precision):
-    """Workgroups that run several patches in turn (the FP32 tet-split form then loads the next
-    patch's tables ahead) compute every patch as one-patch workgroups do: the applies and the
-    solves are bit-identical for 1, 2 and 3 patches per workgroup (3: a partial last group)."""
+    """Workgroups that run several patches in turn compute every patch as one-patch workgroups
+    do: the applies and the solves are bit-identical for 1, 2 and 3 patches per workgroup (3: a
+    partial last group)."""
     gm = fgpu.Mesh(ctx, small_mesh.nodes, small_mesh.vacuum_cells(), *MARKS)
     gm.set_patch_cells(64)
     s = fgpu.System(gm)
