@@ -1546,7 +1546,8 @@ extern "C" int fgpu_sys_apply(fgpu_sys *s, const double *x, double *y) {
 // so that no two blocks of a colour are within distance 2; A applied to the indicator of all
 // blocks of one colour then gives, summed over the free dofs of a block b1, E[b1][b2] for the
 // only block b2 of that colour in b1's closed neighbourhood.  One FP64 apply per colour, all
-// on the device; E^-1 by Gauss-Jordan sweeps on the device, stored in FP32.
+// on the device; E^-1 by blocked Gauss-Jordan exchanges on the device (32 pivots per step),
+// stored in FP16 with a power-of-two scale.
 static int build_coarse(fgpu_sys *s, int shift) {
     fgpu_mesh *m = static_cast<fgpu_mesh *>(s->mesh);
     hipStream_t st = s->ctx->stream;
@@ -1651,16 +1652,23 @@ static int build_coarse(fgpu_sys *s, int shift) {
             throw std::runtime_error("coarse operator is not positive definite");
         }
     double *dA = dalloc<double>((size_t)nc * nc), *dB = dalloc<double>((size_t)nc * nc);
+    const int gb = gj_block();
+    double *dPi = dalloc<double>((size_t)gb * gb), *dV = dalloc<double>((size_t)nc * gb),
+           *dW = dalloc<double>((size_t)nc * gb);
     h2d(dA, E.data(), E.size(), st);
-    for (int k = 0; k < nc; ++k) {
-        launch_gj_step(st, nc, k, dA, dB);
+    for (int k = 0; k < nc; k += gb) {
+        launch_gj_block(st, nc, k, std::min(gb, nc - k), dA, dB, dPi, dV, dW);
         std::swap(dA, dB);
     }
+    HIPCHECK(hipGetLastError());
     // the inverse of an SPD matrix has a positive diagonal
     std::vector<double> dg(nc);
     HIPCHECK(hipMemcpy2DAsync(dg.data(), sizeof(double), dA, (nc + 1) * sizeof(double), sizeof(double), nc,
                               hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
+    dfree(dPi);
+    dfree(dV);
+    dfree(dW);
     double dmax = 0.0;
     for (double v : dg) {
         if (!(v > 0.0) || !std::isfinite(v)) {

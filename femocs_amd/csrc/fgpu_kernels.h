@@ -27,7 +27,9 @@ void launch_gather_cheb(hipStream_t s, const Desc *D, const int2 *tab, int nblk,
 void launch_coarse_indicator(hipStream_t s, int n, const unsigned char *mask, const int *color, int shift, int c,
                              double *u);
 void launch_coarse_sums(hipStream_t s, int n, const unsigned char *mask, const double *q, double *out);
-void launch_gj_step(hipStream_t s, int n, int k, const double *A, double *B);
+int gj_block();
+void launch_gj_block(hipStream_t s, int n, int k, int bb, const double *A, double *B, double *Pi, double *V,
+                     double *W);
 void launch_to_half(hipStream_t s, int n, int ld, const double *a, double scale, _Float16 *out);
 void launch_enforce_dirichlet(hipStream_t s, int n, const unsigned char *mask, const double *g, double *x);
 void launch_lift(hipStream_t s, int n, const unsigned char *mask, const double *g, const double *Ag, const double *diag,

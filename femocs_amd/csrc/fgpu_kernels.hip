@@ -1456,19 +1456,84 @@ __global__ __launch_bounds__(kBlock) void k_coarse_sums(int n, const unsigned ch
     }
 }
 
-// one Gauss-Jordan exchange step on pivot k (no pivoting: E is SPD), out of place:
-// B_kk = 1/a_kk, B_kj = a_kj/a_kk, B_ik = -a_ik/a_kk, B_ij = a_ij - a_ik a_kj / a_kk; after the
-// steps k = 0..n-1 the matrix holds the inverse
-__global__ void k_gj_step(int n, int k, const double *__restrict__ A, double *__restrict__ B) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Gauss-Jordan exchange on a block K of kGjBlock pivots (no pivoting: E is SPD), out of place,
+// with P = A_KK: B_KK = P^-1, B_Kj = P^-1 A_Kj, B_iK = -A_iK P^-1, B_ij = A_ij - A_iK P^-1 A_Kj.
+// One block is the same as its scalar exchanges in turn (b_kk = 1/a_kk, b_kj = a_kj/a_kk,
+// b_ik = -a_ik/a_kk, b_ij = a_ij - a_ik a_kj/a_kk); after every block the matrix holds the
+// inverse.  Three launches per block: P^-1 (one workgroup, scalar exchanges in LDS), the panels
+// V = A_iK P^-1 and W = P^-1 A_Kj, then the update.
+constexpr int kGjBlock = 32;
+
+// P^-1 of the bb x bb block at (k, k), bb <= kGjBlock, into Pi (kGjBlock x kGjBlock, row-major)
+__global__ __launch_bounds__(256) void k_gj_pivot(int n, int k, int bb, const double *__restrict__ A,
+                                                  double *__restrict__ Pi) {
+    __shared__ double P[kGjBlock][kGjBlock + 1];
+    constexpr int kPer = kGjBlock * kGjBlock / 256;
+    for (int e = threadIdx.x; e < kGjBlock * kGjBlock; e += 256) {
+        const int r = e / kGjBlock, c = e % kGjBlock;
+        P[r][c] = (r < bb && c < bb) ? A[(size_t)(k + r) * n + k + c] : 0.0;
+    }
+    __syncthreads();
+    for (int p = 0; p < bb; ++p) {
+        double v[kPer];
+        const double piv = P[p][p];
+#pragma unroll
+        for (int m = 0; m < kPer; ++m) {
+            const int e = threadIdx.x + m * 256, r = e / kGjBlock, c = e % kGjBlock;
+            if (r >= bb || c >= bb) v[m] = 0.0;
+            else if (r == p && c == p) v[m] = 1.0 / piv;
+            else if (r == p) v[m] = P[r][c] / piv;
+            else if (c == p) v[m] = -P[r][c] / piv;
+            else v[m] = P[r][c] - P[r][p] * P[p][c] / piv;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < kPer; ++m) {
+            const int e = threadIdx.x + m * 256;
+            P[e / kGjBlock][e % kGjBlock] = v[m];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < kGjBlock * kGjBlock; e += 256) Pi[e] = P[e / kGjBlock][e % kGjBlock];
+}
+
+// V[i][q] = sum_p A[i][k+p] Pi[p][q] (n x kGjBlock) and W[q][j] = sum_p Pi[q][p] A[k+p][j]
+// (kGjBlock x n), q < bb: threads [0, n bb) for V, [n bb, 2 n bb) for W
+__global__ __launch_bounds__(256) void k_gj_panel(int n, int k, int bb, const double *__restrict__ A,
+                                                  const double *__restrict__ Pi, double *__restrict__ V,
+                                                  double *__restrict__ W) {
+    __shared__ double sP[kGjBlock * kGjBlock];
+    for (int e = threadIdx.x; e < kGjBlock * kGjBlock; e += 256) sP[e] = Pi[e];
+    __syncthreads();
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x, nb = (size_t)n * bb;
+    if (t < nb) {
+        const int i = (int)(t / bb), q = (int)(t % bb);
+        double s = 0.0;
+        for (int p = 0; p < bb; ++p) s += A[(size_t)i * n + k + p] * sP[p * kGjBlock + q];
+        V[(size_t)i * kGjBlock + q] = s;
+    } else if (t < 2 * nb) {
+        const int q = (int)((t - nb) / n), j = (int)((t - nb) % n);
+        double s = 0.0;
+        for (int p = 0; p < bb; ++p) s += sP[q * kGjBlock + p] * A[(size_t)(k + p) * n + j];
+        W[(size_t)q * n + j] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gj_update(int n, int k, int bb, const double *__restrict__ A,
+                                                   const double *__restrict__ Pi, const double *__restrict__ V,
+                                                   const double *__restrict__ W, double *__restrict__ B) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= (size_t)n * n) return;
     const int i = (int)(e / n), j = (int)(e % n);
-    const double piv = A[(size_t)k * n + k];
+    const bool ri = i >= k && i < k + bb, cj = j >= k && j < k + bb;
     double v;
-    if (i == k && j == k) v = 1.0 / piv;
-    else if (i == k) v = A[e] / piv;
-    else if (j == k) v = -A[e] / piv;
-    else v = A[e] - A[(size_t)i * n + k] * A[(size_t)k * n + j] / piv;
+    if (ri && cj) v = Pi[(i - k) * kGjBlock + (j - k)];
+    else if (ri) v = W[(size_t)(i - k) * n + j];
+    else if (cj) v = -V[(size_t)i * kGjBlock + (j - k)];
+    else {
+        v = A[e];
+        for (int q = 0; q < bb; ++q) v -= V[(size_t)i * kGjBlock + q] * A[(size_t)(k + q) * n + j];
+    }
     B[e] = v;
 }
 
@@ -1489,9 +1554,15 @@ void launch_coarse_sums(hipStream_t s, int n, const unsigned char *mask, const d
     const int nb = (n + kDofBlock - 1) / kDofBlock;
     if (nb > 0) hipLaunchKernelGGL(k_coarse_sums, dim3(nb), dim3(kBlock), 0, s, n, mask, q, out);
 }
-void launch_gj_step(hipStream_t s, int n, int k, const double *A, double *B) {
-    const size_t e = (size_t)n * n;
-    if (e) hipLaunchKernelGGL(k_gj_step, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, n, k, A, B);
+int gj_block() { return kGjBlock; }
+// one block exchange on pivots [k, k + bb): Pi holds kGjBlock^2 doubles, V and W n kGjBlock each
+void launch_gj_block(hipStream_t s, int n, int k, int bb, const double *A, double *B, double *Pi, double *V,
+                     double *W) {
+    const size_t e = (size_t)n * n, nb = 2 * (size_t)n * bb;
+    if (!e || bb <= 0 || bb > kGjBlock) return;
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, s, n, k, bb, A, Pi);
+    hipLaunchKernelGGL(k_gj_panel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, n, k, bb, A, Pi, V, W);
+    hipLaunchKernelGGL(k_gj_update, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, n, k, bb, A, Pi, V, W, B);
 }
 void launch_to_half(hipStream_t s, int n, int ld, const double *a, double scale, _Float16 *out) {
     const size_t e = (size_t)n * ld;
